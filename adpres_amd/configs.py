@@ -1,0 +1,199 @@
+"""Seeded synthetic static cores (the workloads the engine is measured on).
+
+Each generator returns a :class:`GeneratedCore`: the :class:`Problem` handed
+to the engine plus the deck data (materials, assembly layout) from which the
+reference build reads the same core.  Node-wise perturbations that no deck
+can express are carried by ``Problem`` and written for the reference as a
+per-node cross-section override file.
+
+Cores
+-----
+``core2d_small``  2D 2-group, 9x9 assemblies of 20 cm with 2x2 nodes
+                  (18x18x2 nodes), 4 fuel types in a radius-4-assembly disk,
+                  reflector ring, vacuum (zero incoming current) outside.
+``core3d_2g``     3D 2-group quarter core, 17x17 assemblies of 21.5 cm with
+                  2x2 nodes, 24 fuel planes of 15 cm + 1 reflector plane top
+                  and bottom (34x34x26 nodes), 8 fuel types, per-node +-2%
+                  perturbation of every cross section, reflective on the
+                  quarter-core faces (west, south).
+``core3d_8g``     3D 8-group quarter core, same layout with 4x4 radial nodes
+                  and 24 axial layers of 15 cm split in 4 (68x68x96 nodes),
+                  full downscatter plus upscatter into groups 6-8, nu-sigf
+                  rescaled per fuel material to a sampled k-infinity.
+"""
+
+from __future__ import annotations
+
+import dataclasses
+from typing import Optional
+
+import numpy as np
+
+from .problem import Problem, build_geometry, node_xs_from_materials, round_deck
+
+NU = 2.43          # nu-sigf / sigf, i.e. fixed kappa/nu for the power
+
+
+@dataclasses.dataclass
+class GeneratedCore:
+    problem: Problem
+    deck: dict                   # keyword arguments of problem.write_deck
+    perturbed: bool              # True if node XS differ from the material tables
+    seed: int
+
+
+def _mk_problem(name, ng, mats, geo, bc, chi, perturb_rng=None, perturb=0.0, **iter_kw):
+    g = build_geometry(**geo)
+    xsigtr, xsiga, xnuf, xsigf, xsigs = (round_deck(a) for a in mats)
+    chi = round_deck(chi)
+    sigtr, siga, nuf, sigf, sigs = node_xs_from_materials(g["mat"], xsigtr, xsiga, xnuf, xsigf, xsigs)
+    if perturb_rng is not None and perturb > 0:
+        def pert(a):
+            return a * perturb_rng.uniform(1.0 - perturb, 1.0 + perturb, size=a.shape)
+        sigtr, siga, nuf, sigf, sigs = pert(sigtr), pert(siga), pert(nuf), pert(sigf), pert(sigs)
+    nmat = len(xsigtr)
+    chi_arr = np.ascontiguousarray(np.asarray(chi, dtype=np.float64).T)   # (G, nmat)
+    p = Problem(ng=ng, nmat=nmat, nxx=g["nxx"], nyy=g["nyy"], nzz=g["nzz"],
+                xdel=g["xdel"], ydel=g["ydel"], zdel=g["zdel"],
+                ystag_min=g["ystag_min"], ystag_max=g["ystag_max"],
+                xstag_min=g["xstag_min"], xstag_max=g["xstag_max"], bc=tuple(bc),
+                ix=g["ix"], iy=g["iy"], iz=g["iz"], mat=g["mat"],
+                sigtr=np.ascontiguousarray(sigtr), siga=np.ascontiguousarray(siga),
+                nuf=np.ascontiguousarray(nuf), sigf=np.ascontiguousarray(sigf),
+                sigs=np.ascontiguousarray(sigs), chi=chi_arr,
+                dc=np.ones((6, ng, len(g["ix"]))), name=name, **iter_kw)
+    deck = dict(ng=ng, xsigtr=xsigtr, xsiga=xsiga, xnuf=xnuf, xsigf=xsigf, xchi=chi,
+                xsigs=xsigs, bc=bc, **geo)
+    return p, deck
+
+
+def _two_group_material(rng, fuel: bool):
+    D1 = rng.uniform(1.2, 1.6); D2 = rng.uniform(0.3, 0.45)
+    sa1 = rng.uniform(0.008, 0.012)
+    sa2 = rng.uniform(0.06, 0.12) if fuel else rng.uniform(0.02, 0.04)
+    nf1 = rng.uniform(0.004, 0.008) if fuel else 0.0
+    nf2 = rng.uniform(0.09, 0.16) if fuel else 0.0
+    s12 = rng.uniform(0.015, 0.02)
+    sigtr = [1.0 / (3.0 * D1), 1.0 / (3.0 * D2)]
+    return (sigtr, [sa1, sa2], [nf1, nf2], [nf1 / NU, nf2 / NU], [[0.0, s12], [0.0, 0.0]])
+
+
+def _stack(mats):
+    return tuple(np.asarray([m[i] for m in mats], dtype=np.float64) for i in range(5))
+
+
+def core2d_small(seed: int = 1, **iter_kw) -> GeneratedCore:
+    rng = np.random.default_rng(seed)
+    nfuel = 4
+    mats = [_two_group_material(rng, True) for _ in range(nfuel)]
+    mats.append(_two_group_material(rng, False))
+    refl = nfuel + 1
+    nx = 9
+    pl = np.full((nx, nx), refl, dtype=np.int32)
+    c = (nx + 1) / 2.0
+    for j in range(1, nx + 1):
+        for i in range(1, nx + 1):
+            if (i - c) ** 2 + (j - c) ** 2 <= 16.0:
+                pl[j - 1, i - 1] = rng.integers(1, nfuel + 1)
+    geo = dict(xsize=[20.0] * nx, xdiv=[2] * nx, ysize=[20.0] * nx, ydiv=[2] * nx,
+               zsize=[10.0, 10.0], zdiv=[1, 1], planars=[pl], zpln=[1, 1])
+    chi = [[1.0, 0.0]] * (nfuel + 1)
+    bc = (1, 1, 1, 1, 2, 2)
+    p, deck = _mk_problem("core2d_small", 2, _stack(mats), geo, bc, chi, **iter_kw)
+    return GeneratedCore(p, deck, False, seed)
+
+
+def _quarter_layout(rng, nx, nfuel, radius):
+    refl = nfuel + 1
+    pl = np.full((nx, nx), refl, dtype=np.int32)
+    for j in range(1, nx + 1):
+        for i in range(1, nx + 1):
+            if (i - 0.5) ** 2 + (j - 0.5) ** 2 <= radius ** 2:
+                pl[j - 1, i - 1] = rng.integers(1, nfuel + 1)
+    return pl
+
+
+def core3d_2g(seed: int = 1, perturb: float = 0.02, **iter_kw) -> GeneratedCore:
+    rng = np.random.default_rng(seed)
+    nfuel = 8
+    mats = [_two_group_material(rng, True) for _ in range(nfuel)]
+    mats.append(_two_group_material(rng, False))
+    refl = nfuel + 1
+    nx = 17
+    fuel_pl = _quarter_layout(rng, nx, nfuel, 14.5)
+    refl_pl = np.full((nx, nx), refl, dtype=np.int32)
+    nz = 26
+    geo = dict(xsize=[21.5] * nx, xdiv=[2] * nx, ysize=[21.5] * nx, ydiv=[2] * nx,
+               zsize=[15.0] * nz, zdiv=[1] * nz, planars=[refl_pl, fuel_pl],
+               zpln=[1] + [2] * (nz - 2) + [1])
+    chi = [[1.0, 0.0]] * (nfuel + 1)
+    bc = (1, 2, 1, 2, 1, 1)
+    kw = dict(nout=3000, nin=20, serc=1e-5, ferc=1e-5, nac=1000, nupd=22)
+    kw.update(iter_kw)
+    p, deck = _mk_problem("core3d_2g", 2, _stack(mats), geo, bc, chi,
+                          perturb_rng=np.random.default_rng(seed + 1000), perturb=perturb, **kw)
+    return GeneratedCore(p, deck, perturb > 0, seed)
+
+
+def kinf(nuf, chi, sigr, sigs_from_to):
+    """k-infinity of one material: nuf^T M^-1 chi with
+    M(g,h) = delta_gh sigr_g - (1-delta_gh) sigs(h->g)."""
+    G = len(nuf)
+    M = -np.asarray(sigs_from_to, dtype=np.float64).T.copy()
+    for g in range(G):
+        M[g, g] = sigr[g]
+    return float(np.asarray(nuf) @ np.linalg.solve(M, np.asarray(chi)))
+
+
+def _eight_group_material(rng, fuel: bool, G: int = 8):
+    D = np.sort(np.exp(rng.uniform(np.log(0.3), np.log(2.0), G)))[::-1]
+    sa = np.sort(np.exp(rng.uniform(np.log(1e-3), np.log(0.15), G)))
+    if not fuel:
+        sa = sa * 0.3
+    sigs = np.zeros((G, G))
+    for g in range(G):
+        for h in range(g + 1, G):
+            sigs[g, h] = rng.uniform(0.002, 0.03) * 0.5 ** (h - g - 1)
+        if g >= 6:
+            for h in range(5, g):
+                sigs[g, h] = rng.uniform(0.0005, 0.003)
+    nf = np.sort(rng.uniform(0.0, 0.2, G)) if fuel else np.zeros(G)
+    chi = np.zeros(G)
+    chi[:3] = rng.dirichlet([6.0, 3.0, 1.0])
+    sigr = sa + sigs.sum(axis=1) - np.diag(sigs)
+    if fuel:
+        target = rng.uniform(1.0, 1.2)
+        nf = nf * (target / kinf(nf, chi, sigr, sigs))
+    sigtr = 1.0 / (3.0 * D)
+    return (list(sigtr), list(sa), list(nf), list(nf / NU), sigs.tolist()), list(chi)
+
+
+def core3d_8g(seed: int = 2, **iter_kw) -> GeneratedCore:
+    rng = np.random.default_rng(seed)
+    nfuel = 8
+    mats, chis = [], []
+    for m in range(nfuel + 1):
+        mt, ch = _eight_group_material(rng, m < nfuel)
+        mats.append(mt); chis.append(ch)
+    chis[-1] = chis[0]
+    refl = nfuel + 1
+    nx = 17
+    fuel_pl = _quarter_layout(rng, nx, nfuel, 14.5)
+    refl_pl = np.full((nx, nx), refl, dtype=np.int32)
+    nz = 24
+    geo = dict(xsize=[21.5] * nx, xdiv=[4] * nx, ysize=[21.5] * nx, ydiv=[4] * nx,
+               zsize=[15.0] * nz, zdiv=[4] * nz, planars=[refl_pl, fuel_pl],
+               zpln=[1] + [2] * (nz - 2) + [1])
+    bc = (1, 2, 1, 2, 1, 1)
+    kw = dict(nout=5000, nin=20, serc=1e-5, ferc=1e-5, nac=1000, nupd=93)
+    kw.update(iter_kw)
+    p, deck = _mk_problem("core3d_8g", 8, _stack(mats), geo, bc, chis, **kw)
+    return GeneratedCore(p, deck, False, seed)
+
+
+CORES = {"core2d_small": core2d_small, "core3d_2g": core3d_2g, "core3d_8g": core3d_8g}
+
+
+def generate(name: str, seed: Optional[int] = None, **kw) -> GeneratedCore:
+    fn = CORES[name]
+    return fn(seed, **kw) if seed is not None else fn(**kw)

@@ -1,0 +1,97 @@
+/*
+ * CPU restatement of ADPRES's static CMFD + SANM/PNM hot path, in FP64.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker: tests, the smoke
+ * entry and the benchmark's CPU-baseline leg call it; the product library
+ * never links it.
+ *
+ * Every routine follows the reference's loop order and operation order
+ * (src/mod_cmfd.f90, src/mod_nodal.f90) so that, compiled without FP
+ * contraction, it reproduces the reference's FP64 results bit for bit on
+ * the same inputs.  Arrays are Fortran column-major with 1-based indices
+ * stored in the integer arrays, exactly as the reference holds them in its
+ * `sdata` module (src/mod_data.f90).
+ */
+#ifndef ADPRES_ORACLE_H
+#define ADPRES_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  ORC_OK = 0,
+  ORC_E_MAX_OUTER = 1,      /* src/mod_cmfd.f90:498-505 */
+  ORC_E_NODAL_UNSTABLE = 2, /* src/mod_nodal.f90:131-142 */
+  ORC_E_LU_PIVOT = 3,       /* src/mod_nodal.f90:856-862 */
+  ORC_E_POWER_NONPOS = 4,   /* src/mod_cmfd.f90:1322-1326 */
+  ORC_E_ARG = 5,            /* bad argument / unsupported geometry */
+};
+
+typedef struct {
+  int nnod, ng, nxx, nyy, nzz, nmat;
+  const int *ix, *iy, *iz, *mat;          /* (N), 1-based */
+  const int *ystag_min, *ystag_max;       /* (nyy) */
+  const int *xstag_min, *xstag_max;       /* (nxx) */
+  const double *xdel, *ydel, *zdel;
+  int bc[6];                              /* east west north south bottom top */
+  const double *D, *sigr, *nuf, *sigf;    /* (N,G) */
+  const double *sigs;                     /* (N,G,G): sigs(n, from, to) */
+  const double *chi;                      /* (nmat,G) */
+  const double *dc;                       /* (N,G,6) */
+  const double *exsrc;                    /* (N,G) */
+} orc_problem;
+
+typedef struct {
+  int nout, nin, nac, nupd;
+  int kern;                               /* 0 FDM, 1 PNM, 2 SANM */
+  double serc, ferc;
+} orc_iter;
+
+typedef struct {
+  double Ke;
+  double *f0;      /* (N,G) */
+  double *fs0;     /* (N) */
+  double *dn;      /* (N,G,6) nodal (D-hat) coupling coefficients */
+  double *df;      /* (N,G,6) FDM (D-tilde) coupling coefficients (output) */
+  double ndmax;
+  int im, jm, km;
+  int first;       /* 1: initialise f0 = 1, Ke = 1, fs0 from f0 */
+} orc_state;
+
+typedef struct {
+  int cap;         /* capacity of the arrays below */
+  int n;           /* outers performed */
+  double *k, *ser, *fer, *ndmax;
+  int *flags;      /* bit0 extrapolated, bit1 nodal update */
+  int *loc;        /* 3 per outer: im jm km after a nodal update */
+} orc_trace;
+
+/* Whole static solve: outer (src/mod_cmfd.f90:415) or outer_ad (:602). */
+int orc_outer(const orc_problem *pb, orc_state *st, const orc_iter *it,
+              int adjoint, orc_trace *tr);
+
+/* coup_coef (src/mod_cmfd.f90:11): fills st->df. */
+void orc_coup_coef(const orc_problem *pb, orc_state *st);
+
+/* One nodal update (src/mod_nodal.f90:18 SANM, :151 PNM), cmode 1 forward
+ * or 0 adjoint.  Reads st->f0, st->Ke, st->df, st->dn; writes st->dn,
+ * st->ndmax (reset to 0 first) and the location. */
+int orc_nodal_update(const orc_problem *pb, orc_state *st, int kern, int cmode);
+
+/* nin BiCGSTAB iterations for group g (0-based) on the CMFD matrix built
+ * from st->df and st->dn (src/mod_cmfd.f90:1203).  x is in/out (N). */
+int orc_bicg(const orc_problem *pb, const orc_state *st, int g, int nin,
+             const double *b, double *x);
+
+/* y = A_g x with the CMFD matrix (src/mod_cmfd.f90:1247). */
+int orc_matvec(const orc_problem *pb, const orc_state *st, int g,
+               const double *x, double *y);
+
+/* PowDis (src/mod_cmfd.f90:1290). */
+int orc_powdis(const orc_problem *pb, const double *f0, double *p);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,164 @@
+"""ctypes binding of the FP64 CPU restatement in liboracle.so.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by the smoke entry and by the
+benchmark's CPU-baseline leg, always as the checker.  The product path never
+imports this module.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+ORC_OK, ORC_E_MAX_OUTER, ORC_E_NODAL_UNSTABLE, ORC_E_LU_PIVOT, ORC_E_POWER_NONPOS, ORC_E_ARG = range(6)
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+class OrcProblem(C.Structure):
+    _fields_ = [("nnod", C.c_int), ("ng", C.c_int), ("nxx", C.c_int), ("nyy", C.c_int),
+                ("nzz", C.c_int), ("nmat", C.c_int),
+                ("ix", _ip), ("iy", _ip), ("iz", _ip), ("mat", _ip),
+                ("ystag_min", _ip), ("ystag_max", _ip), ("xstag_min", _ip), ("xstag_max", _ip),
+                ("xdel", _dp), ("ydel", _dp), ("zdel", _dp), ("bc", C.c_int * 6),
+                ("D", _dp), ("sigr", _dp), ("nuf", _dp), ("sigf", _dp), ("sigs", _dp),
+                ("chi", _dp), ("dc", _dp), ("exsrc", _dp)]
+
+
+class OrcIter(C.Structure):
+    _fields_ = [("nout", C.c_int), ("nin", C.c_int), ("nac", C.c_int), ("nupd", C.c_int),
+                ("kern", C.c_int), ("serc", C.c_double), ("ferc", C.c_double)]
+
+
+class OrcState(C.Structure):
+    _fields_ = [("Ke", C.c_double), ("f0", _dp), ("fs0", _dp), ("dn", _dp), ("df", _dp),
+                ("ndmax", C.c_double), ("im", C.c_int), ("jm", C.c_int), ("km", C.c_int),
+                ("first", C.c_int)]
+
+
+class OrcTrace(C.Structure):
+    _fields_ = [("cap", C.c_int), ("n", C.c_int), ("k", _dp), ("ser", _dp), ("fer", _dp),
+                ("ndmax", _dp), ("flags", _ip), ("loc", _ip)]
+
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        L.orc_outer.argtypes = [C.POINTER(OrcProblem), C.POINTER(OrcState), C.POINTER(OrcIter),
+                                C.c_int, C.POINTER(OrcTrace)]
+        L.orc_nodal_update.argtypes = [C.POINTER(OrcProblem), C.POINTER(OrcState), C.c_int, C.c_int]
+        L.orc_bicg.argtypes = [C.POINTER(OrcProblem), C.POINTER(OrcState), C.c_int, C.c_int, _dp, _dp]
+        L.orc_matvec.argtypes = [C.POINTER(OrcProblem), C.POINTER(OrcState), C.c_int, _dp, _dp]
+        L.orc_powdis.argtypes = [C.POINTER(OrcProblem), _dp, _dp]
+        L.orc_coup_coef.argtypes = [C.POINTER(OrcProblem), C.POINTER(OrcState)]
+        _lib = L
+    return _lib
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _i(a):
+    return a.ctypes.data_as(_ip)
+
+
+class Oracle:
+    """Holds contiguous copies of a Problem's arrays and the solver state."""
+
+    def __init__(self, p):
+        self.p = p
+        self._keep = {}
+        ints = {k: np.ascontiguousarray(getattr(p, k), dtype=np.int32)
+                for k in ("ix", "iy", "iz", "mat", "ystag_min", "ystag_max", "xstag_min", "xstag_max")}
+        dbls = {k: np.ascontiguousarray(getattr(p, k), dtype=np.float64)
+                for k in ("xdel", "ydel", "zdel", "D", "sigr", "nuf", "sigf", "sigs", "chi", "dc", "exsrc")}
+        self._keep.update(ints); self._keep.update(dbls)
+        pb = OrcProblem()
+        pb.nnod, pb.ng, pb.nxx, pb.nyy, pb.nzz, pb.nmat = p.nnod, p.ng, p.nxx, p.nyy, p.nzz, p.nmat
+        for k, v in ints.items():
+            setattr(pb, k, _i(v))
+        for k, v in dbls.items():
+            setattr(pb, k, _d(v))
+        pb.bc = (C.c_int * 6)(*p.bc)
+        self.pb = pb
+        N, G = p.nnod, p.ng
+        self.f0 = np.ones((G, N)); self.fs0 = np.zeros(N)
+        self.dn = np.zeros((6, G, N)); self.df = np.zeros((6, G, N))
+        st = OrcState()
+        st.Ke = 1.0; st.f0 = _d(self.f0); st.fs0 = _d(self.fs0); st.dn = _d(self.dn)
+        st.df = _d(self.df); st.ndmax = 0.0; st.first = 1
+        self.st = st
+
+    @property
+    def Ke(self):
+        return self.st.Ke
+
+    def outer(self, adjoint=None, nout=None, nin=None, serc=None, ferc=None, nac=None, nupd=None,
+              kern=None, trace_cap=None):
+        p = self.p
+        it = OrcIter(nout if nout is not None else p.nout, nin if nin is not None else p.nin,
+                     nac if nac is not None else p.nac, nupd if nupd is not None else p.nupd,
+                     {"FDM": 0, "PNM": 1, "SANM": 2}[kern or p.kern],
+                     serc if serc is not None else p.serc, ferc if ferc is not None else p.ferc)
+        if adjoint is None:
+            adjoint = p.mode == "ADJOINT"
+        cap = trace_cap or it.nout
+        self.trace = {k: np.zeros(cap) for k in ("k", "ser", "fer", "ndmax")}
+        self.trace["flags"] = np.zeros(cap, np.int32)
+        self.trace["loc"] = np.zeros(3 * cap, np.int32)
+        tr = OrcTrace(cap, 0, _d(self.trace["k"]), _d(self.trace["ser"]), _d(self.trace["fer"]),
+                      _d(self.trace["ndmax"]), _i(self.trace["flags"]), _i(self.trace["loc"]))
+        rc = lib().orc_outer(C.byref(self.pb), C.byref(self.st), C.byref(it), int(adjoint), C.byref(tr))
+        self.niter = tr.n
+        for k in ("k", "ser", "fer", "ndmax", "flags"):
+            self.trace[k] = self.trace[k][:tr.n]
+        self.trace["loc"] = self.trace["loc"][:3 * tr.n].reshape(-1, 3)
+        return rc
+
+    def nodal_update(self, Ke, f0, dn, kern=None, cmode=1):
+        self.st.Ke = Ke
+        self.f0[...] = f0
+        self.dn[...] = dn
+        rc = lib().orc_nodal_update(C.byref(self.pb), C.byref(self.st),
+                                    {"FDM": 0, "PNM": 1, "SANM": 2}[kern or self.p.kern], cmode)
+        return rc, self.dn.copy(), self.st.ndmax, (self.st.im, self.st.jm, self.st.km)
+
+    def bicg(self, g, nin, b, x, dn=None):
+        if dn is not None:
+            self.dn[...] = dn
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.array(x, dtype=np.float64, copy=True)
+        rc = lib().orc_bicg(C.byref(self.pb), C.byref(self.st), g, nin, _d(b), _d(x))
+        return rc, x
+
+    def matvec(self, g, x, dn=None):
+        if dn is not None:
+            self.dn[...] = dn
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.zeros_like(x)
+        rc = lib().orc_matvec(C.byref(self.pb), C.byref(self.st), g, _d(x), _d(y))
+        return rc, y
+
+    def powdis(self, f0=None):
+        f0 = np.ascontiguousarray(self.f0 if f0 is None else f0, dtype=np.float64)
+        out = np.zeros(self.p.nnod)
+        rc = lib().orc_powdis(C.byref(self.pb), _d(f0), _d(out))
+        return rc, out

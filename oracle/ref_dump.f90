@@ -1,0 +1,163 @@
+! Repo-owned dump driver for the ADPRES reference build (test infrastructure).
+!
+! Links the reference modules unchanged (sdata, io, xsec, cmfd, nodal) and
+! exposes three actions so that the reference's own FP64 arithmetic can be
+! written out at full precision:
+!
+!   ref_dump DECK solve OUT [XSFILE|-]          whole static solve
+!   ref_dump DECK nodal OUT XSFILE|- STATEFILE  one nodal (SANM/PNM) update
+!   ref_dump DECK bicg  OUT XSFILE|- STATEFILE  one nin-iteration BiCGSTAB per group
+!
+! The deck is read by the reference's own parser (io::inp_read, mod_io.f90:69),
+! cross sections are built by xsec::XS_updt (mod_xsec.f90:11).  An optional
+! XSFILE overrides the per-node sigtr, siga, nuf, sigf, sigs (raw little-endian
+! float64, Fortran order) before xsec::Dsigr_updt (mod_xsec.f90:199) re-derives
+! D and sigr; this is how per-node perturbations that no deck can express are
+! given to the reference.
+!
+! OUT is a raw stream of the problem (geometry and node cross sections as the
+! reference holds them) followed by the action's results.  The layout is read
+! by adpres_amd/problem.py (read_ref_dump).
+
+program ref_dump
+
+  use sdata
+  use io,    only: inp_read
+  use xsec,  only: XS_updt, Dsigr_updt
+  use cmfd,  only: outer, outer_ad, PowDis, matrix_setup, bicg
+  use nodal, only: nodal_update, nodal_update_pnm
+
+  implicit none
+
+  character(len=256) :: action, outname, xsname, stname
+  integer :: u, n, g, nargs, kid, nin_p
+  real(dp), allocatable :: npw(:), bb(:,:), xx(:)
+  real(dp), allocatable :: dnin(:,:,:), dfout(:,:,:), dnout(:,:,:)
+
+  nargs = command_argument_count()
+  if (nargs < 3) then
+    write(*,*) 'usage: ref_dump DECK solve|nodal|bicg OUT [XSFILE|-] [STATEFILE]'
+    stop 2
+  end if
+  call get_command_argument(2, action)
+  call get_command_argument(3, outname)
+  xsname = '-'
+  stname = '-'
+  if (nargs >= 4) call get_command_argument(4, xsname)
+  if (nargs >= 5) call get_command_argument(5, stname)
+
+  call inp_read()
+  call XS_updt(bcon, ftem, mtem, cden, bpos)
+
+  if (trim(xsname) /= '-') then
+    open(newunit=u, file=trim(xsname), access='stream', form='unformatted', &
+         status='old', action='read')
+    read(u) sigtr, siga, nuf, sigf, sigs
+    close(u)
+    call Dsigr_updt()
+  end if
+
+  kid = 2
+  if (kern == ' FDM') kid = 0
+  if (kern == ' PNM') kid = 1
+
+  open(newunit=u, file=trim(outname), access='stream', form='unformatted', &
+       status='replace', action='write')
+  ! ---- problem section ----
+  write(u) int(20240601, 4), int(nnod, 4), int(ng, 4), int(nxx, 4), int(nyy, 4), &
+           int(nzz, 4), int(nmat, 4), int(kid, 4)
+  write(u) int(xeast, 4), int(xwest, 4), int(ynorth, 4), int(ysouth, 4), &
+           int(zbott, 4), int(ztop, 4)
+  write(u) int(nout, 4), int(nin, 4), int(nac, 4), int(nupd, 4), serc, ferc
+  write(u) xdel, ydel, zdel
+  write(u) (int(ystag(n)%smin, 4), n = 1, nyy), (int(ystag(n)%smax, 4), n = 1, nyy)
+  write(u) (int(xstag(n)%smin, 4), n = 1, nxx), (int(xstag(n)%smax, 4), n = 1, nxx)
+  write(u) (int(ix(n), 4), n = 1, nnod), (int(iy(n), 4), n = 1, nnod), &
+           (int(iz(n), 4), n = 1, nnod), (int(mat(n), 4), n = 1, nnod)
+  write(u) sigtr, siga, nuf, sigf, sigs, chi, dc, D, sigr, vdel
+
+  select case (trim(action))
+
+  case ('solve')
+    if (trim(mode) == 'ADJOINT') then
+      call outer_ad(1)
+    else
+      call outer(1)
+    end if
+    allocate(npw(nnod))
+    call PowDis(npw)
+    allocate(dfout(nnod, ng, 6), dnout(nnod, ng, 6))
+    do g = 1, ng
+      do n = 1, nnod
+        dfout(n, g, :) = nod(n, g)%df
+        dnout(n, g, :) = nod(n, g)%dn
+      end do
+    end do
+    write(u) Ke, f0, fs0, npw, dfout, dnout
+
+  case ('nodal')
+    call read_state()
+    call matrix_setup(1)
+    do g = 1, ng
+      do n = 1, nnod
+        nod(n, g)%dn = dnin(n, g, :)
+      end do
+    end do
+    ndmax = 0._dp
+    if (kid == 1) then
+      call nodal_update_pnm(1)
+    else
+      call nodal_update(1)
+    end if
+    allocate(dfout(nnod, ng, 6), dnout(nnod, ng, 6))
+    do g = 1, ng
+      do n = 1, nnod
+        dfout(n, g, :) = nod(n, g)%df
+        dnout(n, g, :) = nod(n, g)%dn
+      end do
+    end do
+    write(u) dfout, dnout, ndmax, int(im, 4), int(jm, 4), int(km, 4)
+
+  case ('bicg')
+    call read_state()
+    call matrix_setup(1)
+    do g = 1, ng
+      do n = 1, nnod
+        nod(n, g)%dn = dnin(n, g, :)
+      end do
+    end do
+    call matrix_setup(0)
+    allocate(xx(nnod))
+    do g = 1, ng
+      xx = f0(:, g)
+      call bicg(nin_p, g, bb(:, g), xx)
+      f0(:, g) = xx
+    end do
+    write(u) f0
+
+  case default
+    write(*,*) 'unknown action ', trim(action)
+    stop 2
+  end select
+
+  close(u)
+
+contains
+
+  subroutine read_state()
+    ! STATEFILE: int32 nin, float64 Ke, f0(N,G), dn(N,G,6), b(N,G)
+    integer :: us, itmp
+    if (trim(stname) == '-') then
+      write(*,*) 'action needs a STATEFILE'
+      stop 2
+    end if
+    allocate(f0(nnod, ng), fs0(nnod), dnin(nnod, ng, 6), bb(nnod, ng))
+    open(newunit=us, file=trim(stname), access='stream', form='unformatted', &
+         status='old', action='read')
+    read(us) itmp
+    nin_p = itmp
+    read(us) Ke, f0, dnin, bb
+    close(us)
+  end subroutine read_state
+
+end program ref_dump
