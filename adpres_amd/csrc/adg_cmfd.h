@@ -1,0 +1,423 @@
+// CMFD kernels: coupling coefficients, 7-point matrix, sources, fused
+// BiCGSTAB and the outer-iteration reductions (src/mod_cmfd.f90).
+#pragma once
+
+#include "adg_common.h"
+
+namespace adg {
+
+// ---------------------------------------------------------------------------
+// coup_coef (src/mod_cmfd.f90:11-137) and matrix_setup (:217-304)
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ bool decode_cell(long t, const Geo &q, int &i, int &j, int &k, long &c) {
+  const long nxy = (long)q.nxx * q.nyy;
+  if (t >= nxy * q.nzz) return false;
+  k = (int)(t / nxy);
+  long r = t - (long)k * nxy;
+  j = (int)(r / q.nxx);
+  i = (int)(r - (long)j * q.nxx);
+  c = ((long)(k + 1) * q.NYP + (j + 1)) * q.NXP + (i + 1);
+  return true;
+}
+
+__device__ __forceinline__ double df_bnd(int bc, double D, double h) {
+#pragma clang fp contract(off)
+  const double alb = 1.e30;
+  if (bc == 0) return 2.0 * alb * D / (2.0 * D + alb * h);
+  if (bc == 1) return D / (2.0 * D + 0.5 * h);
+  return 0.0;
+}
+
+__device__ __forceinline__ double df_int(double D, double Dnb, double hnb, double h) {
+#pragma clang fp contract(off)
+  return 2.0 * D * Dnb / (D * hnb + Dnb * h);
+}
+
+// one thread per (cell, group); blockIdx.y = g
+__global__ void __launch_bounds__(NT) k_coupcoef(Geo q, XS xs, double *df) {
+  int i, j, k;
+  long c;
+  long t = (long)blockIdx.x * NT + threadIdx.x;
+  if (!decode_cell(t, q, i, j, k, c)) return;
+  const int g = blockIdx.y;
+  const long NC = q.NC;
+  const unsigned char fl = q.flag[c];
+  if (!(fl & F_ACTIVE)) return;
+  const double *Dg = xs.D + (size_t)g * NC;
+  const double D = Dg[c];
+  const double h[3] = {q.hx[i], q.hy[j], q.hz[k]};
+  const long st[3] = {1, q.sy, q.sz};
+  for (int s = 0; s < 6; ++s) {
+    const int u = s >> 1;
+    const bool up = (s & 1) == 0;
+    double v;
+    if (fl & f_nb(s)) {
+      const long cn = up ? c + st[u] : c - st[u];
+      const double hnb = (u == 0) ? q.hx[up ? i + 1 : i - 1] : (u == 1) ? q.hy[up ? j + 1 : j - 1]
+                                                                         : q.hz[up ? k + 1 : k - 1];
+      v = df_int(D, Dg[cn], hnb, h[u]);
+    } else {
+      v = df_bnd(q.bc[s], D, h[u]);
+    }
+    df[((size_t)s * q.G + g) * NC + c] = v;
+  }
+}
+
+// Matrix slots per row: 0 z-, 1 y-, 2 x-, 3 diag, 4 x+, 5 y+, 6 z+.  Absent
+// neighbours (and every slot of an inactive cell) hold 0.
+template <typename T>
+__global__ void __launch_bounds__(NT) k_matrix(Geo q, XS xs, const double *df, const double *dn,
+                                               double *A64, T *AT) {
+#pragma clang fp contract(off)
+  int i, j, k;
+  long c;
+  long t = (long)blockIdx.x * NT + threadIdx.x;
+  if (!decode_cell(t, q, i, j, k, c)) return;
+  const int g = blockIdx.y;
+  const long NC = q.NC;
+  const unsigned char fl = q.flag[c];
+  double a[7] = {0, 0, 0, 0, 0, 0, 0};
+  if (fl & F_ACTIVE) {
+    double f[6], d[6];
+    for (int s = 0; s < 6; ++s) {
+      f[s] = df[((size_t)s * q.G + g) * NC + c];
+      d[s] = dn[((size_t)s * q.G + g) * NC + c];
+    }
+    const double hx = q.hx[i], hy = q.hy[j], hz = q.hz[k];
+    if (fl & f_nb(5)) a[0] = -(f[5] - d[5]) / hz;
+    if (fl & f_nb(3)) a[1] = -(f[3] - d[3]) / hy;
+    if (fl & f_nb(1)) a[2] = -(f[1] - d[1]) / hx;
+    a[3] = (f[0] + f[1] - d[0] + d[1]) / hx + (f[2] + f[3] - d[2] + d[3]) / hy +
+           (f[4] + f[5] - d[4] + d[5]) / hz + xs.sigr[(size_t)g * NC + c];
+    if (fl & f_nb(0)) a[4] = -(f[0] + d[0]) / hx;
+    if (fl & f_nb(2)) a[5] = -(f[2] + d[2]) / hy;
+    if (fl & f_nb(4)) a[6] = -(f[4] + d[4]) / hz;
+  }
+  for (int s = 0; s < 7; ++s) {
+    if (A64) A64[((size_t)g * 7 + s) * NC + c] = a[s];
+    if (AT) AT[((size_t)g * 7 + s) * NC + c] = (T)a[s];
+  }
+}
+
+// y(c) = A_g x in the reference's slot order (sp_matvec, src/mod_cmfd.f90:1260-1265);
+// absent slots contribute an exact +0.
+template <typename TA, typename TX>
+__device__ __forceinline__ double stencil(const TA *__restrict__ A, long NC, const TX *__restrict__ x,
+                                          long c, long sy, long sz) {
+#pragma clang fp contract(off)
+  double s = 0.0;
+  s = s + (double)A[0 * NC + c] * (double)x[c - sz];
+  s = s + (double)A[1 * NC + c] * (double)x[c - sy];
+  s = s + (double)A[2 * NC + c] * (double)x[c - 1];
+  s = s + (double)A[3 * NC + c] * (double)x[c];
+  s = s + (double)A[4 * NC + c] * (double)x[c + 1];
+  s = s + (double)A[5 * NC + c] * (double)x[c + sy];
+  s = s + (double)A[6 * NC + c] * (double)x[c + sz];
+  return s;
+}
+
+__global__ void __launch_bounds__(NT) k_matvec(Geo q, const double *A64, int g, const double *x, double *y) {
+  for (long c = q.c0 + (long)blockIdx.x * NT + threadIdx.x; c < q.c0 + q.M; c += (long)gridDim.x * NT)
+    y[c] = stencil(A64 + (size_t)g * 7 * q.NC, q.NC, x, c, q.sy, q.sz);
+}
+
+// ---------------------------------------------------------------------------
+// sources: TSrc / TSrcAd (src/mod_cmfd.f90:1006-1064)
+// ---------------------------------------------------------------------------
+
+__global__ void __launch_bounds__(NT) k_src(Geo q, XS xs, const Scal *sc, int g, int adjoint, int first_group,
+                                            const double *f0, double *f0c, const double *fs0, double *fs0c,
+                                            double *b) {
+#pragma clang fp contract(off)
+  if (sc->done) return;
+  const double Ke = sc->Ke;
+  const long NC = q.NC;
+  const int G = q.G;
+  for (long c = q.c0 + (long)blockIdx.x * NT + threadIdx.x; c < q.c0 + q.M; c += (long)gridDim.x * NT) {
+    if (first_group) fs0c[c] = fs0[c];
+    f0c[(size_t)g * NC + c] = f0[(size_t)g * NC + c];
+    double s0 = 0.0;
+    for (int h = 0; h < G; ++h) {
+      if (h == g) continue;
+      const double sg = adjoint ? xs.sigs[((size_t)h * G + g) * NC + c] : xs.sigs[((size_t)g * G + h) * NC + c];
+      s0 = s0 + sg * f0[(size_t)h * NC + c];
+    }
+    const double w = adjoint ? xs.nuf[(size_t)g * NC + c] : xs.chi[(size_t)g * NC + c];
+    b[c] = w * fs0[c] / Ke + s0 + xs.exsrc[(size_t)g * NC + c];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// BiCGSTAB (src/mod_cmfd.f90:1203-1243), three fused stencil kernels per
+// iteration plus a setup kernel.  TV is the vector/matrix type of the
+// iteration (double, or float for the FP32 form); the residual of the setup
+// is always formed in FP64 from the FP64 matrix and FP64 flux.
+//   setup: r = b - A x (FP64), rs = r, d = 0            dot (rs, r)
+//   K1:    p = r + beta (p - omega v) at c and its 6 neighbours,
+//          v = A p                                     dot (rs, v)
+//   K2:    s = r - alpha v at c and its neighbours,
+//          t = A s                                     dots (t, t), (t, s)
+//   K3:    d += alpha p + omega s,  r = s - omega t     dot (rs, r)
+// For TV = double, d is the flux itself; for float, d is the correction that
+// the last K3 adds to the FP64 flux.
+// ---------------------------------------------------------------------------
+
+template <typename TV>
+__global__ void __launch_bounds__(NT) k_bicg_setup(Geo q, Scal *sc, const double *A64g, const double *b,
+                                                   const double *x, TV *r, TV *rs, TV *d, double *part,
+                                                   unsigned *cnt) {
+#pragma clang fp contract(off)
+  if (sc->done) return;
+  double acc[1] = {0.0};
+  for (long c = q.c0 + (long)blockIdx.x * NT + threadIdx.x; c < q.c0 + q.M; c += (long)gridDim.x * NT) {
+    const double rr = b[c] - stencil(A64g, q.NC, x, c, q.sy, q.sz);
+    const TV rv = (TV)rr;
+    r[c] = rv;
+    rs[c] = rv;
+    if (d) d[c] = (TV)0;
+    acc[0] += (double)rv * (double)rv;
+  }
+  if (reduce_last<1, 0u>(acc, part, cnt)) {
+    const double rho = acc[0];
+    sc->rho_prev = 1.0;
+    sc->alpha = 1.0;
+    sc->omega = 1.0;
+    sc->rho = rho;
+    sc->beta = (rho / 1.0) * (1.0 / 1.0);
+  }
+}
+
+template <typename TV>
+__device__ __forceinline__ TV pdir(const TV *r, const TV *p, const TV *v, long c, TV beta, TV omega, int first) {
+#pragma clang fp contract(off)
+  return first ? r[c] : (TV)(r[c] + beta * (p[c] - omega * v[c]));
+}
+
+template <typename TV>
+__global__ void __launch_bounds__(NT) k_bicg1(Geo q, Scal *sc, const TV *AT, const TV *r, const TV *rs,
+                                              const TV *p, const TV *v, TV *pout, TV *vout, int first,
+                                              double *part, unsigned *cnt) {
+#pragma clang fp contract(off)
+  if (sc->done) return;
+  const TV beta = (TV)sc->beta, omega = (TV)sc->omega;
+  const long NC = q.NC, sy = q.sy, sz = q.sz;
+  double acc[1] = {0.0};
+  for (long c = q.c0 + (long)blockIdx.x * NT + threadIdx.x; c < q.c0 + q.M; c += (long)gridDim.x * NT) {
+    const TV pc = pdir(r, p, v, c, beta, omega, first);
+    double s = 0.0;
+    s = s + (double)(AT[0 * NC + c] * pdir(r, p, v, c - sz, beta, omega, first));
+    s = s + (double)(AT[1 * NC + c] * pdir(r, p, v, c - sy, beta, omega, first));
+    s = s + (double)(AT[2 * NC + c] * pdir(r, p, v, c - 1, beta, omega, first));
+    s = s + (double)(AT[3 * NC + c] * pc);
+    s = s + (double)(AT[4 * NC + c] * pdir(r, p, v, c + 1, beta, omega, first));
+    s = s + (double)(AT[5 * NC + c] * pdir(r, p, v, c + sy, beta, omega, first));
+    s = s + (double)(AT[6 * NC + c] * pdir(r, p, v, c + sz, beta, omega, first));
+    const TV vc = (TV)s;
+    // neighbours read the previous p and v, so the new ones go to the other
+    // buffer of a ping-pong pair
+    pout[c] = pc;
+    vout[c] = vc;
+    acc[0] += (double)rs[c] * (double)vc;
+  }
+  if (reduce_last<1, 0u>(acc, part, cnt)) sc->alpha = sc->rho / acc[0];
+}
+
+template <typename TV>
+__global__ void __launch_bounds__(NT) k_bicg2(Geo q, Scal *sc, const TV *AT, const TV *r, const TV *v, TV *t,
+                                              double *part, unsigned *cnt) {
+#pragma clang fp contract(off)
+  if (sc->done) return;
+  const TV alpha = (TV)sc->alpha;
+  const long NC = q.NC, sy = q.sy, sz = q.sz;
+  double acc[2] = {0.0, 0.0};
+  for (long c = q.c0 + (long)blockIdx.x * NT + threadIdx.x; c < q.c0 + q.M; c += (long)gridDim.x * NT) {
+    const TV sc_ = (TV)(r[c] - alpha * v[c]);
+    double s = 0.0;
+    s = s + (double)(AT[0 * NC + c] * (TV)(r[c - sz] - alpha * v[c - sz]));
+    s = s + (double)(AT[1 * NC + c] * (TV)(r[c - sy] - alpha * v[c - sy]));
+    s = s + (double)(AT[2 * NC + c] * (TV)(r[c - 1] - alpha * v[c - 1]));
+    s = s + (double)(AT[3 * NC + c] * sc_);
+    s = s + (double)(AT[4 * NC + c] * (TV)(r[c + 1] - alpha * v[c + 1]));
+    s = s + (double)(AT[5 * NC + c] * (TV)(r[c + sy] - alpha * v[c + sy]));
+    s = s + (double)(AT[6 * NC + c] * (TV)(r[c + sz] - alpha * v[c + sz]));
+    const TV tc = (TV)s;
+    t[c] = tc;
+    acc[0] += (double)tc * (double)tc;
+    acc[1] += (double)tc * (double)sc_;
+  }
+  if (reduce_last<2, 0u>(acc, part, cnt)) sc->omega = acc[1] / acc[0];
+}
+
+template <typename TV>
+__global__ void __launch_bounds__(NT) k_bicg3(Geo q, Scal *sc, const TV *p, const TV *v, const TV *t, TV *r,
+                                              const TV *rs, TV *d, double *x64, int last, double *part,
+                                              unsigned *cnt) {
+#pragma clang fp contract(off)
+  if (sc->done) return;
+  const TV alpha = (TV)sc->alpha, omega = (TV)sc->omega;
+  double acc[1] = {0.0};
+  for (long c = q.c0 + (long)blockIdx.x * NT + threadIdx.x; c < q.c0 + q.M; c += (long)gridDim.x * NT) {
+    const TV s = (TV)(r[c] - alpha * v[c]);
+    const TV dn = (TV)(d[c] + alpha * p[c] + omega * s);
+    d[c] = dn;
+    if (last && x64) x64[c] = x64[c] + (double)dn;
+    const TV rn = (TV)(s - omega * t[c]);
+    r[c] = rn;
+    acc[0] += (double)rs[c] * (double)rn;
+  }
+  if (reduce_last<1, 0u>(acc, part, cnt)) {
+    sc->rho_prev = sc->rho;
+    sc->rho = acc[0];
+    sc->beta = (sc->rho / sc->rho_prev) * (sc->alpha / sc->omega);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// FSrc/FSrcAd (src/mod_cmfd.f90:956-1002), the error vector and its l2 norm
+// (:480-481, l2norm :1100), RelEg (:1172) -- then extrapolation
+// (fiss_extrp :308-335), Integrate (:1120), the k update (:485) and RelE
+// (:1143).
+// ---------------------------------------------------------------------------
+
+__global__ void __launch_bounds__(NT) k_fsrc_init(Geo q, XS xs, int adjoint, const double *f0, double *fs0) {
+#pragma clang fp contract(off)
+  const long NC = q.NC;
+  for (long c = q.c0 + (long)blockIdx.x * NT + threadIdx.x; c < q.c0 + q.M; c += (long)gridDim.x * NT) {
+    double fs = 0.0;
+    for (int g = 0; g < q.G; ++g) {
+      const double w = adjoint ? xs.chi[(size_t)g * NC + c] : xs.nuf[(size_t)g * NC + c];
+      fs = fs + f0[(size_t)g * NC + c] * w;
+    }
+    fs0[c] = fs;
+  }
+}
+
+// f = Integrate(fs0), e1 = Integrate(1) at the start of outer (:456-458)
+__global__ void __launch_bounds__(NT) k_integrals(Geo q, XS xs, Scal *sc, const double *fs0, double *part,
+                                                  unsigned *cnt) {
+#pragma clang fp contract(off)
+  double acc[2] = {0.0, 0.0};
+  for (long c = q.c0 + (long)blockIdx.x * NT + threadIdx.x; c < q.c0 + q.M; c += (long)gridDim.x * NT) {
+    acc[0] += xs.vdel[c] * fs0[c];
+    acc[1] += xs.vdel[c];
+  }
+  if (reduce_last<2, 0u>(acc, part, cnt)) {
+    sc->f = acc[0];
+    sc->e1 = acc[1];
+  }
+}
+
+__global__ void __launch_bounds__(NT) k_fsrc(Geo q, XS xs, Scal *sc, int adjoint, int extrap, const double *f0,
+                                             const double *f0c, double *fs0, const double *fs0c, double *errn,
+                                             double *part, unsigned *cnt) {
+#pragma clang fp contract(off)
+  if (sc->done) return;
+  const long NC = q.NC;
+  double acc[2] = {0.0, 0.0};   // sum errn^2, max flux rel. error
+  for (long c = q.c0 + (long)blockIdx.x * NT + threadIdx.x; c < q.c0 + q.M; c += (long)gridDim.x * NT) {
+    double fs = 0.0;
+    for (int g = 0; g < q.G; ++g) {
+      const double w = adjoint ? xs.chi[(size_t)g * NC + c] : xs.nuf[(size_t)g * NC + c];
+      const double fg = f0[(size_t)g * NC + c];
+      fs = fs + fg * w;
+      if (fabs(fg) > 1.e-10) acc[1] = fmax(acc[1], fabs(fg - f0c[(size_t)g * NC + c]) / fabs(fg));
+    }
+    fs0[c] = fs;
+    const double e = fs - fs0c[c];
+    errn[c] = e;
+    acc[0] += e * e;
+  }
+  if (reduce_last<2, 2u>(acc, part, cnt)) {
+    const double e2 = sqrt(acc[0]);
+    double coef = 0.0;
+    if (extrap) {
+      const double domiR = e2 / sc->e1;
+      coef = domiR / (1.0 - domiR);
+    }
+    sc->coef = coef;
+    sc->e2 = e2;
+    sc->e1 = e2;
+    sc->fer = acc[1];
+    sc->flags = extrap ? 1 : 0;
+  }
+}
+
+__global__ void __launch_bounds__(NT) k_post(Geo q, XS xs, Scal *sc, int extrap, double *fs0, const double *fs0c,
+                                             const double *errn, double *part, unsigned *cnt) {
+#pragma clang fp contract(off)
+  if (sc->done) return;
+  const double coef = sc->coef;
+  double acc[2] = {0.0, 0.0};   // integral, max fission-source rel. error
+  for (long c = q.c0 + (long)blockIdx.x * NT + threadIdx.x; c < q.c0 + q.M; c += (long)gridDim.x * NT) {
+    double fs = fs0[c];
+    if (extrap) {
+      fs = fs + coef * errn[c];
+      fs0[c] = fs;
+    }
+    acc[0] += xs.vdel[c] * fs;
+    if (fabs(fs) > 1.e-10) acc[1] = fmax(acc[1], fabs(fs - fs0c[c]) / fabs(fs));
+  }
+  if (reduce_last<2, 2u>(acc, part, cnt)) {
+    const double fc = sc->f;
+    sc->f = acc[0];
+    sc->Ke = sc->Ke * acc[0] / fc;
+    sc->ser = acc[1];
+  }
+}
+
+// end of one outer: trace, error checks, convergence test (:491-496)
+__global__ void k_end(Scal *sc, int p, int nodal, double serc, double ferc, double *tk, double *tser,
+                      double *tfer, double *tnd, int *tflags, int *tloc) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  if (sc->done) return;
+  int flags = sc->flags | (nodal ? 2 : 0);
+  const int t = p - 1;
+  tk[t] = sc->Ke;
+  tser[t] = sc->ser;
+  tfer[t] = sc->fer;
+  tnd[t] = sc->ndmax;
+  tflags[t] = flags;
+  tloc[3 * t] = sc->im;
+  tloc[3 * t + 1] = sc->jm;
+  tloc[3 * t + 2] = sc->km;
+  sc->p = p;
+  if (nodal && sc->lu_key != ~0ull) {
+    sc->status = 3;
+    sc->done = 1;
+    return;
+  }
+  if (nodal && sc->ndmax > 1.e3) {
+    sc->status = 2;
+    sc->done = 1;
+    return;
+  }
+  if (sc->ser < serc && sc->fer < ferc && sc->ndmax < 1.e-2) sc->done = 1;
+}
+
+// PowDis (src/mod_cmfd.f90:1290-1333)
+__global__ void __launch_bounds__(NT) k_powdis(Geo q, XS xs, Scal *sc, const double *f0, double *pw, double *part,
+                                               unsigned *cnt) {
+#pragma clang fp contract(off)
+  const long NC = q.NC;
+  double acc[1] = {0.0};
+  for (long c = q.c0 + (long)blockIdx.x * NT + threadIdx.x; c < q.c0 + q.M; c += (long)gridDim.x * NT) {
+    double pc = 0.0;
+    for (int g = 0; g < q.G; ++g) {
+      double w = f0[(size_t)g * NC + c] * xs.sigf[(size_t)g * NC + c] * xs.vdel[c];
+      if (w < 0.0) w = 0.0;
+      pc = pc + w;
+    }
+    pw[c] = pc;
+    acc[0] += pc;
+  }
+  if (reduce_last<1, 0u>(acc, part, cnt)) sc->tpow = acc[0];
+}
+
+__global__ void __launch_bounds__(NT) k_scale(Geo q, const Scal *sc, double *pw) {
+  const double tp = sc->tpow;
+  for (long c = q.c0 + (long)blockIdx.x * NT + threadIdx.x; c < q.c0 + q.M; c += (long)gridDim.x * NT)
+    pw[c] = pw[c] / tp;
+}
+
+}  // namespace adg

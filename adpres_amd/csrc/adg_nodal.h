@@ -1,0 +1,590 @@
+// Nodal (SANM / PNM) two-node update of the D-hat coupling coefficients
+// (src/mod_nodal.f90), reorganised for the GPU.
+//
+// In the reference every quantity a line sweep needs is a function of the
+// state at the start of the update (flux, k, old D-hat, cross sections):
+// the sweep only carries a2/a4/L1 of the previous node forward and writes
+// each face's D-hat exactly once.  The update therefore splits into three
+// data-parallel passes with identical arithmetic:
+//   pass 0  per (node, group):      zeroth transverse leakages S1..S3 (get_source)
+//   pass 1  per (node, direction):  B, A..H, Lm2 -> a2 (G x G LU), a4, L1 (TLUpd1)
+//   pass 2  per (face, direction):  a1 from the 2G x 2G interface system (or the
+//                                   G x G boundary system), a3, surface current,
+//                                   new D-hat, max |change| with its location
+// The argmax is reduced with ties broken by the reference's sweep order
+// (x lines, then y, then z), so ndmax and its (i,j,k) match exactly.
+#pragma once
+
+#include "adg_common.h"
+
+namespace adg {
+
+struct NodArgs {
+  Geo q;
+  XS xs;
+  Scal *sc;
+  const double *f0, *df;
+  double *dn;
+  double *S;            // [3][G][NC]
+  double *a2, *a4, *L1; // [3][G][NC]
+  int kern;             // 1 PNM, 2 SANM
+  int cmode;            // 1 forward, 0 adjoint
+  double *part;
+  unsigned *cnt;
+};
+
+__device__ __forceinline__ long cell_of(const Geo &q, int i, int j, int k) {
+  return ((long)(k + 1) * q.NYP + (j + 1)) * q.NXP + (i + 1);
+}
+
+__device__ __forceinline__ double hdir(const Geo &q, int u, int i, int j, int k) {
+  return u == 0 ? q.hx[i] : (u == 1 ? q.hy[j] : q.hz[k]);
+}
+
+// ---------------------------------------------------------------------------
+// pass 0: Lxyz + get_source (src/mod_nodal.f90:901-1043), cmode /= 2
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(NT) k_nod_src(NodArgs a) {
+#pragma clang fp contract(off)
+  const Geo &q = a.q;
+  int i, j, k;
+  long c;
+  const long nxy = (long)q.nxx * q.nyy;
+  long t = (long)blockIdx.x * NT + threadIdx.x;
+  if (t >= nxy * q.nzz) return;
+  if (a.sc->done) return;
+  k = (int)(t / nxy);
+  { long r = t - (long)k * nxy; j = (int)(r / q.nxx); i = (int)(r - (long)j * q.nxx); }
+  c = cell_of(q, i, j, k);
+  const unsigned char fl = q.flag[c];
+  if (!(fl & F_ACTIVE)) return;
+  const int g = blockIdx.y;
+  const long NC = q.NC;
+  const int G = q.G;
+  const double *f = a.f0 + (size_t)g * NC;
+  const double fn = f[c];
+  const long st[3] = {1, q.sy, q.sz};
+  double L[3];
+  for (int u = 0; u < 3; ++u) {
+    const int sp = 2 * u, sm = 2 * u + 1;
+    const double dfp = a.df[((size_t)sp * G + g) * NC + c], dnp = a.dn[((size_t)sp * G + g) * NC + c];
+    const double dfm = a.df[((size_t)sm * G + g) * NC + c], dnm = a.dn[((size_t)sm * G + g) * NC + c];
+    double jp, jm;
+    if (!(fl & f_nb(sp))) {
+      jp = (q.bc[sp] == 2) ? 0.0 : dfp * fn - dnp * fn;
+    } else {
+      const double fp = f[c + st[u]];
+      jp = -dfp * (fp - fn) - dnp * (fp + fn);
+    }
+    if (!(fl & f_nb(sm))) {
+      jm = (q.bc[sm] == 2) ? 0.0 : -dfm * fn - dnm * fn;
+    } else {
+      const double fm = f[c - st[u]];
+      jm = -dfm * (fn - fm) - dnm * (fn + fm);
+    }
+    L[u] = (jp - jm) / hdir(q, u, i, j, k);
+  }
+  a.S[((size_t)0 * G + g) * NC + c] = L[1] + L[2];
+  a.S[((size_t)1 * G + g) * NC + c] = L[0] + L[2];
+  a.S[((size_t)2 * G + g) * NC + c] = L[0] + L[1];
+}
+
+// ---------------------------------------------------------------------------
+// per-node helpers
+// ---------------------------------------------------------------------------
+
+// get_B (src/mod_nodal.f90:1345-1405); B row-major [g*G+h]
+template <int G>
+__device__ __forceinline__ void nod_B(const NodArgs &a, long c, double h, double Ke, double *B) {
+#pragma clang fp contract(off)
+  const long NC = a.q.NC;
+  const XS &x = a.xs;
+  for (int g = 0; g < G; ++g) {
+    const double D = x.D[(size_t)g * NC + c];
+    for (int hh = 0; hh < G; ++hh) {
+      double dum;
+      if (a.cmode == 1) {
+        if (g == hh) dum = x.sigr[(size_t)g * NC + c] - x.chi[(size_t)g * NC + c] * x.nuf[(size_t)hh * NC + c] / Ke;
+        else dum = -x.sigs[((size_t)g * G + hh) * NC + c] - x.chi[(size_t)g * NC + c] * x.nuf[(size_t)hh * NC + c] / Ke;
+      } else {
+        if (g == hh) dum = x.sigr[(size_t)g * NC + c] - x.chi[(size_t)g * NC + c] * x.nuf[(size_t)hh * NC + c] / Ke;
+        else dum = -x.sigs[((size_t)hh * G + g) * NC + c] - x.chi[(size_t)hh * NC + c] * x.nuf[(size_t)g * NC + c] / Ke;
+      }
+      B[g * G + hh] = 0.25 * (h * h) / D * dum;
+    }
+  }
+}
+
+// get_ABEFGH (src/mod_nodal.f90:1409-1451) or the PNM constants (:180-182)
+struct Coef6 {
+  double A, B, E, F, G, H;
+};
+
+__device__ __forceinline__ Coef6 nod_coef(int kern, double sigr, double D, double h) {
+#pragma clang fp contract(off)
+  Coef6 r;
+  if (kern != 2) {
+    r.A = 1.0 / 15.0; r.B = 1.0 / 35.0; r.E = 2.0 / 7.0; r.F = 2.0 / 5.0; r.G = 10.0; r.H = 6.0;
+    return r;
+  }
+  const double alp = 0.5 * sqrt(sigr / D) * h;
+  const double alp2 = alp * alp;
+  const double sh = sinh(alp), ch = cosh(alp);
+  const double m0c = sh / alp;
+  const double m1s = 3.0 * (ch / alp - sh / alp2);
+  const double m2c = 5.0 * (sh / alp - 3.0 * ch / alp2 + 3.0 * sh / (alp * alp * alp));
+  r.A = (sh - m1s) / (alp2 * m1s);
+  r.B = (ch - m0c - m2c) / (alp2 * m2c);
+  r.E = (m0c / m2c - 3.0 / alp2);
+  r.F = (alp * ch - m1s) / (alp2 * m1s);
+  r.G = (alp * sh - 3.0 * m2c) / (ch - m0c - m2c);
+  r.H = (alp * ch - m1s) / (sh - m1s);
+  return r;
+}
+
+// LU_solve (src/mod_nodal.f90:829-897): Doolittle without pivoting, done in
+// place (L below the diagonal) with the reference's operation order.
+// Returns false if an input diagonal is below the reference's threshold.
+template <int M>
+__device__ __forceinline__ bool lu_solve(double *m, double *b) {
+#pragma clang fp contract(off)
+  bool ok = true;
+  for (int i = 0; i < M; ++i)
+    if (fabs(m[i * M + i]) < (double)10e-5f) ok = false;
+  for (int i = 0; i < M; ++i) {
+    for (int j = i + 1; j < M; ++j) {
+      const double piv = m[j * M + i] / m[i * M + i];
+      for (int k = i + 1; k < M; ++k) m[j * M + k] = m[j * M + k] - piv * m[i * M + k];
+      m[j * M + i] = piv;
+    }
+  }
+  // forward: y(i) = b(i) - sum_k<i L(i,k) y(k)
+  for (int i = 1; i < M; ++i) {
+    double isum = 0.0;
+    for (int k = 0; k < i; ++k) isum = isum + m[i * M + k] * b[k];
+    b[i] = b[i] - isum;
+  }
+  b[M - 1] = b[M - 1] / m[(M - 1) * M + (M - 1)];
+  for (int i = M - 2; i >= 0; --i) {
+    double isum = 0.0;
+    for (int k = i + 1; k < M; ++k) isum = isum + m[i * M + k] * b[k];
+    b[i] = (b[i] - isum) / m[i * M + i];
+  }
+  return ok;
+}
+
+// Neighbourhood of a node along direction u.
+struct Line {
+  bool first, last;    // no lower / no upper neighbour
+  long cm, cp;         // neighbour cells (valid when present)
+  double hm, hn, hp;
+};
+
+__device__ __forceinline__ Line line_at(const Geo &q, int u, int i, int j, int k, long c) {
+  Line L;
+  const unsigned char fl = q.flag[c];
+  const long st = u == 0 ? 1 : (u == 1 ? q.sy : q.sz);
+  L.first = !(fl & f_nb(2 * u + 1));
+  L.last = !(fl & f_nb(2 * u));
+  L.cm = c - st;
+  L.cp = c + st;
+  const double *h = u == 0 ? q.hx : (u == 1 ? q.hy : q.hz);
+  const int ii = u == 0 ? i : (u == 1 ? j : k);
+  L.hn = h[ii];
+  L.hm = L.first ? 0.0 : h[ii - 1];
+  L.hp = L.last ? 0.0 : h[ii + 1];
+  return L;
+}
+
+// TLUpd1 (src/mod_nodal.f90:1047-1199) without the h^2/(4D) factor
+__device__ __forceinline__ double tl1(const Line &L, const double *S, long c, int bclo, int bchi) {
+#pragma clang fp contract(off)
+  const double Sn = S[c];
+  double tm, tp, p1m, p2m, p1p, p2p, hpp;
+  if (L.first) {
+    if (bclo == 2) {
+      tm = 1.0; tp = L.hp / L.hn;
+      p1m = tm + 1.0; p2m = 2.0 * tm + 1.0; p1p = tp + 1.0;
+      hpp = 2.0 * p1m * p1p * (tm + tp + 1.0);
+      return (p1m * p2m * (S[L.cp] - Sn)) / hpp;
+    }
+    tp = L.hp / L.hn; p1p = tp + 1.0;
+    return (S[L.cp] - Sn) / p1p;
+  }
+  if (L.last) {
+    if (bchi == 2) {
+      tm = L.hm / L.hn; tp = 1.0;
+      p1m = tm + 1.0; p1p = tp + 1.0; p2p = 2.0 * tp + 1.0;
+      hpp = 2.0 * p1m * p1p * (tm + tp + 1.0);
+      return (p1p * p2p * (Sn - S[L.cm])) / hpp;
+    }
+    tm = L.hm / L.hn; p1m = tm + 1.0;
+    return (Sn - S[L.cm]) / p1m;
+  }
+  tm = L.hm / L.hn; tp = L.hp / L.hn;
+  p1m = tm + 1.0; p2m = 2.0 * tm + 1.0; p1p = tp + 1.0; p2p = 2.0 * tp + 1.0;
+  hpp = 2.0 * p1m * p1p * (tm + tp + 1.0);
+  return (p1m * p2m * (S[L.cp] - Sn) + p1p * p2p * (Sn - S[L.cm])) / hpp;
+}
+
+// TLUpd2 (src/mod_nodal.f90:1203-1341) without the h^2/(4D) factor
+__device__ __forceinline__ double tl2(const Line &L, const double *S, long c, int bclo, int bchi) {
+#pragma clang fp contract(off)
+  const double Sn = S[c];
+  double tm, tp, p1m, p1p, hpp;
+  if (L.first) {
+    if (bclo == 2) {
+      tm = 1.0; tp = L.hp / L.hn;
+      p1m = tm + 1.0; p1p = tp + 1.0;
+      hpp = 2.0 * p1m * p1p * (tm + tp + 1.0);
+      return (p1m * (S[L.cp] - Sn)) / hpp;
+    }
+    return 0.0;
+  }
+  if (L.last) {
+    if (bchi == 2) {
+      tm = L.hm / L.hn; tp = 1.0;
+      p1m = tm + 1.0; p1p = tp + 1.0;
+      hpp = 2.0 * p1m * p1p * (tm + tp + 1.0);
+      return (p1p * (S[L.cm] - Sn)) / hpp;
+    }
+    return 0.0;
+  }
+  tm = L.hm / L.hn; tp = L.hp / L.hn;
+  p1m = tm + 1.0; p1p = tp + 1.0;
+  hpp = 2.0 * p1m * p1p * (tm + tp + 1.0);
+  return (p1m * (S[L.cp] - Sn) + p1p * (S[L.cm] - Sn)) / hpp;
+}
+
+__device__ __forceinline__ unsigned long long lu_key(unsigned long long order, int i, int j, int k) {
+  return (order << 30) | ((unsigned long long)(i + 1) << 20) | ((unsigned long long)(j + 1) << 10) |
+         (unsigned long long)(k + 1);
+}
+
+// Sweep-order index of the face below cell (i,j,k) along u, i.e. the face
+// at which the reference handles this node as "p"; faces of one direction
+// are numbered in the reference's line order, directions x < y < z.
+__device__ __forceinline__ unsigned long long face_order(const Geo &q, int u, int fi, int j_or_i, int k_or_j) {
+  const unsigned long long nx = q.nxx, ny = q.nyy, nz = q.nzz;
+  const unsigned long long X = (nx + 1) * ny * nz, Y = (ny + 1) * nx * nz;
+  if (u == 0) return ((unsigned long long)k_or_j * ny + j_or_i) * (nx + 1) + fi;          // (k, j, i)
+  if (u == 1) return X + ((unsigned long long)k_or_j * nx + j_or_i) * (ny + 1) + fi;      // (k, i, j)
+  return X + Y + ((unsigned long long)k_or_j * nx + j_or_i) * (nz + 1) + fi;              // (j, i, k)
+}
+
+// ---------------------------------------------------------------------------
+// pass 1: per (node, direction) -- get_a2matvec (:769-825), LU_solve, get_a4
+// (:740-765) and TLUpd1 -- blockIdx.y = direction
+// ---------------------------------------------------------------------------
+template <int G>
+__global__ void __launch_bounds__(NT) k_nod_pass1(NodArgs a) {
+#pragma clang fp contract(off)
+  const Geo &q = a.q;
+  const long nxy = (long)q.nxx * q.nyy;
+  long t = (long)blockIdx.x * NT + threadIdx.x;
+  if (t >= nxy * q.nzz) return;
+  if (a.sc->done) return;
+  int i, j, k;
+  k = (int)(t / nxy);
+  { long r = t - (long)k * nxy; j = (int)(r / q.nxx); i = (int)(r - (long)j * q.nxx); }
+  const long c = cell_of(q, i, j, k);
+  if (!(q.flag[c] & F_ACTIVE)) return;
+  const int u = blockIdx.y;
+  const long NC = q.NC;
+  const Line L = line_at(q, u, i, j, k, c);
+  const int bclo = q.bc[2 * u + 1], bchi = q.bc[2 * u];
+  if (L.first && L.last) {   // single-node line: the reference reads an undefined neighbour
+    atomicMin(&a.sc->lu_key, lu_key(0, i, j, k));
+    return;
+  }
+  const double h = L.hn, Ke = a.sc->Ke;
+  double B[G * G], M[G * G], rhs[G], Lm2[G];
+  Coef6 cf[G];
+  nod_B<G>(a, c, h, Ke, B);
+  for (int g = 0; g < G; ++g) cf[g] = nod_coef(a.kern, a.xs.sigr[(size_t)g * NC + c], a.xs.D[(size_t)g * NC + c], h);
+  const double *Su = a.S + (size_t)u * G * NC;
+  for (int g = 0; g < G; ++g) {
+    const double D = a.xs.D[(size_t)g * NC + c];
+    const double S = 0.25 * (h * h) / D * (Su[(size_t)g * NC + c] - a.xs.exsrc[(size_t)g * NC + c]);
+    double Bf = 0.0;
+    for (int hh = 0; hh < G; ++hh) {
+      M[g * G + hh] = (hh == g) ? B[g * G + hh] * cf[g].E + 3.0 : B[g * G + hh] * cf[g].E;
+      Bf = Bf + B[g * G + hh] * a.f0[(size_t)hh * NC + c];
+    }
+    Lm2[g] = 0.25 * (h * h) / D * tl2(L, Su + (size_t)g * NC, c, bclo, bchi);
+    rhs[g] = Bf - cf[g].E * Lm2[g] + S;
+  }
+  if (!lu_solve<G>(M, rhs)) {
+    const int fi = u == 0 ? i : (u == 1 ? j : k);
+    const int o1 = u == 0 ? j : i, o2 = u == 2 ? j : k;
+    atomicMin(&a.sc->lu_key, lu_key(face_order(q, u, fi, o1, o2), i, j, k));
+  }
+  for (int g = 0; g < G; ++g) {
+    double Bf = 0.0;
+    for (int hh = 0; hh < G; ++hh) Bf = Bf + B[g * G + hh] * rhs[hh];
+    const size_t o = ((size_t)u * G + g) * NC + c;
+    a.a2[o] = rhs[g];
+    a.a4[o] = cf[g].B * (Bf + Lm2[g]);
+    const double D = a.xs.D[(size_t)g * NC + c];
+    a.L1[o] = 0.25 * (h * h) / D * tl1(L, Su + (size_t)g * NC, c, bclo, bchi);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// pass 2: per face -- get_a1matvec(_first/_last) (:483-698), LU_solve,
+// get_a3 (:702-736) and nodal_coup_upd (:282-371).  blockIdx.y = direction.
+// ---------------------------------------------------------------------------
+template <int G>
+__global__ void __launch_bounds__(NT) k_nod_pass2(NodArgs a) {
+#pragma clang fp contract(off)
+  const Geo &q = a.q;
+  const int u = blockIdx.y;
+  const long NC = q.NC;
+  // faces of direction u: (n_u + 1) x (other two extents)
+  const int nu = u == 0 ? q.nxx : (u == 1 ? q.nyy : q.nzz);
+  const int e1 = u == 0 ? q.nyy : q.nxx;          // second index extent (j for x; i for y, z)
+  const int e2 = u == 2 ? q.nyy : q.nzz;          // slowest (k for x, y; j for z)
+  const long nf = (long)(nu + 1) * e1 * e2;
+  long t = (long)blockIdx.x * NT + threadIdx.x;
+  double best = -1.0, bkey = 0.0, bloc = 0.0;
+  if (t < nf && !a.sc->done) {
+    const int fi = (int)(t % (nu + 1));
+    const long r = t / (nu + 1);
+    const int o1 = (int)(r % e1), o2 = (int)(r / e1);
+    int i, j, k;
+    if (u == 0) { i = fi; j = o1; k = o2; }
+    else if (u == 1) { i = o1; j = fi; k = o2; }
+    else { i = o1; j = o2; k = fi; }
+    const long st = u == 0 ? 1 : (u == 1 ? q.sy : q.sz);
+    const long cp = cell_of(q, i, j, k);    // node above the face ("p")
+    const long cn = cp - st;                // node below the face ("n")
+    const bool hasp = (q.flag[cp] & F_ACTIVE) != 0;
+    const bool hasn = (q.flag[cn] & F_ACTIVE) != 0;
+    if (hasp || hasn) {
+      const double Ke = a.sc->Ke;
+      const int sf = 2 * u;                 // upper face of n (x+, y+, z+), sf+1 lower face of p
+      const unsigned long long ord = face_order(q, u, fi, o1, o2);
+      const size_t ub = (size_t)u * G * NC;
+      int li = i, lj = j, lk = k;           // location reported: n, or p for a first node
+      if (hasn) { if (u == 0) li = i - 1; else if (u == 1) lj = j - 1; else lk = k - 1; }
+      double a1[G], a3[G];
+      Coef6 cfn[G], cfp[G];
+      double Bn[G * G], Bp[G * G];
+      if (hasn && hasp) {
+        const double hn = hdir(q, u, u == 0 ? i - 1 : i, u == 1 ? j - 1 : j, u == 2 ? k - 1 : k);
+        const double hp = hdir(q, u, i, j, k);
+        nod_B<G>(a, cn, hn, Ke, Bn);
+        nod_B<G>(a, cp, hp, Ke, Bp);
+        for (int g = 0; g < G; ++g) {
+          cfn[g] = nod_coef(a.kern, a.xs.sigr[(size_t)g * NC + cn], a.xs.D[(size_t)g * NC + cn], hn);
+          cfp[g] = nod_coef(a.kern, a.xs.sigr[(size_t)g * NC + cp], a.xs.D[(size_t)g * NC + cp], hp);
+        }
+        constexpr int M2 = 2 * G;
+        double R[M2 * M2], s[M2];
+        for (int g = 0; g < G; ++g) {
+          const double Ln1 = a.L1[ub + (size_t)g * NC + cn], Lp1 = a.L1[ub + (size_t)g * NC + cp];
+          const double a2n = a.a2[ub + (size_t)g * NC + cn], a4n = a.a4[ub + (size_t)g * NC + cn];
+          const double a2p = a.a2[ub + (size_t)g * NC + cp], a4p = a.a4[ub + (size_t)g * NC + cp];
+          const double Pn = 2.0 * a.xs.D[(size_t)g * NC + cn] / hn;
+          const double Pp = 2.0 * a.xs.D[(size_t)g * NC + cp] / hp;
+          for (int hh = 0; hh < G; ++hh) {
+            if (hh == g) {
+              R[g * M2 + g] = -Pn * (Bn[g * G + hh] * cfn[g].F + 1.0);
+              R[g * M2 + g + G] = Pp * (Bp[g * G + hh] * cfp[g].F + 1.0);
+            } else {
+              R[g * M2 + hh] = -Pn * Bn[g * G + hh] * cfn[g].F;
+              R[g * M2 + hh + G] = Pp * Bp[g * G + hh] * cfp[g].F;
+            }
+          }
+          s[g] = Pn * (3.0 * a2n + cfn[g].G * a4n + cfn[g].F * Ln1) + Pp * (3.0 * a2p + cfp[g].G * a4p - cfp[g].F * Lp1);
+        }
+        for (int g = 0; g < G; ++g) {
+          const double dcn = a.xs.dc[((size_t)sf * G + g) * NC + cn];
+          const double dcp = a.xs.dc[((size_t)(sf + 1) * G + g) * NC + cp];
+          const double Ln1 = a.L1[ub + (size_t)g * NC + cn], Lp1 = a.L1[ub + (size_t)g * NC + cp];
+          const double a2n = a.a2[ub + (size_t)g * NC + cn], a4n = a.a4[ub + (size_t)g * NC + cn];
+          const double a2p = a.a2[ub + (size_t)g * NC + cp], a4p = a.a4[ub + (size_t)g * NC + cp];
+          for (int hh = 0; hh < G; ++hh) {
+            if (hh == g) {
+              R[(g + G) * M2 + g] = dcn * (Bn[g * G + hh] * cfn[g].A + 1.0);
+              R[(g + G) * M2 + g + G] = dcp * (Bp[g * G + hh] * cfp[g].A + 1.0);
+            } else {
+              R[(g + G) * M2 + hh] = dcn * Bn[g * G + hh] * cfn[g].A;
+              R[(g + G) * M2 + hh + G] = dcp * Bp[g * G + hh] * cfp[g].A;
+            }
+          }
+          // the reference pairs An with Ln1 and Ap with Lp1 here (:693-694)
+          s[g + G] = dcp * (a2p + a4p + a.f0[(size_t)g * NC + cp] - cfn[g].A * Ln1) -
+                     dcn * (a2n + a4n + a.f0[(size_t)g * NC + cn] + cfp[g].A * Lp1);
+        }
+        if (!lu_solve<M2>(R, s)) atomicMin(&a.sc->lu_key, lu_key(ord, li, lj, lk));
+        for (int g = 0; g < G; ++g) a1[g] = s[g];
+        for (int g = 0; g < G; ++g) {
+          double Bf = 0.0;
+          for (int hh = 0; hh < G; ++hh) Bf = Bf + Bn[g * G + hh] * a1[hh];
+          a3[g] = cfn[g].A * (Bf + a.L1[ub + (size_t)g * NC + cn]);
+        }
+        for (int g = 0; g < G; ++g) {
+          const double a2n = a.a2[ub + (size_t)g * NC + cn], a4n = a.a4[ub + (size_t)g * NC + cn];
+          const double jp = -2.0 * a.xs.D[(size_t)g * NC + cn] / hn *
+                            (a1[g] + 3.0 * a2n + cfn[g].H * a3[g] + cfn[g].G * a4n);
+          const size_t on = ((size_t)sf * G + g) * NC + cn, op = ((size_t)(sf + 1) * G + g) * NC + cp;
+          const double ndpr = a.dn[on];
+          const double fn = a.f0[(size_t)g * NC + cn], fp = a.f0[(size_t)g * NC + cp];
+          const double v = (a.df[on] * (fn - fp) - jp) / (fn + fp);
+          a.dn[on] = v;
+          a.dn[op] = v;
+          const double nder = fabs(v - ndpr);
+          if (nder > best) { best = nder; bkey = (double)(ord * G + g); }
+        }
+      } else if (hasp) {
+        // first node of a line (get_coefs_first, :375-407)
+        const double hp = hdir(q, u, i, j, k);
+        nod_B<G>(a, cp, hp, Ke, Bp);
+        for (int g = 0; g < G; ++g)
+          cfp[g] = nod_coef(a.kern, a.xs.sigr[(size_t)g * NC + cp], a.xs.D[(size_t)g * NC + cp], hp);
+        const int bc = q.bc[2 * u + 1];
+        double Mx[G * G], b[G];
+        for (int g = 0; g < G; ++g) {
+          const double Lp1 = a.L1[ub + (size_t)g * NC + cp];
+          const double a2p = a.a2[ub + (size_t)g * NC + cp], a4p = a.a4[ub + (size_t)g * NC + cp];
+          const double Pp = 2.0 * a.xs.D[(size_t)g * NC + cp] / hp;
+          const double dc = a.xs.dc[((size_t)(sf + 1) * G + g) * NC + cp];
+          const double Ap = cfp[g].A, Fp = cfp[g].F, Gp = cfp[g].G, Hp = cfp[g].H;
+          const double f0p = a.f0[(size_t)g * NC + cp];
+          for (int hh = 0; hh < G; ++hh) {
+            const double Bv = Bp[g * G + hh];
+            if (bc == 2) Mx[g * G + hh] = (hh == g) ? Pp * (Bv * Fp + 1.0) : Pp * Bv * Fp;
+            else if (bc == 1)
+              Mx[g * G + hh] = (hh == g) ? -dc * (1.0 + Ap * Bv) - 2.0 * Pp * (Ap * Bv * Hp + 1.0)
+                                         : -dc * Ap * Bv - 2.0 * Pp * Ap * Bv * Hp;
+            else Mx[g * G + hh] = (hh == g) ? dc * (1.0 + Ap * Bv) : dc * Ap * Bv;
+          }
+          if (bc == 2) b[g] = Pp * (3.0 * a2p + Gp * a4p - Fp * Lp1);
+          else if (bc == 1) b[g] = 2.0 * Pp * (Ap * Hp * Lp1 - 3.0 * a2p - Gp * a4p) - dc * (a2p + a4p + f0p - Ap * Lp1);
+          else b[g] = dc * (a2p + a4p + f0p - Ap * Lp1);
+        }
+        if (!lu_solve<G>(Mx, b)) atomicMin(&a.sc->lu_key, lu_key(ord, li, lj, lk));
+        for (int g = 0; g < G; ++g) a1[g] = b[g];
+        for (int g = 0; g < G; ++g) {
+          double Bf = 0.0;
+          for (int hh = 0; hh < G; ++hh) Bf = Bf + Bp[g * G + hh] * a1[hh];
+          a3[g] = cfp[g].A * (Bf + a.L1[ub + (size_t)g * NC + cp]);
+        }
+        for (int g = 0; g < G; ++g) {
+          const double a2p = a.a2[ub + (size_t)g * NC + cp], a4p = a.a4[ub + (size_t)g * NC + cp];
+          const double jp = -2.0 * a.xs.D[(size_t)g * NC + cp] / hp *
+                            (a1[g] - 3.0 * a2p + cfp[g].H * a3[g] - cfp[g].G * a4p);
+          const size_t op = ((size_t)(sf + 1) * G + g) * NC + cp;
+          const double ndpr = a.dn[op];
+          const double v = -(jp / a.f0[(size_t)g * NC + cp] + a.df[op]);
+          a.dn[op] = v;
+          const double nder = fabs(v - ndpr);
+          if (nder > best) { best = nder; bkey = (double)(ord * G + g); }
+        }
+      } else {
+        // last node of a line (get_coefs_last, :411-441)
+        const int ni = u == 0 ? i - 1 : i, nj = u == 1 ? j - 1 : j, nk = u == 2 ? k - 1 : k;
+        const double hn = hdir(q, u, ni, nj, nk);
+        nod_B<G>(a, cn, hn, Ke, Bn);
+        for (int g = 0; g < G; ++g)
+          cfn[g] = nod_coef(a.kern, a.xs.sigr[(size_t)g * NC + cn], a.xs.D[(size_t)g * NC + cn], hn);
+        const int bc = q.bc[2 * u];
+        double Mx[G * G], b[G];
+        for (int g = 0; g < G; ++g) {
+          const double Ln1 = a.L1[ub + (size_t)g * NC + cn];
+          const double a2n = a.a2[ub + (size_t)g * NC + cn], a4n = a.a4[ub + (size_t)g * NC + cn];
+          const double Pn = 2.0 * a.xs.D[(size_t)g * NC + cn] / hn;
+          const double dc = a.xs.dc[((size_t)sf * G + g) * NC + cn];
+          const double An = cfn[g].A, Fn = cfn[g].F, Gn = cfn[g].G, Hn = cfn[g].H;
+          const double f0n = a.f0[(size_t)g * NC + cn];
+          for (int hh = 0; hh < G; ++hh) {
+            const double Bv = Bn[g * G + hh];
+            if (bc == 2) Mx[g * G + hh] = (hh == g) ? -Pn * (Bv * Fn + 1.0) : -Pn * Bv * Fn;
+            else if (bc == 1)
+              Mx[g * G + hh] = (hh == g) ? dc * (1.0 + An * Bv) + 2.0 * Pn * (An * Bv * Hn + 1.0)
+                                         : dc * An * Bv + 2.0 * Pn * An * Bv * Hn;
+            else Mx[g * G + hh] = (hh == g) ? dc * (1.0 + An * Bv) : dc * An * Bv;
+          }
+          if (bc == 2) b[g] = Pn * (3.0 * a2n + Gn * a4n + Fn * Ln1);
+          else if (bc == 1) b[g] = -2.0 * Pn * (An * Hn * Ln1 + 3.0 * a2n + Gn * a4n) - dc * (a2n + a4n + f0n + An * Ln1);
+          else b[g] = -dc * (a2n + a4n + f0n + An * Ln1);
+        }
+        if (!lu_solve<G>(Mx, b)) atomicMin(&a.sc->lu_key, lu_key(ord, li, lj, lk));
+        for (int g = 0; g < G; ++g) a1[g] = b[g];
+        for (int g = 0; g < G; ++g) {
+          double Bf = 0.0;
+          for (int hh = 0; hh < G; ++hh) Bf = Bf + Bn[g * G + hh] * a1[hh];
+          a3[g] = cfn[g].A * (Bf + a.L1[ub + (size_t)g * NC + cn]);
+        }
+        for (int g = 0; g < G; ++g) {
+          const double a2n = a.a2[ub + (size_t)g * NC + cn], a4n = a.a4[ub + (size_t)g * NC + cn];
+          const double jp = -2.0 * a.xs.D[(size_t)g * NC + cn] / hn *
+                            (a1[g] + 3.0 * a2n + cfn[g].H * a3[g] + cfn[g].G * a4n);
+          const size_t on = ((size_t)sf * G + g) * NC + cn;
+          const double ndpr = a.dn[on];
+          const double v = -(jp / a.f0[(size_t)g * NC + cn] - a.df[on]);
+          a.dn[on] = v;
+          const double nder = fabs(v - ndpr);
+          if (nder > best) { best = nder; bkey = (double)(ord * G + g); }
+        }
+      }
+      bloc = (double)(((li + 1) << 20) | ((lj + 1) << 10) | (lk + 1));
+    }
+  }
+  // block argmax (largest value, then smallest sweep key) -> partials
+  __shared__ double sv[NT], sk[NT], sl[NT];
+  sv[threadIdx.x] = best;
+  sk[threadIdx.x] = bkey;
+  sl[threadIdx.x] = bloc;
+  __syncthreads();
+  for (int w = NT / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      const int o = threadIdx.x + w;
+      if (sv[o] > sv[threadIdx.x] || (sv[o] == sv[threadIdx.x] && sk[o] < sk[threadIdx.x])) {
+        sv[threadIdx.x] = sv[o]; sk[threadIdx.x] = sk[o]; sl[threadIdx.x] = sl[o];
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const size_t slot = ((size_t)blockIdx.y * gridDim.x + blockIdx.x) * 3;
+    a.part[slot] = sv[0];
+    a.part[slot + 1] = sk[0];
+    a.part[slot + 2] = sl[0];
+  }
+}
+
+// final argmax over the pass-2 partials: one block, sweep order x, y, z
+__global__ void __launch_bounds__(NT) k_nod_argmax(Scal *sc, const double *part, int nslots) {
+  if (sc->done) return;
+  __shared__ double sv[NT], sk[NT], sl[NT];
+  double best = -1.0, bkey = 0.0, bloc = 0.0;
+  for (int s = threadIdx.x; s < nslots; s += NT) {
+    const double v = part[3 * s], kk = part[3 * s + 1];
+    if (v > best || (v == best && kk < bkey)) { best = v; bkey = kk; bloc = part[3 * s + 2]; }
+  }
+  sv[threadIdx.x] = best; sk[threadIdx.x] = bkey; sl[threadIdx.x] = bloc;
+  __syncthreads();
+  for (int w = NT / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      const int o = threadIdx.x + w;
+      if (sv[o] > sv[threadIdx.x] || (sv[o] == sv[threadIdx.x] && sk[o] < sk[threadIdx.x])) {
+        sv[threadIdx.x] = sv[o]; sk[threadIdx.x] = sk[o]; sl[threadIdx.x] = sl[o];
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double v = sv[0] < 0.0 ? 0.0 : sv[0];
+    sc->ndmax = v;
+    const int loc = (int)sl[0];
+    if (sv[0] > 0.0) {
+      sc->im = (loc >> 20) & 1023;
+      sc->jm = (loc >> 10) & 1023;
+      sc->km = loc & 1023;
+    }
+  }
+}
+
+}  // namespace adg

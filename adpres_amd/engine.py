@@ -1,0 +1,238 @@
+"""Python host mirror of the reference's static hot path over the C-ABI.
+
+:class:`Engine` plays the role of the reference's ``cmfd`` module for one
+core: ``outer()`` is ``cmfd::outer`` / ``cmfd::outer_ad``
+(src/mod_cmfd.f90:415,602), ``powdis()`` is ``cmfd::PowDis`` (:1290), and
+``nodal_update``, ``bicg``, ``matvec`` are the kernel-level routines.  The
+solver state (flux, fission source, k, D-hat) lives on the device between
+calls, as the reference keeps it in SAVEd module variables, so repeated
+``outer()`` calls warm-start exactly like the reference's BCSEARCH loop.
+
+Errors are raised as :class:`ADGError` carrying the reference's message
+(where the reference would print and STOP).  There is no CPU fallback: if
+``libadpres_gpu.so`` is missing or no GPU is present, construction fails.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional
+
+import numpy as np
+
+from .problem import Problem
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libadpres_gpu.so")
+
+ADG_OK, ADG_E_MAX_OUTER, ADG_E_NODAL_UNSTABLE, ADG_E_LU_PIVOT, ADG_E_POWER_NONPOS = 0, 1, 2, 3, 4
+ADG_E_ARG, ADG_E_HIP, ADG_E_NOT_SET = 5, 6, 7
+FP64, FP32_REFINE = 0, 1
+PRECISIONS = {"fp64": FP64, "fp32_refine": FP32_REFINE}
+KERNS = {"FDM": 0, "PNM": 1, "SANM": 2}
+
+EXPORTS = ("adg_create", "adg_destroy", "adg_set_xs", "adg_set_state", "adg_get_state", "adg_outer",
+           "adg_powdis", "adg_nodal_update", "adg_bicg", "adg_matvec", "adg_last_timing",
+           "adg_set_profiling", "adg_read_profile",
+           "adg_status_string", "adg_last_error")
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+class AdgGeom(C.Structure):
+    _fields_ = [("nnod", C.c_int), ("ng", C.c_int), ("nmat", C.c_int),
+                ("nxx", C.c_int), ("nyy", C.c_int), ("nzz", C.c_int),
+                ("ix", _ip), ("iy", _ip), ("iz", _ip),
+                ("xdel", _dp), ("ydel", _dp), ("zdel", _dp),
+                ("bc", C.c_int * 6), ("precision", C.c_int)]
+
+
+class AdgIter(C.Structure):
+    _fields_ = [("nout", C.c_int), ("nin", C.c_int), ("nac", C.c_int), ("nupd", C.c_int),
+                ("kern", C.c_int), ("serc", C.c_double), ("ferc", C.c_double)]
+
+
+class AdgTrace(C.Structure):
+    _fields_ = [("cap", C.c_int), ("n", C.c_int), ("k", _dp), ("ser", _dp), ("fer", _dp),
+                ("ndmax", _dp), ("flags", _ip), ("loc", _ip)]
+
+
+class ADGError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"[{status}] {message}")
+        self.status = status
+
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH):
+    """Load the engine library; raises if it is missing (no fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise FileNotFoundError(f"{path} not built: run __graft_entry__.build() or python -m adpres_amd._build")
+    L = C.CDLL(path)
+    L.adg_create.argtypes = [C.POINTER(AdgGeom), C.POINTER(C.c_void_p)]
+    L.adg_create.restype = C.c_int
+    L.adg_destroy.argtypes = [C.c_void_p]
+    L.adg_destroy.restype = None
+    L.adg_set_xs.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp, _ip, _dp, _dp]
+    L.adg_set_state.argtypes = [C.c_void_p, _dp, _dp, C.c_double, _dp, C.c_int]
+    L.adg_get_state.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, _dp]
+    L.adg_outer.argtypes = [C.c_void_p, C.c_int, C.POINTER(AdgIter), C.POINTER(AdgTrace)]
+    L.adg_powdis.argtypes = [C.c_void_p, _dp]
+    L.adg_nodal_update.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _ip]
+    L.adg_bicg.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _dp]
+    L.adg_matvec.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
+    L.adg_last_timing.argtypes = [C.c_void_p, _dp]
+    L.adg_set_profiling.argtypes = [C.c_void_p, C.c_int]
+    L.adg_read_profile.argtypes = [C.c_void_p, _dp]
+    L.adg_status_string.argtypes = [C.c_int]
+    L.adg_status_string.restype = C.c_char_p
+    L.adg_last_error.argtypes = [C.c_void_p]
+    L.adg_last_error.restype = C.c_char_p
+    _lib = L
+    return L
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _i(a):
+    return a.ctypes.data_as(_ip)
+
+
+class Engine:
+    """One core on one GPU behind the C-ABI."""
+
+    def __init__(self, p: Problem, precision: str = "fp64"):
+        self.lib = load_library()
+        self.p = p
+        self.precision = precision
+        self._keep = {}
+        ix = np.ascontiguousarray(p.ix, np.int32); iy = np.ascontiguousarray(p.iy, np.int32)
+        iz = np.ascontiguousarray(p.iz, np.int32)
+        xd = np.ascontiguousarray(p.xdel, np.float64); yd = np.ascontiguousarray(p.ydel, np.float64)
+        zd = np.ascontiguousarray(p.zdel, np.float64)
+        self._keep.update(ix=ix, iy=iy, iz=iz, xd=xd, yd=yd, zd=zd)
+        g = AdgGeom(p.nnod, p.ng, p.nmat, p.nxx, p.nyy, p.nzz, _i(ix), _i(iy), _i(iz), _d(xd), _d(yd), _d(zd),
+                    (C.c_int * 6)(*p.bc), PRECISIONS[precision])
+        h = C.c_void_p()
+        rc = self.lib.adg_create(C.byref(g), C.byref(h))
+        if rc != ADG_OK:
+            raise ADGError(rc, f"adg_create failed: {self.lib.adg_status_string(rc).decode()}")
+        self.h = h
+        self.set_xs(p)
+        self.niter = 0
+        self.trace = None
+
+    # -- lifetime --------------------------------------------------------
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.adg_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc: int, what: str):
+        if rc != ADG_OK:
+            msg = self.lib.adg_last_error(self.h).decode()
+            raise ADGError(rc, f"{what}: {self.lib.adg_status_string(rc).decode()} ({msg})")
+
+    # -- data ------------------------------------------------------------
+    def set_xs(self, p: Problem):
+        arrs = {k: np.ascontiguousarray(getattr(p, k), np.float64)
+                for k in ("D", "sigr", "sigs", "nuf", "sigf", "chi", "dc", "exsrc")}
+        mat = np.ascontiguousarray(p.mat, np.int32)
+        rc = self.lib.adg_set_xs(self.h, _d(arrs["D"]), _d(arrs["sigr"]), _d(arrs["sigs"]), _d(arrs["nuf"]),
+                                 _d(arrs["sigf"]), _d(arrs["chi"]), _i(mat), _d(arrs["dc"]), _d(arrs["exsrc"]))
+        self._check(rc, "adg_set_xs")
+
+    def reset(self):
+        """Start the next outer() from f0 = 1, k = 1, D-hat = 0 (the reference's first call)."""
+        self._check(self.lib.adg_set_state(self.h, None, None, 1.0, None, 1), "adg_set_state")
+
+    def set_state(self, f0, fs0, Ke, dn):
+        f0 = np.ascontiguousarray(f0, np.float64); fs0 = np.ascontiguousarray(fs0, np.float64)
+        dn = np.ascontiguousarray(dn, np.float64)
+        self._check(self.lib.adg_set_state(self.h, _d(f0), _d(fs0), float(Ke), _d(dn), 0), "adg_set_state")
+
+    def state(self):
+        N, G = self.p.nnod, self.p.ng
+        f0 = np.zeros((G, N)); fs0 = np.zeros(N); dn = np.zeros((6, G, N)); df = np.zeros((6, G, N))
+        Ke = C.c_double()
+        self._check(self.lib.adg_get_state(self.h, _d(f0), _d(fs0), C.byref(Ke), _d(dn), _d(df)), "adg_get_state")
+        return dict(f0=f0, fs0=fs0, Ke=Ke.value, dn=dn, df=df)
+
+    @property
+    def Ke(self) -> float:
+        Ke = C.c_double()
+        self._check(self.lib.adg_get_state(self.h, None, None, C.byref(Ke), None, None), "adg_get_state")
+        return Ke.value
+
+    # -- hot path --------------------------------------------------------
+    def outer(self, adjoint: Optional[bool] = None, raise_on_error: bool = True, **ov):
+        p = self.p
+        it = AdgIter(ov.get("nout", p.nout), ov.get("nin", p.nin), ov.get("nac", p.nac), ov.get("nupd", p.nupd),
+                     KERNS[ov.get("kern", p.kern)], ov.get("serc", p.serc), ov.get("ferc", p.ferc))
+        if adjoint is None:
+            adjoint = p.mode == "ADJOINT"
+        cap = it.nout
+        t = {k: np.zeros(cap) for k in ("k", "ser", "fer", "ndmax")}
+        t["flags"] = np.zeros(cap, np.int32)
+        t["loc"] = np.zeros(3 * cap, np.int32)
+        tr = AdgTrace(cap, 0, _d(t["k"]), _d(t["ser"]), _d(t["fer"]), _d(t["ndmax"]), _i(t["flags"]), _i(t["loc"]))
+        rc = self.lib.adg_outer(self.h, 1 if adjoint else 0, C.byref(it), C.byref(tr))
+        n = tr.n
+        self.niter = n
+        self.trace = {k: (v[:n] if k != "loc" else v[:3 * n].reshape(-1, 3)) for k, v in t.items()}
+        if raise_on_error:
+            self._check(rc, "adg_outer")
+        return rc
+
+    def powdis(self) -> np.ndarray:
+        out = np.zeros(self.p.nnod)
+        self._check(self.lib.adg_powdis(self.h, _d(out)), "adg_powdis")
+        return out
+
+    def nodal_update(self, kern: Optional[str] = None, cmode: int = 1):
+        nd = C.c_double()
+        loc = np.zeros(3, np.int32)
+        rc = self.lib.adg_nodal_update(self.h, KERNS[kern or self.p.kern], cmode, C.byref(nd), _i(loc))
+        self._check(rc, "adg_nodal_update")
+        return nd.value, tuple(int(v) for v in loc)
+
+    def bicg(self, g: int, nin: int, b, x):
+        b = np.ascontiguousarray(b, np.float64)
+        x = np.array(x, np.float64, copy=True)
+        self._check(self.lib.adg_bicg(self.h, g, nin, _d(b), _d(x)), "adg_bicg")
+        return x
+
+    def matvec(self, g: int, x):
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.zeros_like(x)
+        self._check(self.lib.adg_matvec(self.h, g, _d(x), _d(y)), "adg_matvec")
+        return y
+
+    def last_timing_ms(self):
+        t = np.zeros(4)
+        self._check(self.lib.adg_last_timing(self.h, _d(t)), "adg_last_timing")
+        return t
+
+    def set_profiling(self, on: bool):
+        self._check(self.lib.adg_set_profiling(self.h, 1 if on else 0), "adg_set_profiling")
+
+    def read_profile(self):
+        t = np.zeros(6)
+        self._check(self.lib.adg_read_profile(self.h, _d(t)), "adg_read_profile")
+        return dict(k1_ms=t[0], k1_n=int(t[1]), gsolve_ms=t[2], gsolve_n=int(t[3]), nodal_ms=t[4],
+                    nodal_n=int(t[5]))

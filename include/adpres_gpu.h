@@ -1,0 +1,135 @@
+/*
+ * adpres_gpu.h -- C-ABI of the MI355X (gfx950) static k-eigenvalue engine
+ * that replaces ADPRES's CMFD + SANM/PNM hot path.
+ *
+ * The reference has no plugin or FFI layer: its hot path is the Fortran
+ * module procedures cmfd::outer / cmfd::outer_ad (src/mod_cmfd.f90:415,602)
+ * working on the global arrays of module sdata (src/mod_data.f90).  This
+ * header is the seam a Fortran host crosses through iso_c_binding instead
+ * (the interface module is shown in INTEGRATION.md).  Every entry point
+ * takes plain pointers to host arrays laid out exactly as the reference
+ * holds them: Fortran column-major, float64, 1-based integer indices.
+ *
+ *   f0(N,G), fs0(N), D/sigr/nuf/sigf/exsrc(N,G), sigs(N,G,G) = sigs(n,from,to),
+ *   chi(nmat,G), dc/dn/df(N,G,6) with faces 1..6 = x+, x-, y+, y-, z+, z-.
+ *
+ * A context owns all device memory; the host keeps ownership of its arrays
+ * (they are copied in by the set calls and copied out by the get calls).
+ * A context is used from one host thread at a time; independent contexts
+ * may be used concurrently.  All functions return an ADG status code.
+ */
+#ifndef ADPRES_GPU_H
+#define ADPRES_GPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes.  The reference prints a message and STOPs in these cases;
+ * the engine returns the code and the host shim prints the message. */
+enum adg_status {
+  ADG_OK = 0,
+  ADG_E_MAX_OUTER = 1,      /* src/mod_cmfd.f90:498-505  NOUT reached              */
+  ADG_E_NODAL_UNSTABLE = 2, /* src/mod_nodal.f90:131-142 max |d(dn)| > 1e3          */
+  ADG_E_LU_PIVOT = 3,       /* src/mod_nodal.f90:856-862 |a_ii| < 1e-4 (node in loc) */
+  ADG_E_POWER_NONPOS = 4,   /* src/mod_cmfd.f90:1322-1326 total power <= 0          */
+  ADG_E_ARG = 5,            /* invalid argument / unsupported geometry              */
+  ADG_E_HIP = 6,            /* HIP runtime error (adg_last_error has the text)       */
+  ADG_E_NOT_SET = 7         /* cross sections not set yet                            */
+};
+
+enum adg_kern { ADG_KERN_FDM = 0, ADG_KERN_PNM = 1, ADG_KERN_SANM = 2 };   /* %KERN */
+enum adg_mode { ADG_FORWARD = 0, ADG_ADJOINT = 1 };                       /* %MODE */
+enum adg_precision {
+  ADG_FP64 = 0,          /* plain FP64 drop-in                                        */
+  ADG_FP32_REFINE = 1    /* FP32 BiCGSTAB on an FP64 defect (residual) each group solve */
+};
+
+/* Geometry, as built by the reference's mod_io (misc, src/mod_io.f90:1293). */
+typedef struct adg_geom {
+  int nnod, ng, nmat;
+  int nxx, nyy, nzz;
+  const int *ix, *iy, *iz;        /* (N) 1-based node coordinates, numbering of misc() */
+  const double *xdel, *ydel, *zdel;
+  int bc[6];                      /* xeast, xwest, ynorth, ysouth, zbott, ztop: 0 zero flux,
+                                     1 zero incoming current, 2 reflective */
+  int precision;                  /* enum adg_precision */
+} adg_geom;
+
+/* Iteration control: the %ITER and %KERN cards (src/mod_io.f90:1522,1593). */
+typedef struct adg_iter {
+  int nout, nin, nac, nupd;
+  int kern;                       /* enum adg_kern */
+  double serc, ferc;
+} adg_iter;
+
+/* Per-outer trace, i.e. what outer prints (src/mod_cmfd.f90:331,367-374,491). */
+typedef struct adg_trace {
+  int cap;                        /* capacity of every array (>= nout)            */
+  int n;                          /* outers performed                              */
+  double *k, *ser, *fer, *ndmax;  /* (cap)                                         */
+  int *flags;                     /* (cap) bit0 extrapolated, bit1 nodal update    */
+  int *loc;                       /* (3*cap) i,j,k of the max D-hat change         */
+} adg_trace;
+
+typedef struct adg_ctx adg_ctx;
+
+/* Replaces the allocation/geometry part of matrix_setup's first call and
+ * set_ind (src/mod_cmfd.f90:217-241,141-213). */
+int adg_create(const adg_geom *geom, adg_ctx **ctx);
+void adg_destroy(adg_ctx *ctx);
+
+/* Node-wise cross sections after xsec::XS_updt (src/mod_xsec.f90:11-46):
+ * D and sigr as made by Dsigr_updt (:199-226); chi per material with the
+ * node material map mat(N); exsrc may be NULL (zero). */
+int adg_set_xs(adg_ctx *ctx, const double *D, const double *sigr, const double *sigs,
+               const double *nuf, const double *sigf, const double *chi, const int *mat,
+               const double *dc, const double *exsrc);
+
+/* Flux, fission source, k and D-hat, i.e. the SAVEd state the reference
+ * keeps between outer calls (src/mod_cmfd.f90:439-454, :25-33).  With
+ * first != 0 the next outer call starts from f0 = 1, k = 1 as the
+ * reference's first call does (the other arguments may be NULL). */
+int adg_set_state(adg_ctx *ctx, const double *f0, const double *fs0, double Ke,
+                  const double *dn, int first);
+int adg_get_state(adg_ctx *ctx, double *f0, double *fs0, double *Ke, double *dn, double *df);
+
+/* cmfd::outer (src/mod_cmfd.f90:415-509) for mode ADG_FORWARD and
+ * cmfd::outer_ad (:602-699) for ADG_ADJOINT.  trace may be NULL. */
+int adg_outer(adg_ctx *ctx, int mode, const adg_iter *iter, adg_trace *trace);
+
+/* cmfd::PowDis (src/mod_cmfd.f90:1290-1333): node powers normalised to 1. */
+int adg_powdis(adg_ctx *ctx, double *p);
+
+/* Kernel-level entry points, each replacing one reference routine. */
+/* nodal::nodal_update / nodal_update_pnm (src/mod_nodal.f90:18,151) followed
+ * by the matrix rebuild of cmfd::nodal_upd (src/mod_cmfd.f90:339-383);
+ * cmode 1 forward, 0 adjoint.  Uses the context's f0, Ke and dn. */
+int adg_nodal_update(adg_ctx *ctx, int kern, int cmode, double *ndmax, int *loc3);
+/* cmfd::bicg (src/mod_cmfd.f90:1203-1243) on group g (0-based): exactly nin
+ * iterations from the initial guess x (in/out, length N). */
+int adg_bicg(adg_ctx *ctx, int g, int nin, const double *b, double *x);
+/* cmfd::sp_matvec (src/mod_cmfd.f90:1247-1268) on group g (0-based). */
+int adg_matvec(adg_ctx *ctx, int g, const double *x, double *y);
+
+/* Timers, the counterpart of the reference's CPU-time buckets fdm_time and
+ * nod_time (src/mod_data.f90:204-217, accumulated at src/mod_cmfd.f90:442-489
+ * and :356-378), measured with HIP events on the context's stream.
+ * adg_last_timing: device time (ms) of the last adg_outer call,
+ *   [0] total, [1] group solves (TSrc + BiCGSTAB), [2] nodal updates, [3] rest.
+ * With profiling on (adg_set_profiling(ctx, 1)), adg_outer also brackets the
+ * phases and one steady-state BiCGSTAB K1 launch per group solve with events;
+ * adg_read_profile returns averages over the last adg_outer call:
+ *   [0] K1 kernel ms, [1] K1 samples, [2] BiCGSTAB group solve ms (setup + nin
+ *   iterations), [3] samples, [4] nodal update ms, [5] samples. */
+int adg_last_timing(adg_ctx *ctx, double *ms4);
+int adg_set_profiling(adg_ctx *ctx, int on);
+int adg_read_profile(adg_ctx *ctx, double *out6);
+
+const char *adg_status_string(int status);
+const char *adg_last_error(adg_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

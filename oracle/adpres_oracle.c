@@ -1047,3 +1047,18 @@ int orc_powdis(const orc_problem *pb, const double *f0, double *p) {
   for (int n = 0; n < N; ++n) p[n] = p[n] / tpow;
   return ORC_OK;
 }
+
+void orc_sanm_coef(double sigr, double D, double h, double *out6) {
+  double alp = 0.5 * sqrt(sigr / D) * h;
+  double alp2 = alp * alp;
+  double sh = sinh(alp), ch = cosh(alp);
+  double m0c = sh / alp;
+  double m1s = 3.0 * (ch / alp - sh / alp2);
+  double m2c = 5.0 * (sh / alp - 3.0 * ch / alp2 + 3.0 * sh / (alp * alp * alp));
+  out6[0] = (sh - m1s) / (alp2 * m1s);
+  out6[1] = (ch - m0c - m2c) / (alp2 * m2c);
+  out6[2] = (m0c / m2c - 3.0 / alp2);
+  out6[3] = (alp * ch - m1s) / (alp2 * m1s);
+  out6[4] = (alp * sh - 3.0 * m2c) / (ch - m0c - m2c);
+  out6[5] = (alp * ch - m1s) / (sh - m1s);
+}

@@ -88,6 +88,10 @@ int orc_bicg(const orc_problem *pb, const orc_state *st, int g, int nin,
 int orc_matvec(const orc_problem *pb, const orc_state *st, int g,
                const double *x, double *y);
 
+/* SANM coefficients A,B,E,F,G,H of one node and group
+ * (get_ABEFGH, src/mod_nodal.f90:1409-1451). */
+void orc_sanm_coef(double sigr, double D, double h, double *out6);
+
 /* PowDis (src/mod_cmfd.f90:1290). */
 int orc_powdis(const orc_problem *pb, const double *f0, double *p);
 

@@ -68,6 +68,8 @@ def lib():
         L.orc_matvec.argtypes = [C.POINTER(OrcProblem), C.POINTER(OrcState), C.c_int, _dp, _dp]
         L.orc_powdis.argtypes = [C.POINTER(OrcProblem), _dp, _dp]
         L.orc_coup_coef.argtypes = [C.POINTER(OrcProblem), C.POINTER(OrcState)]
+        L.orc_sanm_coef.argtypes = [C.c_double, C.c_double, C.c_double, _dp]
+        L.orc_sanm_coef.restype = None
         _lib = L
     return _lib
 
@@ -78,6 +80,13 @@ def _d(a):
 
 def _i(a):
     return a.ctypes.data_as(_ip)
+
+
+def sanm_coef(sigr, D, h):
+    """(A, B, E, F, G, H) of get_ABEFGH for one node and group."""
+    out = np.zeros(6)
+    lib().orc_sanm_coef(float(sigr), float(D), float(h), _d(out))
+    return out
 
 
 class Oracle:

@@ -1,0 +1,135 @@
+"""Generate the golden fixtures of tests/golden/ from the reference build.
+
+Runs oracle/_ref/ref_dump (the unchanged ADPRES Fortran sources compiled by
+oracle/Makefile, plus the repo's dump driver) on the reference's sample decks
+and on the seeded synthetic cores, and stores inputs and full-precision
+outputs as compressed .npz files.  Needs /root/reference and the built
+oracle/_ref; the fixtures it writes are what the tests use everywhere else.
+
+    python tests/golden/make_golden.py [name ...]
+"""
+
+from __future__ import annotations
+
+import hashlib
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from adpres_amd.configs import generate  # noqa: E402
+from adpres_amd.problem import (parse_trace, problem_to_dict, read_ref_dump, write_deck,  # noqa: E402
+                                write_state_file, write_xs_override)
+
+REF = "/root/reference/smpl/static"
+DUMP = os.path.join(ROOT, "oracle", "_ref", "ref_dump")
+
+DECKS = {"iaea3d": "IAEA3Ds", "iaea2d": "IAEA2D", "pnm": "PNM", "biblis": "BIBLIS", "koeberg": "KOEBERG",
+         "dvp": "DVP", "adjoint": "adjoint", "fdm": "FDM"}
+
+
+def run(workdir, deck, action, xs="-", state=None):
+    args = [DUMP, deck, action, "out.bin", xs] + ([state] if state else [])
+    r = subprocess.run(["bash", "-c", "ulimit -s unlimited; exec " + " ".join(args)], cwd=workdir,
+                       capture_output=True, text=True)
+    if r.returncode != 0 or not os.path.exists(os.path.join(workdir, "out.bin")):
+        raise RuntimeError(f"ref_dump failed ({r.returncode}):\n{r.stdout[-2000:]}\n{r.stderr[-2000:]}")
+    return read_ref_dump(os.path.join(workdir, "out.bin"), action) + (r.stdout,)
+
+
+def sha(p):
+    h = hashlib.sha256()
+    for k in ("ix", "iy", "iz", "mat", "xdel", "ydel", "zdel", "sigtr", "siga", "nuf", "sigf", "sigs", "chi", "dc"):
+        h.update(np.ascontiguousarray(getattr(p, k)).tobytes())
+    return h.hexdigest()
+
+
+def trace_arrays(stdout):
+    t = np.asarray(parse_trace(stdout), dtype=np.float64).reshape(-1, 4)
+    return dict(trace_p=t[:, 0].astype(np.int32), trace_k=t[:, 1], trace_ser=t[:, 2], trace_fer=t[:, 3])
+
+
+def save(name, **arrs):
+    path = os.path.join(HERE, name + ".npz")
+    np.savez_compressed(path, **arrs)
+    print(f"{name}: {os.path.getsize(path) / 1024:.1f} KiB")
+
+
+def deck_solve(name, deck):
+    with tempfile.TemporaryDirectory() as d:
+        shutil.copy(os.path.join(REF, deck), os.path.join(d, "deck"))
+        p, res, out = run(d, "deck", "solve")
+    p.name = name
+    p.mode = "ADJOINT" if name == "adjoint" else "FORWARD"
+    save(name + "_solve", **problem_to_dict(p), Ke=res["Ke"], f0=res["f0"], fs0=res["fs0"], pow=res["pow"],
+         dn=res["dn"], df=res["df"], **trace_arrays(out))
+    return p, res
+
+
+def probes(name, deck, p, res, seed):
+    """Nodal-update and BiCGSTAB probes around the converged state."""
+    rng = np.random.default_rng(seed)
+    N, G = p.nnod, p.ng
+    Ke = res["Ke"] * 1.0007
+    f0 = res["f0"] * rng.uniform(0.97, 1.03, size=(G, N))
+    dn = res["dn"] * rng.uniform(0.9, 1.1, size=(6, G, N)) + rng.uniform(-1e-3, 1e-3, size=(6, G, N))
+    b = rng.uniform(0.5, 1.5, size=(G, N))
+    nin = 5
+    with tempfile.TemporaryDirectory() as d:
+        shutil.copy(os.path.join(REF, deck) if deck else deck_path_of(p), os.path.join(d, "deck"))
+        write_state_file(os.path.join(d, "st.bin"), nin, Ke, f0, dn, b)
+        _, nres, _ = run(d, "deck", "nodal", "-", "st.bin")
+        os.remove(os.path.join(d, "out.bin"))
+        _, bres, _ = run(d, "deck", "bicg", "-", "st.bin")
+    save(name + "_probe", **problem_to_dict(p), Ke=Ke, f0=f0, dn_in=dn, b=b, nin=nin, dn_out=nres["dn"],
+         df=nres["df"], ndmax=nres["ndmax"], loc=np.asarray(nres["loc"], np.int32), x_out=bres["x"])
+
+
+def deck_path_of(p):
+    raise RuntimeError("probe without a deck")
+
+
+def synthetic(name, seed, full=True):
+    gc = generate(name, seed)
+    p = gc.problem
+    with tempfile.TemporaryDirectory() as d:
+        it = (p.nout, p.nin, p.serc, p.ferc, p.nac, p.nupd)
+        write_deck(os.path.join(d, "deck"), kern=p.kern, iter_card=it, **gc.deck)
+        xs = "-"
+        if gc.perturbed:
+            write_xs_override(os.path.join(d, "xs.bin"), p)
+            xs = "xs.bin"
+        q, res, out = run(d, "deck", "solve", xs)
+    # the reference must hold exactly the generated problem
+    for k in ("ix", "iy", "iz", "mat", "xdel", "ydel", "zdel", "sigtr", "siga", "nuf", "sigf", "sigs", "chi",
+              "D", "sigr"):
+        assert np.array_equal(getattr(p, k), getattr(q, k)), k
+    extra = problem_to_dict(p) if full else {}
+    save(f"{name}_s{seed}_solve", **extra, seed=seed, problem_sha=sha(p), Ke=res["Ke"], f0=res["f0"],
+         pow=res["pow"], dn=res["dn"] if full else res["dn"][:, :, ::97], **trace_arrays(out))
+
+
+def main(names):
+    if not os.path.exists(DUMP):
+        raise SystemExit("build the reference first: make -C oracle ref")
+    for name, deck in DECKS.items():
+        if names and name not in names:
+            continue
+        p, res = deck_solve(name, deck)
+        if name in ("iaea3d", "koeberg", "dvp", "pnm"):
+            probes(name, deck, p, res, seed=len(name))
+    if not names or "core2d_small" in names:
+        synthetic("core2d_small", 1)
+    if not names or "core3d_2g" in names:
+        synthetic("core3d_2g", 1, full=False)
+
+
+if __name__ == "__main__":
+    main(sys.argv[1:])
