@@ -1,0 +1,270 @@
+! Drop-in replacement of ADPRES's hot-path modules `cmfd` and `nodal`
+! (src/mod_cmfd.f90, src/mod_nodal.f90) for the static path.
+!
+! It exports the names the unchanged host modules use -- outer, outer_ad,
+! PowDis (mod_control, mod_th), outer_fs, outer_th, outer_tr, Integrate
+! (mod_control, mod_th, mod_trans) and nodal::Lxyz (mod_trans) -- and runs
+! the static outer iteration on the GPU through the C-ABI.  The solver state
+! stays in the engine's context between calls (the reference keeps it in
+! SAVEd module variables), and the results are copied back into the host's
+! sdata arrays f0, fs0 and Ke after every call, so the host's outputs
+! (AsmPow, AxiPow, AsmFlux) and callers (BCSEARCH) see what they expect.
+! Printed lines follow the reference's formats (src/mod_cmfd.f90:331,
+! 367-374, 491-494, 500-504, 589-591; src/mod_nodal.f90:131-142).
+!
+! The fixed-source, T-H and transient outer variants are not served by this
+! build; they stop with a message.
+
+module nodal
+  use sdata, only: dp
+  implicit none
+contains
+  subroutine Lxyz(n, g, L1, L2, L3)
+    integer, intent(in)   :: n, g
+    real(dp), intent(out) :: L1, L2, L3
+    L1 = 0._dp; L2 = 0._dp; L3 = 0._dp
+    write(*,*) '  Lxyz (transient transverse leakage) is not served by the GPU build'
+    stop
+  end subroutine
+end module nodal
+
+module cmfd
+
+  use iso_c_binding
+  use sdata, only: dp
+  use adpres_gpu
+  implicit none
+  save
+  private
+  public :: outer, outer_ad, outer_fs, outer_th, outer_tr, PowDis, Integrate
+
+  type(c_ptr) :: ctx = c_null_ptr
+  logical     :: started = .false.
+  integer(c_int), allocatable, target :: gix(:), giy(:), giz(:)
+  real(c_double), allocatable, target :: gxd(:), gyd(:), gzd(:)
+  real(c_double), allocatable, target :: tk(:), tser(:), tfer(:), tnd(:)
+  integer(c_int), allocatable, target :: tfl(:), tloc(:)
+
+contains
+
+  subroutine ensure_context()
+    use sdata, only: nnod, ng, nmat, nxx, nyy, nzz, ix, iy, iz, xdel, ydel, zdel, &
+                     xeast, xwest, ynorth, ysouth, zbott, ztop
+    type(adg_geom_t) :: geo
+    integer(c_int) :: rc
+    character(len=16) :: pstr
+    if (c_associated(ctx)) return
+    allocate(gix(nnod), giy(nnod), giz(nnod), gxd(nxx), gyd(nyy), gzd(nzz))
+    gix = ix; giy = iy; giz = iz
+    gxd = xdel; gyd = ydel; gzd = zdel
+    geo%nnod = nnod; geo%ng = ng; geo%nmat = nmat
+    geo%nxx = nxx; geo%nyy = nyy; geo%nzz = nzz
+    geo%ix = c_loc(gix); geo%iy = c_loc(giy); geo%iz = c_loc(giz)
+    geo%xdel = c_loc(gxd); geo%ydel = c_loc(gyd); geo%zdel = c_loc(gzd)
+    geo%bc = [xeast, xwest, ynorth, ysouth, zbott, ztop]
+    geo%precision = ADG_FP64
+    call get_environment_variable('ADPRES_GPU_PRECISION', pstr)
+    if (trim(pstr) == 'FP32') geo%precision = ADG_FP32_REFINE
+    rc = adg_create(geo, ctx)
+    if (rc /= ADG_OK) then
+      write(*,*) '  GPU ENGINE INITIALISATION FAILED, STATUS ', rc
+      stop
+    end if
+    rc = adg_set_profiling(ctx, 1_c_int)
+  end subroutine ensure_context
+
+  subroutine check(rc, what)
+    integer(c_int), intent(in) :: rc
+    character(len=*), intent(in) :: what
+    if (rc == ADG_OK) return
+    write(*,*) '  ', what, ' FAILED: ', trim(c_string(adg_last_error(ctx)))
+    stop
+  end subroutine
+
+  ! one static solve on the device (shared by outer and outer_ad)
+  subroutine run_outer(popt, mode, label)
+    use sdata, only: nnod, ng, nout, nin, serc, ferc, nac, nupd, kern, f0, fs0, Ke, &
+                     D, sigr, sigs, nuf, sigf, chi, mat, dc, exsrc, ndmax, im, jm, km, &
+                     fdm_time, nod_time
+    use io, only: ounit, scr
+    integer, intent(in) :: popt
+    integer(c_int), intent(in) :: mode
+    character(len=*), intent(in) :: label
+    type(adg_iter_t)  :: it
+    type(adg_trace_t) :: tr
+    integer(c_int)    :: rc, rco, first
+    integer :: p, nlast
+    real(c_double), target :: kk
+    real(c_double), allocatable, target :: rf0(:,:), rfs0(:)
+    real(c_double) :: ms(4)
+    integer(c_int), allocatable :: imat(:)
+
+    call ensure_context()
+    allocate(imat(nnod))
+    imat = mat
+    call check(adg_set_xs(ctx, D, sigr, sigs, nuf, sigf, chi, imat, dc, exsrc), 'ADG_SET_XS')
+    first = 0
+    if (.not. started) then
+      if (.not. allocated(f0)) allocate(f0(nnod, ng))
+      if (.not. allocated(fs0)) allocate(fs0(nnod))
+      call check(adg_set_state(ctx, c_null_ptr, c_null_ptr, 1._c_double, c_null_ptr, 1_c_int), 'ADG_SET_STATE')
+      started = .true.
+    end if
+
+    it%nout = nout; it%nin = nin; it%nac = nac; it%nupd = nupd
+    it%serc = serc; it%ferc = ferc
+    it%kern = ADG_KERN_SANM
+    if (kern == ' FDM') it%kern = ADG_KERN_FDM
+    if (kern == ' PNM') it%kern = ADG_KERN_PNM
+
+    if (allocated(tk)) deallocate(tk, tser, tfer, tnd, tfl, tloc)
+    allocate(tk(nout), tser(nout), tfer(nout), tnd(nout), tfl(nout), tloc(3 * nout))
+    tr%cap = nout; tr%n = 0
+    tr%k = c_loc(tk); tr%ser = c_loc(tser); tr%fer = c_loc(tfer); tr%ndmax = c_loc(tnd)
+    tr%flags = c_loc(tfl); tr%loc = c_loc(tloc)
+
+    rco = adg_outer(ctx, mode, it, tr)
+
+    nlast = tr%n
+    if (rco == ADG_E_NODAL_UNSTABLE .or. rco == ADG_E_LU_PIVOT) nlast = tr%n - 1
+    do p = 1, nlast
+      call print_iteration(popt, p, tr%n)
+    end do
+
+    allocate(rf0(nnod, ng), rfs0(nnod))
+    rc = adg_get_state(ctx, c_loc(rf0), c_loc(rfs0), c_loc(kk), c_null_ptr, c_null_ptr)
+    f0 = rf0
+    fs0 = rfs0
+    Ke = kk
+    if (tr%n > 0) then
+      ndmax = tnd(tr%n); im = tloc(3*tr%n-2); jm = tloc(3*tr%n-1); km = tloc(3*tr%n)
+    end if
+    rc = adg_last_timing(ctx, ms)
+    fdm_time = fdm_time + (ms(1) - ms(3)) * 1.e-3_dp
+    nod_time = nod_time + ms(3) * 1.e-3_dp
+
+    select case (rco)
+    case (ADG_OK)
+    case (ADG_E_MAX_OUTER)
+      write(*,*)
+      write(*,*) '  MAXIMUM NUMBER OF OUTER ITERATION IS REACHED IN ', label, ' CALCULATION.'
+      write(*,*) '  CHECK PROBLEM SPECIFICATION OR CHANGE ITERATION CONTROL (%ITER).'
+      write(*,*) '  PERHAPS BY MAKING FISSION SOURCE INTERPOLATION MORE FREQUENT'
+      write(*,*) '  ADPRES IS STOPING...'
+      stop
+    case (ADG_E_NODAL_UNSTABLE)
+      if (popt > 0 .and. iand(tfl(tr%n), 1) /= 0) call say(' ', '    ...FISSION SOURCE EXTRAPOLATED...')
+      write(*,*)
+      write(*,'(" Error: Max. change in nodal coupling coefficient = ", F10.1)') tnd(tr%n)
+      write(*,*)"The two-node nonlinear iteration seems not stable."
+      write(*,*)"Try to reduce time step size for transient..."
+      write(*,*)"and/or change iteration control using %ITER card, ..."
+      write(*,*)"perhaps by making nodal update less frequent."
+      write(*,*)"Or increase number inner iteration per outer iteration."
+      stop
+    case (ADG_E_LU_PIVOT)
+      write(ounit,*) 'ERROR IN MATRIX DECOMP: DIAGONAL ELEMENTS CLOSE TO ZERO'
+      write(*,*) 'ERROR IN MATRIX DECOMP: DIAGONAL ELEMENTS CLOSE TO ZERO'
+      write(*,*) '  ', trim(c_string(adg_last_error(ctx)))
+      stop
+    case default
+      call check(rco, 'ADG_OUTER')
+    end select
+
+    if (popt > 0) then
+      write(ounit,*)
+      write(ounit,'(2X,"MULTIPLICATION EFFECTIVE (K-EFF) = ", F9.6)') Ke
+      if (scr) then
+        write(*,*)
+        write(*,'(2X,"MULTIPLICATION EFFECTIVE (K-EFF) = ", F9.6)') Ke
+      end if
+    end if
+  end subroutine run_outer
+
+  subroutine say(dummy, line)
+    use io, only: ounit, scr
+    character(len=*), intent(in) :: dummy, line
+    write(ounit,*) line
+    if (scr) write(*,*) line
+    if (.false.) write(*,*) dummy
+  end subroutine
+
+  subroutine print_iteration(popt, p, n)
+    use io, only: ounit, scr
+    integer, intent(in) :: popt, p, n
+    if (popt <= 0 .or. p > n) return
+    if (iand(tfl(p), 1) /= 0) call say(' ', '    ...FISSION SOURCE EXTRAPOLATED...')
+    if (iand(tfl(p), 2) /= 0) then
+      call say(' ', '    .....NODAL COUPLING UPDATED..... ')
+      write(ounit,1145) tnd(p), tloc(3*p-2), tloc(3*p-1), tloc(3*p)
+      if (scr) write(*,1145) tnd(p), tloc(3*p-2), tloc(3*p-1), tloc(3*p)
+    end if
+    write(ounit,'(I5,F13.6,2ES15.5)') p, tk(p), tser(p), tfer(p)
+    if (scr) write(*,'(I5,F13.6,2ES15.5)') p, tk(p), tser(p), tfer(p)
+1145 FORMAT ('MAX. CHANGE IN NODAL COUPLING COEF.= ', ES12.5, &
+             ' AT NODE I = ', I2, ', J = ', I2, ', K = ', I2)
+  end subroutine
+
+  subroutine outer(popt)
+    integer, optional, intent(in) :: popt
+    integer :: po
+    po = 0
+    if (present(popt)) po = popt
+    call run_outer(po, ADG_FORWARD, 'FORWARD')
+  end subroutine outer
+
+  subroutine outer_ad(popt)
+    integer, optional, intent(in) :: popt
+    integer :: po
+    po = 0
+    if (present(popt)) po = popt
+    call run_outer(po, ADG_ADJOINT, 'ADJOINT')
+  end subroutine outer_ad
+
+  subroutine outer_fs(popt)
+    integer, optional, intent(in) :: popt
+    if (present(popt)) continue
+    write(*,*) '  FIXED-SOURCE OUTER ITERATION IS NOT SERVED BY THE GPU BUILD'
+    stop
+  end subroutine outer_fs
+
+  subroutine outer_th(maxn)
+    integer, intent(in) :: maxn
+    if (maxn < 0) continue
+    write(*,*) '  T-H OUTER ITERATION IS NOT SERVED BY THE GPU BUILD'
+    stop
+  end subroutine outer_th
+
+  subroutine outer_tr(ht, maxi)
+    real(dp), intent(in) :: ht
+    logical, intent(out) :: maxi
+    maxi = .false.
+    if (ht < 0._dp) continue
+    write(*,*) '  TRANSIENT OUTER ITERATION IS NOT SERVED BY THE GPU BUILD'
+    stop
+  end subroutine outer_tr
+
+  ! node powers from the device (PowDis, src/mod_cmfd.f90:1290-1333)
+  subroutine PowDis(p)
+    use io, only: ounit
+    real(dp), dimension(:), intent(out) :: p
+    integer(c_int) :: rc
+    rc = adg_powdis(ctx, p)
+    if (rc == ADG_E_POWER_NONPOS) then
+      write(ounit, *) '   ERROR: TOTAL NODES POWER IS ZERO OR LESS'
+      write(ounit, *) '   STOP IN subroutine POWDIS'
+      stop
+    end if
+    call check(rc, 'ADG_POWDIS')
+  end subroutine PowDis
+
+  function Integrate(s) result(intg)
+    real(dp), dimension(:), intent(in) :: s
+    real(dp) :: intg
+    intg = 0._dp
+    if (size(s) >= 0) continue
+    write(*,*) '  Integrate (transient reactivity) is not served by the GPU build'
+    stop
+  end function Integrate
+
+end module cmfd

@@ -1,0 +1,196 @@
+#!/usr/bin/env python3
+"""Benchmark of the static k-eigenvalue hot path on one MI355X.
+
+A step is one complete static solve (the reference's cmfd::outer from its
+first-call state, src/mod_cmfd.f90:415-509, plus PowDis) of a seeded
+synthetic core whose inputs are already resident in HBM.  Metric: k-eff
+solves per second of the plain FP64 drop-in, with the FP32+refinement form
+timed on the same core beside it; both are checked against the FP64
+reference result (k within 1 pcm, node powers within 1e-4).
+
+Also reported:
+  roofline      BiCGSTAB K1 (the fused p-update + 7-point SpMV + (rs,v) dot),
+                13 algorithmic words per node per launch, timed with HIP
+                events on the engine's stream during the timed solves;
+  cpu_baseline  the reference's own Fortran build (oracle/_ref) on one host
+                core, on a bounded sample of the same core.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+WORKLOADS = {
+    "core3d_2g": dict(seed=1, fixture="core3d_2g_s1_solve",
+                      desc="3D 2-group quarter core, 34x34x26 nodes (17x17 assemblies of 21.5 cm, 2x2 nodes, "
+                           "24 fuel planes of 15 cm + 2 reflector planes), SANM, per-node +-2% XS"),
+    "core3d_8g": dict(seed=2, fixture="core3d_8g_s2_solve",
+                      desc="3D 8-group quarter core, 68x68x96 nodes (17x17 assemblies, 4x4 nodes, 24 layers x 4), "
+                           "SANM, downscatter + upscatter into groups 6-8"),
+}
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def time_solves(eng, steps, warmup, profile=True):
+    for _ in range(warmup):
+        eng.reset()
+        eng.outer()
+    eng.set_profiling(profile)
+    ks, niters, tk1, nk1 = [], [], 0.0, 0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        eng.reset()
+        eng.outer()
+        pw = eng.powdis()          # synchronises the stream
+        ks.append(eng.Ke)
+        niters.append(eng.niter)
+        pr = eng.read_profile()
+        tk1 += pr["k1_ms"] * pr["k1_n"]
+        nk1 += pr["k1_n"]
+    dt = time.perf_counter() - t0
+    eng.set_profiling(False)
+    return dict(seconds=dt, solves_per_s=steps / dt, k=ks[-1], niter=niters[-1], pow=pw,
+                k1_avg_ms=(tk1 / nk1) if nk1 else None, k1_samples=nk1)
+
+
+def cpu_baseline(problem, outers_per_solve, nupd):
+    """The reference build (oracle/_ref/ref_dump, one host core) on a bounded
+    sample: a short run of NS outers plus one nodal update, scaled to the
+    full solve's outer count."""
+    ref = os.path.join(ROOT, "oracle", "_ref", "ref_dump")
+    from adpres_amd.problem import write_deck, write_xs_override
+    from adpres_amd.configs import generate
+    if not os.path.exists(ref):
+        return None
+    gc = generate(problem.name, WORKLOADS[problem.name]["seed"])
+    p = gc.problem
+    d = tempfile.mkdtemp(prefix="adg_cpu_")
+    try:
+        ns = 3 if p.nnod * p.ng > 1_000_000 else 40
+        # %ITER that stops after ns outers with one nodal update at ns
+        it = (ns, p.nin, 1e-30, 1e-30, p.nac, ns)
+        write_deck(os.path.join(d, "deck"), kern=p.kern, iter_card=it, **gc.deck)
+        args = [ref, "deck", "solve", "out.bin"]
+        if gc.perturbed:
+            write_xs_override(os.path.join(d, "xs.bin"), p)
+            args.append("xs.bin")
+        t0 = time.perf_counter()
+        r = subprocess.run(["bash", "-c", "ulimit -s unlimited; exec " + " ".join(args)], cwd=d,
+                           capture_output=True, text=True)
+        wall = time.perf_counter() - t0
+        # the reference stops at NOUT (its max-outer STOP); the wall time of the
+        # sample is parse + XS + ns outers + 1 nodal update
+        t_outer_est = wall / (ns + ns / nupd)
+        solve_s = t_outer_est * (outers_per_solve + outers_per_solve / nupd)
+        return {"value": 1.0 / solve_s, "unit": "solves/s", "cores": 1, "kind": "reference",
+                "sample": f"ADPRES Fortran (amdflang -O3) on the same core: {ns} outers + 1 nodal update "
+                          f"({wall:.1f} s wall incl. deck parse), scaled to {outers_per_solve} outers"}
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="core3d_2g", choices=sorted(WORKLOADS))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp32", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this engine runs one core per GPU and this build measures one GPU only; "
+              f"--gpus {args.gpus} is not supported (multi-GPU sharding is a design note)", file=sys.stderr)
+        sys.exit(2)
+
+    from adpres_amd.configs import generate
+    from adpres_amd.engine import Engine
+
+    w = WORKLOADS[args.workload]
+    p = generate(args.workload, w["seed"]).problem
+    fx = os.path.join(ROOT, "tests", "golden", w["fixture"] + ".npz")
+    ref = np.load(fx) if os.path.exists(fx) else None
+    log(f"bench: {args.workload} N={p.nnod} G={p.ng} unknowns={p.unknowns}")
+
+    eng = Engine(p, "fp64")
+    r64 = time_solves(eng, args.steps, args.warmup)
+    eng.close()
+    log(f"bench: fp64 {r64['solves_per_s']:.3f} solves/s k={r64['k']:.8f} outers={r64['niter']}")
+    r32 = None
+    if not args.no_fp32:
+        eng = Engine(p, "fp32_refine")
+        r32 = time_solves(eng, args.steps, args.warmup)
+        eng.close()
+        log(f"bench: fp32_refine {r32['solves_per_s']:.3f} solves/s k={r32['k']:.8f} outers={r32['niter']}")
+
+    def errs(r):
+        if ref is None or r is None:
+            return None
+        m = ref["pow"] > 0
+        return dict(k_err_pcm=abs(r["k"] - float(ref["Ke"])) * 1e5,
+                    pow_max_rel_err=float(np.max(np.abs(r["pow"][m] - ref["pow"][m]) / ref["pow"][m])))
+
+    # roofline of BiCGSTAB K1: r, p, v in; p, v out; 7 coefficients; rs = 13 words/node
+    k1_bytes = 13 * 8 * p.nnod
+    roof = None
+    if r64["k1_avg_ms"]:
+        ach = k1_bytes / (r64["k1_avg_ms"] * 1e-3) / 1e9
+        roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "k_bicg1<double> (fused p-update + 7-point SpMV + (rs,v))",
+                "algorithmic_bytes_per_launch": k1_bytes, "avg_launch_us": round(r64["k1_avg_ms"] * 1e3, 3),
+                "samples": r64["k1_samples"]}
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(p, r64["niter"], p.nupd)
+
+    out = {
+        "metric": "k-eff solves/s (FP64 GPU drop-in; FP32+refine beside it) at 1 pcm",
+        "value": round(r64["solves_per_s"], 4),
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * r64["seconds"] / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded generator, adpres_amd/configs.py)",
+        "config": {"workload": args.workload, "description": w["desc"], "nodes": p.nnod, "groups": p.ng,
+                   "unknowns": p.unknowns, "kern": p.kern,
+                   "iter": {"nout": p.nout, "nin": p.nin, "serc": p.serc, "ferc": p.ferc, "nac": p.nac,
+                            "nupd": p.nupd},
+                   "outers_per_solve": r64["niter"]},
+        "parity_fp64": errs(r64),
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    if r32 is not None:
+        out["fp32_refine"] = {"solves_per_s": round(r32["solves_per_s"], 4),
+                              "speedup_vs_fp64": round(r32["solves_per_s"] / r64["solves_per_s"], 3),
+                              "outers_per_solve": r32["niter"], "parity": errs(r32),
+                              "k1_avg_us": round(r32["k1_avg_ms"] * 1e3, 3) if r32["k1_avg_ms"] else None}
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()
