@@ -66,10 +66,12 @@ __global__ void __launch_bounds__(NT) k_coupcoef(Geo q, XS xs, double *df) {
 
 // Matrix slots per row: 0 z-, 1 y-, 2 x-, 3 diag, 4 x+, 5 y+, 6 z+.  Absent
 // neighbours (and every slot of an inactive cell) hold 0.
+// sched != 0: rebuild only in a nodal-update outer (device-side schedule)
 template <typename T>
 __global__ void __launch_bounds__(NT) k_matrix(Geo q, XS xs, const double *df, const double *dn,
-                                               double *A64, T *AT) {
+                                               double *A64, T *AT, const Scal *sc, int sched) {
 #pragma clang fp contract(off)
+  if (sched && !nodal_outer(sc)) return;
   int i, j, k;
   long c;
   long t = (long)blockIdx.x * NT + threadIdx.x;
@@ -308,11 +310,12 @@ __global__ void __launch_bounds__(NT) k_integrals(Geo q, XS xs, Scal *sc, const 
   }
 }
 
-__global__ void __launch_bounds__(NT) k_fsrc(Geo q, XS xs, Scal *sc, int adjoint, int extrap, const double *f0,
+__global__ void __launch_bounds__(NT) k_fsrc(Geo q, XS xs, Scal *sc, int adjoint, const double *f0,
                                              const double *f0c, double *fs0, const double *fs0c, double *errn,
                                              double *part, unsigned *cnt) {
 #pragma clang fp contract(off)
   if (sc->done) return;
+  const int extrap = ((sc->p + 1) % sc->nac) == 0;   // MOD(p, nac) == 0
   const long NC = q.NC;
   double acc[2] = {0.0, 0.0};   // sum errn^2, max flux rel. error
   for (long c = q.c0 + (long)blockIdx.x * NT + threadIdx.x; c < q.c0 + q.M; c += (long)gridDim.x * NT) {
@@ -343,10 +346,11 @@ __global__ void __launch_bounds__(NT) k_fsrc(Geo q, XS xs, Scal *sc, int adjoint
   }
 }
 
-__global__ void __launch_bounds__(NT) k_post(Geo q, XS xs, Scal *sc, int extrap, double *fs0, const double *fs0c,
+__global__ void __launch_bounds__(NT) k_post(Geo q, XS xs, Scal *sc, double *fs0, const double *fs0c,
                                              const double *errn, double *part, unsigned *cnt) {
 #pragma clang fp contract(off)
   if (sc->done) return;
+  const int extrap = sc->flags & 1;
   const double coef = sc->coef;
   double acc[2] = {0.0, 0.0};   // integral, max fission-source rel. error
   for (long c = q.c0 + (long)blockIdx.x * NT + threadIdx.x; c < q.c0 + q.M; c += (long)gridDim.x * NT) {
@@ -367,11 +371,12 @@ __global__ void __launch_bounds__(NT) k_post(Geo q, XS xs, Scal *sc, int extrap,
 }
 
 // end of one outer: trace, error checks, convergence test (:491-496)
-__global__ void k_end(Scal *sc, int p, int nodal, double serc, double ferc, double *tk, double *tser,
-                      double *tfer, double *tnd, int *tflags, int *tloc) {
+__global__ void k_end(Scal *sc, double *tk, double *tser, double *tfer, double *tnd, int *tflags, int *tloc) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   if (sc->done) return;
-  int flags = sc->flags | (nodal ? 2 : 0);
+  const int nodal = nodal_outer(sc) ? 1 : 0;
+  const int p = sc->p + 1;
+  const int flags = (sc->flags & 1) | (nodal ? 2 : 0);
   const int t = p - 1;
   tk[t] = sc->Ke;
   tser[t] = sc->ser;
@@ -392,7 +397,7 @@ __global__ void k_end(Scal *sc, int p, int nodal, double serc, double ferc, doub
     sc->done = 1;
     return;
   }
-  if (sc->ser < serc && sc->fer < ferc && sc->ndmax < 1.e-2) sc->done = 1;
+  if (sc->ser < sc->serc && sc->fer < sc->ferc && sc->ndmax < 1.e-2) sc->done = 1;
 }
 
 // PowDis (src/mod_cmfd.f90:1290-1333)

@@ -44,8 +44,16 @@ struct Scal {
   int p, done, status, flags;
   int im, jm, km, pad;
   unsigned long long lu_key;     // first LU pivot failure (order key << 30 | packed loc)
-  double res0, res1;             // spare diagnostics
+  // iteration schedule of the running outer call (%ITER): outer p+1 extrapolates
+  // when (p+1) % nac == 0 and updates the D-hats when nodal_on && (p+1) % nupd == 0
+  int nac, nupd, nodal_on, pad2;
+  double serc, ferc;
 };
+
+// true when the outer now running (sc->p + 1) is a nodal-update outer
+__device__ __forceinline__ bool nodal_outer(const Scal *sc) {
+  return !sc->done && sc->nodal_on && ((sc->p + 1) % sc->nupd) == 0;
+}
 
 struct XS {
   const double *D, *sigr, *nuf, *sigf, *chi, *sigs, *dc, *exsrc, *vdel;

@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -71,6 +72,8 @@ struct adg_ctx {
   Scal *hsc = nullptr;   // pinned host mirror
   int nblk = 1;
   bool xs_set = false, first = true;
+  bool graphs = true;     // replay the outer iteration as a captured hipGraph
+  int check_every = 8;    // outers between host polls of the done flag
   double Ke_host = 1.0;
   double timing[4] = {0, 0, 0, 0};
   std::string err;
@@ -83,6 +86,12 @@ struct adg_ctx {
   int ev_next = 0;
   std::vector<ProfRec> recs;
   double prof_out[6] = {0, 0, 0, 0, 0, 0};
+  // trace buffers (device) and the captured outer iteration
+  double *tk = nullptr, *tser = nullptr, *tfer = nullptr, *tnd = nullptr;
+  int *tflags = nullptr, *tloc = nullptr;
+  int tcap = 0;
+  hipGraphExec_t gexec = nullptr;
+  int gkey[4] = {-1, -1, -1, -1};   // adjoint, nin, kern, nupd of the captured iteration
 
   Geo geo() const {
     Geo q;
@@ -154,13 +163,14 @@ void gather(const adg_ctx *c, const double *src_dev, int K, double *dst, std::ve
     for (int n = 0; n < c->N; ++n) dst[(size_t)k * c->N + n] = stage[(size_t)k * c->NC + c->cell[n]];
 }
 
-void launch_matrix(adg_ctx *c) {
+void launch_matrix(adg_ctx *c, int sched) {
   Geo q = c->geo();
   if (c->precision == ADG_FP32_REFINE)
-    hipLaunchKernelGGL(k_matrix<float>, cells_grid(c, c->G), dim3(NT), 0, c->s, q, c->xs(), c->df, c->dn, c->A64, c->A32);
+    hipLaunchKernelGGL(k_matrix<float>, cells_grid(c, c->G), dim3(NT), 0, c->s, q, c->xs(), c->df, c->dn, c->A64, c->A32,
+                       c->sc, sched);
   else
     hipLaunchKernelGGL(k_matrix<double>, cells_grid(c, c->G), dim3(NT), 0, c->s, q, c->xs(), c->df, c->dn, c->A64,
-                       (double *)nullptr);
+                       (double *)nullptr, c->sc, sched);
   HIPCHK(hipGetLastError());
 }
 
@@ -213,12 +223,12 @@ void group_solve(adg_ctx *c, int g, int adjoint, int first_group, int nin) {
 }
 
 // nodal::nodal_update(_pnm) + matrix rebuild (src/mod_cmfd.f90:339-383)
-void nodal_update(adg_ctx *c, int kern, int cmode) {
+void nodal_update(adg_ctx *c, int kern, int cmode, int sched) {
   NodArgs a;
   a.q = c->geo(); a.xs = c->xs(); a.sc = c->sc;
   a.f0 = c->f0; a.df = c->df; a.dn = c->dn;
   a.S = c->S; a.a2 = c->a2; a.a4 = c->a4; a.L1 = c->L1;
-  a.kern = kern; a.cmode = cmode;
+  a.kern = kern; a.cmode = cmode; a.sched = sched;
   a.part = c->npart; a.cnt = c->cnt + 4;
   // pass 2 runs as a (blocks, 3) grid; blocks cover the largest direction
   const dim3 g2(c->npart_blocks, 3, 1);
@@ -232,12 +242,12 @@ void nodal_update(adg_ctx *c, int kern, int cmode) {
     NOD_CASE(1) NOD_CASE(2) NOD_CASE(3) NOD_CASE(4) NOD_CASE(5) NOD_CASE(6) NOD_CASE(7) NOD_CASE(8)
 #undef NOD_CASE
   }
-  hipLaunchKernelGGL(k_nod_argmax, dim3(1), dim3(NT), 0, c->s, c->sc, c->npart, (int)(3 * c->npart_blocks));
+  hipLaunchKernelGGL(k_nod_argmax, dim3(1), dim3(NT), 0, c->s, c->sc, c->npart, (int)(3 * c->npart_blocks), sched);
   HIPCHK(hipGetLastError());
-  launch_matrix(c);
+  launch_matrix(c, sched);
 }
 
-void set_scal_start(adg_ctx *c) {
+void set_scal_start(adg_ctx *c, const adg_iter *it) {
   // host-side initialisation of the device scalars for an outer call
   Scal h;
   memset(&h, 0, sizeof(h));
@@ -248,7 +258,53 @@ void set_scal_start(adg_ctx *c) {
   h.flags = 0;
   h.lu_key = ~0ull;
   h.Ke = c->Ke_host;
+  h.nac = it ? it->nac : 1;
+  h.nupd = it ? it->nupd : 1;
+  h.nodal_on = (it && it->kern != ADG_KERN_FDM) ? 1 : 0;
+  h.serc = it ? it->serc : 0.0;
+  h.ferc = it ? it->ferc : 0.0;
   HIPCHK(hipMemcpy(c->sc, &h, sizeof(Scal), hipMemcpyHostToDevice));
+}
+
+void ensure_trace(adg_ctx *c, int nout) {
+  if (nout <= c->tcap) return;
+  void *ptrs[] = {c->tk, c->tser, c->tfer, c->tnd, c->tflags, c->tloc};
+  for (void *pp : ptrs)
+    if (pp) (void)hipFree(pp);
+  const int cap = std::max(nout, 2 * c->tcap);
+  c->tk = dalloc<double>(cap); c->tser = dalloc<double>(cap); c->tfer = dalloc<double>(cap);
+  c->tnd = dalloc<double>(cap); c->tflags = dalloc<int>(cap); c->tloc = dalloc<int>(3 * (size_t)cap);
+  c->tcap = cap;
+  if (c->gexec) {
+    (void)hipGraphExecDestroy(c->gexec);
+    c->gexec = nullptr;
+  }
+}
+
+// One outer iteration (src/mod_cmfd.f90:465-496) as a fixed kernel sequence:
+// G group solves, FSrc + error norms, extrapolation + k update, the nodal
+// update (its kernels return at once unless this outer is a nodal-update
+// outer -- the schedule lives on the device), and the end-of-outer kernel.
+// p_host is the outer index as seen by the host (used for profiling only).
+void enqueue_iteration(adg_ctx *c, int adjoint, int nin, int kern, int p_host, int nupd) {
+  Geo q = c->geo();
+  XS x = c->xs();
+  for (int gg = 0; gg < c->G; ++gg) {
+    const int g = adjoint ? c->G - 1 - gg : gg;
+    group_solve(c, g, adjoint, gg == 0 ? 1 : 0, nin);
+  }
+  hipLaunchKernelGGL(k_fsrc, dim3(c->nblk), dim3(NT), 0, c->s, q, x, c->sc, adjoint, c->f0, c->f0c, c->fs0, c->fs0c,
+                     c->errn, c->part, c->cnt + 6);
+  hipLaunchKernelGGL(k_post, dim3(c->nblk), dim3(NT), 0, c->s, q, x, c->sc, c->fs0, c->fs0c, c->errn, c->part,
+                     c->cnt + 7);
+  if (kern != ADG_KERN_FDM) {
+    const bool timed = p_host > 0 && (p_host % nupd) == 0;
+    const int en0 = timed ? prof_event(c) : -1;
+    nodal_update(c, kern, adjoint ? 0 : 1, 1);
+    if (timed) prof_pair(c, 2, en0, prof_event(c));
+  }
+  hipLaunchKernelGGL(k_end, dim3(1), dim3(64), 0, c->s, c->sc, c->tk, c->tser, c->tfer, c->tnd, c->tflags, c->tloc);
+  HIPCHK(hipGetLastError());
 }
 
 }  // namespace
@@ -281,6 +337,8 @@ int adg_create(const adg_geom *g, adg_ctx **out) {
   adg_ctx *c = new adg_ctx();
   try {
     c->N = g->nnod; c->G = g->ng; c->nmat = g->nmat;
+    if (const char *ev = getenv("ADG_GRAPH")) c->graphs = atoi(ev) != 0;
+    if (const char *ev = getenv("ADG_CHECK")) c->check_every = std::max(1, atoi(ev));
     c->nxx = g->nxx; c->nyy = g->nyy; c->nzz = g->nzz; c->precision = g->precision;
     c->NXP = c->nxx + 2; c->NYP = c->nyy + 2; c->NZP = c->nzz + 2;
     c->sy = c->NXP; c->sz = (long)c->NXP * c->NYP; c->NC = c->sz * c->NZP;
@@ -397,6 +455,10 @@ void adg_destroy(adg_ctx *c) {
     if (p) (void)hipFree(p);
   if (c->hsc) (void)hipHostFree(c->hsc);
   for (hipEvent_t e : c->evs) (void)hipEventDestroy(e);
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+  void *tptr[] = {c->tk, c->tser, c->tfer, c->tnd, c->tflags, c->tloc};
+  for (void *p : tptr)
+    if (p) (void)hipFree(p);
   if (c->s) (void)hipStreamDestroy(c->s);
   delete c;
 }
@@ -490,14 +552,11 @@ int adg_outer(adg_ctx *c, int mode, const adg_iter *it, adg_trace *tr) {
   if (mode != ADG_FORWARD && mode != ADG_ADJOINT) return fail(c, ADG_E_ARG, "bad mode");
   const int adjoint = mode == ADG_ADJOINT;
   const int nout = it->nout;
-  double *tk = nullptr, *tser = nullptr, *tfer = nullptr, *tnd = nullptr;
-  int *tflags = nullptr, *tloc = nullptr;
   int rc = ADG_OK;
   try {
     Geo q = c->geo();
     XS x = c->xs();
-    tk = dalloc<double>(nout); tser = dalloc<double>(nout); tfer = dalloc<double>(nout); tnd = dalloc<double>(nout);
-    tflags = dalloc<int>(nout); tloc = dalloc<int>(3 * nout);
+    ensure_trace(c, nout);
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
@@ -507,46 +566,52 @@ int adg_outer(adg_ctx *c, int mode, const adg_iter *it, adg_trace *tr) {
       scatter(c, ones.data(), c->G, c->f0, st);
       c->Ke_host = 1.0;
     }
-    set_scal_start(c);
+    set_scal_start(c, it);
     c->ev_next = 0;
     c->recs.clear();
     HIPCHK(hipEventRecord(e0, c->s));
     // matrix_setup(1): D-tilde and the matrix from the current D-hat
     hipLaunchKernelGGL(k_coupcoef, cells_grid(c, c->G), dim3(NT), 0, c->s, q, x, c->df);
-    launch_matrix(c);
+    launch_matrix(c, 0);
     if (c->first) {
       hipLaunchKernelGGL(k_fsrc_init, dim3(c->nblk), dim3(NT), 0, c->s, q, x, adjoint, c->f0, c->fs0);
       c->first = false;
     }
     hipLaunchKernelGGL(k_integrals, dim3(c->nblk), dim3(NT), 0, c->s, q, x, c->sc, c->fs0, c->part, c->cnt + 5);
     HIPCHK(hipGetLastError());
-    int p = 0;
-    bool done = false;
-    int check = 1;
-    for (p = 1; p <= nout && !done; ++p) {
-      for (int gg = 0; gg < c->G; ++gg) {
-        const int g = adjoint ? c->G - 1 - gg : gg;
-        group_solve(c, g, adjoint, gg == 0 ? 1 : 0, it->nin);
+    // replay: a captured graph of one outer (or direct launches when
+    // profiling, whose events need the host-side sequence); the done flag
+    // written by k_end turns the remaining kernels into no-ops, and the host
+    // polls it every `check` outers
+    const bool use_graph = !c->prof && c->graphs;
+    if (use_graph) {
+      const int key[4] = {adjoint, it->nin, it->kern, it->nupd};
+      if (!c->gexec || memcmp(key, c->gkey, sizeof key) != 0) {
+        if (c->gexec) {
+          (void)hipGraphExecDestroy(c->gexec);
+          c->gexec = nullptr;
+        }
+        hipGraph_t graph;
+        HIPCHK(hipStreamBeginCapture(c->s, hipStreamCaptureModeThreadLocal));
+        enqueue_iteration(c, adjoint, it->nin, it->kern, 0, it->nupd);
+        HIPCHK(hipStreamEndCapture(c->s, &graph));
+        HIPCHK(hipGraphInstantiate(&c->gexec, graph, nullptr, nullptr, 0));
+        (void)hipGraphDestroy(graph);
+        memcpy(c->gkey, key, sizeof key);
       }
-      const int extrap = (p % it->nac) == 0 ? 1 : 0;
-      hipLaunchKernelGGL(k_fsrc, dim3(c->nblk), dim3(NT), 0, c->s, q, x, c->sc, adjoint, extrap, c->f0, c->f0c, c->fs0,
-                         c->fs0c, c->errn, c->part, c->cnt + 6);
-      hipLaunchKernelGGL(k_post, dim3(c->nblk), dim3(NT), 0, c->s, q, x, c->sc, extrap, c->fs0, c->fs0c, c->errn,
-                         c->part, c->cnt + 7);
-      const int nodal = ((p % it->nupd) == 0 && it->kern != ADG_KERN_FDM) ? 1 : 0;
-      if (nodal) {
-        const int en0 = prof_event(c);
-        nodal_update(c, it->kern, adjoint ? 0 : 1);
-        prof_pair(c, 2, en0, prof_event(c));
+    }
+    const int check = c->check_every;
+    int launched = 0;
+    while (launched < nout) {
+      const int n = std::min(check, nout - launched);
+      for (int k = 0; k < n; ++k) {
+        ++launched;
+        if (use_graph) HIPCHK(hipGraphLaunch(c->gexec, c->s));
+        else enqueue_iteration(c, adjoint, it->nin, it->kern, launched, it->nupd);
       }
-      hipLaunchKernelGGL(k_end, dim3(1), dim3(64), 0, c->s, c->sc, p, nodal, it->serc, it->ferc, tk, tser, tfer, tnd,
-                         tflags, tloc);
-      HIPCHK(hipGetLastError());
-      if (p % check == 0 || p == nout) {
-        HIPCHK(hipMemcpyAsync(c->hsc, c->sc, sizeof(Scal), hipMemcpyDeviceToHost, c->s));
-        HIPCHK(hipStreamSynchronize(c->s));
-        done = c->hsc->done != 0;
-      }
+      HIPCHK(hipMemcpyAsync(c->hsc, c->sc, sizeof(Scal), hipMemcpyDeviceToHost, c->s));
+      HIPCHK(hipStreamSynchronize(c->s));
+      if (c->hsc->done) break;
     }
     HIPCHK(hipEventRecord(e1, c->s));
     HIPCHK(hipMemcpyAsync(c->hsc, c->sc, sizeof(Scal), hipMemcpyDeviceToHost, c->s));
@@ -580,12 +645,12 @@ int adg_outer(adg_ctx *c, int mode, const adg_iter *it, adg_trace *tr) {
       const int ncp = std::min(nrec, tr->cap);
       tr->n = ncp;
       if (ncp > 0) {
-        if (tr->k) HIPCHK(hipMemcpy(tr->k, tk, ncp * sizeof(double), hipMemcpyDeviceToHost));
-        if (tr->ser) HIPCHK(hipMemcpy(tr->ser, tser, ncp * sizeof(double), hipMemcpyDeviceToHost));
-        if (tr->fer) HIPCHK(hipMemcpy(tr->fer, tfer, ncp * sizeof(double), hipMemcpyDeviceToHost));
-        if (tr->ndmax) HIPCHK(hipMemcpy(tr->ndmax, tnd, ncp * sizeof(double), hipMemcpyDeviceToHost));
-        if (tr->flags) HIPCHK(hipMemcpy(tr->flags, tflags, ncp * sizeof(int), hipMemcpyDeviceToHost));
-        if (tr->loc) HIPCHK(hipMemcpy(tr->loc, tloc, 3 * ncp * sizeof(int), hipMemcpyDeviceToHost));
+        if (tr->k) HIPCHK(hipMemcpy(tr->k, c->tk, ncp * sizeof(double), hipMemcpyDeviceToHost));
+        if (tr->ser) HIPCHK(hipMemcpy(tr->ser, c->tser, ncp * sizeof(double), hipMemcpyDeviceToHost));
+        if (tr->fer) HIPCHK(hipMemcpy(tr->fer, c->tfer, ncp * sizeof(double), hipMemcpyDeviceToHost));
+        if (tr->ndmax) HIPCHK(hipMemcpy(tr->ndmax, c->tnd, ncp * sizeof(double), hipMemcpyDeviceToHost));
+        if (tr->flags) HIPCHK(hipMemcpy(tr->flags, c->tflags, ncp * sizeof(int), hipMemcpyDeviceToHost));
+        if (tr->loc) HIPCHK(hipMemcpy(tr->loc, c->tloc, 3 * ncp * sizeof(int), hipMemcpyDeviceToHost));
       }
     }
     if (h.status != 0) {
@@ -609,9 +674,6 @@ int adg_outer(adg_ctx *c, int mode, const adg_iter *it, adg_trace *tr) {
   } catch (const HipError &e) {
     rc = hip_fail(c, e);
   }
-  void *ptrs[] = {tk, tser, tfer, tnd, tflags, tloc};
-  for (void *pp : ptrs)
-    if (pp) (void)hipFree(pp);
   return rc;
 }
 
@@ -638,11 +700,11 @@ int adg_nodal_update(adg_ctx *c, int kern, int cmode, double *ndmax, int *loc3) 
   if (!c->xs_set) return fail(c, ADG_E_NOT_SET, "adg_set_xs has not been called");
   if (kern != ADG_KERN_PNM && kern != ADG_KERN_SANM) return fail(c, ADG_E_ARG, "nodal kernel must be PNM or SANM");
   try {
-    set_scal_start(c);
+    set_scal_start(c, nullptr);
     Geo q = c->geo();
     hipLaunchKernelGGL(k_coupcoef, cells_grid(c, c->G), dim3(NT), 0, c->s, q, c->xs(), c->df);
     HIPCHK(hipGetLastError());
-    nodal_update(c, kern, cmode);
+    nodal_update(c, kern, cmode, 0);
     HIPCHK(hipMemcpyAsync(c->hsc, c->sc, sizeof(Scal), hipMemcpyDeviceToHost, c->s));
     HIPCHK(hipStreamSynchronize(c->s));
     if (ndmax) *ndmax = c->hsc->ndmax;
@@ -659,10 +721,10 @@ int adg_bicg(adg_ctx *c, int g, int nin, const double *b, double *x) {
   if (!c || !b || !x || g < 0 || g >= c->G || nin < 1) return ADG_E_ARG;
   if (!c->xs_set) return fail(c, ADG_E_NOT_SET, "adg_set_xs has not been called");
   try {
-    set_scal_start(c);
+    set_scal_start(c, nullptr);
     Geo q = c->geo();
     hipLaunchKernelGGL(k_coupcoef, cells_grid(c, c->G), dim3(NT), 0, c->s, q, c->xs(), c->df);
-    launch_matrix(c);
+    launch_matrix(c, 0);
     std::vector<double> st;
     // right-hand side goes to c->b, the initial guess to the flux of group g
     scatter(c, b, 1, c->b, st);
@@ -716,7 +778,7 @@ int adg_matvec(adg_ctx *c, int g, const double *x, double *y) {
   try {
     Geo q = c->geo();
     hipLaunchKernelGGL(k_coupcoef, cells_grid(c, c->G), dim3(NT), 0, c->s, q, c->xs(), c->df);
-    launch_matrix(c);
+    launch_matrix(c, 0);
     std::vector<double> st;
     scatter(c, x, 1, c->b, st);
     HIPCHK(hipMemsetAsync(c->tmp, 0, c->NC * sizeof(double), c->s));

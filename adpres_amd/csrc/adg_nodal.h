@@ -29,6 +29,7 @@ struct NodArgs {
   double *a2, *a4, *L1; // [3][G][NC]
   int kern;             // 1 PNM, 2 SANM
   int cmode;            // 1 forward, 0 adjoint
+  int sched;            // 1: run only in a nodal-update outer (device-side schedule)
   double *part;
   unsigned *cnt;
 };
@@ -52,7 +53,7 @@ __global__ void __launch_bounds__(NT) k_nod_src(NodArgs a) {
   const long nxy = (long)q.nxx * q.nyy;
   long t = (long)blockIdx.x * NT + threadIdx.x;
   if (t >= nxy * q.nzz) return;
-  if (a.sc->done) return;
+  if (a.sc->done || (a.sched && !nodal_outer(a.sc))) return;
   k = (int)(t / nxy);
   { long r = t - (long)k * nxy; j = (int)(r / q.nxx); i = (int)(r - (long)j * q.nxx); }
   c = cell_of(q, i, j, k);
@@ -283,7 +284,7 @@ __global__ void __launch_bounds__(NT) k_nod_pass1(NodArgs a) {
   const long nxy = (long)q.nxx * q.nyy;
   long t = (long)blockIdx.x * NT + threadIdx.x;
   if (t >= nxy * q.nzz) return;
-  if (a.sc->done) return;
+  if (a.sc->done || (a.sched && !nodal_outer(a.sc))) return;
   int i, j, k;
   k = (int)(t / nxy);
   { long r = t - (long)k * nxy; j = (int)(r / q.nxx); i = (int)(r - (long)j * q.nxx); }
@@ -338,6 +339,7 @@ template <int G>
 __global__ void __launch_bounds__(NT) k_nod_pass2(NodArgs a) {
 #pragma clang fp contract(off)
   const Geo &q = a.q;
+  if (a.sched && !nodal_outer(a.sc)) return;   // uniform over the grid
   const int u = blockIdx.y;
   const long NC = q.NC;
   // faces of direction u: (n_u + 1) x (other two extents)
@@ -556,8 +558,8 @@ __global__ void __launch_bounds__(NT) k_nod_pass2(NodArgs a) {
 }
 
 // final argmax over the pass-2 partials: one block, sweep order x, y, z
-__global__ void __launch_bounds__(NT) k_nod_argmax(Scal *sc, const double *part, int nslots) {
-  if (sc->done) return;
+__global__ void __launch_bounds__(NT) k_nod_argmax(Scal *sc, const double *part, int nslots, int sched) {
+  if (sc->done || (sched && !nodal_outer(sc))) return;
   __shared__ double sv[NT], sk[NT], sl[NT];
   double best = -1.0, bkey = 0.0, bloc = 0.0;
   for (int s = threadIdx.x; s < nslots; s += NT) {
