@@ -28,8 +28,9 @@ def run_deck(deck, tmp_path, env=None):
                        capture_output=True, text=True, timeout=120, env=e)
     assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-3000:]
     assert "ADPRES EXIT NORMALLY" in r.stdout
-    rows = [(int(t[0]), float(t[1]), float(t[2]), float(t[3])) for t in
-            (ln.split() for ln in r.stdout.splitlines()) if len(t) == 4 and t[0].isdigit()]
+    line = re.compile(r"^\s*(\d+)\s+(\d+\.\d{6})\s+(\S+E[+-]\d\d)\s+(\S+E[+-]\d\d)\s*$")
+    rows = [(int(m.group(1)), float(m.group(2)), float(m.group(3)), float(m.group(4)))
+            for m in map(line.match, r.stdout.splitlines()) if m]
     k = float(re.search(r"MULTIPLICATION EFFECTIVE \(K-EFF\) =\s+([0-9.]+)", r.stdout).group(1))
     return rows, k, r.stdout
 
