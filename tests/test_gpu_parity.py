@@ -86,10 +86,15 @@ def test_gpu_solve_matches_reference(name, precision):
     assert abs(k - float(d["Ke"])) < K_TOL
     assert pow_err(pw, d["pow"]) < POW_TOL
     if precision == "fp64":
-        # same algorithm, same stopping rule: the trace follows the reference
+        # same algorithm, same stopping rule: the trace follows the reference.
+        # The FDM deck (power iteration + extrapolation every 15 outers, 5
+        # inners) is chaotic in its outer count: a 1e-15 relative change of
+        # the initial flux moves the reference's own 303 outers to 508-783,
+        # so only its early trace and the fixed point are compared.
         n = len(d["trace_k"])
-        assert abs(e.niter - n) <= 2
-        m = min(n, e.niter) - 1
+        if name != "fdm_solve":
+            assert abs(e.niter - n) <= 2
+        m = min(n, e.niter, 20 if name == "fdm_solve" else n) - 1
         assert np.allclose(e.trace["k"][:m], d["trace_k"][:m], atol=2e-6)
     e.close()
 
@@ -102,6 +107,19 @@ def test_gpu_core3d_2g_matches_reference(precision):
     e.outer()
     assert abs(e.Ke - float(d["Ke"])) < K_TOL
     assert pow_err(e.powdis(), d["pow"]) < POW_TOL
+    e.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32_refine"])
+def test_gpu_core3d_8g_matches_reference(precision):
+    """68x68x96 nodes, 8 groups, upscatter into groups 6-8 (3.55M unknowns);
+    the reference took 500 outers and about an hour on one core."""
+    d = golden("core3d_8g_s2_solve")
+    p = generate("core3d_8g", int(d["seed"])).problem
+    e = Engine(p, precision=precision)
+    e.outer()
+    assert abs(e.Ke - float(d["Ke"])) < K_TOL
+    assert pow_err(e.powdis(), d["pow"].astype(np.float64)) < POW_TOL
     e.close()
 
 
