@@ -54,10 +54,12 @@ def test_dropin_iaea3d_published_lines(tmp_path):
     line = [ln for ln in out.splitlines() if ln.startswith("MAX. CHANGE IN NODAL COUPLING COEF.")][0]
     print(line)
     assert line.startswith("MAX. CHANGE IN NODAL COUPLING COEF.=  3.1684")
-    # IAEA3D is symmetric about x = y: the first update's largest change sits
-    # on a near-tie between (6,4,19) and its mirror (4,6,19), resolved by
-    # rounding-level differences of the reductions
-    assert line.endswith("AT NODE I =  6, J =  4, K = 19") or line.endswith("AT NODE I =  4, J =  6, K = 19")
+    # IAEA3D is symmetric about the diagonal through its centre, which sits at
+    # the north-west corner of this deck (x from the west edge <-> y from the
+    # north edge): the first update's largest change is a near-tie between
+    # (6,4,19) and its mirror (14,12,19), resolved by rounding-level
+    # differences of the reductions
+    assert line.endswith("AT NODE I =  6, J =  4, K = 19") or line.endswith("AT NODE I = 14, J = 12, K = 19")
     assert [r[1] for r in rows[:4]] == [0.981424, 1.001319, 1.009804, 1.013500]
 
 

@@ -87,14 +87,15 @@ def test_gpu_solve_matches_reference(name, precision):
     assert pow_err(pw, d["pow"]) < POW_TOL
     if precision == "fp64":
         # same algorithm, same stopping rule: the trace follows the reference.
-        # The FDM deck (power iteration + extrapolation every 15 outers, 5
-        # inners) is chaotic in its outer count: a 1e-15 relative change of
-        # the initial flux moves the reference's own 303 outers to 508-783,
-        # so only its early trace and the fixed point are compared.
+        # The FDM deck (136x136 nodes, only 5 BiCGSTAB iterations per outer,
+        # extrapolation every 15) is chaotic: on the oracle itself a 1e-15
+        # relative change of the initial flux moves k by 1.7e-3 within 4
+        # outers and the outer count from 303 to 508-783.  For it only the
+        # first 3 outers and the fixed point are compared.
         n = len(d["trace_k"])
         if name != "fdm_solve":
             assert abs(e.niter - n) <= 2
-        m = min(n, e.niter, 20 if name == "fdm_solve" else n) - 1
+        m = min(n, e.niter, 4 if name == "fdm_solve" else n) - 1
         assert np.allclose(e.trace["k"][:m], d["trace_k"][:m], atol=2e-6)
     e.close()
 
