@@ -167,8 +167,7 @@ __global__ void __launch_bounds__(NT) k_src(Geo q, XS xs, const Scal *sc, int g,
 
 template <typename TV>
 __global__ void __launch_bounds__(NT) k_bicg_setup(Geo q, Scal *sc, const double *A64g, const double *b,
-                                                   const double *x, TV *r, TV *rs, TV *d, double *part,
-                                                   unsigned *cnt) {
+                                                   const double *x, TV *r, TV *rs, TV *d, double *pout) {
 #pragma clang fp contract(off)
   if (sc->done) return;
   double acc[1] = {0.0};
@@ -180,13 +179,11 @@ __global__ void __launch_bounds__(NT) k_bicg_setup(Geo q, Scal *sc, const double
     if (d) d[c] = (TV)0;
     acc[0] += (double)rv * (double)rv;
   }
-  if (reduce_last<1, 0u>(acc, part, cnt)) {
-    const double rho = acc[0];
-    sc->rho_prev = 1.0;
-    sc->alpha = 1.0;
-    sc->omega = 1.0;
-    sc->rho = rho;
-    sc->beta = (rho / 1.0) * (1.0 / 1.0);
+  block_partial<1, 0u>(acc, pout);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {   // rho = alpha = omega = 1 before iteration 1
+    sc->rho_s[0] = 1.0;
+    sc->alpha_s[0] = 1.0;
+    sc->omega_s[0] = 1.0;
   }
 }
 
@@ -196,25 +193,33 @@ __device__ __forceinline__ TV pdir(const TV *r, const TV *p, const TV *v, long c
   return first ? r[c] : (TV)(r[c] + beta * (p[c] - omega * v[c]));
 }
 
+// iteration it (1-based): rho_it from the partials of the previous kernel
+// (setup or K3 of it-1), beta = (rho_it / rho_{it-1}) (alpha_{it-1} / omega_{it-1})
 template <typename TV>
 __global__ void __launch_bounds__(NT) k_bicg1(Geo q, Scal *sc, const TV *AT, const TV *r, const TV *rs,
-                                              const TV *p, const TV *v, TV *pout, TV *vout, int first,
-                                              double *part, unsigned *cnt) {
+                                              const TV *p, const TV *v, TV *pout, TV *vout, int it,
+                                              const double *pin, double *partout) {
 #pragma clang fp contract(off)
   if (sc->done) return;
-  const TV beta = (TV)sc->beta, omega = (TV)sc->omega;
+  double red[1];
+  reduce_partials<1, 0u>(pin, gridDim.x, red);
+  const int cur = it & 1, prv = cur ^ 1;
+  const double rho = red[0];
+  const double beta = (rho / sc->rho_s[prv]) * (sc->alpha_s[prv] / sc->omega_s[prv]);
+  const TV tb = (TV)beta, tw = (TV)sc->omega_s[prv];
+  const int first = it == 1;
   const long NC = q.NC, sy = q.sy, sz = q.sz;
   double acc[1] = {0.0};
   for (long c = q.c0 + (long)blockIdx.x * NT + threadIdx.x; c < q.c0 + q.M; c += (long)gridDim.x * NT) {
-    const TV pc = pdir(r, p, v, c, beta, omega, first);
+    const TV pc = pdir(r, p, v, c, tb, tw, first);
     double s = 0.0;
-    s = s + (double)(AT[0 * NC + c] * pdir(r, p, v, c - sz, beta, omega, first));
-    s = s + (double)(AT[1 * NC + c] * pdir(r, p, v, c - sy, beta, omega, first));
-    s = s + (double)(AT[2 * NC + c] * pdir(r, p, v, c - 1, beta, omega, first));
+    s = s + (double)(AT[0 * NC + c] * pdir(r, p, v, c - sz, tb, tw, first));
+    s = s + (double)(AT[1 * NC + c] * pdir(r, p, v, c - sy, tb, tw, first));
+    s = s + (double)(AT[2 * NC + c] * pdir(r, p, v, c - 1, tb, tw, first));
     s = s + (double)(AT[3 * NC + c] * pc);
-    s = s + (double)(AT[4 * NC + c] * pdir(r, p, v, c + 1, beta, omega, first));
-    s = s + (double)(AT[5 * NC + c] * pdir(r, p, v, c + sy, beta, omega, first));
-    s = s + (double)(AT[6 * NC + c] * pdir(r, p, v, c + sz, beta, omega, first));
+    s = s + (double)(AT[4 * NC + c] * pdir(r, p, v, c + 1, tb, tw, first));
+    s = s + (double)(AT[5 * NC + c] * pdir(r, p, v, c + sy, tb, tw, first));
+    s = s + (double)(AT[6 * NC + c] * pdir(r, p, v, c + sz, tb, tw, first));
     const TV vc = (TV)s;
     // neighbours read the previous p and v, so the new ones go to the other
     // buffer of a ping-pong pair
@@ -222,15 +227,21 @@ __global__ void __launch_bounds__(NT) k_bicg1(Geo q, Scal *sc, const TV *AT, con
     vout[c] = vc;
     acc[0] += (double)rs[c] * (double)vc;
   }
-  if (reduce_last<1, 0u>(acc, part, cnt)) sc->alpha = sc->rho / acc[0];
+  block_partial<1, 0u>(acc, partout);
+  if (blockIdx.x == 0 && threadIdx.x == 0) sc->rho_s[cur] = rho;
 }
 
+// alpha_it = rho_it / (rs, v)
 template <typename TV>
 __global__ void __launch_bounds__(NT) k_bicg2(Geo q, Scal *sc, const TV *AT, const TV *r, const TV *v, TV *t,
-                                              double *part, unsigned *cnt) {
+                                              int it, const double *pin, double *partout) {
 #pragma clang fp contract(off)
   if (sc->done) return;
-  const TV alpha = (TV)sc->alpha;
+  double red[1];
+  reduce_partials<1, 0u>(pin, gridDim.x, red);
+  const int cur = it & 1;
+  const double alpha_d = sc->rho_s[cur] / red[0];
+  const TV alpha = (TV)alpha_d;
   const long NC = q.NC, sy = q.sy, sz = q.sz;
   double acc[2] = {0.0, 0.0};
   for (long c = q.c0 + (long)blockIdx.x * NT + threadIdx.x; c < q.c0 + q.M; c += (long)gridDim.x * NT) {
@@ -248,16 +259,22 @@ __global__ void __launch_bounds__(NT) k_bicg2(Geo q, Scal *sc, const TV *AT, con
     acc[0] += (double)tc * (double)tc;
     acc[1] += (double)tc * (double)sc_;
   }
-  if (reduce_last<2, 0u>(acc, part, cnt)) sc->omega = acc[1] / acc[0];
+  block_partial<2, 0u>(acc, partout);
+  if (blockIdx.x == 0 && threadIdx.x == 0) sc->alpha_s[cur] = alpha_d;
 }
 
+// omega_it = (t, s) / (t, t);  d += alpha p + omega s;  r = s - omega t
 template <typename TV>
 __global__ void __launch_bounds__(NT) k_bicg3(Geo q, Scal *sc, const TV *p, const TV *v, const TV *t, TV *r,
-                                              const TV *rs, TV *d, double *x64, int last, double *part,
-                                              unsigned *cnt) {
+                                              const TV *rs, TV *d, double *x64, int it, int last,
+                                              const double *pin, double *partout) {
 #pragma clang fp contract(off)
   if (sc->done) return;
-  const TV alpha = (TV)sc->alpha, omega = (TV)sc->omega;
+  double red[2];
+  reduce_partials<2, 0u>(pin, gridDim.x, red);
+  const int cur = it & 1;
+  const double omega_d = red[1] / red[0];
+  const TV alpha = (TV)sc->alpha_s[cur], omega = (TV)omega_d;
   double acc[1] = {0.0};
   for (long c = q.c0 + (long)blockIdx.x * NT + threadIdx.x; c < q.c0 + q.M; c += (long)gridDim.x * NT) {
     const TV s = (TV)(r[c] - alpha * v[c]);
@@ -268,11 +285,8 @@ __global__ void __launch_bounds__(NT) k_bicg3(Geo q, Scal *sc, const TV *p, cons
     r[c] = rn;
     acc[0] += (double)rs[c] * (double)rn;
   }
-  if (reduce_last<1, 0u>(acc, part, cnt)) {
-    sc->rho_prev = sc->rho;
-    sc->rho = acc[0];
-    sc->beta = (sc->rho / sc->rho_prev) * (sc->alpha / sc->omega);
-  }
+  block_partial<1, 0u>(acc, partout);
+  if (blockIdx.x == 0 && threadIdx.x == 0) sc->omega_s[cur] = omega_d;
 }
 
 // ---------------------------------------------------------------------------
@@ -295,27 +309,33 @@ __global__ void __launch_bounds__(NT) k_fsrc_init(Geo q, XS xs, int adjoint, con
   }
 }
 
-// f = Integrate(fs0), e1 = Integrate(1) at the start of outer (:456-458)
-__global__ void __launch_bounds__(NT) k_integrals(Geo q, XS xs, Scal *sc, const double *fs0, double *part,
-                                                  unsigned *cnt) {
+// f = Integrate(fs0), e1 = Integrate(1) at the start of outer (:456-458):
+// partials here, finished by k_fin_integrals
+__global__ void __launch_bounds__(NT) k_integrals(Geo q, XS xs, const double *fs0, double *partout) {
 #pragma clang fp contract(off)
   double acc[2] = {0.0, 0.0};
   for (long c = q.c0 + (long)blockIdx.x * NT + threadIdx.x; c < q.c0 + q.M; c += (long)gridDim.x * NT) {
     acc[0] += xs.vdel[c] * fs0[c];
     acc[1] += xs.vdel[c];
   }
-  if (reduce_last<2, 0u>(acc, part, cnt)) {
-    sc->f = acc[0];
-    sc->e1 = acc[1];
+  block_partial<2, 0u>(acc, partout);
+}
+
+__global__ void __launch_bounds__(NT) k_fin_integrals(Scal *sc, const double *pin, int nblk) {
+  double red[2];
+  reduce_partials<2, 0u>(pin, nblk, red);
+  if (threadIdx.x == 0) {
+    sc->f = red[0];
+    sc->e1 = red[1];
   }
 }
 
+// FSrc, errn = fs0 - fs0c, partials of |errn|^2 and of RelEg
 __global__ void __launch_bounds__(NT) k_fsrc(Geo q, XS xs, Scal *sc, int adjoint, const double *f0,
                                              const double *f0c, double *fs0, const double *fs0c, double *errn,
-                                             double *part, unsigned *cnt) {
+                                             double *partout) {
 #pragma clang fp contract(off)
   if (sc->done) return;
-  const int extrap = ((sc->p + 1) % sc->nac) == 0;   // MOD(p, nac) == 0
   const long NC = q.NC;
   double acc[2] = {0.0, 0.0};   // sum errn^2, max flux rel. error
   for (long c = q.c0 + (long)blockIdx.x * NT + threadIdx.x; c < q.c0 + q.M; c += (long)gridDim.x * NT) {
@@ -331,27 +351,24 @@ __global__ void __launch_bounds__(NT) k_fsrc(Geo q, XS xs, Scal *sc, int adjoint
     errn[c] = e;
     acc[0] += e * e;
   }
-  if (reduce_last<2, 2u>(acc, part, cnt)) {
-    const double e2 = sqrt(acc[0]);
-    double coef = 0.0;
-    if (extrap) {
-      const double domiR = e2 / sc->e1;
-      coef = domiR / (1.0 - domiR);
-    }
-    sc->coef = coef;
-    sc->e2 = e2;
-    sc->e1 = e2;
-    sc->fer = acc[1];
-    sc->flags = extrap ? 1 : 0;
-  }
+  block_partial<2, 2u>(acc, partout);
 }
 
+// e2 = l2norm(errn) and the extrapolation (fiss_extrp, :308-335) when
+// MOD(p, nac) == 0; partials of Integrate(fs0) and of RelE
 __global__ void __launch_bounds__(NT) k_post(Geo q, XS xs, Scal *sc, double *fs0, const double *fs0c,
-                                             const double *errn, double *part, unsigned *cnt) {
+                                             const double *errn, const double *pin, double *partout) {
 #pragma clang fp contract(off)
   if (sc->done) return;
-  const int extrap = sc->flags & 1;
-  const double coef = sc->coef;
+  double red[2];
+  reduce_partials<2, 2u>(pin, gridDim.x, red);
+  const double e2 = sqrt(red[0]);
+  const int extrap = ((sc->p + 1) % sc->nac) == 0;
+  double coef = 0.0;
+  if (extrap) {
+    const double domiR = e2 / sc->e1;
+    coef = domiR / (1.0 - domiR);
+  }
   double acc[2] = {0.0, 0.0};   // integral, max fission-source rel. error
   for (long c = q.c0 + (long)blockIdx.x * NT + threadIdx.x; c < q.c0 + q.M; c += (long)gridDim.x * NT) {
     double fs = fs0[c];
@@ -362,11 +379,25 @@ __global__ void __launch_bounds__(NT) k_post(Geo q, XS xs, Scal *sc, double *fs0
     acc[0] += xs.vdel[c] * fs;
     if (fabs(fs) > 1.e-10) acc[1] = fmax(acc[1], fabs(fs - fs0c[c]) / fabs(fs));
   }
-  if (reduce_last<2, 2u>(acc, part, cnt)) {
+  block_partial<2, 2u>(acc, partout);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    sc->e2 = e2;
+    sc->fer = red[1];
+    sc->flags = extrap;
+  }
+}
+
+// one block: k = k * f / fc (:484-485), ser, e1 = e2
+__global__ void __launch_bounds__(NT) k_fin_outer(Scal *sc, const double *pin, int nblk) {
+  if (sc->done) return;
+  double red[2];
+  reduce_partials<2, 2u>(pin, nblk, red);
+  if (threadIdx.x == 0) {
     const double fc = sc->f;
-    sc->f = acc[0];
-    sc->Ke = sc->Ke * acc[0] / fc;
-    sc->ser = acc[1];
+    sc->f = red[0];
+    sc->Ke = sc->Ke * red[0] / fc;
+    sc->ser = red[1];
+    sc->e1 = sc->e2;
   }
 }
 
@@ -400,9 +431,8 @@ __global__ void k_end(Scal *sc, double *tk, double *tser, double *tfer, double *
   if (sc->ser < sc->serc && sc->fer < sc->ferc && sc->ndmax < 1.e-2) sc->done = 1;
 }
 
-// PowDis (src/mod_cmfd.f90:1290-1333)
-__global__ void __launch_bounds__(NT) k_powdis(Geo q, XS xs, Scal *sc, const double *f0, double *pw, double *part,
-                                               unsigned *cnt) {
+// PowDis (src/mod_cmfd.f90:1290-1333): node powers, then normalisation
+__global__ void __launch_bounds__(NT) k_powdis(Geo q, XS xs, const double *f0, double *pw, double *partout) {
 #pragma clang fp contract(off)
   const long NC = q.NC;
   double acc[1] = {0.0};
@@ -416,11 +446,14 @@ __global__ void __launch_bounds__(NT) k_powdis(Geo q, XS xs, Scal *sc, const dou
     pw[c] = pc;
     acc[0] += pc;
   }
-  if (reduce_last<1, 0u>(acc, part, cnt)) sc->tpow = acc[0];
+  block_partial<1, 0u>(acc, partout);
 }
 
-__global__ void __launch_bounds__(NT) k_scale(Geo q, const Scal *sc, double *pw) {
-  const double tp = sc->tpow;
+__global__ void __launch_bounds__(NT) k_scale(Geo q, Scal *sc, double *pw, const double *pin) {
+  double red[1];
+  reduce_partials<1, 0u>(pin, gridDim.x, red);
+  const double tp = red[0];
+  if (blockIdx.x == 0 && threadIdx.x == 0) sc->tpow = tp;
   for (long c = q.c0 + (long)blockIdx.x * NT + threadIdx.x; c < q.c0 + q.M; c += (long)gridDim.x * NT)
     pw[c] = pw[c] / tp;
 }

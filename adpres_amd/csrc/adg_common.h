@@ -16,7 +16,7 @@ namespace adg {
 
 constexpr int NT = 256;        // threads per block of the flat (stencil) kernels
 constexpr int MAXG = 8;        // largest group count with a compiled nodal kernel
-constexpr int MAXBLK = 2048;   // largest grid of the flat kernels (partials buffer)
+constexpr int MAXBLK = 1024;   // largest grid of the flat kernels (partials per buffer)
 
 // per-cell flag bits: bit0 active node; bit (1+s) neighbour present across
 // face s, faces s = 0..5 = x+, x-, y+, y-, z+, z- (the reference's 1..6)
@@ -33,11 +33,12 @@ struct Geo {
   const unsigned char *flag;
 };
 
-// Device scalars of one context.  Written by the single "last block" of a
-// reduction or by the end-of-outer kernel, read by every later kernel.
+// Device scalars of one context.  Written by block 0 of the kernel that
+// derives them (or by a one-block kernel) and read by later kernels only.
 struct Scal {
-  // BiCGSTAB (src/mod_cmfd.f90:1213-1241)
-  double rho, rho_prev, alpha, omega, beta;
+  // BiCGSTAB (src/mod_cmfd.f90:1213-1241): values of iteration i in slot i & 1,
+  // written by block 0 of the kernel that derives them and read by later kernels
+  double rho_s[2], alpha_s[2], omega_s[2];
   // outer iteration (src/mod_cmfd.f90:456-496)
   double Ke, f, e1, e2, ser, fer, ndmax, coef;
   double tpow;
@@ -60,9 +61,11 @@ struct XS {
 };
 
 // ---------------------------------------------------------------------------
-// reductions: wavefront butterfly (DPP/permute shuffles), 4-wave block
-// combine through LDS, per-block partials and a last-block final pass in a
-// fixed order, so every reduction is deterministic for a given grid size.
+// reductions.  A reducing kernel only writes one partial per block (no
+// fences, no atomics); every block of the kernel that consumes the result
+// re-reduces all partials in the same fixed order, so each block derives
+// bit-identical scalars and the result is deterministic for a given grid.
+// Kernel boundaries order the partial stores before their readers.
 // ---------------------------------------------------------------------------
 
 template <int NV, unsigned MAXMASK>
@@ -77,65 +80,49 @@ __device__ __forceinline__ void wave_reduce(double (&v)[NV]) {
   }
 }
 
-__device__ __forceinline__ double load_agent(const double *p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Returns true in thread 0 of the last block to finish, with v[] holding the
-// grid-wide result.  Must be reached by every thread of every block.
+// block-wide reduction; the result is returned in every thread
 template <int NV, unsigned MAXMASK>
-__device__ bool reduce_last(double (&v)[NV], double *part, unsigned *counter) {
+__device__ __forceinline__ void block_reduce(double (&v)[NV]) {
   __shared__ double sm[(NT / 64) * NV];
-  __shared__ int last;
   wave_reduce<NV, MAXMASK>(v);
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  __syncthreads();
   if (lane == 0) {
 #pragma unroll
     for (int k = 0; k < NV; ++k) sm[w * NV + k] = v[k];
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int k = 0; k < NV; ++k) {
-      double a = sm[k];
-      for (int ww = 1; ww < NT / 64; ++ww)
-        a = ((MAXMASK >> k) & 1u) ? fmax(a, sm[ww * NV + k]) : a + sm[ww * NV + k];
-      part[(size_t)blockIdx.x * NV + k] = a;
-    }
-    __threadfence();
-    unsigned t = atomicAdd(counter, 1u);
-    last = (t == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (!last) return false;
-  __threadfence();
-  double acc[NV];
-#pragma unroll
-  for (int k = 0; k < NV; ++k) acc[k] = 0.0;
-  for (unsigned b = threadIdx.x; b < gridDim.x; b += NT) {
-#pragma unroll
-    for (int k = 0; k < NV; ++k) {
-      double x = load_agent(part + (size_t)b * NV + k);
-      acc[k] = ((MAXMASK >> k) & 1u) ? fmax(acc[k], x) : acc[k] + x;
-    }
-  }
-  wave_reduce<NV, MAXMASK>(acc);
-  __syncthreads();
-  if (lane == 0) {
-#pragma unroll
-    for (int k = 0; k < NV; ++k) sm[w * NV + k] = acc[k];
-  }
-  __syncthreads();
-  if (threadIdx.x != 0) return false;
 #pragma unroll
   for (int k = 0; k < NV; ++k) {
     double a = sm[k];
-    for (int ww = 1; ww < NT / 64; ++ww)
-      a = ((MAXMASK >> k) & 1u) ? fmax(a, sm[ww * NV + k]) : a + sm[ww * NV + k];
+    for (int ww = 1; ww < NT / 64; ++ww) a = ((MAXMASK >> k) & 1u) ? fmax(a, sm[ww * NV + k]) : a + sm[ww * NV + k];
     v[k] = a;
   }
-  *counter = 0u;
-  return true;
+}
+
+// this block's contribution -> part[blockIdx.x * NV + k]
+template <int NV, unsigned MAXMASK>
+__device__ __forceinline__ void block_partial(double (&v)[NV], double *part) {
+  block_reduce<NV, MAXMASK>(v);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) part[(size_t)blockIdx.x * NV + k] = v[k];
+  }
+}
+
+// grid-wide result of the nblk partials written by an earlier kernel
+template <int NV, unsigned MAXMASK>
+__device__ __forceinline__ void reduce_partials(const double *part, int nblk, double (&out)[NV]) {
+#pragma unroll
+  for (int k = 0; k < NV; ++k) out[k] = 0.0;
+  for (int b = threadIdx.x; b < nblk; b += NT) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+      const double x = part[(size_t)b * NV + k];
+      out[k] = ((MAXMASK >> k) & 1u) ? fmax(out[k], x) : out[k] + x;
+    }
+  }
+  block_reduce<NV, MAXMASK>(out);
 }
 
 }  // namespace adg

@@ -174,52 +174,59 @@ void launch_matrix(adg_ctx *c, int sched) {
   HIPCHK(hipGetLastError());
 }
 
-// one group solve: TSrc + bicg (src/mod_cmfd.f90:472-478)
-void group_solve(adg_ctx *c, int g, int adjoint, int first_group, int nin) {
+// bicg (src/mod_cmfd.f90:1203-1243) of group g: b in c->b, x = flux of g.
+// Partials rotate through three buffers: setup -> P0, K1 P0 -> P1,
+// K2 P1 -> P2, K3 P2 -> P0.
+void launch_bicg(adg_ctx *c, int g, int nin) {
   Geo q = c->geo();
   const long NC = c->NC;
   double *x = c->f0 + (size_t)g * NC;
-  hipLaunchKernelGGL(k_src, dim3(c->nblk), dim3(NT), 0, c->s, q, c->xs(), c->sc, g, adjoint, first_group, c->f0, c->f0c,
-                     c->fs0, c->fs0c, c->b);
+  double *P0 = c->part, *P1 = c->part + 2 * MAXBLK, *P2 = c->part + 4 * MAXBLK;
   const double *A64g = c->A64 + (size_t)g * 7 * NC;
-  const int sample = nin > 1 ? 1 : 0;   // steady-state K1 launch that is timed
-  const int eg0 = prof_event(c);
+  const int sample = nin > 1 ? 2 : 1;   // steady-state K1 launch that is timed
   int ek0 = -1, ek1 = -1;
   if (c->precision == ADG_FP32_REFINE) {
     const float *A = c->A32 + (size_t)g * 7 * NC;
     hipLaunchKernelGGL(k_bicg_setup<float>, dim3(c->nblk), dim3(NT), 0, c->s, q, c->sc, A64g, c->b, x, c->r32, c->rs32,
-                       c->d32, c->part, c->cnt);
-    for (int it = 0; it < nin; ++it) {
+                       c->d32, P0);
+    for (int it = 1; it <= nin; ++it) {
       const int cur = it & 1, nxt = cur ^ 1;
       if (it == sample) ek0 = prof_event(c);
       hipLaunchKernelGGL(k_bicg1<float>, dim3(c->nblk), dim3(NT), 0, c->s, q, c->sc, A, c->r32, c->rs32, c->pp32[cur],
-                         c->vv32[cur], c->pp32[nxt], c->vv32[nxt], it == 0 ? 1 : 0, c->part, c->cnt + 1);
+                         c->vv32[cur], c->pp32[nxt], c->vv32[nxt], it, P0, P1);
       if (it == sample) ek1 = prof_event(c);
       hipLaunchKernelGGL(k_bicg2<float>, dim3(c->nblk), dim3(NT), 0, c->s, q, c->sc, A, c->r32, c->vv32[nxt], c->t32,
-                         c->part, c->cnt + 2);
+                         it, P1, P2);
       hipLaunchKernelGGL(k_bicg3<float>, dim3(c->nblk), dim3(NT), 0, c->s, q, c->sc, c->pp32[nxt], c->vv32[nxt], c->t32,
-                         c->r32, c->rs32, c->d32, x, it == nin - 1 ? 1 : 0, c->part, c->cnt + 3);
+                         c->r32, c->rs32, c->d32, x, it, it == nin ? 1 : 0, P2, P0);
     }
   } else {
-    const double *A = A64g;
     hipLaunchKernelGGL(k_bicg_setup<double>, dim3(c->nblk), dim3(NT), 0, c->s, q, c->sc, A64g, c->b, x, c->r64, c->rs64,
-                       (double *)nullptr, c->part, c->cnt);
-    for (int it = 0; it < nin; ++it) {
+                       (double *)nullptr, P0);
+    for (int it = 1; it <= nin; ++it) {
       const int cur = it & 1, nxt = cur ^ 1;
       if (it == sample) ek0 = prof_event(c);
-      hipLaunchKernelGGL(k_bicg1<double>, dim3(c->nblk), dim3(NT), 0, c->s, q, c->sc, A, c->r64, c->rs64, c->pp64[cur],
-                         c->vv64[cur], c->pp64[nxt], c->vv64[nxt], it == 0 ? 1 : 0, c->part, c->cnt + 1);
+      hipLaunchKernelGGL(k_bicg1<double>, dim3(c->nblk), dim3(NT), 0, c->s, q, c->sc, A64g, c->r64, c->rs64,
+                         c->pp64[cur], c->vv64[cur], c->pp64[nxt], c->vv64[nxt], it, P0, P1);
       if (it == sample) ek1 = prof_event(c);
-      hipLaunchKernelGGL(k_bicg2<double>, dim3(c->nblk), dim3(NT), 0, c->s, q, c->sc, A, c->r64, c->vv64[nxt], c->t64,
-                         c->part, c->cnt + 2);
+      hipLaunchKernelGGL(k_bicg2<double>, dim3(c->nblk), dim3(NT), 0, c->s, q, c->sc, A64g, c->r64, c->vv64[nxt],
+                         c->t64, it, P1, P2);
       hipLaunchKernelGGL(k_bicg3<double>, dim3(c->nblk), dim3(NT), 0, c->s, q, c->sc, c->pp64[nxt], c->vv64[nxt],
-                         c->t64, c->r64, c->rs64, x, (double *)nullptr, 0, c->part, c->cnt + 3);
+                         c->t64, c->r64, c->rs64, x, (double *)nullptr, it, 0, P2, P0);
     }
   }
   HIPCHK(hipGetLastError());
-  const int eg1 = prof_event(c);
   prof_pair(c, 0, ek0, ek1);
-  prof_pair(c, 1, eg0, eg1);
+}
+
+// one group solve: TSrc + bicg (src/mod_cmfd.f90:472-478)
+void group_solve(adg_ctx *c, int g, int adjoint, int first_group, int nin) {
+  Geo q = c->geo();
+  const int eg0 = prof_event(c);
+  hipLaunchKernelGGL(k_src, dim3(c->nblk), dim3(NT), 0, c->s, q, c->xs(), c->sc, g, adjoint, first_group, c->f0, c->f0c,
+                     c->fs0, c->fs0c, c->b);
+  launch_bicg(c, g, nin);
+  prof_pair(c, 1, eg0, prof_event(c));
 }
 
 // nodal::nodal_update(_pnm) + matrix rebuild (src/mod_cmfd.f90:339-383)
@@ -293,10 +300,11 @@ void enqueue_iteration(adg_ctx *c, int adjoint, int nin, int kern, int p_host, i
     const int g = adjoint ? c->G - 1 - gg : gg;
     group_solve(c, g, adjoint, gg == 0 ? 1 : 0, nin);
   }
+  double *P1 = c->part + 2 * MAXBLK, *P2 = c->part + 4 * MAXBLK;
   hipLaunchKernelGGL(k_fsrc, dim3(c->nblk), dim3(NT), 0, c->s, q, x, c->sc, adjoint, c->f0, c->f0c, c->fs0, c->fs0c,
-                     c->errn, c->part, c->cnt + 6);
-  hipLaunchKernelGGL(k_post, dim3(c->nblk), dim3(NT), 0, c->s, q, x, c->sc, c->fs0, c->fs0c, c->errn, c->part,
-                     c->cnt + 7);
+                     c->errn, P1);
+  hipLaunchKernelGGL(k_post, dim3(c->nblk), dim3(NT), 0, c->s, q, x, c->sc, c->fs0, c->fs0c, c->errn, P1, P2);
+  hipLaunchKernelGGL(k_fin_outer, dim3(1), dim3(NT), 0, c->s, c->sc, P2, c->nblk);
   if (kern != ADG_KERN_FDM) {
     const bool timed = p_host > 0 && (p_host % nupd) == 0;
     const int en0 = timed ? prof_event(c) : -1;
@@ -413,7 +421,7 @@ int adg_create(const adg_geom *g, adg_ctx **out) {
                           (long)(c->nzz + 1) * c->nxx * c->nyy});
     c->npart_blocks = (int)((maxf + NT - 1) / NT);
     c->npart = dalloc<double>((size_t)3 * 3 * c->npart_blocks);
-    c->part = dalloc<double>((size_t)MAXBLK * 4);
+    c->part = dalloc<double>((size_t)MAXBLK * 6);   // three rotating buffers of 2 x MAXBLK
     c->cnt = dalloc<unsigned>(16);
     c->sc = dalloc<Scal>(1);
     HIPCHK(hipHostMalloc((void **)&c->hsc, sizeof(Scal), hipHostMallocDefault));
@@ -577,7 +585,8 @@ int adg_outer(adg_ctx *c, int mode, const adg_iter *it, adg_trace *tr) {
       hipLaunchKernelGGL(k_fsrc_init, dim3(c->nblk), dim3(NT), 0, c->s, q, x, adjoint, c->f0, c->fs0);
       c->first = false;
     }
-    hipLaunchKernelGGL(k_integrals, dim3(c->nblk), dim3(NT), 0, c->s, q, x, c->sc, c->fs0, c->part, c->cnt + 5);
+    hipLaunchKernelGGL(k_integrals, dim3(c->nblk), dim3(NT), 0, c->s, q, x, c->fs0, c->part);
+    hipLaunchKernelGGL(k_fin_integrals, dim3(1), dim3(NT), 0, c->s, c->sc, c->part, c->nblk);
     HIPCHK(hipGetLastError());
     // replay: a captured graph of one outer (or direct launches when
     // profiling, whose events need the host-side sequence); the done flag
@@ -681,12 +690,12 @@ int adg_powdis(adg_ctx *c, double *p) {
   if (!c || !p) return ADG_E_ARG;
   try {
     Geo q = c->geo();
-    hipLaunchKernelGGL(k_powdis, dim3(c->nblk), dim3(NT), 0, c->s, q, c->xs(), c->sc, c->f0, c->pw, c->part, c->cnt + 8);
+    hipLaunchKernelGGL(k_powdis, dim3(c->nblk), dim3(NT), 0, c->s, q, c->xs(), c->f0, c->pw, c->part);
+    hipLaunchKernelGGL(k_scale, dim3(c->nblk), dim3(NT), 0, c->s, q, c->sc, c->pw, c->part);
+    HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(c->hsc, c->sc, sizeof(Scal), hipMemcpyDeviceToHost, c->s));
     HIPCHK(hipStreamSynchronize(c->s));
     if (!(c->hsc->tpow > 0.0)) return fail(c, ADG_E_POWER_NONPOS, "total node power is zero or less");
-    hipLaunchKernelGGL(k_scale, dim3(c->nblk), dim3(NT), 0, c->s, q, c->sc, c->pw);
-    HIPCHK(hipGetLastError());
     std::vector<double> st;
     gather(c, c->pw, 1, p, st);
   } catch (const HipError &e) {
@@ -733,35 +742,7 @@ int adg_bicg(adg_ctx *c, int g, int nin, const double *b, double *x) {
     gather(c, c->f0, c->G, full.data(), st);
     memcpy(full.data() + (size_t)g * c->N, x, c->N * sizeof(double));
     scatter(c, full.data(), c->G, c->f0, st);
-    const long NC = c->NC;
-    double *xg = c->f0 + (size_t)g * NC;
-    const double *A64g = c->A64 + (size_t)g * 7 * NC;
-    if (c->precision == ADG_FP32_REFINE) {
-      const float *A = c->A32 + (size_t)g * 7 * NC;
-      hipLaunchKernelGGL(k_bicg_setup<float>, dim3(c->nblk), dim3(NT), 0, c->s, q, c->sc, A64g, c->b, xg, c->r32,
-                         c->rs32, c->d32, c->part, c->cnt);
-      for (int it = 0; it < nin; ++it) {
-        const int cur = it & 1, nxt = cur ^ 1;
-        hipLaunchKernelGGL(k_bicg1<float>, dim3(c->nblk), dim3(NT), 0, c->s, q, c->sc, A, c->r32, c->rs32, c->pp32[cur],
-                           c->vv32[cur], c->pp32[nxt], c->vv32[nxt], it == 0 ? 1 : 0, c->part, c->cnt + 1);
-        hipLaunchKernelGGL(k_bicg2<float>, dim3(c->nblk), dim3(NT), 0, c->s, q, c->sc, A, c->r32, c->vv32[nxt], c->t32,
-                           c->part, c->cnt + 2);
-        hipLaunchKernelGGL(k_bicg3<float>, dim3(c->nblk), dim3(NT), 0, c->s, q, c->sc, c->pp32[nxt], c->vv32[nxt],
-                           c->t32, c->r32, c->rs32, c->d32, xg, it == nin - 1 ? 1 : 0, c->part, c->cnt + 3);
-      }
-    } else {
-      hipLaunchKernelGGL(k_bicg_setup<double>, dim3(c->nblk), dim3(NT), 0, c->s, q, c->sc, A64g, c->b, xg, c->r64,
-                         c->rs64, (double *)nullptr, c->part, c->cnt);
-      for (int it = 0; it < nin; ++it) {
-        const int cur = it & 1, nxt = cur ^ 1;
-        hipLaunchKernelGGL(k_bicg1<double>, dim3(c->nblk), dim3(NT), 0, c->s, q, c->sc, A64g, c->r64, c->rs64,
-                           c->pp64[cur], c->vv64[cur], c->pp64[nxt], c->vv64[nxt], it == 0 ? 1 : 0, c->part, c->cnt + 1);
-        hipLaunchKernelGGL(k_bicg2<double>, dim3(c->nblk), dim3(NT), 0, c->s, q, c->sc, A64g, c->r64, c->vv64[nxt],
-                           c->t64, c->part, c->cnt + 2);
-        hipLaunchKernelGGL(k_bicg3<double>, dim3(c->nblk), dim3(NT), 0, c->s, q, c->sc, c->pp64[nxt], c->vv64[nxt],
-                           c->t64, c->r64, c->rs64, xg, (double *)nullptr, 0, c->part, c->cnt + 3);
-      }
-    }
+    launch_bicg(c, g, nin);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(c->s));
     gather(c, c->f0, c->G, full.data(), st);
