@@ -120,7 +120,8 @@ __device__ __forceinline__ double stencil(const TA *__restrict__ A, long NC, con
 }
 
 __global__ void __launch_bounds__(NT) k_matvec(Geo q, const double *A64, int g, const double *x, double *y) {
-  for (long c = q.c0 + (long)blockIdx.x * NT + threadIdx.x; c < q.c0 + q.M; c += (long)gridDim.x * NT)
+  const Range rg = block_range(q);
+  for (long c = rg.lo + threadIdx.x; c < rg.hi; c += NT)
     y[c] = stencil(A64 + (size_t)g * 7 * q.NC, q.NC, x, c, q.sy, q.sz);
 }
 
@@ -132,11 +133,12 @@ __global__ void __launch_bounds__(NT) k_src(Geo q, XS xs, const Scal *sc, int g,
                                             const double *f0, double *f0c, const double *fs0, double *fs0c,
                                             double *b) {
 #pragma clang fp contract(off)
+  const Range rg = block_range(q);
   if (sc->done) return;
   const double Ke = sc->Ke;
   const long NC = q.NC;
   const int G = q.G;
-  for (long c = q.c0 + (long)blockIdx.x * NT + threadIdx.x; c < q.c0 + q.M; c += (long)gridDim.x * NT) {
+  for (long c = rg.lo + threadIdx.x; c < rg.hi; c += NT) {
     if (first_group) fs0c[c] = fs0[c];
     f0c[(size_t)g * NC + c] = f0[(size_t)g * NC + c];
     double s0 = 0.0;
@@ -169,9 +171,10 @@ template <typename TV>
 __global__ void __launch_bounds__(NT) k_bicg_setup(Geo q, Scal *sc, const double *A64g, const double *b,
                                                    const double *x, TV *r, TV *rs, TV *d, double *pout) {
 #pragma clang fp contract(off)
+  const Range rg = block_range(q);
   if (sc->done) return;
   double acc[1] = {0.0};
-  for (long c = q.c0 + (long)blockIdx.x * NT + threadIdx.x; c < q.c0 + q.M; c += (long)gridDim.x * NT) {
+  for (long c = rg.lo + threadIdx.x; c < rg.hi; c += NT) {
     const double rr = b[c] - stencil(A64g, q.NC, x, c, q.sy, q.sz);
     const TV rv = (TV)rr;
     r[c] = rv;
@@ -187,64 +190,100 @@ __global__ void __launch_bounds__(NT) k_bicg_setup(Geo q, Scal *sc, const double
   }
 }
 
+// BiCGSTAB in two stencil kernels per iteration.  With the reference's
+// recurrences (src/mod_cmfd.f90:1213-1241) for iteration i,
+//   KB(i): s = r - alpha_i v at c and its six neighbours, t = A s;
+//          dots (t,t), (t,s), (rs,t), (rs,s)
+//   KA(i+1): omega_i = (t,s)/(t,t),  rho_{i+1} = (rs,s) - omega_i (rs,t)
+//          [= (rs, r_i) with r_i = s - omega_i t],
+//          beta = (rho_{i+1}/rho_i)(alpha_i/omega_i); at c and its neighbours
+//          r_i = s - omega_i t and p_{i+1} = r_i + beta (p_i - omega_i v_i);
+//          x += alpha_i p_i + omega_i s at c; v_{i+1} = A p_{i+1}; dot (rs, v)
+//   KB(i+1): alpha_{i+1} = rho_{i+1} / (rs, v_{i+1}), ...
+//   KT:    the last iteration's x update.
+// In exact arithmetic these are the reference's iterates; the K3 update of
+// the three-kernel form is recomputed where the next stencil reads it.  r, p
+// and v ping-pong between two buffers because neighbours read old values.
 template <typename TV>
-__device__ __forceinline__ TV pdir(const TV *r, const TV *p, const TV *v, long c, TV beta, TV omega, int first) {
+__device__ __forceinline__ TV bicg_pnext(const TV *r, const TV *v, const TV *t, const TV *p, long c, TV alpha,
+                                         TV omega, TV beta, int first, TV &rnew) {
 #pragma clang fp contract(off)
-  return first ? r[c] : (TV)(r[c] + beta * (p[c] - omega * v[c]));
+  if (first) {
+    rnew = r[c];
+    return r[c];
+  }
+  const TV s = (TV)(r[c] - alpha * v[c]);
+  rnew = (TV)(s - omega * t[c]);
+  return (TV)(rnew + beta * (p[c] - omega * v[c]));
 }
 
-// iteration it (1-based): rho_it from the partials of the previous kernel
-// (setup or K3 of it-1), beta = (rho_it / rho_{it-1}) (alpha_{it-1} / omega_{it-1})
 template <typename TV>
-__global__ void __launch_bounds__(NT) k_bicg1(Geo q, Scal *sc, const TV *AT, const TV *r, const TV *rs,
-                                              const TV *p, const TV *v, TV *pout, TV *vout, int it,
-                                              const double *pin, double *partout) {
+__global__ void __launch_bounds__(NT) k_bicgA(Geo q, Scal *sc, const TV *AT, int it, const TV *r, const TV *p,
+                                              const TV *v, const TV *t, const TV *rs, TV *x, TV *rout, TV *pout,
+                                              TV *vout, const double *pin, double *partout) {
 #pragma clang fp contract(off)
   if (sc->done) return;
-  double red[1];
-  reduce_partials<1, 0u>(pin, gridDim.x, red);
-  const int cur = it & 1, prv = cur ^ 1;
-  const double rho = red[0];
-  const double beta = (rho / sc->rho_s[prv]) * (sc->alpha_s[prv] / sc->omega_s[prv]);
-  const TV tb = (TV)beta, tw = (TV)sc->omega_s[prv];
+  const Range rg = block_range(q);
   const int first = it == 1;
+  const int prv = (it - 1) & 1, cur = it & 1;
+  double rho, alpha_d = 0.0, omega_d = 0.0, beta = 0.0;
+  if (first) {
+    double red[1];
+    reduce_partials<1, 0u>(pin, gridDim.x, red);
+    rho = red[0];
+  } else {
+    double red[4];   // (t,t), (t,s), (rs,t), (rs,s) of iteration it-1
+    reduce_partials<4, 0u>(pin, gridDim.x, red);
+    omega_d = red[1] / red[0];
+    rho = red[3] - omega_d * red[2];
+    alpha_d = sc->alpha_s[prv];
+    beta = (rho / sc->rho_s[prv]) * (alpha_d / omega_d);
+  }
+  const TV ta = (TV)alpha_d, tw = (TV)omega_d, tb = (TV)beta;
   const long NC = q.NC, sy = q.sy, sz = q.sz;
   double acc[1] = {0.0};
-  for (long c = q.c0 + (long)blockIdx.x * NT + threadIdx.x; c < q.c0 + q.M; c += (long)gridDim.x * NT) {
-    const TV pc = pdir(r, p, v, c, tb, tw, first);
+  for (long c = rg.lo + threadIdx.x; c < rg.hi; c += NT) {
+    TV rn, dummy;
+    const TV pc = bicg_pnext(r, v, t, p, c, ta, tw, tb, first, rn);
     double s = 0.0;
-    s = s + (double)(AT[0 * NC + c] * pdir(r, p, v, c - sz, tb, tw, first));
-    s = s + (double)(AT[1 * NC + c] * pdir(r, p, v, c - sy, tb, tw, first));
-    s = s + (double)(AT[2 * NC + c] * pdir(r, p, v, c - 1, tb, tw, first));
+    s = s + (double)(AT[0 * NC + c] * bicg_pnext(r, v, t, p, c - sz, ta, tw, tb, first, dummy));
+    s = s + (double)(AT[1 * NC + c] * bicg_pnext(r, v, t, p, c - sy, ta, tw, tb, first, dummy));
+    s = s + (double)(AT[2 * NC + c] * bicg_pnext(r, v, t, p, c - 1, ta, tw, tb, first, dummy));
     s = s + (double)(AT[3 * NC + c] * pc);
-    s = s + (double)(AT[4 * NC + c] * pdir(r, p, v, c + 1, tb, tw, first));
-    s = s + (double)(AT[5 * NC + c] * pdir(r, p, v, c + sy, tb, tw, first));
-    s = s + (double)(AT[6 * NC + c] * pdir(r, p, v, c + sz, tb, tw, first));
+    s = s + (double)(AT[4 * NC + c] * bicg_pnext(r, v, t, p, c + 1, ta, tw, tb, first, dummy));
+    s = s + (double)(AT[5 * NC + c] * bicg_pnext(r, v, t, p, c + sy, ta, tw, tb, first, dummy));
+    s = s + (double)(AT[6 * NC + c] * bicg_pnext(r, v, t, p, c + sz, ta, tw, tb, first, dummy));
     const TV vc = (TV)s;
-    // neighbours read the previous p and v, so the new ones go to the other
-    // buffer of a ping-pong pair
+    if (!first) {
+      const TV sv = (TV)(r[c] - ta * v[c]);
+      x[c] = (TV)(x[c] + ta * p[c] + tw * sv);
+    }
+    rout[c] = rn;
     pout[c] = pc;
     vout[c] = vc;
     acc[0] += (double)rs[c] * (double)vc;
   }
   block_partial<1, 0u>(acc, partout);
-  if (blockIdx.x == 0 && threadIdx.x == 0) sc->rho_s[cur] = rho;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    sc->rho_s[cur] = rho;
+    if (!first) sc->omega_s[prv] = omega_d;
+  }
 }
 
-// alpha_it = rho_it / (rs, v)
 template <typename TV>
-__global__ void __launch_bounds__(NT) k_bicg2(Geo q, Scal *sc, const TV *AT, const TV *r, const TV *v, TV *t,
-                                              int it, const double *pin, double *partout) {
+__global__ void __launch_bounds__(NT) k_bicgB(Geo q, Scal *sc, const TV *AT, int it, const TV *r, const TV *v,
+                                              TV *t, const TV *rs, const double *pin, double *partout) {
 #pragma clang fp contract(off)
   if (sc->done) return;
+  const Range rg = block_range(q);
   double red[1];
   reduce_partials<1, 0u>(pin, gridDim.x, red);
   const int cur = it & 1;
   const double alpha_d = sc->rho_s[cur] / red[0];
   const TV alpha = (TV)alpha_d;
   const long NC = q.NC, sy = q.sy, sz = q.sz;
-  double acc[2] = {0.0, 0.0};
-  for (long c = q.c0 + (long)blockIdx.x * NT + threadIdx.x; c < q.c0 + q.M; c += (long)gridDim.x * NT) {
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  for (long c = rg.lo + threadIdx.x; c < rg.hi; c += NT) {
     const TV sc_ = (TV)(r[c] - alpha * v[c]);
     double s = 0.0;
     s = s + (double)(AT[0 * NC + c] * (TV)(r[c - sz] - alpha * v[c - sz]));
@@ -256,37 +295,34 @@ __global__ void __launch_bounds__(NT) k_bicg2(Geo q, Scal *sc, const TV *AT, con
     s = s + (double)(AT[6 * NC + c] * (TV)(r[c + sz] - alpha * v[c + sz]));
     const TV tc = (TV)s;
     t[c] = tc;
+    const double rsc = (double)rs[c];
     acc[0] += (double)tc * (double)tc;
     acc[1] += (double)tc * (double)sc_;
+    acc[2] += rsc * (double)tc;
+    acc[3] += rsc * (double)sc_;
   }
-  block_partial<2, 0u>(acc, partout);
+  block_partial<4, 0u>(acc, partout);
   if (blockIdx.x == 0 && threadIdx.x == 0) sc->alpha_s[cur] = alpha_d;
 }
 
-// omega_it = (t, s) / (t, t);  d += alpha p + omega s;  r = s - omega t
+// last iteration's x update: x += alpha p + omega s; for the FP32 form the
+// accumulated correction is then added to the FP64 flux
 template <typename TV>
-__global__ void __launch_bounds__(NT) k_bicg3(Geo q, Scal *sc, const TV *p, const TV *v, const TV *t, TV *r,
-                                              const TV *rs, TV *d, double *x64, int it, int last,
-                                              const double *pin, double *partout) {
+__global__ void __launch_bounds__(NT) k_bicgT(Geo q, Scal *sc, int it, const TV *r, const TV *p, const TV *v,
+                                              TV *x, double *x64, const double *pin) {
 #pragma clang fp contract(off)
   if (sc->done) return;
-  double red[2];
-  reduce_partials<2, 0u>(pin, gridDim.x, red);
-  const int cur = it & 1;
+  const Range rg = block_range(q);
+  double red[4];
+  reduce_partials<4, 0u>(pin, gridDim.x, red);
   const double omega_d = red[1] / red[0];
-  const TV alpha = (TV)sc->alpha_s[cur], omega = (TV)omega_d;
-  double acc[1] = {0.0};
-  for (long c = q.c0 + (long)blockIdx.x * NT + threadIdx.x; c < q.c0 + q.M; c += (long)gridDim.x * NT) {
-    const TV s = (TV)(r[c] - alpha * v[c]);
-    const TV dn = (TV)(d[c] + alpha * p[c] + omega * s);
-    d[c] = dn;
-    if (last && x64) x64[c] = x64[c] + (double)dn;
-    const TV rn = (TV)(s - omega * t[c]);
-    r[c] = rn;
-    acc[0] += (double)rs[c] * (double)rn;
+  const TV ta = (TV)sc->alpha_s[it & 1], tw = (TV)omega_d;
+  for (long c = rg.lo + threadIdx.x; c < rg.hi; c += NT) {
+    const TV sv = (TV)(r[c] - ta * v[c]);
+    const TV xn = (TV)(x[c] + ta * p[c] + tw * sv);
+    if (x64) x64[c] = x64[c] + (double)xn;
+    else x[c] = xn;
   }
-  block_partial<1, 0u>(acc, partout);
-  if (blockIdx.x == 0 && threadIdx.x == 0) sc->omega_s[cur] = omega_d;
 }
 
 // ---------------------------------------------------------------------------
@@ -298,8 +334,9 @@ __global__ void __launch_bounds__(NT) k_bicg3(Geo q, Scal *sc, const TV *p, cons
 
 __global__ void __launch_bounds__(NT) k_fsrc_init(Geo q, XS xs, int adjoint, const double *f0, double *fs0) {
 #pragma clang fp contract(off)
+  const Range rg = block_range(q);
   const long NC = q.NC;
-  for (long c = q.c0 + (long)blockIdx.x * NT + threadIdx.x; c < q.c0 + q.M; c += (long)gridDim.x * NT) {
+  for (long c = rg.lo + threadIdx.x; c < rg.hi; c += NT) {
     double fs = 0.0;
     for (int g = 0; g < q.G; ++g) {
       const double w = adjoint ? xs.chi[(size_t)g * NC + c] : xs.nuf[(size_t)g * NC + c];
@@ -313,8 +350,9 @@ __global__ void __launch_bounds__(NT) k_fsrc_init(Geo q, XS xs, int adjoint, con
 // partials here, finished by k_fin_integrals
 __global__ void __launch_bounds__(NT) k_integrals(Geo q, XS xs, const double *fs0, double *partout) {
 #pragma clang fp contract(off)
+  const Range rg = block_range(q);
   double acc[2] = {0.0, 0.0};
-  for (long c = q.c0 + (long)blockIdx.x * NT + threadIdx.x; c < q.c0 + q.M; c += (long)gridDim.x * NT) {
+  for (long c = rg.lo + threadIdx.x; c < rg.hi; c += NT) {
     acc[0] += xs.vdel[c] * fs0[c];
     acc[1] += xs.vdel[c];
   }
@@ -335,10 +373,11 @@ __global__ void __launch_bounds__(NT) k_fsrc(Geo q, XS xs, Scal *sc, int adjoint
                                              const double *f0c, double *fs0, const double *fs0c, double *errn,
                                              double *partout) {
 #pragma clang fp contract(off)
+  const Range rg = block_range(q);
   if (sc->done) return;
   const long NC = q.NC;
   double acc[2] = {0.0, 0.0};   // sum errn^2, max flux rel. error
-  for (long c = q.c0 + (long)blockIdx.x * NT + threadIdx.x; c < q.c0 + q.M; c += (long)gridDim.x * NT) {
+  for (long c = rg.lo + threadIdx.x; c < rg.hi; c += NT) {
     double fs = 0.0;
     for (int g = 0; g < q.G; ++g) {
       const double w = adjoint ? xs.chi[(size_t)g * NC + c] : xs.nuf[(size_t)g * NC + c];
@@ -359,6 +398,7 @@ __global__ void __launch_bounds__(NT) k_fsrc(Geo q, XS xs, Scal *sc, int adjoint
 __global__ void __launch_bounds__(NT) k_post(Geo q, XS xs, Scal *sc, double *fs0, const double *fs0c,
                                              const double *errn, const double *pin, double *partout) {
 #pragma clang fp contract(off)
+  const Range rg = block_range(q);
   if (sc->done) return;
   double red[2];
   reduce_partials<2, 2u>(pin, gridDim.x, red);
@@ -370,7 +410,7 @@ __global__ void __launch_bounds__(NT) k_post(Geo q, XS xs, Scal *sc, double *fs0
     coef = domiR / (1.0 - domiR);
   }
   double acc[2] = {0.0, 0.0};   // integral, max fission-source rel. error
-  for (long c = q.c0 + (long)blockIdx.x * NT + threadIdx.x; c < q.c0 + q.M; c += (long)gridDim.x * NT) {
+  for (long c = rg.lo + threadIdx.x; c < rg.hi; c += NT) {
     double fs = fs0[c];
     if (extrap) {
       fs = fs + coef * errn[c];
@@ -434,9 +474,10 @@ __global__ void k_end(Scal *sc, double *tk, double *tser, double *tfer, double *
 // PowDis (src/mod_cmfd.f90:1290-1333): node powers, then normalisation
 __global__ void __launch_bounds__(NT) k_powdis(Geo q, XS xs, const double *f0, double *pw, double *partout) {
 #pragma clang fp contract(off)
+  const Range rg = block_range(q);
   const long NC = q.NC;
   double acc[1] = {0.0};
-  for (long c = q.c0 + (long)blockIdx.x * NT + threadIdx.x; c < q.c0 + q.M; c += (long)gridDim.x * NT) {
+  for (long c = rg.lo + threadIdx.x; c < rg.hi; c += NT) {
     double pc = 0.0;
     for (int g = 0; g < q.G; ++g) {
       double w = f0[(size_t)g * NC + c] * xs.sigf[(size_t)g * NC + c] * xs.vdel[c];
@@ -450,11 +491,12 @@ __global__ void __launch_bounds__(NT) k_powdis(Geo q, XS xs, const double *f0, d
 }
 
 __global__ void __launch_bounds__(NT) k_scale(Geo q, Scal *sc, double *pw, const double *pin) {
+  const Range rg = block_range(q);
   double red[1];
   reduce_partials<1, 0u>(pin, gridDim.x, red);
   const double tp = red[0];
   if (blockIdx.x == 0 && threadIdx.x == 0) sc->tpow = tp;
-  for (long c = q.c0 + (long)blockIdx.x * NT + threadIdx.x; c < q.c0 + q.M; c += (long)gridDim.x * NT)
+  for (long c = rg.lo + threadIdx.x; c < rg.hi; c += NT)
     pw[c] = pw[c] / tp;
 }
 

@@ -60,6 +60,28 @@ struct XS {
   const double *D, *sigr, *nuf, *sigf, *chi, *sigs, *dc, *exsrc, *vdel;
 };
 
+// Cell range of this block in the flat kernels.  The hardware hands block b
+// to XCD b % 8; with the grid a multiple of 8, logical block
+// (b % 8) * (grid / 8) + b / 8 gives every XCD one contiguous slab of the
+// box, so the z-neighbour planes of a stencil are read through that XCD's
+// own L2.  Each logical block owns `chunk` consecutive cells (a multiple of NT).
+struct Range {
+  long lo, hi;
+};
+__device__ __forceinline__ Range block_range(const Geo &q) {
+  const int nb = gridDim.x, b = blockIdx.x;
+  const int lb = (nb & 7) ? b : (b & 7) * (nb >> 3) + (b >> 3);
+  long chunk = (q.M + nb - 1) / nb;
+  chunk = (chunk + NT - 1) / NT * NT;
+  Range r;
+  r.lo = q.c0 + (long)lb * chunk;
+  r.hi = r.lo + chunk;
+  const long end = q.c0 + q.M;
+  if (r.hi > end) r.hi = end;
+  if (r.lo > end) r.lo = end;
+  return r;
+}
+
 // ---------------------------------------------------------------------------
 // reductions.  A reducing kernel only writes one partial per block (no
 // fences, no atomics); every block of the kernel that consumes the result

@@ -58,10 +58,11 @@ struct adg_ctx {
   double *f0 = nullptr, *f0c = nullptr, *fs0 = nullptr, *fs0c = nullptr, *errn = nullptr;
   double *dn = nullptr, *df = nullptr, *A64 = nullptr, *b = nullptr, *pw = nullptr, *tmp = nullptr;
   // BiCGSTAB vectors: FP64 form
-  double *r64 = nullptr, *rs64 = nullptr, *t64 = nullptr, *pp64[2] = {nullptr, nullptr}, *vv64[2] = {nullptr, nullptr};
+  double *rs64 = nullptr, *t64 = nullptr, *rr64[2] = {nullptr, nullptr}, *pp64[2] = {nullptr, nullptr};
+  double *vv64[2] = {nullptr, nullptr};
   // FP32 form
-  float *A32 = nullptr, *r32 = nullptr, *rs32 = nullptr, *t32 = nullptr, *d32 = nullptr;
-  float *pp32[2] = {nullptr, nullptr}, *vv32[2] = {nullptr, nullptr};
+  float *A32 = nullptr, *rs32 = nullptr, *t32 = nullptr, *d32 = nullptr;
+  float *rr32[2] = {nullptr, nullptr}, *pp32[2] = {nullptr, nullptr}, *vv32[2] = {nullptr, nullptr};
   // nodal
   double *S = nullptr, *a2 = nullptr, *a4 = nullptr, *L1 = nullptr, *npart = nullptr;
   int npart_blocks = 0;
@@ -175,48 +176,45 @@ void launch_matrix(adg_ctx *c, int sched) {
 }
 
 // bicg (src/mod_cmfd.f90:1203-1243) of group g: b in c->b, x = flux of g.
-// Partials rotate through three buffers: setup -> P0, K1 P0 -> P1,
-// K2 P1 -> P2, K3 P2 -> P0.
-void launch_bicg(adg_ctx *c, int g, int nin) {
+// Kernels: setup, then KA(1) KB(1) KA(2) KB(2) ... KB(nin), KT (two stencil
+// kernels per iteration, see adg_cmfd.h).  r, p, v ping-pong: KA(it) reads
+// set (it-1)&1 and writes set it&1.  Partials: setup -> P0, KA(1) P0 -> P1,
+// KB P1 -> P2, KA(it>1) P2 -> P1, KT reads P2.
+template <typename TV>
+void launch_bicg_t(adg_ctx *c, int g, int nin, const TV *A, TV *const *R, TV *const *P, TV *const *V, TV *t,
+                   TV *rs, TV *d) {
   Geo q = c->geo();
   const long NC = c->NC;
-  double *x = c->f0 + (size_t)g * NC;
-  double *P0 = c->part, *P1 = c->part + 2 * MAXBLK, *P2 = c->part + 4 * MAXBLK;
+  double *x64 = c->f0 + (size_t)g * NC;
+  double *P0 = c->part, *P1 = c->part + 4 * MAXBLK, *P2 = c->part + 8 * MAXBLK;
   const double *A64g = c->A64 + (size_t)g * 7 * NC;
-  const int sample = nin > 1 ? 2 : 1;   // steady-state K1 launch that is timed
+  // FP64 form: the iterate is the flux itself; FP32 form: a correction d
+  TV *x = d ? d : (TV *)x64;
+  const int sample = nin > 1 ? 2 : 1;   // steady-state KA launch that is timed
   int ek0 = -1, ek1 = -1;
-  if (c->precision == ADG_FP32_REFINE) {
-    const float *A = c->A32 + (size_t)g * 7 * NC;
-    hipLaunchKernelGGL(k_bicg_setup<float>, dim3(c->nblk), dim3(NT), 0, c->s, q, c->sc, A64g, c->b, x, c->r32, c->rs32,
-                       c->d32, P0);
-    for (int it = 1; it <= nin; ++it) {
-      const int cur = it & 1, nxt = cur ^ 1;
-      if (it == sample) ek0 = prof_event(c);
-      hipLaunchKernelGGL(k_bicg1<float>, dim3(c->nblk), dim3(NT), 0, c->s, q, c->sc, A, c->r32, c->rs32, c->pp32[cur],
-                         c->vv32[cur], c->pp32[nxt], c->vv32[nxt], it, P0, P1);
-      if (it == sample) ek1 = prof_event(c);
-      hipLaunchKernelGGL(k_bicg2<float>, dim3(c->nblk), dim3(NT), 0, c->s, q, c->sc, A, c->r32, c->vv32[nxt], c->t32,
-                         it, P1, P2);
-      hipLaunchKernelGGL(k_bicg3<float>, dim3(c->nblk), dim3(NT), 0, c->s, q, c->sc, c->pp32[nxt], c->vv32[nxt], c->t32,
-                         c->r32, c->rs32, c->d32, x, it, it == nin ? 1 : 0, P2, P0);
-    }
-  } else {
-    hipLaunchKernelGGL(k_bicg_setup<double>, dim3(c->nblk), dim3(NT), 0, c->s, q, c->sc, A64g, c->b, x, c->r64, c->rs64,
-                       (double *)nullptr, P0);
-    for (int it = 1; it <= nin; ++it) {
-      const int cur = it & 1, nxt = cur ^ 1;
-      if (it == sample) ek0 = prof_event(c);
-      hipLaunchKernelGGL(k_bicg1<double>, dim3(c->nblk), dim3(NT), 0, c->s, q, c->sc, A64g, c->r64, c->rs64,
-                         c->pp64[cur], c->vv64[cur], c->pp64[nxt], c->vv64[nxt], it, P0, P1);
-      if (it == sample) ek1 = prof_event(c);
-      hipLaunchKernelGGL(k_bicg2<double>, dim3(c->nblk), dim3(NT), 0, c->s, q, c->sc, A64g, c->r64, c->vv64[nxt],
-                         c->t64, it, P1, P2);
-      hipLaunchKernelGGL(k_bicg3<double>, dim3(c->nblk), dim3(NT), 0, c->s, q, c->sc, c->pp64[nxt], c->vv64[nxt],
-                         c->t64, c->r64, c->rs64, x, (double *)nullptr, it, 0, P2, P0);
-    }
+  hipLaunchKernelGGL(k_bicg_setup<TV>, dim3(c->nblk), dim3(NT), 0, c->s, q, c->sc, A64g, c->b, x64, R[0], rs, d, P0);
+  for (int it = 1; it <= nin; ++it) {
+    const int rd = (it - 1) & 1, wr = it & 1;
+    if (it == sample) ek0 = prof_event(c);
+    hipLaunchKernelGGL(k_bicgA<TV>, dim3(c->nblk), dim3(NT), 0, c->s, q, c->sc, A, it, R[rd], P[rd], V[rd], t, rs, x,
+                       R[wr], P[wr], V[wr], it == 1 ? P0 : P2, P1);
+    if (it == sample) ek1 = prof_event(c);
+    hipLaunchKernelGGL(k_bicgB<TV>, dim3(c->nblk), dim3(NT), 0, c->s, q, c->sc, A, it, R[wr], V[wr], t, rs, P1, P2);
   }
+  const int wr = nin & 1;
+  hipLaunchKernelGGL(k_bicgT<TV>, dim3(c->nblk), dim3(NT), 0, c->s, q, c->sc, nin, R[wr], P[wr], V[wr], x,
+                     d ? x64 : (double *)nullptr, P2);
   HIPCHK(hipGetLastError());
   prof_pair(c, 0, ek0, ek1);
+}
+
+void launch_bicg(adg_ctx *c, int g, int nin) {
+  const long NC = c->NC;
+  if (c->precision == ADG_FP32_REFINE)
+    launch_bicg_t<float>(c, g, nin, c->A32 + (size_t)g * 7 * NC, c->rr32, c->pp32, c->vv32, c->t32, c->rs32, c->d32);
+  else
+    launch_bicg_t<double>(c, g, nin, c->A64 + (size_t)g * 7 * NC, c->rr64, c->pp64, c->vv64, c->t64, c->rs64,
+                          (double *)nullptr);
 }
 
 // one group solve: TSrc + bicg (src/mod_cmfd.f90:472-478)
@@ -300,7 +298,7 @@ void enqueue_iteration(adg_ctx *c, int adjoint, int nin, int kern, int p_host, i
     const int g = adjoint ? c->G - 1 - gg : gg;
     group_solve(c, g, adjoint, gg == 0 ? 1 : 0, nin);
   }
-  double *P1 = c->part + 2 * MAXBLK, *P2 = c->part + 4 * MAXBLK;
+  double *P1 = c->part + 4 * MAXBLK, *P2 = c->part + 8 * MAXBLK;
   hipLaunchKernelGGL(k_fsrc, dim3(c->nblk), dim3(NT), 0, c->s, q, x, c->sc, adjoint, c->f0, c->f0c, c->fs0, c->fs0c,
                      c->errn, P1);
   hipLaunchKernelGGL(k_post, dim3(c->nblk), dim3(NT), 0, c->s, q, x, c->sc, c->fs0, c->fs0c, c->errn, P1, P2);
@@ -409,11 +407,15 @@ int adg_create(const adg_geom *g, adg_ctx **out) {
     c->tmp = dalloc<double>(NC);
     if (c->precision == ADG_FP32_REFINE) {
       c->A32 = dalloc<float>(7 * G * NC);
-      c->r32 = dalloc<float>(NC); c->rs32 = dalloc<float>(NC); c->t32 = dalloc<float>(NC); c->d32 = dalloc<float>(NC);
-      for (int k = 0; k < 2; ++k) { c->pp32[k] = dalloc<float>(NC); c->vv32[k] = dalloc<float>(NC); }
+      c->rs32 = dalloc<float>(NC); c->t32 = dalloc<float>(NC); c->d32 = dalloc<float>(NC);
+      for (int k = 0; k < 2; ++k) {
+        c->rr32[k] = dalloc<float>(NC); c->pp32[k] = dalloc<float>(NC); c->vv32[k] = dalloc<float>(NC);
+      }
     } else {
-      c->r64 = dalloc<double>(NC); c->rs64 = dalloc<double>(NC); c->t64 = dalloc<double>(NC);
-      for (int k = 0; k < 2; ++k) { c->pp64[k] = dalloc<double>(NC); c->vv64[k] = dalloc<double>(NC); }
+      c->rs64 = dalloc<double>(NC); c->t64 = dalloc<double>(NC);
+      for (int k = 0; k < 2; ++k) {
+        c->rr64[k] = dalloc<double>(NC); c->pp64[k] = dalloc<double>(NC); c->vv64[k] = dalloc<double>(NC);
+      }
     }
     c->S = dalloc<double>(3 * G * NC); c->a2 = dalloc<double>(3 * G * NC); c->a4 = dalloc<double>(3 * G * NC);
     c->L1 = dalloc<double>(3 * G * NC);
@@ -421,13 +423,15 @@ int adg_create(const adg_geom *g, adg_ctx **out) {
                           (long)(c->nzz + 1) * c->nxx * c->nyy});
     c->npart_blocks = (int)((maxf + NT - 1) / NT);
     c->npart = dalloc<double>((size_t)3 * 3 * c->npart_blocks);
-    c->part = dalloc<double>((size_t)MAXBLK * 6);   // three rotating buffers of 2 x MAXBLK
+    c->part = dalloc<double>((size_t)MAXBLK * 12);   // three rotating buffers of 4 x MAXBLK
     c->cnt = dalloc<unsigned>(16);
     c->sc = dalloc<Scal>(1);
     HIPCHK(hipHostMalloc((void **)&c->hsc, sizeof(Scal), hipHostMallocDefault));
     long M = (long)c->nzz * c->sz;
+    // grid of the flat kernels: one NT-cell chunk per block up to MAXBLK,
+    // rounded up to a multiple of 8 for the XCD-aware block mapping
     c->nblk = (int)std::min<long>((M + NT - 1) / NT, MAXBLK);
-    if (c->nblk < 1) c->nblk = 1;
+    c->nblk = (c->nblk + 7) / 8 * 8;
     // geometry checks that need the flags: single-node lines
     for (int n = 0; n < c->N; ++n) {
       unsigned char f = fl[c->cell[n]];
@@ -456,8 +460,9 @@ void adg_destroy(adg_ctx *c) {
   if (c->s) (void)hipStreamSynchronize(c->s);
   void *ptrs[] = {c->d_hx, c->d_hy, c->d_hz, c->flag, c->D, c->sigr, c->nuf, c->sigf, c->chi, c->sigs, c->dc,
                   c->exsrc, c->vdel, c->f0, c->f0c, c->fs0, c->fs0c, c->errn, c->dn, c->df, c->A64, c->b, c->pw,
-                  c->tmp, c->r64, c->rs64, c->t64, c->pp64[0], c->pp64[1], c->vv64[0], c->vv64[1], c->A32, c->r32,
-                  c->rs32, c->t32, c->d32, c->pp32[0], c->pp32[1], c->vv32[0], c->vv32[1], c->S, c->a2, c->a4,
+                  c->tmp, c->rr64[0], c->rr64[1], c->rs64, c->t64, c->pp64[0], c->pp64[1], c->vv64[0], c->vv64[1],
+                  c->A32, c->rr32[0], c->rr32[1], c->rs32, c->t32, c->d32, c->pp32[0], c->pp32[1], c->vv32[0],
+                  c->vv32[1], c->S, c->a2, c->a4,
                   c->L1, c->npart, c->part, c->cnt, c->sc};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
