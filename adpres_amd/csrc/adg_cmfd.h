@@ -119,9 +119,9 @@ __device__ __forceinline__ double stencil(const TA *__restrict__ A, long NC, con
   return s;
 }
 
-__global__ void __launch_bounds__(NT) k_matvec(Geo q, const double *A64, int g, const double *x, double *y) {
+__global__ void __launch_bounds__(NTF) k_matvec(Geo q, const double *A64, int g, const double *x, double *y) {
   const Range rg = block_range(q);
-  for (long c = rg.lo + threadIdx.x; c < rg.hi; c += NT)
+  for (long c = rg.lo + threadIdx.x; c < rg.hi; c += NTF)
     y[c] = stencil(A64 + (size_t)g * 7 * q.NC, q.NC, x, c, q.sy, q.sz);
 }
 
@@ -129,7 +129,7 @@ __global__ void __launch_bounds__(NT) k_matvec(Geo q, const double *A64, int g, 
 // sources: TSrc / TSrcAd (src/mod_cmfd.f90:1006-1064)
 // ---------------------------------------------------------------------------
 
-__global__ void __launch_bounds__(NT) k_src(Geo q, XS xs, const Scal *sc, int g, int adjoint, int first_group,
+__global__ void __launch_bounds__(NTF) k_src(Geo q, XS xs, const Scal *sc, int g, int adjoint, int first_group,
                                             const double *f0, double *f0c, const double *fs0, double *fs0c,
                                             double *b) {
 #pragma clang fp contract(off)
@@ -138,7 +138,7 @@ __global__ void __launch_bounds__(NT) k_src(Geo q, XS xs, const Scal *sc, int g,
   const double Ke = sc->Ke;
   const long NC = q.NC;
   const int G = q.G;
-  for (long c = rg.lo + threadIdx.x; c < rg.hi; c += NT) {
+  for (long c = rg.lo + threadIdx.x; c < rg.hi; c += NTF) {
     if (first_group) fs0c[c] = fs0[c];
     f0c[(size_t)g * NC + c] = f0[(size_t)g * NC + c];
     double s0 = 0.0;
@@ -168,13 +168,13 @@ __global__ void __launch_bounds__(NT) k_src(Geo q, XS xs, const Scal *sc, int g,
 // ---------------------------------------------------------------------------
 
 template <typename TV>
-__global__ void __launch_bounds__(NT) k_bicg_setup(Geo q, Scal *sc, const double *A64g, const double *b,
+__global__ void __launch_bounds__(NTF) k_bicg_setup(Geo q, Scal *sc, const double *A64g, const double *b,
                                                    const double *x, TV *r, TV *rs, TV *d, double *pout) {
 #pragma clang fp contract(off)
   const Range rg = block_range(q);
   if (sc->done) return;
   double acc[1] = {0.0};
-  for (long c = rg.lo + threadIdx.x; c < rg.hi; c += NT) {
+  for (long c = rg.lo + threadIdx.x; c < rg.hi; c += NTF) {
     const double rr = b[c] - stencil(A64g, q.NC, x, c, q.sy, q.sz);
     const TV rv = (TV)rr;
     r[c] = rv;
@@ -218,7 +218,7 @@ __device__ __forceinline__ TV bicg_pnext(const TV *r, const TV *v, const TV *t, 
 }
 
 template <typename TV>
-__global__ void __launch_bounds__(NT) k_bicgA(Geo q, Scal *sc, const TV *AT, int it, const TV *r, const TV *p,
+__global__ void __launch_bounds__(NTF) k_bicgA(Geo q, Scal *sc, const TV *AT, int it, const TV *r, const TV *p,
                                               const TV *v, const TV *t, const TV *rs, TV *x, TV *rout, TV *pout,
                                               TV *vout, const double *pin, double *partout) {
 #pragma clang fp contract(off)
@@ -242,7 +242,7 @@ __global__ void __launch_bounds__(NT) k_bicgA(Geo q, Scal *sc, const TV *AT, int
   const TV ta = (TV)alpha_d, tw = (TV)omega_d, tb = (TV)beta;
   const long NC = q.NC, sy = q.sy, sz = q.sz;
   double acc[1] = {0.0};
-  for (long c = rg.lo + threadIdx.x; c < rg.hi; c += NT) {
+  for (long c = rg.lo + threadIdx.x; c < rg.hi; c += NTF) {
     TV rn, dummy;
     const TV pc = bicg_pnext(r, v, t, p, c, ta, tw, tb, first, rn);
     double s = 0.0;
@@ -271,7 +271,7 @@ __global__ void __launch_bounds__(NT) k_bicgA(Geo q, Scal *sc, const TV *AT, int
 }
 
 template <typename TV>
-__global__ void __launch_bounds__(NT) k_bicgB(Geo q, Scal *sc, const TV *AT, int it, const TV *r, const TV *v,
+__global__ void __launch_bounds__(NTF) k_bicgB(Geo q, Scal *sc, const TV *AT, int it, const TV *r, const TV *v,
                                               TV *t, const TV *rs, const double *pin, double *partout) {
 #pragma clang fp contract(off)
   if (sc->done) return;
@@ -283,7 +283,7 @@ __global__ void __launch_bounds__(NT) k_bicgB(Geo q, Scal *sc, const TV *AT, int
   const TV alpha = (TV)alpha_d;
   const long NC = q.NC, sy = q.sy, sz = q.sz;
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  for (long c = rg.lo + threadIdx.x; c < rg.hi; c += NT) {
+  for (long c = rg.lo + threadIdx.x; c < rg.hi; c += NTF) {
     const TV sc_ = (TV)(r[c] - alpha * v[c]);
     double s = 0.0;
     s = s + (double)(AT[0 * NC + c] * (TV)(r[c - sz] - alpha * v[c - sz]));
@@ -308,7 +308,7 @@ __global__ void __launch_bounds__(NT) k_bicgB(Geo q, Scal *sc, const TV *AT, int
 // last iteration's x update: x += alpha p + omega s; for the FP32 form the
 // accumulated correction is then added to the FP64 flux
 template <typename TV>
-__global__ void __launch_bounds__(NT) k_bicgT(Geo q, Scal *sc, int it, const TV *r, const TV *p, const TV *v,
+__global__ void __launch_bounds__(NTF) k_bicgT(Geo q, Scal *sc, int it, const TV *r, const TV *p, const TV *v,
                                               TV *x, double *x64, const double *pin) {
 #pragma clang fp contract(off)
   if (sc->done) return;
@@ -317,7 +317,7 @@ __global__ void __launch_bounds__(NT) k_bicgT(Geo q, Scal *sc, int it, const TV 
   reduce_partials<4, 0u>(pin, gridDim.x, red);
   const double omega_d = red[1] / red[0];
   const TV ta = (TV)sc->alpha_s[it & 1], tw = (TV)omega_d;
-  for (long c = rg.lo + threadIdx.x; c < rg.hi; c += NT) {
+  for (long c = rg.lo + threadIdx.x; c < rg.hi; c += NTF) {
     const TV sv = (TV)(r[c] - ta * v[c]);
     const TV xn = (TV)(x[c] + ta * p[c] + tw * sv);
     if (x64) x64[c] = x64[c] + (double)xn;
@@ -332,11 +332,11 @@ __global__ void __launch_bounds__(NT) k_bicgT(Geo q, Scal *sc, int it, const TV 
 // (:1143).
 // ---------------------------------------------------------------------------
 
-__global__ void __launch_bounds__(NT) k_fsrc_init(Geo q, XS xs, int adjoint, const double *f0, double *fs0) {
+__global__ void __launch_bounds__(NTF) k_fsrc_init(Geo q, XS xs, int adjoint, const double *f0, double *fs0) {
 #pragma clang fp contract(off)
   const Range rg = block_range(q);
   const long NC = q.NC;
-  for (long c = rg.lo + threadIdx.x; c < rg.hi; c += NT) {
+  for (long c = rg.lo + threadIdx.x; c < rg.hi; c += NTF) {
     double fs = 0.0;
     for (int g = 0; g < q.G; ++g) {
       const double w = adjoint ? xs.chi[(size_t)g * NC + c] : xs.nuf[(size_t)g * NC + c];
@@ -348,18 +348,18 @@ __global__ void __launch_bounds__(NT) k_fsrc_init(Geo q, XS xs, int adjoint, con
 
 // f = Integrate(fs0), e1 = Integrate(1) at the start of outer (:456-458):
 // partials here, finished by k_fin_integrals
-__global__ void __launch_bounds__(NT) k_integrals(Geo q, XS xs, const double *fs0, double *partout) {
+__global__ void __launch_bounds__(NTF) k_integrals(Geo q, XS xs, const double *fs0, double *partout) {
 #pragma clang fp contract(off)
   const Range rg = block_range(q);
   double acc[2] = {0.0, 0.0};
-  for (long c = rg.lo + threadIdx.x; c < rg.hi; c += NT) {
+  for (long c = rg.lo + threadIdx.x; c < rg.hi; c += NTF) {
     acc[0] += xs.vdel[c] * fs0[c];
     acc[1] += xs.vdel[c];
   }
   block_partial<2, 0u>(acc, partout);
 }
 
-__global__ void __launch_bounds__(NT) k_fin_integrals(Scal *sc, const double *pin, int nblk) {
+__global__ void __launch_bounds__(NTF) k_fin_integrals(Scal *sc, const double *pin, int nblk) {
   double red[2];
   reduce_partials<2, 0u>(pin, nblk, red);
   if (threadIdx.x == 0) {
@@ -369,7 +369,7 @@ __global__ void __launch_bounds__(NT) k_fin_integrals(Scal *sc, const double *pi
 }
 
 // FSrc, errn = fs0 - fs0c, partials of |errn|^2 and of RelEg
-__global__ void __launch_bounds__(NT) k_fsrc(Geo q, XS xs, Scal *sc, int adjoint, const double *f0,
+__global__ void __launch_bounds__(NTF) k_fsrc(Geo q, XS xs, Scal *sc, int adjoint, const double *f0,
                                              const double *f0c, double *fs0, const double *fs0c, double *errn,
                                              double *partout) {
 #pragma clang fp contract(off)
@@ -377,7 +377,7 @@ __global__ void __launch_bounds__(NT) k_fsrc(Geo q, XS xs, Scal *sc, int adjoint
   if (sc->done) return;
   const long NC = q.NC;
   double acc[2] = {0.0, 0.0};   // sum errn^2, max flux rel. error
-  for (long c = rg.lo + threadIdx.x; c < rg.hi; c += NT) {
+  for (long c = rg.lo + threadIdx.x; c < rg.hi; c += NTF) {
     double fs = 0.0;
     for (int g = 0; g < q.G; ++g) {
       const double w = adjoint ? xs.chi[(size_t)g * NC + c] : xs.nuf[(size_t)g * NC + c];
@@ -395,7 +395,7 @@ __global__ void __launch_bounds__(NT) k_fsrc(Geo q, XS xs, Scal *sc, int adjoint
 
 // e2 = l2norm(errn) and the extrapolation (fiss_extrp, :308-335) when
 // MOD(p, nac) == 0; partials of Integrate(fs0) and of RelE
-__global__ void __launch_bounds__(NT) k_post(Geo q, XS xs, Scal *sc, double *fs0, const double *fs0c,
+__global__ void __launch_bounds__(NTF) k_post(Geo q, XS xs, Scal *sc, double *fs0, const double *fs0c,
                                              const double *errn, const double *pin, double *partout) {
 #pragma clang fp contract(off)
   const Range rg = block_range(q);
@@ -410,7 +410,7 @@ __global__ void __launch_bounds__(NT) k_post(Geo q, XS xs, Scal *sc, double *fs0
     coef = domiR / (1.0 - domiR);
   }
   double acc[2] = {0.0, 0.0};   // integral, max fission-source rel. error
-  for (long c = rg.lo + threadIdx.x; c < rg.hi; c += NT) {
+  for (long c = rg.lo + threadIdx.x; c < rg.hi; c += NTF) {
     double fs = fs0[c];
     if (extrap) {
       fs = fs + coef * errn[c];
@@ -428,7 +428,7 @@ __global__ void __launch_bounds__(NT) k_post(Geo q, XS xs, Scal *sc, double *fs0
 }
 
 // one block: k = k * f / fc (:484-485), ser, e1 = e2
-__global__ void __launch_bounds__(NT) k_fin_outer(Scal *sc, const double *pin, int nblk) {
+__global__ void __launch_bounds__(NTF) k_fin_outer(Scal *sc, const double *pin, int nblk) {
   if (sc->done) return;
   double red[2];
   reduce_partials<2, 2u>(pin, nblk, red);
@@ -472,12 +472,12 @@ __global__ void k_end(Scal *sc, double *tk, double *tser, double *tfer, double *
 }
 
 // PowDis (src/mod_cmfd.f90:1290-1333): node powers, then normalisation
-__global__ void __launch_bounds__(NT) k_powdis(Geo q, XS xs, const double *f0, double *pw, double *partout) {
+__global__ void __launch_bounds__(NTF) k_powdis(Geo q, XS xs, const double *f0, double *pw, double *partout) {
 #pragma clang fp contract(off)
   const Range rg = block_range(q);
   const long NC = q.NC;
   double acc[1] = {0.0};
-  for (long c = rg.lo + threadIdx.x; c < rg.hi; c += NT) {
+  for (long c = rg.lo + threadIdx.x; c < rg.hi; c += NTF) {
     double pc = 0.0;
     for (int g = 0; g < q.G; ++g) {
       double w = f0[(size_t)g * NC + c] * xs.sigf[(size_t)g * NC + c] * xs.vdel[c];
@@ -490,13 +490,13 @@ __global__ void __launch_bounds__(NT) k_powdis(Geo q, XS xs, const double *f0, d
   block_partial<1, 0u>(acc, partout);
 }
 
-__global__ void __launch_bounds__(NT) k_scale(Geo q, Scal *sc, double *pw, const double *pin) {
+__global__ void __launch_bounds__(NTF) k_scale(Geo q, Scal *sc, double *pw, const double *pin) {
   const Range rg = block_range(q);
   double red[1];
   reduce_partials<1, 0u>(pin, gridDim.x, red);
   const double tp = red[0];
   if (blockIdx.x == 0 && threadIdx.x == 0) sc->tpow = tp;
-  for (long c = rg.lo + threadIdx.x; c < rg.hi; c += NT)
+  for (long c = rg.lo + threadIdx.x; c < rg.hi; c += NTF)
     pw[c] = pw[c] / tp;
 }
 

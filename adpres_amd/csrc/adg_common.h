@@ -14,9 +14,10 @@
 
 namespace adg {
 
-constexpr int NT = 256;        // threads per block of the flat (stencil) kernels
+constexpr int NT = 256;        // threads per block of the per-cell / nodal kernels
+constexpr int NTF = 1024;      // threads per block of the flat stencil kernels (16 waves)
 constexpr int MAXG = 8;        // largest group count with a compiled nodal kernel
-constexpr int MAXBLK = 1024;   // largest grid of the flat kernels (partials per buffer)
+constexpr int MAXBLK = 256;    // largest grid of the flat kernels: one 16-wave block per CU
 
 // per-cell flag bits: bit0 active node; bit (1+s) neighbour present across
 // face s, faces s = 0..5 = x+, x-, y+, y-, z+, z- (the reference's 1..6)
@@ -64,7 +65,7 @@ struct XS {
 // to XCD b % 8; with the grid a multiple of 8, logical block
 // (b % 8) * (grid / 8) + b / 8 gives every XCD one contiguous slab of the
 // box, so the z-neighbour planes of a stencil are read through that XCD's
-// own L2.  Each logical block owns `chunk` consecutive cells (a multiple of NT).
+// own L2.  Each logical block owns `chunk` consecutive cells (a multiple of NTF).
 struct Range {
   long lo, hi;
 };
@@ -72,7 +73,7 @@ __device__ __forceinline__ Range block_range(const Geo &q) {
   const int nb = gridDim.x, b = blockIdx.x;
   const int lb = (nb & 7) ? b : (b & 7) * (nb >> 3) + (b >> 3);
   long chunk = (q.M + nb - 1) / nb;
-  chunk = (chunk + NT - 1) / NT * NT;
+  chunk = (chunk + NTF - 1) / NTF * NTF;
   Range r;
   r.lo = q.c0 + (long)lb * chunk;
   r.hi = r.lo + chunk;
@@ -105,7 +106,7 @@ __device__ __forceinline__ void wave_reduce(double (&v)[NV]) {
 // block-wide reduction; the result is returned in every thread
 template <int NV, unsigned MAXMASK>
 __device__ __forceinline__ void block_reduce(double (&v)[NV]) {
-  __shared__ double sm[(NT / 64) * NV];
+  __shared__ double sm[(NTF / 64) * NV];
   wave_reduce<NV, MAXMASK>(v);
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   __syncthreads();
@@ -117,7 +118,7 @@ __device__ __forceinline__ void block_reduce(double (&v)[NV]) {
 #pragma unroll
   for (int k = 0; k < NV; ++k) {
     double a = sm[k];
-    for (int ww = 1; ww < NT / 64; ++ww) a = ((MAXMASK >> k) & 1u) ? fmax(a, sm[ww * NV + k]) : a + sm[ww * NV + k];
+    for (int ww = 1; ww < NTF / 64; ++ww) a = ((MAXMASK >> k) & 1u) ? fmax(a, sm[ww * NV + k]) : a + sm[ww * NV + k];
     v[k] = a;
   }
 }
@@ -137,7 +138,7 @@ template <int NV, unsigned MAXMASK>
 __device__ __forceinline__ void reduce_partials(const double *part, int nblk, double (&out)[NV]) {
 #pragma unroll
   for (int k = 0; k < NV; ++k) out[k] = 0.0;
-  for (int b = threadIdx.x; b < nblk; b += NT) {
+  for (int b = threadIdx.x; b < nblk; b += NTF) {
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
       const double x = part[(size_t)b * NV + k];

@@ -192,17 +192,17 @@ void launch_bicg_t(adg_ctx *c, int g, int nin, const TV *A, TV *const *R, TV *co
   TV *x = d ? d : (TV *)x64;
   const int sample = nin > 1 ? 2 : 1;   // steady-state KA launch that is timed
   int ek0 = -1, ek1 = -1;
-  hipLaunchKernelGGL(k_bicg_setup<TV>, dim3(c->nblk), dim3(NT), 0, c->s, q, c->sc, A64g, c->b, x64, R[0], rs, d, P0);
+  hipLaunchKernelGGL(k_bicg_setup<TV>, dim3(c->nblk), dim3(NTF), 0, c->s, q, c->sc, A64g, c->b, x64, R[0], rs, d, P0);
   for (int it = 1; it <= nin; ++it) {
     const int rd = (it - 1) & 1, wr = it & 1;
     if (it == sample) ek0 = prof_event(c);
-    hipLaunchKernelGGL(k_bicgA<TV>, dim3(c->nblk), dim3(NT), 0, c->s, q, c->sc, A, it, R[rd], P[rd], V[rd], t, rs, x,
+    hipLaunchKernelGGL(k_bicgA<TV>, dim3(c->nblk), dim3(NTF), 0, c->s, q, c->sc, A, it, R[rd], P[rd], V[rd], t, rs, x,
                        R[wr], P[wr], V[wr], it == 1 ? P0 : P2, P1);
     if (it == sample) ek1 = prof_event(c);
-    hipLaunchKernelGGL(k_bicgB<TV>, dim3(c->nblk), dim3(NT), 0, c->s, q, c->sc, A, it, R[wr], V[wr], t, rs, P1, P2);
+    hipLaunchKernelGGL(k_bicgB<TV>, dim3(c->nblk), dim3(NTF), 0, c->s, q, c->sc, A, it, R[wr], V[wr], t, rs, P1, P2);
   }
   const int wr = nin & 1;
-  hipLaunchKernelGGL(k_bicgT<TV>, dim3(c->nblk), dim3(NT), 0, c->s, q, c->sc, nin, R[wr], P[wr], V[wr], x,
+  hipLaunchKernelGGL(k_bicgT<TV>, dim3(c->nblk), dim3(NTF), 0, c->s, q, c->sc, nin, R[wr], P[wr], V[wr], x,
                      d ? x64 : (double *)nullptr, P2);
   HIPCHK(hipGetLastError());
   prof_pair(c, 0, ek0, ek1);
@@ -221,7 +221,7 @@ void launch_bicg(adg_ctx *c, int g, int nin) {
 void group_solve(adg_ctx *c, int g, int adjoint, int first_group, int nin) {
   Geo q = c->geo();
   const int eg0 = prof_event(c);
-  hipLaunchKernelGGL(k_src, dim3(c->nblk), dim3(NT), 0, c->s, q, c->xs(), c->sc, g, adjoint, first_group, c->f0, c->f0c,
+  hipLaunchKernelGGL(k_src, dim3(c->nblk), dim3(NTF), 0, c->s, q, c->xs(), c->sc, g, adjoint, first_group, c->f0, c->f0c,
                      c->fs0, c->fs0c, c->b);
   launch_bicg(c, g, nin);
   prof_pair(c, 1, eg0, prof_event(c));
@@ -299,10 +299,10 @@ void enqueue_iteration(adg_ctx *c, int adjoint, int nin, int kern, int p_host, i
     group_solve(c, g, adjoint, gg == 0 ? 1 : 0, nin);
   }
   double *P1 = c->part + 4 * MAXBLK, *P2 = c->part + 8 * MAXBLK;
-  hipLaunchKernelGGL(k_fsrc, dim3(c->nblk), dim3(NT), 0, c->s, q, x, c->sc, adjoint, c->f0, c->f0c, c->fs0, c->fs0c,
+  hipLaunchKernelGGL(k_fsrc, dim3(c->nblk), dim3(NTF), 0, c->s, q, x, c->sc, adjoint, c->f0, c->f0c, c->fs0, c->fs0c,
                      c->errn, P1);
-  hipLaunchKernelGGL(k_post, dim3(c->nblk), dim3(NT), 0, c->s, q, x, c->sc, c->fs0, c->fs0c, c->errn, P1, P2);
-  hipLaunchKernelGGL(k_fin_outer, dim3(1), dim3(NT), 0, c->s, c->sc, P2, c->nblk);
+  hipLaunchKernelGGL(k_post, dim3(c->nblk), dim3(NTF), 0, c->s, q, x, c->sc, c->fs0, c->fs0c, c->errn, P1, P2);
+  hipLaunchKernelGGL(k_fin_outer, dim3(1), dim3(NTF), 0, c->s, c->sc, P2, c->nblk);
   if (kern != ADG_KERN_FDM) {
     const bool timed = p_host > 0 && (p_host % nupd) == 0;
     const int en0 = timed ? prof_event(c) : -1;
@@ -428,9 +428,9 @@ int adg_create(const adg_geom *g, adg_ctx **out) {
     c->sc = dalloc<Scal>(1);
     HIPCHK(hipHostMalloc((void **)&c->hsc, sizeof(Scal), hipHostMallocDefault));
     long M = (long)c->nzz * c->sz;
-    // grid of the flat kernels: one NT-cell chunk per block up to MAXBLK,
+    // grid of the flat kernels: one NTF-cell chunk per block up to MAXBLK,
     // rounded up to a multiple of 8 for the XCD-aware block mapping
-    c->nblk = (int)std::min<long>((M + NT - 1) / NT, MAXBLK);
+    c->nblk = (int)std::min<long>((M + NTF - 1) / NTF, MAXBLK);
     c->nblk = (c->nblk + 7) / 8 * 8;
     // geometry checks that need the flags: single-node lines
     for (int n = 0; n < c->N; ++n) {
@@ -587,11 +587,11 @@ int adg_outer(adg_ctx *c, int mode, const adg_iter *it, adg_trace *tr) {
     hipLaunchKernelGGL(k_coupcoef, cells_grid(c, c->G), dim3(NT), 0, c->s, q, x, c->df);
     launch_matrix(c, 0);
     if (c->first) {
-      hipLaunchKernelGGL(k_fsrc_init, dim3(c->nblk), dim3(NT), 0, c->s, q, x, adjoint, c->f0, c->fs0);
+      hipLaunchKernelGGL(k_fsrc_init, dim3(c->nblk), dim3(NTF), 0, c->s, q, x, adjoint, c->f0, c->fs0);
       c->first = false;
     }
-    hipLaunchKernelGGL(k_integrals, dim3(c->nblk), dim3(NT), 0, c->s, q, x, c->fs0, c->part);
-    hipLaunchKernelGGL(k_fin_integrals, dim3(1), dim3(NT), 0, c->s, c->sc, c->part, c->nblk);
+    hipLaunchKernelGGL(k_integrals, dim3(c->nblk), dim3(NTF), 0, c->s, q, x, c->fs0, c->part);
+    hipLaunchKernelGGL(k_fin_integrals, dim3(1), dim3(NTF), 0, c->s, c->sc, c->part, c->nblk);
     HIPCHK(hipGetLastError());
     // replay: a captured graph of one outer (or direct launches when
     // profiling, whose events need the host-side sequence); the done flag
@@ -695,8 +695,8 @@ int adg_powdis(adg_ctx *c, double *p) {
   if (!c || !p) return ADG_E_ARG;
   try {
     Geo q = c->geo();
-    hipLaunchKernelGGL(k_powdis, dim3(c->nblk), dim3(NT), 0, c->s, q, c->xs(), c->f0, c->pw, c->part);
-    hipLaunchKernelGGL(k_scale, dim3(c->nblk), dim3(NT), 0, c->s, q, c->sc, c->pw, c->part);
+    hipLaunchKernelGGL(k_powdis, dim3(c->nblk), dim3(NTF), 0, c->s, q, c->xs(), c->f0, c->pw, c->part);
+    hipLaunchKernelGGL(k_scale, dim3(c->nblk), dim3(NTF), 0, c->s, q, c->sc, c->pw, c->part);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(c->hsc, c->sc, sizeof(Scal), hipMemcpyDeviceToHost, c->s));
     HIPCHK(hipStreamSynchronize(c->s));
@@ -768,7 +768,7 @@ int adg_matvec(adg_ctx *c, int g, const double *x, double *y) {
     std::vector<double> st;
     scatter(c, x, 1, c->b, st);
     HIPCHK(hipMemsetAsync(c->tmp, 0, c->NC * sizeof(double), c->s));
-    hipLaunchKernelGGL(k_matvec, dim3(c->nblk), dim3(NT), 0, c->s, q, c->A64, g, c->b, c->tmp);
+    hipLaunchKernelGGL(k_matvec, dim3(c->nblk), dim3(NTF), 0, c->s, q, c->A64, g, c->b, c->tmp);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(c->s));
     gather(c, c->tmp, 1, y, st);

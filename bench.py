@@ -9,9 +9,9 @@ timed on the same core beside it; both are checked against the FP64
 reference result (k within 1 pcm, node powers within 1e-4).
 
 Also reported:
-  roofline      BiCGSTAB K1 (the fused p-update + 7-point SpMV + (rs,v) dot),
-                13 algorithmic words per node per launch, timed with HIP
-                events on the engine's stream during the timed solves;
+  roofline      BiCGSTAB KA (previous x/r update + p update + 7-point SpMV +
+                (rs,v) dot), 17 algorithmic words per node per launch, timed
+                with HIP events on the engine's stream in a profiled solve;
   cpu_baseline  the reference's own Fortran build (oracle/_ref) on one host
                 core, on a bounded sample of the same core.
 
@@ -129,7 +129,11 @@ def main():
     ap.add_argument("--workload", default="core3d_2g", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch kernels directly instead of replaying the captured outer (for profilers)")
     args = ap.parse_args()
+    if args.no_graph:
+        os.environ["ADG_GRAPH"] = "0"
     if args.gpus != 1:
         print(f"bench.py: this engine runs one core per GPU and this build measures one GPU only; "
               f"--gpus {args.gpus} is not supported (multi-GPU sharding is a design note)", file=sys.stderr)
@@ -164,16 +168,17 @@ def main():
         return dict(k_err_pcm=abs(r["k"] - float(ref["Ke"])) * 1e5,
                     pow_max_rel_err=float(np.max(np.abs(r["pow"][m] - ref["pow"][m]) / ref["pow"][m])))
 
-    # roofline of BiCGSTAB K1: r, p, v in; p, v out; 7 coefficients; rs = 13 words/node
-    k1_bytes = 13 * 8 * p.nnod
+    # roofline of the BiCGSTAB stencil kernel KA (k_bicgA): r, p, v, t, x, rs and
+    # 7 matrix coefficients in; r, p, v, x out = 17 words per node per launch
+    k1_bytes = 17 * 8 * p.nnod
     roof = None
     if r64["k1_avg_ms"]:
         ach = k1_bytes / (r64["k1_avg_ms"] * 1e-3) / 1e9
         roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "k_bicg1<double> (fused p-update + 7-point SpMV + (rs,v))",
+                "kernel": "k_bicgA<double> (BiCGSTAB: previous x/r update + p update + 7-point SpMV + (rs,v))",
                 "algorithmic_bytes_per_launch": k1_bytes, "avg_launch_us": round(r64["k1_avg_ms"] * 1e3, 3),
-                "samples": r64["k1_samples"], "timing": "HIP events around one steady-state K1 launch per group "
+                "samples": r64["k1_samples"], "timing": "HIP events around one steady-state KA launch per group "
                                                         "solve, on the engine stream, in a profiled solve after the timed ones",
                 "group_solve_avg_us": round(r64["gsolve_avg_ms"] * 1e3, 2),
                 "nodal_update_avg_us": round(r64["nodal_avg_ms"] * 1e3, 2)}
