@@ -17,15 +17,17 @@ HIP_FLAGS = [
 ]
 
 
-def build(verbose: bool = False) -> str:
-    cmd = [HIPCC, *HIP_FLAGS, "-o", LIB + ".tmp", *SOURCES]
+def build(verbose: bool = False, out: str = LIB, defines=()) -> str:
+    """Compile the engine; `defines` (e.g. ["ADG_NTF=512"]) override the
+    launch-shape defaults for tuning builds written to `out`."""
+    cmd = [HIPCC, *HIP_FLAGS, *("-D" + d for d in defines), "-o", out + ".tmp", *SOURCES]
     r = subprocess.run(cmd, cwd=ROOT, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("hipcc failed:\n" + r.stdout + r.stderr)
     if verbose:
         print(r.stderr)
-    os.replace(LIB + ".tmp", LIB)
-    return LIB
+    os.replace(out + ".tmp", out)
+    return out
 
 
 if __name__ == "__main__":

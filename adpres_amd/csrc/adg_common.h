@@ -14,10 +14,22 @@
 
 namespace adg {
 
-constexpr int NT = 256;        // threads per block of the per-cell / nodal kernels
-constexpr int NTF = 1024;      // threads per block of the flat stencil kernels (16 waves)
-constexpr int MAXG = 8;        // largest group count with a compiled nodal kernel
-constexpr int MAXBLK = 256;    // largest grid of the flat kernels: one 16-wave block per CU
+// Launch shape of the flat stencil kernels (threads per block, largest grid)
+// and how a consumer finishes a reduction (per wavefront without barriers,
+// or per block); the defaults are the measured best on gfx950.
+#ifndef ADG_NTF
+#define ADG_NTF 256
+#endif
+#ifndef ADG_MAXBLK
+#define ADG_MAXBLK 1024
+#endif
+#ifndef ADG_WAVE_REDUCE
+#define ADG_WAVE_REDUCE 1
+#endif
+constexpr int NT = 256;            // threads per block of the per-cell / nodal kernels
+constexpr int NTF = ADG_NTF;       // threads per block of the flat stencil kernels
+constexpr int MAXG = 8;            // largest group count with a compiled nodal kernel
+constexpr int MAXBLK = ADG_MAXBLK; // largest grid of the flat kernels
 
 // per-cell flag bits: bit0 active node; bit (1+s) neighbour present across
 // face s, faces s = 0..5 = x+, x-, y+, y-, z+, z- (the reference's 1..6)
@@ -133,11 +145,26 @@ __device__ __forceinline__ void block_partial(double (&v)[NV], double *part) {
   }
 }
 
-// grid-wide result of the nblk partials written by an earlier kernel
+// grid-wide result of the nblk partials written by an earlier kernel.  In
+// the per-wavefront form every wavefront reduces all partials on its own
+// (same order in every wavefront, so bit-identical results) with no LDS and
+// no barrier, which lets the compiler issue the kernel's stencil loads
+// before the reduction has finished.
 template <int NV, unsigned MAXMASK>
 __device__ __forceinline__ void reduce_partials(const double *part, int nblk, double (&out)[NV]) {
 #pragma unroll
   for (int k = 0; k < NV; ++k) out[k] = 0.0;
+  if (ADG_WAVE_REDUCE) {
+    for (int b = (int)(threadIdx.x & 63); b < nblk; b += 64) {
+#pragma unroll
+      for (int k = 0; k < NV; ++k) {
+        const double x = part[(size_t)b * NV + k];
+        out[k] = ((MAXMASK >> k) & 1u) ? fmax(out[k], x) : out[k] + x;
+      }
+    }
+    wave_reduce<NV, MAXMASK>(out);
+    return;
+  }
   for (int b = threadIdx.x; b < nblk; b += NTF) {
 #pragma unroll
     for (int k = 0; k < NV; ++k) {

@@ -24,7 +24,7 @@ import numpy as np
 from .problem import Problem
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libadpres_gpu.so")
+LIB_PATH = os.environ.get("ADG_LIBRARY") or os.path.join(HERE, "libadpres_gpu.so")
 
 ADG_OK, ADG_E_MAX_OUTER, ADG_E_NODAL_UNSTABLE, ADG_E_LU_PIVOT, ADG_E_POWER_NONPOS = 0, 1, 2, 3, 4
 ADG_E_ARG, ADG_E_HIP, ADG_E_NOT_SET = 5, 6, 7
