@@ -24,7 +24,7 @@ namespace adg {
 #define ADG_MAXBLK 1024
 #endif
 #ifndef ADG_WAVE_REDUCE
-#define ADG_WAVE_REDUCE 1
+#define ADG_WAVE_REDUCE 0
 #endif
 constexpr int NT = 256;            // threads per block of the per-cell / nodal kernels
 constexpr int NTF = ADG_NTF;       // threads per block of the flat stencil kernels
