@@ -204,30 +204,33 @@ __global__ void __launch_bounds__(NTF) k_bicg_setup(Geo q, Scal *sc, const doubl
 // In exact arithmetic these are the reference's iterates; the K3 update of
 // the three-kernel form is recomputed where the next stencil reads it.  r, p
 // and v ping-pong between two buffers because neighbours read old values.
-template <typename TV>
-__device__ __forceinline__ TV bicg_pnext(const TV *r, const TV *v, const TV *t, const TV *p, long c, TV alpha,
-                                         TV omega, TV beta, int first, TV &rnew) {
-#pragma clang fp contract(off)
-  if (first) {
-    rnew = r[c];
-    return r[c];
-  }
-  const TV s = (TV)(r[c] - alpha * v[c]);
-  rnew = (TV)(s - omega * t[c]);
-  return (TV)(rnew + beta * (p[c] - omega * v[c]));
-}
+//
+// Each cell's loads (4 vectors at 7 points and 7 coefficients in KA, 2
+// vectors at 7 points and 7 coefficients in KB) are all issued before any
+// arithmetic, so a wavefront waits for one memory round trip per cell
+// instead of one per stencil point; the first iteration (p = r) is a
+// separate instantiation so that no branch splits the load batch.  The FP64
+// kernels are allowed 128 VGPRs (at most 4 waves per SIMD), which the batch
+// of 35 double loads needs.
 
+// occupancy bound that lets one cell's loads be in flight together
 template <typename TV>
-__global__ void __launch_bounds__(NTF) k_bicgA(Geo q, Scal *sc, const TV *AT, int it, const TV *r, const TV *p,
-                                              const TV *v, const TV *t, const TV *rs, TV *x, TV *rout, TV *pout,
-                                              TV *vout, const double *pin, double *partout) {
+struct StencilWaves {
+  static constexpr int value = sizeof(TV) == 8 ? 4 : 8;
+};
+
+template <typename TV, bool FIRST>
+__global__ void __launch_bounds__(NTF) __attribute__((amdgpu_waves_per_eu(1, StencilWaves<TV>::value)))
+k_bicgA(Geo q, Scal *__restrict__ sc, const TV *__restrict__ AT, int it, const TV *__restrict__ r,
+        const TV *__restrict__ p, const TV *__restrict__ v, const TV *__restrict__ t, const TV *__restrict__ rs,
+        TV *__restrict__ x, TV *__restrict__ rout, TV *__restrict__ pout, TV *__restrict__ vout,
+        const double *__restrict__ pin, double *__restrict__ partout) {
 #pragma clang fp contract(off)
   if (sc->done) return;
   const Range rg = block_range(q);
-  const int first = it == 1;
   const int prv = (it - 1) & 1, cur = it & 1;
   double rho, alpha_d = 0.0, omega_d = 0.0, beta = 0.0;
-  if (first) {
+  if (FIRST) {
     double red[1];
     reduce_partials<1, 0u>(pin, gridDim.x, red);
     rho = red[0];
@@ -241,38 +244,62 @@ __global__ void __launch_bounds__(NTF) k_bicgA(Geo q, Scal *sc, const TV *AT, in
   }
   const TV ta = (TV)alpha_d, tw = (TV)omega_d, tb = (TV)beta;
   const long NC = q.NC, sy = q.sy, sz = q.sz;
+  const long off[7] = {-sz, -sy, -1, 0, 1, sy, sz};   // slot order z-, y-, x-, c, x+, y+, z+
   double acc[1] = {0.0};
   for (long c = rg.lo + threadIdx.x; c < rg.hi; c += NTF) {
-    TV rn, dummy;
-    const TV pc = bicg_pnext(r, v, t, p, c, ta, tw, tb, first, rn);
+    TV R[7], V[7], T[7], P[7], Ac[7];
+#pragma unroll
+    for (int k = 0; k < 7; ++k) {
+      R[k] = r[c + off[k]];
+      if (!FIRST) {
+        V[k] = v[c + off[k]];
+        T[k] = t[c + off[k]];
+        P[k] = p[c + off[k]];
+      }
+      Ac[k] = AT[k * NC + c];
+    }
+    const TV rsc = rs[c];
+    TV xc = 0;
+    if (!FIRST) xc = x[c];
+    TV pn[7], rn;
+    if (FIRST) {
+#pragma unroll
+      for (int k = 0; k < 7; ++k) pn[k] = R[k];
+      rn = R[3];
+    } else {
+#pragma unroll
+      for (int k = 0; k < 7; ++k) {
+        const TV s_ = (TV)(R[k] - ta * V[k]);
+        const TV r_ = (TV)(s_ - tw * T[k]);
+        if (k == 3) rn = r_;
+        pn[k] = (TV)(r_ + tb * (P[k] - tw * V[k]));
+      }
+    }
     double s = 0.0;
-    s = s + (double)(AT[0 * NC + c] * bicg_pnext(r, v, t, p, c - sz, ta, tw, tb, first, dummy));
-    s = s + (double)(AT[1 * NC + c] * bicg_pnext(r, v, t, p, c - sy, ta, tw, tb, first, dummy));
-    s = s + (double)(AT[2 * NC + c] * bicg_pnext(r, v, t, p, c - 1, ta, tw, tb, first, dummy));
-    s = s + (double)(AT[3 * NC + c] * pc);
-    s = s + (double)(AT[4 * NC + c] * bicg_pnext(r, v, t, p, c + 1, ta, tw, tb, first, dummy));
-    s = s + (double)(AT[5 * NC + c] * bicg_pnext(r, v, t, p, c + sy, ta, tw, tb, first, dummy));
-    s = s + (double)(AT[6 * NC + c] * bicg_pnext(r, v, t, p, c + sz, ta, tw, tb, first, dummy));
+#pragma unroll
+    for (int k = 0; k < 7; ++k) s = s + (double)(Ac[k] * pn[k]);
     const TV vc = (TV)s;
-    if (!first) {
-      const TV sv = (TV)(r[c] - ta * v[c]);
-      x[c] = (TV)(x[c] + ta * p[c] + tw * sv);
+    if (!FIRST) {
+      const TV sv = (TV)(R[3] - ta * V[3]);
+      x[c] = (TV)(xc + ta * P[3] + tw * sv);
     }
     rout[c] = rn;
-    pout[c] = pc;
+    pout[c] = pn[3];
     vout[c] = vc;
-    acc[0] += (double)rs[c] * (double)vc;
+    acc[0] += (double)rsc * (double)vc;
   }
   block_partial<1, 0u>(acc, partout);
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     sc->rho_s[cur] = rho;
-    if (!first) sc->omega_s[prv] = omega_d;
+    if (!FIRST) sc->omega_s[prv] = omega_d;
   }
 }
 
 template <typename TV>
-__global__ void __launch_bounds__(NTF) k_bicgB(Geo q, Scal *sc, const TV *AT, int it, const TV *r, const TV *v,
-                                              TV *t, const TV *rs, const double *pin, double *partout) {
+__global__ void __launch_bounds__(NTF) __attribute__((amdgpu_waves_per_eu(1, StencilWaves<TV>::value)))
+k_bicgB(Geo q, Scal *__restrict__ sc, const TV *__restrict__ AT, int it, const TV *__restrict__ r,
+        const TV *__restrict__ v, TV *__restrict__ t, const TV *__restrict__ rs, const double *__restrict__ pin,
+        double *__restrict__ partout) {
 #pragma clang fp contract(off)
   if (sc->done) return;
   const Range rg = block_range(q);
@@ -282,24 +309,29 @@ __global__ void __launch_bounds__(NTF) k_bicgB(Geo q, Scal *sc, const TV *AT, in
   const double alpha_d = sc->rho_s[cur] / red[0];
   const TV alpha = (TV)alpha_d;
   const long NC = q.NC, sy = q.sy, sz = q.sz;
+  const long off[7] = {-sz, -sy, -1, 0, 1, sy, sz};
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
   for (long c = rg.lo + threadIdx.x; c < rg.hi; c += NTF) {
-    const TV sc_ = (TV)(r[c] - alpha * v[c]);
+    TV R[7], V[7], Ac[7];
+#pragma unroll
+    for (int k = 0; k < 7; ++k) {
+      R[k] = r[c + off[k]];
+      V[k] = v[c + off[k]];
+      Ac[k] = AT[k * NC + c];
+    }
+    const double rsc = (double)rs[c];
+    TV sv[7];
+#pragma unroll
+    for (int k = 0; k < 7; ++k) sv[k] = (TV)(R[k] - alpha * V[k]);
     double s = 0.0;
-    s = s + (double)(AT[0 * NC + c] * (TV)(r[c - sz] - alpha * v[c - sz]));
-    s = s + (double)(AT[1 * NC + c] * (TV)(r[c - sy] - alpha * v[c - sy]));
-    s = s + (double)(AT[2 * NC + c] * (TV)(r[c - 1] - alpha * v[c - 1]));
-    s = s + (double)(AT[3 * NC + c] * sc_);
-    s = s + (double)(AT[4 * NC + c] * (TV)(r[c + 1] - alpha * v[c + 1]));
-    s = s + (double)(AT[5 * NC + c] * (TV)(r[c + sy] - alpha * v[c + sy]));
-    s = s + (double)(AT[6 * NC + c] * (TV)(r[c + sz] - alpha * v[c + sz]));
+#pragma unroll
+    for (int k = 0; k < 7; ++k) s = s + (double)(Ac[k] * sv[k]);
     const TV tc = (TV)s;
     t[c] = tc;
-    const double rsc = (double)rs[c];
     acc[0] += (double)tc * (double)tc;
-    acc[1] += (double)tc * (double)sc_;
+    acc[1] += (double)tc * (double)sv[3];
     acc[2] += rsc * (double)tc;
-    acc[3] += rsc * (double)sc_;
+    acc[3] += rsc * (double)sv[3];
   }
   block_partial<4, 0u>(acc, partout);
   if (blockIdx.x == 0 && threadIdx.x == 0) sc->alpha_s[cur] = alpha_d;

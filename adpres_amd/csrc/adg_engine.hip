@@ -1,6 +1,7 @@
 // Host side of the gfx950 static k-eigenvalue engine: context, data
 // movement between the reference's node list and the padded device box, the
 // outer-iteration driver and the C-ABI of include/adpres_gpu.h.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -140,6 +141,18 @@ int prof_event(adg_ctx *c) {
   return k;
 }
 
+// index of an unrecorded event for hipExtLaunchKernelGGL, which records its
+// start/stop events at the kernel's own start and end, or -1
+int prof_slot(adg_ctx *c) {
+  if (!c->prof || c->ev_next >= MAX_PROF_EVENTS) return -1;
+  if (c->ev_next == (int)c->evs.size()) {
+    hipEvent_t e;
+    HIPCHK(hipEventCreate(&e));
+    c->evs.push_back(e);
+  }
+  return c->ev_next++;
+}
+
 void prof_pair(adg_ctx *c, int type, int e0, int e1) {
   if (e0 >= 0 && e1 >= 0) c->recs.push_back({type, e0, e1});
 }
@@ -190,15 +203,24 @@ void launch_bicg_t(adg_ctx *c, int g, int nin, const TV *A, TV *const *R, TV *co
   const double *A64g = c->A64 + (size_t)g * 7 * NC;
   // FP64 form: the iterate is the flux itself; FP32 form: a correction d
   TV *x = d ? d : (TV *)x64;
-  const int sample = nin > 1 ? 2 : 1;   // steady-state KA launch that is timed
+  const int sample = nin > 1 ? 2 : 0;   // steady-state KA launch that is timed (none if nin = 1)
   int ek0 = -1, ek1 = -1;
   hipLaunchKernelGGL(k_bicg_setup<TV>, dim3(c->nblk), dim3(NTF), 0, c->s, q, c->sc, A64g, c->b, x64, R[0], rs, d, P0);
   for (int it = 1; it <= nin; ++it) {
     const int rd = (it - 1) & 1, wr = it & 1;
-    if (it == sample) ek0 = prof_event(c);
-    hipLaunchKernelGGL(k_bicgA<TV>, dim3(c->nblk), dim3(NTF), 0, c->s, q, c->sc, A, it, R[rd], P[rd], V[rd], t, rs, x,
-                       R[wr], P[wr], V[wr], it == 1 ? P0 : P2, P1);
-    if (it == sample) ek1 = prof_event(c);
+    if (it == sample) {
+      ek0 = prof_slot(c);
+      ek1 = prof_slot(c);
+    }
+    if (it == sample && ek0 >= 0 && ek1 >= 0)
+      hipExtLaunchKernelGGL((k_bicgA<TV, false>), dim3(c->nblk), dim3(NTF), 0, c->s, c->evs[ek0], c->evs[ek1], 0, q,
+                            c->sc, A, it, R[rd], P[rd], V[rd], t, rs, x, R[wr], P[wr], V[wr], P2, P1);
+    else if (it == 1)
+      hipLaunchKernelGGL((k_bicgA<TV, true>), dim3(c->nblk), dim3(NTF), 0, c->s, q, c->sc, A, it, R[rd], P[rd], V[rd],
+                         t, rs, x, R[wr], P[wr], V[wr], P0, P1);
+    else
+      hipLaunchKernelGGL((k_bicgA<TV, false>), dim3(c->nblk), dim3(NTF), 0, c->s, q, c->sc, A, it, R[rd], P[rd], V[rd],
+                         t, rs, x, R[wr], P[wr], V[wr], P2, P1);
     hipLaunchKernelGGL(k_bicgB<TV>, dim3(c->nblk), dim3(NTF), 0, c->s, q, c->sc, A, it, R[wr], V[wr], t, rs, P1, P2);
   }
   const int wr = nin & 1;
@@ -430,7 +452,9 @@ int adg_create(const adg_geom *g, adg_ctx **out) {
     long M = (long)c->nzz * c->sz;
     // grid of the flat kernels: one NTF-cell chunk per block up to MAXBLK,
     // rounded up to a multiple of 8 for the XCD-aware block mapping
-    c->nblk = (int)std::min<long>((M + NTF - 1) / NTF, MAXBLK);
+    // (the FP32 form streams half the bytes and is best at half the grid)
+    const long maxblk = c->precision == ADG_FP32_REFINE ? MAXBLK / 2 : MAXBLK;
+    c->nblk = (int)std::min<long>((M + NTF - 1) / NTF, maxblk);
     c->nblk = (c->nblk + 7) / 8 * 8;
     // geometry checks that need the flags: single-node lines
     for (int n = 0; n < c->N; ++n) {

@@ -11,7 +11,9 @@ reference result (k within 1 pcm, node powers within 1e-4).
 Also reported:
   roofline      BiCGSTAB KA (previous x/r update + p update + 7-point SpMV +
                 (rs,v) dot), 17 algorithmic words per node per launch, timed
-                with HIP events on the engine's stream in a profiled solve;
+                with start/stop events attached to one KA dispatch per group
+                solve in a profiled solve; traffic from the committed
+                rocprofv3 counter summary of the same kernel;
   cpu_baseline  the reference's own Fortran build (oracle/_ref) on one host
                 core, on a bounded sample of the same core.
 
@@ -121,16 +123,61 @@ def cpu_baseline(problem, outers_per_solve, nupd):
         shutil.rmtree(d, ignore_errors=True)
 
 
+def measured_traffic(workload, kernel):
+    """HBM-side bytes per launch of `kernel` from the committed counter
+    summary profiles/pmc_bicg_<workload>.txt (FETCH_SIZE and WRITE_SIZE runs
+    of --kernel-probe, scaled by the 256 MiB copy calibration), or None."""
+    path = os.path.join(ROOT, "profiles", f"pmc_bicg_{workload}.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith(kernel + " "):
+            return int(round(float(line[len(kernel):].split()[3]) * 1e6))
+    return None
+
+
+def kernel_probe(p):
+    """Short program for hardware-counter runs (rocprofv3 --pmc): two device
+    copies of 256 MiB (the calibration kernel of known size), then one
+    BiCGSTAB call (group 1, nin=20, from a flat guess) in each precision."""
+    import ctypes
+    from adpres_amd.engine import Engine
+    hip = ctypes.CDLL("libamdhip64.so")
+    nb = 256 << 20
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    for ptr in (a, b):
+        if hip.hipMalloc(ctypes.byref(ptr), ctypes.c_size_t(nb)) != 0:
+            raise SystemExit("hipMalloc failed")
+    hip.hipMemset(a, 1, ctypes.c_size_t(nb))
+    for _ in range(2):
+        hip.hipMemcpy(b, a, ctypes.c_size_t(nb), 3)
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    rng = np.random.default_rng(0)
+    rhs = rng.uniform(0.5, 1.5, p.nnod)
+    x0 = np.ones(p.nnod)
+    out = {"probe": "kernel-counters", "copy_bytes": nb, "nodes": p.nnod}
+    for prec in ("fp64", "fp32_refine"):
+        e = Engine(p, prec)
+        x = e.bicg(0, 20, rhs, x0)
+        out[prec + "_xmax"] = float(np.abs(x).max())
+        e.close()
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="core3d_2g", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="core3d_8g", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch kernels directly instead of replaying the captured outer (for profilers)")
+    ap.add_argument("--kernel-probe", action="store_true",
+                    help="run only a copy of known size and one BiCGSTAB call per precision (for counter runs)")
     args = ap.parse_args()
     if args.no_graph:
         os.environ["ADG_GRAPH"] = "0"
@@ -149,6 +196,9 @@ def main():
     if ref is not None and "p_ix" not in ref.files:
         ref = dict(Ke=ref["Ke"], pow=ref["pow"].astype(np.float64))
     log(f"bench: {args.workload} N={p.nnod} G={p.ng} unknowns={p.unknowns}")
+    if args.kernel_probe:
+        kernel_probe(p)
+        return
 
     eng = Engine(p, "fp64")
     r64 = time_solves(eng, args.steps, args.warmup)
@@ -175,11 +225,15 @@ def main():
     if r64["k1_avg_ms"]:
         ach = k1_bytes / (r64["k1_avg_ms"] * 1e-3) / 1e9
         roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(ach / HBM_PEAK_GBS, 4),
+                "traffic": measured_traffic(args.workload, "void adg::k_bicgA<double, false>"),
+                "traffic_source": f"profiles/pmc_bicg_{args.workload}.txt (rocprofv3 FETCH_SIZE x2 + WRITE_SIZE x1 "
+                                  f"KiB per dispatch, calibrated on a 256 MiB copy)",
                 "kernel": "k_bicgA<double> (BiCGSTAB: previous x/r update + p update + 7-point SpMV + (rs,v))",
                 "algorithmic_bytes_per_launch": k1_bytes, "avg_launch_us": round(r64["k1_avg_ms"] * 1e3, 3),
-                "samples": r64["k1_samples"], "timing": "HIP events around one steady-state KA launch per group "
-                                                        "solve, on the engine stream, in a profiled solve after the timed ones",
+                "samples": r64["k1_samples"], "timing": "HIP events attached to one steady-state KA dispatch per group "
+                                                        "solve (hipExtLaunchKernelGGL start/stop events), on the engine "
+                                                        "stream, in a profiled solve after the timed ones",
                 "group_solve_avg_us": round(r64["gsolve_avg_ms"] * 1e3, 2),
                 "nodal_update_avg_us": round(r64["nodal_avg_ms"] * 1e3, 2)}
     cpu = None
