@@ -20,6 +20,9 @@ Cores
                   and 24 axial layers of 15 cm split in 4 (68x68x96 nodes),
                   full downscatter plus upscatter into groups 6-8, nu-sigf
                   rescaled per fuel material to a sampled k-infinity.
+``batch_cores``   the batching workload: 256 uneven cores (2D and 3D, 2-4
+                  groups, 5k-60k nodes) with instance-wide and per-node
+                  cross-section perturbations (see batch_core).
 """
 
 from __future__ import annotations
@@ -189,6 +192,82 @@ def core3d_8g(seed: int = 2, **iter_kw) -> GeneratedCore:
     kw.update(iter_kw)
     p, deck = _mk_problem("core3d_8g", 8, _stack(mats), geo, bc, chis, **kw)
     return GeneratedCore(p, deck, False, seed)
+
+
+def _few_group_material(rng, fuel: bool, G: int):
+    """The 2-group material of the small cores for G = 2; for G = 3, 4 its
+    fast group split into G - 1 groups (D decreasing in [1.2, 2.0], small
+    absorption, a down-scatter chain) above the same thermal group."""
+    if G == 2:
+        return _two_group_material(rng, fuel)
+    D = list(np.sort(rng.uniform(1.2, 2.0, G - 1))[::-1]) + [rng.uniform(0.3, 0.45)]
+    sa = list(rng.uniform(0.002, 0.012, G - 1)) + [rng.uniform(0.06, 0.12) if fuel else rng.uniform(0.02, 0.04)]
+    nf = (list(rng.uniform(0.001, 0.008, G - 1)) + [rng.uniform(0.09, 0.16)]) if fuel else [0.0] * G
+    sigs = np.zeros((G, G))
+    for g in range(G - 1):
+        sigs[g, g + 1] = rng.uniform(0.015, 0.05) if g < G - 2 else rng.uniform(0.015, 0.02)
+    sigtr = [1.0 / (3.0 * d) for d in D]
+    return (sigtr, sa, nf, [x / NU for x in nf], sigs.tolist())
+
+
+def batch_core(seed: int, index: int, nin: int = 20) -> GeneratedCore:
+    """Instance `index` of the seeded batch of uneven static cores: 2D (two
+    10 cm planes, reflective top and bottom) or 3D (fuel planes of 15 cm
+    between two reflector planes), 2-4 groups, 5k-60k nodes, quarter-core
+    layout (reflective west/south), SANM.  Cross sections: the 2-group
+    ranges of the small cores (split fast groups for G > 2), an instance-wide
+    +-5% factor on the thermal absorption and thermal nu-sigf, and per-node
+    multipliers U[0.97, 1.03] on every cross section."""
+    rng = np.random.default_rng([seed, index])
+    G = int(rng.integers(2, 5))
+    three_d = bool(rng.integers(0, 2))
+    target = float(np.exp(rng.uniform(np.log(5000.0), np.log(60000.0))))
+    nx = int(rng.integers(9, 18))
+    nfuel = int(rng.integers(4, 9))
+    mats = [_few_group_material(rng, True, G) for _ in range(nfuel)]
+    mats.append(_few_group_material(rng, False, G))
+    fa, ff = rng.uniform(0.95, 1.05, 2)
+    for m in mats[:nfuel]:
+        m[1][G - 1] *= fa
+        m[2][G - 1] *= ff
+        m[3][G - 1] = m[2][G - 1] / NU
+    refl = nfuel + 1
+    fuel_pl = _quarter_layout(rng, nx, nfuel, nx - 2.5)
+    nzf = int(rng.integers(6, 25))
+    planes = nzf + 2 if three_d else 2
+
+    def nodes(d):
+        return (nx * d) ** 2 * planes
+
+    r = np.sqrt(target / planes) / nx
+    cands = [d for d in {max(1, int(np.floor(r))), max(1, int(np.ceil(r)))} if 5000 <= nodes(d) <= 60000]
+    if not cands:   # no radial split fits: keep the nearest and move the plane count
+        d = max(1, int(round(r)))
+        if three_d:
+            nzf = int(np.clip(round(target / (nx * d) ** 2) - 2, 6, 24))
+            planes = nzf + 2
+        cands = [d]
+    ndiv = min(cands, key=lambda d: abs(nodes(d) - target))
+    if three_d:
+        refl_pl = np.full((nx, nx), refl, dtype=np.int32)
+        nz = nzf + 2
+        geo = dict(xsize=[21.5] * nx, xdiv=[ndiv] * nx, ysize=[21.5] * nx, ydiv=[ndiv] * nx,
+                   zsize=[15.0] * nz, zdiv=[1] * nz, planars=[refl_pl, fuel_pl], zpln=[1] + [2] * nzf + [1])
+        bc = (1, 2, 1, 2, 1, 1)
+    else:
+        geo = dict(xsize=[21.5] * nx, xdiv=[ndiv] * nx, ysize=[21.5] * nx, ydiv=[ndiv] * nx,
+                   zsize=[10.0, 10.0], zdiv=[1, 1], planars=[fuel_pl], zpln=[1, 1])
+        bc = (1, 2, 1, 2, 2, 2)
+    chi = [[1.0] + [0.0] * (G - 1)] * (nfuel + 1)
+    kw = dict(nout=3000, nin=nin, serc=1e-5, ferc=1e-5, nac=1000, nupd=22)
+    p, deck = _mk_problem(f"batch_s{seed}_{index}", G, _stack(mats), geo, bc, chi,
+                          perturb_rng=np.random.default_rng([seed, index, 1]), perturb=0.03, **kw)
+    return GeneratedCore(p, deck, True, seed)
+
+
+def batch_cores(seed: int = 3, n: int = 256):
+    """The batching workload: `n` uneven cores of batch_core."""
+    return [batch_core(seed, i) for i in range(n)]
 
 
 CORES = {"core2d_small": core2d_small, "core3d_2g": core3d_2g, "core3d_8g": core3d_8g}

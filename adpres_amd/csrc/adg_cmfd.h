@@ -1,15 +1,27 @@
 // CMFD kernels: coupling coefficients, 7-point matrix, sources, fused
 // BiCGSTAB and the outer-iteration reductions (src/mod_cmfd.f90).
+//
+// Every kernel runs over a batch of cores (one core is a batch of one): the
+// launch table gives each block its core and its share of that core's
+// cells, and all scalars (k, BiCGSTAB coefficients, norms, the schedule)
+// are per core.  Reductions are segmented by core: a block's partial goes to
+// its slot, and a consumer block reduces only its own core's slots.
 #pragma once
 
 #include "adg_common.h"
 
 namespace adg {
 
+// the core's first block writes its per-core scalars
+__device__ __forceinline__ bool core_leader(const BlkView &v) {
+  return v.b.slot == v.b.pfirst && threadIdx.x == 0;
+}
+
 // ---------------------------------------------------------------------------
 // coup_coef (src/mod_cmfd.f90:11-137) and matrix_setup (:217-304)
 // ---------------------------------------------------------------------------
 
+// local node index t of core q (k slowest, then j, then i) -> (i, j, k) and cell
 __device__ __forceinline__ bool decode_cell(long t, const Geo &q, int &i, int &j, int &k, long &c) {
   const long nxy = (long)q.nxx * q.nyy;
   if (t >= nxy * q.nzz) return false;
@@ -17,7 +29,7 @@ __device__ __forceinline__ bool decode_cell(long t, const Geo &q, int &i, int &j
   long r = t - (long)k * nxy;
   j = (int)(r / q.nxx);
   i = (int)(r - (long)j * q.nxx);
-  c = ((long)(k + 1) * q.NYP + (j + 1)) * q.NXP + (i + 1);
+  c = q.base + ((long)(k + 1) * q.NYP + (j + 1)) * q.NXP + (i + 1);
   return true;
 }
 
@@ -34,12 +46,14 @@ __device__ __forceinline__ double df_int(double D, double Dnb, double hnb, doubl
   return 2.0 * D * Dnb / (D * hnb + Dnb * h);
 }
 
-// one thread per (cell, group); blockIdx.y = g
-__global__ void __launch_bounds__(NT) k_coupcoef(Geo q, XS xs, double *df) {
+// one thread per (node, group) of the per-node table; blockIdx.y = g
+__global__ void __launch_bounds__(NT) k_coupcoef(LV lv, XS xs, double *df) {
+  const BlkView v = blk_view(lv);
+  const Geo &q = *v.q;
   int i, j, k;
   long c;
-  long t = (long)blockIdx.x * NT + threadIdx.x;
-  if (!decode_cell(t, q, i, j, k, c)) return;
+  const long t = v.b.lo + threadIdx.x;
+  if (t >= v.b.hi || !decode_cell(t, q, i, j, k, c)) return;
   const int g = blockIdx.y;
   const long NC = q.NC;
   const unsigned char fl = q.flag[c];
@@ -51,31 +65,33 @@ __global__ void __launch_bounds__(NT) k_coupcoef(Geo q, XS xs, double *df) {
   for (int s = 0; s < 6; ++s) {
     const int u = s >> 1;
     const bool up = (s & 1) == 0;
-    double v;
+    double val;
     if (fl & f_nb(s)) {
       const long cn = up ? c + st[u] : c - st[u];
       const double hnb = (u == 0) ? q.hx[up ? i + 1 : i - 1] : (u == 1) ? q.hy[up ? j + 1 : j - 1]
                                                                          : q.hz[up ? k + 1 : k - 1];
-      v = df_int(D, Dg[cn], hnb, h[u]);
+      val = df_int(D, Dg[cn], hnb, h[u]);
     } else {
-      v = df_bnd(q.bc[s], D, h[u]);
+      val = df_bnd(q.bc[s], D, h[u]);
     }
-    df[((size_t)s * q.G + g) * NC + c] = v;
+    df[((size_t)s * q.G + g) * NC + c] = val;
   }
 }
 
 // Matrix slots per row: 0 z-, 1 y-, 2 x-, 3 diag, 4 x+, 5 y+, 6 z+.  Absent
 // neighbours (and every slot of an inactive cell) hold 0.
-// sched != 0: rebuild only in a nodal-update outer (device-side schedule)
+// sched != 0: rebuild only in a nodal-update outer of the core (device-side schedule)
 template <typename T>
-__global__ void __launch_bounds__(NT) k_matrix(Geo q, XS xs, const double *df, const double *dn,
-                                               double *A64, T *AT, const Scal *sc, int sched) {
+__global__ void __launch_bounds__(NT) k_matrix(LV lv, XS xs, const double *df, const double *dn, double *A64,
+                                               T *AT, int sched) {
 #pragma clang fp contract(off)
-  if (sched && !nodal_outer(sc)) return;
+  const BlkView v = blk_view(lv);
+  const Geo &q = *v.q;
+  if (sched && !nodal_outer(v.sc)) return;
   int i, j, k;
   long c;
-  long t = (long)blockIdx.x * NT + threadIdx.x;
-  if (!decode_cell(t, q, i, j, k, c)) return;
+  const long t = v.b.lo + threadIdx.x;
+  if (t >= v.b.hi || !decode_cell(t, q, i, j, k, c)) return;
   const int g = blockIdx.y;
   const long NC = q.NC;
   const unsigned char fl = q.flag[c];
@@ -119,26 +135,26 @@ __device__ __forceinline__ double stencil(const TA *__restrict__ A, long NC, con
   return s;
 }
 
-__global__ void __launch_bounds__(NTF) k_matvec(Geo q, const double *A64, int g, const double *x, double *y) {
-  const Range rg = block_range(q);
-  for (long c = rg.lo + threadIdx.x; c < rg.hi; c += NTF)
-    y[c] = stencil(A64 + (size_t)g * 7 * q.NC, q.NC, x, c, q.sy, q.sz);
+__global__ void __launch_bounds__(NTF) k_matvec(LV lv, const double *A64, int g, const double *x, double *y) {
+  const BlkView v = blk_view(lv);
+  for (long c = v.b.lo + threadIdx.x; c < v.b.hi; c += NTF)
+    y[c] = stencil(A64 + (size_t)g * 7 * lv.NC, lv.NC, x, c, v.b.sy, v.b.sz);
 }
 
 // ---------------------------------------------------------------------------
 // sources: TSrc / TSrcAd (src/mod_cmfd.f90:1006-1064)
 // ---------------------------------------------------------------------------
 
-__global__ void __launch_bounds__(NTF) k_src(Geo q, XS xs, const Scal *sc, int g, int adjoint, int first_group,
-                                            const double *f0, double *f0c, const double *fs0, double *fs0c,
-                                            double *b) {
+template <int TAB>
+__global__ void __launch_bounds__(NTF) k_src(LV lv, XS xs, int g, int adjoint, int first_group, const double *f0,
+                                            double *f0c, const double *fs0, double *fs0c, double *b) {
 #pragma clang fp contract(off)
-  const Range rg = block_range(q);
-  if (sc->done) return;
-  const double Ke = sc->Ke;
-  const long NC = q.NC;
-  const int G = q.G;
-  for (long c = rg.lo + threadIdx.x; c < rg.hi; c += NTF) {
+  const BlkView v = blk_view_t<TAB>(lv);
+  if (v.sc->done) return;
+  const double Ke = v.sc->Ke;
+  const long NC = lv.NC;
+  const int G = lv.G;
+  for (long c = v.b.lo + threadIdx.x; c < v.b.hi; c += NTF) {
     if (first_group) fs0c[c] = fs0[c];
     f0c[(size_t)g * NC + c] = f0[(size_t)g * NC + c];
     double s0 = 0.0;
@@ -153,40 +169,35 @@ __global__ void __launch_bounds__(NTF) k_src(Geo q, XS xs, const Scal *sc, int g
 }
 
 // ---------------------------------------------------------------------------
-// BiCGSTAB (src/mod_cmfd.f90:1203-1243), three fused stencil kernels per
-// iteration plus a setup kernel.  TV is the vector/matrix type of the
-// iteration (double, or float for the FP32 form); the residual of the setup
-// is always formed in FP64 from the FP64 matrix and FP64 flux.
+// BiCGSTAB (src/mod_cmfd.f90:1203-1243): a setup kernel, two fused stencil
+// kernels per iteration and a final update.  TV is the vector/matrix type
+// of the iteration (double, or float for the FP32 form); the residual of
+// the setup is always formed in FP64 from the FP64 matrix and FP64 flux.
 //   setup: r = b - A x (FP64), rs = r, d = 0            dot (rs, r)
-//   K1:    p = r + beta (p - omega v) at c and its 6 neighbours,
-//          v = A p                                     dot (rs, v)
-//   K2:    s = r - alpha v at c and its neighbours,
-//          t = A s                                     dots (t, t), (t, s)
-//   K3:    d += alpha p + omega s,  r = s - omega t     dot (rs, r)
 // For TV = double, d is the flux itself; for float, d is the correction that
-// the last K3 adds to the FP64 flux.
+// the final kernel adds to the FP64 flux.
 // ---------------------------------------------------------------------------
 
-template <typename TV>
-__global__ void __launch_bounds__(NTF) k_bicg_setup(Geo q, Scal *sc, const double *A64g, const double *b,
-                                                   const double *x, TV *r, TV *rs, TV *d, double *pout) {
+template <typename TV, int TAB>
+__global__ void __launch_bounds__(NTF) k_bicg_setup(LV lv, const double *A64g, const double *b, const double *x,
+                                                   TV *r, TV *rs, TV *d, double *pout) {
 #pragma clang fp contract(off)
-  const Range rg = block_range(q);
-  if (sc->done) return;
+  const BlkView v = blk_view_t<TAB>(lv);
+  if (v.sc->done) return;
   double acc[1] = {0.0};
-  for (long c = rg.lo + threadIdx.x; c < rg.hi; c += NTF) {
-    const double rr = b[c] - stencil(A64g, q.NC, x, c, q.sy, q.sz);
+  for (long c = v.b.lo + threadIdx.x; c < v.b.hi; c += NTF) {
+    const double rr = b[c] - stencil(A64g, lv.NC, x, c, v.b.sy, v.b.sz);
     const TV rv = (TV)rr;
     r[c] = rv;
     rs[c] = rv;
     if (d) d[c] = (TV)0;
     acc[0] += (double)rv * (double)rv;
   }
-  block_partial<1, 0u>(acc, pout);
-  if (blockIdx.x == 0 && threadIdx.x == 0) {   // rho = alpha = omega = 1 before iteration 1
-    sc->rho_s[0] = 1.0;
-    sc->alpha_s[0] = 1.0;
-    sc->omega_s[0] = 1.0;
+  block_partial<1, 0u>(acc, pout, v.b.slot);
+  if (core_leader(v)) {   // rho = alpha = omega = 1 before iteration 1
+    v.sc->rho_s[0] = 1.0;
+    v.sc->alpha_s[0] = 1.0;
+    v.sc->omega_s[0] = 1.0;
   }
 }
 
@@ -201,7 +212,7 @@ __global__ void __launch_bounds__(NTF) k_bicg_setup(Geo q, Scal *sc, const doubl
 //          x += alpha_i p_i + omega_i s at c; v_{i+1} = A p_{i+1}; dot (rs, v)
 //   KB(i+1): alpha_{i+1} = rho_{i+1} / (rs, v_{i+1}), ...
 //   KT:    the last iteration's x update.
-// In exact arithmetic these are the reference's iterates; the K3 update of
+// In exact arithmetic these are the reference's iterates; the x/r update of
 // the three-kernel form is recomputed where the next stencil reads it.  r, p
 // and v ping-pong between two buffers because neighbours read old values.
 //
@@ -219,34 +230,35 @@ struct StencilWaves {
   static constexpr int value = sizeof(TV) == 8 ? 4 : 8;
 };
 
-template <typename TV, bool FIRST>
+template <typename TV, bool FIRST, int TAB>
 __global__ void __launch_bounds__(NTF) __attribute__((amdgpu_waves_per_eu(1, StencilWaves<TV>::value)))
-k_bicgA(Geo q, Scal *__restrict__ sc, const TV *__restrict__ AT, int it, const TV *__restrict__ r,
-        const TV *__restrict__ p, const TV *__restrict__ v, const TV *__restrict__ t, const TV *__restrict__ rs,
-        TV *__restrict__ x, TV *__restrict__ rout, TV *__restrict__ pout, TV *__restrict__ vout,
-        const double *__restrict__ pin, double *__restrict__ partout) {
+k_bicgA(LV lv, const TV *__restrict__ AT, int it, const TV *__restrict__ r, const TV *__restrict__ p,
+        const TV *__restrict__ v, const TV *__restrict__ t, const TV *__restrict__ rs, TV *__restrict__ x,
+        TV *__restrict__ rout, TV *__restrict__ pout, TV *__restrict__ vout, const double *__restrict__ pin,
+        double *__restrict__ partout) {
 #pragma clang fp contract(off)
+  const BlkView bv = blk_view_t<TAB>(lv);
+  Scal *__restrict__ sc = bv.sc;
   if (sc->done) return;
-  const Range rg = block_range(q);
   const int prv = (it - 1) & 1, cur = it & 1;
   double rho, alpha_d = 0.0, omega_d = 0.0, beta = 0.0;
   if (FIRST) {
     double red[1];
-    reduce_partials<1, 0u>(pin, gridDim.x, red);
+    reduce_partials<1, 0u>(pin, bv.b.pfirst, bv.b.pn, red);
     rho = red[0];
   } else {
     double red[4];   // (t,t), (t,s), (rs,t), (rs,s) of iteration it-1
-    reduce_partials<4, 0u>(pin, gridDim.x, red);
+    reduce_partials<4, 0u>(pin, bv.b.pfirst, bv.b.pn, red);
     omega_d = red[1] / red[0];
     rho = red[3] - omega_d * red[2];
     alpha_d = sc->alpha_s[prv];
     beta = (rho / sc->rho_s[prv]) * (alpha_d / omega_d);
   }
   const TV ta = (TV)alpha_d, tw = (TV)omega_d, tb = (TV)beta;
-  const long NC = q.NC, sy = q.sy, sz = q.sz;
+  const long NC = lv.NC, sy = bv.b.sy, sz = bv.b.sz;
   const long off[7] = {-sz, -sy, -1, 0, 1, sy, sz};   // slot order z-, y-, x-, c, x+, y+, z+
   double acc[1] = {0.0};
-  for (long c = rg.lo + threadIdx.x; c < rg.hi; c += NTF) {
+  for (long c = bv.b.lo + threadIdx.x; c < bv.b.hi; c += NTF) {
     TV R[7], V[7], T[7], P[7], Ac[7];
 #pragma unroll
     for (int k = 0; k < 7; ++k) {
@@ -288,30 +300,30 @@ k_bicgA(Geo q, Scal *__restrict__ sc, const TV *__restrict__ AT, int it, const T
     vout[c] = vc;
     acc[0] += (double)rsc * (double)vc;
   }
-  block_partial<1, 0u>(acc, partout);
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
+  block_partial<1, 0u>(acc, partout, bv.b.slot);
+  if (core_leader(bv)) {
     sc->rho_s[cur] = rho;
     if (!FIRST) sc->omega_s[prv] = omega_d;
   }
 }
 
-template <typename TV>
+template <typename TV, int TAB>
 __global__ void __launch_bounds__(NTF) __attribute__((amdgpu_waves_per_eu(1, StencilWaves<TV>::value)))
-k_bicgB(Geo q, Scal *__restrict__ sc, const TV *__restrict__ AT, int it, const TV *__restrict__ r,
-        const TV *__restrict__ v, TV *__restrict__ t, const TV *__restrict__ rs, const double *__restrict__ pin,
-        double *__restrict__ partout) {
+k_bicgB(LV lv, const TV *__restrict__ AT, int it, const TV *__restrict__ r, const TV *__restrict__ v,
+        TV *__restrict__ t, const TV *__restrict__ rs, const double *__restrict__ pin, double *__restrict__ partout) {
 #pragma clang fp contract(off)
+  const BlkView bv = blk_view_t<TAB>(lv);
+  Scal *__restrict__ sc = bv.sc;
   if (sc->done) return;
-  const Range rg = block_range(q);
   double red[1];
-  reduce_partials<1, 0u>(pin, gridDim.x, red);
+  reduce_partials<1, 0u>(pin, bv.b.pfirst, bv.b.pn, red);
   const int cur = it & 1;
   const double alpha_d = sc->rho_s[cur] / red[0];
   const TV alpha = (TV)alpha_d;
-  const long NC = q.NC, sy = q.sy, sz = q.sz;
+  const long NC = lv.NC, sy = bv.b.sy, sz = bv.b.sz;
   const long off[7] = {-sz, -sy, -1, 0, 1, sy, sz};
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  for (long c = rg.lo + threadIdx.x; c < rg.hi; c += NTF) {
+  for (long c = bv.b.lo + threadIdx.x; c < bv.b.hi; c += NTF) {
     TV R[7], V[7], Ac[7];
 #pragma unroll
     for (int k = 0; k < 7; ++k) {
@@ -333,23 +345,23 @@ k_bicgB(Geo q, Scal *__restrict__ sc, const TV *__restrict__ AT, int it, const T
     acc[2] += rsc * (double)tc;
     acc[3] += rsc * (double)sv[3];
   }
-  block_partial<4, 0u>(acc, partout);
-  if (blockIdx.x == 0 && threadIdx.x == 0) sc->alpha_s[cur] = alpha_d;
+  block_partial<4, 0u>(acc, partout, bv.b.slot);
+  if (core_leader(bv)) sc->alpha_s[cur] = alpha_d;
 }
 
 // last iteration's x update: x += alpha p + omega s; for the FP32 form the
 // accumulated correction is then added to the FP64 flux
-template <typename TV>
-__global__ void __launch_bounds__(NTF) k_bicgT(Geo q, Scal *sc, int it, const TV *r, const TV *p, const TV *v,
-                                              TV *x, double *x64, const double *pin) {
+template <typename TV, int TAB>
+__global__ void __launch_bounds__(NTF) k_bicgT(LV lv, int it, const TV *r, const TV *p, const TV *v, TV *x,
+                                              double *x64, const double *pin) {
 #pragma clang fp contract(off)
-  if (sc->done) return;
-  const Range rg = block_range(q);
+  const BlkView bv = blk_view_t<TAB>(lv);
+  if (bv.sc->done) return;
   double red[4];
-  reduce_partials<4, 0u>(pin, gridDim.x, red);
+  reduce_partials<4, 0u>(pin, bv.b.pfirst, bv.b.pn, red);
   const double omega_d = red[1] / red[0];
-  const TV ta = (TV)sc->alpha_s[it & 1], tw = (TV)omega_d;
-  for (long c = rg.lo + threadIdx.x; c < rg.hi; c += NTF) {
+  const TV ta = (TV)bv.sc->alpha_s[it & 1], tw = (TV)omega_d;
+  for (long c = bv.b.lo + threadIdx.x; c < bv.b.hi; c += NTF) {
     const TV sv = (TV)(r[c] - ta * v[c]);
     const TV xn = (TV)(x[c] + ta * p[c] + tw * sv);
     if (x64) x64[c] = x64[c] + (double)xn;
@@ -364,13 +376,15 @@ __global__ void __launch_bounds__(NTF) k_bicgT(Geo q, Scal *sc, int it, const TV
 // (:1143).
 // ---------------------------------------------------------------------------
 
-__global__ void __launch_bounds__(NTF) k_fsrc_init(Geo q, XS xs, int adjoint, const double *f0, double *fs0) {
+// fs0 = FSrc(f0) for the cores whose outer call starts from the first-call state
+__global__ void __launch_bounds__(NTF) k_fsrc_init(LV lv, XS xs, int adjoint, const double *f0, double *fs0) {
 #pragma clang fp contract(off)
-  const Range rg = block_range(q);
-  const long NC = q.NC;
-  for (long c = rg.lo + threadIdx.x; c < rg.hi; c += NTF) {
+  const BlkView v = blk_view(lv);
+  if (!v.sc->init) return;
+  const long NC = lv.NC;
+  for (long c = v.b.lo + threadIdx.x; c < v.b.hi; c += NTF) {
     double fs = 0.0;
-    for (int g = 0; g < q.G; ++g) {
+    for (int g = 0; g < lv.G; ++g) {
       const double w = adjoint ? xs.chi[(size_t)g * NC + c] : xs.nuf[(size_t)g * NC + c];
       fs = fs + f0[(size_t)g * NC + c] * w;
     }
@@ -380,20 +394,24 @@ __global__ void __launch_bounds__(NTF) k_fsrc_init(Geo q, XS xs, int adjoint, co
 
 // f = Integrate(fs0), e1 = Integrate(1) at the start of outer (:456-458):
 // partials here, finished by k_fin_integrals
-__global__ void __launch_bounds__(NTF) k_integrals(Geo q, XS xs, const double *fs0, double *partout) {
+__global__ void __launch_bounds__(NTF) k_integrals(LV lv, XS xs, const double *fs0, double *partout) {
 #pragma clang fp contract(off)
-  const Range rg = block_range(q);
+  const BlkView v = blk_view(lv);
   double acc[2] = {0.0, 0.0};
-  for (long c = rg.lo + threadIdx.x; c < rg.hi; c += NTF) {
+  for (long c = v.b.lo + threadIdx.x; c < v.b.hi; c += NTF) {
     acc[0] += xs.vdel[c] * fs0[c];
     acc[1] += xs.vdel[c];
   }
-  block_partial<2, 0u>(acc, partout);
+  block_partial<2, 0u>(acc, partout, v.b.slot);
 }
 
-__global__ void __launch_bounds__(NTF) k_fin_integrals(Scal *sc, const double *pin, int nblk) {
+// one block per core (blockIdx.x = core)
+__global__ void __launch_bounds__(NTF) k_fin_integrals(LV lv, const double *pin) {
+  Scal *sc = lv.sc + blockIdx.x;
+  int first, n;
+  core_slots(lv, blockIdx.x, first, n);
   double red[2];
-  reduce_partials<2, 0u>(pin, nblk, red);
+  reduce_partials<2, 0u>(pin, first, n, red);
   if (threadIdx.x == 0) {
     sc->f = red[0];
     sc->e1 = red[1];
@@ -401,17 +419,16 @@ __global__ void __launch_bounds__(NTF) k_fin_integrals(Scal *sc, const double *p
 }
 
 // FSrc, errn = fs0 - fs0c, partials of |errn|^2 and of RelEg
-__global__ void __launch_bounds__(NTF) k_fsrc(Geo q, XS xs, Scal *sc, int adjoint, const double *f0,
-                                             const double *f0c, double *fs0, const double *fs0c, double *errn,
-                                             double *partout) {
+__global__ void __launch_bounds__(NTF) k_fsrc(LV lv, XS xs, int adjoint, const double *f0, const double *f0c,
+                                             double *fs0, const double *fs0c, double *errn, double *partout) {
 #pragma clang fp contract(off)
-  const Range rg = block_range(q);
-  if (sc->done) return;
-  const long NC = q.NC;
+  const BlkView v = blk_view(lv);
+  if (v.sc->done) return;
+  const long NC = lv.NC;
   double acc[2] = {0.0, 0.0};   // sum errn^2, max flux rel. error
-  for (long c = rg.lo + threadIdx.x; c < rg.hi; c += NTF) {
+  for (long c = v.b.lo + threadIdx.x; c < v.b.hi; c += NTF) {
     double fs = 0.0;
-    for (int g = 0; g < q.G; ++g) {
+    for (int g = 0; g < lv.G; ++g) {
       const double w = adjoint ? xs.chi[(size_t)g * NC + c] : xs.nuf[(size_t)g * NC + c];
       const double fg = f0[(size_t)g * NC + c];
       fs = fs + fg * w;
@@ -422,18 +439,19 @@ __global__ void __launch_bounds__(NTF) k_fsrc(Geo q, XS xs, Scal *sc, int adjoin
     errn[c] = e;
     acc[0] += e * e;
   }
-  block_partial<2, 2u>(acc, partout);
+  block_partial<2, 2u>(acc, partout, v.b.slot);
 }
 
 // e2 = l2norm(errn) and the extrapolation (fiss_extrp, :308-335) when
 // MOD(p, nac) == 0; partials of Integrate(fs0) and of RelE
-__global__ void __launch_bounds__(NTF) k_post(Geo q, XS xs, Scal *sc, double *fs0, const double *fs0c,
-                                             const double *errn, const double *pin, double *partout) {
+__global__ void __launch_bounds__(NTF) k_post(LV lv, XS xs, double *fs0, const double *fs0c, const double *errn,
+                                             const double *pin, double *partout) {
 #pragma clang fp contract(off)
-  const Range rg = block_range(q);
+  const BlkView v = blk_view(lv);
+  Scal *sc = v.sc;
   if (sc->done) return;
   double red[2];
-  reduce_partials<2, 2u>(pin, gridDim.x, red);
+  reduce_partials<2, 2u>(pin, v.b.pfirst, v.b.pn, red);
   const double e2 = sqrt(red[0]);
   const int extrap = ((sc->p + 1) % sc->nac) == 0;
   double coef = 0.0;
@@ -442,7 +460,7 @@ __global__ void __launch_bounds__(NTF) k_post(Geo q, XS xs, Scal *sc, double *fs
     coef = domiR / (1.0 - domiR);
   }
   double acc[2] = {0.0, 0.0};   // integral, max fission-source rel. error
-  for (long c = rg.lo + threadIdx.x; c < rg.hi; c += NTF) {
+  for (long c = v.b.lo + threadIdx.x; c < v.b.hi; c += NTF) {
     double fs = fs0[c];
     if (extrap) {
       fs = fs + coef * errn[c];
@@ -451,19 +469,22 @@ __global__ void __launch_bounds__(NTF) k_post(Geo q, XS xs, Scal *sc, double *fs
     acc[0] += xs.vdel[c] * fs;
     if (fabs(fs) > 1.e-10) acc[1] = fmax(acc[1], fabs(fs - fs0c[c]) / fabs(fs));
   }
-  block_partial<2, 2u>(acc, partout);
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
+  block_partial<2, 2u>(acc, partout, v.b.slot);
+  if (core_leader(v)) {
     sc->e2 = e2;
     sc->fer = red[1];
     sc->flags = extrap;
   }
 }
 
-// one block: k = k * f / fc (:484-485), ser, e1 = e2
-__global__ void __launch_bounds__(NTF) k_fin_outer(Scal *sc, const double *pin, int nblk) {
+// one block per core: k = k * f / fc (:484-485), ser, e1 = e2
+__global__ void __launch_bounds__(NTF) k_fin_outer(LV lv, const double *pin) {
+  Scal *sc = lv.sc + blockIdx.x;
   if (sc->done) return;
+  int first, n;
+  core_slots(lv, blockIdx.x, first, n);
   double red[2];
-  reduce_partials<2, 2u>(pin, nblk, red);
+  reduce_partials<2, 2u>(pin, first, n, red);
   if (threadIdx.x == 0) {
     const double fc = sc->f;
     sc->f = red[0];
@@ -473,22 +494,28 @@ __global__ void __launch_bounds__(NTF) k_fin_outer(Scal *sc, const double *pin, 
   }
 }
 
-// end of one outer: trace, error checks, convergence test (:491-496)
-__global__ void k_end(Scal *sc, double *tk, double *tser, double *tfer, double *tnd, int *tflags, int *tloc) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// end of one outer, one thread per core: trace, error checks, convergence
+// test (:491-496).  The trace arrays hold `cap` outers per core.
+__global__ void k_end(Scal *scs, int ncore, int cap, double *tk, double *tser, double *tfer, double *tnd,
+                      int *tflags, int *tloc) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= ncore) return;
+  Scal *sc = scs + b;
   if (sc->done) return;
   const int nodal = nodal_outer(sc) ? 1 : 0;
   const int p = sc->p + 1;
   const int flags = (sc->flags & 1) | (nodal ? 2 : 0);
-  const int t = p - 1;
-  tk[t] = sc->Ke;
-  tser[t] = sc->ser;
-  tfer[t] = sc->fer;
-  tnd[t] = sc->ndmax;
-  tflags[t] = flags;
-  tloc[3 * t] = sc->im;
-  tloc[3 * t + 1] = sc->jm;
-  tloc[3 * t + 2] = sc->km;
+  if (p <= cap) {
+    const size_t t = (size_t)b * cap + (p - 1);
+    tk[t] = sc->Ke;
+    tser[t] = sc->ser;
+    tfer[t] = sc->fer;
+    tnd[t] = sc->ndmax;
+    tflags[t] = flags;
+    tloc[3 * t] = sc->im;
+    tloc[3 * t + 1] = sc->jm;
+    tloc[3 * t + 2] = sc->km;
+  }
   sc->p = p;
   if (nodal && sc->lu_key != ~0ull) {
     sc->status = 3;
@@ -504,14 +531,14 @@ __global__ void k_end(Scal *sc, double *tk, double *tser, double *tfer, double *
 }
 
 // PowDis (src/mod_cmfd.f90:1290-1333): node powers, then normalisation
-__global__ void __launch_bounds__(NTF) k_powdis(Geo q, XS xs, const double *f0, double *pw, double *partout) {
+__global__ void __launch_bounds__(NTF) k_powdis(LV lv, XS xs, const double *f0, double *pw, double *partout) {
 #pragma clang fp contract(off)
-  const Range rg = block_range(q);
-  const long NC = q.NC;
+  const BlkView v = blk_view(lv);
+  const long NC = lv.NC;
   double acc[1] = {0.0};
-  for (long c = rg.lo + threadIdx.x; c < rg.hi; c += NTF) {
+  for (long c = v.b.lo + threadIdx.x; c < v.b.hi; c += NTF) {
     double pc = 0.0;
-    for (int g = 0; g < q.G; ++g) {
+    for (int g = 0; g < lv.G; ++g) {
       double w = f0[(size_t)g * NC + c] * xs.sigf[(size_t)g * NC + c] * xs.vdel[c];
       if (w < 0.0) w = 0.0;
       pc = pc + w;
@@ -519,16 +546,16 @@ __global__ void __launch_bounds__(NTF) k_powdis(Geo q, XS xs, const double *f0, 
     pw[c] = pc;
     acc[0] += pc;
   }
-  block_partial<1, 0u>(acc, partout);
+  block_partial<1, 0u>(acc, partout, v.b.slot);
 }
 
-__global__ void __launch_bounds__(NTF) k_scale(Geo q, Scal *sc, double *pw, const double *pin) {
-  const Range rg = block_range(q);
+__global__ void __launch_bounds__(NTF) k_scale(LV lv, double *pw, const double *pin) {
+  const BlkView v = blk_view(lv);
   double red[1];
-  reduce_partials<1, 0u>(pin, gridDim.x, red);
+  reduce_partials<1, 0u>(pin, v.b.pfirst, v.b.pn, red);
   const double tp = red[0];
-  if (blockIdx.x == 0 && threadIdx.x == 0) sc->tpow = tp;
-  for (long c = rg.lo + threadIdx.x; c < rg.hi; c += NTF)
+  if (core_leader(v)) v.sc->tpow = tp;
+  for (long c = v.b.lo + threadIdx.x; c < v.b.hi; c += NTF)
     pw[c] = pw[c] / tp;
 }
 

@@ -36,14 +36,45 @@ constexpr int MAXBLK = ADG_MAXBLK; // largest grid of the flat kernels
 constexpr unsigned char F_ACTIVE = 1;
 __host__ __device__ inline unsigned char f_nb(int s) { return (unsigned char)(2u << s); }
 
+// Geometry of one core.  A context solves a batch of B >= 1 cores that
+// share the group count: core b's padded box occupies cells
+// [base, base + (nzz+2) * sz) of every node-wise device array, and a group
+// (or face, or group pair) index strides by NC, the batch's total cell count.
 struct Geo {
   int nxx, nyy, nzz, G;
   int NXP, NYP;
-  long sy, sz, NC;     // strides and total cells of one group box
-  long c0, M;          // first cell of interior planes, cells in interior planes
+  long sy, sz, NC;     // strides of this core's box; NC = group stride of the batch arrays
+  long base;           // first cell of this core's box
+  long c0, M;          // first cell of its interior planes (base + sz), cells in interior planes
   int bc[6];           // per face s: boundary code (east, west, north, south, top, bottom)
+  int pfirst, pn;      // partial slots [pfirst, pfirst + pn) of its blocks in the flat launches
+  int ffirst, fnb;     // its blocks [ffirst, ffirst + fnb) of the face (nodal pass 2) launches
+  int pad0, pad1;
   const double *hx, *hy, *hz;
-  const unsigned char *flag;
+  const unsigned char *flag;   // batch-wide, indexed by cell
+};
+
+// One block's share of a launch over the batch: flat launches give it the
+// cell range [lo, hi), its partial slot, its core's partial slots
+// [pfirst, pfirst + pn) and the core's strides; per-node and per-face
+// launches give it the local node (or face) indexes [lo, hi) of its core.
+struct Blk {
+  int core, slot, pfirst, pn;
+  long lo, hi, sy, sz;
+};
+
+// What every batch kernel receives: per-core geometry and scalars, the
+// block table of the launch, and the batch-wide group stride and group
+// count.  A flat launch over a single core has no table (blk == nullptr):
+// its blocks are derived from the remaining fields, so the drop-in path
+// reads no table before its first loads.
+struct LV {
+  const Geo *geo;
+  struct Scal *sc;
+  const Blk *blk;
+  long NC;
+  int G, nb;                     // groups; single core: blocks of its flat launches
+  long c0, end, chunk, sy, sz;   // single core: interior cells [c0, end), cells per logical block, strides
 };
 
 // Device scalars of one context.  Written by block 0 of the kernel that
@@ -56,7 +87,7 @@ struct Scal {
   double Ke, f, e1, e2, ser, fer, ndmax, coef;
   double tpow;
   int p, done, status, flags;
-  int im, jm, km, pad;
+  int im, jm, km, init;   // init: the next outer call starts from the first-call state
   unsigned long long lu_key;     // first LU pivot failure (order key << 30 | packed loc)
   // iteration schedule of the running outer call (%ITER): outer p+1 extrapolates
   // when (p+1) % nac == 0 and updates the D-hats when nodal_on && (p+1) % nupd == 0
@@ -73,34 +104,64 @@ struct XS {
   const double *D, *sigr, *nuf, *sigf, *chi, *sigs, *dc, *exsrc, *vdel;
 };
 
-// Cell range of this block in the flat kernels.  The hardware hands block b
-// to XCD b % 8; with the grid a multiple of 8, logical block
-// (b % 8) * (grid / 8) + b / 8 gives every XCD one contiguous slab of the
-// box, so the z-neighbour planes of a stencil are read through that XCD's
-// own L2.  Each logical block owns `chunk` consecutive cells (a multiple of NTF).
 struct Range {
   long lo, hi;
 };
-__device__ __forceinline__ Range block_range(const Geo &q) {
-  const int nb = gridDim.x, b = blockIdx.x;
-  const int lb = (nb & 7) ? b : (b & 7) * (nb >> 3) + (b >> 3);
-  long chunk = (q.M + nb - 1) / nb;
-  chunk = (chunk + NTF - 1) / NTF * NTF;
-  Range r;
-  r.lo = q.c0 + (long)lb * chunk;
-  r.hi = r.lo + chunk;
-  const long end = q.c0 + q.M;
-  if (r.hi > end) r.hi = end;
-  if (r.lo > end) r.lo = end;
-  return r;
+
+// The block's entry of the launch table (or, for a single core, the same
+// entry computed: the hardware hands block b to XCD b % 8, and with the grid
+// a multiple of 8 logical block (b % 8) * (grid / 8) + b / 8 gives every
+// XCD one contiguous slab of the box, so the z-neighbour planes of a
+// stencil are read through that XCD's own L2) and its core.
+struct BlkView {
+  Blk b;
+  const Geo *q;
+  Scal *sc;
+};
+// TAB: the launch has a table (a batch); the hot kernels are compiled for
+// each case so that the single-core view costs no branch before its loads
+template <int TAB>   // 1 table, 0 single core, -1 decided at run time
+__device__ __forceinline__ BlkView blk_view_t(const LV &lv) {
+  BlkView v;
+  if (TAB == 1 || (TAB < 0 && lv.blk)) {
+    v.b = lv.blk[blockIdx.x];
+  } else {
+    const int nb = gridDim.x, b = blockIdx.x;
+    const int lb = (nb & 7) ? b : (b & 7) * (nb >> 3) + (b >> 3);
+    const long lo = lv.c0 + (long)lb * lv.chunk;
+    v.b.core = 0;
+    v.b.slot = b;
+    v.b.pfirst = 0;
+    v.b.pn = nb;
+    v.b.lo = lo < lv.end ? lo : lv.end;
+    v.b.hi = lo + lv.chunk < lv.end ? lo + lv.chunk : lv.end;
+    v.b.sy = lv.sy;
+    v.b.sz = lv.sz;
+  }
+  v.q = lv.geo + v.b.core;
+  v.sc = lv.sc + v.b.core;
+  return v;
+}
+__device__ __forceinline__ BlkView blk_view(const LV &lv) { return blk_view_t<-1>(lv); }
+
+// partial slots of core `core` in a per-core (grid = cores) launch
+__device__ __forceinline__ void core_slots(const LV &lv, int core, int &first, int &n) {
+  if (lv.blk) {
+    first = lv.geo[core].pfirst;
+    n = lv.geo[core].pn;
+  } else {
+    first = 0;
+    n = lv.nb;
+  }
 }
 
 // ---------------------------------------------------------------------------
 // reductions.  A reducing kernel only writes one partial per block (no
-// fences, no atomics); every block of the kernel that consumes the result
-// re-reduces all partials in the same fixed order, so each block derives
-// bit-identical scalars and the result is deterministic for a given grid.
-// Kernel boundaries order the partial stores before their readers.
+// fences, no atomics) into the block's slot; every block of the kernel that
+// consumes the result re-reduces the partials of its core's slots in the
+// same fixed order, so each block derives bit-identical scalars and the
+// result is deterministic for a given block table.  Kernel boundaries order
+// the partial stores before their readers.
 // ---------------------------------------------------------------------------
 
 template <int NV, unsigned MAXMASK>
@@ -115,7 +176,7 @@ __device__ __forceinline__ void wave_reduce(double (&v)[NV]) {
   }
 }
 
-// block-wide reduction; the result is returned in every thread
+// block-wide reduction over NTF threads; the result is returned in every thread
 template <int NV, unsigned MAXMASK>
 __device__ __forceinline__ void block_reduce(double (&v)[NV]) {
   __shared__ double sm[(NTF / 64) * NV];
@@ -130,45 +191,46 @@ __device__ __forceinline__ void block_reduce(double (&v)[NV]) {
 #pragma unroll
   for (int k = 0; k < NV; ++k) {
     double a = sm[k];
-    for (int ww = 1; ww < NTF / 64; ++ww) a = ((MAXMASK >> k) & 1u) ? fmax(a, sm[ww * NV + k]) : a + sm[ww * NV + k];
+    for (int ww = 1; ww < NTF / 64; ++ww)
+      a = ((MAXMASK >> k) & 1u) ? fmax(a, sm[ww * NV + k]) : a + sm[ww * NV + k];
     v[k] = a;
   }
 }
 
-// this block's contribution -> part[blockIdx.x * NV + k]
+// this block's contribution -> part[slot * NV + k]
 template <int NV, unsigned MAXMASK>
-__device__ __forceinline__ void block_partial(double (&v)[NV], double *part) {
+__device__ __forceinline__ void block_partial(double (&v)[NV], double *part, int slot) {
   block_reduce<NV, MAXMASK>(v);
   if (threadIdx.x == 0) {
 #pragma unroll
-    for (int k = 0; k < NV; ++k) part[(size_t)blockIdx.x * NV + k] = v[k];
+    for (int k = 0; k < NV; ++k) part[(size_t)slot * NV + k] = v[k];
   }
 }
 
-// grid-wide result of the nblk partials written by an earlier kernel.  In
-// the per-wavefront form every wavefront reduces all partials on its own
-// (same order in every wavefront, so bit-identical results) with no LDS and
-// no barrier, which lets the compiler issue the kernel's stencil loads
-// before the reduction has finished.
+// result over the partial slots [first, first + n) written by an earlier
+// kernel.  In the per-wavefront form every wavefront reduces all partials
+// on its own (same order in every wavefront, so bit-identical results) with
+// no LDS and no barrier.
 template <int NV, unsigned MAXMASK>
-__device__ __forceinline__ void reduce_partials(const double *part, int nblk, double (&out)[NV]) {
+__device__ __forceinline__ void reduce_partials(const double *part, int first, int n, double (&out)[NV]) {
 #pragma unroll
   for (int k = 0; k < NV; ++k) out[k] = 0.0;
+  const double *pp = part + (size_t)first * NV;
   if (ADG_WAVE_REDUCE) {
-    for (int b = (int)(threadIdx.x & 63); b < nblk; b += 64) {
+    for (int b = (int)(threadIdx.x & 63); b < n; b += 64) {
 #pragma unroll
       for (int k = 0; k < NV; ++k) {
-        const double x = part[(size_t)b * NV + k];
+        const double x = pp[(size_t)b * NV + k];
         out[k] = ((MAXMASK >> k) & 1u) ? fmax(out[k], x) : out[k] + x;
       }
     }
     wave_reduce<NV, MAXMASK>(out);
     return;
   }
-  for (int b = threadIdx.x; b < nblk; b += NTF) {
+  for (int b = threadIdx.x; b < n; b += NTF) {
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
-      const double x = part[(size_t)b * NV + k];
+      const double x = pp[(size_t)b * NV + k];
       out[k] = ((MAXMASK >> k) & 1u) ? fmax(out[k], x) : out[k] + x;
     }
   }

@@ -19,10 +19,10 @@
 
 namespace adg {
 
+// lv.blk is the per-node table for pass 0/1 and the per-face table for pass 2
 struct NodArgs {
-  Geo q;
+  LV lv;
   XS xs;
-  Scal *sc;
   const double *f0, *df;
   double *dn;
   double *S;            // [3][G][NC]
@@ -31,11 +31,10 @@ struct NodArgs {
   int cmode;            // 1 forward, 0 adjoint
   int sched;            // 1: run only in a nodal-update outer (device-side schedule)
   double *part;
-  unsigned *cnt;
 };
 
 __device__ __forceinline__ long cell_of(const Geo &q, int i, int j, int k) {
-  return ((long)(k + 1) * q.NYP + (j + 1)) * q.NXP + (i + 1);
+  return q.base + ((long)(k + 1) * q.NYP + (j + 1)) * q.NXP + (i + 1);
 }
 
 __device__ __forceinline__ double hdir(const Geo &q, int u, int i, int j, int k) {
@@ -47,13 +46,14 @@ __device__ __forceinline__ double hdir(const Geo &q, int u, int i, int j, int k)
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(NT) k_nod_src(NodArgs a) {
 #pragma clang fp contract(off)
-  const Geo &q = a.q;
+  const BlkView v = blk_view(a.lv);
+  const Geo &q = *v.q;
   int i, j, k;
   long c;
   const long nxy = (long)q.nxx * q.nyy;
-  long t = (long)blockIdx.x * NT + threadIdx.x;
-  if (t >= nxy * q.nzz) return;
-  if (a.sc->done || (a.sched && !nodal_outer(a.sc))) return;
+  const long t = v.b.lo + threadIdx.x;
+  if (t >= v.b.hi || t >= nxy * q.nzz) return;
+  if (v.sc->done || (a.sched && !nodal_outer(v.sc))) return;
   k = (int)(t / nxy);
   { long r = t - (long)k * nxy; j = (int)(r / q.nxx); i = (int)(r - (long)j * q.nxx); }
   c = cell_of(q, i, j, k);
@@ -96,9 +96,8 @@ __global__ void __launch_bounds__(NT) k_nod_src(NodArgs a) {
 
 // get_B (src/mod_nodal.f90:1345-1405); B row-major [g*G+h]
 template <int G>
-__device__ __forceinline__ void nod_B(const NodArgs &a, long c, double h, double Ke, double *B) {
+__device__ __forceinline__ void nod_B(const NodArgs &a, long NC, long c, double h, double Ke, double *B) {
 #pragma clang fp contract(off)
-  const long NC = a.q.NC;
   const XS &x = a.xs;
   for (int g = 0; g < G; ++g) {
     const double D = x.D[(size_t)g * NC + c];
@@ -280,11 +279,13 @@ __device__ __forceinline__ unsigned long long face_order(const Geo &q, int u, in
 template <int G>
 __global__ void __launch_bounds__(NT) k_nod_pass1(NodArgs a) {
 #pragma clang fp contract(off)
-  const Geo &q = a.q;
+  const BlkView v = blk_view(a.lv);
+  const Geo &q = *v.q;
+  Scal *sc = v.sc;
   const long nxy = (long)q.nxx * q.nyy;
-  long t = (long)blockIdx.x * NT + threadIdx.x;
-  if (t >= nxy * q.nzz) return;
-  if (a.sc->done || (a.sched && !nodal_outer(a.sc))) return;
+  const long t = v.b.lo + threadIdx.x;
+  if (t >= v.b.hi || t >= nxy * q.nzz) return;
+  if (sc->done || (a.sched && !nodal_outer(sc))) return;
   int i, j, k;
   k = (int)(t / nxy);
   { long r = t - (long)k * nxy; j = (int)(r / q.nxx); i = (int)(r - (long)j * q.nxx); }
@@ -295,13 +296,13 @@ __global__ void __launch_bounds__(NT) k_nod_pass1(NodArgs a) {
   const Line L = line_at(q, u, i, j, k, c);
   const int bclo = q.bc[2 * u + 1], bchi = q.bc[2 * u];
   if (L.first && L.last) {   // single-node line: the reference reads an undefined neighbour
-    atomicMin(&a.sc->lu_key, lu_key(0, i, j, k));
+    atomicMin(&sc->lu_key, lu_key(0, i, j, k));
     return;
   }
-  const double h = L.hn, Ke = a.sc->Ke;
+  const double h = L.hn, Ke = sc->Ke;
   double B[G * G], M[G * G], rhs[G], Lm2[G];
   Coef6 cf[G];
-  nod_B<G>(a, c, h, Ke, B);
+  nod_B<G>(a, NC, c, h, Ke, B);
   for (int g = 0; g < G; ++g) cf[g] = nod_coef(a.kern, a.xs.sigr[(size_t)g * NC + c], a.xs.D[(size_t)g * NC + c], h);
   const double *Su = a.S + (size_t)u * G * NC;
   for (int g = 0; g < G; ++g) {
@@ -318,7 +319,7 @@ __global__ void __launch_bounds__(NT) k_nod_pass1(NodArgs a) {
   if (!lu_solve<G>(M, rhs)) {
     const int fi = u == 0 ? i : (u == 1 ? j : k);
     const int o1 = u == 0 ? j : i, o2 = u == 2 ? j : k;
-    atomicMin(&a.sc->lu_key, lu_key(face_order(q, u, fi, o1, o2), i, j, k));
+    atomicMin(&sc->lu_key, lu_key(face_order(q, u, fi, o1, o2), i, j, k));
   }
   for (int g = 0; g < G; ++g) {
     double Bf = 0.0;
@@ -338,8 +339,10 @@ __global__ void __launch_bounds__(NT) k_nod_pass1(NodArgs a) {
 template <int G>
 __global__ void __launch_bounds__(NT) k_nod_pass2(NodArgs a) {
 #pragma clang fp contract(off)
-  const Geo &q = a.q;
-  if (a.sched && !nodal_outer(a.sc)) return;   // uniform over the grid
+  const BlkView v = blk_view(a.lv);
+  const Geo &q = *v.q;
+  Scal *sc = v.sc;
+  if (a.sched && !nodal_outer(sc)) return;   // uniform over the core's blocks
   const int u = blockIdx.y;
   const long NC = q.NC;
   // faces of direction u: (n_u + 1) x (other two extents)
@@ -347,9 +350,9 @@ __global__ void __launch_bounds__(NT) k_nod_pass2(NodArgs a) {
   const int e1 = u == 0 ? q.nyy : q.nxx;          // second index extent (j for x; i for y, z)
   const int e2 = u == 2 ? q.nyy : q.nzz;          // slowest (k for x, y; j for z)
   const long nf = (long)(nu + 1) * e1 * e2;
-  long t = (long)blockIdx.x * NT + threadIdx.x;
+  const long t = v.b.lo + threadIdx.x;
   double best = -1.0, bkey = 0.0, bloc = 0.0;
-  if (t < nf && !a.sc->done) {
+  if (t < nf && !sc->done) {
     const int fi = (int)(t % (nu + 1));
     const long r = t / (nu + 1);
     const int o1 = (int)(r % e1), o2 = (int)(r / e1);
@@ -363,7 +366,7 @@ __global__ void __launch_bounds__(NT) k_nod_pass2(NodArgs a) {
     const bool hasp = (q.flag[cp] & F_ACTIVE) != 0;
     const bool hasn = (q.flag[cn] & F_ACTIVE) != 0;
     if (hasp || hasn) {
-      const double Ke = a.sc->Ke;
+      const double Ke = sc->Ke;
       const int sf = 2 * u;                 // upper face of n (x+, y+, z+), sf+1 lower face of p
       const unsigned long long ord = face_order(q, u, fi, o1, o2);
       const size_t ub = (size_t)u * G * NC;
@@ -375,8 +378,8 @@ __global__ void __launch_bounds__(NT) k_nod_pass2(NodArgs a) {
       if (hasn && hasp) {
         const double hn = hdir(q, u, u == 0 ? i - 1 : i, u == 1 ? j - 1 : j, u == 2 ? k - 1 : k);
         const double hp = hdir(q, u, i, j, k);
-        nod_B<G>(a, cn, hn, Ke, Bn);
-        nod_B<G>(a, cp, hp, Ke, Bp);
+        nod_B<G>(a, NC, cn, hn, Ke, Bn);
+        nod_B<G>(a, NC, cp, hp, Ke, Bp);
         for (int g = 0; g < G; ++g) {
           cfn[g] = nod_coef(a.kern, a.xs.sigr[(size_t)g * NC + cn], a.xs.D[(size_t)g * NC + cn], hn);
           cfp[g] = nod_coef(a.kern, a.xs.sigr[(size_t)g * NC + cp], a.xs.D[(size_t)g * NC + cp], hp);
@@ -419,7 +422,7 @@ __global__ void __launch_bounds__(NT) k_nod_pass2(NodArgs a) {
           s[g + G] = dcp * (a2p + a4p + a.f0[(size_t)g * NC + cp] - cfn[g].A * Ln1) -
                      dcn * (a2n + a4n + a.f0[(size_t)g * NC + cn] + cfp[g].A * Lp1);
         }
-        if (!lu_solve<M2>(R, s)) atomicMin(&a.sc->lu_key, lu_key(ord, li, lj, lk));
+        if (!lu_solve<M2>(R, s)) atomicMin(&sc->lu_key, lu_key(ord, li, lj, lk));
         for (int g = 0; g < G; ++g) a1[g] = s[g];
         for (int g = 0; g < G; ++g) {
           double Bf = 0.0;
@@ -442,7 +445,7 @@ __global__ void __launch_bounds__(NT) k_nod_pass2(NodArgs a) {
       } else if (hasp) {
         // first node of a line (get_coefs_first, :375-407)
         const double hp = hdir(q, u, i, j, k);
-        nod_B<G>(a, cp, hp, Ke, Bp);
+        nod_B<G>(a, NC, cp, hp, Ke, Bp);
         for (int g = 0; g < G; ++g)
           cfp[g] = nod_coef(a.kern, a.xs.sigr[(size_t)g * NC + cp], a.xs.D[(size_t)g * NC + cp], hp);
         const int bc = q.bc[2 * u + 1];
@@ -466,7 +469,7 @@ __global__ void __launch_bounds__(NT) k_nod_pass2(NodArgs a) {
           else if (bc == 1) b[g] = 2.0 * Pp * (Ap * Hp * Lp1 - 3.0 * a2p - Gp * a4p) - dc * (a2p + a4p + f0p - Ap * Lp1);
           else b[g] = dc * (a2p + a4p + f0p - Ap * Lp1);
         }
-        if (!lu_solve<G>(Mx, b)) atomicMin(&a.sc->lu_key, lu_key(ord, li, lj, lk));
+        if (!lu_solve<G>(Mx, b)) atomicMin(&sc->lu_key, lu_key(ord, li, lj, lk));
         for (int g = 0; g < G; ++g) a1[g] = b[g];
         for (int g = 0; g < G; ++g) {
           double Bf = 0.0;
@@ -488,7 +491,7 @@ __global__ void __launch_bounds__(NT) k_nod_pass2(NodArgs a) {
         // last node of a line (get_coefs_last, :411-441)
         const int ni = u == 0 ? i - 1 : i, nj = u == 1 ? j - 1 : j, nk = u == 2 ? k - 1 : k;
         const double hn = hdir(q, u, ni, nj, nk);
-        nod_B<G>(a, cn, hn, Ke, Bn);
+        nod_B<G>(a, NC, cn, hn, Ke, Bn);
         for (int g = 0; g < G; ++g)
           cfn[g] = nod_coef(a.kern, a.xs.sigr[(size_t)g * NC + cn], a.xs.D[(size_t)g * NC + cn], hn);
         const int bc = q.bc[2 * u];
@@ -512,7 +515,7 @@ __global__ void __launch_bounds__(NT) k_nod_pass2(NodArgs a) {
           else if (bc == 1) b[g] = -2.0 * Pn * (An * Hn * Ln1 + 3.0 * a2n + Gn * a4n) - dc * (a2n + a4n + f0n + An * Ln1);
           else b[g] = -dc * (a2n + a4n + f0n + An * Ln1);
         }
-        if (!lu_solve<G>(Mx, b)) atomicMin(&a.sc->lu_key, lu_key(ord, li, lj, lk));
+        if (!lu_solve<G>(Mx, b)) atomicMin(&sc->lu_key, lu_key(ord, li, lj, lk));
         for (int g = 0; g < G; ++g) a1[g] = b[g];
         for (int g = 0; g < G; ++g) {
           double Bf = 0.0;
@@ -557,14 +560,21 @@ __global__ void __launch_bounds__(NT) k_nod_pass2(NodArgs a) {
   }
 }
 
-// final argmax over the pass-2 partials: one block, sweep order x, y, z
-__global__ void __launch_bounds__(NT) k_nod_argmax(Scal *sc, const double *part, int nslots, int sched) {
+// final argmax over the pass-2 partials, one block per core (blockIdx.x =
+// core): its face blocks [ffirst, ffirst + fnb) of each direction, nfb face
+// blocks per direction in the launch
+__global__ void __launch_bounds__(NT) k_nod_argmax(LV lv, const double *part, int nfb, int sched) {
+  const Geo &q = lv.geo[blockIdx.x];
+  Scal *sc = lv.sc + blockIdx.x;
   if (sc->done || (sched && !nodal_outer(sc))) return;
   __shared__ double sv[NT], sk[NT], sl[NT];
   double best = -1.0, bkey = 0.0, bloc = 0.0;
+  const int nslots = 3 * q.fnb;
   for (int s = threadIdx.x; s < nslots; s += NT) {
-    const double v = part[3 * s], kk = part[3 * s + 1];
-    if (v > best || (v == best && kk < bkey)) { best = v; bkey = kk; bloc = part[3 * s + 2]; }
+    const int u = s / q.fnb, fb = s - u * q.fnb;
+    const size_t o = ((size_t)u * nfb + q.ffirst + fb) * 3;
+    const double v = part[o], kk = part[o + 1];
+    if (v > best || (v == best && kk < bkey)) { best = v; bkey = kk; bloc = part[o + 2]; }
   }
   sv[threadIdx.x] = best; sk[threadIdx.x] = bkey; sl[threadIdx.x] = bloc;
   __syncthreads();

@@ -35,6 +35,8 @@ KERNS = {"FDM": 0, "PNM": 1, "SANM": 2}
 EXPORTS = ("adg_create", "adg_destroy", "adg_set_xs", "adg_set_state", "adg_get_state", "adg_outer",
            "adg_powdis", "adg_nodal_update", "adg_bicg", "adg_matvec", "adg_last_timing",
            "adg_set_profiling", "adg_read_profile",
+           "adg_create_batch", "adg_core_count", "adg_set_xs_core", "adg_set_state_core", "adg_get_state_core",
+           "adg_outer_batch", "adg_trace_core", "adg_powdis_core",
            "adg_status_string", "adg_last_error")
 
 _dp = C.POINTER(C.c_double)
@@ -91,6 +93,14 @@ def load_library(path: str = LIB_PATH):
     L.adg_last_timing.argtypes = [C.c_void_p, _dp]
     L.adg_set_profiling.argtypes = [C.c_void_p, C.c_int]
     L.adg_read_profile.argtypes = [C.c_void_p, _dp]
+    L.adg_create_batch.argtypes = [C.POINTER(AdgGeom), C.c_int, C.POINTER(C.c_void_p)]
+    L.adg_core_count.argtypes = [C.c_void_p]
+    L.adg_set_xs_core.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _ip, _dp, _dp]
+    L.adg_set_state_core.argtypes = [C.c_void_p, C.c_int, _dp, _dp, C.c_double, _dp, C.c_int]
+    L.adg_get_state_core.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp, _dp, _dp]
+    L.adg_outer_batch.argtypes = [C.c_void_p, C.c_int, C.POINTER(AdgIter), _ip, _ip, _dp]
+    L.adg_trace_core.argtypes = [C.c_void_p, C.c_int, C.POINTER(AdgTrace)]
+    L.adg_powdis_core.argtypes = [C.c_void_p, C.c_int, _dp]
     L.adg_status_string.argtypes = [C.c_int]
     L.adg_status_string.restype = C.c_char_p
     L.adg_last_error.argtypes = [C.c_void_p]
@@ -107,6 +117,29 @@ def _i(a):
     return a.ctypes.data_as(_ip)
 
 
+def _geom(p: Problem, precision: str, keep: dict):
+    ix = np.ascontiguousarray(p.ix, np.int32); iy = np.ascontiguousarray(p.iy, np.int32)
+    iz = np.ascontiguousarray(p.iz, np.int32)
+    xd = np.ascontiguousarray(p.xdel, np.float64); yd = np.ascontiguousarray(p.ydel, np.float64)
+    zd = np.ascontiguousarray(p.zdel, np.float64)
+    keep.setdefault("geom", []).append((ix, iy, iz, xd, yd, zd))
+    return AdgGeom(p.nnod, p.ng, p.nmat, p.nxx, p.nyy, p.nzz, _i(ix), _i(iy), _i(iz), _d(xd), _d(yd), _d(zd),
+                   (C.c_int * 6)(*p.bc), PRECISIONS[precision])
+
+
+def _xs_args(p: Problem):
+    arrs = {k: np.ascontiguousarray(getattr(p, k), np.float64)
+            for k in ("D", "sigr", "sigs", "nuf", "sigf", "chi", "dc", "exsrc")}
+    mat = np.ascontiguousarray(p.mat, np.int32)
+    return (arrs, mat), (_d(arrs["D"]), _d(arrs["sigr"]), _d(arrs["sigs"]), _d(arrs["nuf"]), _d(arrs["sigf"]),
+                         _d(arrs["chi"]), _i(mat), _d(arrs["dc"]), _d(arrs["exsrc"]))
+
+
+def _iter(p: Problem, ov: dict):
+    return AdgIter(ov.get("nout", p.nout), ov.get("nin", p.nin), ov.get("nac", p.nac), ov.get("nupd", p.nupd),
+                   KERNS[ov.get("kern", p.kern)], ov.get("serc", p.serc), ov.get("ferc", p.ferc))
+
+
 class Engine:
     """One core on one GPU behind the C-ABI."""
 
@@ -115,13 +148,7 @@ class Engine:
         self.p = p
         self.precision = precision
         self._keep = {}
-        ix = np.ascontiguousarray(p.ix, np.int32); iy = np.ascontiguousarray(p.iy, np.int32)
-        iz = np.ascontiguousarray(p.iz, np.int32)
-        xd = np.ascontiguousarray(p.xdel, np.float64); yd = np.ascontiguousarray(p.ydel, np.float64)
-        zd = np.ascontiguousarray(p.zdel, np.float64)
-        self._keep.update(ix=ix, iy=iy, iz=iz, xd=xd, yd=yd, zd=zd)
-        g = AdgGeom(p.nnod, p.ng, p.nmat, p.nxx, p.nyy, p.nzz, _i(ix), _i(iy), _i(iz), _d(xd), _d(yd), _d(zd),
-                    (C.c_int * 6)(*p.bc), PRECISIONS[precision])
+        g = _geom(p, precision, self._keep)
         h = C.c_void_p()
         rc = self.lib.adg_create(C.byref(g), C.byref(h))
         if rc != ADG_OK:
@@ -150,12 +177,8 @@ class Engine:
 
     # -- data ------------------------------------------------------------
     def set_xs(self, p: Problem):
-        arrs = {k: np.ascontiguousarray(getattr(p, k), np.float64)
-                for k in ("D", "sigr", "sigs", "nuf", "sigf", "chi", "dc", "exsrc")}
-        mat = np.ascontiguousarray(p.mat, np.int32)
-        rc = self.lib.adg_set_xs(self.h, _d(arrs["D"]), _d(arrs["sigr"]), _d(arrs["sigs"]), _d(arrs["nuf"]),
-                                 _d(arrs["sigf"]), _d(arrs["chi"]), _i(mat), _d(arrs["dc"]), _d(arrs["exsrc"]))
-        self._check(rc, "adg_set_xs")
+        _, args = _xs_args(p)
+        self._check(self.lib.adg_set_xs(self.h, *args), "adg_set_xs")
 
     def reset(self):
         """Start the next outer() from f0 = 1, k = 1, D-hat = 0 (the reference's first call)."""
@@ -182,8 +205,7 @@ class Engine:
     # -- hot path --------------------------------------------------------
     def outer(self, adjoint: Optional[bool] = None, raise_on_error: bool = True, **ov):
         p = self.p
-        it = AdgIter(ov.get("nout", p.nout), ov.get("nin", p.nin), ov.get("nac", p.nac), ov.get("nupd", p.nupd),
-                     KERNS[ov.get("kern", p.kern)], ov.get("serc", p.serc), ov.get("ferc", p.ferc))
+        it = _iter(p, ov)
         if adjoint is None:
             adjoint = p.mode == "ADJOINT"
         cap = it.nout
@@ -236,3 +258,87 @@ class Engine:
         self._check(self.lib.adg_read_profile(self.h, _d(t)), "adg_read_profile")
         return dict(k1_ms=t[0], k1_n=int(t[1]), gsolve_ms=t[2], gsolve_n=int(t[3]), nodal_ms=t[4],
                     nodal_n=int(t[5]))
+
+
+class Batch:
+    """Many independent cores with one group count in one context: every
+    kernel launch and every replay of the captured outer iteration covers all
+    cores, and each core keeps its own k, schedule, convergence test and
+    status (the batching form; the reference solves one core per process).
+    The %ITER of the first problem (or the overrides) applies to all cores."""
+
+    def __init__(self, problems, precision: str = "fp64"):
+        self.lib = load_library()
+        self.problems = list(problems)
+        if not self.problems or len({q.ng for q in self.problems}) != 1:
+            raise ValueError("a batch needs at least one core and one group count")
+        self.precision = precision
+        self._keep = {}
+        geoms = (AdgGeom * len(self.problems))(*[_geom(q, precision, self._keep) for q in self.problems])
+        h = C.c_void_p()
+        rc = self.lib.adg_create_batch(geoms, len(self.problems), C.byref(h))
+        if rc != ADG_OK:
+            raise ADGError(rc, f"adg_create_batch failed: {self.lib.adg_status_string(rc).decode()}")
+        self.h = h
+        for b, q in enumerate(self.problems):
+            _, args = _xs_args(q)
+            self._check(self.lib.adg_set_xs_core(self.h, b, *args), "adg_set_xs_core")
+        self.status = self.niter = self.keff = None
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.adg_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc: int, what: str):
+        if rc != ADG_OK:
+            msg = self.lib.adg_last_error(self.h).decode()
+            raise ADGError(rc, f"{what}: {self.lib.adg_status_string(rc).decode()} ({msg})")
+
+    def __len__(self):
+        return len(self.problems)
+
+    def reset(self):
+        for b in range(len(self)):
+            self._check(self.lib.adg_set_state_core(self.h, b, None, None, 1.0, None, 1), "adg_set_state_core")
+
+    def outer(self, adjoint: bool = False, **ov):
+        """Solve every core; returns (status, outers, k) arrays.  Per-core
+        failures are reported in `status`, not raised."""
+        it = _iter(self.problems[0], ov)
+        n = len(self)
+        st = np.zeros(n, np.int32); ni = np.zeros(n, np.int32); ke = np.zeros(n)
+        rc = self.lib.adg_outer_batch(self.h, 1 if adjoint else 0, C.byref(it), _i(st), _i(ni), _d(ke))
+        if rc in (ADG_E_ARG, ADG_E_NOT_SET, ADG_E_HIP):
+            self._check(rc, "adg_outer_batch")
+        self.status, self.niter, self.keff = st, ni, ke
+        return st, ni, ke
+
+    def powdis(self, b: int) -> np.ndarray:
+        out = np.zeros(self.problems[b].nnod)
+        self._check(self.lib.adg_powdis_core(self.h, b, _d(out)), "adg_powdis_core")
+        return out
+
+    def state(self, b: int):
+        q = self.problems[b]
+        N, G = q.nnod, q.ng
+        f0 = np.zeros((G, N)); fs0 = np.zeros(N); dn = np.zeros((6, G, N)); df = np.zeros((6, G, N))
+        Ke = C.c_double()
+        self._check(self.lib.adg_get_state_core(self.h, b, _d(f0), _d(fs0), C.byref(Ke), _d(dn), _d(df)),
+                    "adg_get_state_core")
+        return dict(f0=f0, fs0=fs0, Ke=Ke.value, dn=dn, df=df)
+
+    def trace(self, b: int, cap: int):
+        t = {k: np.zeros(cap) for k in ("k", "ser", "fer", "ndmax")}
+        t["flags"] = np.zeros(cap, np.int32)
+        t["loc"] = np.zeros(3 * cap, np.int32)
+        tr = AdgTrace(cap, 0, _d(t["k"]), _d(t["ser"]), _d(t["fer"]), _d(t["ndmax"]), _i(t["flags"]), _i(t["loc"]))
+        self._check(self.lib.adg_trace_core(self.h, b, C.byref(tr)), "adg_trace_core")
+        n = tr.n
+        return {k: (v[:n] if k != "loc" else v[:3 * n].reshape(-1, 3)) for k, v in t.items()}

@@ -126,6 +126,33 @@ int adg_last_timing(adg_ctx *ctx, double *ms4);
 int adg_set_profiling(adg_ctx *ctx, int on);
 int adg_read_profile(adg_ctx *ctx, double *out6);
 
+/* Batching form.  The reference solves one core per process; these entry
+ * points have no counterpart there and serve callers that solve many
+ * independent cores at once (perturbed or differently sized cores, same
+ * group count and precision form): all cores of a batch advance together
+ * through the same kernels, one hipGraph replay per outer iteration covering
+ * every core, and each core keeps its own k, schedule, convergence test and
+ * status exactly as cmfd::outer has them for one core.
+ * adg_create_batch takes an array of ncore geometries; the *_core calls are
+ * the single-core calls above for core `core` (0-based).  The single-core
+ * calls act on core 0; adg_nodal_update, adg_bicg and adg_matvec need a
+ * single-core context. */
+int adg_create_batch(const adg_geom *geoms, int ncore, adg_ctx **ctx);
+int adg_core_count(adg_ctx *ctx);
+int adg_set_xs_core(adg_ctx *ctx, int core, const double *D, const double *sigr, const double *sigs,
+                    const double *nuf, const double *sigf, const double *chi, const int *mat,
+                    const double *dc, const double *exsrc);
+int adg_set_state_core(adg_ctx *ctx, int core, const double *f0, const double *fs0, double Ke,
+                       const double *dn, int first);
+int adg_get_state_core(adg_ctx *ctx, int core, double *f0, double *fs0, double *Ke, double *dn,
+                       double *df);
+/* cmfd::outer / outer_ad for every core of the batch with one %ITER; per
+ * core: status (enum adg_status), outers performed and k (arrays of ncore,
+ * each may be NULL).  Returns the first non-OK core status. */
+int adg_outer_batch(adg_ctx *ctx, int mode, const adg_iter *iter, int *status, int *niter, double *keff);
+int adg_trace_core(adg_ctx *ctx, int core, adg_trace *trace);
+int adg_powdis_core(adg_ctx *ctx, int core, double *p);
+
 const char *adg_status_string(int status);
 const char *adg_last_error(adg_ctx *ctx);
 

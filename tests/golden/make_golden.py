@@ -126,6 +126,56 @@ def synthetic(name, seed, full=True, workdir=None):
          dn=res["dn"] if full else res["dn"][:, :, ::97], **trace_arrays(out))
 
 
+def _batch_one(args):
+    """Reference solve of one instance of the batching workload (worker);
+    results are cached under `cache` so an interrupted run resumes."""
+    seed, i, cache = args
+    path = os.path.join(cache, f"{i}.npz")
+    if os.path.exists(path):
+        z = np.load(path)
+        return i, float(z["Ke"]), int(z["niter"]), str(z["sha"]), z["pow"], str(z["msg"])
+    from adpres_amd.configs import batch_core
+    gc = batch_core(seed, i)
+    p = gc.problem
+    with tempfile.TemporaryDirectory() as d:
+        it = (p.nout, p.nin, p.serc, p.ferc, p.nac, p.nupd)
+        write_deck(os.path.join(d, "deck"), kern=p.kern, iter_card=it, **gc.deck)
+        write_xs_override(os.path.join(d, "xs.bin"), p)
+        try:
+            q, res, out = run(d, "deck", "solve", "xs.bin")
+        except (RuntimeError, ValueError) as e:
+            log = open(os.path.join(d, "deck.out")).read()[-600:] if os.path.exists(os.path.join(d, "deck.out")) else ""
+            msg = (str(e)[-600:] + log).replace("\n", " | ")
+            np.savez(path, Ke=np.nan, niter=-1, sha=sha(p), pow=np.zeros(0, np.float32), msg=msg)
+            return i, float("nan"), -1, sha(p), np.zeros(0, np.float32), msg
+    for k in ("ix", "iy", "iz", "mat", "sigtr", "siga", "nuf", "sigf", "sigs", "D", "sigr"):
+        assert np.array_equal(getattr(p, k), getattr(q, k)), (i, k)
+    n = len(parse_trace(out))
+    pw = res["pow"].astype(np.float32)
+    np.savez(path, Ke=res["Ke"], niter=n, sha=sha(p), pow=pw, msg="")
+    return i, float(res["Ke"]), n, sha(p), pw, ""
+
+
+def batch(seed=3, n=256, workers=7, sample=8, cache="/tmp/adg_batch_golden"):
+    """The batching workload: reference k and outer count of every instance
+    (NaN and -1 where the reference stops without a result, with its message
+    printed), node powers (float32) of `sample` seeded converged instances."""
+    from multiprocessing import Pool
+    os.makedirs(cache, exist_ok=True)
+    with Pool(workers) as pool:
+        out = sorted(pool.map(_batch_one, [(seed, i, cache) for i in range(n)], chunksize=1))
+    for o in out:
+        if o[2] < 0:
+            print(f"core {o[0]}: reference stopped: {o[5][-400:]}")
+    ok = np.array([o[2] >= 0 for o in out])
+    pick = np.sort(np.random.default_rng(seed).choice(np.nonzero(ok)[0], sample, replace=False))
+    arrs = dict(seed=seed, n=n, Ke=np.array([o[1] for o in out]), niter=np.array([o[2] for o in out], np.int32),
+                problem_sha=np.array([o[3] for o in out]), pow_cores=pick.astype(np.int32))
+    for i in pick:
+        arrs[f"pow_{int(i)}"] = out[i][4]
+    save(f"batch_s{seed}_solve", **arrs)
+
+
 def main(names):
     if not os.path.exists(DUMP):
         raise SystemExit("build the reference first: make -C oracle ref")
@@ -139,6 +189,8 @@ def main(names):
         synthetic("core2d_small", 1)
     if not names or "core3d_2g" in names:
         synthetic("core3d_2g", 1, full=False)
+    if "batch" in names:          # 256 reference solves on 7 host cores
+        batch()
     if "core3d_8g" in names:      # about an hour of reference CPU time
         synthetic("core3d_8g", 2, full=False, workdir=os.environ.get("CORE3D_8G_RUN"))
 

@@ -59,3 +59,15 @@ def test_generators_reproduce_the_fixture_cores():
     ref = problem_from_dict(d2)
     for k in ("ix", "iy", "iz", "mat", "sigtr", "sigs", "D", "sigr"):
         assert np.array_equal(getattr(q, k), getattr(ref, k)), k
+
+
+def test_batch_needs_one_group_count_without_touching_the_gpu():
+    import pytest
+    from adpres_amd.configs import batch_core
+    from adpres_amd.engine import Batch
+    cores = [batch_core(3, i).problem for i in range(6)]
+    assert len({q.ng for q in cores}) > 1
+    with pytest.raises(ValueError):
+        Batch(cores)
+    with pytest.raises(ValueError):
+        Batch([])

@@ -1,0 +1,91 @@
+"""Batching form: many independent cores in one context (adg_create_batch),
+every launch and every graph replay covering all of them.
+
+Bars: a batch of one core is the drop-in path itself (bit-identical); a
+batch of uneven cores gives every core what its own single-core solve gives
+up to the summation order of the reductions (FP64: k within 1e-9, node
+powers within 1e-7 relative; FP32+refinement, whose FP32 iterates carry
+that order into the stopping outer: k within 1e-7, powers within 1e-5;
+outer count within 2 for both); over the 256-core workload every
+core's k is within 1 pcm of the reference's own FP64 solve of that core, and
+the sampled cores' node powers within 1e-4 relative, in FP64 and in
+FP32+refinement.
+"""
+
+import numpy as np
+import pytest
+
+from conftest import golden
+from adpres_amd.configs import batch_core
+from adpres_amd.engine import ADG_OK, Batch, Engine
+
+pytestmark = pytest.mark.gpu
+
+SEED = 3
+
+
+def rel_err(p, ref):
+    m = ref > 0
+    return float(np.max(np.abs(p[m] - ref[m]) / ref[m]))
+
+
+def test_gpu_batch_of_one_is_the_drop_in_path():
+    p = batch_core(SEED, 0).problem
+    e = Engine(p)
+    e.outer()
+    k1, n1, p1 = e.Ke, e.niter, e.powdis()
+    e.close()
+    b = Batch([p])
+    st, ni, ke = b.outer()
+    assert st[0] == ADG_OK and ni[0] == n1 and ke[0] == k1
+    assert np.array_equal(b.powdis(0), p1)
+    b.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32_refine"])
+def test_gpu_batch_matches_single_core_solves(precision):
+    cores = [c.problem for c in (batch_core(SEED, i) for i in range(24)) if c.problem.ng == 2][:5]
+    assert len(cores) == 5 and len({q.nzz > 2 for q in cores}) == 2   # 2D and 3D, uneven sizes
+    single = []
+    for q in cores:
+        e = Engine(q, precision)
+        e.outer()
+        single.append((e.Ke, e.niter, e.powdis()))
+        e.close()
+    ktol, ptol = (1e-9, 1e-7) if precision == "fp64" else (1e-7, 1e-5)
+    b = Batch(cores, precision)
+    st, ni, ke = b.outer()
+    assert np.all(st == ADG_OK)
+    for j, (k, n, pw) in enumerate(single):
+        assert abs(ke[j] - k) < ktol, j
+        assert abs(int(ni[j]) - n) <= 2, j
+        assert rel_err(b.powdis(j), pw) < ptol, j
+    # a second call resumes every core from its converged state
+    st2, ni2, ke2 = b.outer()
+    assert np.all(st2 == ADG_OK) and np.all(ni2 < ni // 4 + 2)
+    assert np.max(np.abs(ke2 - ke)) < 1e-5
+    b.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32_refine"])
+def test_gpu_batch256_matches_reference(precision):
+    d = golden(f"batch_s{SEED}_solve")
+    n = int(d["n"])
+    cores = [batch_core(SEED, i).problem for i in range(n)]
+    keff = np.zeros(n)
+    powers = {}
+    sample = [int(i) for i in d["pow_cores"]]
+    for G in sorted({q.ng for q in cores}):
+        idx = [i for i in range(n) if cores[i].ng == G]
+        b = Batch([cores[i] for i in idx], precision)
+        st, ni, ke = b.outer()
+        assert np.all(st == ADG_OK), (G, st)
+        keff[idx] = ke
+        for j, i in enumerate(idx):
+            if i in sample:
+                powers[i] = b.powdis(j)
+        b.close()
+    err = np.abs(keff - d["Ke"])
+    assert err.max() < 1e-5, (int(err.argmax()), float(err.max()))
+    for i in sample:
+        assert rel_err(powers[i], d[f"pow_{i}"].astype(np.float64)) < 1e-4, i
