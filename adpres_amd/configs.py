@@ -21,8 +21,9 @@ Cores
                   full downscatter plus upscatter into groups 6-8, nu-sigf
                   rescaled per fuel material to a sampled k-infinity.
 ``batch_cores``   the batching workload: 256 uneven cores (2D and 3D, 2-4
-                  groups, 5k-60k nodes) with instance-wide and per-node
-                  cross-section perturbations (see batch_core).
+                  groups, 5k-60k nodes, each with its own nodal-update
+                  interval) with instance-wide and per-node cross-section
+                  perturbations (see batch_core).
 """
 
 from __future__ import annotations
@@ -212,17 +213,29 @@ def _few_group_material(rng, fuel: bool, G: int):
 
 def batch_core(seed: int, index: int, nin: int = 20) -> GeneratedCore:
     """Instance `index` of the seeded batch of uneven static cores: 2D (two
-    10 cm planes, reflective top and bottom) or 3D (fuel planes of 15 cm
-    between two reflector planes), 2-4 groups, 5k-60k nodes, quarter-core
-    layout (reflective west/south), SANM.  Cross sections: the 2-group
-    ranges of the small cores (split fast groups for G > 2), an instance-wide
-    +-5% factor on the thermal absorption and thermal nu-sigf, and per-node
-    multipliers U[0.97, 1.03] on every cross section."""
+    10 cm planes, reflective top and bottom; 17-20 assemblies across) or 3D
+    (9-17 assemblies across, 6-24 fuel planes of 15 cm between two reflector
+    planes), 2-3 radial nodes per 21.5 cm assembly, 2-4 groups, 5k-60k nodes,
+    quarter-core layout (reflective west/south), SANM with the reference's
+    default nodal-update interval ceil((nxx+nyy+nzz)/2.5).  Cross sections:
+    the 2-group ranges of the small cores (split fast groups for G > 2), an
+    instance-wide +-5% factor on the thermal absorption and thermal
+    nu-sigf, and per-node multipliers U[0.97, 1.03] on every cross section."""
     rng = np.random.default_rng([seed, index])
     G = int(rng.integers(2, 5))
     three_d = bool(rng.integers(0, 2))
     target = float(np.exp(rng.uniform(np.log(5000.0), np.log(60000.0))))
-    nx = int(rng.integers(9, 18))
+    if three_d:
+        shapes = [(nx, d, nzf) for nx in range(9, 18) for d in range(2, 4) for nzf in range(6, 25)]
+        nodes = np.array([(nx * d) ** 2 * (nzf + 2) for nx, d, nzf in shapes], float)
+    else:
+        shapes = [(nx, d, 0) for nx in range(17, 21) for d in range(2, 4)]
+        nodes = np.array([(nx * d) ** 2 * 2 for nx, d, _ in shapes], float)
+    ok = (nodes >= 5000) & (nodes <= 60000)
+    near = np.nonzero(ok & (np.abs(nodes - target) <= 0.15 * target))[0]
+    if len(near) == 0:
+        near = [int(np.argmin(np.where(ok, np.abs(nodes - target), np.inf)))]
+    nx, ndiv, nzf = shapes[int(rng.choice(near))]
     nfuel = int(rng.integers(4, 9))
     mats = [_few_group_material(rng, True, G) for _ in range(nfuel)]
     mats.append(_few_group_material(rng, False, G))
@@ -233,21 +246,6 @@ def batch_core(seed: int, index: int, nin: int = 20) -> GeneratedCore:
         m[3][G - 1] = m[2][G - 1] / NU
     refl = nfuel + 1
     fuel_pl = _quarter_layout(rng, nx, nfuel, nx - 2.5)
-    nzf = int(rng.integers(6, 25))
-    planes = nzf + 2 if three_d else 2
-
-    def nodes(d):
-        return (nx * d) ** 2 * planes
-
-    r = np.sqrt(target / planes) / nx
-    cands = [d for d in {max(1, int(np.floor(r))), max(1, int(np.ceil(r)))} if 5000 <= nodes(d) <= 60000]
-    if not cands:   # no radial split fits: keep the nearest and move the plane count
-        d = max(1, int(round(r)))
-        if three_d:
-            nzf = int(np.clip(round(target / (nx * d) ** 2) - 2, 6, 24))
-            planes = nzf + 2
-        cands = [d]
-    ndiv = min(cands, key=lambda d: abs(nodes(d) - target))
     if three_d:
         refl_pl = np.full((nx, nx), refl, dtype=np.int32)
         nz = nzf + 2
@@ -255,11 +253,13 @@ def batch_core(seed: int, index: int, nin: int = 20) -> GeneratedCore:
                    zsize=[15.0] * nz, zdiv=[1] * nz, planars=[refl_pl, fuel_pl], zpln=[1] + [2] * nzf + [1])
         bc = (1, 2, 1, 2, 1, 1)
     else:
+        nz = 2
         geo = dict(xsize=[21.5] * nx, xdiv=[ndiv] * nx, ysize=[21.5] * nx, ydiv=[ndiv] * nx,
                    zsize=[10.0, 10.0], zdiv=[1, 1], planars=[fuel_pl], zpln=[1, 1])
         bc = (1, 2, 1, 2, 2, 2)
     chi = [[1.0] + [0.0] * (G - 1)] * (nfuel + 1)
-    kw = dict(nout=3000, nin=nin, serc=1e-5, ferc=1e-5, nac=1000, nupd=22)
+    nupd = int(np.ceil((2 * nx * ndiv + nz) / 2.5))
+    kw = dict(nout=3000, nin=nin, serc=1e-5, ferc=1e-5, nac=1000, nupd=nupd)
     p, deck = _mk_problem(f"batch_s{seed}_{index}", G, _stack(mats), geo, bc, chi,
                           perturb_rng=np.random.default_rng([seed, index, 1]), perturb=0.03, **kw)
     return GeneratedCore(p, deck, True, seed)

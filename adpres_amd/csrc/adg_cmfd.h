@@ -80,10 +80,20 @@ __global__ void __launch_bounds__(NT) k_coupcoef(LV lv, XS xs, double *df) {
 
 // Matrix slots per row: 0 z-, 1 y-, 2 x-, 3 diag, 4 x+, 5 y+, 6 z+.  Absent
 // neighbours (and every slot of an inactive cell) hold 0.
-// sched != 0: rebuild only in a nodal-update outer of the core (device-side schedule)
+// sched != 0: rebuild only in a nodal-update outer of the core (device-side schedule).
+// transpose != 0: the transposed matrix, row n slot s = the forward matrix's
+// row m (the neighbour across face s) slot opposite(s), evaluated from m's
+// own face coefficients so that it is the exact transpose.
+__device__ __forceinline__ double offdiag(const double *df, const double *dn, int G, long NC, int g, long c, int s,
+                                          double h) {
+#pragma clang fp contract(off)
+  const double f = df[((size_t)s * G + g) * NC + c], d = dn[((size_t)s * G + g) * NC + c];
+  return (s & 1) ? -(f - d) / h : -(f + d) / h;   // lower faces 1/3/5: -(df - dn)/h; upper: -(df + dn)/h
+}
+
 template <typename T>
 __global__ void __launch_bounds__(NT) k_matrix(LV lv, XS xs, const double *df, const double *dn, double *A64,
-                                               T *AT, int sched) {
+                                               T *AT, int sched, int transpose) {
 #pragma clang fp contract(off)
   const BlkView v = blk_view(lv);
   const Geo &q = *v.q;
@@ -94,23 +104,34 @@ __global__ void __launch_bounds__(NT) k_matrix(LV lv, XS xs, const double *df, c
   if (t >= v.b.hi || !decode_cell(t, q, i, j, k, c)) return;
   const int g = blockIdx.y;
   const long NC = q.NC;
+  const int G = q.G;
   const unsigned char fl = q.flag[c];
   double a[7] = {0, 0, 0, 0, 0, 0, 0};
   if (fl & F_ACTIVE) {
     double f[6], d[6];
     for (int s = 0; s < 6; ++s) {
-      f[s] = df[((size_t)s * q.G + g) * NC + c];
-      d[s] = dn[((size_t)s * q.G + g) * NC + c];
+      f[s] = df[((size_t)s * G + g) * NC + c];
+      d[s] = dn[((size_t)s * G + g) * NC + c];
     }
     const double hx = q.hx[i], hy = q.hy[j], hz = q.hz[k];
-    if (fl & f_nb(5)) a[0] = -(f[5] - d[5]) / hz;
-    if (fl & f_nb(3)) a[1] = -(f[3] - d[3]) / hy;
-    if (fl & f_nb(1)) a[2] = -(f[1] - d[1]) / hx;
     a[3] = (f[0] + f[1] - d[0] + d[1]) / hx + (f[2] + f[3] - d[2] + d[3]) / hy +
            (f[4] + f[5] - d[4] + d[5]) / hz + xs.sigr[(size_t)g * NC + c];
-    if (fl & f_nb(0)) a[4] = -(f[0] + d[0]) / hx;
-    if (fl & f_nb(2)) a[5] = -(f[2] + d[2]) / hy;
-    if (fl & f_nb(4)) a[6] = -(f[4] + d[4]) / hz;
+    if (!transpose) {
+      if (fl & f_nb(5)) a[0] = -(f[5] - d[5]) / hz;
+      if (fl & f_nb(3)) a[1] = -(f[3] - d[3]) / hy;
+      if (fl & f_nb(1)) a[2] = -(f[1] - d[1]) / hx;
+      if (fl & f_nb(0)) a[4] = -(f[0] + d[0]) / hx;
+      if (fl & f_nb(2)) a[5] = -(f[2] + d[2]) / hy;
+      if (fl & f_nb(4)) a[6] = -(f[4] + d[4]) / hz;
+    } else {
+      // slot -> (face of n, neighbour offset, opposite face, neighbour size)
+      if (fl & f_nb(5)) a[0] = offdiag(df, dn, G, NC, g, c - q.sz, 4, q.hz[k - 1]);
+      if (fl & f_nb(3)) a[1] = offdiag(df, dn, G, NC, g, c - q.sy, 2, q.hy[j - 1]);
+      if (fl & f_nb(1)) a[2] = offdiag(df, dn, G, NC, g, c - 1, 0, q.hx[i - 1]);
+      if (fl & f_nb(0)) a[4] = offdiag(df, dn, G, NC, g, c + 1, 1, q.hx[i + 1]);
+      if (fl & f_nb(2)) a[5] = offdiag(df, dn, G, NC, g, c + q.sy, 3, q.hy[j + 1]);
+      if (fl & f_nb(4)) a[6] = offdiag(df, dn, G, NC, g, c + q.sz, 5, q.hz[k + 1]);
+    }
   }
   for (int s = 0; s < 7; ++s) {
     if (A64) A64[((size_t)g * 7 + s) * NC + c] = a[s];
@@ -376,11 +397,13 @@ __global__ void __launch_bounds__(NTF) k_bicgT(LV lv, int it, const TV *r, const
 // (:1143).
 // ---------------------------------------------------------------------------
 
-// fs0 = FSrc(f0) for the cores whose outer call starts from the first-call state
-__global__ void __launch_bounds__(NTF) k_fsrc_init(LV lv, XS xs, int adjoint, const double *f0, double *fs0) {
+// fs0 = FSrc(f0) for the cores whose outer call starts from the first-call
+// state (every core with force != 0)
+__global__ void __launch_bounds__(NTF) k_fsrc_init(LV lv, XS xs, int adjoint, const double *f0, double *fs0,
+                                                  int force) {
 #pragma clang fp contract(off)
   const BlkView v = blk_view(lv);
-  if (!v.sc->init) return;
+  if (!force && !v.sc->init) return;
   const long NC = lv.NC;
   for (long c = v.b.lo + threadIdx.x; c < v.b.hi; c += NTF) {
     double fs = 0.0;

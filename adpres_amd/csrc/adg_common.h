@@ -93,6 +93,7 @@ struct Scal {
   // when (p+1) % nac == 0 and updates the D-hats when nodal_on && (p+1) % nupd == 0
   int nac, nupd, nodal_on, pad2;
   double serc, ferc;
+  double aux;                    // generalized-adjoint outer: psi^T F phi of the projection
 };
 
 // true when the outer now running (sc->p + 1) is a nodal-update outer

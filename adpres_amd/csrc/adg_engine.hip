@@ -22,6 +22,7 @@
 #include <vector>
 
 #include "../../include/adpres_gpu.h"
+#include "adg_adjoint.h"
 #include "adg_cmfd.h"
 #include "adg_common.h"
 #include "adg_nodal.h"
@@ -113,7 +114,14 @@ struct adg_ctx {
   int *tflags = nullptr, *tloc = nullptr;
   int tcap = 0;
   hipGraphExec_t gexec = nullptr;
-  int gkey[4] = {-1, -1, -1, -1};   // adjoint, nin, kern, nupd of the captured iteration
+  int gkey[6] = {-1, -1, -1, -1, -1, -1};   // adjoint, nin, kern, nupd, variant, transpose of the captured outer
+  // differentiation (single core): saved forward state, adjoint, generalized
+  // adjoint and gradients
+  double *psi = nullptr, *phis = nullptr, *fs0s = nullptr, *exs = nullptr, *gam = nullptr, *dJdP = nullptr;
+  double *grad = nullptr;
+  bool transpose = false, have_psi = false;
+  int variant = 0;                 // 0 eigenvalue outer, 1 generalized-adjoint outer (k fixed, projection)
+  double psifphi = 0.0, kfwd = 0.0;
 
   long s_c0 = 0, s_end = 0, s_chunk = 0;   // single core: flat blocks computed in the kernels
   LV lv(const Blk *t) const {
@@ -327,12 +335,13 @@ void launch_coupcoef(adg_ctx *c) {
 }
 
 void launch_matrix(adg_ctx *c, int sched) {
+  const int tr = c->transpose ? 1 : 0;
   if (c->precision == ADG_FP32_REFINE)
     hipLaunchKernelGGL(k_matrix<float>, dim3(c->ncell, c->G), dim3(NT), 0, c->s, c->lv(c->tcell), c->xs(), c->df,
-                       c->dn, c->A64, c->A32, sched);
+                       c->dn, c->A64, c->A32, sched, tr);
   else
     hipLaunchKernelGGL(k_matrix<double>, dim3(c->ncell, c->G), dim3(NT), 0, c->s, c->lv(c->tcell), c->xs(), c->df,
-                       c->dn, c->A64, (double *)nullptr, sched);
+                       c->dn, c->A64, (double *)nullptr, sched, tr);
   HIPCHK(hipGetLastError());
 }
 
@@ -434,11 +443,13 @@ void nodal_update(adg_ctx *c, int kern, int cmode, int sched) {
   launch_matrix(c, sched);
 }
 
-// host-side initialisation of the device scalars of every core for an outer call
-void set_scal_start(adg_ctx *c, const adg_iter *it) {
+// host-side initialisation of the device scalars of every core for an outer
+// call; nit = 1: one %ITER for all cores, nit = B: one per core
+void set_scal_start(adg_ctx *c, const adg_iter *its, int nit = 1) {
   std::vector<Scal> h(c->B);
   HIPCHK(hipMemcpy(h.data(), c->sc, c->B * sizeof(Scal), hipMemcpyDeviceToHost));
   for (int b = 0; b < c->B; ++b) {
+    const adg_iter *it = its ? its + (nit == c->B ? b : 0) : nullptr;
     Scal &s = h[b];
     s.done = 0;
     s.status = 0;
@@ -450,8 +461,10 @@ void set_scal_start(adg_ctx *c, const adg_iter *it) {
     s.nac = it ? it->nac : 1;
     s.nupd = it ? it->nupd : 1;
     s.nodal_on = (it && it->kern != ADG_KERN_FDM) ? 1 : 0;
+    if (!s.nodal_on) s.ndmax = 0.0;
     s.serc = it ? it->serc : 0.0;
     s.ferc = it ? it->ferc : 0.0;
+    s.aux = c->psifphi;
   }
   HIPCHK(hipMemcpy(c->sc, h.data(), c->B * sizeof(Scal), hipMemcpyHostToDevice));
 }
@@ -486,6 +499,15 @@ void enqueue_iteration(adg_ctx *c, int adjoint, int nin, int kern, int p_host, i
     group_solve(c, g, adjoint, gg == 0 ? 1 : 0, nin);
   }
   double *P1 = c->P(1), *P2 = c->P(2);
+  if (c->variant == 1) {   // generalized adjoint: change norm, projection of psi, adjoint fission source
+    hipLaunchKernelGGL(k_gpt_norm, dim3(c->nflat), dim3(NTF), 0, c->s, lv, x, c->f0, c->f0c, c->phis, P1);
+    hipLaunchKernelGGL(k_gpt_fin, dim3(c->B), dim3(NTF), 0, c->s, lv, P1);
+    hipLaunchKernelGGL(k_gpt_proj, dim3(c->nflat), dim3(NTF), 0, c->s, lv, x, c->f0, c->psi, c->fs0);
+    hipLaunchKernelGGL(k_end, dim3((c->B + 63) / 64), dim3(64), 0, c->s, c->sc, c->B, c->tcap, c->tk, c->tser,
+                       c->tfer, c->tnd, c->tflags, c->tloc);
+    HIPCHK(hipGetLastError());
+    return;
+  }
   hipLaunchKernelGGL(k_fsrc, dim3(c->nflat), dim3(NTF), 0, c->s, lv, x, adjoint, c->f0, c->f0c, c->fs0, c->fs0c,
                      c->errn, P1);
   hipLaunchKernelGGL(k_post, dim3(c->nflat), dim3(NTF), 0, c->s, lv, x, c->fs0, c->fs0c, c->errn, P1, P2);
@@ -527,11 +549,19 @@ int core_status(adg_ctx *c, int b, const Scal &h) {
   return rc;
 }
 
-// the outer driver for every core of the batch; per-core statuses in c->core
-int run_outer(adg_ctx *c, int mode, const adg_iter *it) {
+// the outer driver for every core of the batch; per-core statuses in c->core.
+// nit = 1: one %ITER for all cores; nit = B: one per core (nout, nin and the
+// kernel must then agree, the other controls are the core's own)
+int run_outer(adg_ctx *c, int mode, const adg_iter *it, int nit = 1) {
   if (!c->all_xs()) return fail(c, ADG_E_NOT_SET, "adg_set_xs has not been called for every core");
-  if (it->nout < 1 || it->nin < 1 || it->nac < 1 || it->nupd < 1 || it->kern < 0 || it->kern > 2)
-    return fail(c, ADG_E_ARG, "bad iteration control");
+  if (nit != 1 && nit != c->B) return fail(c, ADG_E_ARG, "iteration controls: one, or one per core");
+  for (int b = 0; b < nit; ++b) {
+    const adg_iter &q = it[b];
+    if (q.nout < 1 || q.nin < 1 || q.nac < 1 || q.nupd < 1 || q.kern < 0 || q.kern > 2)
+      return fail(c, ADG_E_ARG, "bad iteration control");
+    if (q.nout != it->nout || q.nin != it->nin || q.kern != it->kern)
+      return fail(c, ADG_E_ARG, "nout, nin and the nodal kernel must be the same for every core");
+  }
   if (mode != ADG_FORWARD && mode != ADG_ADJOINT) return fail(c, ADG_E_ARG, "bad mode");
   const int adjoint = mode == ADG_ADJOINT;
   const int nout = it->nout;
@@ -552,7 +582,7 @@ int run_outer(adg_ctx *c, int mode, const adg_iter *it) {
       any_first = true;
     }
   }
-  set_scal_start(c, it);
+  set_scal_start(c, it, nit);
   c->ev_next = 0;
   c->recs.clear();
   HIPCHK(hipEventRecord(e0, c->s));
@@ -560,7 +590,7 @@ int run_outer(adg_ctx *c, int mode, const adg_iter *it) {
   launch_coupcoef(c);
   launch_matrix(c, 0);
   if (any_first) {
-    hipLaunchKernelGGL(k_fsrc_init, dim3(c->nflat), dim3(NTF), 0, c->s, lv, x, adjoint, c->f0, c->fs0);
+    hipLaunchKernelGGL(k_fsrc_init, dim3(c->nflat), dim3(NTF), 0, c->s, lv, x, adjoint, c->f0, c->fs0, 0);
     for (auto &h : c->core) h.first = false;
   }
   hipLaunchKernelGGL(k_integrals, dim3(c->nflat), dim3(NTF), 0, c->s, lv, x, c->fs0, c->P(0));
@@ -572,7 +602,7 @@ int run_outer(adg_ctx *c, int mode, const adg_iter *it) {
   // polls the flags every `check` outers
   const bool use_graph = !c->prof && c->graphs;
   if (use_graph) {
-    const int key[4] = {adjoint, it->nin, it->kern, it->nupd};
+    const int key[6] = {adjoint, it->nin, it->kern, it->nupd, c->variant, c->transpose ? 1 : 0};
     if (!c->gexec || memcmp(key, c->gkey, sizeof key) != 0) {
       if (c->gexec) {
         (void)hipGraphExecDestroy(c->gexec);
@@ -655,6 +685,55 @@ void copy_trace(adg_ctx *c, int b, adg_trace *tr) {
   if (tr->ndmax) HIPCHK(hipMemcpy(tr->ndmax, c->tnd + o, ncp * sizeof(double), hipMemcpyDeviceToHost));
   if (tr->flags) HIPCHK(hipMemcpy(tr->flags, c->tflags + o, ncp * sizeof(int), hipMemcpyDeviceToHost));
   if (tr->loc) HIPCHK(hipMemcpy(tr->loc, c->tloc + 3 * o, 3 * ncp * sizeof(int), hipMemcpyDeviceToHost));
+}
+
+// ---------------------------------------------------------------------------
+// differentiation (single-core contexts)
+// ---------------------------------------------------------------------------
+
+int nparam(int G) { return 3 * G + G * (G - 1); }
+
+void diff_alloc(adg_ctx *c) {
+  if (c->psi) return;
+  const size_t NC = c->NC, G = c->G;
+  c->psi = dalloc<double>(G * NC); c->phis = dalloc<double>(G * NC); c->gam = dalloc<double>(G * NC);
+  c->exs = dalloc<double>(G * NC); c->fs0s = dalloc<double>(NC); c->dJdP = dalloc<double>(NC);
+  c->grad = dalloc<double>((size_t)nparam(c->G) * NC);
+}
+
+void save_forward(adg_ctx *c) {
+  const size_t n = (size_t)c->G * c->NC;
+  HIPCHK(hipMemcpyAsync(c->phis, c->f0, n * sizeof(double), hipMemcpyDeviceToDevice, c->s));
+  HIPCHK(hipMemcpyAsync(c->fs0s, c->fs0, c->NC * sizeof(double), hipMemcpyDeviceToDevice, c->s));
+  c->kfwd = c->core[0].Ke_host;
+}
+
+void set_device_k(adg_ctx *c, double k) {
+  c->core[0].Ke_host = k;
+  HIPCHK(hipStreamSynchronize(c->s));
+  Scal s;
+  HIPCHK(hipMemcpy(&s, c->sc, sizeof(Scal), hipMemcpyDeviceToHost));
+  s.Ke = k;
+  HIPCHK(hipMemcpy(c->sc, &s, sizeof(Scal), hipMemcpyHostToDevice));
+}
+
+void restore_forward(adg_ctx *c) {
+  const size_t n = (size_t)c->G * c->NC;
+  HIPCHK(hipMemcpyAsync(c->f0, c->phis, n * sizeof(double), hipMemcpyDeviceToDevice, c->s));
+  HIPCHK(hipMemcpyAsync(c->fs0, c->fs0s, c->NC * sizeof(double), hipMemcpyDeviceToDevice, c->s));
+  c->transpose = false;
+  c->variant = 0;
+  set_device_k(c, c->kfwd);
+}
+
+// sum of the nflat single-value partials in slot order
+double host_sum(adg_ctx *c, const double *part) {
+  std::vector<double> h(c->nflat);
+  HIPCHK(hipMemcpyAsync(h.data(), part, c->nflat * sizeof(double), hipMemcpyDeviceToHost, c->s));
+  HIPCHK(hipStreamSynchronize(c->s));
+  double s = 0.0;
+  for (double v : h) s += v;
+  return s;
 }
 
 bool valid_geom(const adg_geom *g) {
@@ -833,7 +912,7 @@ void adg_destroy(adg_ctx *c) {
                   c->dn, c->df, c->A64, c->b, c->pw, c->tmp, c->rr64[0], c->rr64[1], c->rs64, c->t64, c->pp64[0],
                   c->pp64[1], c->vv64[0], c->vv64[1], c->A32, c->rr32[0], c->rr32[1], c->rs32, c->t32, c->d32,
                   c->pp32[0], c->pp32[1], c->vv32[0], c->vv32[1], c->S, c->a2, c->a4, c->L1, c->npart, c->part,
-                  c->sc};
+                  c->sc, c->psi, c->phis, c->fs0s, c->exs, c->gam, c->dJdP, c->grad};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (c->hsc) (void)hipHostFree(c->hsc);
@@ -961,11 +1040,11 @@ int adg_outer(adg_ctx *c, int mode, const adg_iter *it, adg_trace *tr) {
   return rc;
 }
 
-int adg_outer_batch(adg_ctx *c, int mode, const adg_iter *it, int *status, int *niter, double *keff) {
+int adg_outer_batch(adg_ctx *c, int mode, const adg_iter *it, int nit, int *status, int *niter, double *keff) {
   if (!c || !it) return ADG_E_ARG;
   int rc;
   try {
-    rc = run_outer(c, mode, it);
+    rc = run_outer(c, mode, it, nit);
     if (rc == ADG_E_NOT_SET || rc == ADG_E_ARG) return rc;
     for (int b = 0; b < c->B; ++b) {
       if (status) status[b] = c->core[b].status;
@@ -1075,6 +1154,129 @@ int adg_matvec(adg_ctx *c, int g, const double *x, double *y) {
     return hip_fail(c, e);
   }
   return ADG_OK;
+}
+
+int adg_adjoint(adg_ctx *c, const adg_iter *it, double *kadj) {
+  if (!c || !it) return ADG_E_ARG;
+  if (c->B != 1) return fail(c, ADG_E_ARG, "the differentiation entry points take a single-core context");
+  if (!c->all_xs()) return fail(c, ADG_E_NOT_SET, "adg_set_xs has not been called");
+  if (c->core[0].first) return fail(c, ADG_E_ARG, "no forward solution: call adg_outer first");
+  int rc;
+  try {
+    diff_alloc(c);
+    save_forward(c);
+    const LV lv = c->lv(c->tflat);
+    // start from the forward flux with its adjoint fission source
+    hipLaunchKernelGGL(k_fsrc_init, dim3(c->nflat), dim3(NTF), 0, c->s, lv, c->xs(), 1, c->f0, c->fs0, 1);
+    c->transpose = true;
+    c->variant = 0;
+    adg_iter ia = *it;
+    ia.kern = ADG_KERN_FDM;   // D-hat held at its converged value
+    rc = run_outer(c, ADG_ADJOINT, &ia);
+    if (rc == ADG_OK || rc == ADG_E_MAX_OUTER) {
+      HIPCHK(hipMemcpyAsync(c->psi, c->f0, (size_t)c->G * c->NC * sizeof(double), hipMemcpyDeviceToDevice, c->s));
+      if (kadj) *kadj = c->core[0].Ke_host;
+      hipLaunchKernelGGL(k_fdot, dim3(c->nflat), dim3(NTF), 0, c->s, lv, c->xs(), c->phis, c->psi, c->P(0));
+      c->psifphi = host_sum(c, c->P(0));
+    }
+    restore_forward(c);
+    c->have_psi = rc == ADG_OK;
+  } catch (const HipError &e) {
+    rc = hip_fail(c, e);
+  }
+  return rc;
+}
+
+int adg_set_adjoint(adg_ctx *c, const double *psi) {
+  if (!c || !psi || c->B != 1) return ADG_E_ARG;
+  if (!c->all_xs()) return fail(c, ADG_E_NOT_SET, "adg_set_xs has not been called");
+  try {
+    diff_alloc(c);
+    std::vector<double> st;
+    scatter(c, 0, psi, c->G, c->psi, st);
+    hipLaunchKernelGGL(k_fdot, dim3(c->nflat), dim3(NTF), 0, c->s, c->lv(c->tflat), c->xs(), c->f0, c->psi, c->P(0));
+    c->psifphi = host_sum(c, c->P(0));
+    c->have_psi = true;
+  } catch (const HipError &e) {
+    return hip_fail(c, e);
+  }
+  return ADG_OK;
+}
+
+int adg_get_adjoint(adg_ctx *c, double *psi, double *psifphi) {
+  if (!c || c->B != 1) return ADG_E_ARG;
+  if (!c->have_psi) return fail(c, ADG_E_NOT_SET, "adg_adjoint has not been called");
+  try {
+    std::vector<double> st;
+    if (psi) gather(c, 0, c->psi, c->G, psi, st);
+    if (psifphi) *psifphi = c->psifphi;
+  } catch (const HipError &e) {
+    return hip_fail(c, e);
+  }
+  return ADG_OK;
+}
+
+int adg_backward(adg_ctx *c, const adg_iter *it, double dJdk, const double *dJdp, double *grad) {
+  if (!c || !it || !grad) return ADG_E_ARG;
+  if (c->B != 1) return fail(c, ADG_E_ARG, "the differentiation entry points take a single-core context");
+  if (!c->have_psi) return fail(c, ADG_E_NOT_SET, "adg_adjoint has not been called");
+  int rc = ADG_OK;
+  try {
+    const LV lv = c->lv(c->tflat);
+    const XS x = c->xs();
+    const int N = c->core[0].N, G = c->G;
+    std::vector<double> st;
+    const double k = c->core[0].Ke_host;
+    bool resp = false;
+    if (dJdp) {
+      // node powers at the forward state (PowDis) and dJ/dP of the unnormalised powers
+      hipLaunchKernelGGL(k_powdis, dim3(c->nflat), dim3(NTF), 0, c->s, lv, x, c->f0, c->pw, c->P(0));
+      hipLaunchKernelGGL(k_scale, dim3(c->nflat), dim3(NTF), 0, c->s, lv, c->pw, c->P(0));
+      HIPCHK(hipMemcpyAsync(c->hsc, c->sc, sizeof(Scal), hipMemcpyDeviceToHost, c->s));
+      HIPCHK(hipStreamSynchronize(c->s));
+      const double T = c->hsc->tpow;
+      if (!(T > 0.0)) return fail(c, ADG_E_POWER_NONPOS, "total node power is zero or less");
+      std::vector<double> p(N), q(N);
+      gather(c, 0, c->pw, 1, p.data(), st);
+      double sp = 0.0;
+      for (int n = 0; n < N; ++n) sp += dJdp[n] * p[n];
+      for (int n = 0; n < N; ++n) {
+        q[n] = (dJdp[n] - sp) / T;
+        if (q[n] != 0.0) resp = true;
+      }
+      scatter(c, 0, q.data(), 1, c->dJdP, st);
+    }
+    if (resp) {
+      // generalized adjoint: S* as the external source of an adjoint outer with k fixed
+      const size_t n = (size_t)G * c->NC;
+      HIPCHK(hipMemcpyAsync(c->exs, c->exsrc, n * sizeof(double), hipMemcpyDeviceToDevice, c->s));
+      hipLaunchKernelGGL(k_sstar, dim3(c->nflat), dim3(NTF), 0, c->s, lv, x, c->f0, c->dJdP, c->exsrc);
+      save_forward(c);
+      HIPCHK(hipMemsetAsync(c->f0, 0, n * sizeof(double), c->s));
+      HIPCHK(hipMemsetAsync(c->fs0, 0, c->NC * sizeof(double), c->s));
+      c->transpose = true;
+      c->variant = 1;
+      adg_iter ig = *it;
+      ig.kern = ADG_KERN_FDM;
+      rc = run_outer(c, ADG_ADJOINT, &ig);
+      HIPCHK(hipMemcpyAsync(c->gam, c->f0, n * sizeof(double), hipMemcpyDeviceToDevice, c->s));
+      HIPCHK(hipMemcpyAsync(c->exsrc, c->exs, n * sizeof(double), hipMemcpyDeviceToDevice, c->s));
+      restore_forward(c);
+      if (rc != ADG_OK) return rc;
+    }
+    const double apsi = dJdk * k * k / c->psifphi;
+    const int P = nparam(G);
+    HIPCHK(hipMemsetAsync(c->grad, 0, (size_t)P * c->NC * sizeof(double), c->s));
+    hipLaunchKernelGGL(k_grad, dim3(c->nflat), dim3(NTF), 0, c->s, lv, x, c->f0, c->psi, apsi,
+                       resp ? c->gam : (const double *)nullptr, resp ? c->dJdP : (const double *)nullptr, 1.0 / k,
+                       c->grad);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->s));
+    gather(c, 0, c->grad, P, grad, st);
+  } catch (const HipError &e) {
+    rc = hip_fail(c, e);
+  }
+  return rc;
 }
 
 int adg_set_profiling(adg_ctx *c, int on) {

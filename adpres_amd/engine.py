@@ -37,6 +37,7 @@ EXPORTS = ("adg_create", "adg_destroy", "adg_set_xs", "adg_set_state", "adg_get_
            "adg_set_profiling", "adg_read_profile",
            "adg_create_batch", "adg_core_count", "adg_set_xs_core", "adg_set_state_core", "adg_get_state_core",
            "adg_outer_batch", "adg_trace_core", "adg_powdis_core",
+           "adg_adjoint", "adg_get_adjoint", "adg_set_adjoint", "adg_backward",
            "adg_status_string", "adg_last_error")
 
 _dp = C.POINTER(C.c_double)
@@ -98,9 +99,13 @@ def load_library(path: str = LIB_PATH):
     L.adg_set_xs_core.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _ip, _dp, _dp]
     L.adg_set_state_core.argtypes = [C.c_void_p, C.c_int, _dp, _dp, C.c_double, _dp, C.c_int]
     L.adg_get_state_core.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp, _dp, _dp]
-    L.adg_outer_batch.argtypes = [C.c_void_p, C.c_int, C.POINTER(AdgIter), _ip, _ip, _dp]
+    L.adg_outer_batch.argtypes = [C.c_void_p, C.c_int, C.POINTER(AdgIter), C.c_int, _ip, _ip, _dp]
     L.adg_trace_core.argtypes = [C.c_void_p, C.c_int, C.POINTER(AdgTrace)]
     L.adg_powdis_core.argtypes = [C.c_void_p, C.c_int, _dp]
+    L.adg_adjoint.argtypes = [C.c_void_p, C.POINTER(AdgIter), _dp]
+    L.adg_get_adjoint.argtypes = [C.c_void_p, _dp, _dp]
+    L.adg_set_adjoint.argtypes = [C.c_void_p, _dp]
+    L.adg_backward.argtypes = [C.c_void_p, C.POINTER(AdgIter), C.c_double, _dp, _dp]
     L.adg_status_string.argtypes = [C.c_int]
     L.adg_status_string.restype = C.c_char_p
     L.adg_last_error.argtypes = [C.c_void_p]
@@ -245,6 +250,37 @@ class Engine:
         self._check(self.lib.adg_matvec(self.h, g, _d(x), _d(y)), "adg_matvec")
         return y
 
+    # -- differentiation ----------------------------------------------------
+    def adjoint(self, **ov) -> float:
+        """Discrete adjoint of the converged CMFD operator (D-hat fixed); returns its eigenvalue."""
+        it = _iter(self.p, ov)
+        ka = C.c_double()
+        self._check(self.lib.adg_adjoint(self.h, C.byref(it), C.byref(ka)), "adg_adjoint")
+        return ka.value
+
+    def adjoint_flux(self):
+        N, G = self.p.nnod, self.p.ng
+        psi = np.zeros((G, N))
+        pfp = C.c_double()
+        self._check(self.lib.adg_get_adjoint(self.h, _d(psi), C.byref(pfp)), "adg_get_adjoint")
+        return psi, pfp.value
+
+    def set_adjoint_flux(self, psi):
+        psi = np.ascontiguousarray(psi, np.float64)
+        self._check(self.lib.adg_set_adjoint(self.h, _d(psi)), "adg_set_adjoint")
+
+    def backward(self, dJdk: float, dJdp=None, **ov) -> np.ndarray:
+        """Per-node gradient (P, N) of dJdk * k + <dJdp, node powers>; rows D_g, sigma_a_g,
+        nu-sigma_f_g, sigma_s(g->h) (g != h)."""
+        N, G = self.p.nnod, self.p.ng
+        it = _iter(self.p, ov)
+        P = 3 * G + G * (G - 1)
+        out = np.zeros((P, N))
+        dp = None if dJdp is None else np.ascontiguousarray(dJdp, np.float64)
+        self._check(self.lib.adg_backward(self.h, C.byref(it), float(dJdk), None if dp is None else _d(dp), _d(out)),
+                    "adg_backward")
+        return out
+
     def last_timing_ms(self):
         t = np.zeros(4)
         self._check(self.lib.adg_last_timing(self.h, _d(t)), "adg_last_timing")
@@ -265,7 +301,8 @@ class Batch:
     kernel launch and every replay of the captured outer iteration covers all
     cores, and each core keeps its own k, schedule, convergence test and
     status (the batching form; the reference solves one core per process).
-    The %ITER of the first problem (or the overrides) applies to all cores."""
+    Each core runs with its own %ITER (nac, nupd, serc, ferc); nout, nin and
+    the nodal kernel must agree, and overrides apply to every core."""
 
     def __init__(self, problems, precision: str = "fp64"):
         self.lib = load_library()
@@ -311,10 +348,10 @@ class Batch:
     def outer(self, adjoint: bool = False, **ov):
         """Solve every core; returns (status, outers, k) arrays.  Per-core
         failures are reported in `status`, not raised."""
-        it = _iter(self.problems[0], ov)
         n = len(self)
+        its = (AdgIter * n)(*[_iter(q, ov) for q in self.problems])
         st = np.zeros(n, np.int32); ni = np.zeros(n, np.int32); ke = np.zeros(n)
-        rc = self.lib.adg_outer_batch(self.h, 1 if adjoint else 0, C.byref(it), _i(st), _i(ni), _d(ke))
+        rc = self.lib.adg_outer_batch(self.h, 1 if adjoint else 0, its, n, _i(st), _i(ni), _d(ke))
         if rc in (ADG_E_ARG, ADG_E_NOT_SET, ADG_E_HIP):
             self._check(rc, "adg_outer_batch")
         self.status, self.niter, self.keff = st, ni, ke

@@ -146,12 +146,43 @@ int adg_set_state_core(adg_ctx *ctx, int core, const double *f0, const double *f
                        const double *dn, int first);
 int adg_get_state_core(adg_ctx *ctx, int core, double *f0, double *fs0, double *Ke, double *dn,
                        double *df);
-/* cmfd::outer / outer_ad for every core of the batch with one %ITER; per
- * core: status (enum adg_status), outers performed and k (arrays of ncore,
- * each may be NULL).  Returns the first non-OK core status. */
-int adg_outer_batch(adg_ctx *ctx, int mode, const adg_iter *iter, int *status, int *niter, double *keff);
+/* cmfd::outer / outer_ad for every core of the batch.  iter holds niter_in
+ * entries: 1 (one %ITER for every core) or ncore (each core's own; nout, nin
+ * and kern must agree, nac, nupd, serc and ferc are per core).  Per core:
+ * status (enum adg_status), outers performed and k (arrays of ncore, each
+ * may be NULL).  Returns the first non-OK core status. */
+int adg_outer_batch(adg_ctx *ctx, int mode, const adg_iter *iter, int niter_in, int *status, int *niter,
+                    double *keff);
 int adg_trace_core(adg_ctx *ctx, int core, adg_trace *trace);
 int adg_powdis_core(adg_ctx *ctx, int core, double *p);
+
+/* Differentiation form.  The reference's only adjoint is cmfd::outer_ad's
+ * physical adjoint (src/mod_cmfd.f90:602-699: the same matrices, transposed
+ * group coupling); these entry points have no counterpart there.  For a
+ * single-core context holding a converged forward solution (adg_outer):
+ * adg_adjoint solves the discrete adjoint M^T psi = (1/k) F^T psi of the
+ * converged CMFD operator (transposed 7-point matrices, D-hat held at its
+ * converged value, no nodal update) with the given %ITER, returns its
+ * eigenvalue (equal to k) and leaves the forward state untouched.
+ * adg_backward returns, per node, the gradient of J = dJdk * k + (response
+ * of the PowDis node powers with gradient dJdp, which may be NULL) with
+ * respect to the node's D_g, sigma_a_g, nu-sigma_f_g (sigma_f_g keeping its
+ * ratio to nu-sigma_f_g) and sigma_s(g->h), g != h, in that order: grad is
+ * (N, 3G + G(G-1)) column-major.  The node-power response needs the
+ * generalized adjoint, a fixed-source adjoint outer with k fixed (the
+ * reference's outer_fs, src/mod_cmfd.f90:513-598, in adjoint form) whose
+ * fundamental-mode component is projected out after every outer; iter
+ * controls it (serc, ferc bound the relative 2-norm change per outer).
+ * D-hat is held fixed: these are the gradients of the converged CMFD
+ * eigenproblem, not of the nodal D-hat update. */
+int adg_adjoint(adg_ctx *ctx, const adg_iter *iter, double *kadj);
+int adg_get_adjoint(adg_ctx *ctx, double *psi, double *psifphi);
+/* Install an adjoint flux psi(N,G) of the caller's (for instance the
+ * reference's physical adjoint from adg_outer in ADG_ADJOINT mode), so that
+ * adg_backward with dJdk = 1 and no dJdp gives the first-order perturbation
+ * estimate of dk with it; the context must hold the forward flux. */
+int adg_set_adjoint(adg_ctx *ctx, const double *psi);
+int adg_backward(adg_ctx *ctx, const adg_iter *iter, double dJdk, const double *dJdp, double *grad);
 
 const char *adg_status_string(int status);
 const char *adg_last_error(adg_ctx *ctx);

@@ -176,6 +176,24 @@ def batch(seed=3, n=256, workers=7, sample=8, cache="/tmp/adg_batch_golden"):
     save(f"batch_s{seed}_solve", **arrs)
 
 
+def adjoint_fdm(seed=1):
+    """The 34x34x26-node 2-group core with the finite-difference kernel,
+    solved by the reference in %MODE FORWARD and %MODE ADJOINT (outer_ad)
+    to 1e-10: both k and the reference's adjoint flux."""
+    gc = generate("core3d_2g", seed)
+    p = gc.problem
+    out = {}
+    for mode in ("FORWARD", "ADJOINT"):
+        with tempfile.TemporaryDirectory() as d:
+            it = (20000, p.nin, 1e-10, 1e-10, 1000, p.nupd)
+            write_deck(os.path.join(d, "deck"), kern="FDM", mode=mode, iter_card=it, **gc.deck)
+            write_xs_override(os.path.join(d, "xs.bin"), p)
+            q, res, log = run(d, "deck", "solve", "xs.bin")
+        out[mode] = res
+    save(f"core3d_2g_s{seed}_fdm_adjoint", seed=seed, problem_sha=sha(p), Ke=out["FORWARD"]["Ke"],
+         Ke_adj=out["ADJOINT"]["Ke"], psi=out["ADJOINT"]["f0"])
+
+
 def main(names):
     if not os.path.exists(DUMP):
         raise SystemExit("build the reference first: make -C oracle ref")
@@ -189,6 +207,8 @@ def main(names):
         synthetic("core2d_small", 1)
     if not names or "core3d_2g" in names:
         synthetic("core3d_2g", 1, full=False)
+    if "adjoint_fdm" in names:
+        adjoint_fdm()
     if "batch" in names:          # 256 reference solves on 7 host cores
         batch()
     if "core3d_8g" in names:      # about an hour of reference CPU time

