@@ -17,6 +17,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -48,6 +49,24 @@ T *dalloc(size_t n) {
   if (n == 0) n = 1;
   HIPCHK(hipMalloc(&p, n * sizeof(T)));
   HIPCHK(hipMemset(p, 0, n * sizeof(T)));
+  return (T *)p;
+}
+
+// Copies of a running context go through its own stream (never the null
+// stream), so that contexts driven from several host threads do not
+// synchronise with each other or with another thread's graph capture.
+void scopy(hipStream_t s, void *dst, const void *src, size_t bytes, hipMemcpyKind kind) {
+  HIPCHK(hipMemcpyAsync(dst, src, bytes, kind, s));
+  HIPCHK(hipStreamSynchronize(s));
+}
+
+template <typename T>
+T *dalloc_s(hipStream_t s, size_t n) {
+  void *p = nullptr;
+  if (n == 0) n = 1;
+  HIPCHK(hipMalloc(&p, n * sizeof(T)));
+  HIPCHK(hipMemsetAsync(p, 0, n * sizeof(T), s));
+  HIPCHK(hipStreamSynchronize(s));
   return (T *)p;
 }
 
@@ -199,16 +218,16 @@ void scatter(const adg_ctx *c, int b, const double *src, int K, double *dst_dev,
   for (int k = 0; k < K; ++k)
     for (int n = 0; n < h.N; ++n) stage[(size_t)k * h.box + (h.cell[n] - h.base)] = src[(size_t)k * h.N + n];
   for (int k = 0; k < K; ++k)
-    HIPCHK(hipMemcpy(dst_dev + (size_t)k * c->NC + h.base, stage.data() + (size_t)k * h.box, h.box * sizeof(double),
-                     hipMemcpyHostToDevice));
+    scopy(c->s, dst_dev + (size_t)k * c->NC + h.base, stage.data() + (size_t)k * h.box, h.box * sizeof(double),
+          hipMemcpyHostToDevice);
 }
 
 void gather(const adg_ctx *c, int b, const double *src_dev, int K, double *dst, std::vector<double> &stage) {
   const CoreHost &h = c->core[b];
   stage.resize((size_t)K * h.box);
   for (int k = 0; k < K; ++k)
-    HIPCHK(hipMemcpy(stage.data() + (size_t)k * h.box, src_dev + (size_t)k * c->NC + h.base, h.box * sizeof(double),
-                     hipMemcpyDeviceToHost));
+    scopy(c->s, stage.data() + (size_t)k * h.box, src_dev + (size_t)k * c->NC + h.base, h.box * sizeof(double),
+          hipMemcpyDeviceToHost);
   for (int k = 0; k < K; ++k)
     for (int n = 0; n < h.N; ++n) dst[(size_t)k * h.N + n] = stage[(size_t)k * h.box + (h.cell[n] - h.base)];
 }
@@ -447,7 +466,7 @@ void nodal_update(adg_ctx *c, int kern, int cmode, int sched) {
 // call; nit = 1: one %ITER for all cores, nit = B: one per core
 void set_scal_start(adg_ctx *c, const adg_iter *its, int nit = 1) {
   std::vector<Scal> h(c->B);
-  HIPCHK(hipMemcpy(h.data(), c->sc, c->B * sizeof(Scal), hipMemcpyDeviceToHost));
+  scopy(c->s, h.data(), c->sc, c->B * sizeof(Scal), hipMemcpyDeviceToHost);
   for (int b = 0; b < c->B; ++b) {
     const adg_iter *it = its ? its + (nit == c->B ? b : 0) : nullptr;
     Scal &s = h[b];
@@ -466,7 +485,7 @@ void set_scal_start(adg_ctx *c, const adg_iter *its, int nit = 1) {
     s.ferc = it ? it->ferc : 0.0;
     s.aux = c->psifphi;
   }
-  HIPCHK(hipMemcpy(c->sc, h.data(), c->B * sizeof(Scal), hipMemcpyHostToDevice));
+  scopy(c->s, c->sc, h.data(), c->B * sizeof(Scal), hipMemcpyHostToDevice);
 }
 
 void ensure_trace(adg_ctx *c, int nout) {
@@ -476,8 +495,8 @@ void ensure_trace(adg_ctx *c, int nout) {
     if (pp) (void)hipFree(pp);
   const int cap = std::max(nout, 2 * c->tcap);
   const size_t n = (size_t)cap * c->B;
-  c->tk = dalloc<double>(n); c->tser = dalloc<double>(n); c->tfer = dalloc<double>(n);
-  c->tnd = dalloc<double>(n); c->tflags = dalloc<int>(n); c->tloc = dalloc<int>(3 * n);
+  c->tk = dalloc_s<double>(c->s, n); c->tser = dalloc_s<double>(c->s, n); c->tfer = dalloc_s<double>(c->s, n);
+  c->tnd = dalloc_s<double>(c->s, n); c->tflags = dalloc_s<int>(c->s, n); c->tloc = dalloc_s<int>(c->s, 3 * n);
   c->tcap = cap;
   if (c->gexec) {   // the captured k_end holds the old trace pointers
     (void)hipGraphExecDestroy(c->gexec);
@@ -608,6 +627,9 @@ int run_outer(adg_ctx *c, int mode, const adg_iter *it, int nit = 1) {
         (void)hipGraphExecDestroy(c->gexec);
         c->gexec = nullptr;
       }
+      // one capture at a time in the process (contexts may run on several host threads)
+      static std::mutex capture_lock;
+      std::lock_guard<std::mutex> lk(capture_lock);
       hipGraph_t graph;
       HIPCHK(hipStreamBeginCapture(c->s, hipStreamCaptureModeThreadLocal));
       enqueue_iteration(c, adjoint, it->nin, it->kern, 0, it->nupd);
@@ -679,12 +701,12 @@ void copy_trace(adg_ctx *c, int b, adg_trace *tr) {
   tr->n = ncp;
   if (ncp <= 0) return;
   const size_t o = (size_t)b * c->tcap;
-  if (tr->k) HIPCHK(hipMemcpy(tr->k, c->tk + o, ncp * sizeof(double), hipMemcpyDeviceToHost));
-  if (tr->ser) HIPCHK(hipMemcpy(tr->ser, c->tser + o, ncp * sizeof(double), hipMemcpyDeviceToHost));
-  if (tr->fer) HIPCHK(hipMemcpy(tr->fer, c->tfer + o, ncp * sizeof(double), hipMemcpyDeviceToHost));
-  if (tr->ndmax) HIPCHK(hipMemcpy(tr->ndmax, c->tnd + o, ncp * sizeof(double), hipMemcpyDeviceToHost));
-  if (tr->flags) HIPCHK(hipMemcpy(tr->flags, c->tflags + o, ncp * sizeof(int), hipMemcpyDeviceToHost));
-  if (tr->loc) HIPCHK(hipMemcpy(tr->loc, c->tloc + 3 * o, 3 * ncp * sizeof(int), hipMemcpyDeviceToHost));
+  if (tr->k) scopy(c->s, tr->k, c->tk + o, ncp * sizeof(double), hipMemcpyDeviceToHost);
+  if (tr->ser) scopy(c->s, tr->ser, c->tser + o, ncp * sizeof(double), hipMemcpyDeviceToHost);
+  if (tr->fer) scopy(c->s, tr->fer, c->tfer + o, ncp * sizeof(double), hipMemcpyDeviceToHost);
+  if (tr->ndmax) scopy(c->s, tr->ndmax, c->tnd + o, ncp * sizeof(double), hipMemcpyDeviceToHost);
+  if (tr->flags) scopy(c->s, tr->flags, c->tflags + o, ncp * sizeof(int), hipMemcpyDeviceToHost);
+  if (tr->loc) scopy(c->s, tr->loc, c->tloc + 3 * o, 3 * ncp * sizeof(int), hipMemcpyDeviceToHost);
 }
 
 // ---------------------------------------------------------------------------
@@ -710,11 +732,10 @@ void save_forward(adg_ctx *c) {
 
 void set_device_k(adg_ctx *c, double k) {
   c->core[0].Ke_host = k;
-  HIPCHK(hipStreamSynchronize(c->s));
   Scal s;
-  HIPCHK(hipMemcpy(&s, c->sc, sizeof(Scal), hipMemcpyDeviceToHost));
+  scopy(c->s, &s, c->sc, sizeof(Scal), hipMemcpyDeviceToHost);
   s.Ke = k;
-  HIPCHK(hipMemcpy(c->sc, &s, sizeof(Scal), hipMemcpyHostToDevice));
+  scopy(c->s, c->sc, &s, sizeof(Scal), hipMemcpyHostToDevice);
 }
 
 void restore_forward(adg_ctx *c) {
@@ -991,9 +1012,9 @@ int adg_set_state_core(adg_ctx *c, int core, const double *f0, const double *fs0
     h.Ke_host = Ke;
     h.first = false;
     Scal s;
-    HIPCHK(hipMemcpy(&s, c->sc + core, sizeof(Scal), hipMemcpyDeviceToHost));
+    scopy(c->s, &s, c->sc + core, sizeof(Scal), hipMemcpyDeviceToHost);
     s.Ke = Ke;
-    HIPCHK(hipMemcpy(c->sc + core, &s, sizeof(Scal), hipMemcpyHostToDevice));
+    scopy(c->s, c->sc + core, &s, sizeof(Scal), hipMemcpyHostToDevice);
   } catch (const HipError &e) {
     return hip_fail(c, e);
   }
@@ -1015,7 +1036,7 @@ int adg_get_state_core(adg_ctx *c, int core, double *f0, double *fs0, double *Ke
     if (df) gather(c, core, c->df, 6 * c->G, df, st);
     if (Ke) {
       Scal s;
-      HIPCHK(hipMemcpy(&s, c->sc + core, sizeof(Scal), hipMemcpyDeviceToHost));
+      scopy(c->s, &s, c->sc + core, sizeof(Scal), hipMemcpyDeviceToHost);
       *Ke = s.Ke;
     }
   } catch (const HipError &e) {

@@ -47,6 +47,84 @@ WORKLOADS = {
 }
 
 
+BATCH_SEED = 3
+
+
+def batch_bench(args):
+    """The batching workload: 256 uneven cores (2D/3D, 2-4 groups, 5k-60k
+    nodes) in one context per group count, the contexts driven concurrently
+    (one host thread and one stream each).  A step solves all 256 cores from
+    the first-call state; value = converged solves per second (cores the
+    reference itself stops as nodally unstable stop the same way and are not
+    counted), FP32+refinement (the workload's form), with FP64 beside it."""
+    import threading
+    from adpres_amd.configs import batch_core
+    from adpres_amd.engine import ADG_OK, Batch
+    t0 = time.perf_counter()
+    cores = [batch_core(BATCH_SEED, i).problem for i in range(256)]
+    log(f"bench: batch256 generated in {time.perf_counter() - t0:.1f} s: {sum(q.nnod for q in cores)} nodes, "
+        f"{sum(q.unknowns for q in cores)} unknowns")
+    fx = os.path.join(ROOT, "tests", "golden", f"batch_s{BATCH_SEED}_solve.npz")
+    ref = np.load(fx) if os.path.exists(fx) else None
+    classes = {}
+    for i, q in enumerate(cores):
+        classes.setdefault(q.ng, []).append(i)
+
+    def run(precision):
+        bats = {G: Batch([cores[i] for i in idx], precision) for G, idx in classes.items()}
+        keff = np.zeros(len(cores))
+        niter = np.zeros(len(cores), np.int32)
+        status = np.zeros(len(cores), np.int32)
+
+        def solve(G):
+            bats[G].reset()
+            st, ni, ke = bats[G].outer()
+            keff[classes[G]] = ke
+            niter[classes[G]] = ni
+            status[classes[G]] = st
+
+        def step():
+            th = [threading.Thread(target=solve, args=(G,)) for G in bats]
+            for t in th:
+                t.start()
+            for t in th:
+                t.join()
+
+        for _ in range(args.warmup):
+            step()
+        t1 = time.perf_counter()
+        for _ in range(args.steps):
+            step()
+        dt = (time.perf_counter() - t1) / args.steps
+        for b in bats.values():
+            b.close()
+        solved = int(np.sum(status == ADG_OK))
+        out = {"seconds_per_step": dt, "solves_per_s": solved / dt, "cores_solved": solved,
+               "cores_stopped": int(len(cores) - solved), "max_outers": int(niter.max())}
+        if ref is not None:
+            m = np.isfinite(ref["Ke"])
+            both = m & (status == ADG_OK)
+            out["k_err_pcm_max"] = float(np.max(np.abs(keff[both] - ref["Ke"][both])) * 1e5)
+            out["reference_cores_solved"] = int(m.sum())
+            out["outcome_differs_from_reference"] = [int(i) for i in np.nonzero(m != (status == ADG_OK))[0]]
+        log(f"bench: batch256 {precision}: {out}")
+        return out
+
+    r32 = run("fp32_refine")
+    r64 = run("fp64") if not args.no_fp32 else None
+    out = {
+        "metric": "k-eff solves/s (256-core batch, FP32+refine; FP64 beside it) at 1 pcm",
+        "value": round(r32["solves_per_s"], 3), "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(r32["seconds_per_step"] * 1e3, 2), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32 (FP64 defect correction)",
+        "data": "synthetic (seeded generator, adpres_amd/configs.py batch_core)",
+        "config": {"workload": "batch256", "cores": len(cores), "nodes": int(sum(q.nnod for q in cores)),
+                   "unknowns": int(sum(q.unknowns for q in cores)),
+                   "contexts": {str(G): len(idx) for G, idx in classes.items()}},
+        "fp32_refine": r32, "fp64": r64}
+    print(json.dumps(out))
+
+
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
@@ -171,7 +249,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="core3d_8g", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="core3d_8g", choices=sorted(WORKLOADS) + ["batch256"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-graph", action="store_true",
@@ -185,6 +263,10 @@ def main():
         print(f"bench.py: this engine runs one core per GPU and this build measures one GPU only; "
               f"--gpus {args.gpus} is not supported (multi-GPU sharding is a design note)", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "batch256":
+        batch_bench(args)
+        return
 
     from adpres_amd.configs import generate
     from adpres_amd.engine import Engine
