@@ -228,11 +228,21 @@ __device__ __forceinline__ void reduce_partials(const double *part, int first, i
     wave_reduce<NV, MAXMASK>(out);
     return;
   }
-  for (int b = threadIdx.x; b < n; b += NTF) {
+  // four partials per thread are requested before any is added (same
+  // summation order as one at a time: slots t, t + NTF, t + 2 NTF, ...)
+  for (int b0 = threadIdx.x; b0 < n; b0 += 4 * NTF) {
+    double x[4][NV];
 #pragma unroll
-    for (int k = 0; k < NV; ++k) {
-      const double x = pp[(size_t)b * NV + k];
-      out[k] = ((MAXMASK >> k) & 1u) ? fmax(out[k], x) : out[k] + x;
+    for (int u = 0; u < 4; ++u) {
+      const int b = b0 + u * NTF;
+#pragma unroll
+      for (int k = 0; k < NV; ++k) x[u][k] = b < n ? pp[(size_t)b * NV + k] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      if (b0 + u * NTF >= n) break;
+#pragma unroll
+      for (int k = 0; k < NV; ++k) out[k] = ((MAXMASK >> k) & 1u) ? fmax(out[k], x[u][k]) : out[k] + x[u][k];
     }
   }
   block_reduce<NV, MAXMASK>(out);

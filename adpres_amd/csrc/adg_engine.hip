@@ -97,6 +97,8 @@ struct adg_ctx {
   Geo *geo = nullptr;
   Blk *tflat = nullptr, *tcell = nullptr, *tface = nullptr;
   int nflat = 0, ncell = 0, nface = 0;
+  int flat_cap = 0;                // allocated entries of tflat (and 12 x of part)
+  long flat_param = 0;             // launch shape of the flat kernels (see build_flat)
   double *d_hx = nullptr, *d_hy = nullptr, *d_hz = nullptr;
   unsigned char *flag = nullptr;
   double *D = nullptr, *sigr = nullptr, *nuf = nullptr, *sigf = nullptr, *chi = nullptr, *sigs = nullptr;
@@ -236,20 +238,20 @@ void gather(const adg_ctx *c, int b, const double *src_dev, int K, double *dst, 
 // launch tables
 // ---------------------------------------------------------------------------
 
-// Flat launches.  One core: up to maxblk blocks of NTF-multiple chunks, and
+// Flat launches.  One core (param = largest grid): NTF-multiple chunks, and
 // the hardware's block b -> XCD b % 8 dispatch turned into contiguous slabs
 // per XCD (logical block (b % 8) * (grid / 8) + b / 8), so a stencil's
-// z-neighbour planes are read through the XCD's own L2.  A batch: one
-// NTF-cell chunk per block, cores assigned whole to XCDs (largest first,
-// to the least loaded XCD), each XCD's block list interleaved at stride 8;
-// lists are padded with empty blocks whose partial slots no core reads.
-void build_flat(adg_ctx *c, std::vector<Blk> &t) {
+// z-neighbour planes are read through the XCD's own L2.  A batch (param =
+// cells per block, a multiple of NTF): cores assigned whole to XCDs (largest
+// first, to the least loaded XCD), each XCD's block list interleaved at
+// stride 8; lists are padded with empty blocks whose partial slots no core
+// reads.
+void build_flat(adg_ctx *c, std::vector<Blk> &t, long param) {
   t.clear();
   if (c->B == 1) {
     CoreHost &h = c->core[0];
     const long M = (long)h.nzz * h.sz;
-    const long maxblk = c->precision == ADG_FP32_REFINE ? MAXBLK / 2 : MAXBLK;
-    int nb = (int)std::min<long>((M + NTF - 1) / NTF, maxblk);
+    int nb = (int)std::min<long>((M + NTF - 1) / NTF, param);
     nb = (nb + 7) / 8 * 8;
     long chunk = (M + nb - 1) / nb;
     chunk = (chunk + NTF - 1) / NTF * NTF;
@@ -269,10 +271,11 @@ void build_flat(adg_ctx *c, std::vector<Blk> &t) {
     c->s_chunk = chunk;
     return;
   }
+  const long chunk = param;
   std::vector<int> nbk(c->B);
   for (int b = 0; b < c->B; ++b) {
     const CoreHost &h = c->core[b];
-    nbk[b] = (int)(((long)h.nzz * h.sz + NTF - 1) / NTF);
+    nbk[b] = (int)(((long)h.nzz * h.sz + chunk - 1) / chunk);
   }
   int slot = 0;
   for (int b = 0; b < c->B; ++b) {
@@ -290,8 +293,8 @@ void build_flat(adg_ctx *c, std::vector<Blk> &t) {
     const CoreHost &h = c->core[b];
     const long c0 = h.base + h.sz, end = c0 + (long)h.nzz * h.sz;
     for (int j = 0; j < nbk[b]; ++j) {
-      const long lo = c0 + (long)j * NTF;
-      xl[x].push_back(Blk{b, h.pfirst + j, h.pfirst, h.pn, lo, std::min(lo + NTF, end), h.sy, h.sz});
+      const long lo = c0 + (long)j * chunk;
+      xl[x].push_back(Blk{b, h.pfirst + j, h.pfirst, h.pn, lo, std::min(lo + chunk, end), h.sy, h.sz});
     }
     load[x] += nbk[b];
   }
@@ -304,28 +307,7 @@ void build_flat(adg_ctx *c, std::vector<Blk> &t) {
       t[8 * j + x] = j < xl[x].size() ? xl[x][j] : Blk{0, pad++, c->core[0].pfirst, c->core[0].pn, 0, 0, 1, 1};
 }
 
-void build_tables(adg_ctx *c) {
-  std::vector<Blk> tf, tc, tfc;
-  build_flat(c, tf);
-  for (int b = 0; b < c->B; ++b) {
-    CoreHost &h = c->core[b];
-    const long nn = (long)h.nxx * h.nyy * h.nzz;
-    for (long t0 = 0; t0 < nn; t0 += NT) tc.push_back(Blk{b, 0, h.pfirst, h.pn, t0, nn, h.sy, h.sz});
-    const long maxf = std::max({(long)(h.nxx + 1) * h.nyy * h.nzz, (long)(h.nyy + 1) * h.nxx * h.nzz,
-                                (long)(h.nzz + 1) * h.nxx * h.nyy});
-    h.ffirst = (int)tfc.size();
-    for (long t0 = 0; t0 < maxf; t0 += NT) tfc.push_back(Blk{b, 0, h.pfirst, h.pn, t0, maxf, h.sy, h.sz});
-    h.fnb = (int)tfc.size() - h.ffirst;
-  }
-  c->nflat = (int)tf.size();
-  c->ncell = (int)tc.size();
-  c->nface = (int)tfc.size();
-  c->tflat = dalloc<Blk>(tf.size());
-  c->tcell = dalloc<Blk>(tc.size());
-  c->tface = dalloc<Blk>(tfc.size());
-  HIPCHK(hipMemcpy(c->tflat, tf.data(), tf.size() * sizeof(Blk), hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(c->tcell, tc.data(), tc.size() * sizeof(Blk), hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(c->tface, tfc.data(), tfc.size() * sizeof(Blk), hipMemcpyHostToDevice));
+void upload_geo(adg_ctx *c) {
   std::vector<Geo> gq(c->B);
   for (int b = 0; b < c->B; ++b) {
     const CoreHost &h = c->core[b];
@@ -341,8 +323,61 @@ void build_tables(adg_ctx *c) {
     q.hx = c->d_hx + h.hoff[0]; q.hy = c->d_hy + h.hoff[1]; q.hz = c->d_hz + h.hoff[2];
     q.flag = c->flag;
   }
-  c->geo = dalloc<Geo>(c->B);
-  HIPCHK(hipMemcpy(c->geo, gq.data(), gq.size() * sizeof(Geo), hipMemcpyHostToDevice));
+  if (!c->geo) c->geo = dalloc<Geo>(c->B);
+  scopy(c->s, c->geo, gq.data(), gq.size() * sizeof(Geo), hipMemcpyHostToDevice);
+}
+
+// default launch shape of the flat kernels: one core up to MAXBLK blocks
+// (the FP32 form, streaming half the bytes, up to MAXBLK / 2), a batch one
+// NTF-cell chunk per block
+long default_flat_param(const adg_ctx *c) {
+  if (c->B > 1) return NTF;
+  return c->precision == ADG_FP32_REFINE ? MAXBLK / 2 : MAXBLK;
+}
+
+// (re)build the flat launch table, its partial buffers and the per-core slot
+// ranges; a captured outer holds the old grid, so it is dropped
+void install_flat(adg_ctx *c, long param) {
+  std::vector<Blk> tf;
+  build_flat(c, tf, param);
+  if ((int)tf.size() > c->flat_cap) {
+    if (c->tflat) HIPCHK(hipFree(c->tflat));
+    if (c->part) HIPCHK(hipFree(c->part));
+    c->tflat = nullptr;
+    c->part = nullptr;
+    c->flat_cap = (int)tf.size();
+    c->tflat = dalloc<Blk>(tf.size());
+    c->part = dalloc<double>((size_t)12 * tf.size());   // three rotating buffers of 4 x grid
+  }
+  c->nflat = (int)tf.size();
+  scopy(c->s, c->tflat, tf.data(), tf.size() * sizeof(Blk), hipMemcpyHostToDevice);
+  c->flat_param = param;
+  upload_geo(c);
+  if (c->gexec) {
+    (void)hipGraphExecDestroy(c->gexec);
+    c->gexec = nullptr;
+  }
+}
+
+void build_tables(adg_ctx *c) {
+  std::vector<Blk> tc, tfc;
+  for (int b = 0; b < c->B; ++b) {
+    CoreHost &h = c->core[b];
+    const long nn = (long)h.nxx * h.nyy * h.nzz;
+    for (long t0 = 0; t0 < nn; t0 += NT) tc.push_back(Blk{b, 0, 0, 0, t0, nn, h.sy, h.sz});
+    const long maxf = std::max({(long)(h.nxx + 1) * h.nyy * h.nzz, (long)(h.nyy + 1) * h.nxx * h.nzz,
+                                (long)(h.nzz + 1) * h.nxx * h.nyy});
+    h.ffirst = (int)tfc.size();
+    for (long t0 = 0; t0 < maxf; t0 += NT) tfc.push_back(Blk{b, 0, 0, 0, t0, maxf, h.sy, h.sz});
+    h.fnb = (int)tfc.size() - h.ffirst;
+  }
+  c->ncell = (int)tc.size();
+  c->nface = (int)tfc.size();
+  c->tcell = dalloc<Blk>(tc.size());
+  c->tface = dalloc<Blk>(tfc.size());
+  scopy(c->s, c->tcell, tc.data(), tc.size() * sizeof(Blk), hipMemcpyHostToDevice);
+  scopy(c->s, c->tface, tfc.data(), tfc.size() * sizeof(Blk), hipMemcpyHostToDevice);
+  install_flat(c, default_flat_param(c));
 }
 
 // ---------------------------------------------------------------------------
@@ -880,7 +915,6 @@ int create_batch(const adg_geom *geoms, int ncore, adg_ctx **out) {
     c->L1 = dalloc<double>(3 * G * NC);
     build_tables(c);
     c->npart = dalloc<double>((size_t)3 * 3 * c->nface);
-    c->part = dalloc<double>((size_t)12 * c->nflat);   // three rotating buffers of 4 x nflat
     c->sc = dalloc<Scal>(ncore);
     HIPCHK(hipHostMalloc((void **)&c->hsc, ncore * sizeof(Scal), hipHostMallocDefault));
   } catch (const HipError &e) {
@@ -1298,6 +1332,71 @@ int adg_backward(adg_ctx *c, const adg_iter *it, double dJdk, const double *dJdp
     rc = hip_fail(c, e);
   }
   return rc;
+}
+
+int adg_autotune(adg_ctx *c, int nin, double *out4) {
+  if (!c || nin < 1) return ADG_E_ARG;
+  if (!c->all_xs()) return fail(c, ADG_E_NOT_SET, "adg_set_xs has not been called");
+  try {
+    // candidates: one core, the largest grid; a batch, cells per block
+    std::vector<long> cand;
+    if (c->B == 1) {
+      const CoreHost &h = c->core[0];
+      const long full = ((long)h.nzz * h.sz + NTF - 1) / NTF;   // one NTF chunk per block
+      for (long g : {256L, 512L, 1024L, 2048L})
+        if (g < full) cand.push_back(g);
+      cand.push_back(full);
+    } else {
+      cand = {NTF, 2L * NTF, 4L * NTF};
+    }
+    const size_t nf = (size_t)c->G * c->NC;
+    double *keep = dalloc_s<double>(c->s, nf);
+    HIPCHK(hipMemcpyAsync(keep, c->f0, nf * sizeof(double), hipMemcpyDeviceToDevice, c->s));
+    set_scal_start(c, nullptr);
+    launch_coupcoef(c);
+    launch_matrix(c, 0);
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    const bool prof = c->prof;
+    c->prof = false;
+    long best = c->flat_param;
+    double tbest = 1e300;
+    for (long cp : cand) {
+      install_flat(c, cp);
+      launch_bicg(c, 0, nin);   // warm-up
+      float ms = 1e30f;
+      for (int rep = 0; rep < 3; ++rep) {
+        HIPCHK(hipEventRecord(e0, c->s));
+        launch_bicg(c, 0, nin);
+        HIPCHK(hipEventRecord(e1, c->s));
+        HIPCHK(hipEventSynchronize(e1));
+        float t = 0.f;
+        HIPCHK(hipEventElapsedTime(&t, e0, e1));
+        ms = std::min(ms, t);
+      }
+      if (ms < tbest) {
+        tbest = ms;
+        best = cp;
+      }
+    }
+    c->prof = prof;
+    install_flat(c, best);
+    HIPCHK(hipMemcpyAsync(c->f0, keep, nf * sizeof(double), hipMemcpyDeviceToDevice, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    HIPCHK(hipFree(keep));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (out4) {
+      out4[0] = (double)best;
+      out4[1] = tbest;
+      out4[2] = (double)c->nflat;
+      out4[3] = (double)cand.size();
+    }
+  } catch (const HipError &e) {
+    return hip_fail(c, e);
+  }
+  return ADG_OK;
 }
 
 int adg_set_profiling(adg_ctx *c, int on) {

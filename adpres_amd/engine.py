@@ -34,7 +34,7 @@ KERNS = {"FDM": 0, "PNM": 1, "SANM": 2}
 
 EXPORTS = ("adg_create", "adg_destroy", "adg_set_xs", "adg_set_state", "adg_get_state", "adg_outer",
            "adg_powdis", "adg_nodal_update", "adg_bicg", "adg_matvec", "adg_last_timing",
-           "adg_set_profiling", "adg_read_profile",
+           "adg_set_profiling", "adg_read_profile", "adg_autotune",
            "adg_create_batch", "adg_core_count", "adg_set_xs_core", "adg_set_state_core", "adg_get_state_core",
            "adg_outer_batch", "adg_trace_core", "adg_powdis_core",
            "adg_adjoint", "adg_get_adjoint", "adg_set_adjoint", "adg_backward",
@@ -94,6 +94,7 @@ def load_library(path: str = LIB_PATH):
     L.adg_last_timing.argtypes = [C.c_void_p, _dp]
     L.adg_set_profiling.argtypes = [C.c_void_p, C.c_int]
     L.adg_read_profile.argtypes = [C.c_void_p, _dp]
+    L.adg_autotune.argtypes = [C.c_void_p, C.c_int, _dp]
     L.adg_create_batch.argtypes = [C.POINTER(AdgGeom), C.c_int, C.POINTER(C.c_void_p)]
     L.adg_core_count.argtypes = [C.c_void_p]
     L.adg_set_xs_core.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _ip, _dp, _dp]
@@ -250,6 +251,12 @@ class Engine:
         self._check(self.lib.adg_matvec(self.h, g, _d(x), _d(y)), "adg_matvec")
         return y
 
+    def autotune(self, nin: Optional[int] = None) -> dict:
+        """Pick the fastest launch shape of the flat kernels (see adg_autotune)."""
+        out = np.zeros(4)
+        self._check(self.lib.adg_autotune(self.h, int(nin or self.p.nin), _d(out)), "adg_autotune")
+        return dict(shape=int(out[0]), group_solve_ms=float(out[1]), grid=int(out[2]), candidates=int(out[3]))
+
     # -- differentiation ----------------------------------------------------
     def adjoint(self, **ov) -> float:
         """Discrete adjoint of the converged CMFD operator (D-hat fixed); returns its eigenvalue."""
@@ -356,6 +363,12 @@ class Batch:
             self._check(rc, "adg_outer_batch")
         self.status, self.niter, self.keff = st, ni, ke
         return st, ni, ke
+
+    def autotune(self, nin: Optional[int] = None) -> dict:
+        """Pick the fastest cells-per-block of the flat kernels (see adg_autotune)."""
+        out = np.zeros(4)
+        self._check(self.lib.adg_autotune(self.h, int(nin or self.problems[0].nin), _d(out)), "adg_autotune")
+        return dict(shape=int(out[0]), group_solve_ms=float(out[1]), grid=int(out[2]), candidates=int(out[3]))
 
     def powdis(self, b: int) -> np.ndarray:
         out = np.zeros(self.problems[b].nnod)

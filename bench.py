@@ -72,6 +72,8 @@ def batch_bench(args):
 
     def run(precision):
         bats = {G: Batch([cores[i] for i in idx], precision) for G, idx in classes.items()}
+        tunes = {G: b.autotune() for G, b in bats.items()}
+        log(f"bench: batch256 {precision} autotune {tunes}")
         keff = np.zeros(len(cores))
         niter = np.zeros(len(cores), np.int32)
         status = np.zeros(len(cores), np.int32)
@@ -100,6 +102,7 @@ def batch_bench(args):
             b.close()
         solved = int(np.sum(status == ADG_OK))
         out = {"seconds_per_step": dt, "solves_per_s": solved / dt, "cores_solved": solved,
+               "cells_per_block": {str(G): t["shape"] for G, t in tunes.items()},
                "cores_stopped": int(len(cores) - solved), "max_outers": int(niter.max())}
         if ref is not None:
             m = np.isfinite(ref["Ke"])
@@ -283,12 +286,16 @@ def main():
         return
 
     eng = Engine(p, "fp64")
+    tune64 = eng.autotune()
+    log(f"bench: autotune fp64 {tune64}")
     r64 = time_solves(eng, args.steps, args.warmup)
     eng.close()
     log(f"bench: fp64 {r64['solves_per_s']:.3f} solves/s k={r64['k']:.8f} outers={r64['niter']}")
     r32 = None
     if not args.no_fp32:
         eng = Engine(p, "fp32_refine")
+        tune32 = eng.autotune()
+        log(f"bench: autotune fp32 {tune32}")
         r32 = time_solves(eng, args.steps, args.warmup)
         eng.close()
         log(f"bench: fp32_refine {r32['solves_per_s']:.3f} solves/s k={r32['k']:.8f} outers={r32['niter']}")
@@ -339,7 +346,7 @@ def main():
                    "unknowns": p.unknowns, "kern": p.kern,
                    "iter": {"nout": p.nout, "nin": p.nin, "serc": p.serc, "ferc": p.ferc, "nac": p.nac,
                             "nupd": p.nupd},
-                   "outers_per_solve": r64["niter"]},
+                   "outers_per_solve": r64["niter"], "launch_shape_fp64": tune64},
         "parity_fp64": errs(r64),
         "roofline": roof,
         "cpu_baseline": cpu,
@@ -347,7 +354,7 @@ def main():
     if r32 is not None:
         out["fp32_refine"] = {"solves_per_s": round(r32["solves_per_s"], 4),
                               "speedup_vs_fp64": round(r32["solves_per_s"] / r64["solves_per_s"], 3),
-                              "outers_per_solve": r32["niter"], "parity": errs(r32),
+                              "outers_per_solve": r32["niter"], "parity": errs(r32), "launch_shape": tune32,
                               "k1_avg_us": round(r32["k1_avg_ms"] * 1e3, 3) if r32["k1_avg_ms"] else None}
     print(json.dumps(out))
 

@@ -123,6 +123,15 @@ int adg_matvec(adg_ctx *ctx, int g, const double *x, double *y);
  *   [0] K1 kernel ms, [1] K1 samples, [2] BiCGSTAB group solve ms (setup + nin
  *   iterations), [3] samples, [4] nodal update ms, [5] samples. */
 int adg_last_timing(adg_ctx *ctx, double *ms4);
+/* Launch-shape autotuner (no counterpart in the reference): times one
+ * BiCGSTAB group solve of nin iterations on the context's matrices for each
+ * candidate shape of the flat kernels (one core: the largest grid, 256 to
+ * 2048 blocks; a batch: 256, 512 or 1024 cells per block) and keeps the
+ * fastest.  The solution state is left as it was; results stay within the
+ * parity bars for any shape (only the summation order of the reductions
+ * depends on it).  out4: chosen shape, its ms per group solve, grid size,
+ * candidates tried (may be NULL). */
+int adg_autotune(adg_ctx *ctx, int nin, double *out4);
 int adg_set_profiling(adg_ctx *ctx, int on);
 int adg_read_profile(adg_ctx *ctx, double *out6);
 

@@ -148,3 +148,18 @@ def test_gpu_warm_restart_keeps_state():
     e.outer()
     assert abs(e.niter - n1) <= 2
     e.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32_refine"])
+def test_gpu_autotuned_shape_keeps_parity(precision):
+    """The autotuner may pick any launch shape; only the summation order of
+    the reductions depends on it, so the solve stays within the bars."""
+    d = golden("core3d_2g_s1_solve")
+    p = generate("core3d_2g", int(d["seed"])).problem
+    e = Engine(p, precision=precision)
+    t = e.autotune()
+    assert t["candidates"] >= 1 and t["group_solve_ms"] > 0
+    e.outer()
+    assert abs(e.Ke - float(d["Ke"])) < K_TOL
+    assert pow_err(e.powdis(), d["pow"]) < POW_TOL
+    e.close()
