@@ -43,25 +43,19 @@ struct HipError {
     if (_e != hipSuccess) throw HipError{_e, #x}; \
   } while (0)
 
-template <typename T>
-T *dalloc(size_t n) {
-  void *p = nullptr;
-  if (n == 0) n = 1;
-  HIPCHK(hipMalloc(&p, n * sizeof(T)));
-  HIPCHK(hipMemset(p, 0, n * sizeof(T)));
-  return (T *)p;
-}
-
-// Copies of a running context go through its own stream (never the null
-// stream), so that contexts driven from several host threads do not
-// synchronise with each other or with another thread's graph capture.
+// Every fill and copy of a context goes through its own stream (never the
+// null stream) and is waited for.  The context stream is non-blocking, so a
+// null-stream fill (hipMemset returns before the fill lands) could otherwise
+// land after a later upload on the context stream; contexts driven from
+// several host threads also must not synchronise with each other or with
+// another thread's graph capture.
 void scopy(hipStream_t s, void *dst, const void *src, size_t bytes, hipMemcpyKind kind) {
   HIPCHK(hipMemcpyAsync(dst, src, bytes, kind, s));
   HIPCHK(hipStreamSynchronize(s));
 }
 
 template <typename T>
-T *dalloc_s(hipStream_t s, size_t n) {
+T *dalloc(hipStream_t s, size_t n) {
   void *p = nullptr;
   if (n == 0) n = 1;
   HIPCHK(hipMalloc(&p, n * sizeof(T)));
@@ -323,7 +317,7 @@ void upload_geo(adg_ctx *c) {
     q.hx = c->d_hx + h.hoff[0]; q.hy = c->d_hy + h.hoff[1]; q.hz = c->d_hz + h.hoff[2];
     q.flag = c->flag;
   }
-  if (!c->geo) c->geo = dalloc<Geo>(c->B);
+  if (!c->geo) c->geo = dalloc<Geo>(c->s, c->B);
   scopy(c->s, c->geo, gq.data(), gq.size() * sizeof(Geo), hipMemcpyHostToDevice);
 }
 
@@ -346,8 +340,8 @@ void install_flat(adg_ctx *c, long param) {
     c->tflat = nullptr;
     c->part = nullptr;
     c->flat_cap = (int)tf.size();
-    c->tflat = dalloc<Blk>(tf.size());
-    c->part = dalloc<double>((size_t)12 * tf.size());   // three rotating buffers of 4 x grid
+    c->tflat = dalloc<Blk>(c->s, tf.size());
+    c->part = dalloc<double>(c->s, (size_t)12 * tf.size());   // three rotating buffers of 4 x grid
   }
   c->nflat = (int)tf.size();
   scopy(c->s, c->tflat, tf.data(), tf.size() * sizeof(Blk), hipMemcpyHostToDevice);
@@ -373,8 +367,8 @@ void build_tables(adg_ctx *c) {
   }
   c->ncell = (int)tc.size();
   c->nface = (int)tfc.size();
-  c->tcell = dalloc<Blk>(tc.size());
-  c->tface = dalloc<Blk>(tfc.size());
+  c->tcell = dalloc<Blk>(c->s, tc.size());
+  c->tface = dalloc<Blk>(c->s, tfc.size());
   scopy(c->s, c->tcell, tc.data(), tc.size() * sizeof(Blk), hipMemcpyHostToDevice);
   scopy(c->s, c->tface, tfc.data(), tfc.size() * sizeof(Blk), hipMemcpyHostToDevice);
   install_flat(c, default_flat_param(c));
@@ -530,8 +524,8 @@ void ensure_trace(adg_ctx *c, int nout) {
     if (pp) (void)hipFree(pp);
   const int cap = std::max(nout, 2 * c->tcap);
   const size_t n = (size_t)cap * c->B;
-  c->tk = dalloc_s<double>(c->s, n); c->tser = dalloc_s<double>(c->s, n); c->tfer = dalloc_s<double>(c->s, n);
-  c->tnd = dalloc_s<double>(c->s, n); c->tflags = dalloc_s<int>(c->s, n); c->tloc = dalloc_s<int>(c->s, 3 * n);
+  c->tk = dalloc<double>(c->s, n); c->tser = dalloc<double>(c->s, n); c->tfer = dalloc<double>(c->s, n);
+  c->tnd = dalloc<double>(c->s, n); c->tflags = dalloc<int>(c->s, n); c->tloc = dalloc<int>(c->s, 3 * n);
   c->tcap = cap;
   if (c->gexec) {   // the captured k_end holds the old trace pointers
     (void)hipGraphExecDestroy(c->gexec);
@@ -753,9 +747,9 @@ int nparam(int G) { return 3 * G + G * (G - 1); }
 void diff_alloc(adg_ctx *c) {
   if (c->psi) return;
   const size_t NC = c->NC, G = c->G;
-  c->psi = dalloc<double>(G * NC); c->phis = dalloc<double>(G * NC); c->gam = dalloc<double>(G * NC);
-  c->exs = dalloc<double>(G * NC); c->fs0s = dalloc<double>(NC); c->dJdP = dalloc<double>(NC);
-  c->grad = dalloc<double>((size_t)nparam(c->G) * NC);
+  c->psi = dalloc<double>(c->s, G * NC); c->phis = dalloc<double>(c->s, G * NC); c->gam = dalloc<double>(c->s, G * NC);
+  c->exs = dalloc<double>(c->s, G * NC); c->fs0s = dalloc<double>(c->s, NC); c->dJdP = dalloc<double>(c->s, NC);
+  c->grad = dalloc<double>(c->s, (size_t)nparam(c->G) * NC);
 }
 
 void save_forward(adg_ctx *c) {
@@ -880,42 +874,42 @@ int create_batch(const adg_geom *geoms, int ncore, adg_ctx **out) {
     c->NC = base;
     HIPCHK(hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
     const size_t NC = c->NC, G = c->G;
-    c->d_hx = dalloc<double>(hx); c->d_hy = dalloc<double>(hy); c->d_hz = dalloc<double>(hz);
-    c->flag = dalloc<unsigned char>(NC);
-    c->vdel = dalloc<double>(NC);
+    c->d_hx = dalloc<double>(c->s, hx); c->d_hy = dalloc<double>(c->s, hy); c->d_hz = dalloc<double>(c->s, hz);
+    c->flag = dalloc<unsigned char>(c->s, NC);
+    c->vdel = dalloc<double>(c->s, NC);
     for (int b = 0; b < ncore; ++b) {
       const CoreHost &h = c->core[b];
-      HIPCHK(hipMemcpy(c->d_hx + h.hoff[0], h.hx.data(), h.nxx * sizeof(double), hipMemcpyHostToDevice));
-      HIPCHK(hipMemcpy(c->d_hy + h.hoff[1], h.hy.data(), h.nyy * sizeof(double), hipMemcpyHostToDevice));
-      HIPCHK(hipMemcpy(c->d_hz + h.hoff[2], h.hz.data(), h.nzz * sizeof(double), hipMemcpyHostToDevice));
-      HIPCHK(hipMemcpy(c->flag + h.base, fls[b].data(), h.box, hipMemcpyHostToDevice));
-      HIPCHK(hipMemcpy(c->vdel + h.base, vds[b].data(), h.box * sizeof(double), hipMemcpyHostToDevice));
+      scopy(c->s, c->d_hx + h.hoff[0], h.hx.data(), h.nxx * sizeof(double), hipMemcpyHostToDevice);
+      scopy(c->s, c->d_hy + h.hoff[1], h.hy.data(), h.nyy * sizeof(double), hipMemcpyHostToDevice);
+      scopy(c->s, c->d_hz + h.hoff[2], h.hz.data(), h.nzz * sizeof(double), hipMemcpyHostToDevice);
+      scopy(c->s, c->flag + h.base, fls[b].data(), h.box, hipMemcpyHostToDevice);
+      scopy(c->s, c->vdel + h.base, vds[b].data(), h.box * sizeof(double), hipMemcpyHostToDevice);
     }
-    c->D = dalloc<double>(G * NC); c->sigr = dalloc<double>(G * NC); c->nuf = dalloc<double>(G * NC);
-    c->sigf = dalloc<double>(G * NC); c->chi = dalloc<double>(G * NC); c->sigs = dalloc<double>(G * G * NC);
-    c->dc = dalloc<double>(6 * G * NC); c->exsrc = dalloc<double>(G * NC);
-    c->f0 = dalloc<double>(G * NC); c->f0c = dalloc<double>(G * NC); c->fs0 = dalloc<double>(NC);
-    c->fs0c = dalloc<double>(NC); c->errn = dalloc<double>(NC);
-    c->dn = dalloc<double>(6 * G * NC); c->df = dalloc<double>(6 * G * NC);
-    c->A64 = dalloc<double>(7 * G * NC); c->b = dalloc<double>(NC); c->pw = dalloc<double>(NC);
-    c->tmp = dalloc<double>(NC);
+    c->D = dalloc<double>(c->s, G * NC); c->sigr = dalloc<double>(c->s, G * NC); c->nuf = dalloc<double>(c->s, G * NC);
+    c->sigf = dalloc<double>(c->s, G * NC); c->chi = dalloc<double>(c->s, G * NC); c->sigs = dalloc<double>(c->s, G * G * NC);
+    c->dc = dalloc<double>(c->s, 6 * G * NC); c->exsrc = dalloc<double>(c->s, G * NC);
+    c->f0 = dalloc<double>(c->s, G * NC); c->f0c = dalloc<double>(c->s, G * NC); c->fs0 = dalloc<double>(c->s, NC);
+    c->fs0c = dalloc<double>(c->s, NC); c->errn = dalloc<double>(c->s, NC);
+    c->dn = dalloc<double>(c->s, 6 * G * NC); c->df = dalloc<double>(c->s, 6 * G * NC);
+    c->A64 = dalloc<double>(c->s, 7 * G * NC); c->b = dalloc<double>(c->s, NC); c->pw = dalloc<double>(c->s, NC);
+    c->tmp = dalloc<double>(c->s, NC);
     if (c->precision == ADG_FP32_REFINE) {
-      c->A32 = dalloc<float>(7 * G * NC);
-      c->rs32 = dalloc<float>(NC); c->t32 = dalloc<float>(NC); c->d32 = dalloc<float>(NC);
+      c->A32 = dalloc<float>(c->s, 7 * G * NC);
+      c->rs32 = dalloc<float>(c->s, NC); c->t32 = dalloc<float>(c->s, NC); c->d32 = dalloc<float>(c->s, NC);
       for (int k = 0; k < 2; ++k) {
-        c->rr32[k] = dalloc<float>(NC); c->pp32[k] = dalloc<float>(NC); c->vv32[k] = dalloc<float>(NC);
+        c->rr32[k] = dalloc<float>(c->s, NC); c->pp32[k] = dalloc<float>(c->s, NC); c->vv32[k] = dalloc<float>(c->s, NC);
       }
     } else {
-      c->rs64 = dalloc<double>(NC); c->t64 = dalloc<double>(NC);
+      c->rs64 = dalloc<double>(c->s, NC); c->t64 = dalloc<double>(c->s, NC);
       for (int k = 0; k < 2; ++k) {
-        c->rr64[k] = dalloc<double>(NC); c->pp64[k] = dalloc<double>(NC); c->vv64[k] = dalloc<double>(NC);
+        c->rr64[k] = dalloc<double>(c->s, NC); c->pp64[k] = dalloc<double>(c->s, NC); c->vv64[k] = dalloc<double>(c->s, NC);
       }
     }
-    c->S = dalloc<double>(3 * G * NC); c->a2 = dalloc<double>(3 * G * NC); c->a4 = dalloc<double>(3 * G * NC);
-    c->L1 = dalloc<double>(3 * G * NC);
+    c->S = dalloc<double>(c->s, 3 * G * NC); c->a2 = dalloc<double>(c->s, 3 * G * NC); c->a4 = dalloc<double>(c->s, 3 * G * NC);
+    c->L1 = dalloc<double>(c->s, 3 * G * NC);
     build_tables(c);
-    c->npart = dalloc<double>((size_t)3 * 3 * c->nface);
-    c->sc = dalloc<Scal>(ncore);
+    c->npart = dalloc<double>(c->s, (size_t)3 * 3 * c->nface);
+    c->sc = dalloc<Scal>(c->s, ncore);
     HIPCHK(hipHostMalloc((void **)&c->hsc, ncore * sizeof(Scal), hipHostMallocDefault));
   } catch (const HipError &e) {
     const int rc = hip_fail(c, e);
@@ -1350,7 +1344,7 @@ int adg_autotune(adg_ctx *c, int nin, double *out4) {
       cand = {NTF, 2L * NTF, 4L * NTF};
     }
     const size_t nf = (size_t)c->G * c->NC;
-    double *keep = dalloc_s<double>(c->s, nf);
+    double *keep = dalloc<double>(c->s, nf);
     HIPCHK(hipMemcpyAsync(keep, c->f0, nf * sizeof(double), hipMemcpyDeviceToDevice, c->s));
     set_scal_start(c, nullptr);
     launch_coupcoef(c);
