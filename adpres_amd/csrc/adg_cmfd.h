@@ -534,7 +534,8 @@ __global__ void __launch_bounds__(NTF) k_post(LV lv, XS xs, double *fs0, const d
   }
 }
 
-// one block per core: k = k * f / fc (:484-485), ser, e1 = e2
+// one block per core: k = k * f / fc (:484-485; not in the fixed-source
+// outer, outer_fs :513-598, where k keeps its value), ser, e1 = e2
 __global__ void __launch_bounds__(NTF) k_fin_outer(LV lv, const double *pin) {
   Scal *sc = lv.sc + blockIdx.x;
   if (sc->done) return;
@@ -545,7 +546,7 @@ __global__ void __launch_bounds__(NTF) k_fin_outer(LV lv, const double *pin) {
   if (threadIdx.x == 0) {
     const double fc = sc->f;
     sc->f = red[0];
-    sc->Ke = sc->Ke * red[0] / fc;
+    if (!sc->fixed_k) sc->Ke = sc->Ke * red[0] / fc;
     sc->ser = red[1];
     sc->e1 = sc->e2;
   }

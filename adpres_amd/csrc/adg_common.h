@@ -91,7 +91,8 @@ struct Scal {
   unsigned long long lu_key;     // first LU pivot failure (order key << 30 | packed loc)
   // iteration schedule of the running outer call (%ITER): outer p+1 extrapolates
   // when (p+1) % nac == 0 and updates the D-hats when nodal_on && (p+1) % nupd == 0
-  int nac, nupd, nodal_on, pad2;
+  int nac, nupd, nodal_on;
+  int fixed_k;                   // fixed-source outer (outer_fs): k is not updated
   double serc, ferc;
   double aux;                    // generalized-adjoint outer: psi^T F phi of the projection
 };

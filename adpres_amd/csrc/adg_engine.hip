@@ -493,7 +493,7 @@ void nodal_update(adg_ctx *c, int kern, int cmode, int sched) {
 
 // host-side initialisation of the device scalars of every core for an outer
 // call; nit = 1: one %ITER for all cores, nit = B: one per core
-void set_scal_start(adg_ctx *c, const adg_iter *its, int nit = 1) {
+void set_scal_start(adg_ctx *c, const adg_iter *its, int nit = 1, int fixed_k = 0) {
   std::vector<Scal> h(c->B);
   scopy(c->s, h.data(), c->sc, c->B * sizeof(Scal), hipMemcpyDeviceToHost);
   for (int b = 0; b < c->B; ++b) {
@@ -513,6 +513,7 @@ void set_scal_start(adg_ctx *c, const adg_iter *its, int nit = 1) {
     s.serc = it ? it->serc : 0.0;
     s.ferc = it ? it->ferc : 0.0;
     s.aux = c->psifphi;
+    s.fixed_k = fixed_k;
   }
   scopy(c->s, c->sc, h.data(), c->B * sizeof(Scal), hipMemcpyHostToDevice);
 }
@@ -610,7 +611,7 @@ int run_outer(adg_ctx *c, int mode, const adg_iter *it, int nit = 1) {
     if (q.nout != it->nout || q.nin != it->nin || q.kern != it->kern)
       return fail(c, ADG_E_ARG, "nout, nin and the nodal kernel must be the same for every core");
   }
-  if (mode != ADG_FORWARD && mode != ADG_ADJOINT) return fail(c, ADG_E_ARG, "bad mode");
+  if (mode != ADG_FORWARD && mode != ADG_ADJOINT && mode != ADG_FIXED_SOURCE) return fail(c, ADG_E_ARG, "bad mode");
   const int adjoint = mode == ADG_ADJOINT;
   const int nout = it->nout;
   const LV lv = c->lv(c->tflat);
@@ -630,7 +631,7 @@ int run_outer(adg_ctx *c, int mode, const adg_iter *it, int nit = 1) {
       any_first = true;
     }
   }
-  set_scal_start(c, it, nit);
+  set_scal_start(c, it, nit, mode == ADG_FIXED_SOURCE ? 1 : 0);
   c->ev_next = 0;
   c->recs.clear();
   HIPCHK(hipEventRecord(e0, c->s));

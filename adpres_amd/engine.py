@@ -28,6 +28,7 @@ LIB_PATH = os.environ.get("ADG_LIBRARY") or os.path.join(HERE, "libadpres_gpu.so
 
 ADG_OK, ADG_E_MAX_OUTER, ADG_E_NODAL_UNSTABLE, ADG_E_LU_PIVOT, ADG_E_POWER_NONPOS = 0, 1, 2, 3, 4
 ADG_E_ARG, ADG_E_HIP, ADG_E_NOT_SET = 5, 6, 7
+ADG_FORWARD, ADG_ADJOINT, ADG_FIXED_SOURCE = 0, 1, 2       # enum adg_mode
 FP64, FP32_REFINE = 0, 1
 PRECISIONS = {"fp64": FP64, "fp32_refine": FP32_REFINE}
 KERNS = {"FDM": 0, "PNM": 1, "SANM": 2}
@@ -209,17 +210,24 @@ class Engine:
         return Ke.value
 
     # -- hot path --------------------------------------------------------
-    def outer(self, adjoint: Optional[bool] = None, raise_on_error: bool = True, **ov):
+    def outer(self, adjoint: Optional[bool] = None, raise_on_error: bool = True,
+              fixed_source: Optional[bool] = None, **ov):
+        """cmfd::outer, outer_ad (adjoint) or outer_fs (fixed_source: the
+        forward outer with the external source and k held at its value); the
+        default follows the problem's %MODE."""
         p = self.p
         it = _iter(p, ov)
         if adjoint is None:
             adjoint = p.mode == "ADJOINT"
+        if fixed_source is None:
+            fixed_source = p.mode == "FIXEDSRC"
+        mode = ADG_FIXED_SOURCE if fixed_source else (ADG_ADJOINT if adjoint else ADG_FORWARD)
         cap = it.nout
         t = {k: np.zeros(cap) for k in ("k", "ser", "fer", "ndmax")}
         t["flags"] = np.zeros(cap, np.int32)
         t["loc"] = np.zeros(3 * cap, np.int32)
         tr = AdgTrace(cap, 0, _d(t["k"]), _d(t["ser"]), _d(t["fer"]), _d(t["ndmax"]), _i(t["flags"]), _i(t["loc"]))
-        rc = self.lib.adg_outer(self.h, 1 if adjoint else 0, C.byref(it), C.byref(tr))
+        rc = self.lib.adg_outer(self.h, mode, C.byref(it), C.byref(tr))
         n = tr.n
         self.niter = n
         self.trace = {k: (v[:n] if k != "loc" else v[:3 * n].reshape(-1, 3)) for k, v in t.items()}
@@ -352,13 +360,14 @@ class Batch:
         for b in range(len(self)):
             self._check(self.lib.adg_set_state_core(self.h, b, None, None, 1.0, None, 1), "adg_set_state_core")
 
-    def outer(self, adjoint: bool = False, **ov):
+    def outer(self, adjoint: bool = False, fixed_source: bool = False, **ov):
         """Solve every core; returns (status, outers, k) arrays.  Per-core
         failures are reported in `status`, not raised."""
         n = len(self)
         its = (AdgIter * n)(*[_iter(q, ov) for q in self.problems])
         st = np.zeros(n, np.int32); ni = np.zeros(n, np.int32); ke = np.zeros(n)
-        rc = self.lib.adg_outer_batch(self.h, 1 if adjoint else 0, its, n, _i(st), _i(ni), _d(ke))
+        mode = ADG_FIXED_SOURCE if fixed_source else (ADG_ADJOINT if adjoint else ADG_FORWARD)
+        rc = self.lib.adg_outer_batch(self.h, mode, its, n, _i(st), _i(ni), _d(ke))
         if rc in (ADG_E_ARG, ADG_E_NOT_SET, ADG_E_HIP):
             self._check(rc, "adg_outer_batch")
         self.status, self.niter, self.keff = st, ni, ke

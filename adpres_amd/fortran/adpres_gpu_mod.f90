@@ -12,7 +12,7 @@ module adpres_gpu
   integer(c_int), parameter :: ADG_OK = 0, ADG_E_MAX_OUTER = 1, ADG_E_NODAL_UNSTABLE = 2, &
                                ADG_E_LU_PIVOT = 3, ADG_E_POWER_NONPOS = 4
   integer(c_int), parameter :: ADG_KERN_FDM = 0, ADG_KERN_PNM = 1, ADG_KERN_SANM = 2
-  integer(c_int), parameter :: ADG_FORWARD = 0, ADG_ADJOINT = 1
+  integer(c_int), parameter :: ADG_FORWARD = 0, ADG_ADJOINT = 1, ADG_FIXED_SOURCE = 2
   integer(c_int), parameter :: ADG_FP64 = 0, ADG_FP32_REFINE = 1
 
   type, bind(C) :: adg_geom_t

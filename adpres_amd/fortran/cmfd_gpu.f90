@@ -10,10 +10,11 @@
 ! sdata arrays f0, fs0 and Ke after every call, so the host's outputs
 ! (AsmPow, AxiPow, AsmFlux) and callers (BCSEARCH) see what they expect.
 ! Printed lines follow the reference's formats (src/mod_cmfd.f90:331,
-! 367-374, 491-494, 500-504, 589-591; src/mod_nodal.f90:131-142).
+! 367-374, 491-494, 500-504, 583-584, 589-591; src/mod_nodal.f90:131-142).
 !
-! The fixed-source, T-H and transient outer variants are not served by this
-! build; they stop with a message.
+! The fixed-source outer (outer_fs) runs on the GPU with k held fixed.  The
+! T-H and transient outer variants are not served by this build; they stop
+! with a message.
 
 module nodal
   use sdata, only: dp
@@ -40,6 +41,7 @@ module cmfd
 
   type(c_ptr) :: ctx = c_null_ptr
   logical     :: started = .false.
+  logical     :: fixed_mode = .false.   ! outer_fs prints no k
   integer(c_int), allocatable, target :: gix(:), giy(:), giz(:)
   real(c_double), allocatable, target :: gxd(:), gyd(:), gzd(:)
   real(c_double), allocatable, target :: tk(:), tser(:), tfer(:), tnd(:)
@@ -100,6 +102,7 @@ contains
     integer(c_int), allocatable :: imat(:)
 
     call ensure_context()
+    fixed_mode = mode == ADG_FIXED_SOURCE
     allocate(imat(nnod))
     imat = mat
     call check(adg_set_xs(ctx, D, sigr, sigs, nuf, sigf, chi, imat, dc, exsrc), 'ADG_SET_XS')
@@ -171,7 +174,7 @@ contains
       call check(rco, 'ADG_OUTER')
     end select
 
-    if (popt > 0) then
+    if (popt > 0 .and. .not. fixed_mode) then
       write(ounit,*)
       write(ounit,'(2X,"MULTIPLICATION EFFECTIVE (K-EFF) = ", F9.6)') Ke
       if (scr) then
@@ -199,8 +202,13 @@ contains
       write(ounit,1145) tnd(p), tloc(3*p-2), tloc(3*p-1), tloc(3*p)
       if (scr) write(*,1145) tnd(p), tloc(3*p-2), tloc(3*p-1), tloc(3*p)
     end if
-    write(ounit,'(I5,F13.6,2ES15.5)') p, tk(p), tser(p), tfer(p)
-    if (scr) write(*,'(I5,F13.6,2ES15.5)') p, tk(p), tser(p), tfer(p)
+    if (fixed_mode) then
+      write(ounit,'(I5,2ES15.5)') p, tser(p), tfer(p)
+      if (scr) write(*,'(I5,2ES15.5)') p, tser(p), tfer(p)
+    else
+      write(ounit,'(I5,F13.6,2ES15.5)') p, tk(p), tser(p), tfer(p)
+      if (scr) write(*,'(I5,F13.6,2ES15.5)') p, tk(p), tser(p), tfer(p)
+    end if
 1145 FORMAT ('MAX. CHANGE IN NODAL COUPLING COEF.= ', ES12.5, &
              ' AT NODE I = ', I2, ', J = ', I2, ', K = ', I2)
   end subroutine
@@ -223,9 +231,10 @@ contains
 
   subroutine outer_fs(popt)
     integer, optional, intent(in) :: popt
-    if (present(popt)) continue
-    write(*,*) '  FIXED-SOURCE OUTER ITERATION IS NOT SERVED BY THE GPU BUILD'
-    stop
+    integer :: po
+    po = 0
+    if (present(popt)) po = popt
+    call run_outer(po, ADG_FIXED_SOURCE, 'FIXED-SOURCE')
   end subroutine outer_fs
 
   subroutine outer_th(maxn)

@@ -333,6 +333,9 @@ def read_ref_dump(path: str, action: str = "solve"):
         res["pow"] = r.take("<f8", N)
         res["df"] = r.take("<f8", 6 * G * N).reshape(6, G, N)
         res["dn"] = r.take("<f8", 6 * G * N).reshape(6, G, N)
+        if r.off < len(r.buf):      # a fixed-source deck: its %ESRC source follows (outer_fs)
+            p.exsrc = r.take("<f8", G * N).reshape(sh2)
+            p.mode = "FIXEDSRC"
     elif action == "nodal":
         res["df"] = r.take("<f8", 6 * G * N).reshape(6, G, N)
         res["dn"] = r.take("<f8", 6 * G * N).reshape(6, G, N)
@@ -343,16 +346,20 @@ def read_ref_dump(path: str, action: str = "solve"):
     return p, res
 
 
-def parse_trace(text: str):
-    """Per-outer trace lines 'p k ser fer' printed by outer (src/mod_cmfd.f90:491-494)."""
+def parse_trace(text: str, fixed_source: bool = False):
+    """Per-outer trace lines 'p k ser fer' printed by outer (src/mod_cmfd.f90:491-494),
+    or 'p ser fer' printed by outer_fs (:583-584), returned as (p, k, ser, fer)
+    with k = NaN for the latter."""
     rows = []
+    width = 3 if fixed_source else 4
     for line in text.splitlines():
         t = line.split()
-        if len(t) == 4 and t[0].isdigit():
+        if len(t) == width and t[0].isdigit():
             try:
-                rows.append((int(t[0]), float(t[1]), float(t[2]), float(t[3])))
+                v = [float(x) for x in t[1:]]
             except ValueError:
-                pass
+                continue
+            rows.append((int(t[0]), *(([float("nan")] + v) if fixed_source else v)))
     return rows
 
 

@@ -39,7 +39,7 @@ enum adg_status {
 };
 
 enum adg_kern { ADG_KERN_FDM = 0, ADG_KERN_PNM = 1, ADG_KERN_SANM = 2 };   /* %KERN */
-enum adg_mode { ADG_FORWARD = 0, ADG_ADJOINT = 1 };                       /* %MODE */
+enum adg_mode { ADG_FORWARD = 0, ADG_ADJOINT = 1, ADG_FIXED_SOURCE = 2 };  /* %MODE */
 enum adg_precision {
   ADG_FP64 = 0,          /* plain FP64 drop-in                                        */
   ADG_FP32_REFINE = 1    /* FP32 BiCGSTAB on an FP64 defect (residual) each group solve */
@@ -94,8 +94,11 @@ int adg_set_state(adg_ctx *ctx, const double *f0, const double *fs0, double Ke,
                   const double *dn, int first);
 int adg_get_state(adg_ctx *ctx, double *f0, double *fs0, double *Ke, double *dn, double *df);
 
-/* cmfd::outer (src/mod_cmfd.f90:415-509) for mode ADG_FORWARD and
- * cmfd::outer_ad (:602-699) for ADG_ADJOINT.  trace may be NULL. */
+/* cmfd::outer (src/mod_cmfd.f90:415-509) for mode ADG_FORWARD,
+ * cmfd::outer_ad (:602-699) for ADG_ADJOINT and cmfd::outer_fs (:513-598)
+ * for ADG_FIXED_SOURCE (the forward outer with the external source of
+ * adg_set_xs and k held at its value: 1 on the first call).  trace may be
+ * NULL. */
 int adg_outer(adg_ctx *ctx, int mode, const adg_iter *iter, adg_trace *trace);
 
 /* cmfd::PowDis (src/mod_cmfd.f90:1290-1333): node powers normalised to 1. */

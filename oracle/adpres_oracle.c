@@ -952,8 +952,11 @@ static double releg(const double *nw, const double *od, int N, int G) {
 }
 
 /* outer (src/mod_cmfd.f90:415-509) and outer_ad (:602-699) */
-int orc_outer(const orc_problem *pb, orc_state *st, const orc_iter *it, int adjoint, orc_trace *tr) {
+int orc_outer(const orc_problem *pb, orc_state *st, const orc_iter *it, int mode, orc_trace *tr) {
   int N = pb->nnod, G = pb->ng;
+  /* mode 0: outer, 1: outer_ad, 2: outer_fs (src/mod_cmfd.f90:513-598: the
+     forward outer with k held at its value, no k update) */
+  const int adjoint = mode == 1, fixed = mode == 2;
   geo_t q;
   int rc = geo_init(&q, pb);
   if (rc != ORC_OK) { geo_free(&q); return rc; }
@@ -1003,8 +1006,10 @@ int orc_outer(const orc_problem *pb, orc_state *st, const orc_iter *it, int adjo
       flags |= 1;
     }
     e1 = e2;
-    f = integrate(pb, vdel, st->fs0);
-    st->Ke = Keo * f / fc;
+    if (!fixed) {
+      f = integrate(pb, vdel, st->fs0);
+      st->Ke = Keo * f / fc;
+    }
     double ser = rele(st->fs0, fs0c, N);
     double fer = releg(st->f0, f0c, N, G);
     if (p % it->nupd == 0 && it->kern != 0) {

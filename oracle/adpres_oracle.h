@@ -67,9 +67,10 @@ typedef struct {
   int *loc;        /* 3 per outer: im jm km after a nodal update */
 } orc_trace;
 
-/* Whole static solve: outer (src/mod_cmfd.f90:415) or outer_ad (:602). */
+/* Whole static solve: mode 0 outer (src/mod_cmfd.f90:415), 1 outer_ad
+   (:602), 2 outer_fs (:513, fixed source: k is not updated). */
 int orc_outer(const orc_problem *pb, orc_state *st, const orc_iter *it,
-              int adjoint, orc_trace *tr);
+              int mode, orc_trace *tr);
 
 /* coup_coef (src/mod_cmfd.f90:11): fills st->df. */
 void orc_coup_coef(const orc_problem *pb, orc_state *st);

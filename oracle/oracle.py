@@ -121,7 +121,7 @@ class Oracle:
         return self.st.Ke
 
     def outer(self, adjoint=None, nout=None, nin=None, serc=None, ferc=None, nac=None, nupd=None,
-              kern=None, trace_cap=None):
+              kern=None, trace_cap=None, fixed_source=None):
         p = self.p
         it = OrcIter(nout if nout is not None else p.nout, nin if nin is not None else p.nin,
                      nac if nac is not None else p.nac, nupd if nupd is not None else p.nupd,
@@ -129,13 +129,16 @@ class Oracle:
                      serc if serc is not None else p.serc, ferc if ferc is not None else p.ferc)
         if adjoint is None:
             adjoint = p.mode == "ADJOINT"
+        if fixed_source is None:
+            fixed_source = p.mode == "FIXEDSRC"
+        mode = 2 if fixed_source else int(bool(adjoint))
         cap = trace_cap or it.nout
         self.trace = {k: np.zeros(cap) for k in ("k", "ser", "fer", "ndmax")}
         self.trace["flags"] = np.zeros(cap, np.int32)
         self.trace["loc"] = np.zeros(3 * cap, np.int32)
         tr = OrcTrace(cap, 0, _d(self.trace["k"]), _d(self.trace["ser"]), _d(self.trace["fer"]),
                       _d(self.trace["ndmax"]), _i(self.trace["flags"]), _i(self.trace["loc"]))
-        rc = lib().orc_outer(C.byref(self.pb), C.byref(self.st), C.byref(it), int(adjoint), C.byref(tr))
+        rc = lib().orc_outer(C.byref(self.pb), C.byref(self.st), C.byref(it), mode, C.byref(tr))
         self.niter = tr.n
         for k in ("k", "ser", "fer", "ndmax", "flags"):
             self.trace[k] = self.trace[k][:tr.n]

@@ -4,7 +4,8 @@
 ! exposes three actions so that the reference's own FP64 arithmetic can be
 ! written out at full precision:
 !
-!   ref_dump DECK solve OUT [XSFILE|-]          whole static solve
+!   ref_dump DECK solve OUT [XSFILE|-]          whole static solve (outer,
+!                                               outer_ad or outer_fs by %MODE)
 !   ref_dump DECK nodal OUT XSFILE|- STATEFILE  one nodal (SANM/PNM) update
 !   ref_dump DECK bicg  OUT XSFILE|- STATEFILE  one nin-iteration BiCGSTAB per group
 !
@@ -24,7 +25,7 @@ program ref_dump
   use sdata
   use io,    only: inp_read
   use xsec,  only: XS_updt, Dsigr_updt
-  use cmfd,  only: outer, outer_ad, PowDis, matrix_setup, bicg
+  use cmfd,  only: outer, outer_ad, outer_fs, PowDis, matrix_setup, bicg
   use nodal, only: nodal_update, nodal_update_pnm
 
   implicit none
@@ -81,6 +82,8 @@ program ref_dump
   case ('solve')
     if (trim(mode) == 'ADJOINT') then
       call outer_ad(1)
+    else if (trim(mode) == 'FIXEDSRC') then
+      call outer_fs(1)
     else
       call outer(1)
     end if
@@ -94,6 +97,8 @@ program ref_dump
       end do
     end do
     write(u) Ke, f0, fs0, npw, dfout, dnout
+    ! a fixed-source deck's external source (%ESRC, mod_io.f90:1375) follows
+    if (trim(mode) == 'FIXEDSRC') write(u) exsrc
 
   case ('nodal')
     call read_state()

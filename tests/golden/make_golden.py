@@ -32,7 +32,7 @@ REF = "/root/reference/smpl/static"
 DUMP = os.path.join(ROOT, "oracle", "_ref", "ref_dump")
 
 DECKS = {"iaea3d": "IAEA3Ds", "iaea2d": "IAEA2D", "pnm": "PNM", "biblis": "BIBLIS", "koeberg": "KOEBERG",
-         "dvp": "DVP", "adjoint": "adjoint", "fdm": "FDM"}
+         "dvp": "DVP", "adjoint": "adjoint", "fdm": "FDM", "fixed_source": "fixed_source"}
 
 
 def run(workdir, deck, action, xs="-", state=None):
@@ -51,8 +51,8 @@ def sha(p):
     return h.hexdigest()
 
 
-def trace_arrays(stdout):
-    t = np.asarray(parse_trace(stdout), dtype=np.float64).reshape(-1, 4)
+def trace_arrays(stdout, fixed_source=False):
+    t = np.asarray(parse_trace(stdout, fixed_source), dtype=np.float64).reshape(-1, 4)
     return dict(trace_p=t[:, 0].astype(np.int32), trace_k=t[:, 1], trace_ser=t[:, 2], trace_fer=t[:, 3])
 
 
@@ -67,9 +67,10 @@ def deck_solve(name, deck):
         shutil.copy(os.path.join(REF, deck), os.path.join(d, "deck"))
         p, res, out = run(d, "deck", "solve")
     p.name = name
-    p.mode = "ADJOINT" if name == "adjoint" else "FORWARD"
+    if p.mode != "FIXEDSRC":
+        p.mode = "ADJOINT" if name == "adjoint" else "FORWARD"
     save(name + "_solve", **problem_to_dict(p), Ke=res["Ke"], f0=res["f0"], fs0=res["fs0"], pow=res["pow"],
-         dn=res["dn"], df=res["df"], **trace_arrays(out))
+         dn=res["dn"], df=res["df"], **trace_arrays(out, p.mode == "FIXEDSRC"))
     return p, res
 
 

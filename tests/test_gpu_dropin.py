@@ -66,3 +66,24 @@ def test_dropin_iaea3d_published_lines(tmp_path):
 def test_dropin_fp32_refine(tmp_path):
     rows, k, out = run_deck("IAEA3Ds", tmp_path, env={"ADPRES_GPU_PRECISION": "FP32"})
     assert abs(k - 1.029082) <= 1.5e-6
+
+
+def test_dropin_fixed_source(tmp_path):
+    """%MODE FIXEDSRC: the host's fixedsrc() calls outer_fs, served on the GPU
+    with k held at 1; its printed 'p ser fer' lines (src/mod_cmfd.f90:583-584)
+    follow the reference's."""
+    deck = "fixed_source"
+    shutil.copy(os.path.join(GOLDEN, "decks", deck), tmp_path / deck)
+    r = subprocess.run(["bash", "-c", f"ulimit -s unlimited; exec {EXE} {deck}"], cwd=tmp_path,
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-3000:]
+    assert "ADPRES EXIT NORMALLY" in r.stdout
+    line = re.compile(r"^\s*(\d+)\s+(\S+E[+-]\d\d)\s+(\S+E[+-]\d\d)\s*$")
+    rows = [(int(m.group(1)), float(m.group(2)), float(m.group(3)))
+            for m in map(line.match, r.stdout.splitlines()) if m]
+    d = golden("fixed_source_solve")
+    n = len(d["trace_ser"])
+    assert abs(len(rows) - n) <= 2
+    # the early outers print identical digits (same algorithm, same arithmetic order)
+    assert np.allclose([q[1] for q in rows[:20]], d["trace_ser"][:20], rtol=1e-4)
+    assert "MULTIPLICATION EFFECTIVE" not in r.stdout

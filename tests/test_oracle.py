@@ -33,10 +33,14 @@ def test_oracle_bit_exact_vs_reference_solve(oracle_lib, name):
         assert np.array_equal(o.df, d["df"])
     rc, pw = o.powdis()
     assert rc == 0 and np.array_equal(pw, d["pow"])
-    # printed trace (F13.6 / ES15.5) reproduced digit for digit
+    # printed trace (F13.6 / ES15.5) reproduced digit for digit; outer_fs
+    # prints no k (it stays at its first-call value 1)
     n = len(d["trace_k"])
     assert o.niter == n
-    assert np.array_equal(np.round(o.trace["k"], 6), d["trace_k"])
+    if str(d["p_mode"]) == "FIXEDSRC":
+        assert np.all(np.isnan(d["trace_k"])) and np.all(o.trace["k"] == 1.0)
+    else:
+        assert np.array_equal(np.round(o.trace["k"], 6), d["trace_k"])
     for key in ("ser", "fer"):
         got = np.array([float("%.5E" % v) for v in o.trace[key]])
         assert np.array_equal(got, d["trace_" + key]), key
@@ -180,6 +184,33 @@ def test_oracle_fdm_k_is_dominant_eigenvalue(oracle_lib):
     ev = scipy.linalg.eigvals(np.linalg.solve(M, F))
     kmax = np.max(ev.real)
     assert abs(o.Ke - kmax) < 1e-9
+
+
+def test_oracle_fdm_fixed_source_is_direct_solve(oracle_lib):
+    """outer_fs with FDM on a subcritical core converges to the solution of
+    (M - F) phi = exsrc (k held at 1), assembled independently."""
+    import dataclasses
+    d = golden("core2d_small_s1_solve")
+    p0 = problem_from_dict(d)
+    scale = 0.8 / float(d["Ke"])
+    rng = np.random.default_rng(11)
+    ex = rng.uniform(0.0, 1.0, size=(p0.ng, p0.nnod)) * (p0.nuf > 0)
+    p = dataclasses.replace(p0, nuf=p0.nuf * scale, exsrc=ex, mode="FIXEDSRC").with_iter(
+        kern="FDM", nout=20000, nin=10, serc=1e-13, ferc=1e-13, nac=1000)
+    o = oracle_lib.Oracle(p)
+    assert o.outer() == 0
+    assert o.Ke == 1.0
+    N, G = p.nnod, p.ng
+    M = np.zeros((G * N, G * N))
+    chi = p.chi[:, p.mat - 1]
+    for g in range(G):
+        M[g * N:(g + 1) * N, g * N:(g + 1) * N] = assemble_dense(p, o, g)
+        for h in range(G):
+            if h != g:
+                M[g * N:(g + 1) * N, h * N:(h + 1) * N] -= np.diag(p.sigs[g, h])
+            M[g * N:(g + 1) * N, h * N:(h + 1) * N] -= np.diag(chi[g] * p.nuf[h])
+    phi = np.linalg.solve(M, ex.reshape(-1)).reshape(G, N)
+    assert np.max(np.abs(o.f0 - phi)) / np.max(np.abs(phi)) < 1e-9
 
 
 def test_oracle_bicg_converges_to_direct_solve(oracle_lib):
