@@ -12,9 +12,10 @@
 ! Printed lines follow the reference's formats (src/mod_cmfd.f90:331,
 ! 367-374, 491-494, 500-504, 583-584, 589-591; src/mod_nodal.f90:131-142).
 !
-! The fixed-source outer (outer_fs) runs on the GPU with k held fixed.  The
-! T-H and transient outer variants are not served by this build; they stop
-! with a message.
+! The fixed-source outer (outer_fs) runs on the GPU with k held fixed, and
+! the T-H outer (outer_th) as a bounded forward outer between T-H updates.
+! The transient outer variant is not served by this build; it stops with a
+! message.
 
 module nodal
   use sdata, only: dp
@@ -83,15 +84,17 @@ contains
     stop
   end subroutine
 
-  ! one static solve on the device (shared by outer and outer_ad)
-  subroutine run_outer(popt, mode, label)
+  ! one static solve on the device (shared by outer, outer_ad, outer_fs and
+  ! outer_th; maxn: outer_th's own outer limit, reached without a stop)
+  subroutine run_outer(popt, mode, label, maxn)
     use sdata, only: nnod, ng, nout, nin, serc, ferc, nac, nupd, kern, f0, fs0, Ke, &
                      D, sigr, sigs, nuf, sigf, chi, mat, dc, exsrc, ndmax, im, jm, km, &
-                     fdm_time, nod_time
+                     fdm_time, nod_time, ser, fer
     use io, only: ounit, scr
     integer, intent(in) :: popt
     integer(c_int), intent(in) :: mode
     character(len=*), intent(in) :: label
+    integer, optional, intent(in) :: maxn
     type(adg_iter_t)  :: it
     type(adg_trace_t) :: tr
     integer(c_int)    :: rc, rco, first
@@ -115,14 +118,15 @@ contains
     end if
 
     it%nout = nout; it%nin = nin; it%nac = nac; it%nupd = nupd
+    if (present(maxn)) it%nout = maxn
     it%serc = serc; it%ferc = ferc
     it%kern = ADG_KERN_SANM
     if (kern == ' FDM') it%kern = ADG_KERN_FDM
     if (kern == ' PNM') it%kern = ADG_KERN_PNM
 
     if (allocated(tk)) deallocate(tk, tser, tfer, tnd, tfl, tloc)
-    allocate(tk(nout), tser(nout), tfer(nout), tnd(nout), tfl(nout), tloc(3 * nout))
-    tr%cap = nout; tr%n = 0
+    allocate(tk(it%nout), tser(it%nout), tfer(it%nout), tnd(it%nout), tfl(it%nout), tloc(3 * it%nout))
+    tr%cap = it%nout; tr%n = 0
     tr%k = c_loc(tk); tr%ser = c_loc(tser); tr%fer = c_loc(tfer); tr%ndmax = c_loc(tnd)
     tr%flags = c_loc(tfl); tr%loc = c_loc(tloc)
 
@@ -141,11 +145,13 @@ contains
     Ke = kk
     if (tr%n > 0) then
       ndmax = tnd(tr%n); im = tloc(3*tr%n-2); jm = tloc(3*tr%n-1); km = tloc(3*tr%n)
+      ser = tser(tr%n); fer = tfer(tr%n)   ! the last outer's errors, as the callers print them
     end if
     rc = adg_last_timing(ctx, ms)
     fdm_time = fdm_time + (ms(1) - ms(3)) * 1.e-3_dp
     nod_time = nod_time + ms(3) * 1.e-3_dp
 
+    if (present(maxn) .and. rco == ADG_E_MAX_OUTER) rco = ADG_OK
     select case (rco)
     case (ADG_OK)
     case (ADG_E_MAX_OUTER)
@@ -237,11 +243,15 @@ contains
     call run_outer(po, ADG_FIXED_SOURCE, 'FIXED-SOURCE')
   end subroutine outer_fs
 
+  ! outer_th (src/mod_cmfd.f90:703-796): at most maxn outers of the forward
+  ! iteration between two T-H updates, printing nothing; without an %ITER
+  ! card the nodal update interval becomes nth / 2
   subroutine outer_th(maxn)
+    use sdata, only: nth, nupd
+    use io, only: biter
     integer, intent(in) :: maxn
-    if (maxn < 0) continue
-    write(*,*) '  T-H OUTER ITERATION IS NOT SERVED BY THE GPU BUILD'
-    stop
+    if (biter == 0) nupd = int(nth/2)
+    call run_outer(0, ADG_FORWARD, 'T-H', maxn)
   end subroutine outer_th
 
   subroutine outer_tr(ht, maxi)

@@ -177,6 +177,67 @@ def batch(seed=3, n=256, workers=7, sample=8, cache="/tmp/adg_batch_golden"):
     save(f"batch_s{seed}_solve", **arrs)
 
 
+SEARCH_LINE = r"^\s*(\d+)\s+(\d+\.\d\d)\s+(\d+\.\d{5})\s+(\S+E[+-]\d\d)\s+(\S+E[+-]\d\d)\s*$"
+
+
+def bcsearch():
+    """The reference program on its critical-boron search deck without T-H
+    (smpl/static/CBCsearch: cbsearch, src/mod_th.f90:752-837, calling outer(0)
+    after every cross-section update): the printed search table."""
+    import re
+    exe = os.path.join(ROOT, "oracle", "_ref", "adpres")
+    with tempfile.TemporaryDirectory() as d:
+        shutil.copy(os.path.join(REF, "CBCsearch"), os.path.join(d, "CBCsearch"))
+        r = subprocess.run(["bash", "-c", f"ulimit -s unlimited; exec {exe} CBCsearch"], cwd=d,
+                           capture_output=True, text=True)
+    if r.returncode != 0 or "ADPRES EXIT NORMALLY" not in r.stdout:
+        raise RuntimeError(r.stdout[-2000:])
+    rows = [m.groups() for m in map(re.compile(SEARCH_LINE).match, r.stdout.splitlines()) if m]
+    save("cbcsearch_search", n=np.array([int(q[0]) for q in rows], np.int32),
+         bcon=np.array([float(q[1]) for q in rows]), k=np.array([float(q[2]) for q in rows]),
+         ser=np.array([float(q[3]) for q in rows]), fer=np.array([float(q[4]) for q in rows]))
+
+
+NEACRP_DECKS = ("A1", "B1", "C1")
+NEACRP_FILES = ("neacrp_xsec", "neacrp_geom", "neacrp_cbcs", "neacrp_ftem", "neacrp_mtem", "neacrp_cden")
+SEARCH_TH_LINE = (r"^\s*(\d+)\s+(\d+\.\d\d)\s+(\d+\.\d{5})\s+(\S+E[+-]\d\d)\s+(\S+E[+-]\d\d)"
+                  r"\s+(\S+E[+-]\d\d)\s*$")
+
+
+def neacrp():
+    """The reference program on its NEACRP critical-boron searches with T-H
+    feedback (smpl/static/NEACRP, cases A1, B1, C1: cbsearcht,
+    src/mod_th.f90:840-959, whose th_iter calls outer_th between T-H
+    updates).  The decks' FILE cards name the author's machine; the copies
+    under decks/neacrp/ name the data files beside them.  Stores the printed
+    search tables."""
+    import re
+    exe = os.path.join(ROOT, "oracle", "_ref", "adpres")
+    out = os.path.join(HERE, "decks", "neacrp")
+    os.makedirs(out, exist_ok=True)
+    for fn in NEACRP_FILES:
+        shutil.copy(os.path.join(REF, "NEACRP", fn), os.path.join(out, fn))
+    arrs = {}
+    for deck in NEACRP_DECKS:
+        text = open(os.path.join(REF, "NEACRP", deck)).read().replace("/home/imronuke/ADPRES/smpl/static/NEACRP/",
+                                                                         "./")
+        with open(os.path.join(out, deck), "w") as fh:
+            fh.write(text)
+        with tempfile.TemporaryDirectory() as d:
+            for fn in NEACRP_FILES + (deck,):
+                shutil.copy(os.path.join(out, fn), os.path.join(d, fn))
+            r = subprocess.run(["bash", "-c", f"ulimit -s unlimited; exec {exe} {deck}"], cwd=d,
+                               capture_output=True, text=True)
+        if r.returncode != 0 or "ADPRES EXIT NORMALLY" not in r.stdout:
+            raise RuntimeError(r.stdout[-2000:])
+        rows = [m.groups() for m in map(re.compile(SEARCH_TH_LINE).match, r.stdout.splitlines()) if m]
+        arrs[deck + "_n"] = np.array([int(q[0]) for q in rows], np.int32)
+        arrs[deck + "_bcon"] = np.array([float(q[1]) for q in rows])
+        arrs[deck + "_k"] = np.array([float(q[2]) for q in rows])
+        arrs[deck + "_therr"] = np.array([float(q[5]) for q in rows])
+    save("neacrp_search", **arrs)
+
+
 def adjoint_fdm(seed=1):
     """The 34x34x26-node 2-group core with the finite-difference kernel,
     solved by the reference in %MODE FORWARD and %MODE ADJOINT (outer_ad)
@@ -210,6 +271,10 @@ def main(names):
         synthetic("core3d_2g", 1, full=False)
     if "adjoint_fdm" in names:
         adjoint_fdm()
+    if not names or "bcsearch" in names:
+        bcsearch()
+    if not names or "neacrp" in names:
+        neacrp()
     if "batch" in names:          # 256 reference solves on 7 host cores
         batch()
     if "core3d_8g" in names:      # about an hour of reference CPU time

@@ -87,3 +87,57 @@ def test_dropin_fixed_source(tmp_path):
     # the early outers print identical digits (same algorithm, same arithmetic order)
     assert np.allclose([q[1] for q in rows[:20]], d["trace_ser"][:20], rtol=1e-4)
     assert "MULTIPLICATION EFFECTIVE" not in r.stdout
+
+
+def test_dropin_critical_boron_search(tmp_path):
+    """%MODE BCSEARCH without T-H (the reference's CBCsearch deck): the host's
+    cbsearch (src/mod_th.f90:752-837) updates the cross sections and calls
+    outer(0) again, warm-started, until k = 1; every call re-uploads the cross
+    sections and continues from the engine's state.  The printed search table
+    (boron, k) follows the reference's."""
+    deck = "CBCsearch"
+    shutil.copy(os.path.join(GOLDEN, "decks", deck), tmp_path / deck)
+    r = subprocess.run(["bash", "-c", f"ulimit -s unlimited; exec {EXE} {deck}"], cwd=tmp_path,
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-3000:]
+    assert "ADPRES EXIT NORMALLY" in r.stdout
+    line = re.compile(r"^\s*(\d+)\s+(\d+\.\d\d)\s+(\d+\.\d{5})\s+(\S+E[+-]\d\d)\s+(\S+E[+-]\d\d)\s*$")
+    rows = [(int(m.group(1)), float(m.group(2)), float(m.group(3)), float(m.group(4)))
+            for m in map(line.match, r.stdout.splitlines()) if m]
+    d = golden("cbcsearch_search")
+    assert [q[0] for q in rows] == list(d["n"])
+    # ser and fer of each search step's last outer (the callers print sdata's)
+    assert all(0.0 < q[3] < 1e-5 for q in rows)
+    # printed to 0.01 ppm and 1e-5 in k: one unit of the last printed digit
+    assert np.max(np.abs(np.array([q[1] for q in rows]) - d["bcon"])) <= 0.0100001
+    assert np.max(np.abs(np.array([q[2] for q in rows]) - d["k"])) <= 1.00001e-5
+
+
+@pytest.mark.parametrize("deck", ["A1", "B1", "C1"])
+def test_dropin_neacrp_boron_search_with_th(deck, tmp_path):
+    """%MODE BCSEARCH with T-H feedback (NEACRP cases, zero power): the host's
+    cbsearcht / th_iter (src/mod_th.f90:840-959, 11-91) alternate T-H and
+    cross-section updates with outer_th calls of at most nth outers each,
+    served on the GPU.  The printed search table (boron, k, T-H error)
+    follows the reference's."""
+    src = os.path.join(GOLDEN, "decks", "neacrp")
+    for fn in os.listdir(src):
+        shutil.copy(os.path.join(src, fn), tmp_path / fn)
+    r = subprocess.run(["bash", "-c", f"ulimit -s unlimited; exec {EXE} {deck}"], cwd=tmp_path,
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-3000:]
+    assert "ADPRES EXIT NORMALLY" in r.stdout
+    line = re.compile(r"^\s*(\d+)\s+(\d+\.\d\d)\s+(\d+\.\d{5})\s+(\S+E[+-]\d\d)\s+(\S+E[+-]\d\d)"
+                      r"\s+(\S+E[+-]\d\d)\s*$")
+    rows = [(int(m.group(1)), float(m.group(2)), float(m.group(3)), float(m.group(6)))
+            for m in map(line.match, r.stdout.splitlines()) if m]
+    d = golden("neacrp_search")
+    assert [q[0] for q in rows] == list(d[deck + "_n"])
+    # k to one unit of its last printed digit.  Each outer_th call stops
+    # after nth outers, unconverged, and the secant step amplifies the k of
+    # those states: A1's third guess moves by (dbcon / dk^2) ~ 1.8e6 ppm per
+    # unit k, so boron is held to 0.05 ppm, below the search's own
+    # resolution (|k - 1| < 1e-5 is ~0.1 ppm at these boron worths)
+    assert np.max(np.abs(np.array([q[2] for q in rows]) - d[deck + "_k"])) <= 1.00001e-5
+    assert np.max(np.abs(np.array([q[1] for q in rows]) - d[deck + "_bcon"])) <= 0.05
+    assert np.allclose([q[3] for q in rows], d[deck + "_therr"], rtol=1e-2, atol=1e-7)
