@@ -15,6 +15,29 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP engine through the C-ABI)")
 
 
+def output_tokens(text):
+    """The numbers (with a decimal point) of the result blocks of an ADPRES
+    .out file: from the radial power distribution (AsmPow, src/mod_io.f90)
+    to the CPU-time breakdown, i.e. the power, axial, flux and T-H summary
+    tables.  Integers (row, plane and position labels) are left out: the
+    position of a maximum flips between mirror nodes on a printed tie."""
+    import re
+    lines = text.splitlines()
+    try:
+        i0 = next(i for i, ln in enumerate(lines) if "Radial Power Distribution" in ln)
+    except StopIteration:
+        return []
+    i1 = next((i for i, ln in enumerate(lines) if "CPU time breakdown" in ln and i > i0), len(lines))
+    num = re.compile(r"[-+]?\d+\.\d+(?:E[-+]\d+)?")
+    return [t for ln in lines[i0:i1] for t in num.findall(ln)]
+
+
+def printed_unit(tok):
+    """One unit of the last printed digit of a number token."""
+    mant, _, exp = tok.partition("E")
+    return 10.0 ** ((int(exp) if exp else 0) - len(mant.split(".")[1]))
+
+
 def golden(name):
     """Load tests/golden/<name>.npz (written by tests/golden/make_golden.py)."""
     return np.load(os.path.join(GOLDEN, name + ".npz"), allow_pickle=False)

@@ -238,6 +238,37 @@ def neacrp():
     save("neacrp_search", **arrs)
 
 
+SWEEP_DECKS = ("IAEA3Ds", "IAEA2D", "PNM", "BIBLIS", "KOEBERG", "DVP", "FDM", "FDM-1D", "adjoint", "fixed_source",
+               "CBCsearch", "MOX/part1_aro_serpent", "MOX/part1_ari_serpent", "MOX/part1_aro_helios",
+               "MOX/part1_ari_helios")
+
+
+def outputs():
+    """The result blocks (power, axial, flux, T-H tables) of the reference
+    program's .out file for every self-contained static sample deck and the
+    NEACRP searches: the regression sweep of the drop-in's outputs."""
+    sys.path.insert(0, os.path.dirname(HERE))
+    from conftest import output_tokens
+    exe = os.path.join(ROOT, "oracle", "_ref", "adpres")
+    toks = {}
+    jobs = [(os.path.basename(d), [os.path.join(REF, d)]) for d in SWEEP_DECKS]
+    nea = os.path.join(HERE, "decks", "neacrp")
+    jobs += [(d, [os.path.join(nea, f) for f in os.listdir(nea)]) for d in NEACRP_DECKS]
+    for deck, files in jobs:
+        if deck not in NEACRP_DECKS and not os.path.exists(os.path.join(HERE, "decks", deck)):
+            shutil.copy(files[0], os.path.join(HERE, "decks", deck))
+        with tempfile.TemporaryDirectory() as d:
+            for f in files:
+                shutil.copy(f, os.path.join(d, os.path.basename(f)))
+            r = subprocess.run(["bash", "-c", f"ulimit -s unlimited; exec {exe} {deck}"], cwd=d,
+                               capture_output=True, text=True)
+            if r.returncode != 0 or "ADPRES EXIT NORMALLY" not in r.stdout:
+                raise RuntimeError(deck + r.stdout[-2000:])
+            toks[deck] = np.array(output_tokens(open(os.path.join(d, deck + ".out")).read()))
+        print(deck, len(toks[deck]))
+    save("outputs_sweep", **{k.replace("-", "_"): v for k, v in toks.items()})
+
+
 def adjoint_fdm(seed=1):
     """The 34x34x26-node 2-group core with the finite-difference kernel,
     solved by the reference in %MODE FORWARD and %MODE ADJOINT (outer_ad)
@@ -275,6 +306,8 @@ def main(names):
         bcsearch()
     if not names or "neacrp" in names:
         neacrp()
+    if not names or "outputs" in names:
+        outputs()
     if "batch" in names:          # 256 reference solves on 7 host cores
         batch()
     if "core3d_8g" in names:      # about an hour of reference CPU time

@@ -141,3 +141,32 @@ def test_dropin_neacrp_boron_search_with_th(deck, tmp_path):
     assert np.max(np.abs(np.array([q[2] for q in rows]) - d[deck + "_k"])) <= 1.00001e-5
     assert np.max(np.abs(np.array([q[1] for q in rows]) - d[deck + "_bcon"])) <= 0.05
     assert np.allclose([q[3] for q in rows], d[deck + "_therr"], rtol=1e-2, atol=1e-7)
+
+
+SWEEP = ["IAEA3Ds", "IAEA2D", "PNM", "BIBLIS", "KOEBERG", "DVP", "FDM", "FDM-1D", "adjoint", "fixed_source",
+         "CBCsearch", "part1_aro_serpent", "part1_ari_serpent", "part1_aro_helios", "part1_ari_helios",
+         "A1", "B1", "C1"]
+
+
+@pytest.mark.parametrize("deck", SWEEP)
+def test_dropin_output_files_match_reference(deck, tmp_path):
+    """Regression sweep over the reference's self-contained static sample
+    decks (and the NEACRP searches): the result blocks that ADPRES's own
+    output routines write into <deck>.out from the GPU solution (radial and
+    axial power, radial flux, T-H summary) equal the reference's to one unit
+    of the last printed digit."""
+    from conftest import output_tokens, printed_unit
+    src = os.path.join(GOLDEN, "decks", "neacrp") if deck in ("A1", "B1", "C1") else None
+    files = [os.path.join(src, f) for f in os.listdir(src)] if src else [os.path.join(GOLDEN, "decks", deck)]
+    for f in files:
+        shutil.copy(f, tmp_path / os.path.basename(f))
+    r = subprocess.run(["bash", "-c", f"ulimit -s unlimited; exec {EXE} {deck}"], cwd=tmp_path,
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-3000:]
+    assert "ADPRES EXIT NORMALLY" in r.stdout
+    got = output_tokens(open(tmp_path / (deck + ".out")).read())
+    ref = [str(t) for t in golden("outputs_sweep")[deck.replace("-", "_")]]
+    assert len(got) == len(ref) > 0
+    bad = [(i, a, b) for i, (a, b) in enumerate(zip(got, ref))
+           if abs(float(a) - float(b)) > 1.0001 * printed_unit(b)]
+    assert not bad, bad[:10]
