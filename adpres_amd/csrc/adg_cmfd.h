@@ -166,8 +166,10 @@ __global__ void __launch_bounds__(NTF) k_matvec(LV lv, const double *A64, int g,
 // sources: TSrc / TSrcAd (src/mod_cmfd.f90:1006-1064)
 // ---------------------------------------------------------------------------
 
+// smode 0: TSrc (forward, :1006-1033), 1: TSrcAd (:1037-1064), 2: TSrcTr
+// (transient, :1068-1096: fission source weighted by 1 - beta + dfis, no 1/k)
 template <int TAB>
-__global__ void __launch_bounds__(NTF) k_src(LV lv, XS xs, int g, int adjoint, int first_group, const double *f0,
+__global__ void __launch_bounds__(NTF) k_src(LV lv, XS xs, int g, int smode, int first_group, const double *f0,
                                             double *f0c, const double *fs0, double *fs0c, double *b) {
 #pragma clang fp contract(off)
   const BlkView v = blk_view_t<TAB>(lv);
@@ -175,6 +177,7 @@ __global__ void __launch_bounds__(NTF) k_src(LV lv, XS xs, int g, int adjoint, i
   const double Ke = v.sc->Ke;
   const long NC = lv.NC;
   const int G = lv.G;
+  const int adjoint = smode == 1;
   for (long c = v.b.lo + threadIdx.x; c < v.b.hi; c += NTF) {
     if (first_group) fs0c[c] = fs0[c];
     f0c[(size_t)g * NC + c] = f0[(size_t)g * NC + c];
@@ -183,6 +186,10 @@ __global__ void __launch_bounds__(NTF) k_src(LV lv, XS xs, int g, int adjoint, i
       if (h == g) continue;
       const double sg = adjoint ? xs.sigs[((size_t)h * G + g) * NC + c] : xs.sigs[((size_t)g * G + h) * NC + c];
       s0 = s0 + sg * f0[(size_t)h * NC + c];
+    }
+    if (smode == 2) {
+      b[c] = xs.wtr[c] * xs.chi[(size_t)g * NC + c] * fs0[c] + s0 + xs.exsrc[(size_t)g * NC + c];
+      continue;
     }
     const double w = adjoint ? xs.nuf[(size_t)g * NC + c] : xs.chi[(size_t)g * NC + c];
     b[c] = w * fs0[c] / Ke + s0 + xs.exsrc[(size_t)g * NC + c];

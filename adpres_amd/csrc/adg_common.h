@@ -104,6 +104,7 @@ __device__ __forceinline__ bool nodal_outer(const Scal *sc) {
 
 struct XS {
   const double *D, *sigr, *nuf, *sigf, *chi, *sigs, *dc, *exsrc, *vdel;
+  const double *wtr;   // transient fission weight 1 - beta + dfis per cell (outer_tr)
 };
 
 struct Range {

@@ -97,6 +97,7 @@ struct adg_ctx {
   unsigned char *flag = nullptr;
   double *D = nullptr, *sigr = nullptr, *nuf = nullptr, *sigf = nullptr, *chi = nullptr, *sigs = nullptr;
   double *dc = nullptr, *exsrc = nullptr, *vdel = nullptr;
+  double *wtr = nullptr;   // transient fission weight per cell (adg_set_transient)
   double *f0 = nullptr, *f0c = nullptr, *fs0 = nullptr, *fs0c = nullptr, *errn = nullptr;
   double *dn = nullptr, *df = nullptr, *A64 = nullptr, *b = nullptr, *pw = nullptr, *tmp = nullptr;
   // BiCGSTAB vectors: FP64 form
@@ -129,7 +130,7 @@ struct adg_ctx {
   int *tflags = nullptr, *tloc = nullptr;
   int tcap = 0;
   hipGraphExec_t gexec = nullptr;
-  int gkey[6] = {-1, -1, -1, -1, -1, -1};   // adjoint, nin, kern, nupd, variant, transpose of the captured outer
+  int gkey[6] = {-1, -1, -1, -1, -1, -1};   // source mode, nin, kern, nupd, variant, transpose of the captured outer
   // differentiation (single core): saved forward state, adjoint, generalized
   // adjoint and gradients
   double *psi = nullptr, *phis = nullptr, *fs0s = nullptr, *exs = nullptr, *gam = nullptr, *dJdP = nullptr;
@@ -153,7 +154,7 @@ struct adg_ctx {
   XS xs() const {
     XS x;
     x.D = D; x.sigr = sigr; x.nuf = nuf; x.sigf = sigf; x.chi = chi; x.sigs = sigs;
-    x.dc = dc; x.exsrc = exsrc; x.vdel = vdel;
+    x.dc = dc; x.exsrc = exsrc; x.vdel = vdel; x.wtr = wtr;
     return x;
   }
   double *P(int k) const { return part + (size_t)4 * nflat * k; }
@@ -452,13 +453,13 @@ void launch_bicg(adg_ctx *c, int g, int nin) {
 }
 
 // one group solve: TSrc + bicg (src/mod_cmfd.f90:472-478)
-void group_solve(adg_ctx *c, int g, int adjoint, int first_group, int nin) {
+void group_solve(adg_ctx *c, int g, int smode, int first_group, int nin) {
   const int eg0 = prof_event(c);
   if (c->B > 1)
-    hipLaunchKernelGGL(k_src<1>, dim3(c->nflat), dim3(NTF), 0, c->s, c->lv(c->tflat), c->xs(), g, adjoint,
+    hipLaunchKernelGGL(k_src<1>, dim3(c->nflat), dim3(NTF), 0, c->s, c->lv(c->tflat), c->xs(), g, smode,
                        first_group, c->f0, c->f0c, c->fs0, c->fs0c, c->b);
   else
-    hipLaunchKernelGGL(k_src<0>, dim3(c->nflat), dim3(NTF), 0, c->s, c->lv(c->tflat), c->xs(), g, adjoint,
+    hipLaunchKernelGGL(k_src<0>, dim3(c->nflat), dim3(NTF), 0, c->s, c->lv(c->tflat), c->xs(), g, smode,
                        first_group, c->f0, c->f0c, c->fs0, c->fs0c, c->b);
   launch_bicg(c, g, nin);
   prof_pair(c, 1, eg0, prof_event(c));
@@ -493,7 +494,7 @@ void nodal_update(adg_ctx *c, int kern, int cmode, int sched) {
 
 // host-side initialisation of the device scalars of every core for an outer
 // call; nit = 1: one %ITER for all cores, nit = B: one per core
-void set_scal_start(adg_ctx *c, const adg_iter *its, int nit = 1, int fixed_k = 0) {
+void set_scal_start(adg_ctx *c, const adg_iter *its, int nit = 1, int fixed_k = 0, int frozen = 0) {
   std::vector<Scal> h(c->B);
   scopy(c->s, h.data(), c->sc, c->B * sizeof(Scal), hipMemcpyDeviceToHost);
   for (int b = 0; b < c->B; ++b) {
@@ -508,8 +509,10 @@ void set_scal_start(adg_ctx *c, const adg_iter *its, int nit = 1, int fixed_k = 
     s.init = c->core[b].first ? 1 : 0;
     s.nac = it ? it->nac : 1;
     s.nupd = it ? it->nupd : 1;
-    s.nodal_on = (it && it->kern != ADG_KERN_FDM) ? 1 : 0;
-    if (!s.nodal_on) s.ndmax = 0.0;
+    // frozen: the D-hats stay as they are (no nodal update) and ndmax keeps
+    // its last value, as in outer_ad called with popt = 0
+    s.nodal_on = (it && it->kern != ADG_KERN_FDM && !frozen) ? 1 : 0;
+    if (!(it && it->kern != ADG_KERN_FDM)) s.ndmax = 0.0;
     s.serc = it ? it->serc : 0.0;
     s.ferc = it ? it->ferc : 0.0;
     s.aux = c->psifphi;
@@ -539,13 +542,15 @@ void ensure_trace(adg_ctx *c, int nout) {
 // update (its kernels return at once unless this outer is a nodal-update
 // outer of the core -- the schedule lives on the device), and the
 // end-of-outer kernel.  p_host is the outer index seen by the host
-// (profiling only).
-void enqueue_iteration(adg_ctx *c, int adjoint, int nin, int kern, int p_host, int nupd) {
+// (profiling only).  smode: 0 forward (outer, outer_fs, outer_th), 1
+// adjoint (outer_ad), 2 transient (outer_tr).
+void enqueue_iteration(adg_ctx *c, int smode, int nin, int kern, int p_host, int nupd) {
   const LV lv = c->lv(c->tflat);
   const XS x = c->xs();
+  const int adjoint = smode == 1;
   for (int gg = 0; gg < c->G; ++gg) {
     const int g = adjoint ? c->G - 1 - gg : gg;
-    group_solve(c, g, adjoint, gg == 0 ? 1 : 0, nin);
+    group_solve(c, g, smode, gg == 0 ? 1 : 0, nin);
   }
   double *P1 = c->P(1), *P2 = c->P(2);
   if (c->variant == 1) {   // generalized adjoint: change norm, projection of psi, adjoint fission source
@@ -564,7 +569,7 @@ void enqueue_iteration(adg_ctx *c, int adjoint, int nin, int kern, int p_host, i
   if (kern != ADG_KERN_FDM) {
     const bool timed = p_host > 0 && (p_host % nupd) == 0;
     const int en0 = timed ? prof_event(c) : -1;
-    nodal_update(c, kern, adjoint ? 0 : 1, 1);
+    nodal_update(c, kern, adjoint ? 0 : (smode == 2 ? 2 : 1), 1);
     if (timed) prof_pair(c, 2, en0, prof_event(c));
   }
   hipLaunchKernelGGL(k_end, dim3((c->B + 63) / 64), dim3(64), 0, c->s, c->sc, c->B, c->tcap, c->tk, c->tser, c->tfer,
@@ -611,8 +616,11 @@ int run_outer(adg_ctx *c, int mode, const adg_iter *it, int nit = 1) {
     if (q.nout != it->nout || q.nin != it->nin || q.kern != it->kern)
       return fail(c, ADG_E_ARG, "nout, nin and the nodal kernel must be the same for every core");
   }
-  if (mode != ADG_FORWARD && mode != ADG_ADJOINT && mode != ADG_FIXED_SOURCE) return fail(c, ADG_E_ARG, "bad mode");
-  const int adjoint = mode == ADG_ADJOINT;
+  if (mode != ADG_FORWARD && mode != ADG_ADJOINT && mode != ADG_FIXED_SOURCE && mode != ADG_TRANSIENT &&
+      mode != ADG_ADJOINT_FROZEN)
+    return fail(c, ADG_E_ARG, "bad mode");
+  const int adjoint = mode == ADG_ADJOINT || mode == ADG_ADJOINT_FROZEN;
+  const int smode = adjoint ? 1 : (mode == ADG_TRANSIENT ? 2 : 0);
   const int nout = it->nout;
   const LV lv = c->lv(c->tflat);
   const XS x = c->xs();
@@ -631,7 +639,8 @@ int run_outer(adg_ctx *c, int mode, const adg_iter *it, int nit = 1) {
       any_first = true;
     }
   }
-  set_scal_start(c, it, nit, mode == ADG_FIXED_SOURCE ? 1 : 0);
+  set_scal_start(c, it, nit, (mode == ADG_FIXED_SOURCE || mode == ADG_TRANSIENT) ? 1 : 0,
+                 mode == ADG_ADJOINT_FROZEN ? 1 : 0);
   c->ev_next = 0;
   c->recs.clear();
   HIPCHK(hipEventRecord(e0, c->s));
@@ -651,7 +660,7 @@ int run_outer(adg_ctx *c, int mode, const adg_iter *it, int nit = 1) {
   // polls the flags every `check` outers
   const bool use_graph = !c->prof && c->graphs;
   if (use_graph) {
-    const int key[6] = {adjoint, it->nin, it->kern, it->nupd, c->variant, c->transpose ? 1 : 0};
+    const int key[6] = {smode, it->nin, it->kern, it->nupd, c->variant, c->transpose ? 1 : 0};
     if (!c->gexec || memcmp(key, c->gkey, sizeof key) != 0) {
       if (c->gexec) {
         (void)hipGraphExecDestroy(c->gexec);
@@ -662,7 +671,7 @@ int run_outer(adg_ctx *c, int mode, const adg_iter *it, int nit = 1) {
       std::lock_guard<std::mutex> lk(capture_lock);
       hipGraph_t graph;
       HIPCHK(hipStreamBeginCapture(c->s, hipStreamCaptureModeThreadLocal));
-      enqueue_iteration(c, adjoint, it->nin, it->kern, 0, it->nupd);
+      enqueue_iteration(c, smode, it->nin, it->kern, 0, it->nupd);
       HIPCHK(hipStreamEndCapture(c->s, &graph));
       HIPCHK(hipGraphInstantiate(&c->gexec, graph, nullptr, nullptr, 0));
       (void)hipGraphDestroy(graph);
@@ -676,7 +685,7 @@ int run_outer(adg_ctx *c, int mode, const adg_iter *it, int nit = 1) {
     for (int k = 0; k < n; ++k) {
       ++launched;
       if (use_graph) HIPCHK(hipGraphLaunch(c->gexec, c->s));
-      else enqueue_iteration(c, adjoint, it->nin, it->kern, launched, it->nupd);
+      else enqueue_iteration(c, smode, it->nin, it->kern, launched, it->nupd);
     }
     HIPCHK(hipMemcpyAsync(c->hsc, c->sc, c->B * sizeof(Scal), hipMemcpyDeviceToHost, c->s));
     HIPCHK(hipStreamSynchronize(c->s));
@@ -894,6 +903,7 @@ int create_batch(const adg_geom *geoms, int ncore, adg_ctx **out) {
     c->dn = dalloc<double>(c->s, 6 * G * NC); c->df = dalloc<double>(c->s, 6 * G * NC);
     c->A64 = dalloc<double>(c->s, 7 * G * NC); c->b = dalloc<double>(c->s, NC); c->pw = dalloc<double>(c->s, NC);
     c->tmp = dalloc<double>(c->s, NC);
+    c->wtr = dalloc<double>(c->s, NC);
     if (c->precision == ADG_FP32_REFINE) {
       c->A32 = dalloc<float>(c->s, 7 * G * NC);
       c->rs32 = dalloc<float>(c->s, NC); c->t32 = dalloc<float>(c->s, NC); c->d32 = dalloc<float>(c->s, NC);
@@ -1136,6 +1146,39 @@ int adg_powdis_core(adg_ctx *c, int core, double *p) {
 }
 
 int adg_powdis(adg_ctx *c, double *p) { return adg_powdis_core(c, 0, p); }
+
+int adg_set_transient_core(adg_ctx *c, int core, const double *wtr) {
+  if (!core_ok(c, core) || !wtr) return ADG_E_ARG;
+  try {
+    std::vector<double> st;
+    scatter(c, core, wtr, 1, c->wtr, st);
+  } catch (const HipError &e) {
+    return hip_fail(c, e);
+  }
+  return ADG_OK;
+}
+
+int adg_set_transient(adg_ctx *c, const double *wtr) { return adg_set_transient_core(c, 0, wtr); }
+
+int adg_leakage(adg_ctx *c, double *L) {
+  if (!c || !L) return ADG_E_ARG;
+  if (c->B != 1) return fail(c, ADG_E_ARG, "kernel-level entry points take a single-core context");
+  try {
+    NodArgs a;
+    memset(&a, 0, sizeof a);
+    a.lv = c->lv(c->tcell);
+    a.xs = c->xs();
+    a.f0 = c->f0; a.df = c->df; a.dn = c->dn;
+    hipLaunchKernelGGL(k_leak, dim3(c->ncell, c->G), dim3(NT), 0, c->s, a, c->S);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->s));
+    std::vector<double> st;
+    gather(c, 0, c->S, 3 * c->G, L, st);
+  } catch (const HipError &e) {
+    return hip_fail(c, e);
+  }
+  return ADG_OK;
+}
 
 int adg_nodal_update(adg_ctx *c, int kern, int cmode, double *ndmax, int *loc3) {
   if (!c) return ADG_E_ARG;

@@ -28,7 +28,7 @@ struct NodArgs {
   double *S;            // [3][G][NC]
   double *a2, *a4, *L1; // [3][G][NC]
   int kern;             // 1 PNM, 2 SANM
-  int cmode;            // 1 forward, 0 adjoint
+  int cmode;            // 1 forward, 0 adjoint, 2 transient
   int sched;            // 1: run only in a nodal-update outer (device-side schedule)
   double *part;
 };
@@ -42,8 +42,42 @@ __device__ __forceinline__ double hdir(const Geo &q, int u, int i, int j, int k)
 }
 
 // ---------------------------------------------------------------------------
-// pass 0: Lxyz + get_source (src/mod_nodal.f90:901-1043), cmode /= 2
+// pass 0: Lxyz + get_source (src/mod_nodal.f90:901-1043)
 // ---------------------------------------------------------------------------
+
+// Lxyz (src/mod_nodal.f90:901-1005): zeroth transverse leakages of cell c,
+// group g, from the CMFD currents of its six faces
+__device__ __forceinline__ void nod_lxyz(const Geo &q, const double *f, const double *df, const double *dn,
+                                         unsigned char fl, long c, int i, int j, int k, int g, double (&L)[3]) {
+#pragma clang fp contract(off)
+  const long NC = q.NC;
+  const int G = q.G;
+  const double fn = f[c];
+  const long st[3] = {1, q.sy, q.sz};
+  for (int u = 0; u < 3; ++u) {
+    const int sp = 2 * u, sm = 2 * u + 1;
+    const double dfp = df[((size_t)sp * G + g) * NC + c], dnp = dn[((size_t)sp * G + g) * NC + c];
+    const double dfm = df[((size_t)sm * G + g) * NC + c], dnm = dn[((size_t)sm * G + g) * NC + c];
+    double jp, jm;
+    if (!(fl & f_nb(sp))) {
+      jp = (q.bc[sp] == 2) ? 0.0 : dfp * fn - dnp * fn;
+    } else {
+      const double fp = f[c + st[u]];
+      jp = -dfp * (fp - fn) - dnp * (fp + fn);
+    }
+    if (!(fl & f_nb(sm))) {
+      jm = (q.bc[sm] == 2) ? 0.0 : -dfm * fn - dnm * fn;
+    } else {
+      const double fm = f[c - st[u]];
+      jm = -dfm * (fn - fm) - dnm * (fn + fm);
+    }
+    L[u] = (jp - jm) / hdir(q, u, i, j, k);
+  }
+}
+
+// get_source: S_u = sum of the other two leakages; the transient form
+// (cmode 2, :1031-1035) subtracts the external source here instead of in
+// get_a2matvec
 __global__ void __launch_bounds__(NT) k_nod_src(NodArgs a) {
 #pragma clang fp contract(off)
   const BlkView v = blk_view(a.lv);
@@ -62,32 +96,42 @@ __global__ void __launch_bounds__(NT) k_nod_src(NodArgs a) {
   const int g = blockIdx.y;
   const long NC = q.NC;
   const int G = q.G;
-  const double *f = a.f0 + (size_t)g * NC;
-  const double fn = f[c];
-  const long st[3] = {1, q.sy, q.sz};
   double L[3];
-  for (int u = 0; u < 3; ++u) {
-    const int sp = 2 * u, sm = 2 * u + 1;
-    const double dfp = a.df[((size_t)sp * G + g) * NC + c], dnp = a.dn[((size_t)sp * G + g) * NC + c];
-    const double dfm = a.df[((size_t)sm * G + g) * NC + c], dnm = a.dn[((size_t)sm * G + g) * NC + c];
-    double jp, jm;
-    if (!(fl & f_nb(sp))) {
-      jp = (q.bc[sp] == 2) ? 0.0 : dfp * fn - dnp * fn;
-    } else {
-      const double fp = f[c + st[u]];
-      jp = -dfp * (fp - fn) - dnp * (fp + fn);
-    }
-    if (!(fl & f_nb(sm))) {
-      jm = (q.bc[sm] == 2) ? 0.0 : -dfm * fn - dnm * fn;
-    } else {
-      const double fm = f[c - st[u]];
-      jm = -dfm * (fn - fm) - dnm * (fn + fm);
-    }
-    L[u] = (jp - jm) / hdir(q, u, i, j, k);
+  nod_lxyz(q, a.f0 + (size_t)g * NC, a.df, a.dn, fl, c, i, j, k, g, L);
+  if (a.cmode == 2) {
+    const double ex = a.xs.exsrc[(size_t)g * NC + c];
+    a.S[((size_t)0 * G + g) * NC + c] = L[1] + L[2] - ex;
+    a.S[((size_t)1 * G + g) * NC + c] = L[0] + L[2] - ex;
+    a.S[((size_t)2 * G + g) * NC + c] = L[0] + L[1] - ex;
+  } else {
+    a.S[((size_t)0 * G + g) * NC + c] = L[1] + L[2];
+    a.S[((size_t)1 * G + g) * NC + c] = L[0] + L[2];
+    a.S[((size_t)2 * G + g) * NC + c] = L[0] + L[1];
   }
-  a.S[((size_t)0 * G + g) * NC + c] = L[1] + L[2];
-  a.S[((size_t)1 * G + g) * NC + c] = L[0] + L[2];
-  a.S[((size_t)2 * G + g) * NC + c] = L[0] + L[1];
+}
+
+// Lxyz of every node and group into out[u][g][cell] (the leakages the
+// host's transient reactivity reads, src/mod_trans.f90:675-677)
+__global__ void __launch_bounds__(NT) k_leak(NodArgs a, double *out) {
+#pragma clang fp contract(off)
+  const BlkView v = blk_view(a.lv);
+  const Geo &q = *v.q;
+  int i, j, k;
+  long c;
+  const long nxy = (long)q.nxx * q.nyy;
+  const long t = v.b.lo + threadIdx.x;
+  if (t >= v.b.hi || t >= nxy * q.nzz) return;
+  k = (int)(t / nxy);
+  { long r = t - (long)k * nxy; j = (int)(r / q.nxx); i = (int)(r - (long)j * q.nxx); }
+  c = cell_of(q, i, j, k);
+  const unsigned char fl = q.flag[c];
+  if (!(fl & F_ACTIVE)) return;
+  const int g = blockIdx.y;
+  const long NC = q.NC;
+  const int G = q.G;
+  double L[3];
+  nod_lxyz(q, a.f0 + (size_t)g * NC, a.df, a.dn, fl, c, i, j, k, g, L);
+  for (int u = 0; u < 3; ++u) out[((size_t)u * G + g) * NC + c] = L[u];
 }
 
 // ---------------------------------------------------------------------------
@@ -106,6 +150,12 @@ __device__ __forceinline__ void nod_B(const NodArgs &a, long NC, long c, double 
       if (a.cmode == 1) {
         if (g == hh) dum = x.sigr[(size_t)g * NC + c] - x.chi[(size_t)g * NC + c] * x.nuf[(size_t)hh * NC + c] / Ke;
         else dum = -x.sigs[((size_t)g * G + hh) * NC + c] - x.chi[(size_t)g * NC + c] * x.nuf[(size_t)hh * NC + c] / Ke;
+      } else if (a.cmode == 2) {
+        // transient (:1379-1391): fission weighted by 1 - beta + dfis, no 1/k;
+        // the diagonal term carries chi twice, as the reference has it
+        const double wc = x.wtr[c] * x.chi[(size_t)g * NC + c];
+        if (g == hh) dum = x.sigr[(size_t)g * NC + c] - wc * x.chi[(size_t)g * NC + c] * x.nuf[(size_t)hh * NC + c];
+        else dum = -x.sigs[((size_t)g * G + hh) * NC + c] - wc * x.nuf[(size_t)hh * NC + c];
       } else {
         if (g == hh) dum = x.sigr[(size_t)g * NC + c] - x.chi[(size_t)g * NC + c] * x.nuf[(size_t)hh * NC + c] / Ke;
         else dum = -x.sigs[((size_t)hh * G + g) * NC + c] - x.chi[(size_t)hh * NC + c] * x.nuf[(size_t)g * NC + c] / Ke;
@@ -307,7 +357,10 @@ __global__ void __launch_bounds__(NT) k_nod_pass1(NodArgs a) {
   const double *Su = a.S + (size_t)u * G * NC;
   for (int g = 0; g < G; ++g) {
     const double D = a.xs.D[(size_t)g * NC + c];
-    const double S = 0.25 * (h * h) / D * (Su[(size_t)g * NC + c] - a.xs.exsrc[(size_t)g * NC + c]);
+    // get_a2matvec (:788-805): the external source enters here except in the
+    // transient form, where get_source has taken it out of S already
+    const double S = a.cmode == 2 ? 0.25 * (h * h) / D * Su[(size_t)g * NC + c]
+                                  : 0.25 * (h * h) / D * (Su[(size_t)g * NC + c] - a.xs.exsrc[(size_t)g * NC + c]);
     double Bf = 0.0;
     for (int hh = 0; hh < G; ++hh) {
       M[g * G + hh] = (hh == g) ? B[g * G + hh] * cf[g].E + 3.0 : B[g * G + hh] * cf[g].E;

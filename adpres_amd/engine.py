@@ -28,7 +28,7 @@ LIB_PATH = os.environ.get("ADG_LIBRARY") or os.path.join(HERE, "libadpres_gpu.so
 
 ADG_OK, ADG_E_MAX_OUTER, ADG_E_NODAL_UNSTABLE, ADG_E_LU_PIVOT, ADG_E_POWER_NONPOS = 0, 1, 2, 3, 4
 ADG_E_ARG, ADG_E_HIP, ADG_E_NOT_SET = 5, 6, 7
-ADG_FORWARD, ADG_ADJOINT, ADG_FIXED_SOURCE = 0, 1, 2       # enum adg_mode
+ADG_FORWARD, ADG_ADJOINT, ADG_FIXED_SOURCE, ADG_TRANSIENT, ADG_ADJOINT_FROZEN = 0, 1, 2, 3, 4   # enum adg_mode
 FP64, FP32_REFINE = 0, 1
 PRECISIONS = {"fp64": FP64, "fp32_refine": FP32_REFINE}
 KERNS = {"FDM": 0, "PNM": 1, "SANM": 2}
@@ -39,6 +39,7 @@ EXPORTS = ("adg_create", "adg_destroy", "adg_set_xs", "adg_set_state", "adg_get_
            "adg_create_batch", "adg_core_count", "adg_set_xs_core", "adg_set_state_core", "adg_get_state_core",
            "adg_outer_batch", "adg_trace_core", "adg_powdis_core",
            "adg_adjoint", "adg_get_adjoint", "adg_set_adjoint", "adg_backward",
+           "adg_set_transient", "adg_set_transient_core", "adg_leakage",
            "adg_status_string", "adg_last_error")
 
 _dp = C.POINTER(C.c_double)
@@ -104,6 +105,9 @@ def load_library(path: str = LIB_PATH):
     L.adg_outer_batch.argtypes = [C.c_void_p, C.c_int, C.POINTER(AdgIter), C.c_int, _ip, _ip, _dp]
     L.adg_trace_core.argtypes = [C.c_void_p, C.c_int, C.POINTER(AdgTrace)]
     L.adg_powdis_core.argtypes = [C.c_void_p, C.c_int, _dp]
+    L.adg_set_transient.argtypes = [C.c_void_p, _dp]
+    L.adg_set_transient_core.argtypes = [C.c_void_p, C.c_int, _dp]
+    L.adg_leakage.argtypes = [C.c_void_p, _dp]
     L.adg_adjoint.argtypes = [C.c_void_p, C.POINTER(AdgIter), _dp]
     L.adg_get_adjoint.argtypes = [C.c_void_p, _dp, _dp]
     L.adg_set_adjoint.argtypes = [C.c_void_p, _dp]
@@ -211,17 +215,22 @@ class Engine:
 
     # -- hot path --------------------------------------------------------
     def outer(self, adjoint: Optional[bool] = None, raise_on_error: bool = True,
-              fixed_source: Optional[bool] = None, **ov):
-        """cmfd::outer, outer_ad (adjoint) or outer_fs (fixed_source: the
-        forward outer with the external source and k held at its value); the
-        default follows the problem's %MODE."""
+              fixed_source: Optional[bool] = None, transient: bool = False, nodal: bool = True, **ov):
+        """cmfd::outer, outer_ad (adjoint), outer_fs (fixed_source: the
+        forward outer with the external source and k held at its value) or
+        outer_tr (transient: the external source and the fission weight of
+        set_transient, no k); the default follows the problem's %MODE.
+        nodal=False with adjoint: outer_ad called with popt = 0, the D-hats of
+        the forward state kept (no nodal update)."""
         p = self.p
         it = _iter(p, ov)
         if adjoint is None:
             adjoint = p.mode == "ADJOINT"
         if fixed_source is None:
             fixed_source = p.mode == "FIXEDSRC"
-        mode = ADG_FIXED_SOURCE if fixed_source else (ADG_ADJOINT if adjoint else ADG_FORWARD)
+        mode = ADG_TRANSIENT if transient else (
+            ADG_FIXED_SOURCE if fixed_source else (
+                (ADG_ADJOINT if nodal else ADG_ADJOINT_FROZEN) if adjoint else ADG_FORWARD))
         cap = it.nout
         t = {k: np.zeros(cap) for k in ("k", "ser", "fer", "ndmax")}
         t["flags"] = np.zeros(cap, np.int32)
@@ -234,6 +243,17 @@ class Engine:
         if raise_on_error:
             self._check(rc, "adg_outer")
         return rc
+
+    def set_transient(self, wtr) -> None:
+        """Per-node fission weight 1 - beta + dfis of the transient outer."""
+        w = np.ascontiguousarray(wtr, np.float64)
+        self._check(self.lib.adg_set_transient(self.h, _d(w)), "adg_set_transient")
+
+    def leakage(self) -> np.ndarray:
+        """nodal::Lxyz of every node and group: (3, G, N)."""
+        out = np.zeros((3, self.p.ng, self.p.nnod))
+        self._check(self.lib.adg_leakage(self.h, _d(out)), "adg_leakage")
+        return out
 
     def powdis(self) -> np.ndarray:
         out = np.zeros(self.p.nnod)

@@ -12,7 +12,8 @@ module adpres_gpu
   integer(c_int), parameter :: ADG_OK = 0, ADG_E_MAX_OUTER = 1, ADG_E_NODAL_UNSTABLE = 2, &
                                ADG_E_LU_PIVOT = 3, ADG_E_POWER_NONPOS = 4
   integer(c_int), parameter :: ADG_KERN_FDM = 0, ADG_KERN_PNM = 1, ADG_KERN_SANM = 2
-  integer(c_int), parameter :: ADG_FORWARD = 0, ADG_ADJOINT = 1, ADG_FIXED_SOURCE = 2
+  integer(c_int), parameter :: ADG_FORWARD = 0, ADG_ADJOINT = 1, ADG_FIXED_SOURCE = 2, ADG_TRANSIENT = 3
+  integer(c_int), parameter :: ADG_ADJOINT_FROZEN = 4
   integer(c_int), parameter :: ADG_FP64 = 0, ADG_FP32_REFINE = 1
 
   type, bind(C) :: adg_geom_t
@@ -95,6 +96,18 @@ module adpres_gpu
     type(c_ptr) function adg_last_error(ctx) bind(C, name='adg_last_error')
       import :: c_ptr
       type(c_ptr), value :: ctx
+    end function
+
+    integer(c_int) function adg_set_transient(ctx, wtr) bind(C, name='adg_set_transient')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value          :: ctx
+      real(c_double), intent(in)  :: wtr(*)
+    end function
+
+    integer(c_int) function adg_leakage(ctx, L) bind(C, name='adg_leakage')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value          :: ctx
+      real(c_double), intent(out) :: L(*)
     end function
 
   end interface

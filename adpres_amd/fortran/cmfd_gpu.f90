@@ -1,5 +1,5 @@
 ! Drop-in replacement of ADPRES's hot-path modules `cmfd` and `nodal`
-! (src/mod_cmfd.f90, src/mod_nodal.f90) for the static path.
+! (src/mod_cmfd.f90, src/mod_nodal.f90).
 !
 ! It exports the names the unchanged host modules use -- outer, outer_ad,
 ! PowDis (mod_control, mod_th), outer_fs, outer_th, outer_tr, Integrate
@@ -12,23 +12,11 @@
 ! Printed lines follow the reference's formats (src/mod_cmfd.f90:331,
 ! 367-374, 491-494, 500-504, 583-584, 589-591; src/mod_nodal.f90:131-142).
 !
-! The fixed-source outer (outer_fs) runs on the GPU with k held fixed, and
-! the T-H outer (outer_th) as a bounded forward outer between T-H updates.
-! The transient outer variant is not served by this build; it stops with a
-! message.
+! The fixed-source outer (outer_fs) runs on the GPU with k held fixed, the
+! T-H outer (outer_th) as a bounded forward outer between T-H updates, and
+! the transient outer (outer_tr) with the time step's external source formed
+! here on the host and the transient nodal update on the device.
 
-module nodal
-  use sdata, only: dp
-  implicit none
-contains
-  subroutine Lxyz(n, g, L1, L2, L3)
-    integer, intent(in)   :: n, g
-    real(dp), intent(out) :: L1, L2, L3
-    L1 = 0._dp; L2 = 0._dp; L3 = 0._dp
-    write(*,*) '  Lxyz (transient transverse leakage) is not served by the GPU build'
-    stop
-  end subroutine
-end module nodal
 
 module cmfd
 
@@ -38,11 +26,13 @@ module cmfd
   implicit none
   save
   private
-  public :: outer, outer_ad, outer_fs, outer_th, outer_tr, PowDis, Integrate
+  public :: outer, outer_ad, outer_fs, outer_th, outer_tr, PowDis, Integrate, leakage_of
 
   type(c_ptr) :: ctx = c_null_ptr
   logical     :: started = .false.
   logical     :: fixed_mode = .false.   ! outer_fs prints no k
+  real(c_double), allocatable :: leak(:,:,:)   ! Lxyz of the device state (nnod, ng, 3)
+  logical     :: leak_ok = .false.
   integer(c_int), allocatable, target :: gix(:), giy(:), giz(:)
   real(c_double), allocatable, target :: gxd(:), gyd(:), gzd(:)
   real(c_double), allocatable, target :: tk(:), tser(:), tfer(:), tnd(:)
@@ -84,9 +74,10 @@ contains
     stop
   end subroutine
 
-  ! one static solve on the device (shared by outer, outer_ad, outer_fs and
-  ! outer_th; maxn: outer_th's own outer limit, reached without a stop)
-  subroutine run_outer(popt, mode, label, maxn)
+  ! one solve on the device (shared by outer, outer_ad, outer_fs, outer_th
+  ! and outer_tr; maxn: an outer limit reached without a stop, reported in
+  ! hit when present)
+  subroutine run_outer(popt, mode, label, maxn, hit)
     use sdata, only: nnod, ng, nout, nin, serc, ferc, nac, nupd, kern, f0, fs0, Ke, &
                      D, sigr, sigs, nuf, sigf, chi, mat, dc, exsrc, ndmax, im, jm, km, &
                      fdm_time, nod_time, ser, fer
@@ -95,6 +86,7 @@ contains
     integer(c_int), intent(in) :: mode
     character(len=*), intent(in) :: label
     integer, optional, intent(in) :: maxn
+    logical, optional, intent(out) :: hit
     type(adg_iter_t)  :: it
     type(adg_trace_t) :: tr
     integer(c_int)    :: rc, rco, first
@@ -147,10 +139,13 @@ contains
       ndmax = tnd(tr%n); im = tloc(3*tr%n-2); jm = tloc(3*tr%n-1); km = tloc(3*tr%n)
       ser = tser(tr%n); fer = tfer(tr%n)   ! the last outer's errors, as the callers print them
     end if
+    leak_ok = .false.
+    call update_s0(mode)
     rc = adg_last_timing(ctx, ms)
     fdm_time = fdm_time + (ms(1) - ms(3)) * 1.e-3_dp
     nod_time = nod_time + ms(3) * 1.e-3_dp
 
+    if (present(hit)) hit = rco == ADG_E_MAX_OUTER
     if (present(maxn) .and. rco == ADG_E_MAX_OUTER) rco = ADG_OK
     select case (rco)
     case (ADG_OK)
@@ -232,7 +227,13 @@ contains
     integer :: po
     po = 0
     if (present(popt)) po = popt
-    call run_outer(po, ADG_ADJOINT, 'ADJOINT')
+    ! with popt = 0 (rod_eject) the reference keeps the forward D-hats
+    ! (src/mod_cmfd.f90:677-679)
+    if (po > 0) then
+      call run_outer(po, ADG_ADJOINT, 'ADJOINT')
+    else
+      call run_outer(po, ADG_ADJOINT_FROZEN, 'ADJOINT')
+    end if
   end subroutine outer_ad
 
   subroutine outer_fs(popt)
@@ -254,14 +255,120 @@ contains
     call run_outer(0, ADG_FORWARD, 'T-H', maxn)
   end subroutine outer_th
 
+  ! outer_tr (src/mod_cmfd.f90:800-880): the transient fixed-source outer of
+  ! one time step ht, after the step's external source; maxi when nout
+  ! outers did not converge
   subroutine outer_tr(ht, maxi)
+    use sdata, only: nnod, nout, tbeta, mat, dfis
     real(dp), intent(in) :: ht
     logical, intent(out) :: maxi
-    maxi = .false.
-    if (ht < 0._dp) continue
-    write(*,*) '  TRANSIENT OUTER ITERATION IS NOT SERVED BY THE GPU BUILD'
-    stop
+    real(c_double), allocatable :: w(:)
+    integer :: n
+    call transient_source(ht)
+    allocate(w(nnod))
+    do n = 1, nnod
+      w(n) = 1._dp - tbeta(mat(n)) + dfis(n)
+    end do
+    call ensure_context()
+    call check(adg_set_transient(ctx, w), 'ADG_SET_TRANSIENT')
+    call run_outer(0, ADG_TRANSIENT, 'TRANSIENT', nout, maxi)
   end subroutine outer_tr
+
+  ! The transient's external source and delayed-fission weight of a time
+  ! step ht, into sdata's exsrc and dfis (the reference's get_exsrc,
+  ! src/mod_cmfd.f90:884-952): precursors decayed over the step, the
+  ! previous step's flux and its theta-method terms (bth, sth).  Without
+  ! branch tables the decay factors depend on the precursor group only and
+  ! are formed once per group.
+  subroutine transient_source(ht)
+    use sdata, only: nnod, ng, nf, lamb, c0, iBeta, chi, mat, velo, fst, ft, omeg, dfis, nuf, &
+                     bth, sth, s0, tbeta, sigrp, L, m, exsrc
+    use io, only: bxtab
+    real(dp), intent(in) :: ht
+    real(dp), allocatable :: pxg(:), a1g(:), a2g(:)
+    real(dp) :: pxe, a1, a2, dt, dtp, pthet, lam, ib, ve
+    integer :: n, i, g
+    allocate(pxg(nf), a1g(nf), a2g(nf))
+    if (bxtab /= 1) then
+      do i = 1, nf
+        pxg(i) = exp(-lamb(i)*ht)
+        a1g(i) = (1._dp - pxg(i)) / (lamb(i)*ht)
+        a2g(i) = 1._dp - a1g(i)
+        a1g(i) = a1g(i) - pxg(i)
+      end do
+    end if
+    dfis = 0._dp
+    do n = 1, nnod
+      dt = 0._dp; dtp = 0._dp
+      do i = 1, nf
+        if (bxtab == 1) then
+          lam = m(mat(n))%lamb(i); ib = m(mat(n))%iBeta(i)
+          pxe = exp(-lam*ht)
+          if (nuf(n,ng) > 0.) then
+            a1 = (1._dp - pxe) / (lam*ht)
+          else
+            a1 = 0._dp
+          end if
+          a2 = 1._dp - a1
+          a1 = a1 - pxe
+        else
+          lam = lamb(i); ib = iBeta(i)
+          pxe = pxg(i); a1 = a1g(i); a2 = a2g(i)
+        end if
+        dfis(n) = dfis(n) + ib * a2
+        dt  = dt + lam * c0(n,i) * pxe + ib * a1 * fst(n)
+        dtp = dtp + lam * c0(n,i)
+      end do
+      do g = 1, ng
+        if (bxtab == 1) then
+          ve = m(mat(n))%velo(g)
+        else
+          ve = velo(g)
+        end if
+        pthet = -L(n,g) - sigrp(n,g) * ft(n,g) + s0(n,g) &
+              + (1._dp - tbeta(mat(n))) * chi(mat(n),g) * fst(n) + chi(mat(n),g) * dtp
+        exsrc(n,g) = chi(mat(n),g) * dt + exp(omeg(n,g) * ht) * ft(n,g) / (sth * ve * ht) + bth * pthet
+      end do
+    end do
+  end subroutine transient_source
+
+  ! sdata's s0 as the reference leaves it after a solve: TSrc, TSrcAd and
+  ! TSrcTr (src/mod_cmfd.f90:1006-1096) clear all of s0 and fill the group
+  ! being solved, so after the sweep it holds the in-scatter source of the
+  ! group solved last (the last group; the first in the adjoint sweep) from
+  ! the final flux.  The transient source reads it.
+  subroutine update_s0(mode)
+    use sdata, only: nnod, ng, sigs, f0, s0
+    integer(c_int), intent(in) :: mode
+    integer :: g, h, n
+    if (.not. allocated(s0)) allocate(s0(nnod, ng))
+    s0 = 0._dp
+    g = ng
+    if (mode == ADG_ADJOINT .or. mode == ADG_ADJOINT_FROZEN) g = 1
+    do h = 1, ng
+      if (h == g) cycle
+      do n = 1, nnod
+        if (mode == ADG_ADJOINT .or. mode == ADG_ADJOINT_FROZEN) then
+          s0(n,g) = s0(n,g) + sigs(n,g,h) * f0(n,h)
+        else
+          s0(n,g) = s0(n,g) + sigs(n,h,g) * f0(n,h)
+        end if
+      end do
+    end do
+  end subroutine update_s0
+
+  ! Lxyz of node n, group g of the device state, fetched once per solve
+  subroutine leakage_of(n, g, L1, L2, L3)
+    use sdata, only: nnod, ng
+    integer, intent(in)   :: n, g
+    real(dp), intent(out) :: L1, L2, L3
+    if (.not. leak_ok) then
+      if (.not. allocated(leak)) allocate(leak(nnod, ng, 3))
+      call check(adg_leakage(ctx, leak), 'ADG_LEAKAGE')
+      leak_ok = .true.
+    end if
+    L1 = leak(n,g,1); L2 = leak(n,g,2); L3 = leak(n,g,3)
+  end subroutine leakage_of
 
   ! node powers from the device (PowDis, src/mod_cmfd.f90:1290-1333)
   subroutine PowDis(p)
@@ -277,13 +384,30 @@ contains
     call check(rc, 'ADG_POWDIS')
   end subroutine PowDis
 
+  ! Integrate (src/mod_cmfd.f90:1120-1139): volume integral over the nodes
   function Integrate(s) result(intg)
+    use sdata, only: nnod, vdel
     real(dp), dimension(:), intent(in) :: s
     real(dp) :: intg
+    integer :: n
     intg = 0._dp
-    if (size(s) >= 0) continue
-    write(*,*) '  Integrate (transient reactivity) is not served by the GPU build'
-    stop
+    do n = 1, nnod
+      intg = intg + vdel(n) * s(n)
+    end do
   end function Integrate
 
 end module cmfd
+
+! nodal::Lxyz for the host's transient routines (mod_trans's reactivity):
+! the zeroth transverse leakages of the device state
+module nodal
+  use sdata, only: dp
+  implicit none
+contains
+  subroutine Lxyz(n, g, L1, L2, L3)
+    use cmfd, only: leakage_of
+    integer, intent(in)   :: n, g
+    real(dp), intent(out) :: L1, L2, L3
+    call leakage_of(n, g, L1, L2, L3)
+  end subroutine
+end module nodal

@@ -343,6 +343,31 @@ def read_ref_dump(path: str, action: str = "solve"):
         res["loc"] = tuple(int(v) for v in r.take("<i4", 3))
     elif action == "bicg":
         res["x"] = r.take("<f8", G * N).reshape(G, N)
+    elif action == "fwdadj":
+        res["Ke_s"] = float(r.take("<f8", 1)[0])
+        res["f0_s"] = r.take("<f8", G * N).reshape(G, N)
+        res["fs0_s"] = r.take("<f8", N)
+        res["Ke"] = float(r.take("<f8", 1)[0])
+        res["f0"] = r.take("<f8", G * N).reshape(G, N)
+        res["fs0"] = r.take("<f8", N)
+        res["dn"] = r.take("<f8", 6 * G * N).reshape(6, G, N)
+        res["ndmax"] = float(r.take("<f8", 1)[0])
+    elif action == "trstep":
+        res["Ke_s"] = float(r.take("<f8", 1)[0])
+        res["f0_s"] = r.take("<f8", G * N).reshape(G, N)
+        res["fs0_s"] = r.take("<f8", N)
+        res["dn_s"] = r.take("<f8", 6 * G * N).reshape(6, G, N)
+        res["ht"] = float(r.take("<f8", 1)[0])
+        res["tbeta"] = r.take("<f8", int(nmat))
+        res["sigr_tr"] = r.take("<f8", G * N).reshape(G, N)
+        res["dfis"] = r.take("<f8", N)
+        res["exsrc"] = r.take("<f8", G * N).reshape(G, N)
+        res["Ke"] = float(r.take("<f8", 1)[0])
+        res["f0"] = r.take("<f8", G * N).reshape(G, N)
+        res["fs0"] = r.take("<f8", N)
+        res["dn"] = r.take("<f8", 6 * G * N).reshape(6, G, N)
+        res["df"] = r.take("<f8", 6 * G * N).reshape(6, G, N)
+        res["maxi"] = int(r.take("<i4", 1)[0])
     return p, res
 
 

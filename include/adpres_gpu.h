@@ -39,7 +39,8 @@ enum adg_status {
 };
 
 enum adg_kern { ADG_KERN_FDM = 0, ADG_KERN_PNM = 1, ADG_KERN_SANM = 2 };   /* %KERN */
-enum adg_mode { ADG_FORWARD = 0, ADG_ADJOINT = 1, ADG_FIXED_SOURCE = 2 };  /* %MODE */
+enum adg_mode { ADG_FORWARD = 0, ADG_ADJOINT = 1, ADG_FIXED_SOURCE = 2, ADG_TRANSIENT = 3,
+                ADG_ADJOINT_FROZEN = 4 };  /* %MODE */
 enum adg_precision {
   ADG_FP64 = 0,          /* plain FP64 drop-in                                        */
   ADG_FP32_REFINE = 1    /* FP32 BiCGSTAB on an FP64 defect (residual) each group solve */
@@ -97,12 +98,27 @@ int adg_get_state(adg_ctx *ctx, double *f0, double *fs0, double *Ke, double *dn,
 /* cmfd::outer (src/mod_cmfd.f90:415-509) for mode ADG_FORWARD,
  * cmfd::outer_ad (:602-699) for ADG_ADJOINT and cmfd::outer_fs (:513-598)
  * for ADG_FIXED_SOURCE (the forward outer with the external source of
- * adg_set_xs and k held at its value: 1 on the first call).  trace may be
- * NULL. */
+ * adg_set_xs and k held at its value: 1 on the first call), cmfd::outer_tr
+ * (:800-880) for ADG_TRANSIENT (the external source is the caller's
+ * get_exsrc result, passed to adg_set_xs; the fission source is weighted by
+ * adg_set_transient's 1 - beta + dfis and not divided by k; the nodal update
+ * runs in its transient form, cmode 2; ADG_E_MAX_OUTER is outer_tr's
+ * maxi = .TRUE.) and cmfd::outer_ad called with popt = 0 for
+ * ADG_ADJOINT_FROZEN (the adjoint outer with the forward D-hats kept, no
+ * nodal update, :677-679: rod_eject's adjoint).  trace may be NULL. */
 int adg_outer(adg_ctx *ctx, int mode, const adg_iter *iter, adg_trace *trace);
 
 /* cmfd::PowDis (src/mod_cmfd.f90:1290-1333): node powers normalised to 1. */
 int adg_powdis(adg_ctx *ctx, double *p);
+/* The per-node factor (1 - tbeta(mat(n)) + dfis(n)) of TSrcTr
+ * (src/mod_cmfd.f90:1068-1096) and of the transient get_B
+ * (src/mod_nodal.f90:1379-1391), for ADG_TRANSIENT outers: wtr(N). */
+int adg_set_transient(adg_ctx *ctx, const double *wtr);
+int adg_set_transient_core(adg_ctx *ctx, int core, const double *wtr);
+/* nodal::Lxyz (src/mod_nodal.f90:901-1005) of every node and group of the
+ * context's current flux and D-tilde/D-hat: L(N,G,3), the x, y and z zeroth
+ * transverse leakages (single-core context). */
+int adg_leakage(adg_ctx *ctx, double *L);
 
 /* Kernel-level entry points, each replacing one reference routine. */
 /* nodal::nodal_update / nodal_update_pnm (src/mod_nodal.f90:18,151) followed

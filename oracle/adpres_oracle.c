@@ -210,7 +210,7 @@ static int bicg_q(const geo_t *q, const double *A, int imax, int g, const double
 typedef struct {
   const geo_t *q;
   orc_state *st;
-  int cmode;               /* 1 forward, 0 adjoint */
+  int cmode;               /* 1 forward, 0 adjoint, 2 transient */
   int G;
   double *S[3];            /* S1, S2, S3 (N,G) */
   double *Bcn, *Bcp;       /* (G,G) row-major [g*G+h] */
@@ -287,16 +287,23 @@ static void lxyz(const nod_t *w, int n, int g, double *L1, double *L2, double *L
 #undef DN
 }
 
-/* get_source (src/mod_nodal.f90:1009-1043), cmode /= 2 */
+/* get_source (src/mod_nodal.f90:1009-1043); cmode 2 subtracts exsrc here */
 static void get_source(nod_t *w) {
   int N = w->q->N, G = w->q->G;
+  const double *ex = w->q->pb->exsrc;
   for (int g = 0; g < G; ++g)
     for (int n = 0; n < N; ++n) {
       double L1, L2, L3;
       lxyz(w, n, g, &L1, &L2, &L3);
-      w->S[0][IDX2(n, g, N)] = L2 + L3;
-      w->S[1][IDX2(n, g, N)] = L1 + L3;
-      w->S[2][IDX2(n, g, N)] = L1 + L2;
+      if (w->cmode == 2) {
+        w->S[0][IDX2(n, g, N)] = L2 + L3 - ex[IDX2(n, g, N)];
+        w->S[1][IDX2(n, g, N)] = L1 + L3 - ex[IDX2(n, g, N)];
+        w->S[2][IDX2(n, g, N)] = L1 + L2 - ex[IDX2(n, g, N)];
+      } else {
+        w->S[0][IDX2(n, g, N)] = L2 + L3;
+        w->S[1][IDX2(n, g, N)] = L1 + L3;
+        w->S[2][IDX2(n, g, N)] = L1 + L2;
+      }
     }
 }
 
@@ -416,7 +423,8 @@ static double tlupd2(nod_t *w, int u, int n, int g) {
   return 0.25 * (hn * hn) / pb->D[IDX2(n, g, N)] * L;
 }
 
-/* get_B (src/mod_nodal.f90:1345-1405), cmode 1 forward / 0 adjoint */
+/* get_B (src/mod_nodal.f90:1345-1405), cmode 1 forward / 0 adjoint / 2
+   transient (chi twice in the diagonal term, as the reference has it) */
 static void get_B(const nod_t *w, int u, int n, double *B) {
   const orc_problem *pb = w->q->pb;
   int N = w->q->N, G = w->q->G;
@@ -429,6 +437,10 @@ static void get_B(const nod_t *w, int u, int n, double *B) {
       if (w->cmode == 1) {
         if (g == hh) dum = pb->sigr[IDX2(n, g, N)] - pb->chi[IDX2(mt, g, pb->nmat)] * pb->nuf[IDX2(n, hh, N)] / Ke;
         else dum = -pb->sigs[IDX3(n, hh, g, N, G)] - pb->chi[IDX2(mt, g, pb->nmat)] * pb->nuf[IDX2(n, hh, N)] / Ke;
+      } else if (w->cmode == 2) {
+        double c = pb->chi[IDX2(mt, g, pb->nmat)];
+        if (g == hh) dum = pb->sigr[IDX2(n, g, N)] - pb->wtr[n] * c * c * pb->nuf[IDX2(n, hh, N)];
+        else dum = -pb->sigs[IDX3(n, hh, g, N, G)] - pb->wtr[n] * c * pb->nuf[IDX2(n, hh, N)];
       } else {
         if (g == hh) dum = pb->sigr[IDX2(n, g, N)] - pb->chi[IDX2(mt, g, pb->nmat)] * pb->nuf[IDX2(n, hh, N)] / Ke;
         else dum = -pb->sigs[IDX3(n, g, hh, N, G)] - pb->chi[IDX2(mt, hh, pb->nmat)] * pb->nuf[IDX2(n, g, N)] / Ke;
@@ -507,7 +519,8 @@ static void get_a2matvec(nod_t *w, int u, int n, double *A, double *b) {
   for (int g = 0; g < G; ++g) Bf[g] = 0.0;
   for (int g = 0; g < G; ++g) {
     double Su = w->S[u - 1][IDX2(n, g, N)];
-    double S = 0.25 * (h * h) / pb->D[IDX2(n, g, N)] * (Su - pb->exsrc[IDX2(n, g, N)]);
+    double S = w->cmode == 2 ? 0.25 * (h * h) / pb->D[IDX2(n, g, N)] * Su
+                             : 0.25 * (h * h) / pb->D[IDX2(n, g, N)] * (Su - pb->exsrc[IDX2(n, g, N)]);
     for (int hh = 0; hh < G; ++hh) {
       if (hh == g) A[g * G + g] = w->Bcp[g * G + hh] * w->Ep[g] + 3.0;
       else A[g * G + hh] = w->Bcp[g * G + hh] * w->Ep[g];
@@ -832,12 +845,33 @@ static int nodal_update_q(const geo_t *q, orc_state *st, int kern, int cmode) {
 }
 
 int orc_nodal_update(const orc_problem *pb, orc_state *st, int kern, int cmode) {
+  if (cmode == 2 && !pb->wtr) return ORC_E_ARG;
   geo_t q;
   int rc = geo_init(&q, pb);
   if (rc == ORC_OK) {
     coup_coef_q(&q, st->df);
     st->ndmax = 0.0;
     rc = nodal_update_q(&q, st, kern, cmode);
+  }
+  geo_free(&q);
+  return rc;
+}
+
+/* Lxyz (src/mod_nodal.f90:901-1005) of every node and group of the state
+   (D-tilde recomputed from D, as coup_coef at the start of an outer):
+   L(N,G,3) column-major */
+int orc_leakage(const orc_problem *pb, orc_state *st, double *L) {
+  geo_t q;
+  int rc = geo_init(&q, pb);
+  if (rc == ORC_OK) {
+    int N = pb->nnod, G = pb->ng;
+    nod_t w;
+    memset(&w, 0, sizeof w);
+    w.q = &q; w.st = st; w.G = G;
+    coup_coef_q(&q, st->df);
+    for (int g = 0; g < G; ++g)
+      for (int n = 0; n < N; ++n)
+        lxyz(&w, n, g, &L[IDX3(n, g, 0, N, G)], &L[IDX3(n, g, 1, N, G)], &L[IDX3(n, g, 2, N, G)]);
   }
   geo_free(&q);
   return rc;
@@ -898,8 +932,9 @@ static void fsrc(const orc_problem *pb, const double *f0, double *fs, int adjoin
 }
 
 /* TSrc (src/mod_cmfd.f90:1006-1033) / TSrcAd (:1037-1064) */
+/* TSrc / TSrcAd / TSrcTr (src/mod_cmfd.f90:1006-1096): transient = TSrcTr */
 static void tsrc(const orc_problem *pb, int g, double Keff, const double *f0, const double *fs0,
-                 double *s0g, double *bs, int adjoint) {
+                 double *s0g, double *bs, int adjoint, int transient) {
   int N = pb->nnod, G = pb->ng;
   for (int n = 0; n < N; ++n) s0g[n] = 0.0;
   for (int h = 0; h < G; ++h)
@@ -909,6 +944,10 @@ static void tsrc(const orc_problem *pb, int g, double Keff, const double *f0, co
         s0g[n] = s0g[n] + sg * f0[IDX2(n, h, N)];
       }
   for (int n = 0; n < N; ++n) {
+    if (transient) {
+      bs[n] = pb->wtr[n] * pb->chi[IDX2(pb->mat[n] - 1, g, pb->nmat)] * fs0[n] + s0g[n] + pb->exsrc[IDX2(n, g, N)];
+      continue;
+    }
     double w = adjoint ? pb->nuf[IDX2(n, g, N)] : pb->chi[IDX2(pb->mat[n] - 1, g, pb->nmat)];
     bs[n] = w * fs0[n] / Keff + s0g[n] + pb->exsrc[IDX2(n, g, N)];
   }
@@ -955,8 +994,12 @@ static double releg(const double *nw, const double *od, int N, int G) {
 int orc_outer(const orc_problem *pb, orc_state *st, const orc_iter *it, int mode, orc_trace *tr) {
   int N = pb->nnod, G = pb->ng;
   /* mode 0: outer, 1: outer_ad, 2: outer_fs (src/mod_cmfd.f90:513-598: the
-     forward outer with k held at its value, no k update) */
-  const int adjoint = mode == 1, fixed = mode == 2;
+     forward outer with k held at its value, no k update), 3: outer_tr
+     (:800-880: TSrcTr sources, no k update, nodal update in cmode 2) */
+  const int adjoint = mode == 1 || mode == 4, transient = mode == 3, fixed = mode == 2 || mode == 3;
+  /* mode 4: outer_ad with popt = 0 (:677-679): no nodal update */
+  const int nodal = it->kern != 0 && mode != 4;
+  if (transient && !pb->wtr) return ORC_E_ARG;
   geo_t q;
   int rc = geo_init(&q, pb);
   if (rc != ORC_OK) { geo_free(&q); return rc; }
@@ -992,7 +1035,7 @@ int orc_outer(const orc_problem *pb, orc_state *st, const orc_iter *it, int mode
     double Keo = st->Ke;
     for (int gg = 0; gg < G; ++gg) {
       int g = adjoint ? G - 1 - gg : gg;
-      tsrc(pb, g, st->Ke, st->f0, st->fs0, s0g, bs, adjoint);
+      tsrc(pb, g, st->Ke, st->f0, st->fs0, s0g, bs, adjoint, transient);
       bicg_q(&q, A, it->nin, g, bs, st->f0 + (size_t)N * g);
     }
     fsrc(pb, st->f0, st->fs0, adjoint);
@@ -1012,10 +1055,10 @@ int orc_outer(const orc_problem *pb, orc_state *st, const orc_iter *it, int mode
     }
     double ser = rele(st->fs0, fs0c, N);
     double fer = releg(st->f0, f0c, N, G);
-    if (p % it->nupd == 0 && it->kern != 0) {
+    if (p % it->nupd == 0 && nodal) {
       /* nodal_upd (src/mod_cmfd.f90:339-383) */
       st->ndmax = 0.0;
-      rc = nodal_update_q(&q, st, it->kern, adjoint ? 0 : 1);
+      rc = nodal_update_q(&q, st, it->kern, adjoint ? 0 : (transient ? 2 : 1));
       flags |= 2;
       if (rc != ORC_OK) goto done;
       matrix_setup_q(&q, st->df, st->dn, A);

@@ -40,6 +40,7 @@ typedef struct {
   const double *chi;                      /* (nmat,G) */
   const double *dc;                       /* (N,G,6) */
   const double *exsrc;                    /* (N,G) */
+  const double *wtr;                      /* (N) transient weight 1 - tbeta + dfis (mode 3, cmode 2) */
 } orc_problem;
 
 typedef struct {
@@ -68,9 +69,16 @@ typedef struct {
 } orc_trace;
 
 /* Whole static solve: mode 0 outer (src/mod_cmfd.f90:415), 1 outer_ad
-   (:602), 2 outer_fs (:513, fixed source: k is not updated). */
+   (:602), 2 outer_fs (:513, fixed source: k is not updated), 3 outer_tr
+   (:800, one transient step: TSrcTr source with pb->wtr, no k update,
+   nodal update in cmode 2; exsrc and wtr are the step's get_exsrc
+   result), 4 outer_ad called with popt = 0 (:677-679: no nodal update,
+   the D-hats stay those of the forward state). */
 int orc_outer(const orc_problem *pb, orc_state *st, const orc_iter *it,
               int mode, orc_trace *tr);
+
+/* Lxyz (src/mod_nodal.f90:901-1005) of the state: L(N,G,3). */
+int orc_leakage(const orc_problem *pb, orc_state *st, double *L);
 
 /* coup_coef (src/mod_cmfd.f90:11): fills st->df. */
 void orc_coup_coef(const orc_problem *pb, orc_state *st);

@@ -29,7 +29,7 @@ class OrcProblem(C.Structure):
                 ("ystag_min", _ip), ("ystag_max", _ip), ("xstag_min", _ip), ("xstag_max", _ip),
                 ("xdel", _dp), ("ydel", _dp), ("zdel", _dp), ("bc", C.c_int * 6),
                 ("D", _dp), ("sigr", _dp), ("nuf", _dp), ("sigf", _dp), ("sigs", _dp),
-                ("chi", _dp), ("dc", _dp), ("exsrc", _dp)]
+                ("chi", _dp), ("dc", _dp), ("exsrc", _dp), ("wtr", _dp)]
 
 
 class OrcIter(C.Structure):
@@ -65,6 +65,7 @@ def lib():
                                 C.c_int, C.POINTER(OrcTrace)]
         L.orc_nodal_update.argtypes = [C.POINTER(OrcProblem), C.POINTER(OrcState), C.c_int, C.c_int]
         L.orc_bicg.argtypes = [C.POINTER(OrcProblem), C.POINTER(OrcState), C.c_int, C.c_int, _dp, _dp]
+        L.orc_leakage.argtypes = [C.POINTER(OrcProblem), C.POINTER(OrcState), _dp]
         L.orc_matvec.argtypes = [C.POINTER(OrcProblem), C.POINTER(OrcState), C.c_int, _dp, _dp]
         L.orc_powdis.argtypes = [C.POINTER(OrcProblem), _dp, _dp]
         L.orc_coup_coef.argtypes = [C.POINTER(OrcProblem), C.POINTER(OrcState)]
@@ -107,6 +108,7 @@ class Oracle:
         for k, v in dbls.items():
             setattr(pb, k, _d(v))
         pb.bc = (C.c_int * 6)(*p.bc)
+        self.set_transient(getattr(p, "wtr", None), pb)
         self.pb = pb
         N, G = p.nnod, p.ng
         self.f0 = np.ones((G, N)); self.fs0 = np.zeros(N)
@@ -120,8 +122,19 @@ class Oracle:
     def Ke(self):
         return self.st.Ke
 
+    def set_transient(self, wtr, pb=None):
+        """Per-node weight 1 - tbeta + dfis of the transient source and
+        nodal update (None: no transient form)."""
+        pb = pb if pb is not None else self.pb
+        if wtr is None:
+            pb.wtr = None
+            return
+        w = np.ascontiguousarray(wtr, dtype=np.float64)
+        self._keep["wtr"] = w
+        pb.wtr = _d(w)
+
     def outer(self, adjoint=None, nout=None, nin=None, serc=None, ferc=None, nac=None, nupd=None,
-              kern=None, trace_cap=None, fixed_source=None):
+              kern=None, trace_cap=None, fixed_source=None, transient=False, nodal=True):
         p = self.p
         it = OrcIter(nout if nout is not None else p.nout, nin if nin is not None else p.nin,
                      nac if nac is not None else p.nac, nupd if nupd is not None else p.nupd,
@@ -131,7 +144,7 @@ class Oracle:
             adjoint = p.mode == "ADJOINT"
         if fixed_source is None:
             fixed_source = p.mode == "FIXEDSRC"
-        mode = 2 if fixed_source else int(bool(adjoint))
+        mode = 3 if transient else (2 if fixed_source else (int(bool(adjoint)) if nodal or not adjoint else 4))
         cap = trace_cap or it.nout
         self.trace = {k: np.zeros(cap) for k in ("k", "ser", "fer", "ndmax")}
         self.trace["flags"] = np.zeros(cap, np.int32)
@@ -144,6 +157,14 @@ class Oracle:
             self.trace[k] = self.trace[k][:tr.n]
         self.trace["loc"] = self.trace["loc"][:3 * tr.n].reshape(-1, 3)
         return rc
+
+    def leakage(self):
+        """Lxyz of every node and group of the current state: (3, G, N)."""
+        out = np.zeros((3, self.p.ng, self.p.nnod))
+        rc = lib().orc_leakage(C.byref(self.pb), C.byref(self.st), _d(out))
+        if rc != 0:
+            raise RuntimeError(f"orc_leakage: {rc}")
+        return out
 
     def nodal_update(self, Ke, f0, dn, kern=None, cmode=1):
         self.st.Ke = Ke

@@ -8,6 +8,8 @@
 !                                               outer_ad or outer_fs by %MODE)
 !   ref_dump DECK nodal OUT XSFILE|- STATEFILE  one nodal (SANM/PNM) update
 !   ref_dump DECK bicg  OUT XSFILE|- STATEFILE  one nin-iteration BiCGSTAB per group
+!   ref_dump DECK trstep OUT XSFILE|-           steady state, then one transient
+!                                               step (outer_tr) of a %EJCT deck
 !
 ! The deck is read by the reference's own parser (io::inp_read, mod_io.f90:69),
 ! cross sections are built by xsec::XS_updt (mod_xsec.f90:11).  An optional
@@ -25,13 +27,16 @@ program ref_dump
   use sdata
   use io,    only: inp_read
   use xsec,  only: XS_updt, Dsigr_updt
-  use cmfd,  only: outer, outer_ad, outer_fs, PowDis, matrix_setup, bicg
+  use cmfd,  only: outer, outer_ad, outer_fs, outer_tr, PowDis, matrix_setup, bicg
   use nodal, only: nodal_update, nodal_update_pnm
 
   implicit none
 
   character(len=256) :: action, outname, xsname, stname
-  integer :: u, n, g, nargs, kid, nin_p
+  integer :: u, n, g, nargs, kid, nin_p, j
+  real(dp) :: kes, ht
+  logical :: maxi
+  real(dp), allocatable :: f0s(:,:), fs0s(:), dns(:,:,:)
   real(dp), allocatable :: npw(:), bb(:,:), xx(:)
   real(dp), allocatable :: dnin(:,:,:), dfout(:,:,:), dnout(:,:,:)
 
@@ -139,6 +144,74 @@ program ref_dump
       f0(:, g) = xx
     end do
     write(u) f0
+
+  case ('fwdadj')
+    ! outer(0), then outer_ad(0) warm-started from the forward state (the
+    ! sequence of rod_eject, src/mod_trans.f90:53-66, without KNE1)
+    call outer(0)
+    kes = Ke
+    allocate(f0s(nnod, ng), fs0s(nnod))
+    f0s = f0
+    fs0s = fs0
+    call outer_ad(0)
+    allocate(dnout(nnod, ng, 6))
+    do g = 1, ng
+      do n = 1, nnod
+        dnout(n, g, :) = nod(n, g)%dn
+      end do
+    end do
+    write(u) kes, f0s, fs0s, Ke, f0, fs0, dnout, ndmax
+
+  case ('trstep')
+    ! the steady state, then the set-up rod_eject and trans_calc make for a
+    ! time step (src/mod_trans.f90:50-100, 400-416), with a rod-like
+    ! removal increase in part of the core, then one outer_tr
+    call outer(0)
+    kes = Ke
+    allocate(f0s(nnod, ng), fs0s(nnod), dns(nnod, ng, 6))
+    f0s = f0
+    fs0s = fs0
+    do g = 1, ng
+      do n = 1, nnod
+        dns(n, g, :) = nod(n, g)%dn
+      end do
+    end do
+    allocate(c0(nnod, nf), omeg(nnod, ng), L(nnod, ng))
+    allocate(ft(nnod, ng), fst(nnod), dfis(nnod), sigrp(nnod, ng))
+    tbeta = 0._dp
+    do n = 1, nmat
+      do j = 1, nf
+        tbeta(n) = tbeta(n) + iBeta(j)
+      end do
+    end do
+    do j = 1, nf
+      do n = 1, nnod
+        c0(n, j) = iBeta(j) / lamb(j) * fs0(n)
+      end do
+    end do
+    omeg = 0._dp
+    L = 0._dp
+    ht = 0.25_dp
+    ft = f0
+    fst = fs0
+    sigrp = sigr
+    do g = 1, ng
+      do n = 1, nnod
+        if (2 * iz(n) > nzz .and. ix(n) <= 3) sigr(n, g) = sigr(n, g) * 1.002_dp
+        sigr(n, g) = sigr(n, g) + 1._dp / (sth * velo(g) * ht)
+      end do
+    end do
+    call outer_tr(ht, maxi)
+    allocate(dfout(nnod, ng, 6), dnout(nnod, ng, 6))
+    do g = 1, ng
+      do n = 1, nnod
+        dfout(n, g, :) = nod(n, g)%df
+        dnout(n, g, :) = nod(n, g)%dn
+      end do
+    end do
+    j = 0
+    if (maxi) j = 1
+    write(u) kes, f0s, fs0s, dns, ht, tbeta, sigr, dfis, exsrc, Ke, f0, fs0, dnout, dfout, int(j, 4)
 
   case default
     write(*,*) 'unknown action ', trim(action)

@@ -269,6 +269,78 @@ def outputs():
     save("outputs_sweep", **{k.replace("-", "_"): v for k, v in toks.items()})
 
 
+TRANS_LINE = r"^\s*(\d+)\s+(\d+\.\d{3})\s+(-?\d+\.\d{4})\s+(\S+E[+-]\d\d)((?:\s+\d+\.\d\d)*)\s*$"
+
+
+LMW_TIGHT_ITER = "3000 2 1.e-8 1.e-8 13 60 20 60"
+
+
+def transients():
+    """The reference program on its LMW rod-movement transient
+    (smpl/transient/LMW: rod_eject, src/mod_trans.f90:15-170, one outer_tr
+    per time step): the printed table of time, reactivity, relative power
+    and bank positions.  Also for LMW_tight, the same deck with its %ITER
+    tolerances tightened from 1e-5 to 1e-8: the deck's own tolerance leaves
+    iteration errors that the Crank-Nicolson steps carry as an undamped
+    step-to-step oscillation (the reference's tight and loose runs differ by
+    2e-3 in relative power), so a comparison of two solvers needs converged
+    steps."""
+    import re
+    exe = os.path.join(ROOT, "oracle", "_ref", "adpres")
+    src = os.path.join(os.path.dirname(REF), "transient", "LMW")
+    text = open(src).read()
+    assert "1200 2 1.e-5 1.e-5 13 60 20 60" in text
+    decks = {"LMW": text, "LMW_tight": text.replace("1200 2 1.e-5 1.e-5 13 60 20 60", LMW_TIGHT_ITER)}
+    for name, body in decks.items():
+        with open(os.path.join(HERE, "decks", name), "w") as fh:
+            fh.write(body)
+        with tempfile.TemporaryDirectory() as d:
+            shutil.copy(os.path.join(HERE, "decks", name), os.path.join(d, name))
+            r = subprocess.run(["bash", "-c", f"ulimit -s unlimited; exec {exe} {name}"], cwd=d,
+                               capture_output=True, text=True)
+        if r.returncode != 0 or "ADPRES EXIT NORMALLY" not in r.stdout:
+            raise RuntimeError(r.stdout[-2000:])
+        rows = [m.groups() for m in map(re.compile(TRANS_LINE).match, r.stdout.splitlines()) if m]
+        save(name.lower() + "_transient", step=np.array([int(q[0]) for q in rows], np.int32),
+             time=np.array([float(q[1]) for q in rows]), rho=np.array([float(q[2]) for q in rows]),
+             power=np.array([float(q[3]) for q in rows]),
+             banks=np.array([[float(x) for x in q[4].split()] for q in rows]))
+
+
+def trstep():
+    """One transient step of the reference on the LMW deck (ref_dump trstep:
+    the steady state, then rod_eject / trans_calc's set-up with a rod-like
+    removal increase and one outer_tr, src/mod_cmfd.f90:800-880): the
+    step's inputs (steady state, modified removal, get_exsrc's source and
+    dfis) and its results at full precision."""
+    with tempfile.TemporaryDirectory() as d:
+        shutil.copy(os.path.join(HERE, "decks", "LMW"), os.path.join(d, "deck"))
+        p, res, out = run(d, "deck", "trstep")
+    p.name = "lmw_trstep"
+    p.mode = "RODEJECT"
+    keep = {k: np.asarray(v) for k, v in res.items() if k not in ("vdel", "dn", "df")}
+    # the step's D-hat and D-tilde as SHA-256 of their float64 bytes (N, G, 6 order)
+    for k in ("dn", "df"):
+        keep[k + "_sha"] = np.asarray(hashlib.sha256(np.ascontiguousarray(res[k]).tobytes()).hexdigest())
+    save("lmw_trstep", **problem_to_dict(p), **keep)
+
+
+def fwdadj():
+    """rod_eject's steady-state pair on LMW_tight (ref_dump fwdadj): outer(0),
+    then outer_ad(0) warm-started from it, which keeps the forward D-hats
+    (src/mod_cmfd.f90:677-679).  The forward state is stored as hashes (the
+    oracle recomputes it bit for bit); the adjoint state in full."""
+    with tempfile.TemporaryDirectory() as d:
+        shutil.copy(os.path.join(HERE, "decks", "LMW_tight"), os.path.join(d, "deck"))
+        p, res, out = run(d, "deck", "fwdadj")
+    p.name = "lmw_fwdadj"
+    p.mode = "RODEJECT"
+    sha = lambda a: np.asarray(hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest())
+    save("lmw_fwdadj", **problem_to_dict(p), Ke_s=res["Ke_s"], f0_s_sha=sha(res["f0_s"]),
+         fs0_s_sha=sha(res["fs0_s"]), Ke=res["Ke"], f0=res["f0"], fs0=res["fs0"], dn_sha=sha(res["dn"]),
+         ndmax=res["ndmax"])
+
+
 def adjoint_fdm(seed=1):
     """The 34x34x26-node 2-group core with the finite-difference kernel,
     solved by the reference in %MODE FORWARD and %MODE ADJOINT (outer_ad)
@@ -308,6 +380,10 @@ def main(names):
         neacrp()
     if not names or "outputs" in names:
         outputs()
+    if not names or "transients" in names:
+        transients()
+        trstep()
+        fwdadj()
     if "batch" in names:          # 256 reference solves on 7 host cores
         batch()
     if "core3d_8g" in names:      # about an hour of reference CPU time

@@ -170,3 +170,43 @@ def test_dropin_output_files_match_reference(deck, tmp_path):
     bad = [(i, a, b) for i, (a, b) in enumerate(zip(got, ref))
            if abs(float(a) - float(b)) > 1.0001 * printed_unit(b)]
     assert not bad, bad[:10]
+
+
+@pytest.mark.parametrize("deck", ["LMW_tight", "LMW"])
+def test_dropin_lmw_transient(deck, tmp_path):
+    """%MODE RODEJECT (the reference's LMW rod-movement transient, 240
+    Crank-Nicolson time steps): the host's rod_eject / trans_calc
+    (src/mod_trans.f90:15-170, 345-470) calls outer_tr every step, served on
+    the GPU with the step's external source formed by the drop-in and the
+    transient (cmode 2) nodal update on the device; the host's reactivity
+    reads Lxyz of the device state.
+
+    LMW_tight is the deck with its %ITER tolerances at 1e-8: the printed
+    table follows the reference's to one unit of the last printed digit of
+    the reactivity (1e-4 $) and to 2e-4 in relative power.  With the deck's
+    own 1e-5 the steps stop with iteration errors that the Crank-Nicolson
+    scheme carries as an undamped step-to-step oscillation (the reference's
+    own 1e-5 and 1e-8 runs differ by 2e-3 in power), so there the pair means
+    of consecutive steps are compared: power within 2e-3, reactivity within
+    5e-4 $."""
+    shutil.copy(os.path.join(GOLDEN, "decks", deck), tmp_path / deck)
+    r = subprocess.run(["bash", "-c", f"ulimit -s unlimited; exec {EXE} {deck}"], cwd=tmp_path,
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-3000:]
+    assert "ADPRES EXIT NORMALLY" in r.stdout
+    line = re.compile(r"^\s*(\d+)\s+(\d+\.\d{3})\s+(-?\d+\.\d{4})\s+(\S+E[+-]\d\d)((?:\s+\d+\.\d\d)*)\s*$")
+    rows = [m.groups() for m in map(line.match, r.stdout.splitlines()) if m]
+    d = golden(deck.lower() + "_transient")
+    assert [int(q[0]) for q in rows] == list(d["step"])
+    assert np.array_equal([float(q[1]) for q in rows], d["time"])
+    assert np.array_equal([[float(x) for x in q[4].split()] for q in rows], d["banks"])
+    rho = np.array([float(q[2]) for q in rows])
+    pw = np.array([float(q[3]) for q in rows])
+    print(deck, "max |drho| ($):", np.max(np.abs(rho - d["rho"])), " max rel dP:", np.max(np.abs(pw / d["power"] - 1)))
+    if deck == "LMW_tight":
+        assert np.max(np.abs(rho - d["rho"])) <= 1.0001e-4
+        assert np.max(np.abs(pw / d["power"] - 1)) <= 2e-4
+    else:
+        pair = lambda a: 0.5 * (a[1:-1:2] + a[2::2])
+        assert np.max(np.abs(pair(pw) / pair(d["power"]) - 1)) <= 2e-3
+        assert np.max(np.abs(pair(rho) - pair(d["rho"]))) <= 5e-4

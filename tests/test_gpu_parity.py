@@ -171,3 +171,74 @@ def test_gpu_autotuned_shape_keeps_parity(precision):
     assert abs(e.Ke - float(d["Ke"])) < K_TOL
     assert pow_err(e.powdis(), d["pow"]) < POW_TOL
     e.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32_refine"])
+def test_gpu_transient_step_matches_reference(precision):
+    """outer_tr on the GPU (ADG_TRANSIENT: TSrcTr source weighted by
+    adg_set_transient's 1 - beta + dfis, no k update, nodal update in cmode
+    2) from the steady state of the LMW deck, with the reference's removal,
+    external source and dfis of that step (oracle/ref_dump.f90 trstep):
+    the step's flux within 1e-4 relative of the reference's, k untouched."""
+    import dataclasses
+    d = golden("lmw_trstep")
+    p = problem_from_dict(d)
+    e = Engine(p, precision=precision)
+    e.outer()
+    assert abs(e.Ke - float(d["Ke_s"])) < K_TOL
+    k0 = e.Ke
+    e.set_xs(dataclasses.replace(p, sigr=d["sigr_tr"], exsrc=d["exsrc"]))
+    e.set_transient((1.0 - d["tbeta"][p.mat - 1]) + d["dfis"])
+    assert e.outer(transient=True) == 0
+    assert e.Ke == k0
+    f0 = e.state()["f0"]
+    assert np.max(np.abs(f0 / d["f0"] - 1)) < POW_TOL
+    assert np.max(np.abs(f0 / d["f0_s"] - 1)) > 1e-2     # the step moved the flux
+    e.close()
+
+
+def test_gpu_leakage_matches_oracle_and_matrix():
+    """adg_leakage (nodal::Lxyz of the device state, what the host's
+    transient reactivity reads) against the oracle's Lxyz of the same state,
+    and against the 7-point matrix: the three leakages of a node sum to
+    (A phi)_n - sigr phi_n, since each matrix row is the net current of the
+    node's faces plus its removal."""
+    from oracle.oracle import Oracle
+    d = golden("iaea3d_solve")
+    p = problem_from_dict(d)
+    e = Engine(p)
+    e.set_state(d["f0"], d["fs0"], float(d["Ke"]), d["dn"])
+    e.matvec(0, d["f0"][0])          # D-tilde of the state
+    L = e.leakage()
+    o = Oracle(p)
+    o.f0[...] = d["f0"]
+    o.dn[...] = d["dn"]
+    Lo = o.leakage()
+    scale = np.max(np.abs(Lo))
+    assert np.max(np.abs(L - Lo)) <= 1e-13 * scale
+    for g in range(p.ng):
+        net = e.matvec(g, d["f0"][g]) - p.sigr[g] * d["f0"][g]
+        assert np.max(np.abs(L[:, g].sum(axis=0) - net)) <= 1e-12 * scale, g
+    e.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32_refine"])
+def test_gpu_frozen_adjoint_matches_reference(precision):
+    """ADG_ADJOINT_FROZEN (outer_ad with popt = 0: forward D-hats kept,
+    src/mod_cmfd.f90:677-679) warm-started from the forward solve of the
+    LMW deck at tight tolerances: adjoint k within 1 pcm and the adjoint flux
+    within 1e-4 relative of the reference's, D-hats untouched."""
+    d = golden("lmw_fwdadj")
+    p = problem_from_dict(d)
+    e = Engine(p, precision=precision)
+    e.outer()
+    assert abs(e.Ke - float(d["Ke_s"])) < K_TOL
+    dn = e.state()["dn"]
+    e.outer(adjoint=True, nodal=False)
+    assert abs(e.Ke - float(d["Ke"])) < K_TOL
+    st = e.state()
+    assert np.array_equal(st["dn"], dn)
+    ref = d["f0"]
+    m = np.abs(ref) > 1e-12 * np.max(np.abs(ref))
+    assert np.max(np.abs(st["f0"][m] / ref[m] - 1)) < POW_TOL
+    e.close()

@@ -253,3 +253,83 @@ def test_sanm_coefficients_tend_to_pnm(oracle_lib, D, sigr, h):
     m1s = 3 * (np.cosh(a) / a - np.sinh(a) / a ** 2)
     assert np.isclose(c[0], (np.sinh(a) - m1s) / (a * a * m1s), rtol=1e-12)
     assert np.isclose(c[5], (a * np.cosh(a) - m1s) / (np.sinh(a) - m1s), rtol=1e-12)
+
+
+def _transient_step(oracle_lib, d):
+    """The oracle's steady state of the LMW deck, then one transient step
+    (outer_tr) from it with the reference's removal, external source and
+    weight of that step."""
+    import dataclasses
+    p = problem_from_dict(d)
+    o = oracle_lib.Oracle(p)
+    assert o.outer() == 0
+    p2 = dataclasses.replace(p, sigr=d["sigr_tr"], exsrc=d["exsrc"])
+    o2 = oracle_lib.Oracle(p2)
+    o2.f0[...] = o.f0
+    o2.fs0[...] = o.fs0
+    o2.dn[...] = o.dn
+    o2.st.Ke = o.Ke
+    o2.st.ndmax = o.st.ndmax
+    o2.st.first = 0
+    o2.set_transient((1.0 - d["tbeta"][p.mat - 1]) + d["dfis"])
+    return o, o2, o2.outer(transient=True)
+
+
+def test_oracle_transient_step_bit_exact(oracle_lib):
+    """outer_tr (src/mod_cmfd.f90:800-880: TSrcTr source, no k update,
+    nodal update in its transient form cmode 2) against the reference's own
+    step of the LMW deck (oracle/ref_dump.f90 trstep), bit for bit."""
+    import hashlib
+    d = golden("lmw_trstep")
+    o, o2, rc = _transient_step(oracle_lib, d)
+    assert o.Ke == float(d["Ke_s"]) and np.array_equal(o.f0, d["f0_s"]) and np.array_equal(o.fs0, d["fs0_s"])
+    assert np.array_equal(o.dn, d["dn_s"])
+    assert rc == 0 and int(d["maxi"]) == 0
+    assert o2.Ke == float(d["Ke"])
+    assert np.array_equal(o2.f0, d["f0"]) and np.array_equal(o2.fs0, d["fs0"])
+    assert hashlib.sha256(np.ascontiguousarray(o2.dn).tobytes()).hexdigest() == str(d["dn_sha"])
+    assert hashlib.sha256(np.ascontiguousarray(o2.df).tobytes()).hexdigest() == str(d["df_sha"])
+    # the step moved the state: the transient form is exercised, including
+    # its nodal update
+    assert np.max(np.abs(o2.f0 / o.f0 - 1)) > 1e-2 and np.max(np.abs(o2.dn - o.dn)) > 1e-4
+
+
+def test_oracle_frozen_adjoint_bit_exact(oracle_lib):
+    """outer_ad called with popt = 0, as rod_eject does (src/mod_trans.f90:65,
+    src/mod_cmfd.f90:677-679): warm-started from the forward state with its
+    D-hats kept, no nodal update.  Bit for bit against the reference on the
+    LMW deck at tight tolerances; the same warm start with adjoint nodal
+    updates diverges there (nodal instability), which is why the reference
+    freezes them."""
+    import hashlib
+    sha = lambda a: hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+    d = golden("lmw_fwdadj")
+    p = problem_from_dict(d)
+    o = oracle_lib.Oracle(p)
+    assert o.outer() == 0
+    assert o.Ke == float(d["Ke_s"]) and sha(o.f0) == str(d["f0_s_sha"]) and sha(o.fs0) == str(d["fs0_s_sha"])
+    dn_fwd = o.dn.copy()
+    assert o.outer(adjoint=True, nodal=False) == 0
+    assert o.Ke == float(d["Ke"])
+    assert np.array_equal(o.f0, d["f0"]) and np.array_equal(o.fs0, d["fs0"])
+    assert sha(o.dn) == str(d["dn_sha"]) and np.array_equal(o.dn, dn_fwd)
+    assert o.st.ndmax == float(d["ndmax"])
+
+
+@pytest.mark.parametrize("name", ["iaea3d_solve", "koeberg_solve"])
+def test_oracle_leakages_sum_to_matrix_net_current(oracle_lib, name):
+    """Lxyz (src/mod_nodal.f90:901-1005) of a converged state: the three
+    leakages of a node sum to (A phi)_n - sigr phi_n of the 7-point matrix,
+    each of whose rows is the net current through the node's faces plus its
+    removal (on converged D-hats, which vanish on reflective faces)."""
+    d = golden(name)
+    p = problem_from_dict(d)
+    o = oracle_lib.Oracle(p)
+    o.f0[...] = d["f0"]
+    o.dn[...] = d["dn"]
+    L = o.leakage()
+    scale = np.max(np.abs(L))
+    for g in range(p.ng):
+        rc, y = o.matvec(g, d["f0"][g], dn=d["dn"])
+        assert rc == 0
+        assert np.max(np.abs(L[:, g].sum(axis=0) - (y - p.sigr[g] * d["f0"][g]))) <= 1e-12 * scale, g
