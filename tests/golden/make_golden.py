@@ -290,7 +290,11 @@ def transients():
     src = os.path.join(os.path.dirname(REF), "transient", "LMW")
     text = open(src).read()
     assert "1200 2 1.e-5 1.e-5 13 60 20 60" in text
-    decks = {"LMW": text, "LMW_tight": text.replace("1200 2 1.e-5 1.e-5 13 60 20 60", LMW_TIGHT_ITER)}
+    a1t = open(os.path.join(os.path.dirname(REF), "transient", "NEACRP", "A1t")).read()
+    assert "1200 3 1.e-5 1.e-5 15 20 20 80" in a1t
+    decks = {"LMW": text, "LMW_tight": text.replace("1200 2 1.e-5 1.e-5 13 60 20 60", LMW_TIGHT_ITER),
+             # NEACRP rod ejection at hot zero power with T-H feedback (rod_eject_th)
+             "A1t": a1t, "A1t_tight": a1t.replace("1200 3 1.e-5 1.e-5 15 20 20 80", "3000 3 1.e-8 1.e-8 15 20 20 80")}
     for name, body in decks.items():
         with open(os.path.join(HERE, "decks", name), "w") as fh:
             fh.write(body)

@@ -172,8 +172,8 @@ def test_dropin_output_files_match_reference(deck, tmp_path):
     assert not bad, bad[:10]
 
 
-@pytest.mark.parametrize("deck", ["LMW_tight", "LMW"])
-def test_dropin_lmw_transient(deck, tmp_path):
+@pytest.mark.parametrize("deck", ["LMW_tight", "LMW", "A1t_tight", "A1t"])
+def test_dropin_transient(deck, tmp_path):
     """%MODE RODEJECT (the reference's LMW rod-movement transient, 240
     Crank-Nicolson time steps): the host's rod_eject / trans_calc
     (src/mod_trans.f90:15-170, 345-470) calls outer_tr every step, served on
@@ -181,14 +181,21 @@ def test_dropin_lmw_transient(deck, tmp_path):
     transient (cmode 2) nodal update on the device; the host's reactivity
     reads Lxyz of the device state.
 
-    LMW_tight is the deck with its %ITER tolerances at 1e-8: the printed
+    A1t is the NEACRP rod ejection at hot zero power with T-H feedback
+    (rod_eject_th, src/mod_trans.f90:172-330: 280 steps, the power rising
+    from 1e-6 to 0.2 through a Doppler-limited excursion).
+
+    The *_tight decks have their %ITER tolerances at 1e-8: the printed
     table follows the reference's to one unit of the last printed digit of
-    the reactivity (1e-4 $) and to 2e-4 in relative power.  With the deck's
-    own 1e-5 the steps stop with iteration errors that the Crank-Nicolson
-    scheme carries as an undamped step-to-step oscillation (the reference's
-    own 1e-5 and 1e-8 runs differ by 2e-3 in power), so there the pair means
-    of consecutive steps are compared: power within 2e-3, reactivity within
-    5e-4 $."""
+    the reactivity (1e-4 $) and to 2e-4 in relative power.  With the decks'
+    own 1e-5 the steps stop with iteration errors of that size, which the
+    scheme carries forward: LMW's Crank-Nicolson steps as an undamped
+    step-to-step oscillation (the reference's own 1e-5 and 1e-8 runs differ
+    by 2e-3 in power), so there the pair means of consecutive steps are
+    compared, power within 2e-3 and reactivity within 5e-4 $; A1t's steep
+    excursion as a shift of the power rise (the reference's own 1e-5 and
+    1e-8 runs differ by 1.2e-2 in power and 6e-4 $), held to 2e-2 and
+    1e-3 $."""
     shutil.copy(os.path.join(GOLDEN, "decks", deck), tmp_path / deck)
     r = subprocess.run(["bash", "-c", f"ulimit -s unlimited; exec {EXE} {deck}"], cwd=tmp_path,
                        capture_output=True, text=True, timeout=300)
@@ -203,9 +210,12 @@ def test_dropin_lmw_transient(deck, tmp_path):
     rho = np.array([float(q[2]) for q in rows])
     pw = np.array([float(q[3]) for q in rows])
     print(deck, "max |drho| ($):", np.max(np.abs(rho - d["rho"])), " max rel dP:", np.max(np.abs(pw / d["power"] - 1)))
-    if deck == "LMW_tight":
+    if deck.endswith("_tight"):
         assert np.max(np.abs(rho - d["rho"])) <= 1.0001e-4
         assert np.max(np.abs(pw / d["power"] - 1)) <= 2e-4
+    elif deck == "A1t":
+        assert np.max(np.abs(rho - d["rho"])) <= 1e-3
+        assert np.max(np.abs(pw / d["power"] - 1)) <= 2e-2
     else:
         pair = lambda a: 0.5 * (a[1:-1:2] + a[2::2])
         assert np.max(np.abs(pair(pw) / pair(d["power"]) - 1)) <= 2e-3
