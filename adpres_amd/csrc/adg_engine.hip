@@ -114,6 +114,7 @@ struct adg_ctx {
   Scal *hsc = nullptr;             // pinned host mirror (B)
   bool graphs = true;              // replay the outer iteration as a captured hipGraph
   int check_every = 8;             // outers between host polls of the done flags
+  bool nod_coop = true;            // cooperative pass 2 for G >= 5 (ADG_NOD_COOP=0: per-thread)
   double timing[4] = {0, 0, 0, 0};
   std::string err;
   // profiling: event pairs recorded on the context stream during adg_outer
@@ -473,9 +474,14 @@ void nodal_update(adg_ctx *c, int kern, int cmode, int sched) {
   a.S = c->S; a.a2 = c->a2; a.a4 = c->a4; a.L1 = c->L1;
   a.kern = kern; a.cmode = cmode; a.sched = sched;
   a.part = c->npart;
+  a.coop = 0;
   NodArgs an = a, af = a;
   an.lv = c->lv(c->tcell);
   af.lv = c->lv(c->tface);
+  // interior faces of larger group counts in the cooperative pass 2 (one
+  // lane per row of the 2G x 2G system); ADG_NOD_COOP=0 keeps them in the
+  // per-thread kernel
+  af.coop = c->G >= 5 && c->nod_coop ? 1 : 0;
   const dim3 g2(c->nface, 3, 1);
   switch (c->G) {
 #define NOD_CASE(GG)                                                                    \
@@ -483,6 +489,7 @@ void nodal_update(adg_ctx *c, int kern, int cmode, int sched) {
     hipLaunchKernelGGL(k_nod_src, dim3(c->ncell, GG), dim3(NT), 0, c->s, an);           \
     hipLaunchKernelGGL(k_nod_pass1<GG>, dim3(c->ncell, 3), dim3(NT), 0, c->s, an);      \
     hipLaunchKernelGGL(k_nod_pass2<GG>, g2, dim3(NT), 0, c->s, af);                     \
+    if (af.coop) hipLaunchKernelGGL(k_nod_pass2c<GG>, g2, dim3(NT), 0, c->s, af);       \
     break;
     NOD_CASE(1) NOD_CASE(2) NOD_CASE(3) NOD_CASE(4) NOD_CASE(5) NOD_CASE(6) NOD_CASE(7) NOD_CASE(8)
 #undef NOD_CASE
@@ -866,6 +873,7 @@ int create_batch(const adg_geom *geoms, int ncore, adg_ctx **out) {
     c->precision = geoms[0].precision;
     if (const char *ev = getenv("ADG_GRAPH")) c->graphs = atoi(ev) != 0;
     if (const char *ev = getenv("ADG_CHECK")) c->check_every = std::max(1, atoi(ev));
+    if (const char *ev = getenv("ADG_NOD_COOP")) c->nod_coop = atoi(ev) != 0;
     c->core.resize(ncore);
     std::vector<std::vector<unsigned char>> fls(ncore);
     std::vector<std::vector<double>> vds(ncore);

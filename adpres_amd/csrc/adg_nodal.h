@@ -30,6 +30,7 @@ struct NodArgs {
   int kern;             // 1 PNM, 2 SANM
   int cmode;            // 1 forward, 0 adjoint, 2 transient
   int sched;            // 1: run only in a nodal-update outer (device-side schedule)
+  int coop;             // pass 2 leaves the interior faces to k_nod_pass2c
   double *part;
 };
 
@@ -138,31 +139,35 @@ __global__ void __launch_bounds__(NT) k_leak(NodArgs a, double *out) {
 // per-node helpers
 // ---------------------------------------------------------------------------
 
-// get_B (src/mod_nodal.f90:1345-1405); B row-major [g*G+h]
+// get_B (src/mod_nodal.f90:1345-1405), row g: B(g, 0..G-1)
 template <int G>
-__device__ __forceinline__ void nod_B(const NodArgs &a, long NC, long c, double h, double Ke, double *B) {
+__device__ __forceinline__ void nod_B_row(const NodArgs &a, long NC, long c, double h, double Ke, int g, double *B) {
 #pragma clang fp contract(off)
   const XS &x = a.xs;
-  for (int g = 0; g < G; ++g) {
-    const double D = x.D[(size_t)g * NC + c];
-    for (int hh = 0; hh < G; ++hh) {
-      double dum;
-      if (a.cmode == 1) {
-        if (g == hh) dum = x.sigr[(size_t)g * NC + c] - x.chi[(size_t)g * NC + c] * x.nuf[(size_t)hh * NC + c] / Ke;
-        else dum = -x.sigs[((size_t)g * G + hh) * NC + c] - x.chi[(size_t)g * NC + c] * x.nuf[(size_t)hh * NC + c] / Ke;
-      } else if (a.cmode == 2) {
-        // transient (:1379-1391): fission weighted by 1 - beta + dfis, no 1/k;
-        // the diagonal term carries chi twice, as the reference has it
-        const double wc = x.wtr[c] * x.chi[(size_t)g * NC + c];
-        if (g == hh) dum = x.sigr[(size_t)g * NC + c] - wc * x.chi[(size_t)g * NC + c] * x.nuf[(size_t)hh * NC + c];
-        else dum = -x.sigs[((size_t)g * G + hh) * NC + c] - wc * x.nuf[(size_t)hh * NC + c];
-      } else {
-        if (g == hh) dum = x.sigr[(size_t)g * NC + c] - x.chi[(size_t)g * NC + c] * x.nuf[(size_t)hh * NC + c] / Ke;
-        else dum = -x.sigs[((size_t)hh * G + g) * NC + c] - x.chi[(size_t)hh * NC + c] * x.nuf[(size_t)g * NC + c] / Ke;
-      }
-      B[g * G + hh] = 0.25 * (h * h) / D * dum;
+  const double D = x.D[(size_t)g * NC + c];
+  for (int hh = 0; hh < G; ++hh) {
+    double dum;
+    if (a.cmode == 1) {
+      if (g == hh) dum = x.sigr[(size_t)g * NC + c] - x.chi[(size_t)g * NC + c] * x.nuf[(size_t)hh * NC + c] / Ke;
+      else dum = -x.sigs[((size_t)g * G + hh) * NC + c] - x.chi[(size_t)g * NC + c] * x.nuf[(size_t)hh * NC + c] / Ke;
+    } else if (a.cmode == 2) {
+      // transient (:1379-1391): fission weighted by 1 - beta + dfis, no 1/k;
+      // the diagonal term carries chi twice, as the reference has it
+      const double wc = x.wtr[c] * x.chi[(size_t)g * NC + c];
+      if (g == hh) dum = x.sigr[(size_t)g * NC + c] - wc * x.chi[(size_t)g * NC + c] * x.nuf[(size_t)hh * NC + c];
+      else dum = -x.sigs[((size_t)g * G + hh) * NC + c] - wc * x.nuf[(size_t)hh * NC + c];
+    } else {
+      if (g == hh) dum = x.sigr[(size_t)g * NC + c] - x.chi[(size_t)g * NC + c] * x.nuf[(size_t)hh * NC + c] / Ke;
+      else dum = -x.sigs[((size_t)hh * G + g) * NC + c] - x.chi[(size_t)hh * NC + c] * x.nuf[(size_t)g * NC + c] / Ke;
     }
+    B[hh] = 0.25 * (h * h) / D * dum;
   }
+}
+
+// get_B, all rows; B row-major [g*G+h]
+template <int G>
+__device__ __forceinline__ void nod_B(const NodArgs &a, long NC, long c, double h, double Ke, double *B) {
+  for (int g = 0; g < G; ++g) nod_B_row<G>(a, NC, c, h, Ke, g, B + g * G);
 }
 
 // get_ABEFGH (src/mod_nodal.f90:1409-1451) or the PNM constants (:180-182)
@@ -418,7 +423,7 @@ __global__ void __launch_bounds__(NT) k_nod_pass2(NodArgs a) {
     const long cn = cp - st;                // node below the face ("n")
     const bool hasp = (q.flag[cp] & F_ACTIVE) != 0;
     const bool hasn = (q.flag[cn] & F_ACTIVE) != 0;
-    if (hasp || hasn) {
+    if ((hasp || hasn) && !(a.coop && hasp && hasn)) {
       const double Ke = sc->Ke;
       const int sf = 2 * u;                 // upper face of n (x+, y+, z+), sf+1 lower face of p
       const unsigned long long ord = face_order(q, u, fi, o1, o2);
@@ -607,6 +612,200 @@ __global__ void __launch_bounds__(NT) k_nod_pass2(NodArgs a) {
   }
   if (threadIdx.x == 0) {
     const size_t slot = ((size_t)blockIdx.y * gridDim.x + blockIdx.x) * 3;
+    a.part[slot] = sv[0];
+    a.part[slot + 1] = sk[0];
+    a.part[slot + 2] = sl[0];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// pass 2, interior faces, cooperative form for larger group counts: 16 lanes
+// per face (four faces per wavefront), lane 4 r + f owning row r of face f's
+// 2G x 2G system of get_a1matvec
+// (:630-698).  The elimination broadcasts the pivot row with lane shuffles,
+// so no lane holds more than its own row; every entry sees the operations
+// of the per-thread lu_solve in the same order.  A block covers the same
+// 256-face range as k_nod_pass2 (16 faces at a time) and merges its argmax
+// partial with the one that k_nod_pass2 (boundary faces only, coop set)
+// wrote for the block.
+// ---------------------------------------------------------------------------
+template <int G>
+__global__ void __launch_bounds__(NT) k_nod_pass2c(NodArgs a) {
+#pragma clang fp contract(off)
+  constexpr int M2 = 2 * G;
+  static_assert(M2 <= 16, "16 lanes per face");
+  const BlkView v = blk_view(a.lv);
+  const Geo &q = *v.q;
+  Scal *sc = v.sc;
+  if (a.sched && !nodal_outer(sc)) return;   // uniform over the core's blocks
+  const int u = blockIdx.y;
+  const long NC = q.NC;
+  const int nu = u == 0 ? q.nxx : (u == 1 ? q.nyy : q.nzz);
+  const int e1 = u == 0 ? q.nyy : q.nxx;
+  const int e2 = u == 2 ? q.nyy : q.nzz;
+  const long nf = (long)(nu + 1) * e1 * e2;
+  // lane = 4 r + f: the four faces f of a wavefront side by side, so that a
+  // row's loads of four neighbouring cells share cache lines
+  const int lane = threadIdx.x & 63, r = lane >> 2, fw = lane & 3;
+  const int slot16 = (int)(threadIdx.x >> 6) * 4 + fw;
+  const int g = r % G;                         // group of this lane's row (rows r and r + G)
+  const bool rowlane = r < M2;
+  double best = -1.0, bkey = 0.0, bloc = 0.0;
+  for (int it = 0; it < NT / 16; ++it) {
+    const long t = v.b.lo + (long)it * 16 + slot16;
+    if (!(t < v.b.hi && t < nf) || sc->done) continue;   // uniform over the 16 lanes of the face
+    // faces enumerated with i fastest in every direction (the argmax key
+    // below does not depend on the enumeration)
+    int fi, o1, o2;
+    if (u == 0) {
+      fi = (int)(t % (nu + 1));
+      const long rr = t / (nu + 1);
+      o1 = (int)(rr % e1); o2 = (int)(rr / e1);
+    } else {
+      o1 = (int)(t % e1);
+      const long rr = t / e1;
+      fi = (int)(rr % (nu + 1)); o2 = (int)(rr / (nu + 1));
+    }
+    int i, j, k;
+    if (u == 0) { i = fi; j = o1; k = o2; }
+    else if (u == 1) { i = o1; j = fi; k = o2; }
+    else { i = o1; j = o2; k = fi; }
+    const long st = u == 0 ? 1 : (u == 1 ? q.sy : q.sz);
+    const long cp = cell_of(q, i, j, k);
+    const long cn = cp - st;
+    if (!((q.flag[cp] & F_ACTIVE) && (q.flag[cn] & F_ACTIVE))) continue;   // interior faces only
+    const double Ke = sc->Ke;
+    const int sf = 2 * u;
+    const unsigned long long ord = face_order(q, u, fi, o1, o2);
+    const size_t ub = (size_t)u * G * NC;
+    int li = i, lj = j, lk = k;
+    if (u == 0) li = i - 1; else if (u == 1) lj = j - 1; else lk = k - 1;
+    const double hn = hdir(q, u, u == 0 ? i - 1 : i, u == 1 ? j - 1 : j, u == 2 ? k - 1 : k);
+    const double hp = hdir(q, u, i, j, k);
+    // this lane's group: rows g of B at n and p, SANM/PNM coefficients
+    double Bn[G], Bp[G];
+    nod_B_row<G>(a, NC, cn, hn, Ke, g, Bn);
+    nod_B_row<G>(a, NC, cp, hp, Ke, g, Bp);
+    const Coef6 cfn = nod_coef(a.kern, a.xs.sigr[(size_t)g * NC + cn], a.xs.D[(size_t)g * NC + cn], hn);
+    const Coef6 cfp = nod_coef(a.kern, a.xs.sigr[(size_t)g * NC + cp], a.xs.D[(size_t)g * NC + cp], hp);
+    const double Ln1 = a.L1[ub + (size_t)g * NC + cn], Lp1 = a.L1[ub + (size_t)g * NC + cp];
+    const double a2n = a.a2[ub + (size_t)g * NC + cn], a4n = a.a4[ub + (size_t)g * NC + cn];
+    const double a2p = a.a2[ub + (size_t)g * NC + cp], a4p = a.a4[ub + (size_t)g * NC + cp];
+    // row r of the system
+    double R[M2], s;
+    if (r < G) {
+      const double Pn = 2.0 * a.xs.D[(size_t)g * NC + cn] / hn;
+      const double Pp = 2.0 * a.xs.D[(size_t)g * NC + cp] / hp;
+#pragma unroll
+      for (int hh = 0; hh < G; ++hh) {
+        if (hh == g) {
+          R[hh] = -Pn * (Bn[hh] * cfn.F + 1.0);
+          R[hh + G] = Pp * (Bp[hh] * cfp.F + 1.0);
+        } else {
+          R[hh] = -Pn * Bn[hh] * cfn.F;
+          R[hh + G] = Pp * Bp[hh] * cfp.F;
+        }
+      }
+      s = Pn * (3.0 * a2n + cfn.G * a4n + cfn.F * Ln1) + Pp * (3.0 * a2p + cfp.G * a4p - cfp.F * Lp1);
+    } else {
+      const double dcn = a.xs.dc[((size_t)sf * G + g) * NC + cn];
+      const double dcp = a.xs.dc[((size_t)(sf + 1) * G + g) * NC + cp];
+#pragma unroll
+      for (int hh = 0; hh < G; ++hh) {
+        if (hh == g) {
+          R[hh] = dcn * (Bn[hh] * cfn.A + 1.0);
+          R[hh + G] = dcp * (Bp[hh] * cfp.A + 1.0);
+        } else {
+          R[hh] = dcn * Bn[hh] * cfn.A;
+          R[hh + G] = dcp * Bp[hh] * cfp.A;
+        }
+      }
+      // the reference pairs An with Ln1 and Ap with Lp1 here (:693-694)
+      s = dcp * (a2p + a4p + a.f0[(size_t)g * NC + cp] - cfn.A * Ln1) -
+          dcn * (a2n + a4n + a.f0[(size_t)g * NC + cn] + cfp.A * Lp1);
+    }
+    // LU_solve (:829-897): the input-diagonal test, then the elimination
+    double dg = 0.0;
+#pragma unroll
+    for (int c2 = 0; c2 < M2; ++c2) if (c2 == r) dg = R[c2];
+    if (rowlane && fabs(dg) < (double)10e-5f) atomicMin(&sc->lu_key, lu_key(ord, li, lj, lk));
+#pragma unroll
+    for (int p = 0; p < M2; ++p) {
+      // pivot row p (its entries p..M2-1, final after the steps before p)
+      double pr[M2];
+#pragma unroll
+      for (int c2 = p; c2 < M2; ++c2) pr[c2] = __shfl(R[c2], 4 * p + fw, 64);
+      if (r > p && rowlane) {
+        const double piv = R[p] / pr[p];
+#pragma unroll
+        for (int c2 = p + 1; c2 < M2; ++c2) R[c2] = R[c2] - piv * pr[c2];
+        R[p] = piv;
+      }
+    }
+    // forward substitution y(i) = b(i) - sum_k<i L(i,k) y(k), k ascending
+    double isum = 0.0, y = 0.0;
+#pragma unroll
+    for (int p = 0; p < M2; ++p) {
+      if (r == p) y = s - isum;
+      const double yp = __shfl(y, 4 * p + fw, 64);
+      if (r > p) isum = isum + R[p] * yp;
+    }
+    // back substitution x(i) = (y(i) - sum_k>i U(i,k) x(k)) / U(i,i), k ascending
+    double xs[M2];
+#pragma unroll
+    for (int p = M2 - 1; p >= 0; --p) {
+      double xv;
+      if (p == M2 - 1) {
+        xv = y / R[p];
+      } else {
+        double bs = 0.0;
+#pragma unroll
+        for (int c2 = p + 1; c2 < M2; ++c2) bs = bs + R[c2] * xs[c2];
+        xv = (y - bs) / R[p];
+      }
+      xs[p] = __shfl(xv, 4 * p + fw, 64);
+    }
+    // get_a3 and nodal_coup_upd for group g (lanes 0..G-1)
+    if (r < G) {
+      double Bf = 0.0;
+#pragma unroll
+      for (int hh = 0; hh < G; ++hh) Bf = Bf + Bn[hh] * xs[hh];
+      const double a3 = cfn.A * (Bf + Ln1);
+      const double jp = -2.0 * a.xs.D[(size_t)g * NC + cn] / hn * (xs[g] + 3.0 * a2n + cfn.H * a3 + cfn.G * a4n);
+      const size_t on = ((size_t)sf * G + g) * NC + cn, op = ((size_t)(sf + 1) * G + g) * NC + cp;
+      const double ndpr = a.dn[on];
+      const double fn = a.f0[(size_t)g * NC + cn], fp = a.f0[(size_t)g * NC + cp];
+      const double dv = (a.df[on] * (fn - fp) - jp) / (fn + fp);
+      a.dn[on] = dv;
+      a.dn[op] = dv;
+      const double nder = fabs(dv - ndpr), key = (double)(ord * G + g);
+      if (nder > best || (nder == best && key < bkey)) {
+        best = nder;
+        bkey = key;
+        bloc = (double)(((li + 1) << 20) | ((lj + 1) << 10) | (lk + 1));
+      }
+    }
+  }
+  // block argmax (largest value, then smallest sweep key), merged with the
+  // boundary faces' partial of this block
+  __shared__ double sv[NT], sk[NT], sl[NT];
+  sv[threadIdx.x] = best;
+  sk[threadIdx.x] = bkey;
+  sl[threadIdx.x] = bloc;
+  __syncthreads();
+  for (int w = NT / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      const int o = threadIdx.x + w;
+      if (sv[o] > sv[threadIdx.x] || (sv[o] == sv[threadIdx.x] && sk[o] < sk[threadIdx.x])) {
+        sv[threadIdx.x] = sv[o]; sk[threadIdx.x] = sk[o]; sl[threadIdx.x] = sl[o];
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const size_t slot = ((size_t)blockIdx.y * gridDim.x + blockIdx.x) * 3;
+    const double v0 = a.part[slot], k0 = a.part[slot + 1];
+    if (v0 > sv[0] || (v0 == sv[0] && k0 < sk[0])) return;
     a.part[slot] = sv[0];
     a.part[slot + 1] = sk[0];
     a.part[slot + 2] = sl[0];
