@@ -299,6 +299,59 @@ __device__ __forceinline__ void kb_load(KBIn<TV> &L, const TV *__restrict__ AT, 
   L.rsc = rs[c];
 }
 
+// one cell of KA from its loaded operands
+template <typename TV, bool FIRST>
+__device__ __forceinline__ void ka_cell(const KAIn<TV, FIRST> &L, long c, TV ta, TV tw, TV tb, TV *__restrict__ x,
+                                        TV *__restrict__ rout, TV *__restrict__ pout, TV *__restrict__ vout,
+                                        double &acc) {
+#pragma clang fp contract(off)
+  TV pn[7], rn;
+  if (FIRST) {
+#pragma unroll
+    for (int k = 0; k < 7; ++k) pn[k] = L.R[k];
+    rn = L.R[3];
+  } else {
+#pragma unroll
+    for (int k = 0; k < 7; ++k) {
+      const TV s_ = (TV)(L.R[k] - ta * L.V[k]);
+      const TV r_ = (TV)(s_ - tw * L.T[k]);
+      if (k == 3) rn = r_;
+      pn[k] = (TV)(r_ + tb * (L.P[k] - tw * L.V[k]));
+    }
+  }
+  double s = 0.0;
+#pragma unroll
+  for (int k = 0; k < 7; ++k) s = s + (double)(L.Ac[k] * pn[k]);
+  const TV vc = (TV)s;
+  if (!FIRST) {
+    const TV sv = (TV)(L.R[3] - ta * L.V[3]);
+    x[c] = (TV)(L.xc + ta * L.P[3] + tw * sv);
+  }
+  rout[c] = rn;
+  pout[c] = pn[3];
+  vout[c] = vc;
+  acc += (double)L.rsc * (double)vc;
+}
+
+// one cell of KB from its loaded operands
+template <typename TV>
+__device__ __forceinline__ void kb_cell(const KBIn<TV> &L, long c, TV alpha, TV *__restrict__ t, double (&acc)[4]) {
+#pragma clang fp contract(off)
+  const double rsc = (double)L.rsc;
+  TV sv[7];
+#pragma unroll
+  for (int k = 0; k < 7; ++k) sv[k] = (TV)(L.R[k] - alpha * L.V[k]);
+  double s = 0.0;
+#pragma unroll
+  for (int k = 0; k < 7; ++k) s = s + (double)(L.Ac[k] * sv[k]);
+  const TV tc = (TV)s;
+  t[c] = tc;
+  acc[0] += (double)tc * (double)tc;
+  acc[1] += (double)tc * (double)sv[3];
+  acc[2] += rsc * (double)tc;
+  acc[3] += rsc * (double)sv[3];
+}
+
 // With a single core (TAB == 0) the first cell's operands are requested
 // before the scalar prologue (the done flag, the partial sums of the
 // previous kernel and their reduction), which they do not depend on, so the
@@ -336,34 +389,20 @@ k_bicgA(LV lv, const TV *__restrict__ AT, int it, const TV *__restrict__ r, cons
   }
   const TV ta = (TV)alpha_d, tw = (TV)omega_d, tb = (TV)beta;
   double acc[1] = {0.0};
-  for (long c = c0; c < bv.b.hi; c += NTF) {
+  // ADG_KUNROLL cells per thread in flight: their loads are all issued
+  // before the first is used; the cells are still summed in order
+  for (long c = c0; c < bv.b.hi; c += ADG_KUNROLL * NTF) {
     if (TAB != 0 || c != c0) ka_load<TV, FIRST>(L, AT, r, p, v, t, rs, x, NC, off, c);
-    TV pn[7], rn;
-    if (FIRST) {
-#pragma unroll
-      for (int k = 0; k < 7; ++k) pn[k] = L.R[k];
-      rn = L.R[3];
-    } else {
-#pragma unroll
-      for (int k = 0; k < 7; ++k) {
-        const TV s_ = (TV)(L.R[k] - ta * L.V[k]);
-        const TV r_ = (TV)(s_ - tw * L.T[k]);
-        if (k == 3) rn = r_;
-        pn[k] = (TV)(r_ + tb * (L.P[k] - tw * L.V[k]));
-      }
-    }
-    double s = 0.0;
-#pragma unroll
-    for (int k = 0; k < 7; ++k) s = s + (double)(L.Ac[k] * pn[k]);
-    const TV vc = (TV)s;
-    if (!FIRST) {
-      const TV sv = (TV)(L.R[3] - ta * L.V[3]);
-      x[c] = (TV)(L.xc + ta * L.P[3] + tw * sv);
-    }
-    rout[c] = rn;
-    pout[c] = pn[3];
-    vout[c] = vc;
-    acc[0] += (double)L.rsc * (double)vc;
+#if ADG_KUNROLL == 2
+    KAIn<TV, FIRST> L2;
+    const long c2 = c + NTF;
+    const bool h2 = c2 < bv.b.hi;
+    if (h2) ka_load<TV, FIRST>(L2, AT, r, p, v, t, rs, x, NC, off, c2);
+#endif
+    ka_cell<TV, FIRST>(L, c, ta, tw, tb, x, rout, pout, vout, acc[0]);
+#if ADG_KUNROLL == 2
+    if (h2) ka_cell<TV, FIRST>(L2, c2, ta, tw, tb, x, rout, pout, vout, acc[0]);
+#endif
   }
   block_partial<1, 0u>(acc, partout, bv.b.slot);
   if (core_leader(bv)) {
@@ -391,21 +430,18 @@ k_bicgB(LV lv, const TV *__restrict__ AT, int it, const TV *__restrict__ r, cons
   const double alpha_d = sc->rho_s[cur] / red[0];
   const TV alpha = (TV)alpha_d;
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  for (long c = c0; c < bv.b.hi; c += NTF) {
+  for (long c = c0; c < bv.b.hi; c += ADG_KUNROLL * NTF) {
     if (TAB != 0 || c != c0) kb_load<TV>(L, AT, r, v, rs, NC, off, c);
-    const double rsc = (double)L.rsc;
-    TV sv[7];
-#pragma unroll
-    for (int k = 0; k < 7; ++k) sv[k] = (TV)(L.R[k] - alpha * L.V[k]);
-    double s = 0.0;
-#pragma unroll
-    for (int k = 0; k < 7; ++k) s = s + (double)(L.Ac[k] * sv[k]);
-    const TV tc = (TV)s;
-    t[c] = tc;
-    acc[0] += (double)tc * (double)tc;
-    acc[1] += (double)tc * (double)sv[3];
-    acc[2] += rsc * (double)tc;
-    acc[3] += rsc * (double)sv[3];
+#if ADG_KUNROLL == 2
+    KBIn<TV> L2;
+    const long c2 = c + NTF;
+    const bool h2 = c2 < bv.b.hi;
+    if (h2) kb_load<TV>(L2, AT, r, v, rs, NC, off, c2);
+#endif
+    kb_cell<TV>(L, c, alpha, t, acc);
+#if ADG_KUNROLL == 2
+    if (h2) kb_cell<TV>(L2, c2, alpha, t, acc);
+#endif
   }
   block_partial<4, 0u>(acc, partout, bv.b.slot);
   if (core_leader(bv)) sc->alpha_s[cur] = alpha_d;

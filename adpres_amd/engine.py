@@ -393,6 +393,15 @@ class Batch:
         self.status, self.niter, self.keff = st, ni, ke
         return st, ni, ke
 
+    def set_profiling(self, on: bool):
+        self._check(self.lib.adg_set_profiling(self.h, 1 if on else 0), "adg_set_profiling")
+
+    def read_profile(self):
+        t = np.zeros(6)
+        self._check(self.lib.adg_read_profile(self.h, _d(t)), "adg_read_profile")
+        return dict(k1_ms=t[0], k1_n=int(t[1]), gsolve_ms=t[2], gsolve_n=int(t[3]), nodal_ms=t[4],
+                    nodal_n=int(t[5]))
+
     def autotune(self, nin: Optional[int] = None) -> dict:
         """Pick the fastest cells-per-block of the flat kernels (see adg_autotune)."""
         out = np.zeros(4)

@@ -98,10 +98,38 @@ def batch_bench(args):
         for _ in range(args.steps):
             step()
         dt = (time.perf_counter() - t1) / args.steps
+        keff_t, status_t, niter_t = keff.copy(), status.copy(), niter.copy()
+        # roofline of KA over a whole context: HIP events on one steady-state
+        # KA launch per group solve, in the first outers of a fresh solve
+        # (every core still iterating, so each launch covers every core)
+        wsz = 4 if precision == "fp32_refine" else 8
+        roof_ctx, tot_b, tot_t = {}, 0.0, 0.0
+        for G, b in bats.items():
+            nodes = int(sum(cores[i].nnod for i in classes[G]))
+            b.set_profiling(True)
+            b.reset()
+            b.outer(nout=6)
+            pr = b.read_profile()
+            b.set_profiling(False)
+            if pr["k1_n"]:
+                nb = 17 * wsz * nodes
+                roof_ctx[str(G)] = {"nodes": nodes, "bytes_per_launch": nb, "avg_launch_us": round(pr["k1_ms"] * 1e3, 3),
+                                    "GBps": round(nb / (pr["k1_ms"] * 1e-3) / 1e9, 1), "samples": pr["k1_n"]}
+                tot_b += nb * pr["k1_n"]
+                tot_t += pr["k1_ms"] * 1e-3 * pr["k1_n"]
+        keff[:], status[:], niter[:] = keff_t, status_t, niter_t
         for b in bats.values():
             b.close()
         solved = int(np.sum(status == ADG_OK))
-        out = {"seconds_per_step": dt, "solves_per_s": solved / dt, "cores_solved": solved,
+        roof = None
+        if tot_t > 0:
+            ach = tot_b / tot_t / 1e9
+            roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                    "kernel": f"k_bicgA<{'float' if wsz == 4 else 'double'}> launched over every core of a context "
+                              "(17 words per node)",
+                    "contexts": roof_ctx}
+        out = {"seconds_per_step": dt, "solves_per_s": solved / dt, "cores_solved": solved, "roofline": roof,
                "cells_per_block": {str(G): t["shape"] for G, t in tunes.items()},
                "cores_stopped": int(len(cores) - solved), "max_outers": int(niter.max())}
         if ref is not None:
@@ -124,12 +152,27 @@ def batch_bench(args):
         "config": {"workload": "batch256", "cores": len(cores), "nodes": int(sum(q.nnod for q in cores)),
                    "unknowns": int(sum(q.unknowns for q in cores)),
                    "contexts": {str(G): len(idx) for G, idx in classes.items()}},
-        "fp32_refine": r32, "fp64": r64}
+        "roofline": r32["roofline"], "fp32_refine": r32, "fp64": r64}
     print(json.dumps(out))
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
+
+
+def lu_flops(m):
+    """Exact flop count of the reference's no-pivot LU_solve of order m
+    (src/mod_nodal.f90:829-897): elimination, forward and back substitution."""
+    return (m - 1) * m * (2 * m - 1) // 3 + 3 * m * (m - 1) // 2 + (m * m - 1) + (m * m + m - 1)
+
+
+def nodal_flops(p):
+    """LU flops of one nodal update: per direction u, N - L_u interface
+    systems of order 2G and N + 2 L_u node and boundary systems of order G
+    (L_u lines along u); assembly and transcendentals are not counted."""
+    G, N = p.ng, p.nnod
+    lines = [len(set(zip(p.iy, p.iz))), len(set(zip(p.ix, p.iz))), len(set(zip(p.ix, p.iy)))]
+    return sum((N - L) * lu_flops(2 * G) + (N + 2 * L) * lu_flops(G) for L in lines)
 
 
 def time_solves(eng, steps, warmup):
@@ -330,6 +373,14 @@ def main():
                                                         "stream, in a profiled solve after the timed ones",
                 "group_solve_avg_us": round(r64["gsolve_avg_ms"] * 1e3, 2),
                 "nodal_update_avg_us": round(r64["nodal_avg_ms"] * 1e3, 2)}
+        if r64["nodal_avg_ms"]:
+            nf = nodal_flops(p)
+            gf = nf / (r64["nodal_avg_ms"] * 1e-3) / 1e9
+            roof["nodal"] = {"bound": "valu", "flops_per_update": nf, "avg_update_us": round(r64["nodal_avg_ms"] * 1e3, 2),
+                             "achieved": round(gf, 1), "peak": 78600.0, "unit": "GFLOP/s (FP64 vector)",
+                             "frac": round(gf / 78600.0, 5),
+                             "note": "LU_solve flops only (no-pivot LU of order 2G per interface, G per node and "
+                                     "boundary); assembly, transcendentals and the D-hat epilogue not counted"}
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(p, r64["niter"], p.nupd)
