@@ -72,6 +72,7 @@ struct CoreHost {
   long base = 0;                   // first cell of its box in the batch arrays
   int bcface[6] = {0, 0, 0, 0, 0, 0};
   std::vector<long> cell;          // node n -> cell of the batch arrays
+  long coff = 0;                   // its node list's offset in the device cell list
   std::vector<double> hx, hy, hz;
   long hoff[3] = {0, 0, 0};        // offsets of hx, hy, hz in the device arrays
   bool xs_set = false, first = true;
@@ -98,6 +99,9 @@ struct adg_ctx {
   double *D = nullptr, *sigr = nullptr, *nuf = nullptr, *sigf = nullptr, *chi = nullptr, *sigs = nullptr;
   double *dc = nullptr, *exsrc = nullptr, *vdel = nullptr;
   double *wtr = nullptr;   // transient fission weight per cell (adg_set_transient)
+  long *d_cell = nullptr;  // node -> cell of every core (offset coff per core)
+  double *dstage = nullptr;   // node-ordered staging of host arrays
+  size_t stage_cap = 0;
   double *f0 = nullptr, *f0c = nullptr, *fs0 = nullptr, *fs0c = nullptr, *errn = nullptr;
   double *dn = nullptr, *df = nullptr, *A64 = nullptr, *b = nullptr, *pw = nullptr, *tmp = nullptr;
   // BiCGSTAB vectors: FP64 form
@@ -210,24 +214,57 @@ void prof_pair(adg_ctx *c, int type, int e0, int e1) {
 }
 
 // scatter a Fortran (N, K) array of core b into K group/face slots of its box
-void scatter(const adg_ctx *c, int b, const double *src, int K, double *dst_dev, std::vector<double> &stage) {
-  const CoreHost &h = c->core[b];
-  stage.assign((size_t)K * h.box, 0.0);
-  for (int k = 0; k < K; ++k)
-    for (int n = 0; n < h.N; ++n) stage[(size_t)k * h.box + (h.cell[n] - h.base)] = src[(size_t)k * h.N + n];
-  for (int k = 0; k < K; ++k)
-    scopy(c->s, dst_dev + (size_t)k * c->NC + h.base, stage.data() + (size_t)k * h.box, h.box * sizeof(double),
-          hipMemcpyHostToDevice);
+// Host arrays in the reference's node order (K components of N nodes) move
+// in one copy to a node-ordered device staging buffer, and a kernel places
+// them into the padded boxes (and back): the cells outside the footprint
+// keep the zeros they were allocated with.
+__global__ void k_scatter(const double *src, const long *cell, long N, long total, long NC, double *dst) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
+    const long k = i / N, n = i - k * N;
+    dst[k * NC + cell[n]] = src[i];
+  }
 }
 
-void gather(const adg_ctx *c, int b, const double *src_dev, int K, double *dst, std::vector<double> &stage) {
+__global__ void k_gather(const double *src, const long *cell, long N, long total, long NC, double *dst) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
+    const long k = i / N, n = i - k * N;
+    dst[i] = src[k * NC + cell[n]];
+  }
+}
+
+void ensure_stage(adg_ctx *c, size_t n) {
+  if (n <= c->stage_cap) return;
+  if (c->dstage) HIPCHK(hipFree(c->dstage));
+  c->dstage = nullptr;
+  c->stage_cap = 0;
+  c->dstage = dalloc<double>(c->s, n);
+  c->stage_cap = n;
+}
+
+void scatter(adg_ctx *c, int b, const double *src, int K, double *dst_dev, std::vector<double> &) {
   const CoreHost &h = c->core[b];
-  stage.resize((size_t)K * h.box);
-  for (int k = 0; k < K; ++k)
-    scopy(c->s, stage.data() + (size_t)k * h.box, src_dev + (size_t)k * c->NC + h.base, h.box * sizeof(double),
-          hipMemcpyDeviceToHost);
-  for (int k = 0; k < K; ++k)
-    for (int n = 0; n < h.N; ++n) dst[(size_t)k * h.N + n] = stage[(size_t)k * h.box + (h.cell[n] - h.base)];
+  const long total = (long)K * h.N;
+  if (total == 0) return;
+  ensure_stage(c, (size_t)total);
+  HIPCHK(hipMemcpyAsync(c->dstage, src, (size_t)total * sizeof(double), hipMemcpyHostToDevice, c->s));
+  const int nb = (int)std::min<long>((total + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(256), 0, c->s, c->dstage, c->d_cell + h.coff, (long)h.N, total,
+                     c->NC, dst_dev);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->s));
+}
+
+void gather(adg_ctx *c, int b, const double *src_dev, int K, double *dst, std::vector<double> &) {
+  const CoreHost &h = c->core[b];
+  const long total = (long)K * h.N;
+  if (total == 0) return;
+  ensure_stage(c, (size_t)total);
+  const int nb = (int)std::min<long>((total + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_gather, dim3(nb), dim3(256), 0, c->s, src_dev, c->d_cell + h.coff, (long)h.N, total, c->NC,
+                     c->dstage);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(dst, c->dstage, (size_t)total * sizeof(double), hipMemcpyDeviceToHost, c->s));
+  HIPCHK(hipStreamSynchronize(c->s));
 }
 
 // ---------------------------------------------------------------------------
@@ -891,6 +928,15 @@ int create_batch(const adg_geom *geoms, int ncore, adg_ctx **out) {
     }
     c->NC = base;
     HIPCHK(hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
+    {
+      std::vector<long> cl;
+      for (auto &h : c->core) {
+        h.coff = (long)cl.size();
+        cl.insert(cl.end(), h.cell.begin(), h.cell.end());
+      }
+      c->d_cell = dalloc<long>(c->s, cl.size());
+      scopy(c->s, c->d_cell, cl.data(), cl.size() * sizeof(long), hipMemcpyHostToDevice);
+    }
     const size_t NC = c->NC, G = c->G;
     c->d_hx = dalloc<double>(c->s, hx); c->d_hy = dalloc<double>(c->s, hy); c->d_hz = dalloc<double>(c->s, hz);
     c->flag = dalloc<unsigned char>(c->s, NC);
@@ -980,7 +1026,8 @@ void adg_destroy(adg_ctx *c) {
                   c->dn, c->df, c->A64, c->b, c->pw, c->tmp, c->rr64[0], c->rr64[1], c->rs64, c->t64, c->pp64[0],
                   c->pp64[1], c->vv64[0], c->vv64[1], c->A32, c->rr32[0], c->rr32[1], c->rs32, c->t32, c->d32,
                   c->pp32[0], c->pp32[1], c->vv32[0], c->vv32[1], c->S, c->a2, c->a4, c->L1, c->npart, c->part,
-                  c->sc, c->psi, c->phis, c->fs0s, c->exs, c->gam, c->dJdP, c->grad};
+                  c->sc, c->psi, c->phis, c->fs0s, c->exs, c->gam, c->dJdP, c->grad, c->wtr, c->d_cell,
+                  c->dstage};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (c->hsc) (void)hipHostFree(c->hsc);
