@@ -168,7 +168,9 @@ __global__ void __launch_bounds__(NTF) k_matvec(LV lv, const double *A64, int g,
 
 // smode 0: TSrc (forward, :1006-1033), 1: TSrcAd (:1037-1064), 2: TSrcTr
 // (transient, :1068-1096: fission source weighted by 1 - beta + dfis, no 1/k)
-template <int TAB>
+// (compiled per group count GG, so that the in-scatter loads of a cell are
+// issued together)
+template <int TAB, int GG>
 __global__ void __launch_bounds__(NTF) k_src(LV lv, XS xs, int g, int smode, int first_group, const double *f0,
                                             double *f0c, const double *fs0, double *fs0c, double *b) {
 #pragma clang fp contract(off)
@@ -176,12 +178,13 @@ __global__ void __launch_bounds__(NTF) k_src(LV lv, XS xs, int g, int smode, int
   if (v.sc->done) return;
   const double Ke = v.sc->Ke;
   const long NC = lv.NC;
-  const int G = lv.G;
+  constexpr int G = GG;
   const int adjoint = smode == 1;
   for (long c = v.b.lo + threadIdx.x; c < v.b.hi; c += NTF) {
     if (first_group) fs0c[c] = fs0[c];
     f0c[(size_t)g * NC + c] = f0[(size_t)g * NC + c];
     double s0 = 0.0;
+    #pragma unroll
     for (int h = 0; h < G; ++h) {
       if (h == g) continue;
       const double sg = adjoint ? xs.sigs[((size_t)h * G + g) * NC + c] : xs.sigs[((size_t)g * G + h) * NC + c];

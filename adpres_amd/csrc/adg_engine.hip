@@ -119,6 +119,7 @@ struct adg_ctx {
   bool graphs = true;              // replay the outer iteration as a captured hipGraph
   int check_every = 8;             // outers between host polls of the done flags
   bool nod_coop = true;            // cooperative pass 2 for G >= 5 (ADG_NOD_COOP=0: per-thread)
+  bool src_full = true;            // source kernel on one chunk per block (ADG_SRC_FULL=0: flat shape)
   double timing[4] = {0, 0, 0, 0};
   std::string err;
   // profiling: event pairs recorded on the context stream during adg_outer
@@ -493,12 +494,31 @@ void launch_bicg(adg_ctx *c, int g, int nin) {
 // one group solve: TSrc + bicg (src/mod_cmfd.f90:472-478)
 void group_solve(adg_ctx *c, int g, int smode, int first_group, int nin) {
   const int eg0 = prof_event(c);
-  if (c->B > 1)
-    hipLaunchKernelGGL(k_src<1>, dim3(c->nflat), dim3(NTF), 0, c->s, c->lv(c->tflat), c->xs(), g, smode,
-                       first_group, c->f0, c->f0c, c->fs0, c->fs0c, c->b);
-  else
-    hipLaunchKernelGGL(k_src<0>, dim3(c->nflat), dim3(NTF), 0, c->s, c->lv(c->tflat), c->xs(), g, smode,
-                       first_group, c->f0, c->f0c, c->fs0, c->fs0c, c->b);
+  // the source kernel has no reduction, so a single core's launch may use
+  // one NTF-cell chunk per block (all cells in flight at once) instead of
+  // the flat shape that the reductions of the BiCGSTAB kernels favour
+  LV lvs = c->lv(c->tflat);
+  int nbs = c->nflat;
+  if (c->B == 1 && c->src_full) {
+    const long cells = c->s_end - c->s_c0;
+    nbs = (int)((cells + NTF - 1) / NTF);
+    nbs = (nbs + 7) / 8 * 8;
+    lvs.chunk = NTF;
+    lvs.nb = nbs;
+  }
+  switch (c->G) {
+#define SRC_CASE(GG)                                                                                          \
+  case GG:                                                                                                    \
+    if (c->B > 1)                                                                                             \
+      hipLaunchKernelGGL((k_src<1, GG>), dim3(c->nflat), dim3(NTF), 0, c->s, c->lv(c->tflat), c->xs(), g, smode, \
+                         first_group, c->f0, c->f0c, c->fs0, c->fs0c, c->b);                                  \
+    else                                                                                                      \
+      hipLaunchKernelGGL((k_src<0, GG>), dim3(nbs), dim3(NTF), 0, c->s, lvs, c->xs(), g, smode,                \
+                         first_group, c->f0, c->f0c, c->fs0, c->fs0c, c->b);                                  \
+    break;
+    SRC_CASE(1) SRC_CASE(2) SRC_CASE(3) SRC_CASE(4) SRC_CASE(5) SRC_CASE(6) SRC_CASE(7) SRC_CASE(8)
+#undef SRC_CASE
+  }
   launch_bicg(c, g, nin);
   prof_pair(c, 1, eg0, prof_event(c));
 }
@@ -911,6 +931,7 @@ int create_batch(const adg_geom *geoms, int ncore, adg_ctx **out) {
     if (const char *ev = getenv("ADG_GRAPH")) c->graphs = atoi(ev) != 0;
     if (const char *ev = getenv("ADG_CHECK")) c->check_every = std::max(1, atoi(ev));
     if (const char *ev = getenv("ADG_NOD_COOP")) c->nod_coop = atoi(ev) != 0;
+    if (const char *ev = getenv("ADG_SRC_FULL")) c->src_full = atoi(ev) != 0;
     c->core.resize(ncore);
     std::vector<std::vector<unsigned char>> fls(ncore);
     std::vector<std::vector<double>> vds(ncore);
