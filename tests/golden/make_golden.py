@@ -290,11 +290,15 @@ def transients():
     src = os.path.join(os.path.dirname(REF), "transient", "LMW")
     text = open(src).read()
     assert "1200 2 1.e-5 1.e-5 13 60 20 60" in text
-    a1t = open(os.path.join(os.path.dirname(REF), "transient", "NEACRP", "A1t")).read()
-    assert "1200 3 1.e-5 1.e-5 15 20 20 80" in a1t
+    nea = os.path.join(os.path.dirname(REF), "transient", "NEACRP")
+    a1t, b1t, c1t = (open(os.path.join(nea, n)).read() for n in ("A1t", "B1t", "C1t"))
+    assert "1200 3 1.e-5 1.e-5 15 20 20 80" in a1t and "1200 3 1.e-5 1.e-5 15 20 20 80" in b1t
+    assert "1200 3 1.e-5 1.e-5 20 40 20 80" in c1t
     decks = {"LMW": text, "LMW_tight": text.replace("1200 2 1.e-5 1.e-5 13 60 20 60", LMW_TIGHT_ITER),
              # NEACRP rod ejection at hot zero power with T-H feedback (rod_eject_th)
-             "A1t": a1t, "A1t_tight": a1t.replace("1200 3 1.e-5 1.e-5 15 20 20 80", "3000 3 1.e-8 1.e-8 15 20 20 80")}
+             "A1t": a1t, "A1t_tight": a1t.replace("1200 3 1.e-5 1.e-5 15 20 20 80", "3000 3 1.e-8 1.e-8 15 20 20 80"),
+             "B1t_tight": b1t.replace("1200 3 1.e-5 1.e-5 15 20 20 80", "3000 3 1.e-8 1.e-8 15 20 20 80"),
+             "C1t_tight": c1t.replace("1200 3 1.e-5 1.e-5 20 40 20 80", "3000 3 1.e-8 1.e-8 20 40 20 80")}
     for name, body in decks.items():
         with open(os.path.join(HERE, "decks", name), "w") as fh:
             fh.write(body)

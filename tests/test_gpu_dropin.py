@@ -172,7 +172,7 @@ def test_dropin_output_files_match_reference(deck, tmp_path):
     assert not bad, bad[:10]
 
 
-@pytest.mark.parametrize("deck", ["LMW_tight", "LMW", "A1t_tight", "A1t"])
+@pytest.mark.parametrize("deck", ["LMW_tight", "LMW", "A1t_tight", "A1t", "B1t_tight", "C1t_tight"])
 def test_dropin_transient(deck, tmp_path):
     """%MODE RODEJECT (the reference's LMW rod-movement transient, 240
     Crank-Nicolson time steps): the host's rod_eject / trans_calc
@@ -181,9 +181,9 @@ def test_dropin_transient(deck, tmp_path):
     transient (cmode 2) nodal update on the device; the host's reactivity
     reads Lxyz of the device state.
 
-    A1t is the NEACRP rod ejection at hot zero power with T-H feedback
-    (rod_eject_th, src/mod_trans.f90:172-330: 280 steps, the power rising
-    from 1e-6 to 0.2 through a Doppler-limited excursion).
+    A1t, B1t and C1t are NEACRP rod ejections at hot zero power with T-H
+    feedback (rod_eject_th, src/mod_trans.f90:172-330; A1t: 280 steps, the
+    power rising from 1e-6 to 0.2 through a Doppler-limited excursion).
 
     The *_tight decks have their %ITER tolerances at 1e-8: the printed
     table follows the reference's to one unit of the last printed digit of
