@@ -302,6 +302,17 @@ __device__ __forceinline__ void kb_load(KBIn<TV> &L, const TV *__restrict__ AT, 
   L.rsc = rs[c];
 }
 
+// stores of the stencil kernels' vector outputs; with ADG_NT_STORE they are
+// streaming (non-temporal) stores
+template <typename TV>
+__device__ __forceinline__ void vstore(TV *p, TV v) {
+#if ADG_NT_STORE
+  __builtin_nontemporal_store(v, p);
+#else
+  *p = v;
+#endif
+}
+
 // one cell of KA from its loaded operands
 template <typename TV, bool FIRST>
 __device__ __forceinline__ void ka_cell(const KAIn<TV, FIRST> &L, long c, TV ta, TV tw, TV tb, TV *__restrict__ x,
@@ -328,11 +339,11 @@ __device__ __forceinline__ void ka_cell(const KAIn<TV, FIRST> &L, long c, TV ta,
   const TV vc = (TV)s;
   if (!FIRST) {
     const TV sv = (TV)(L.R[3] - ta * L.V[3]);
-    x[c] = (TV)(L.xc + ta * L.P[3] + tw * sv);
+    vstore<TV>(x + c, (TV)(L.xc + ta * L.P[3] + tw * sv));
   }
-  rout[c] = rn;
-  pout[c] = pn[3];
-  vout[c] = vc;
+  vstore<TV>(rout + c, rn);
+  vstore<TV>(pout + c, pn[3]);
+  vstore<TV>(vout + c, vc);
   acc += (double)L.rsc * (double)vc;
 }
 
@@ -348,7 +359,7 @@ __device__ __forceinline__ void kb_cell(const KBIn<TV> &L, long c, TV alpha, TV 
 #pragma unroll
   for (int k = 0; k < 7; ++k) s = s + (double)(L.Ac[k] * sv[k]);
   const TV tc = (TV)s;
-  t[c] = tc;
+  vstore<TV>(t + c, tc);
   acc[0] += (double)tc * (double)tc;
   acc[1] += (double)tc * (double)sv[3];
   acc[2] += rsc * (double)tc;

@@ -17,6 +17,9 @@ namespace adg {
 // Launch shape of the flat stencil kernels (threads per block, largest grid)
 // and how a consumer finishes a reduction (per wavefront without barriers,
 // or per block); the defaults are the measured best on gfx950.
+#ifndef ADG_NT_STORE
+#define ADG_NT_STORE 0   // streaming stores for the BiCGSTAB kernels' vector outputs
+#endif
 #ifndef ADG_KUNROLL
 #define ADG_KUNROLL 1   // cells per thread in flight in the BiCGSTAB stencil kernels (1 or 2)
 #endif
