@@ -110,3 +110,33 @@ def test_gpu_batch256_matches_reference(precision):
     for i in sample:
         if solved[i]:
             assert rel_err(powers[i], d[f"pow_{i}"].astype(np.float64)) < 1e-4, i
+
+
+def test_gpu_contexts_on_concurrent_threads_match_sequential():
+    """Independent contexts driven from several host threads at once (each on
+    its own stream, graph captures serialized inside the library) give the
+    same results as the same solves run one after another."""
+    import threading
+    cores = [batch_core(SEED, i).problem for i in range(6)]
+    seq = []
+    for q in cores:
+        e = Engine(q)
+        e.outer()
+        seq.append((e.Ke, e.niter, e.powdis()))
+        e.close()
+    res = [None] * len(cores)
+
+    def run(i):
+        e = Engine(cores[i])
+        e.outer()
+        res[i] = (e.Ke, e.niter, e.powdis())
+        e.close()
+
+    th = [threading.Thread(target=run, args=(i,)) for i in range(len(cores))]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    for i, (k, n, pw) in enumerate(seq):
+        assert res[i] is not None, i
+        assert res[i][0] == k and res[i][1] == n and np.array_equal(res[i][2], pw), i
