@@ -352,6 +352,41 @@ def read_ref_dump(path: str, action: str = "solve"):
         res["fs0"] = r.take("<f8", N)
         res["dn"] = r.take("<f8", 6 * G * N).reshape(6, G, N)
         res["ndmax"] = float(r.take("<f8", 1)[0])
+    elif action == "xsupd":
+        flags = [int(v) for v in r.take("<i4", 6)]
+        # the feedback tables in the form oracle.xs_update and
+        # Engine.set_xs_tables take
+        fl = dict(zip(("boron", "ftem", "mtem", "cden", "rod"), flags[:5]))
+        nb = flags[5]
+        M = int(nmat)
+        tab = {"nmat": M, "nb": nb}
+        tab.update(zip(("rbcon", "rftem", "rmtem", "rcden", "coreh", "pos0", "ssize"),
+                       (float(v) for v in r.take("<f8", 7))))
+
+        def tables():
+            t = [r.take("<f8", M * G).reshape(G, M) for _ in range(4)]
+            t.append(r.take("<f8", M * G * G).reshape(G, G, M))
+            return t
+        tab["base"] = tables()
+        for key in ("boron", "ftem", "mtem", "cden", "rod"):
+            tab[key] = tables() if fl[key] else None
+        tab["fbmap"] = r.take("<i4", int(nxx) * int(nyy)).reshape(int(nyy), int(nxx)) if fl["rod"] else None
+        res["tables"] = tab
+        states = []
+        for _ in range(2):
+            st = {"bcon": float(r.take("<f8", 1)[0])}
+            for key in ("ftem", "mtem", "cden"):
+                if fl[key]:
+                    st[key] = r.take("<f8", N)
+            if fl["rod"]:
+                st["bpos"] = r.take("<f8", nb)
+            for key in ("sigtr", "siga", "nuf", "sigf"):
+                st[key] = r.take("<f8", G * N).reshape(G, N)
+            st["sigs"] = r.take("<f8", G * G * N).reshape(G, G, N)
+            st["D"] = r.take("<f8", G * N).reshape(G, N)
+            st["sigr"] = r.take("<f8", G * N).reshape(G, N)
+            states.append(st)
+        res["states"] = states
     elif action == "trstep":
         res["Ke_s"] = float(r.take("<f8", 1)[0])
         res["f0_s"] = r.take("<f8", G * N).reshape(G, N)
@@ -416,3 +451,27 @@ def problem_from_dict(d, prefix: str = "p_") -> Problem:
     for k in ("ng", "nmat", "nxx", "nyy", "nzz", "nout", "nin", "nac", "nupd"):
         kw[k] = int(kw[k])
     return Problem(**kw)
+
+
+XS_KINDS = ("base", "boron", "ftem", "mtem", "cden", "rod")
+XS_TABLE_NAMES = ("sigtr", "siga", "nuf", "sigf", "sigs")
+XS_REFS = ("rbcon", "rftem", "rmtem", "rcden", "coreh", "pos0", "ssize")
+
+
+def xs_tables_from_dict(d, prefix: str = "t_"):
+    """The feedback tables of xsec::XS_updt (src/mod_xsec.f90:11-46) from a
+    fixture's arrays: {"nmat", "nb", the reference values of XS_REFS, per
+    kind of XS_KINDS the five tables of XS_TABLE_NAMES (sigtr..sigf as
+    (G, nmat), sigs as (G_to, G_from, nmat); None where the feedback is
+    off) and "fbmap" (nyy, nxx) or None}."""
+    files = set(getattr(d, "files", d.keys()))
+    t = {"nmat": int(np.asarray(d[prefix + "nmat"])), "nb": int(np.asarray(d[prefix + "nb"]))}
+    for k in XS_REFS:
+        t[k] = float(np.asarray(d[prefix + k]))
+    for kind in XS_KINDS:
+        key = f"{prefix}{kind}_sigtr"
+        t[kind] = [np.asarray(d[f"{prefix}{kind}_{x}"]) for x in XS_TABLE_NAMES] \
+            if key in files and np.asarray(d[key]).ndim == 2 else None
+    fb = np.asarray(d[prefix + "fbmap"]) if prefix + "fbmap" in files else None
+    t["fbmap"] = fb if fb is not None and fb.ndim == 2 else None
+    return t

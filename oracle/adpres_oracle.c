@@ -1110,3 +1110,112 @@ void orc_sanm_coef(double sigr, double D, double h, double *out6) {
   out6[4] = (alp * sh - 3.0 * m2c) / (ch - m0c - m2c);
   out6[5] = (alp * ch - m1s) / (sh - m1s);
 }
+
+/* ---- cross-section update: xsec::XS_updt (src/mod_xsec.f90:11-46) ----- */
+
+int orc_xs_update(const orc_problem *pb, const orc_xs_tables *t, double bcon, const double *ftem,
+                  const double *mtem, const double *cden, const double *bpos, double *sigtr,
+                  double *siga, double *nuf, double *sigf, double *sigs, double *D, double *sigr) {
+  const int N = pb->nnod, G = pb->ng, M = t->nmat;
+  double *X[5] = {sigtr, siga, nuf, sigf, sigs};
+  for (int k = 0; k < 5; ++k)
+    if (!t->tab[0][k]) return ORC_E_ARG;
+  for (int n = 0; n < N; ++n)
+    if (pb->mat[n] < 1 || pb->mat[n] > M) return ORC_E_ARG;
+  if ((t->tab[2][0] && !ftem) || (t->tab[3][0] && !mtem) || (t->tab[4][0] && !cden) ||
+      (t->tab[5][0] && (!bpos || !t->fbmap)))
+    return ORC_E_ARG;
+
+  /* base_updt (:172-196) */
+  for (int n = 0; n < N; ++n) {
+    const int m = pb->mat[n] - 1;
+    for (int k = 0; k < 4; ++k)
+      for (int g = 0; g < G; ++g) X[k][n + (long)N * g] = t->tab[0][k][m + (long)M * g];
+    for (int h = 0; h < G; ++h)
+      for (int g = 0; g < G; ++g) sigs[n + (long)N * (g + G * h)] = t->tab[0][4][m + (long)M * (g + G * h)];
+  }
+  /* bcon_updt, ftem_updt, mtem_updt, cden_updt (:336-436): one pass each,
+     X(n,g) = X(n,g) + table(mat(n),g) * (parameter - reference value) */
+  for (int kind = 1; kind <= 4; ++kind) {
+    if (!t->tab[kind][0]) continue;
+    for (int n = 0; n < N; ++n) {
+      const int m = pb->mat[n] - 1;
+      double dv;
+      if (kind == 1) dv = bcon - t->rbcon;
+      else if (kind == 2) dv = sqrt(ftem[n]) - sqrt(t->rftem);
+      else if (kind == 3) dv = mtem[n] - t->rmtem;
+      else dv = cden[n] - t->rcden;
+      for (int g = 0; g < G; ++g) {
+        for (int k = 0; k < 4; ++k)
+          X[k][n + (long)N * g] = X[k][n + (long)N * g] + t->tab[kind][k][m + (long)M * g] * dv;
+        for (int h = 0; h < G; ++h)
+          sigs[n + (long)N * (g + G * h)] =
+              sigs[n + (long)N * (g + G * h)] + t->tab[kind][4][m + (long)M * (g + G * h)] * dv;
+      }
+    }
+  }
+  /* crod_updt (:230-287): from the top of each rodded column down, whole
+     nodes get the rod increment until the node holding the rod tip, which
+     gets it weighted by its rodded fraction; then negative siga, nuf, sigf
+     and sigs of the column are set to zero */
+  if (t->tab[5][0]) {
+    const int nx = pb->nxx, ny = pb->nyy, nz = pb->nzz;
+    int *xyz = (int *)calloc((size_t)nx * ny * nz, sizeof(int));
+    if (!xyz) return ORC_E_ARG;
+    for (int n = 0; n < N; ++n)
+      xyz[(pb->ix[n] - 1) + (long)nx * ((pb->iy[n] - 1) + (long)ny * (pb->iz[n] - 1))] = n + 1;
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i) {
+        const int b = t->fbmap[i + nx * j];
+        if (b <= 0) continue;
+        if (b > t->nb) { free(xyz); return ORC_E_ARG; }
+        for (int k = 0; k < nz; ++k)
+          if (!xyz[i + (long)nx * (j + (long)ny * k)]) { free(xyz); return ORC_E_ARG; }
+        const double rodh = t->coreh - t->pos0 - bpos[b - 1] * t->ssize;
+        double dum = 0.0;
+        for (int k = nz - 1; k >= 0; --k) {
+          const int n = xyz[i + (long)nx * (j + (long)ny * k)] - 1, m = pb->mat[n] - 1;
+          const int part = rodh >= dum && rodh <= dum + pb->zdel[k];
+          const double vf = part ? (rodh - dum) / pb->zdel[k] : 0.0;
+          for (int g = 0; g < G; ++g) {
+            for (int kk = 0; kk < 4; ++kk) {
+              const double d = t->tab[5][kk][m + (long)M * g];
+              X[kk][n + (long)N * g] = X[kk][n + (long)N * g] + (part ? vf * d : d);
+            }
+            for (int h = 0; h < G; ++h) {
+              const double d = t->tab[5][4][m + (long)M * (g + G * h)];
+              sigs[n + (long)N * (g + G * h)] = sigs[n + (long)N * (g + G * h)] + (part ? vf * d : d);
+            }
+          }
+          if (part) break;
+          dum = dum + pb->zdel[k];
+        }
+        for (int k = nz - 1; k >= 0; --k) {
+          const int n = xyz[i + (long)nx * (j + (long)ny * k)] - 1;
+          for (int g = 0; g < G; ++g) {
+            for (int kk = 1; kk < 4; ++kk)
+              if (X[kk][n + (long)N * g] < 0.0) X[kk][n + (long)N * g] = 0.0;
+            for (int h = 0; h < G; ++h)
+              if (sigs[n + (long)N * (g + G * h)] < 0.0) sigs[n + (long)N * (g + G * h)] = 0.0;
+          }
+        }
+      }
+    free(xyz);
+  }
+  /* Dsigr_updt (:199-226) and check_xs (:86-150) */
+  for (int n = 0; n < N; ++n)
+    for (int g = 0; g < G; ++g) {
+      /* the thresholds are default-real (single precision) literals there */
+      if (sigtr[n + (long)N * g] < (double)1.e-5f) return ORC_E_XS;
+      D[n + (long)N * g] = 1.0 / (3.0 * sigtr[n + (long)N * g]);
+      double dum = 0.0;
+      for (int h = 0; h < G; ++h)
+        if (g != h) dum = dum + sigs[n + (long)N * (g + G * h)];
+      sigr[n + (long)N * g] = siga[n + (long)N * g] + dum;
+    }
+  for (long i = 0; i < (long)N * G; ++i)
+    if (D[i] < (double)1.e-20f || sigr[i] < 0.0 || nuf[i] < 0.0) return ORC_E_XS;
+  for (long i = 0; i < (long)N * G * G; ++i)
+    if (sigs[i] < 0.0) return ORC_E_XS;
+  return ORC_OK;
+}

@@ -26,6 +26,7 @@ enum {
   ORC_E_LU_PIVOT = 3,       /* src/mod_nodal.f90:856-862 */
   ORC_E_POWER_NONPOS = 4,   /* src/mod_cmfd.f90:1322-1326 */
   ORC_E_ARG = 5,            /* bad argument / unsupported geometry */
+  ORC_E_XS = 6,             /* src/mod_xsec.f90:215,86-150: the reference STOPs */
 };
 
 typedef struct {
@@ -76,6 +77,29 @@ typedef struct {
    the D-hats stay those of the forward state). */
 int orc_outer(const orc_problem *pb, orc_state *st, const orc_iter *it,
               int mode, orc_trace *tr);
+
+/* The feedback tables XS_updt reads (sdata, src/mod_data.f90; filled by
+   src/mod_io.f90's %XSEC, %CBCS/%BCON, %FTEM, %MTEM, %CDEN and %CROD
+   readers).  tab[kind][xs]: kind 0 base x*, 1 boron c*, 2 fuel temperature
+   f*, 3 moderator temperature m*, 4 coolant density l*, 5 control rod d*;
+   xs 0 sigtr, 1 siga, 2 nuf, 3 sigf as (nmat,G), 4 sigs as (nmat,G,G)
+   (from, to).  tab[kind][0] == NULL: that feedback is off (its b* flag 0).
+   fbmap (nxx,nyy): the rod bank of each node column, 0 none. */
+typedef struct {
+  int nmat, nb;
+  double rbcon, rftem, rmtem, rcden, coreh, pos0, ssize;
+  const double *tab[6][5];
+  const int *fbmap;
+} orc_xs_tables;
+
+/* xsec::XS_updt (src/mod_xsec.f90:11-46): base_updt, bcon/ftem/mtem/cden
+   updates, crod_updt, Dsigr_updt and check_xs, at boron bcon, node fuel
+   and moderator temperatures and coolant densities (N) and bank positions
+   bpos (nb, steps).  Arrays the tables leave off may be NULL.  Outputs are
+   (N,G) and sigs (N,G,G).  ORC_E_XS where the reference would STOP. */
+int orc_xs_update(const orc_problem *pb, const orc_xs_tables *t, double bcon, const double *ftem,
+                  const double *mtem, const double *cden, const double *bpos, double *sigtr,
+                  double *siga, double *nuf, double *sigf, double *sigs, double *D, double *sigr);
 
 /* Lxyz (src/mod_nodal.f90:901-1005) of the state: L(N,G,3). */
 int orc_leakage(const orc_problem *pb, orc_state *st, double *L);

@@ -32,6 +32,12 @@ class OrcProblem(C.Structure):
                 ("chi", _dp), ("dc", _dp), ("exsrc", _dp), ("wtr", _dp)]
 
 
+class OrcXsTables(C.Structure):
+    _fields_ = [("nmat", C.c_int), ("nb", C.c_int)] + \
+        [(k, C.c_double) for k in ("rbcon", "rftem", "rmtem", "rcden", "coreh", "pos0", "ssize")] + \
+        [("tab", (_dp * 5) * 6), ("fbmap", _ip)]
+
+
 class OrcIter(C.Structure):
     _fields_ = [("nout", C.c_int), ("nin", C.c_int), ("nac", C.c_int), ("nupd", C.c_int),
                 ("kern", C.c_int), ("serc", C.c_double), ("ferc", C.c_double)]
@@ -71,6 +77,8 @@ def lib():
         L.orc_coup_coef.argtypes = [C.POINTER(OrcProblem), C.POINTER(OrcState)]
         L.orc_sanm_coef.argtypes = [C.c_double, C.c_double, C.c_double, _dp]
         L.orc_sanm_coef.restype = None
+        L.orc_xs_update.argtypes = [C.POINTER(OrcProblem), C.POINTER(OrcXsTables), C.c_double,
+                                    _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
         _lib = L
     return _lib
 
@@ -195,3 +203,43 @@ class Oracle:
         out = np.zeros(self.p.nnod)
         rc = lib().orc_powdis(C.byref(self.pb), _d(f0), _d(out))
         return rc, out
+
+
+XS_KINDS = ("base", "boron", "ftem", "mtem", "cden", "rod")
+
+
+def xs_update(p, tables, bcon, ftem=None, mtem=None, cden=None, bpos=None):
+    """xsec::XS_updt (src/mod_xsec.f90:11-46) of Problem p's nodes with the
+    feedback tables (a dict: "nmat", "nb", the reference values "rbcon",
+    "rftem", "rmtem", "rcden", "coreh", "pos0", "ssize", per present kind of
+    XS_KINDS a list of the five tables sigtr, siga, nuf, sigf as (G, nmat)
+    and sigs as (G_to, G_from, nmat), and "fbmap" (nyy, nxx) with the rod).
+    Returns (status, dict of sigtr, siga, nuf, sigf, D, sigr as (G, N) and
+    sigs as (G_to, G_from, N))."""
+    N, G = p.nnod, p.ng
+    keep = []
+
+    def dp(a):
+        if a is None:
+            return None
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        keep.append(a)
+        return _d(a)
+    t = OrcXsTables()
+    t.nmat, t.nb = int(tables["nmat"]), int(tables.get("nb", 0))
+    for k in ("rbcon", "rftem", "rmtem", "rcden", "coreh", "pos0", "ssize"):
+        setattr(t, k, float(tables.get(k, 0.0)))
+    for i, kind in enumerate(XS_KINDS):
+        if tables.get(kind) is not None:
+            for j in range(5):
+                t.tab[i][j] = dp(tables[kind][j])
+    if tables.get("fbmap") is not None:
+        fb = np.ascontiguousarray(tables["fbmap"], dtype=np.int32)
+        keep.append(fb)
+        t.fbmap = _i(fb)
+    pb = Oracle(p).pb
+    out = {k: np.zeros((G, N)) for k in ("sigtr", "siga", "nuf", "sigf", "D", "sigr")}
+    out["sigs"] = np.zeros((G, G, N))
+    rc = lib().orc_xs_update(C.byref(pb), C.byref(t), float(bcon), dp(ftem), dp(mtem), dp(cden), dp(bpos),
+                             *[_d(out[k]) for k in ("sigtr", "siga", "nuf", "sigf", "sigs", "D", "sigr")])
+    return rc, out

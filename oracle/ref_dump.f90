@@ -10,6 +10,9 @@
 !   ref_dump DECK bicg  OUT XSFILE|- STATEFILE  one nin-iteration BiCGSTAB per group
 !   ref_dump DECK trstep OUT XSFILE|-           steady state, then one transient
 !                                               step (outer_tr) of a %EJCT deck
+!   ref_dump DECK xsupd OUT -                    the cross-section tables and
+!                                               XS_updt at the deck's and at
+!                                               perturbed T-H / rod parameters
 !
 ! The deck is read by the reference's own parser (io::inp_read, mod_io.f90:69),
 ! cross sections are built by xsec::XS_updt (mod_xsec.f90:11).  An optional
@@ -25,7 +28,7 @@
 program ref_dump
 
   use sdata
-  use io,    only: inp_read
+  use io,    only: inp_read, bbcon, bcbcs, bftem, bmtem, bcden, bcrod
   use xsec,  only: XS_updt, Dsigr_updt
   use cmfd,  only: outer, outer_ad, outer_fs, outer_tr, PowDis, matrix_setup, bicg
   use nodal, only: nodal_update, nodal_update_pnm
@@ -162,6 +165,30 @@ program ref_dump
     end do
     write(u) kes, f0s, fs0s, Ke, f0, fs0, dnout, ndmax
 
+  case ('xsupd')
+    j = 0
+    if (bbcon == 1 .or. bcbcs == 1) j = 1
+    write(u) int(j, 4), int(bftem, 4), int(bmtem, 4), int(bcden, 4), int(bcrod, 4), int(nb, 4)
+    write(u) rbcon, rftem, rmtem, rcden, coreh, pos0, ssize
+    write(u) xsigtr, xsiga, xnuf, xsigf, xsigs
+    if (j == 1) write(u) csigtr, csiga, cnuf, csigf, csigs
+    if (bftem == 1) write(u) fsigtr, fsiga, fnuf, fsigf, fsigs
+    if (bmtem == 1) write(u) msigtr, msiga, mnuf, msigf, msigs
+    if (bcden == 1) write(u) lsigtr, lsiga, lnuf, lsigf, lsigs
+    if (bcrod == 1) write(u) dsigtr, dsiga, dnuf, dsigf, dsigs, (int(fbmap(:, n), 4), n = 1, nyy)
+    call write_xs_state()
+    ! perturbed parameters: boron, node-wise temperatures and density, and
+    ! every bank moved to a partial insertion
+    bcon = bcon + 137._dp
+    do n = 1, nnod
+      if (bftem == 1) ftem(n) = ftem(n) + 40._dp * real(mod(n, 7) - 3, dp)
+      if (bmtem == 1) mtem(n) = mtem(n) + 2._dp * real(mod(n, 5) - 2, dp)
+      if (bcden == 1) cden(n) = cden(n) * (1._dp + 0.01_dp * real(mod(n, 3) - 1, dp))
+    end do
+    if (bcrod == 1) bpos = bpos * 0.5_dp + 17.25_dp
+    call XS_updt(bcon, ftem, mtem, cden, bpos)
+    call write_xs_state()
+
   case ('trstep')
     ! the steady state, then the set-up rod_eject and trans_calc make for a
     ! time step (src/mod_trans.f90:50-100, 400-416), with a rod-like
@@ -221,6 +248,16 @@ program ref_dump
   close(u)
 
 contains
+
+  ! the parameters of an XS_updt call and the node cross sections it made
+  subroutine write_xs_state()
+    write(u) bcon
+    if (bftem == 1) write(u) ftem
+    if (bmtem == 1) write(u) mtem
+    if (bcden == 1) write(u) cden
+    if (bcrod == 1) write(u) bpos
+    write(u) sigtr, siga, nuf, sigf, sigs, D, sigr
+  end subroutine write_xs_state
 
   subroutine read_state()
     ! STATEFILE: int32 nin, float64 Ke, f0(N,G), dn(N,G,6), b(N,G)

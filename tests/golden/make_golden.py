@@ -349,6 +349,36 @@ def fwdadj():
          ndmax=res["ndmax"])
 
 
+def xsupd():
+    """xsec::XS_updt on the NEACRP A1 deck (every feedback table: boron,
+    fuel and moderator temperature, coolant density, control rods;
+    ref_dump xsupd): the tables, and the node cross sections at the deck's
+    parameters and at perturbed node-wise parameters with every bank partly
+    inserted.  The first state's outputs are the problem's own arrays."""
+    src = os.path.join(HERE, "decks", "neacrp")
+    with tempfile.TemporaryDirectory() as d:
+        for fn in NEACRP_FILES:
+            shutil.copy(os.path.join(src, fn), os.path.join(d, fn))
+        shutil.copy(os.path.join(src, "A1"), os.path.join(d, "deck"))
+        p, res, out = run(d, "deck", "xsupd")
+    p.name = "neacrp_a1_xsupd"
+    t = res["tables"]
+    arrs = {"t_nmat": t["nmat"], "t_nb": t["nb"], "t_fbmap": t["fbmap"]}
+    for k in ("rbcon", "rftem", "rmtem", "rcden", "coreh", "pos0", "ssize"):
+        arrs["t_" + k] = t[k]
+    for kind in ("base", "boron", "ftem", "mtem", "cden", "rod"):
+        for j, x in enumerate(("sigtr", "siga", "nuf", "sigf", "sigs")):
+            arrs[f"t_{kind}_{x}"] = t[kind][j]
+    s0, s1 = res["states"]
+    for i, st in enumerate((s0, s1)):
+        for k in ("bcon", "ftem", "mtem", "cden", "bpos"):
+            arrs[f"s{i}_{k}"] = st[k]
+    for k in ("sigtr", "siga", "nuf", "sigf", "sigs", "D", "sigr"):
+        assert np.array_equal(s0[k], getattr(p, k)), k
+        arrs["s1_" + k] = s1[k]
+    save("neacrp_a1_xsupd", **problem_to_dict(p), **arrs)
+
+
 def adjoint_fdm(seed=1):
     """The 34x34x26-node 2-group core with the finite-difference kernel,
     solved by the reference in %MODE FORWARD and %MODE ADJOINT (outer_ad)
@@ -392,6 +422,8 @@ def main(names):
         transients()
         trstep()
         fwdadj()
+    if not names or "xsupd" in names:
+        xsupd()
     if "batch" in names:          # 256 reference solves on 7 host cores
         batch()
     if "core3d_8g" in names:      # about an hour of reference CPU time
