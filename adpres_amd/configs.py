@@ -273,6 +273,39 @@ def batch_cores(seed: int = 3, n: int = 256):
 CORES = {"core2d_small": core2d_small, "core3d_2g": core3d_2g, "core3d_8g": core3d_8g}
 
 
+def feedback_tables(gc: GeneratedCore, seed: int = 7, nb: int = 4):
+    """Feedback tables in the form of the reference's XS_updt
+    (src/mod_xsec.f90:11-46) for a generated core: its material cross
+    sections as the base tables; boron, fuel-temperature, moderator-
+    temperature and coolant-density coefficients that move each entry by up
+    to 1e-5 per ppm, 1e-3 per sqrt(K), 1e-3 per K and 5e-2 per g/cc of its
+    base value; nb control-rod banks over random full-height columns, each
+    adding 2 % to sigtr and 10 % to siga.  Returns (tables, parameters)
+    where the parameters (bcon, ftem, mtem, cden, bpos) are a typical state
+    about the reference values."""
+    p, d = gc.problem, gc.deck
+    rng = np.random.default_rng(seed)
+    base = [np.ascontiguousarray(np.asarray(d[k], dtype=np.float64).T) for k in ("xsigtr", "xsiga", "xnuf", "xsigf")]
+    base.append(np.ascontiguousarray(np.transpose(np.asarray(d["xsigs"], dtype=np.float64), (2, 1, 0))))
+    coreh = float(np.sum(p.zdel))
+    t = dict(nmat=p.nmat, nb=nb, rbcon=1000.0, rftem=900.0, rmtem=560.0, rcden=0.75, coreh=coreh, pos0=0.0,
+             ssize=coreh / 200.0, base=base)
+    for kind, scale in (("boron", 1e-5), ("ftem", 1e-3), ("mtem", 1e-3), ("cden", 5e-2)):
+        t[kind] = [b * rng.uniform(-scale, scale, b.shape) for b in base]
+    t["rod"] = [0.02 * base[0], 0.1 * base[1]] + [np.zeros_like(b) for b in base[2:]]
+    cols = np.zeros((p.nyy, p.nxx), np.int64)
+    np.add.at(cols, (p.iy - 1, p.ix - 1), 1)
+    full = np.argwhere(cols == p.nzz)
+    fb = np.zeros((p.nyy, p.nxx), np.int32)
+    pick = full[rng.permutation(len(full))[:max(nb, len(full) // 8)]]
+    fb[pick[:, 0], pick[:, 1]] = 1 + np.arange(len(pick)) % nb
+    t["fbmap"] = fb
+    N = p.nnod
+    par = (1200.0, 900.0 * rng.uniform(0.8, 1.4, N), 560.0 + rng.uniform(-5.0, 15.0, N),
+           0.75 * rng.uniform(0.97, 1.02, N), rng.uniform(0.0, 200.0, nb))
+    return t, par
+
+
 def generate(name: str, seed: Optional[int] = None, **kw) -> GeneratedCore:
     fn = CORES[name]
     return fn(seed, **kw) if seed is not None else fn(**kw)

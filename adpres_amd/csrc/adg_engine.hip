@@ -27,6 +27,7 @@
 #include "adg_cmfd.h"
 #include "adg_common.h"
 #include "adg_nodal.h"
+#include "adg_xs.h"
 
 using namespace adg;
 
@@ -79,6 +80,11 @@ struct CoreHost {
   double Ke_host = 1.0;
   int pfirst = 0, pn = 0, ffirst = 0, fnb = 0;
   int status = 0, niter = 0;
+  // feedback tables of adg_set_xs_tables (device)
+  double *xtab = nullptr;          // [6 kinds][4 G M + G G M], then zdum(nzz), zdel(nzz)
+  int *xnode = nullptr;            // (2N) material - 1, then bank | (level << 16) (0: no rod)
+  int xon = 0, xnb = 0, xnmat = 0;
+  double xref[7] = {0, 0, 0, 0, 0, 0, 0};   // rbcon rftem rmtem rcden coreh pos0 ssize
 };
 
 }  // namespace
@@ -101,6 +107,8 @@ struct adg_ctx {
   double *wtr = nullptr;   // transient fission weight per cell (adg_set_transient)
   long *d_cell = nullptr;  // node -> cell of every core (offset coff per core)
   double *dstage = nullptr;   // node-ordered staging of host arrays
+  int *xerr = nullptr;        // adg_update_xs's stop flag
+  hipEvent_t xev[2] = {nullptr, nullptr};   // bracket adg_update_xs's kernel
   size_t stage_cap = 0;
   double *f0 = nullptr, *f0c = nullptr, *fs0 = nullptr, *fs0c = nullptr, *errn = nullptr;
   double *dn = nullptr, *df = nullptr, *A64 = nullptr, *b = nullptr, *pw = nullptr, *tmp = nullptr;
@@ -1027,6 +1035,7 @@ const char *adg_status_string(int status) {
     case ADG_E_ARG: return "invalid argument";
     case ADG_E_HIP: return "HIP runtime error";
     case ADG_E_NOT_SET: return "cross sections not set";
+    case ADG_E_XS: return "cross section out of range (the reference stops in XS_updt)";
   }
   return "unknown status";
 }
@@ -1057,6 +1066,13 @@ void adg_destroy(adg_ctx *c) {
   void *tptr[] = {c->tk, c->tser, c->tfer, c->tnd, c->tflags, c->tloc};
   for (void *p : tptr)
     if (p) (void)hipFree(p);
+  for (CoreHost &h : c->core) {
+    if (h.xtab) (void)hipFree(h.xtab);
+    if (h.xnode) (void)hipFree(h.xnode);
+  }
+  if (c->xerr) (void)hipFree(c->xerr);
+  for (hipEvent_t e : c->xev)
+    if (e) (void)hipEventDestroy(e);
   if (c->s) (void)hipStreamDestroy(c->s);
   delete c;
 }
@@ -1101,6 +1117,192 @@ int adg_set_xs_core(adg_ctx *c, int core, const double *D, const double *sigr, c
     return hip_fail(c, e);
   }
   return ADG_OK;
+}
+
+int adg_set_xs_tables_core(adg_ctx *c, int core, const adg_xs_tables *t, const int *mat, const double *chi,
+                           const double *dc) {
+  if (!core_ok(c, core)) return ADG_E_ARG;
+  if (!t || !mat || !chi || !dc) return fail(c, ADG_E_ARG, "null argument");
+  CoreHost &h = c->core[core];
+  const int N = h.N, G = c->G, M = t->nmat;
+  if (M != h.nmat) return fail(c, ADG_E_ARG, "table material count differs from the geometry's");
+  for (int q = 0; q < 5; ++q)
+    if (!t->tab[ADG_XS_BASE][q]) return fail(c, ADG_E_ARG, "base table missing");
+  int on = 0;
+  for (int kd = 1; kd < 6; ++kd) {
+    if (!t->tab[kd][0]) continue;
+    for (int q = 0; q < 5; ++q)
+      if (!t->tab[kd][q]) return fail(c, ADG_E_ARG, "incomplete feedback table");
+    on |= 1 << kd;
+  }
+  for (int n = 0; n < N; ++n)
+    if (mat[n] < 1 || mat[n] > M) return fail(c, ADG_E_ARG, "material index out of range");
+  const bool rod = on & (1 << XK_ROD);
+  if (rod && (!t->fbmap || t->nb < 1 || t->nb > 0xffff || h.nzz > 0x7fff))
+    return fail(c, ADG_E_ARG, "rod tables need fbmap and 1..65535 banks");
+  try {
+    const long K = (4L * G + (long)G * G) * M;
+    std::vector<double> tab(6 * K + 2L * h.nzz, 0.0);
+    for (int kd = 0; kd < 6; ++kd) {
+      if (kd && !(on & (1 << kd))) continue;
+      for (int q = 0; q < 4; ++q)
+        std::memcpy(&tab[kd * K + (long)q * G * M], t->tab[kd][q], sizeof(double) * G * M);
+      std::memcpy(&tab[kd * K + 4L * G * M], t->tab[kd][4], sizeof(double) * G * G * M);
+    }
+    // crod_updt's running depth from the core top, summed in its order
+    double dum = 0.0;
+    for (int k = h.nzz - 1; k >= 0; --k) {
+      tab[6 * K + k] = dum;
+      tab[6 * K + h.nzz + k] = h.hz[k];
+      dum = dum + h.hz[k];
+    }
+    std::vector<int> node(2L * N, 0);
+    std::vector<int> colcount;
+    if (rod) colcount.assign((size_t)h.nxx * h.nyy, 0);
+    for (int n = 0; n < N; ++n) {
+      node[n] = mat[n] - 1;
+      if (!rod) continue;
+      const long cc = h.cell[n] - h.base;
+      const int i = (int)(cc % h.NXP) - 1, j = (int)((cc / h.NXP) % h.NYP) - 1, k = (int)(cc / ((long)h.NXP * h.NYP)) - 1;
+      const int b = t->fbmap[i + h.nxx * j];
+      if (b < 0 || b > t->nb) return fail(c, ADG_E_ARG, "fbmap bank out of range");
+      if (b) {
+        node[N + n] = b | (k << 16);
+        ++colcount[i + h.nxx * j];
+      }
+    }
+    if (rod)
+      for (int j = 0; j < h.nyy; ++j)
+        for (int i = 0; i < h.nxx; ++i)
+          if (t->fbmap[i + h.nxx * j] > 0 && colcount[i + h.nxx * j] != h.nzz)
+            return fail(c, ADG_E_ARG, "rodded column without a node at every level");
+    if (h.xtab) HIPCHK(hipFree(h.xtab));
+    h.xtab = nullptr;
+    if (h.xnode) HIPCHK(hipFree(h.xnode));
+    h.xnode = nullptr;
+    h.xtab = dalloc<double>(c->s, tab.size());
+    scopy(c->s, h.xtab, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice);
+    h.xnode = dalloc<int>(c->s, node.size());
+    scopy(c->s, h.xnode, node.data(), node.size() * sizeof(int), hipMemcpyHostToDevice);
+    if (!c->xerr) c->xerr = dalloc<int>(c->s, 1);
+    for (hipEvent_t &e : c->xev)
+      if (!e) HIPCHK(hipEventCreate(&e));
+    h.xon = on;
+    h.xnb = rod ? t->nb : 0;
+    h.xnmat = M;
+    const double refs[7] = {t->rbcon, t->rftem, t->rmtem, t->rcden, t->coreh, t->pos0, t->ssize};
+    std::memcpy(h.xref, refs, sizeof refs);
+    // what adg_set_xs takes besides the node cross sections
+    std::vector<double> st;
+    std::vector<double> ch((size_t)G * N);
+    for (int g = 0; g < G; ++g)
+      for (int n = 0; n < N; ++n) ch[(size_t)g * N + n] = chi[(size_t)g * M + (mat[n] - 1)];
+    scatter(c, core, ch.data(), G, c->chi, st);
+    scatter(c, core, dc, 6 * G, c->dc, st);
+    std::vector<double> z((size_t)G * N, 0.0);
+    scatter(c, core, z.data(), G, c->exsrc, st);
+    h.xs_set = false;
+  } catch (const HipError &e) {
+    return hip_fail(c, e);
+  }
+  return ADG_OK;
+}
+
+int adg_set_xs_tables(adg_ctx *c, const adg_xs_tables *t, const int *mat, const double *chi, const double *dc) {
+  return adg_set_xs_tables_core(c, 0, t, mat, chi, dc);
+}
+
+int adg_update_xs_core(adg_ctx *c, int core, double bcon, const double *ftem, const double *mtem,
+                       const double *cden, const double *bpos) {
+  if (!core_ok(c, core)) return ADG_E_ARG;
+  CoreHost &h = c->core[core];
+  if (!h.xtab) return fail(c, ADG_E_NOT_SET, "adg_set_xs_tables has not been called");
+  const int N = h.N, on = h.xon;
+  const double *in[4] = {(on & (1 << XK_FTEM)) ? ftem : nullptr, (on & (1 << XK_MTEM)) ? mtem : nullptr,
+                         (on & (1 << XK_CDEN)) ? cden : nullptr, (on & (1 << XK_ROD)) ? bpos : nullptr};
+  const int need[4] = {XK_FTEM, XK_MTEM, XK_CDEN, XK_ROD};
+  for (int q = 0; q < 4; ++q)
+    if ((on & (1 << need[q])) && !in[q]) return fail(c, ADG_E_ARG, "parameter array of an active feedback is null");
+  try {
+    ensure_stage(c, 3L * N + std::max(h.xnb, 1));
+    for (int q = 0; q < 4; ++q)
+      if (in[q])
+        HIPCHK(hipMemcpyAsync(c->dstage + (long)q * N, in[q], sizeof(double) * (q < 3 ? N : h.xnb),
+                              hipMemcpyHostToDevice, c->s));
+    HIPCHK(hipMemsetAsync(c->xerr, 0, sizeof(int), c->s));
+    XsArgs a;
+    const long K = (4L * c->G + (long)c->G * c->G) * h.xnmat;
+    a.tab = h.xtab;
+    a.zdum = h.xtab + 6 * K;
+    a.zdel = h.xtab + 6 * K + h.nzz;
+    a.mat0 = h.xnode;
+    a.rodk = h.xnode + N;
+    a.ftem = in[0] ? c->dstage : nullptr;
+    a.mtem = in[1] ? c->dstage + N : nullptr;
+    a.cden = in[2] ? c->dstage + 2L * N : nullptr;
+    a.bpos = in[3] ? c->dstage + 3L * N : nullptr;
+    a.cell = c->d_cell + h.coff;
+    a.D = c->D; a.sigr = c->sigr; a.nuf = c->nuf; a.sigf = c->sigf; a.sigs = c->sigs;
+    a.err = c->xerr;
+    a.NC = c->NC;
+    a.N = N; a.G = c->G; a.M = h.xnmat; a.nzz = h.nzz; a.on = on;
+    a.bcon = bcon;
+    a.rbcon = h.xref[0]; a.rftem = h.xref[1]; a.rmtem = h.xref[2]; a.rcden = h.xref[3];
+    a.rod0 = h.xref[4] - h.xref[5];
+    a.ssize = h.xref[6];
+    const dim3 grid((N + 255) / 256), blk(256);
+    HIPCHK(hipEventRecord(c->xev[0], c->s));
+    switch (c->G) {
+      case 2: hipLaunchKernelGGL(k_xs_update<2>, grid, blk, 0, c->s, a); break;
+      case 4: hipLaunchKernelGGL(k_xs_update<4>, grid, blk, 0, c->s, a); break;
+      case 8: hipLaunchKernelGGL(k_xs_update<8>, grid, blk, 0, c->s, a); break;
+      default: hipLaunchKernelGGL(k_xs_update<0>, grid, blk, 0, c->s, a); break;
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(c->xev[1], c->s));
+    int err = 0;
+    HIPCHK(hipMemcpyAsync(&err, c->xerr, sizeof(int), hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, c->xev[0], c->xev[1]));
+    c->timing[0] = ms;
+    c->timing[1] = c->timing[2] = c->timing[3] = 0.0;
+    if (err) {
+      h.xs_set = false;
+      return fail(c, ADG_E_XS, "cross section out of range: sigtr < 1e-5, or a negative removal, nu-fission or "
+                                "scattering cross section (src/mod_xsec.f90:215,86-150)");
+    }
+    h.xs_set = true;
+  } catch (const HipError &e) {
+    return hip_fail(c, e);
+  }
+  return ADG_OK;
+}
+
+int adg_update_xs(adg_ctx *c, double bcon, const double *ftem, const double *mtem, const double *cden,
+                  const double *bpos) {
+  return adg_update_xs_core(c, 0, bcon, ftem, mtem, cden, bpos);
+}
+
+int adg_get_xs_core(adg_ctx *c, int core, double *D, double *sigr, double *nuf, double *sigf, double *sigs) {
+  if (!core_ok(c, core)) return ADG_E_ARG;
+  if (!c->core[core].xs_set) return fail(c, ADG_E_NOT_SET, "no cross sections set");
+  try {
+    std::vector<double> st;
+    const int G = c->G;
+    if (D) gather(c, core, c->D, G, D, st);
+    if (sigr) gather(c, core, c->sigr, G, sigr, st);
+    if (nuf) gather(c, core, c->nuf, G, nuf, st);
+    if (sigf) gather(c, core, c->sigf, G, sigf, st);
+    if (sigs) gather(c, core, c->sigs, G * G, sigs, st);
+  } catch (const HipError &e) {
+    return hip_fail(c, e);
+  }
+  return ADG_OK;
+}
+
+int adg_get_xs(adg_ctx *c, double *D, double *sigr, double *nuf, double *sigf, double *sigs) {
+  return adg_get_xs_core(c, 0, D, sigr, nuf, sigf, sigs);
 }
 
 int adg_set_xs(adg_ctx *c, const double *D, const double *sigr, const double *sigs, const double *nuf,

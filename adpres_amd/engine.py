@@ -27,7 +27,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("ADG_LIBRARY") or os.path.join(HERE, "libadpres_gpu.so")
 
 ADG_OK, ADG_E_MAX_OUTER, ADG_E_NODAL_UNSTABLE, ADG_E_LU_PIVOT, ADG_E_POWER_NONPOS = 0, 1, 2, 3, 4
-ADG_E_ARG, ADG_E_HIP, ADG_E_NOT_SET = 5, 6, 7
+ADG_E_ARG, ADG_E_HIP, ADG_E_NOT_SET, ADG_E_XS = 5, 6, 7, 8
 ADG_FORWARD, ADG_ADJOINT, ADG_FIXED_SOURCE, ADG_TRANSIENT, ADG_ADJOINT_FROZEN = 0, 1, 2, 3, 4   # enum adg_mode
 FP64, FP32_REFINE = 0, 1
 PRECISIONS = {"fp64": FP64, "fp32_refine": FP32_REFINE}
@@ -40,6 +40,8 @@ EXPORTS = ("adg_create", "adg_destroy", "adg_set_xs", "adg_set_state", "adg_get_
            "adg_outer_batch", "adg_trace_core", "adg_powdis_core",
            "adg_adjoint", "adg_get_adjoint", "adg_set_adjoint", "adg_backward",
            "adg_set_transient", "adg_set_transient_core", "adg_leakage",
+           "adg_set_xs_tables", "adg_set_xs_tables_core", "adg_update_xs", "adg_update_xs_core",
+           "adg_get_xs", "adg_get_xs_core",
            "adg_status_string", "adg_last_error")
 
 _dp = C.POINTER(C.c_double)
@@ -52,6 +54,12 @@ class AdgGeom(C.Structure):
                 ("ix", _ip), ("iy", _ip), ("iz", _ip),
                 ("xdel", _dp), ("ydel", _dp), ("zdel", _dp),
                 ("bc", C.c_int * 6), ("precision", C.c_int)]
+
+
+class AdgXsTables(C.Structure):
+    _fields_ = [("nmat", C.c_int), ("nb", C.c_int)] + \
+        [(k, C.c_double) for k in ("rbcon", "rftem", "rmtem", "rcden", "coreh", "pos0", "ssize")] + \
+        [("tab", (_dp * 5) * 6), ("fbmap", _ip)]
 
 
 class AdgIter(C.Structure):
@@ -108,6 +116,12 @@ def load_library(path: str = LIB_PATH):
     L.adg_set_transient.argtypes = [C.c_void_p, _dp]
     L.adg_set_transient_core.argtypes = [C.c_void_p, C.c_int, _dp]
     L.adg_leakage.argtypes = [C.c_void_p, _dp]
+    L.adg_set_xs_tables.argtypes = [C.c_void_p, C.POINTER(AdgXsTables), _ip, _dp, _dp]
+    L.adg_set_xs_tables_core.argtypes = [C.c_void_p, C.c_int, C.POINTER(AdgXsTables), _ip, _dp, _dp]
+    L.adg_update_xs.argtypes = [C.c_void_p, C.c_double, _dp, _dp, _dp, _dp]
+    L.adg_update_xs_core.argtypes = [C.c_void_p, C.c_int, C.c_double, _dp, _dp, _dp, _dp]
+    L.adg_get_xs.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, _dp]
+    L.adg_get_xs_core.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp, _dp, _dp]
     L.adg_adjoint.argtypes = [C.c_void_p, C.POINTER(AdgIter), _dp]
     L.adg_get_adjoint.argtypes = [C.c_void_p, _dp, _dp]
     L.adg_set_adjoint.argtypes = [C.c_void_p, _dp]
@@ -144,6 +158,41 @@ def _xs_args(p: Problem):
     mat = np.ascontiguousarray(p.mat, np.int32)
     return (arrs, mat), (_d(arrs["D"]), _d(arrs["sigr"]), _d(arrs["sigs"]), _d(arrs["nuf"]), _d(arrs["sigf"]),
                          _d(arrs["chi"]), _i(mat), _d(arrs["dc"]), _d(arrs["exsrc"]))
+
+
+XS_KINDS = ("base", "boron", "ftem", "mtem", "cden", "rod")
+XS_REFS = ("rbcon", "rftem", "rmtem", "rcden", "coreh", "pos0", "ssize")
+
+
+def _xs_tables(t: dict, keep: list) -> "AdgXsTables":
+    """adg_xs_tables from the dict form of the feedback tables: "nmat",
+    "nb", the reference values XS_REFS, per kind of XS_KINDS a list of five
+    arrays (sigtr, siga, nuf, sigf as (G, nmat), sigs as (G_to, G_from,
+    nmat)) or None, and "fbmap" (nyy, nxx) or None."""
+    s = AdgXsTables()
+    s.nmat, s.nb = int(t["nmat"]), int(t.get("nb", 0))
+    for k in XS_REFS:
+        setattr(s, k, float(t.get(k, 0.0)))
+    for i, kind in enumerate(XS_KINDS):
+        if t.get(kind) is None:
+            continue
+        for j in range(5):
+            a = np.ascontiguousarray(t[kind][j], np.float64)
+            keep.append(a)
+            s.tab[i][j] = _d(a)
+    if t.get("fbmap") is not None:
+        fb = np.ascontiguousarray(t["fbmap"], np.int32)
+        keep.append(fb)
+        s.fbmap = _i(fb)
+    return s
+
+
+def _opt(a, keep: list):
+    if a is None:
+        return None
+    a = np.ascontiguousarray(a, np.float64)
+    keep.append(a)
+    return _d(a)
 
 
 def _iter(p: Problem, ov: dict):
@@ -190,6 +239,33 @@ class Engine:
     def set_xs(self, p: Problem):
         _, args = _xs_args(p)
         self._check(self.lib.adg_set_xs(self.h, *args), "adg_set_xs")
+
+    def set_xs_tables(self, tables: dict):
+        """Install the feedback tables of xsec::XS_updt on the device (with
+        the problem's material map, chi and discontinuity factors); the node
+        cross sections then come from update_xs."""
+        keep = []
+        t = _xs_tables(tables, keep)
+        mat = np.ascontiguousarray(self.p.mat, np.int32)
+        chi = np.ascontiguousarray(self.p.chi, np.float64)
+        dc = np.ascontiguousarray(self.p.dc, np.float64)
+        self._check(self.lib.adg_set_xs_tables(self.h, C.byref(t), _i(mat), _d(chi), _d(dc)), "adg_set_xs_tables")
+
+    def update_xs(self, bcon: float, ftem=None, mtem=None, cden=None, bpos=None):
+        """xsec::XS_updt on the device for the given T-H and rod parameters."""
+        keep = []
+        self._check(self.lib.adg_update_xs(self.h, float(bcon), _opt(ftem, keep), _opt(mtem, keep),
+                                           _opt(cden, keep), _opt(bpos, keep)), "adg_update_xs")
+
+    def get_xs(self) -> dict:
+        """The node cross sections the context holds: D, sigr, nuf, sigf
+        (G, N) and sigs (G_to, G_from, N)."""
+        N, G = self.p.nnod, self.p.ng
+        out = {k: np.zeros((G, N)) for k in ("D", "sigr", "nuf", "sigf")}
+        out["sigs"] = np.zeros((G, G, N))
+        self._check(self.lib.adg_get_xs(self.h, *[_d(out[k]) for k in ("D", "sigr", "nuf", "sigf", "sigs")]),
+                    "adg_get_xs")
+        return out
 
     def reset(self):
         """Start the next outer() from f0 = 1, k = 1, D-hat = 0 (the reference's first call)."""

@@ -202,6 +202,36 @@ def time_solves(eng, steps, warmup):
                 gsolve_avg_ms=pr["gsolve_ms"], nodal_avg_ms=pr["nodal_ms"])
 
 
+def xs_update_probe(eng, p, workload, seed, reps=10):
+    """The device cross-section update (adg_update_xs: XS_updt with boron,
+    temperature, density and rod feedback) on the workload's geometry with
+    synthetic feedback tables: its kernel time from HIP events and the
+    host-inclusive time of the call (parameter upload, kernel, status read)."""
+    from adpres_amd.configs import feedback_tables, generate
+    t, par = feedback_tables(generate(workload, seed))
+    eng.set_xs_tables(t)
+    for _ in range(3):
+        eng.update_xs(*par)
+    kern, wall = [], []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        eng.update_xs(*par)
+        wall.append(time.perf_counter() - t0)
+        kern.append(eng.last_timing_ms()[0])
+    G = p.ng
+    # per node: material, rod column, cell index and three parameters in;
+    # D, sigr, nuf, sigf (G each) and sigs (G^2) out
+    per_node = 4 + 4 + 8 + 3 * 8 + 8 * (4 * G + G * G)
+    nbytes = per_node * p.nnod
+    us = float(np.mean(kern)) * 1e3
+    ach = nbytes / (us * 1e-6) / 1e9
+    return {"bound": "hbm", "kernel": "k_xs_update", "algorithmic_bytes_per_launch": nbytes,
+            "bytes_per_node": per_node, "avg_launch_us": round(us, 2), "achieved": round(ach, 1),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+            "call_ms_with_upload": round(float(np.mean(wall)) * 1e3, 3), "samples": reps,
+            "timing": "HIP events around the kernel on the engine stream (adg_last_timing after adg_update_xs)"}
+
+
 def cpu_baseline(problem, outers_per_solve, nupd):
     """The reference build (oracle/_ref/ref_dump: ADPRES's Fortran, amdflang -O3,
     one host core) on the same core.  Small cores run the whole solve with the
@@ -337,6 +367,7 @@ def main():
     t0 = time.perf_counter()
     eng.set_xs(p)
     up_s = time.perf_counter() - t0
+    xs_dev = xs_update_probe(eng, p, args.workload, w["seed"])
     eng.close()
     log(f"bench: fp64 {r64['solves_per_s']:.3f} solves/s k={r64['k']:.8f} outers={r64['niter']}")
     r32 = None
@@ -373,6 +404,7 @@ def main():
                                                         "stream, in a profiled solve after the timed ones",
                 "group_solve_avg_us": round(r64["gsolve_avg_ms"] * 1e3, 2),
                 "nodal_update_avg_us": round(r64["nodal_avg_ms"] * 1e3, 2)}
+        roof["xs_update"] = xs_dev
         if r64["nodal_avg_ms"]:
             nf = nodal_flops(p)
             gf = nf / (r64["nodal_avg_ms"] * 1e-3) / 1e9

@@ -35,7 +35,9 @@ enum adg_status {
   ADG_E_POWER_NONPOS = 4,   /* src/mod_cmfd.f90:1322-1326 total power <= 0          */
   ADG_E_ARG = 5,            /* invalid argument / unsupported geometry              */
   ADG_E_HIP = 6,            /* HIP runtime error (adg_last_error has the text)       */
-  ADG_E_NOT_SET = 7         /* cross sections not set yet                            */
+  ADG_E_NOT_SET = 7,        /* cross sections not set yet                            */
+  ADG_E_XS = 8              /* src/mod_xsec.f90:215,86-150 the reference STOPs:      */
+                            /* sigtr < 1e-5, D ~ 0 or negative sigr, nuf or sigs     */
 };
 
 enum adg_kern { ADG_KERN_FDM = 0, ADG_KERN_PNM = 1, ADG_KERN_SANM = 2 };   /* %KERN */
@@ -108,6 +110,46 @@ int adg_get_state(adg_ctx *ctx, double *f0, double *fs0, double *Ke, double *dn,
  * nodal update, :677-679: rod_eject's adjoint).  trace may be NULL. */
 int adg_outer(adg_ctx *ctx, int mode, const adg_iter *iter, adg_trace *trace);
 
+/* The feedback tables of xsec::XS_updt as sdata holds them
+ * (src/mod_data.f90), filled by the %XSEC, %CBCS/%BCON, %FTEM, %MTEM, %CDEN
+ * and %CROD readers of src/mod_io.f90.  tab[kind][xs]: kind ADG_XS_BASE
+ * (x*), ADG_XS_BORON (c*), ADG_XS_FTEM (f*), ADG_XS_MTEM (m*), ADG_XS_CDEN
+ * (l*), ADG_XS_ROD (d*); xs 0 sigtr, 1 siga, 2 nuf, 3 sigf as (nmat,G) and
+ * 4 sigs as (nmat,G,G) (from, to), column-major.  tab[kind][0] == NULL:
+ * that feedback is off (its b* flag is 0; boron is on for bbcon or bcbcs).
+ * fbmap (nxx,nyy): the rod bank of each node column, 0 none.  The reference
+ * values rbcon, rftem, rmtem, rcden and the rod geometry coreh, pos0, ssize
+ * are sdata's. */
+enum adg_xs_kind { ADG_XS_BASE = 0, ADG_XS_BORON = 1, ADG_XS_FTEM = 2, ADG_XS_MTEM = 3, ADG_XS_CDEN = 4,
+                   ADG_XS_ROD = 5 };
+typedef struct adg_xs_tables {
+  int nmat, nb;
+  double rbcon, rftem, rmtem, rcden, coreh, pos0, ssize;
+  const double *tab[6][5];
+  const int *fbmap;
+} adg_xs_tables;
+
+/* Installs the tables on the device, with the node material map mat(N),
+ * the fission spectrum chi(nmat,G) and discontinuity factors dc(N,G,6) (what
+ * adg_set_xs takes besides the node cross sections; the external source is
+ * set to zero).  The node cross sections are then made by adg_update_xs. */
+int adg_set_xs_tables(adg_ctx *ctx, const adg_xs_tables *t, const int *mat, const double *chi,
+                      const double *dc);
+/* xsec::XS_updt (src/mod_xsec.f90:11-46) on the device: the node D, sigr,
+ * nuf, sigf and sigs for boron concentration bcon, node fuel temperatures
+ * ftem(N), moderator temperatures mtem(N), coolant densities cden(N) and
+ * bank positions bpos(nb) (steps), bit for bit the reference's; arrays
+ * whose feedback is off may be NULL.  Replaces the host XS_updt call before
+ * each outer of the T-H iteration and the boron searches (src/mod_th.f90:48,
+ * 774-794; src/mod_control.f90:28-107).  ADG_E_XS where the reference would
+ * stop; the context's cross sections are then not to be used. */
+int adg_update_xs(adg_ctx *ctx, double bcon, const double *ftem, const double *mtem, const double *cden,
+                  const double *bpos);
+/* The node cross sections the context holds (after adg_set_xs or
+ * adg_update_xs), in node order: D, sigr, nuf, sigf (N,G), sigs (N,G,G)
+ * (from, to); NULL skips an array. */
+int adg_get_xs(adg_ctx *ctx, double *D, double *sigr, double *nuf, double *sigf, double *sigs);
+
 /* cmfd::PowDis (src/mod_cmfd.f90:1290-1333): node powers normalised to 1. */
 int adg_powdis(adg_ctx *ctx, double *p);
 /* The per-node factor (1 - tbeta(mat(n)) + dfis(n)) of TSrcTr
@@ -135,7 +177,8 @@ int adg_matvec(adg_ctx *ctx, int g, const double *x, double *y);
  * nod_time (src/mod_data.f90:204-217, accumulated at src/mod_cmfd.f90:442-489
  * and :356-378), measured with HIP events on the context's stream.
  * adg_last_timing: device time (ms) of the last adg_outer call,
- *   [0] total, [1] group solves (TSrc + BiCGSTAB), [2] nodal updates, [3] rest.
+ *   [0] total, [1] group solves (TSrc + BiCGSTAB), [2] nodal updates, [3] rest;
+ *   after adg_update_xs, [0] its kernel's time and [1..3] zero.
  * With profiling on (adg_set_profiling(ctx, 1)), adg_outer also brackets the
  * phases and one steady-state BiCGSTAB K1 launch per group solve with events;
  * adg_read_profile returns averages over the last adg_outer call:
@@ -182,6 +225,12 @@ int adg_get_state_core(adg_ctx *ctx, int core, double *f0, double *fs0, double *
 int adg_outer_batch(adg_ctx *ctx, int mode, const adg_iter *iter, int niter_in, int *status, int *niter,
                     double *keff);
 int adg_trace_core(adg_ctx *ctx, int core, adg_trace *trace);
+int adg_set_xs_tables_core(adg_ctx *ctx, int core, const adg_xs_tables *t, const int *mat,
+                           const double *chi, const double *dc);
+int adg_update_xs_core(adg_ctx *ctx, int core, double bcon, const double *ftem, const double *mtem,
+                       const double *cden, const double *bpos);
+int adg_get_xs_core(adg_ctx *ctx, int core, double *D, double *sigr, double *nuf, double *sigf,
+                    double *sigs);
 int adg_powdis_core(adg_ctx *ctx, int core, double *p);
 
 /* Differentiation form.  The reference's only adjoint is cmfd::outer_ad's
