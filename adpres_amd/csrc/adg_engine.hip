@@ -128,6 +128,7 @@ struct adg_ctx {
   int check_every = 8;             // outers between host polls of the done flags
   bool nod_coop = true;            // cooperative pass 2 for G >= 5 (ADG_NOD_COOP=0: per-thread)
   bool src_full = true;            // source kernel on one chunk per block (ADG_SRC_FULL=0: flat shape)
+  bool xs_lds = true;              // cross-section tables staged in LDS when they fit (ADG_XS_LDS=0: caches)
   double timing[4] = {0, 0, 0, 0};
   std::string err;
   // profiling: event pairs recorded on the context stream during adg_outer
@@ -940,6 +941,7 @@ int create_batch(const adg_geom *geoms, int ncore, adg_ctx **out) {
     if (const char *ev = getenv("ADG_CHECK")) c->check_every = std::max(1, atoi(ev));
     if (const char *ev = getenv("ADG_NOD_COOP")) c->nod_coop = atoi(ev) != 0;
     if (const char *ev = getenv("ADG_SRC_FULL")) c->src_full = atoi(ev) != 0;
+    if (const char *ev = getenv("ADG_XS_LDS")) c->xs_lds = atoi(ev) != 0;
     c->core.resize(ncore);
     std::vector<std::vector<unsigned char>> fls(ncore);
     std::vector<std::vector<double>> vds(ncore);
@@ -1250,14 +1252,26 @@ int adg_update_xs_core(adg_ctx *c, int core, double bcon, const double *ftem, co
     a.rbcon = h.xref[0]; a.rftem = h.xref[1]; a.rmtem = h.xref[2]; a.rcden = h.xref[3];
     a.rod0 = h.xref[4] - h.xref[5];
     a.ssize = h.xref[6];
-    const dim3 grid((N + 255) / 256), blk(256);
+    // LDS slots: base (boron folded in) and each present kind after boron
+    int slots = 1;
+    for (int kd = XK_FTEM; kd < XK_N; ++kd) slots += (on >> kd) & 1;
+    const size_t lds = sizeof(double) * slots * K;
+    const bool use_lds = c->xs_lds && lds <= 64 * 1024;
+    const dim3 blk(use_lds ? 512 : 256);
+    const dim3 grid = use_lds ? dim3((unsigned)std::min<long>((N + 511) / 512, 1024)) : dim3((N + 255) / 256);
     HIPCHK(hipEventRecord(c->xev[0], c->s));
+#define XS_LAUNCH(GG)                                                                  \
+  if (use_lds)                                                                         \
+    hipLaunchKernelGGL(k_xs_update_lds<GG>, grid, blk, lds, c->s, a);                  \
+  else                                                                                 \
+    hipLaunchKernelGGL(k_xs_update<GG>, grid, blk, 0, c->s, a);
     switch (c->G) {
-      case 2: hipLaunchKernelGGL(k_xs_update<2>, grid, blk, 0, c->s, a); break;
-      case 4: hipLaunchKernelGGL(k_xs_update<4>, grid, blk, 0, c->s, a); break;
-      case 8: hipLaunchKernelGGL(k_xs_update<8>, grid, blk, 0, c->s, a); break;
-      default: hipLaunchKernelGGL(k_xs_update<0>, grid, blk, 0, c->s, a); break;
+      case 2: XS_LAUNCH(2) break;
+      case 4: XS_LAUNCH(4) break;
+      case 8: XS_LAUNCH(8) break;
+      default: XS_LAUNCH(0) break;
     }
+#undef XS_LAUNCH
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(c->xev[1], c->s));
     int err = 0;

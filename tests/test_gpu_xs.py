@@ -69,11 +69,14 @@ def test_gpu_xs_update_matches_oracle_random_states():
     e.close()
 
 
+@pytest.mark.parametrize("lds", ["1", "0"])
 @pytest.mark.parametrize("case", [("core3d_8g", 2), ("batch", 7), ("batch", 0)])
-def test_gpu_xs_update_synthetic_tables(case):
+def test_gpu_xs_update_synthetic_tables(case, lds, monkeypatch):
     """Synthetic feedback tables (every feedback, rods in an eighth of the
     columns) on the 8-group core, a 3-group and a 4-group core (the kernel's
-    run-time and compiled group counts): bit-exact against the oracle."""
+    run-time and compiled group counts), with the tables staged in LDS and
+    read through the caches: bit-exact against the oracle."""
+    monkeypatch.setenv("ADG_XS_LDS", lds)
     from adpres_amd.configs import batch_core, feedback_tables, generate
     from oracle.oracle import xs_update
     name, seed = case
