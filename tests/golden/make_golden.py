@@ -243,10 +243,34 @@ SWEEP_DECKS = ("IAEA3Ds", "IAEA2D", "PNM", "BIBLIS", "KOEBERG", "DVP", "FDM", "F
                "MOX/part1_ari_helios")
 
 
+MOX_XTAB_DECKS = ("part2_serpent", "part2_helios", "part3_serpent", "part3_helios")
+
+
+def mox_xtab():
+    """The MOX part-2/3 boron searches with T-H feedback and branch-table
+    cross sections (%XTAB, XStab_updt / brInterp, src/mod_xsec.f90:48-84,
+    520-788; part 3 with rodded branch tables, crod_tab_updt): the decks
+    name their table files at the author's paths; the copies under
+    decks/mox/ name the files beside them."""
+    import re
+    out = os.path.join(HERE, "decks", "mox")
+    os.makedirs(out, exist_ok=True)
+    xsec = os.path.join(os.path.dirname(REF), "xsec")
+    for sub_ in ("SERPENT_CMM", "HELIOS"):
+        for fn in os.listdir(os.path.join(xsec, sub_)):
+            shutil.copy(os.path.join(xsec, sub_, fn), os.path.join(out, fn))
+    for deck in MOX_XTAB_DECKS:
+        text = open(os.path.join(REF, "MOX", deck)).read()
+        text = re.sub(r"\S*/smpl/xsec/(SERPENT_CMM|HELIOS)/", "./", text)
+        with open(os.path.join(out, deck), "w") as fh:
+            fh.write(text)
+
+
 def outputs():
     """The result blocks (power, axial, flux, T-H tables) of the reference
-    program's .out file for every self-contained static sample deck and the
-    NEACRP searches: the regression sweep of the drop-in's outputs."""
+    program's .out file for every self-contained static sample deck, the
+    NEACRP searches and the MOX branch-table searches: the regression sweep
+    of the drop-in's outputs."""
     sys.path.insert(0, os.path.dirname(HERE))
     from conftest import output_tokens
     exe = os.path.join(ROOT, "oracle", "_ref", "adpres")
@@ -254,8 +278,11 @@ def outputs():
     jobs = [(os.path.basename(d), [os.path.join(REF, d)]) for d in SWEEP_DECKS]
     nea = os.path.join(HERE, "decks", "neacrp")
     jobs += [(d, [os.path.join(nea, f) for f in os.listdir(nea)]) for d in NEACRP_DECKS]
+    mox_xtab()
+    mox = os.path.join(HERE, "decks", "mox")
+    jobs += [(d, [os.path.join(mox, f) for f in os.listdir(mox)]) for d in MOX_XTAB_DECKS]
     for deck, files in jobs:
-        if deck not in NEACRP_DECKS and not os.path.exists(os.path.join(HERE, "decks", deck)):
+        if deck not in NEACRP_DECKS + MOX_XTAB_DECKS and not os.path.exists(os.path.join(HERE, "decks", deck)):
             shutil.copy(files[0], os.path.join(HERE, "decks", deck))
         with tempfile.TemporaryDirectory() as d:
             for f in files:

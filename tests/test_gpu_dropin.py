@@ -145,18 +145,20 @@ def test_dropin_neacrp_boron_search_with_th(deck, tmp_path):
 
 SWEEP = ["IAEA3Ds", "IAEA2D", "PNM", "BIBLIS", "KOEBERG", "DVP", "FDM", "FDM-1D", "adjoint", "fixed_source",
          "CBCsearch", "part1_aro_serpent", "part1_ari_serpent", "part1_aro_helios", "part1_ari_helios",
-         "A1", "B1", "C1"]
+         "A1", "B1", "C1", "part2_serpent", "part2_helios", "part3_serpent", "part3_helios"]
 
 
 @pytest.mark.parametrize("deck", SWEEP)
 def test_dropin_output_files_match_reference(deck, tmp_path):
     """Regression sweep over the reference's self-contained static sample
-    decks (and the NEACRP searches): the result blocks that ADPRES's own
+    decks, the NEACRP searches and the MOX part-2/3 searches with branch
+    tables (%XTAB: the host's XStab_updt, rodded tables in part 3): the result blocks that ADPRES's own
     output routines write into <deck>.out from the GPU solution (radial and
     axial power, radial flux, T-H summary) equal the reference's to one unit
     of the last printed digit."""
     from conftest import output_tokens, printed_unit
-    src = os.path.join(GOLDEN, "decks", "neacrp") if deck in ("A1", "B1", "C1") else None
+    src = (os.path.join(GOLDEN, "decks", "neacrp") if deck in ("A1", "B1", "C1") else
+           os.path.join(GOLDEN, "decks", "mox") if deck.startswith(("part2_", "part3_")) else None)
     files = [os.path.join(src, f) for f in os.listdir(src)] if src else [os.path.join(GOLDEN, "decks", deck)]
     for f in files:
         shutil.copy(f, tmp_path / os.path.basename(f))
