@@ -387,6 +387,40 @@ def read_ref_dump(path: str, action: str = "solve"):
             st["sigr"] = r.take("<f8", G * N).reshape(G, N)
             states.append(st)
         res["states"] = states
+    elif action == "xtab":
+        # branch tables in the flat form of branch_tables_from_dict
+        bcrod, nb = (int(v) for v in r.take("<i4", 2))
+        M = int(nmat)
+        dims = r.take("<i4", 5 * M).reshape(M, 5)
+        E = 4 * G + G * G + 6 * G
+        pts, xs, rxs = [], [], []
+        for mm in range(M):
+            nd, nbr, nf, nm, trod = (int(v) for v in dims[mm])
+            pts.append(r.take("<f8", nd + nbr + nf + nm))
+            xs.append(r.take("<f8", nd * nbr * nf * nm * E))
+            if trod == 1:
+                rxs.append(r.take("<f8", nd * nbr * nf * nm * E))
+        coreh, pos0, ssize = (float(v) for v in r.take("<f8", 3))
+        fb = r.take("<i4", int(nxx) * int(nyy)).reshape(int(nyy), int(nxx)) if bcrod else None
+        res["branch"] = {"nmat": M, "nb": nb if bcrod else 0, "dims": dims.astype(np.int32),
+                         "points": np.concatenate(pts), "xsec": np.concatenate(xs),
+                         "rxsec": np.concatenate(rxs) if rxs else None,
+                         "coreh": coreh, "pos0": pos0, "ssize": ssize, "fbmap": fb}
+        states = []
+        for _ in range(2):
+            st = {"bcon": float(r.take("<f8", 1)[0])}
+            for key in ("ftem", "mtem", "cden"):
+                st[key] = r.take("<f8", N)
+            if bcrod:
+                st["bpos"] = r.take("<f8", nb)
+            for key in ("sigtr", "siga", "nuf", "sigf"):
+                st[key] = r.take("<f8", G * N).reshape(G, N)
+            st["sigs"] = r.take("<f8", G * G * N).reshape(G, G, N)
+            st["D"] = r.take("<f8", G * N).reshape(G, N)
+            st["sigr"] = r.take("<f8", G * N).reshape(G, N)
+            st["dc"] = r.take("<f8", 6 * G * N).reshape(6, G, N)
+            states.append(st)
+        res["states"] = states
     elif action == "trstep":
         res["Ke_s"] = float(r.take("<f8", 1)[0])
         res["f0_s"] = r.take("<f8", G * N).reshape(G, N)
@@ -474,4 +508,20 @@ def xs_tables_from_dict(d, prefix: str = "t_"):
             if key in files and np.asarray(d[key]).ndim == 2 else None
     fb = np.asarray(d[prefix + "fbmap"]) if prefix + "fbmap" in files else None
     t["fbmap"] = fb if fb is not None and fb.ndim == 2 else None
+    return t
+
+
+def branch_tables_from_dict(d, prefix: str = "b_"):
+    """Flat branch tables of xsec::XStab_updt (src/mod_xsec.f90:48-84) from
+    a fixture's arrays: {"nmat", "nb", "dims" (nmat, 5: nd, nb, nf, nm,
+    trod), "points", "xsec", "rxsec" or None, "coreh", "pos0", "ssize",
+    "fbmap" (nyy, nxx) or None}; the layouts are those of adg_xtab_tables."""
+    t = {"nmat": int(np.asarray(d[prefix + "nmat"])), "nb": int(np.asarray(d[prefix + "nb"]))}
+    for k in ("dims", "points", "xsec"):
+        t[k] = np.asarray(d[prefix + k])
+    for k in ("coreh", "pos0", "ssize"):
+        t[k] = float(np.asarray(d[prefix + k]))
+    for k in ("rxsec", "fbmap"):
+        a = np.asarray(d[prefix + k])
+        t[k] = None if a.dtype == object or a.ndim == 0 else a
     return t

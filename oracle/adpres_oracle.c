@@ -1219,3 +1219,197 @@ int orc_xs_update(const orc_problem *pb, const orc_xs_tables *t, double bcon, co
     if (sigs[i] < 0.0) return ORC_E_XS;
   return ORC_OK;
 }
+
+/* ---- branch-table update: xsec::XStab_updt (src/mod_xsec.f90:48-84) ---- */
+
+/* the two branch points around x (brInterp's bracket search, :553-660):
+   inside the range the first interval holding x; outside, the end interval
+   when x is within 20 % (rel) or 100 ppm (boron) of the end point, both
+   default-real literals there; otherwise the reference stops */
+static int br_bracket(const double *p, int n, double x, int rel, int *i1, int *i2) {
+  *i1 = 0;
+  *i2 = 0;
+  if (n <= 1) return 0;
+  if (x >= p[0] && x <= p[n - 1]) {
+    for (int s = 1; s < n; ++s)
+      if (x >= p[s - 1] && x <= p[s]) {
+        *i1 = s - 1;
+        *i2 = s;
+        break;
+      }
+  } else if (x < p[0] && (rel ? (p[0] - x) / p[0] < (double)0.2f : (p[0] - x) < (double)100.f)) {
+    *i1 = 0;
+    *i2 = 1;
+  } else if (x > p[n - 1] && (rel ? (x - p[n - 1]) / p[n - 1] < (double)0.2f : (x - p[n - 1]) < (double)100.f)) {
+    *i1 = n - 2;
+    *i2 = n - 1;
+  } else {
+    return ORC_E_XS;
+  }
+  return 0;
+}
+
+/* brInterp (:520-788) for one material: tab is its (rodded or unrodded)
+   record block, P its points; out (E) */
+static int br_interp(const int *dm, const double *P, const double *tab, int E, double cd, double bc, double ft,
+                     double mt, double *xs /* 8 E scratch */, double *out) {
+  const int nd = dm[0], nb = dm[1], nf = dm[2], nm = dm[3];
+  const double *pd = P, *pbr = P + nd, *pf = P + nd + nb, *pm = P + nd + nb + nf;
+  int s1, s2, t1, t2, u1, u2, v1, v2;
+  if (br_bracket(pd, nd, cd, 1, &s1, &s2) || br_bracket(pbr, nb, bc, 0, &t1, &t2) ||
+      br_bracket(pf, nf, ft, 1, &u1, &u2) || br_bracket(pm, nm, mt, 1, &v1, &v2))
+    return ORC_E_XS;
+#define REC(s, t, u, v) (tab + (long)E * ((s) + (long)nd * ((t) + (long)nb * ((u) + (long)nf * (v)))))
+  const int cs[8] = {s1, s1, s1, s1, s2, s2, s2, s2}, ct[8] = {t1, t1, t2, t2, t1, t1, t2, t2};
+  const int cu[8] = {u1, u2, u1, u2, u1, u2, u1, u2};
+  /* moderator temperature */
+  if (nm > 1) {
+    const double radx = (mt - pm[v1]) / (pm[v2] - pm[v1]);
+    for (int c = 0; c < 8; ++c) {
+      const double *A = REC(cs[c], ct[c], cu[c], v1), *B = REC(cs[c], ct[c], cu[c], v2);
+      for (int e = 0; e < E; ++e) xs[c * E + e] = A[e] + radx * (B[e] - A[e]);
+    }
+  } else {
+    for (int c = 0; c < 8; ++c) memcpy(xs + c * E, REC(cs[c], ct[c], cu[c], v1), sizeof(double) * E);
+  }
+#undef REC
+  /* fuel temperature: pairs (1,2) (3,4) (5,6) (7,8) */
+  if (nf > 1) {
+    const double radx = (ft - pf[u1]) / (pf[u2] - pf[u1]);
+    for (int c = 0; c < 8; c += 2)
+      for (int e = 0; e < E; ++e) xs[c * E + e] = xs[c * E + e] + radx * (xs[(c + 1) * E + e] - xs[c * E + e]);
+  }
+  /* boron: pairs (1,3) (5,7) */
+  if (nb > 1) {
+    const double radx = (bc - pbr[t1]) / (pbr[t2] - pbr[t1]);
+    for (int c = 0; c < 8; c += 4)
+      for (int e = 0; e < E; ++e) xs[c * E + e] = xs[c * E + e] + radx * (xs[(c + 2) * E + e] - xs[c * E + e]);
+  }
+  /* coolant density: (1,5) */
+  if (nd > 1) {
+    const double r = (cd - pd[s1]) / (pd[s2] - pd[s1]);
+    for (int e = 0; e < E; ++e) xs[e] = xs[e] + r * (xs[4 * E + e] - xs[e]);
+  }
+  memcpy(out, xs, sizeof(double) * E);
+  return 0;
+}
+
+int orc_xtab_update(const orc_problem *pb, const orc_xtab_tables *t, double bcon, const double *ftem,
+                    const double *mtem, const double *cden, const double *bpos, double *sigtr,
+                    double *siga, double *nuf, double *sigf, double *sigs, double *D, double *sigr,
+                    double *dc) {
+  const int N = pb->nnod, G = pb->ng, M = t->nmat, E = 4 * G + G * G + 6 * G;
+  if (!t->dims || !t->points || !t->xsec || !ftem || !mtem || !cden) return ORC_E_ARG;
+  for (int n = 0; n < N; ++n)
+    if (pb->mat[n] < 1 || pb->mat[n] > M) return ORC_E_ARG;
+  long *poff = (long *)calloc((size_t)M, sizeof(long)), *xoff = (long *)calloc((size_t)M, sizeof(long));
+  long *roff = (long *)calloc((size_t)M, sizeof(long));
+  double *xs = (double *)malloc(sizeof(double) * 8 * E), *rec = (double *)malloc(sizeof(double) * E);
+  double *rr = (double *)malloc(sizeof(double) * E);
+  int *xyz = NULL, rc = ORC_OK;
+  if (!poff || !xoff || !roff || !xs || !rec || !rr) { rc = ORC_E_ARG; goto done; }
+  {
+    long p = 0, x = 0, r = 0;
+    for (int m = 0; m < M; ++m) {
+      const int *dm = t->dims + 5 * m;
+      const long nr = (long)dm[0] * dm[1] * dm[2] * dm[3];
+      poff[m] = p; xoff[m] = x; roff[m] = dm[4] == 1 ? r : -1;
+      p += dm[0] + dm[1] + dm[2] + dm[3];
+      x += nr * E;
+      if (dm[4] == 1) r += nr * E;
+    }
+  }
+  /* the unrodded interpolation of every node (XStab_updt :70-74) */
+  for (int n = 0; n < N; ++n) {
+    const int m = pb->mat[n] - 1;
+    if (br_interp(t->dims + 5 * m, t->points + poff[m], t->xsec + xoff[m], E, cden[n], bcon, ftem[n], mtem[n],
+                  xs, rec)) { rc = ORC_E_XS; goto done; }
+    for (int g = 0; g < G; ++g) {
+      sigtr[n + (long)N * g] = rec[g];
+      siga[n + (long)N * g] = rec[G + g];
+      nuf[n + (long)N * g] = rec[2 * G + g];
+      sigf[n + (long)N * g] = rec[3 * G + g];
+    }
+    for (int h = 0; h < G; ++h)
+      for (int g = 0; g < G; ++g) sigs[n + (long)N * (g + G * h)] = rec[4 * G + g + G * h];
+    for (int f = 0; f < 6; ++f)
+      for (int g = 0; g < G; ++g) dc[n + (long)N * (g + G * f)] = rec[4 * G + G * G + g + G * f];
+  }
+  /* crod_tab_updt (:291-376): rodded columns from the top down to the tip,
+     (1 - vfrac) X + vfrac R at the tip, R above it; then the column's
+     negative siga, nuf, sigf, sigs and dc set to zero */
+  if (t->fbmap) {
+    const int nx = pb->nxx, ny = pb->nyy, nz = pb->nzz;
+    if (!bpos || !t->rxsec) { rc = ORC_E_ARG; goto done; }
+    xyz = (int *)calloc((size_t)nx * ny * nz, sizeof(int));
+    if (!xyz) { rc = ORC_E_ARG; goto done; }
+    for (int n = 0; n < N; ++n)
+      xyz[(pb->ix[n] - 1) + (long)nx * ((pb->iy[n] - 1) + (long)ny * (pb->iz[n] - 1))] = n + 1;
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i) {
+        const int b = t->fbmap[i + nx * j];
+        if (b <= 0) continue;
+        if (b > t->nb) { rc = ORC_E_ARG; goto done; }
+        for (int k = 0; k < nz; ++k)
+          if (!xyz[i + (long)nx * (j + (long)ny * k)]) { rc = ORC_E_ARG; goto done; }
+        const double rodh = t->coreh - t->pos0 - bpos[b - 1] * t->ssize;
+        double dum = 0.0;
+        for (int k = nz - 1; k >= 0; --k) {
+          const int n = xyz[i + (long)nx * (j + (long)ny * k)] - 1, m = pb->mat[n] - 1;
+          if (roff[m] < 0) { rc = ORC_E_XS; goto done; }
+          if (br_interp(t->dims + 5 * m, t->points + poff[m], t->rxsec + roff[m], E, cden[n], bcon, ftem[n],
+                        mtem[n], xs, rr)) { rc = ORC_E_XS; goto done; }
+          const int part = rodh >= dum && rodh <= dum + pb->zdel[k];
+          const double vf = part ? (rodh - dum) / pb->zdel[k] : 0.0;
+          double *X[4] = {sigtr, siga, nuf, sigf};
+          for (int q = 0; q < 4; ++q)
+            for (int g = 0; g < G; ++g) {
+              double *x = &X[q][n + (long)N * g];
+              *x = part ? (1.0 - vf) * *x + vf * rr[q * G + g] : rr[q * G + g];
+            }
+          for (int h = 0; h < G; ++h)
+            for (int g = 0; g < G; ++g) {
+              double *x = &sigs[n + (long)N * (g + G * h)];
+              *x = part ? (1.0 - vf) * *x + vf * rr[4 * G + g + G * h] : rr[4 * G + g + G * h];
+            }
+          for (int f = 0; f < 6; ++f)
+            for (int g = 0; g < G; ++g) {
+              double *x = &dc[n + (long)N * (g + G * f)];
+              const double r = rr[4 * G + G * G + g + G * f];
+              *x = part ? (1.0 - vf) * *x + vf * r : r;
+            }
+          if (part) break;
+          dum = dum + pb->zdel[k];
+        }
+        for (int k = nz - 1; k >= 0; --k) {
+          const int n = xyz[i + (long)nx * (j + (long)ny * k)] - 1;
+          for (int g = 0; g < G; ++g) {
+            if (siga[n + (long)N * g] < 0.0) siga[n + (long)N * g] = 0.0;
+            if (nuf[n + (long)N * g] < 0.0) nuf[n + (long)N * g] = 0.0;
+            if (sigf[n + (long)N * g] < 0.0) sigf[n + (long)N * g] = 0.0;
+            for (int h = 0; h < G; ++h)
+              if (sigs[n + (long)N * (g + G * h)] < 0.0) sigs[n + (long)N * (g + G * h)] = 0.0;
+            for (int f = 0; f < 6; ++f)
+              if (dc[n + (long)N * (g + G * f)] < 0.0) dc[n + (long)N * (g + G * f)] = 0.0;
+          }
+        }
+      }
+  }
+  /* Dsigr_updt (:199-226) and check_xs (:86-150) */
+  for (int n = 0; n < N; ++n)
+    for (int g = 0; g < G; ++g) {
+      if (sigtr[n + (long)N * g] < (double)1.e-5f) { rc = ORC_E_XS; goto done; }
+      D[n + (long)N * g] = 1.0 / (3.0 * sigtr[n + (long)N * g]);
+      double dum = 0.0;
+      for (int h = 0; h < G; ++h)
+        if (g != h) dum = dum + sigs[n + (long)N * (g + G * h)];
+      sigr[n + (long)N * g] = siga[n + (long)N * g] + dum;
+    }
+  for (long i = 0; i < (long)N * G; ++i)
+    if (D[i] < (double)1.e-20f || sigr[i] < 0.0 || nuf[i] < 0.0) { rc = ORC_E_XS; goto done; }
+  for (long i = 0; i < (long)N * G * G; ++i)
+    if (sigs[i] < 0.0) { rc = ORC_E_XS; goto done; }
+done:
+  free(poff); free(xoff); free(roff); free(xs); free(rec); free(rr); free(xyz);
+  return rc;
+}

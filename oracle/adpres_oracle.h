@@ -101,6 +101,32 @@ int orc_xs_update(const orc_problem *pb, const orc_xs_tables *t, double bcon, co
                   const double *mtem, const double *cden, const double *bpos, double *sigtr,
                   double *siga, double *nuf, double *sigf, double *sigs, double *D, double *sigr);
 
+/* Branch tables of a %XTAB deck (sdata's m(:), src/mod_data.f90:176-192,
+   read by src/mod_io.f90), flattened.  dims (nmat,5) row-major: nd, nb, nf,
+   nm (points of coolant density, boron, fuel and moderator temperature)
+   and trod (1: rodded set present) per material; points: per material
+   pd, pb, pf, pm concatenated; xsec: per material its nd*nb*nf*nm branch
+   records (s fastest, then t, u, v) of E = 4G + G^2 + 6G doubles: sigtr,
+   siga, nuf, sigf (G), sigs (G,G) (from, to), dc (G,6); rxsec: the same for
+   the rodded sets of the trod = 1 materials. */
+typedef struct {
+  int nmat, nb;
+  const int *dims;
+  const double *points, *xsec, *rxsec;
+  double coreh, pos0, ssize;
+  const int *fbmap;                       /* (nxx,nyy) or NULL: no rods */
+} orc_xtab_tables;
+
+/* xsec::XStab_updt (src/mod_xsec.f90:48-84): brInterp (:520-788) per
+   node, crod_tab_updt (:291-376) with the same parameters, Dsigr_updt and
+   check_xs.  Outputs as orc_xs_update plus dc (N,G,6).  ORC_E_XS where the
+   reference would STOP (also a parameter outside a branch range, or a
+   rodded node of a material without rodded data). */
+int orc_xtab_update(const orc_problem *pb, const orc_xtab_tables *t, double bcon, const double *ftem,
+                    const double *mtem, const double *cden, const double *bpos, double *sigtr,
+                    double *siga, double *nuf, double *sigf, double *sigs, double *D, double *sigr,
+                    double *dc);
+
 /* Lxyz (src/mod_nodal.f90:901-1005) of the state: L(N,G,3). */
 int orc_leakage(const orc_problem *pb, orc_state *st, double *L);
 

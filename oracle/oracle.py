@@ -38,6 +38,12 @@ class OrcXsTables(C.Structure):
         [("tab", (_dp * 5) * 6), ("fbmap", _ip)]
 
 
+class OrcXtabTables(C.Structure):
+    _fields_ = [("nmat", C.c_int), ("nb", C.c_int), ("dims", _ip), ("points", _dp), ("xsec", _dp),
+                ("rxsec", _dp), ("coreh", C.c_double), ("pos0", C.c_double), ("ssize", C.c_double),
+                ("fbmap", _ip)]
+
+
 class OrcIter(C.Structure):
     _fields_ = [("nout", C.c_int), ("nin", C.c_int), ("nac", C.c_int), ("nupd", C.c_int),
                 ("kern", C.c_int), ("serc", C.c_double), ("ferc", C.c_double)]
@@ -77,6 +83,8 @@ def lib():
         L.orc_coup_coef.argtypes = [C.POINTER(OrcProblem), C.POINTER(OrcState)]
         L.orc_sanm_coef.argtypes = [C.c_double, C.c_double, C.c_double, _dp]
         L.orc_sanm_coef.restype = None
+        L.orc_xtab_update.argtypes = [C.POINTER(OrcProblem), C.POINTER(OrcXtabTables), C.c_double,
+                                      _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
         L.orc_xs_update.argtypes = [C.POINTER(OrcProblem), C.POINTER(OrcXsTables), C.c_double,
                                     _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
         _lib = L
@@ -242,4 +250,38 @@ def xs_update(p, tables, bcon, ftem=None, mtem=None, cden=None, bpos=None):
     out["sigs"] = np.zeros((G, G, N))
     rc = lib().orc_xs_update(C.byref(pb), C.byref(t), float(bcon), dp(ftem), dp(mtem), dp(cden), dp(bpos),
                              *[_d(out[k]) for k in ("sigtr", "siga", "nuf", "sigf", "sigs", "D", "sigr")])
+    return rc, out
+
+
+def xtab_update(p, tables, bcon, ftem, mtem, cden, bpos=None):
+    """xsec::XStab_updt (src/mod_xsec.f90:48-84) of Problem p's nodes with
+    branch tables in flat form (a dict: "nmat", "nb", "dims" (nmat, 5),
+    "points", "xsec", "rxsec" or None, "coreh", "pos0", "ssize", "fbmap"
+    (nyy, nxx) or None).  Returns (status, dict of sigtr, siga, nuf, sigf,
+    D, sigr as (G, N), sigs as (G_to, G_from, N) and dc as (6, G, N))."""
+    N, G = p.nnod, p.ng
+    keep = []
+
+    def dp(a):
+        if a is None:
+            return None
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        keep.append(a)
+        return _d(a)
+
+    def ip(a):
+        if a is None:
+            return None
+        a = np.ascontiguousarray(a, dtype=np.int32)
+        keep.append(a)
+        return _i(a)
+    t = OrcXtabTables(int(tables["nmat"]), int(tables.get("nb", 0)), ip(tables["dims"]), dp(tables["points"]),
+                      dp(tables["xsec"]), dp(tables.get("rxsec")), float(tables.get("coreh", 0.0)),
+                      float(tables.get("pos0", 0.0)), float(tables.get("ssize", 0.0)), ip(tables.get("fbmap")))
+    pb = Oracle(p).pb
+    out = {k: np.zeros((G, N)) for k in ("sigtr", "siga", "nuf", "sigf", "D", "sigr")}
+    out["sigs"] = np.zeros((G, G, N))
+    out["dc"] = np.zeros((6, G, N))
+    rc = lib().orc_xtab_update(C.byref(pb), C.byref(t), float(bcon), dp(ftem), dp(mtem), dp(cden), dp(bpos),
+                               *[_d(out[k]) for k in ("sigtr", "siga", "nuf", "sigf", "sigs", "D", "sigr", "dc")])
     return rc, out

@@ -13,6 +13,8 @@
 !   ref_dump DECK xsupd OUT -                    the cross-section tables and
 !                                               XS_updt at the deck's and at
 !                                               perturbed T-H / rod parameters
+!   ref_dump DECK xtab OUT -                     the same for a %XTAB deck: the
+!                                               branch tables and XStab_updt
 !
 ! The deck is read by the reference's own parser (io::inp_read, mod_io.f90:69),
 ! cross sections are built by xsec::XS_updt (mod_xsec.f90:11).  An optional
@@ -28,15 +30,15 @@
 program ref_dump
 
   use sdata
-  use io,    only: inp_read, bbcon, bcbcs, bftem, bmtem, bcden, bcrod
-  use xsec,  only: XS_updt, Dsigr_updt
+  use io,    only: inp_read, bbcon, bcbcs, bftem, bmtem, bcden, bcrod, bxtab
+  use xsec,  only: XS_updt, XStab_updt, Dsigr_updt
   use cmfd,  only: outer, outer_ad, outer_fs, outer_tr, PowDis, matrix_setup, bicg
   use nodal, only: nodal_update, nodal_update_pnm
 
   implicit none
 
   character(len=256) :: action, outname, xsname, stname
-  integer :: u, n, g, nargs, kid, nin_p, j
+  integer :: u, n, g, nargs, kid, nin_p, j, mm, s, t, q, v
   real(dp) :: kes, ht
   logical :: maxi
   real(dp), allocatable :: f0s(:,:), fs0s(:), dns(:,:,:)
@@ -56,7 +58,11 @@ program ref_dump
   if (nargs >= 5) call get_command_argument(5, stname)
 
   call inp_read()
-  call XS_updt(bcon, ftem, mtem, cden, bpos)
+  if (bxtab == 1) then
+    call XStab_updt(bcon, ftem, mtem, cden, bpos)
+  else
+    call XS_updt(bcon, ftem, mtem, cden, bpos)
+  end if
 
   if (trim(xsname) /= '-') then
     open(newunit=u, file=trim(xsname), access='stream', form='unformatted', &
@@ -189,6 +195,53 @@ program ref_dump
     call XS_updt(bcon, ftem, mtem, cden, bpos)
     call write_xs_state()
 
+  case ('xtab')
+    ! per material: branch dimensions, rod flag, branch points, then every
+    ! branch point's cross sections (and the rodded set), s fastest
+    write(u) int(bcrod, 4), int(nb, 4)
+    write(u) (int(m(mm)%nd, 4), int(m(mm)%nb, 4), int(m(mm)%nf, 4), int(m(mm)%nm, 4), &
+              int(m(mm)%trod, 4), mm = 1, nmat)
+    do mm = 1, nmat
+      write(u) m(mm)%pd, m(mm)%pb, m(mm)%pf, m(mm)%pm
+      do v = 1, m(mm)%nm
+        do q = 1, m(mm)%nf
+          do t = 1, m(mm)%nb
+            do s = 1, m(mm)%nd
+              write(u) m(mm)%xsec(s, t, q, v)%sigtr, m(mm)%xsec(s, t, q, v)%siga, &
+                       m(mm)%xsec(s, t, q, v)%nuf, m(mm)%xsec(s, t, q, v)%sigf, &
+                       m(mm)%xsec(s, t, q, v)%sigs, m(mm)%xsec(s, t, q, v)%dc
+            end do
+          end do
+        end do
+      end do
+      if (m(mm)%trod == 1) then
+        do v = 1, m(mm)%nm
+          do q = 1, m(mm)%nf
+            do t = 1, m(mm)%nb
+              do s = 1, m(mm)%nd
+                write(u) m(mm)%rxsec(s, t, q, v)%sigtr, m(mm)%rxsec(s, t, q, v)%siga, &
+                         m(mm)%rxsec(s, t, q, v)%nuf, m(mm)%rxsec(s, t, q, v)%sigf, &
+                         m(mm)%rxsec(s, t, q, v)%sigs, m(mm)%rxsec(s, t, q, v)%dc
+              end do
+            end do
+          end do
+        end do
+      end if
+    end do
+    write(u) coreh, pos0, ssize
+    if (bcrod == 1) write(u) (int(fbmap(:, n), 4), n = 1, nyy)
+    call write_xtab_state()
+    ! perturbed parameters inside the branch ranges, every bank part-way in
+    bcon = bcon + 37._dp
+    do n = 1, nnod
+      ftem(n) = ftem(n) * (1._dp + 0.05_dp * real(mod(n, 7) - 3, dp) / 3._dp)
+      mtem(n) = mtem(n) + 1.5_dp * real(mod(n, 5) - 2, dp)
+      cden(n) = cden(n) * (1._dp + 0.005_dp * real(mod(n, 3) - 1, dp))
+    end do
+    if (bcrod == 1) bpos = bpos * 0.5_dp + 17.25_dp
+    call XStab_updt(bcon, ftem, mtem, cden, bpos)
+    call write_xtab_state()
+
   case ('trstep')
     ! the steady state, then the set-up rod_eject and trans_calc make for a
     ! time step (src/mod_trans.f90:50-100, 400-416), with a rod-like
@@ -258,6 +311,13 @@ contains
     if (bcrod == 1) write(u) bpos
     write(u) sigtr, siga, nuf, sigf, sigs, D, sigr
   end subroutine write_xs_state
+
+  ! the parameters of an XStab_updt call and the node cross sections it made
+  subroutine write_xtab_state()
+    write(u) bcon, ftem, mtem, cden
+    if (bcrod == 1) write(u) bpos
+    write(u) sigtr, siga, nuf, sigf, sigs, D, sigr, dc
+  end subroutine write_xtab_state
 
   subroutine read_state()
     ! STATEFILE: int32 nin, float64 Ke, f0(N,G), dn(N,G,6), b(N,G)

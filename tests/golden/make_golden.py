@@ -406,6 +406,33 @@ def xsupd():
     save("neacrp_a1_xsupd", **problem_to_dict(p), **arrs)
 
 
+def xtab():
+    """xsec::XStab_updt on the MOX part-3 HELIOS deck (branch tables with
+    one to three points per parameter, materials with and without rodded
+    sets, eight rod banks; ref_dump xtab): the flat branch tables, and the
+    node cross sections and discontinuity factors at the deck's parameters
+    and at perturbed node-wise parameters with every bank part-way in.  The
+    first state's outputs are the problem's own arrays."""
+    src = os.path.join(HERE, "decks", "mox")
+    with tempfile.TemporaryDirectory() as d:
+        for fn in os.listdir(src):
+            shutil.copy(os.path.join(src, fn), os.path.join(d, fn))
+        shutil.copy(os.path.join(src, "part3_helios"), os.path.join(d, "deck"))
+        p, res, out = run(d, "deck", "xtab")
+    p.name = "mox_part3_helios_xtab"
+    b = res["branch"]
+    arrs = {"b_" + k: b[k] for k in ("nmat", "nb", "dims", "points", "xsec", "rxsec", "coreh", "pos0", "ssize",
+                                     "fbmap")}
+    s0, s1 = res["states"]
+    for i, st in enumerate((s0, s1)):
+        for k in ("bcon", "ftem", "mtem", "cden", "bpos"):
+            arrs[f"s{i}_{k}"] = st[k]
+    for k in ("sigtr", "siga", "nuf", "sigf", "sigs", "D", "sigr", "dc"):
+        assert np.array_equal(s0[k], getattr(p, k)), k
+        arrs["s1_" + k] = s1[k]
+    save("mox_part3_helios_xtab", **problem_to_dict(p), **arrs)
+
+
 def adjoint_fdm(seed=1):
     """The 34x34x26-node 2-group core with the finite-difference kernel,
     solved by the reference in %MODE FORWARD and %MODE ADJOINT (outer_ad)
@@ -451,6 +478,8 @@ def main(names):
         fwdadj()
     if not names or "xsupd" in names:
         xsupd()
+    if not names or "xtab" in names:
+        xtab()
     if "batch" in names:          # 256 reference solves on 7 host cores
         batch()
     if "core3d_8g" in names:      # about an hour of reference CPU time

@@ -17,8 +17,8 @@ import scipy.linalg
 
 from conftest import golden, probe_fixtures, solve_fixtures
 from adpres_amd.configs import generate, kinf
-from adpres_amd.problem import (Problem, build_geometry, node_xs_from_materials, problem_from_dict,
-                                xs_tables_from_dict)
+from adpres_amd.problem import (Problem, branch_tables_from_dict, build_geometry, node_xs_from_materials,
+                                problem_from_dict, xs_tables_from_dict)
 
 
 @pytest.mark.parametrize("name", [n for n in solve_fixtures()])
@@ -416,3 +416,86 @@ def test_oracle_xs_update_rod_geometry(oracle_lib):
     assert np.array_equal(out["sigtr"][:, un], out0["sigtr"][:, un])
     assert np.array_equal(out["siga"][:, un], out0["siga"][:, un])
     assert np.all(out0["siga"] > 0)
+
+
+XTAB_OUT = XS_OUT + ("dc",)
+
+
+def test_oracle_xtab_update_bit_exact(oracle_lib):
+    """xsec::XStab_updt (src/mod_xsec.f90:48-84: brInterp per node,
+    crod_tab_updt, Dsigr_updt) on the MOX part-3 HELIOS deck: bit for bit
+    the reference's node cross sections and discontinuity factors at the
+    deck's parameters and at perturbed node-wise parameters with every bank
+    part-way in."""
+    d = golden("mox_part3_helios_xtab")
+    p = problem_from_dict(d)
+    t = branch_tables_from_dict(d)
+    assert t["rxsec"] is not None and t["fbmap"] is not None
+    for i in (0, 1):
+        rc, out = oracle_lib.xtab_update(p, t, float(d[f"s{i}_bcon"]), d[f"s{i}_ftem"], d[f"s{i}_mtem"],
+                                         d[f"s{i}_cden"], d[f"s{i}_bpos"])
+        assert rc == 0
+        for k in XTAB_OUT:
+            ref = getattr(p, k) if i == 0 else d["s1_" + k]
+            assert np.array_equal(out[k], ref), (i, k)
+
+
+def linear_branch_tables(p, seed=3):
+    """Branch tables (3 points per parameter, every material) whose records
+    are an affine function of the four branch parameters, no rods."""
+    rng = np.random.default_rng(seed)
+    G, M = p.ng, p.nmat
+    E = 4 * G + G * G + 6 * G
+    pts = [np.array([0.60, 0.70, 0.80]), np.array([0.0, 1000.0, 2000.0]), np.array([500.0, 900.0, 1300.0]),
+           np.array([550.0, 570.0, 590.0])]
+    c = rng.uniform(0.5, 1.5, (M, 5, E)) * np.array([1.0, 0.1, 1e-5, 1e-4, 1e-3])[None, :, None]
+    xsec = []
+    for m in range(M):
+        s, t, u, v = np.meshgrid(*pts, indexing="ij")              # (nd, nb, nf, nm)
+        rec = (c[m, 0][None, None, None, None, :] + c[m, 1] * s[..., None] + c[m, 2] * t[..., None]
+               + c[m, 3] * u[..., None] + c[m, 4] * v[..., None])
+        xsec.append(np.transpose(rec, (3, 2, 1, 0, 4)).ravel())     # s fastest
+    tab = {"nmat": M, "nb": 0, "dims": np.tile(np.array([3, 3, 3, 3, 0], np.int32), (M, 1)),
+           "points": np.concatenate([np.concatenate(pts)] * M), "xsec": np.concatenate(xsec), "rxsec": None,
+           "coreh": 0.0, "pos0": 0.0, "ssize": 0.0, "fbmap": None}
+    return tab, c, pts
+
+
+def test_oracle_xtab_interpolation_is_multilinear(oracle_lib):
+    """What the branch interpolation must give whatever its implementation
+    (every parameter with three points, the moderator-temperature stage
+    included, which the sample decks do not branch on): on tables affine in
+    the four parameters the result is that affine function, inside the
+    ranges and in the allowed extrapolation (20 % of the end point, 100 ppm
+    of boron); at the first branch point of every parameter it is the
+    table's record bit for bit; beyond the allowed extrapolation the
+    reference stops."""
+    d = golden("mox_part3_helios_xtab")
+    p = problem_from_dict(d)
+    tab, c, pts = linear_branch_tables(p)
+    N, G = p.nnod, p.ng
+    rng = np.random.default_rng(4)
+    cd = rng.uniform(0.55, 0.85, N)          # 0.55: 8 % below the first point
+    bc = 1450.0
+    ft = rng.uniform(450.0, 1400.0, N)
+    mt = rng.uniform(545.0, 600.0, N)
+    rc, out = oracle_lib.xtab_update(p, tab, bc, ft, mt, cd)
+    assert rc == 0
+    m = p.mat - 1
+    E = 4 * G + G * G + 6 * G
+    lin = (c[m, 0] + c[m, 1] * cd[:, None] + c[m, 2] * bc + c[m, 3] * ft[:, None] + c[m, 4] * mt[:, None])   # (N, E)
+    assert np.allclose(out["sigtr"], lin[:, :G].T, rtol=1e-12, atol=0)
+    assert np.allclose(out["siga"], lin[:, G:2 * G].T, rtol=1e-12, atol=0)
+    assert np.allclose(out["sigs"], lin[:, 4 * G:4 * G + G * G].T.reshape(G, G, N), rtol=1e-12, atol=0)
+    assert np.allclose(out["dc"], lin[:, 4 * G + G * G:E].T.reshape(6, G, N), rtol=1e-12, atol=0)
+    # at the first point of every grid: the record itself
+    one = np.ones(N)
+    rc, out = oracle_lib.xtab_update(p, tab, 0.0, 500.0 * one, 550.0 * one, 0.60 * one)
+    assert rc == 0
+    rec0 = tab["xsec"].reshape(p.nmat, 81, E)[m, 0]
+    assert np.array_equal(out["sigtr"], rec0[:, :G].T) and np.array_equal(out["nuf"], rec0[:, 2 * G:3 * G].T)
+    # beyond the extrapolation the reference stops
+    rc, _ = oracle_lib.xtab_update(p, tab, 2150.0, ft, mt, cd)
+    assert rc == 6
+    rc, _ = oracle_lib.xtab_update(p, tab, bc, ft, mt, cd * 0.7)
+    assert rc == 6
