@@ -85,6 +85,10 @@ struct CoreHost {
   int *xnode = nullptr;            // (2N) material - 1, then bank | (level << 16) (0: no rod)
   int xon = 0, xnb = 0, xnmat = 0;
   double xref[7] = {0, 0, 0, 0, 0, 0, 0};   // rbcon rftem rmtem rcden coreh pos0 ssize
+  // branch tables of adg_set_xtab_tables (xkind 2; 1: feedback tables)
+  int xkind = 0;
+  BrMat *xmeta = nullptr;
+  long xpts = 0, xrec = 0, xrrec = 0;   // doubles of points, records, rodded records in xtab
 };
 
 }  // namespace
@@ -1071,6 +1075,7 @@ void adg_destroy(adg_ctx *c) {
   for (CoreHost &h : c->core) {
     if (h.xtab) (void)hipFree(h.xtab);
     if (h.xnode) (void)hipFree(h.xnode);
+    if (h.xmeta) (void)hipFree(h.xmeta);
   }
   if (c->xerr) (void)hipFree(c->xerr);
   for (hipEvent_t e : c->xev)
@@ -1121,6 +1126,46 @@ int adg_set_xs_core(adg_ctx *c, int core, const double *D, const double *sigr, c
   return ADG_OK;
 }
 
+// node material map, each node's rod column/level and the levels' running
+// depth from the top (crod_updt's and crod_tab_updt's walk), for the table
+// forms of the cross-section update; "" or the reason the input is rejected
+std::string xs_node_setup(const CoreHost &h, const int *mat, int M, const int *fbmap, int nb,
+                          std::vector<int> &node, std::vector<double> &zdz) {
+  const int N = h.N;
+  const bool rod = fbmap != nullptr;
+  for (int n = 0; n < N; ++n)
+    if (mat[n] < 1 || mat[n] > M) return "material index out of range";
+  if (rod && (nb < 1 || nb > 0xffff || h.nzz > 0x7fff)) return "rod tables need fbmap and 1..65535 banks";
+  zdz.assign(2L * h.nzz, 0.0);
+  double dum = 0.0;
+  for (int k = h.nzz - 1; k >= 0; --k) {
+    zdz[k] = dum;
+    zdz[h.nzz + k] = h.hz[k];
+    dum = dum + h.hz[k];
+  }
+  node.assign(2L * N, 0);
+  std::vector<int> colcount;
+  if (rod) colcount.assign((size_t)h.nxx * h.nyy, 0);
+  for (int n = 0; n < N; ++n) {
+    node[n] = mat[n] - 1;
+    if (!rod) continue;
+    const long cc = h.cell[n] - h.base;
+    const int i = (int)(cc % h.NXP) - 1, j = (int)((cc / h.NXP) % h.NYP) - 1, k = (int)(cc / ((long)h.NXP * h.NYP)) - 1;
+    const int b = fbmap[i + h.nxx * j];
+    if (b < 0 || b > nb) return "fbmap bank out of range";
+    if (b) {
+      node[N + n] = b | (k << 16);
+      ++colcount[i + h.nxx * j];
+    }
+  }
+  if (rod)
+    for (int j = 0; j < h.nyy; ++j)
+      for (int i = 0; i < h.nxx; ++i)
+        if (fbmap[i + h.nxx * j] > 0 && colcount[i + h.nxx * j] != h.nzz)
+          return "rodded column without a node at every level";
+  return "";
+}
+
 int adg_set_xs_tables_core(adg_ctx *c, int core, const adg_xs_tables *t, const int *mat, const double *chi,
                            const double *dc) {
   if (!core_ok(c, core)) return ADG_E_ARG;
@@ -1137,11 +1182,12 @@ int adg_set_xs_tables_core(adg_ctx *c, int core, const adg_xs_tables *t, const i
       if (!t->tab[kd][q]) return fail(c, ADG_E_ARG, "incomplete feedback table");
     on |= 1 << kd;
   }
-  for (int n = 0; n < N; ++n)
-    if (mat[n] < 1 || mat[n] > M) return fail(c, ADG_E_ARG, "material index out of range");
   const bool rod = on & (1 << XK_ROD);
-  if (rod && (!t->fbmap || t->nb < 1 || t->nb > 0xffff || h.nzz > 0x7fff))
-    return fail(c, ADG_E_ARG, "rod tables need fbmap and 1..65535 banks");
+  if (rod && !t->fbmap) return fail(c, ADG_E_ARG, "rod tables need fbmap");
+  std::vector<int> node;
+  std::vector<double> zdz;
+  const std::string e = xs_node_setup(h, mat, M, rod ? t->fbmap : nullptr, t->nb, node, zdz);
+  if (!e.empty()) return fail(c, ADG_E_ARG, e);
   try {
     const long K = (4L * G + (long)G * G) * M;
     std::vector<double> tab(6 * K + 2L * h.nzz, 0.0);
@@ -1151,33 +1197,7 @@ int adg_set_xs_tables_core(adg_ctx *c, int core, const adg_xs_tables *t, const i
         std::memcpy(&tab[kd * K + (long)q * G * M], t->tab[kd][q], sizeof(double) * G * M);
       std::memcpy(&tab[kd * K + 4L * G * M], t->tab[kd][4], sizeof(double) * G * G * M);
     }
-    // crod_updt's running depth from the core top, summed in its order
-    double dum = 0.0;
-    for (int k = h.nzz - 1; k >= 0; --k) {
-      tab[6 * K + k] = dum;
-      tab[6 * K + h.nzz + k] = h.hz[k];
-      dum = dum + h.hz[k];
-    }
-    std::vector<int> node(2L * N, 0);
-    std::vector<int> colcount;
-    if (rod) colcount.assign((size_t)h.nxx * h.nyy, 0);
-    for (int n = 0; n < N; ++n) {
-      node[n] = mat[n] - 1;
-      if (!rod) continue;
-      const long cc = h.cell[n] - h.base;
-      const int i = (int)(cc % h.NXP) - 1, j = (int)((cc / h.NXP) % h.NYP) - 1, k = (int)(cc / ((long)h.NXP * h.NYP)) - 1;
-      const int b = t->fbmap[i + h.nxx * j];
-      if (b < 0 || b > t->nb) return fail(c, ADG_E_ARG, "fbmap bank out of range");
-      if (b) {
-        node[N + n] = b | (k << 16);
-        ++colcount[i + h.nxx * j];
-      }
-    }
-    if (rod)
-      for (int j = 0; j < h.nyy; ++j)
-        for (int i = 0; i < h.nxx; ++i)
-          if (t->fbmap[i + h.nxx * j] > 0 && colcount[i + h.nxx * j] != h.nzz)
-            return fail(c, ADG_E_ARG, "rodded column without a node at every level");
+    std::memcpy(&tab[6 * K], zdz.data(), sizeof(double) * zdz.size());
     if (h.xtab) HIPCHK(hipFree(h.xtab));
     h.xtab = nullptr;
     if (h.xnode) HIPCHK(hipFree(h.xnode));
@@ -1192,6 +1212,7 @@ int adg_set_xs_tables_core(adg_ctx *c, int core, const adg_xs_tables *t, const i
     h.xon = on;
     h.xnb = rod ? t->nb : 0;
     h.xnmat = M;
+    h.xkind = 1;
     const double refs[7] = {t->rbcon, t->rftem, t->rmtem, t->rcden, t->coreh, t->pos0, t->ssize};
     std::memcpy(h.xref, refs, sizeof refs);
     // what adg_set_xs takes besides the node cross sections
@@ -1214,6 +1235,80 @@ int adg_set_xs_tables(adg_ctx *c, const adg_xs_tables *t, const int *mat, const 
   return adg_set_xs_tables_core(c, 0, t, mat, chi, dc);
 }
 
+int adg_set_xtab_tables_core(adg_ctx *c, int core, const adg_xtab_tables *t, const int *mat, const double *chi) {
+  if (!core_ok(c, core)) return ADG_E_ARG;
+  if (!t || !mat || !chi || !t->dims || !t->points || !t->xsec) return fail(c, ADG_E_ARG, "null argument");
+  CoreHost &h = c->core[core];
+  const int N = h.N, G = c->G, M = t->nmat, E = 4 * G + G * G + 6 * G;
+  if (M != h.nmat) return fail(c, ADG_E_ARG, "table material count differs from the geometry's");
+  const bool rod = t->fbmap != nullptr;
+  std::vector<BrMat> meta(M);
+  long np = 0, nx = 0, nr = 0;
+  for (int m = 0; m < M; ++m) {
+    const int *dm = t->dims + 5 * m;
+    for (int q = 0; q < 4; ++q)
+      if (dm[q] < 1 || dm[q] > 64) return fail(c, ADG_E_ARG, "branch dimension outside 1..64");
+    const long nrec = (long)dm[0] * dm[1] * dm[2] * dm[3];
+    meta[m] = BrMat{dm[0], dm[1], dm[2], dm[3], dm[4], 0, np, nx, dm[4] == 1 ? nr : 0};
+    np += dm[0] + dm[1] + dm[2] + dm[3];
+    nx += nrec * E;
+    if (dm[4] == 1) nr += nrec * E;
+  }
+  if (rod && nr > 0 && !t->rxsec) return fail(c, ADG_E_ARG, "rodded records missing");
+  std::vector<int> node;
+  std::vector<double> zdz;
+  const std::string e = xs_node_setup(h, mat, M, t->fbmap, t->nb, node, zdz);
+  if (!e.empty()) return fail(c, ADG_E_ARG, e);
+  try {
+    // device block: points, records, rodded records, zdum, zdel
+    std::vector<double> blk(np + nx + nr + zdz.size());
+    std::memcpy(blk.data(), t->points, sizeof(double) * np);
+    std::memcpy(blk.data() + np, t->xsec, sizeof(double) * nx);
+    if (nr) std::memcpy(blk.data() + np + nx, t->rxsec, sizeof(double) * nr);
+    std::memcpy(blk.data() + np + nx + nr, zdz.data(), sizeof(double) * zdz.size());
+    if (h.xtab) HIPCHK(hipFree(h.xtab));
+    h.xtab = nullptr;
+    if (h.xnode) HIPCHK(hipFree(h.xnode));
+    h.xnode = nullptr;
+    if (h.xmeta) HIPCHK(hipFree(h.xmeta));
+    h.xmeta = nullptr;
+    h.xtab = dalloc<double>(c->s, blk.size());
+    scopy(c->s, h.xtab, blk.data(), blk.size() * sizeof(double), hipMemcpyHostToDevice);
+    h.xnode = dalloc<int>(c->s, node.size());
+    scopy(c->s, h.xnode, node.data(), node.size() * sizeof(int), hipMemcpyHostToDevice);
+    h.xmeta = dalloc<BrMat>(c->s, meta.size());
+    scopy(c->s, h.xmeta, meta.data(), meta.size() * sizeof(BrMat), hipMemcpyHostToDevice);
+    if (!c->xerr) c->xerr = dalloc<int>(c->s, 1);
+    for (hipEvent_t &ev : c->xev)
+      if (!ev) HIPCHK(hipEventCreate(&ev));
+    h.xkind = 2;
+    h.xpts = np;
+    h.xrec = nx;
+    h.xrrec = nr;
+    h.xnb = rod ? t->nb : 0;
+    h.xnmat = M;
+    h.xon = (1 << XK_FTEM) | (1 << XK_MTEM) | (1 << XK_CDEN) | (rod ? (1 << XK_ROD) : 0);
+    h.xref[4] = t->coreh;
+    h.xref[5] = t->pos0;
+    h.xref[6] = t->ssize;
+    std::vector<double> st;
+    std::vector<double> ch((size_t)G * N);
+    for (int g = 0; g < G; ++g)
+      for (int n = 0; n < N; ++n) ch[(size_t)g * N + n] = chi[(size_t)g * M + (mat[n] - 1)];
+    scatter(c, core, ch.data(), G, c->chi, st);
+    std::vector<double> z((size_t)G * N, 0.0);
+    scatter(c, core, z.data(), G, c->exsrc, st);
+    h.xs_set = false;
+  } catch (const HipError &ex) {
+    return hip_fail(c, ex);
+  }
+  return ADG_OK;
+}
+
+int adg_set_xtab_tables(adg_ctx *c, const adg_xtab_tables *t, const int *mat, const double *chi) {
+  return adg_set_xtab_tables_core(c, 0, t, mat, chi);
+}
+
 int adg_update_xs_core(adg_ctx *c, int core, double bcon, const double *ftem, const double *mtem,
                        const double *cden, const double *bpos) {
   if (!core_ok(c, core)) return ADG_E_ARG;
@@ -1232,6 +1327,38 @@ int adg_update_xs_core(adg_ctx *c, int core, double bcon, const double *ftem, co
         HIPCHK(hipMemcpyAsync(c->dstage + (long)q * N, in[q], sizeof(double) * (q < 3 ? N : h.xnb),
                               hipMemcpyHostToDevice, c->s));
     HIPCHK(hipMemsetAsync(c->xerr, 0, sizeof(int), c->s));
+    if (h.xkind == 2) {
+      XtabArgs b;
+      b.mats = h.xmeta;
+      b.points = h.xtab;
+      b.xsec = h.xtab + h.xpts;
+      b.rxsec = h.xtab + h.xpts + h.xrec;
+      b.zdum = h.xtab + h.xpts + h.xrec + h.xrrec;
+      b.zdel = b.zdum + h.nzz;
+      b.mat0 = h.xnode;
+      b.rodk = h.xnode + N;
+      b.ftem = c->dstage;
+      b.mtem = c->dstage + N;
+      b.cden = c->dstage + 2L * N;
+      b.bpos = in[3] ? c->dstage + 3L * N : nullptr;
+      b.cell = c->d_cell + h.coff;
+      b.D = c->D; b.sigr = c->sigr; b.nuf = c->nuf; b.sigf = c->sigf; b.sigs = c->sigs; b.dc = c->dc;
+      b.err = c->xerr;
+      b.NC = c->NC;
+      b.N = N; b.G = c->G; b.nzz = h.nzz; b.rods = (on >> XK_ROD) & 1;
+      b.bcon = bcon;
+      b.rod0 = h.xref[4] - h.xref[5];
+      b.ssize = h.xref[6];
+      const dim3 grid((N + 255) / 256), blk(256);
+      HIPCHK(hipEventRecord(c->xev[0], c->s));
+      switch (c->G) {
+        case 2: hipLaunchKernelGGL(k_xtab_update<2>, grid, blk, 0, c->s, b); break;
+        case 4: hipLaunchKernelGGL(k_xtab_update<4>, grid, blk, 0, c->s, b); break;
+        default: hipLaunchKernelGGL(k_xtab_update<0>, grid, blk, 0, c->s, b); break;
+      }
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipEventRecord(c->xev[1], c->s));
+    } else {
     XsArgs a;
     const long K = (4L * c->G + (long)c->G * c->G) * h.xnmat;
     a.tab = h.xtab;
@@ -1274,6 +1401,7 @@ int adg_update_xs_core(adg_ctx *c, int core, double bcon, const double *ftem, co
 #undef XS_LAUNCH
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(c->xev[1], c->s));
+    }
     int err = 0;
     HIPCHK(hipMemcpyAsync(&err, c->xerr, sizeof(int), hipMemcpyDeviceToHost, c->s));
     HIPCHK(hipStreamSynchronize(c->s));
@@ -1298,7 +1426,8 @@ int adg_update_xs(adg_ctx *c, double bcon, const double *ftem, const double *mte
   return adg_update_xs_core(c, 0, bcon, ftem, mtem, cden, bpos);
 }
 
-int adg_get_xs_core(adg_ctx *c, int core, double *D, double *sigr, double *nuf, double *sigf, double *sigs) {
+int adg_get_xs_core(adg_ctx *c, int core, double *D, double *sigr, double *nuf, double *sigf, double *sigs,
+                    double *dc) {
   if (!core_ok(c, core)) return ADG_E_ARG;
   if (!c->core[core].xs_set) return fail(c, ADG_E_NOT_SET, "no cross sections set");
   try {
@@ -1309,14 +1438,15 @@ int adg_get_xs_core(adg_ctx *c, int core, double *D, double *sigr, double *nuf, 
     if (nuf) gather(c, core, c->nuf, G, nuf, st);
     if (sigf) gather(c, core, c->sigf, G, sigf, st);
     if (sigs) gather(c, core, c->sigs, G * G, sigs, st);
+    if (dc) gather(c, core, c->dc, 6 * G, dc, st);
   } catch (const HipError &e) {
     return hip_fail(c, e);
   }
   return ADG_OK;
 }
 
-int adg_get_xs(adg_ctx *c, double *D, double *sigr, double *nuf, double *sigf, double *sigs) {
-  return adg_get_xs_core(c, 0, D, sigr, nuf, sigf, sigs);
+int adg_get_xs(adg_ctx *c, double *D, double *sigr, double *nuf, double *sigf, double *sigs, double *dc) {
+  return adg_get_xs_core(c, 0, D, sigr, nuf, sigf, sigs, dc);
 }
 
 int adg_set_xs(adg_ctx *c, const double *D, const double *sigr, const double *sigs, const double *nuf,

@@ -173,4 +173,201 @@ __global__ void __launch_bounds__(512) k_xs_update_lds(XsArgs a) {
     xs_node<GG>(a, n, sh, ks);
 }
 
+// ---------------------------------------------------------------------------
+// Branch tables (%XTAB decks): xsec::XStab_updt (src/mod_xsec.f90:48-84),
+// brInterp (:520-788) per node and crod_tab_updt (:291-376).  Each material
+// holds nd x nb x nf x nm branch records (coolant density, boron, fuel and
+// moderator temperature points) of E = 4G + G^2 + 6G doubles (sigtr, siga,
+// nuf, sigf, sigs (from, to), dc (g, face)).  A node brackets its four
+// parameters once, then every entry runs brInterp's chain on the eight
+// corner records: moderator stage on all eight, fuel on four pairs, boron
+// on two, density on one, in the reference's order; rodded nodes blend the
+// rodded set's result in as crod_tab_updt does.
+// ---------------------------------------------------------------------------
+
+struct BrMat {
+  int nd, nb, nf, nm, trod, pad;
+  long poff, xoff, roff;   // offsets of its points, records and rodded records
+};
+
+struct XtabArgs {
+  const BrMat *mats;
+  const double *points, *xsec, *rxsec;
+  const double *zdum, *zdel;
+  const int *mat0, *rodk;
+  const double *ftem, *mtem, *cden, *bpos;
+  const long *cell;
+  double *D, *sigr, *nuf, *sigf, *sigs, *dc;
+  int *err;
+  long NC;
+  int N, G, nzz, rods;
+  double bcon, rod0, ssize;
+};
+
+// brInterp's bracket search (:553-660): inside the range the first interval
+// holding x; outside, the end interval when x is within 20 % (rel) or
+// 100 ppm of the end point (default-real literals there); else false
+__device__ __forceinline__ bool br_bracket(const double *p, int n, double x, bool rel, int &i1, int &i2) {
+#pragma clang fp contract(off)
+  i1 = 0;
+  i2 = 0;
+  if (n <= 1) return true;
+  if (x >= p[0] && x <= p[n - 1]) {
+    for (int s = 1; s < n; ++s)
+      if (x >= p[s - 1] && x <= p[s]) {
+        i1 = s - 1;
+        i2 = s;
+        break;
+      }
+    return true;
+  }
+  if (x < p[0] && (rel ? (p[0] - x) / p[0] < (double)0.2f : (p[0] - x) < (double)100.f)) {
+    i1 = 0;
+    i2 = 1;
+    return true;
+  }
+  if (x > p[n - 1] && (rel ? (x - p[n - 1]) / p[n - 1] < (double)0.2f : (x - p[n - 1]) < (double)100.f)) {
+    i1 = n - 2;
+    i2 = n - 1;
+    return true;
+  }
+  return false;
+}
+
+struct BrPos {
+  long o[8];      // corner records at v1 (doubles from the material's first record)
+  long dv;        // v2 record - v1 record
+  double rm, rf, rb, rd;
+  int nd, nb, nf, nm;
+};
+
+__device__ __forceinline__ bool br_setup(const XtabArgs &a, const BrMat &mm, int E, double cd, double bc, double ft,
+                                         double mt, BrPos &P) {
+#pragma clang fp contract(off)
+  const double *pd = a.points + mm.poff, *pb = pd + mm.nd, *pf = pb + mm.nb, *pm = pf + mm.nf;
+  int s1, s2, t1, t2, u1, u2, v1, v2;
+  bool ok = br_bracket(pd, mm.nd, cd, true, s1, s2);
+  ok &= br_bracket(pb, mm.nb, bc, false, t1, t2);
+  ok &= br_bracket(pf, mm.nf, ft, true, u1, u2);
+  ok &= br_bracket(pm, mm.nm, mt, true, v1, v2);
+  if (!ok) return false;
+  P.nd = mm.nd; P.nb = mm.nb; P.nf = mm.nf; P.nm = mm.nm;
+  P.rm = mm.nm > 1 ? (mt - pm[v1]) / (pm[v2] - pm[v1]) : 0.0;
+  P.rf = mm.nf > 1 ? (ft - pf[u1]) / (pf[u2] - pf[u1]) : 0.0;
+  P.rb = mm.nb > 1 ? (bc - pb[t1]) / (pb[t2] - pb[t1]) : 0.0;
+  P.rd = mm.nd > 1 ? (cd - pd[s1]) / (pd[s2] - pd[s1]) : 0.0;
+  const long snb = (long)mm.nd * mm.nb, snf = snb * mm.nf;
+  const int cs[8] = {s1, s1, s1, s1, s2, s2, s2, s2}, ct[8] = {t1, t1, t2, t2, t1, t1, t2, t2};
+  const int cu[8] = {u1, u2, u1, u2, u1, u2, u1, u2};
+#pragma unroll
+  for (int c = 0; c < 8; ++c) P.o[c] = (long)E * (cs[c] + (long)mm.nd * ct[c] + snb * cu[c] + snf * v1);
+  P.dv = (long)E * snf * (v2 - v1);
+  return true;
+}
+
+// one entry e of brInterp's chain on table block T
+__device__ __forceinline__ double br_elem(const double *T, const BrPos &P, int e) {
+#pragma clang fp contract(off)
+  double c[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const double A = T[P.o[k] + e];
+    if (P.nm > 1) {
+      const double B = T[P.o[k] + P.dv + e];
+      c[k] = A + P.rm * (B - A);
+    } else {
+      c[k] = A;
+    }
+  }
+  if (P.nf > 1) {
+#pragma unroll
+    for (int k = 0; k < 8; k += 2) c[k] = c[k] + P.rf * (c[k + 1] - c[k]);
+  }
+  if (P.nb > 1) {
+    c[0] = c[0] + P.rb * (c[2] - c[0]);
+    c[4] = c[4] + P.rb * (c[6] - c[4]);
+  }
+  if (P.nd > 1) c[0] = c[0] + P.rd * (c[4] - c[0]);
+  return c[0];
+}
+
+template <int GG>
+__global__ void __launch_bounds__(256) k_xtab_update(XtabArgs a) {
+#pragma clang fp contract(off)
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= a.N) return;
+  const int G = GG > 0 ? GG : a.G, E = 4 * G + G * G + 6 * G;
+  const BrMat mm = a.mats[a.mat0[n]];
+  const long cell = a.cell[n];
+  BrPos P;
+  if (!br_setup(a, mm, E, a.cden[n], a.bcon, a.ftem[n], a.mtem[n], P)) {
+    *a.err = 1;
+    return;
+  }
+  // rod state as in k_xs_update (crod_tab_updt walks the column the same way)
+  int rmode = 0;
+  double vf = 0.0;
+  bool incol = false;
+  if (a.rods && a.rodk[n]) {
+    incol = true;
+    const int b = a.rodk[n] & 0xffff, k = a.rodk[n] >> 16;
+    const double rodh = a.rod0 - a.bpos[b - 1] * a.ssize;
+    const bool above = k < a.nzz - 1 && rodh >= 0.0 && rodh <= a.zdum[k];
+    if (!above) {
+      if (rodh >= a.zdum[k] && rodh <= a.zdum[k] + a.zdel[k]) {
+        rmode = 2;
+        vf = (rodh - a.zdum[k]) / a.zdel[k];
+      } else {
+        rmode = 1;
+      }
+    }
+  }
+  if (rmode && mm.trod != 1) {     // a rodded node of a material without rodded data
+    *a.err = 1;
+    return;
+  }
+  const double *TX = a.xsec + mm.xoff, *TR = a.rxsec + (rmode ? mm.roff : 0);
+  auto val = [&](int e) {
+    double x = br_elem(TX, P, e);
+    if (rmode) {
+      const double r = br_elem(TR, P, e);
+      x = rmode == 2 ? (1.0 - vf) * x + vf * r : r;
+    }
+    return x;
+  };
+  bool bad = false;
+  for (int g = 0; g < G; ++g) {
+    const double st = val(g);
+    double sa = val(G + g), nf = val(2 * G + g), sf = val(3 * G + g);
+    if (incol) {
+      if (sa < 0.0) sa = 0.0;
+      if (nf < 0.0) nf = 0.0;
+      if (sf < 0.0) sf = 0.0;
+    }
+    double dum = 0.0;
+    for (int h = 0; h < G; ++h) {
+      double v = val(4 * G + g + G * h);
+      if (incol && v < 0.0) v = 0.0;
+      bad |= v < 0.0;
+      a.sigs[(long)(h * G + g) * a.NC + cell] = v;
+      if (h != g) dum = dum + v;
+    }
+    bad |= st < (double)1.e-5f;
+    const double D = 1.0 / (3.0 * st);
+    const double sr = sa + dum;
+    bad |= D < (double)1.e-20f || sr < 0.0 || nf < 0.0;
+    a.D[g * a.NC + cell] = D;
+    a.sigr[g * a.NC + cell] = sr;
+    a.nuf[g * a.NC + cell] = nf;
+    a.sigf[g * a.NC + cell] = sf;
+  }
+  for (int f = 0; f < 6; ++f)
+    for (int g = 0; g < G; ++g) {
+      double v = val(4 * G + G * G + g + G * f);
+      if (incol && v < 0.0) v = 0.0;
+      a.dc[(long)(f * G + g) * a.NC + cell] = v;
+    }
+  if (bad) *a.err = 1;
+}
+
 }  // namespace adg

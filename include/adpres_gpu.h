@@ -135,6 +135,31 @@ typedef struct adg_xs_tables {
  * set to zero).  The node cross sections are then made by adg_update_xs. */
 int adg_set_xs_tables(adg_ctx *ctx, const adg_xs_tables *t, const int *mat, const double *chi,
                       const double *dc);
+/* Branch tables of a %XTAB deck (sdata's m(:), src/mod_data.f90:176-192,
+ * read by src/mod_io.f90), flattened.  dims (nmat,5) row-major: nd, nb, nf,
+ * nm (branch points of coolant density, boron, fuel and moderator
+ * temperature) and trod (1: rodded set present) per material; points: per
+ * material pd, pb, pf, pm concatenated; xsec: per material its
+ * nd*nb*nf*nm branch records (density fastest, then boron, fuel, moderator)
+ * of E = 4G + G^2 + 6G doubles: sigtr, siga, nuf, sigf (G), sigs (G,G)
+ * (from, to), dc (G,6); rxsec: the same for the rodded sets of the trod = 1
+ * materials (NULL without); fbmap (nxx,nyy) or NULL without rods. */
+typedef struct adg_xtab_tables {
+  int nmat, nb;
+  const int *dims;
+  const double *points, *xsec, *rxsec;
+  double coreh, pos0, ssize;
+  const int *fbmap;
+} adg_xtab_tables;
+
+/* Installs branch tables instead of feedback tables: adg_update_xs then
+ * runs xsec::XStab_updt (src/mod_xsec.f90:48-84: brInterp per node,
+ * :520-788, and crod_tab_updt, :291-376, with the same parameters) and
+ * also makes the node discontinuity factors dc; ADG_E_XS also where a
+ * parameter leaves a branch range by more than brInterp extrapolates or a
+ * rodded node's material has no rodded set. */
+int adg_set_xtab_tables(adg_ctx *ctx, const adg_xtab_tables *t, const int *mat, const double *chi);
+
 /* xsec::XS_updt (src/mod_xsec.f90:11-46) on the device: the node D, sigr,
  * nuf, sigf and sigs for boron concentration bcon, node fuel temperatures
  * ftem(N), moderator temperatures mtem(N), coolant densities cden(N) and
@@ -147,8 +172,8 @@ int adg_update_xs(adg_ctx *ctx, double bcon, const double *ftem, const double *m
                   const double *bpos);
 /* The node cross sections the context holds (after adg_set_xs or
  * adg_update_xs), in node order: D, sigr, nuf, sigf (N,G), sigs (N,G,G)
- * (from, to); NULL skips an array. */
-int adg_get_xs(adg_ctx *ctx, double *D, double *sigr, double *nuf, double *sigf, double *sigs);
+ * (from, to), dc (N,G,6); NULL skips an array. */
+int adg_get_xs(adg_ctx *ctx, double *D, double *sigr, double *nuf, double *sigf, double *sigs, double *dc);
 
 /* cmfd::PowDis (src/mod_cmfd.f90:1290-1333): node powers normalised to 1. */
 int adg_powdis(adg_ctx *ctx, double *p);
@@ -227,10 +252,12 @@ int adg_outer_batch(adg_ctx *ctx, int mode, const adg_iter *iter, int niter_in, 
 int adg_trace_core(adg_ctx *ctx, int core, adg_trace *trace);
 int adg_set_xs_tables_core(adg_ctx *ctx, int core, const adg_xs_tables *t, const int *mat,
                            const double *chi, const double *dc);
+int adg_set_xtab_tables_core(adg_ctx *ctx, int core, const adg_xtab_tables *t, const int *mat,
+                             const double *chi);
 int adg_update_xs_core(adg_ctx *ctx, int core, double bcon, const double *ftem, const double *mtem,
                        const double *cden, const double *bpos);
 int adg_get_xs_core(adg_ctx *ctx, int core, double *D, double *sigr, double *nuf, double *sigf,
-                    double *sigs);
+                    double *sigs, double *dc);
 int adg_powdis_core(adg_ctx *ctx, int core, double *p);
 
 /* Differentiation form.  The reference's only adjoint is cmfd::outer_ad's

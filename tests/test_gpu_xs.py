@@ -148,3 +148,134 @@ def test_gpu_xs_update_reports_reference_stop():
     e.update_xs(*state(d, 1))
     assert np.array_equal(e.get_xs()["D"], d["s1_D"])
     e.close()
+
+
+# ---------------------------------------------------------------------------
+# branch tables (%XTAB decks): xsec::XStab_updt
+# ---------------------------------------------------------------------------
+
+XTAB_DEV = XS_DEV + ("dc",)
+
+
+def xtab_state(d, i):
+    return (float(d[f"s{i}_bcon"]), d[f"s{i}_ftem"], d[f"s{i}_mtem"], d[f"s{i}_cden"], d[f"s{i}_bpos"])
+
+
+def test_gpu_xtab_update_matches_reference():
+    """XStab_updt on the device for the MOX part-3 HELIOS deck (branch
+    tables with one to three points per parameter, rodded sets, eight
+    banks): bit for bit the reference's node cross sections and
+    discontinuity factors at both dumped states."""
+    from adpres_amd.problem import branch_tables_from_dict
+    d = golden("mox_part3_helios_xtab")
+    p = problem_from_dict(d)
+    e = Engine(p)
+    e.set_xtab_tables(branch_tables_from_dict(d))
+    for i in (0, 1):
+        print(f"mox part3 helios state {i}", flush=True)
+        e.update_xs(*xtab_state(d, i))
+        out = e.get_xs()
+        for k in XTAB_DEV:
+            ref = getattr(p, k) if i == 0 else d["s1_" + k]
+            assert np.array_equal(out[k], ref), (i, k)
+    e.close()
+
+
+def test_gpu_xtab_update_matches_oracle():
+    """Random node-wise parameters inside the branch ranges and in the
+    allowed extrapolation, rod tips inside nodes, on node boundaries, at the
+    top and fully in; and affine tables with three points on every
+    parameter (the moderator-temperature stage the decks do not branch on):
+    bit-exact against the oracle."""
+    import sys
+    import os
+    sys.path.insert(0, os.path.dirname(__file__))
+    from test_oracle import linear_branch_tables
+    from oracle.oracle import xtab_update
+    from adpres_amd.problem import branch_tables_from_dict
+    d = golden("mox_part3_helios_xtab")
+    p = problem_from_dict(d)
+    t = branch_tables_from_dict(d)
+    e = Engine(p)
+    e.set_xtab_tables(t)
+    rng = np.random.default_rng(31)
+    N = p.nnod
+    coreh = float(np.sum(p.zdel))
+    zb = np.cumsum(p.zdel[::-1])
+    for trial in range(3):
+        ftem = rng.uniform(600.0, 1500.0, N)
+        mtem = rng.uniform(500.0, 600.0, N)
+        cden = rng.uniform(0.60, 0.75, N)
+        bcon = [10.0, 1990.0, 2050.0][trial]
+        depths = np.concatenate([rng.uniform(0, coreh, 4), zb[rng.integers(0, len(zb), 2)], [0.0, coreh]])
+        depths = rng.permutation(depths)[:t["nb"]]
+        bpos = (t["coreh"] - t["pos0"] - depths) / t["ssize"]
+        print(f"random xtab state {trial}", flush=True)
+        rc, ref = xtab_update(p, t, bcon, ftem, mtem, cden, bpos)
+        assert rc == 0
+        e.update_xs(bcon, ftem, mtem, cden, bpos)
+        out = e.get_xs()
+        for k in XTAB_DEV:
+            assert np.array_equal(out[k], ref[k]), (trial, k)
+    # affine tables, every stage of the interpolation
+    lt, _, _ = linear_branch_tables(p)
+    e.set_xtab_tables(lt)
+    par = (1450.0, rng.uniform(450.0, 1400.0, N), rng.uniform(545.0, 600.0, N), rng.uniform(0.55, 0.85, N))
+    rc, ref = xtab_update(p, lt, *par)
+    assert rc == 0
+    e.update_xs(*par)
+    out = e.get_xs()
+    for k in XTAB_DEV:
+        assert np.array_equal(out[k], ref[k]), k
+    e.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32_refine"])
+def test_gpu_solve_on_device_branch_cross_sections(precision):
+    """A solve on branch-table cross sections made by adg_update_xs is the
+    solve on the oracle's cross sections and discontinuity factors for the
+    same parameters uploaded by adg_set_xs: identical k, flux and outer
+    count, and k within 1 pcm of the oracle's solve."""
+    import dataclasses
+    from oracle.oracle import xtab_update
+    from adpres_amd.problem import branch_tables_from_dict
+    d = golden("mox_part3_helios_xtab")
+    p = problem_from_dict(d)
+    t = branch_tables_from_dict(d)
+    par = (1500.0,) + xtab_state(d, 1)[1:]      # the perturbed state at 1500 ppm
+    rc, x = xtab_update(p, t, *par)
+    assert rc == 0
+    q = dataclasses.replace(p, **x)
+    res = []
+    for dev in (True, False):
+        e = Engine(q, precision=precision)
+        if dev:
+            e.set_xtab_tables(t)
+            e.update_xs(*par)
+        e.outer()
+        res.append((e.Ke, e.state()["f0"], e.niter))
+        e.close()
+    assert res[0][0] == res[1][0]
+    assert np.array_equal(res[0][1], res[1][1])
+    assert res[0][2] == res[1][2]
+    from oracle.oracle import Oracle
+    o = Oracle(q)
+    assert o.outer() == 0
+    assert abs(res[0][0] - o.Ke) < 1e-5
+
+
+def test_gpu_xtab_update_reports_out_of_range():
+    """A boron concentration beyond brInterp's 100 ppm extrapolation is
+    where the reference stops: ADG_E_XS, and the context recovers."""
+    from adpres_amd.problem import branch_tables_from_dict
+    d = golden("mox_part3_helios_xtab")
+    p = problem_from_dict(d)
+    e = Engine(p)
+    e.set_xtab_tables(branch_tables_from_dict(d))
+    b, ft, mt, cd, bp = xtab_state(d, 1)
+    with pytest.raises(ADGError) as ei:
+        e.update_xs(2150.0, ft, mt, cd, bp)
+    assert ei.value.status == ADG_E_XS
+    e.update_xs(b, ft, mt, cd, bp)
+    assert np.array_equal(e.get_xs()["dc"], d["s1_dc"])
+    e.close()
