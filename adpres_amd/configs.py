@@ -309,3 +309,45 @@ def feedback_tables(gc: GeneratedCore, seed: int = 7, nb: int = 4):
 def generate(name: str, seed: Optional[int] = None, **kw) -> GeneratedCore:
     fn = CORES[name]
     return fn(seed, **kw) if seed is not None else fn(**kw)
+
+
+def branch_tables(gc: GeneratedCore, seed: int = 9, nb: int = 4):
+    """Branch tables in the flat form of the reference's XStab_updt
+    (src/mod_xsec.f90:48-84) for a generated core: three branch points of
+    coolant density, boron, fuel and moderator temperature per material,
+    each record the material's cross sections scaled by up to +-3 % per
+    parameter step (unit discontinuity factors scaled the same way), and a
+    rodded set (2 % more sigtr, 10 % more siga) for every material, with
+    feedback_tables' rod banks.  Returns (tables, parameters) with the
+    parameters inside the branch ranges."""
+    p, d = gc.problem, gc.deck
+    rng = np.random.default_rng(seed)
+    G, M = p.ng, p.nmat
+    pts = [np.array([0.66, 0.71, 0.76]), np.array([0.0, 1000.0, 2000.0]), np.array([560.0, 900.0, 1320.0]),
+           np.array([550.0, 570.0, 590.0])]
+    recs, rrecs = [], []
+    for m in range(M):
+        b = np.concatenate([np.asarray(d[k], dtype=np.float64)[m] for k in ("xsigtr", "xsiga", "xnuf", "xsigf")]
+                           + [np.asarray(d["xsigs"], dtype=np.float64)[m].ravel(order="F")]    # (from, to), from fastest
+                           + [np.ones(6 * G)])
+        slope = rng.uniform(-0.03, 0.03, (4, b.size))
+        for v in range(3):
+            for u in range(3):
+                for t in range(3):
+                    for s_ in range(3):
+                        f = 1.0 + slope[0] * (s_ - 1) + slope[1] * (t - 1) + slope[2] * (u - 1) + slope[3] * (v - 1)
+                        r = b * f
+                        recs.append(r)
+                        rr = r.copy()
+                        rr[:G] *= 1.02
+                        rr[G:2 * G] *= 1.10
+                        rrecs.append(rr)
+    ft, _ = feedback_tables(gc, seed, nb)
+    t = {"nmat": M, "nb": nb, "dims": np.tile(np.array([3, 3, 3, 3, 1], np.int32), (M, 1)),
+         "points": np.concatenate([np.concatenate(pts)] * M), "xsec": np.concatenate(recs),
+         "rxsec": np.concatenate(rrecs), "coreh": ft["coreh"], "pos0": 0.0, "ssize": ft["ssize"],
+         "fbmap": ft["fbmap"]}
+    N = p.nnod
+    par = (1200.0, rng.uniform(600.0, 1300.0, N), rng.uniform(552.0, 588.0, N), rng.uniform(0.67, 0.75, N),
+           rng.uniform(0.0, 200.0, nb))
+    return t, par

@@ -1349,13 +1349,23 @@ int adg_update_xs_core(adg_ctx *c, int core, double bcon, const double *ftem, co
       b.bcon = bcon;
       b.rod0 = h.xref[4] - h.xref[5];
       b.ssize = h.xref[6];
-      const dim3 grid((N + 255) / 256), blk(256);
+      const int Er = 4 * c->G + c->G * c->G + 6 * c->G;
+      const size_t lds = sizeof(double) * XT_NODES * Er;
+      const bool tiled = c->xs_lds && lds <= 48 * 1024;
+      const dim3 grid = tiled ? dim3((N + XT_NODES - 1) / XT_NODES) : dim3((N + 255) / 256), blk(256);
       HIPCHK(hipEventRecord(c->xev[0], c->s));
+#define XT_LAUNCH(GG)                                                                  \
+  if (tiled)                                                                           \
+    hipLaunchKernelGGL(k_xtab_tile<GG>, grid, blk, lds, c->s, b);                      \
+  else                                                                                 \
+    hipLaunchKernelGGL(k_xtab_update<GG>, grid, blk, 0, c->s, b);
       switch (c->G) {
-        case 2: hipLaunchKernelGGL(k_xtab_update<2>, grid, blk, 0, c->s, b); break;
-        case 4: hipLaunchKernelGGL(k_xtab_update<4>, grid, blk, 0, c->s, b); break;
-        default: hipLaunchKernelGGL(k_xtab_update<0>, grid, blk, 0, c->s, b); break;
+        case 2: XT_LAUNCH(2) break;
+        case 4: XT_LAUNCH(4) break;
+        case 8: XT_LAUNCH(8) break;
+        default: XT_LAUNCH(0) break;
       }
+#undef XT_LAUNCH
       HIPCHK(hipGetLastError());
       HIPCHK(hipEventRecord(c->xev[1], c->s));
     } else {

@@ -370,4 +370,116 @@ __global__ void __launch_bounds__(256) k_xtab_update(XtabArgs a) {
   if (bad) *a.err = 1;
 }
 
+// The same update in tiles of XT_NODES nodes per block, so that both the
+// table reads and the output writes are coalesced: a node's entries are
+// computed by consecutive lanes (the eight corner records of one node are
+// read as contiguous runs), kept in LDS, and written out node-fastest per
+// output plane.  Phase 0 brackets each node and finds its rod state, phase
+// 1 runs the per-entry chains, phase 2 forms D and sigr from the tile (the
+// reference's h order) and writes every plane.
+constexpr int XT_NODES = 32;
+
+template <int GG>
+__global__ void __launch_bounds__(256) k_xtab_tile(XtabArgs a) {
+#pragma clang fp contract(off)
+  extern __shared__ double tile[];              // [XT_NODES][E]
+  __shared__ BrPos pos[XT_NODES];
+  __shared__ double vfr[XT_NODES];
+  __shared__ long cl[XT_NODES], tx[XT_NODES], tr[XT_NODES];
+  __shared__ int st[XT_NODES];                  // bit0 valid, bit1 in rodded column, bits 2-3 rod mode
+  const int G = GG > 0 ? GG : a.G, E = 4 * G + G * G + 6 * G;
+  const int n0 = blockIdx.x * XT_NODES, nn = min(XT_NODES, a.N - n0);
+  const int tid = threadIdx.x;
+  if (tid < nn) {
+    const int n = n0 + tid;
+    const BrMat mm = a.mats[a.mat0[n]];
+    int flags = 0;
+    BrPos P;
+    if (br_setup(a, mm, E, a.cden[n], a.bcon, a.ftem[n], a.mtem[n], P)) {
+      int rmode = 0;
+      double vf = 0.0;
+      bool incol = false;
+      if (a.rods && a.rodk[n]) {
+        incol = true;
+        const int b = a.rodk[n] & 0xffff, k = a.rodk[n] >> 16;
+        const double rodh = a.rod0 - a.bpos[b - 1] * a.ssize;
+        const bool above = k < a.nzz - 1 && rodh >= 0.0 && rodh <= a.zdum[k];
+        if (!above) {
+          if (rodh >= a.zdum[k] && rodh <= a.zdum[k] + a.zdel[k]) {
+            rmode = 2;
+            vf = (rodh - a.zdum[k]) / a.zdel[k];
+          } else {
+            rmode = 1;
+          }
+        }
+      }
+      if (rmode && mm.trod != 1) {
+        *a.err = 1;
+      } else {
+        flags = 1 | (incol ? 2 : 0) | (rmode << 2);
+        vfr[tid] = vf;
+        tx[tid] = mm.xoff;
+        tr[tid] = rmode ? mm.roff : 0;
+      }
+    } else {
+      *a.err = 1;
+    }
+    pos[tid] = P;
+    st[tid] = flags;
+    cl[tid] = a.cell[n];
+  }
+  __syncthreads();
+  for (int i = tid; i < nn * E; i += blockDim.x) {
+    const int j = i / E, e = i - j * E;
+    const int f = st[j];
+    if (!(f & 1)) continue;
+    const int rmode = f >> 2;
+    double x = br_elem(a.xsec + tx[j], pos[j], e);
+    if (rmode) {
+      const double r = br_elem(a.rxsec + tr[j], pos[j], e);
+      x = rmode == 2 ? (1.0 - vfr[j]) * x + vfr[j] * r : r;
+    }
+    if ((f & 2) && e >= G && x < 0.0) x = 0.0;    // the rodded column's clamp (not sigtr)
+    tile[i] = x;
+  }
+  __syncthreads();
+  const int NP = 4 * G + G * G + 6 * G;          // output planes: D, sigr, nuf, sigf, sigs, dc
+  bool bad = false;
+  for (int i = tid; i < nn * NP; i += blockDim.x) {
+    const int p = i / nn, j = i - p * nn;
+    if (!(st[j] & 1)) continue;
+    const double *t = tile + (long)j * E;
+    const long cell = cl[j];
+    if (p < G) {
+      const double s = t[p];
+      bad |= s < (double)1.e-5f;
+      const double D = 1.0 / (3.0 * s);
+      bad |= D < (double)1.e-20f;
+      a.D[p * a.NC + cell] = D;
+    } else if (p < 2 * G) {
+      const int g = p - G;
+      double dum = 0.0;
+      for (int h = 0; h < G; ++h)
+        if (h != g) dum = dum + t[4 * G + g + G * h];
+      const double sr = t[G + g] + dum;
+      bad |= sr < 0.0;
+      a.sigr[g * a.NC + cell] = sr;
+    } else if (p < 3 * G) {
+      const double v = t[p];
+      bad |= v < 0.0;
+      a.nuf[(p - 2 * G) * a.NC + cell] = v;
+    } else if (p < 4 * G) {
+      a.sigf[(p - 3 * G) * a.NC + cell] = t[p];
+    } else if (p < 4 * G + G * G) {
+      const int q = p - 4 * G, g = q % G, h = q / G;   // record index g + G h: from g, to h
+      const double v = t[p];
+      bad |= v < 0.0;
+      a.sigs[(long)(h * G + g) * a.NC + cell] = v;
+    } else {
+      a.dc[(long)(p - 4 * G - G * G) * a.NC + cell] = t[p];
+    }
+  }
+  if (bad) *a.err = 1;
+}
+
 }  // namespace adg

@@ -207,29 +207,51 @@ def xs_update_probe(eng, p, workload, seed, reps=10):
     temperature, density and rod feedback) on the workload's geometry with
     synthetic feedback tables: its kernel time from HIP events and the
     host-inclusive time of the call (parameter upload, kernel, status read)."""
-    from adpres_amd.configs import feedback_tables, generate
-    t, par = feedback_tables(generate(workload, seed))
-    eng.set_xs_tables(t)
-    for _ in range(3):
-        eng.update_xs(*par)
-    kern, wall = [], []
-    for _ in range(reps):
-        t0 = time.perf_counter()
-        eng.update_xs(*par)
-        wall.append(time.perf_counter() - t0)
-        kern.append(eng.last_timing_ms()[0])
+    from adpres_amd.configs import branch_tables, feedback_tables, generate
+    gc = generate(workload, seed)
     G = p.ng
+
+    def timed(par):
+        for _ in range(3):
+            eng.update_xs(*par)
+        kern, wall = [], []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            eng.update_xs(*par)
+            wall.append(time.perf_counter() - t0)
+            kern.append(eng.last_timing_ms()[0])
+        return float(np.mean(kern)) * 1e3, float(np.mean(wall)) * 1e3
+
+    def roof(kernel, per_node, us, wall_ms):
+        nbytes = per_node * p.nnod
+        ach = nbytes / (us * 1e-6) / 1e9
+        return {"bound": "hbm", "kernel": kernel, "algorithmic_bytes_per_launch": nbytes,
+                "bytes_per_node": per_node, "avg_launch_us": round(us, 2), "achieved": round(ach, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+                "call_ms_with_upload": round(wall_ms, 3), "samples": reps,
+                "timing": "HIP events around the kernel on the engine stream (adg_last_timing after adg_update_xs)"}
+    t, par = feedback_tables(gc)
+    eng.set_xs_tables(t)
     # per node: material, rod column, cell index and three parameters in;
     # D, sigr, nuf, sigf (G each) and sigs (G^2) out
-    per_node = 4 + 4 + 8 + 3 * 8 + 8 * (4 * G + G * G)
-    nbytes = per_node * p.nnod
-    us = float(np.mean(kern)) * 1e3
-    ach = nbytes / (us * 1e-6) / 1e9
-    return {"bound": "hbm", "kernel": "k_xs_update", "algorithmic_bytes_per_launch": nbytes,
-            "bytes_per_node": per_node, "avg_launch_us": round(us, 2), "achieved": round(ach, 1),
-            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
-            "call_ms_with_upload": round(float(np.mean(wall)) * 1e3, 3), "samples": reps,
-            "timing": "HIP events around the kernel on the engine stream (adg_last_timing after adg_update_xs)"}
+    out = roof("k_xs_update_lds", 4 + 4 + 8 + 3 * 8 + 8 * (4 * G + G * G), *timed(par))
+    # branch tables (three points on each of the four parameters, rodded
+    # sets): the same reads, and the discontinuity factors (6G) out too
+    bt, bpar = branch_tables(gc)
+    eng.set_xtab_tables(bt)
+    us, wall = timed(bpar)
+    E = 4 * G + G * G + 6 * G
+    br = roof("k_xtab_tile", 4 + 4 + 8 + 3 * 8 + 8 * E, us, wall)
+    # the interpolation reads 16 table values per entry (eight corners at two
+    # moderator points; rodded nodes twice) from the cache-resident tables:
+    # that, not HBM, bounds this kernel
+    tb = 8 * 16 * E * p.nnod
+    br["table_read_bytes_per_launch"] = tb
+    br["table_read_GBs"] = round(tb / (us * 1e-6) / 1e9, 1)
+    br["note"] = ("bound by the table reads through L1/L2 (16 per entry, rodded nodes not counted), "
+                  "not by HBM; frac is against HBM for the algorithmic bytes only")
+    out["branch_tables"] = br
+    return out
 
 
 def cpu_baseline(problem, outers_per_solve, nupd):

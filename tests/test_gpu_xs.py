@@ -181,12 +181,15 @@ def test_gpu_xtab_update_matches_reference():
     e.close()
 
 
-def test_gpu_xtab_update_matches_oracle():
+@pytest.mark.parametrize("lds", ["1", "0"])
+def test_gpu_xtab_update_matches_oracle(lds, monkeypatch):
     """Random node-wise parameters inside the branch ranges and in the
     allowed extrapolation, rod tips inside nodes, on node boundaries, at the
     top and fully in; and affine tables with three points on every
     parameter (the moderator-temperature stage the decks do not branch on):
-    bit-exact against the oracle."""
+    bit-exact against the oracle, with the tiled kernel (entries through
+    LDS) and the one-node-per-thread kernel."""
+    monkeypatch.setenv("ADG_XS_LDS", lds)
     import sys
     import os
     sys.path.insert(0, os.path.dirname(__file__))
@@ -278,4 +281,25 @@ def test_gpu_xtab_update_reports_out_of_range():
     assert ei.value.status == ADG_E_XS
     e.update_xs(b, ft, mt, cd, bp)
     assert np.array_equal(e.get_xs()["dc"], d["s1_dc"])
+    e.close()
+
+
+@pytest.mark.parametrize("lds", ["1", "0"])
+def test_gpu_xtab_update_synthetic_8_groups(lds, monkeypatch):
+    """Synthetic branch tables (three points on every parameter, rodded
+    sets) on the 8-group core: bit-exact against the oracle."""
+    from adpres_amd.configs import branch_tables, generate
+    from oracle.oracle import xtab_update
+    monkeypatch.setenv("ADG_XS_LDS", lds)
+    gc = generate("core3d_8g", 2)
+    p = gc.problem
+    t, par = branch_tables(gc)
+    rc, ref = xtab_update(p, t, *par)
+    assert rc == 0
+    e = Engine(p)
+    e.set_xtab_tables(t)
+    e.update_xs(*par)
+    out = e.get_xs()
+    for k in XTAB_DEV:
+        assert np.array_equal(out[k], ref[k]), k
     e.close()
