@@ -525,3 +525,39 @@ def branch_tables_from_dict(d, prefix: str = "b_"):
         a = np.asarray(d[prefix + k])
         t[k] = None if a.dtype == object or a.ndim == 0 else a
     return t
+
+
+def deck_layout(path: str):
+    """The assembly layout of a deck's %GEOM card (src/mod_io.f90's
+    inp_geom1): nx, ny, nz and the node divisions xdiv (nx), ydiv (ny),
+    zdiv (nz), reading "k*v" repeats and dropping "!" comments."""
+    toks, on = [], False
+    for line in open(path):
+        body = line.split("!")[0].strip()
+        if body.startswith("%"):
+            if on:
+                break
+            on = body.upper().startswith("%GEOM")
+            continue
+        if on and body:
+            for t in body.split():
+                if "*" in t:
+                    k, v = t.split("*")
+                    toks += [v] * int(k)
+                else:
+                    toks.append(t)
+    nx, ny, nz = (int(t) for t in toks[:3])
+    pos = 3
+
+    def take(n):
+        nonlocal pos
+        out = toks[pos:pos + n]
+        pos += n
+        return out
+    take(nx)
+    xdiv = [int(float(t)) for t in take(nx)]
+    take(ny)
+    ydiv = [int(float(t)) for t in take(ny)]
+    take(nz)
+    zdiv = [int(float(t)) for t in take(nz)]
+    return nx, ny, nz, np.array(xdiv, np.int32), np.array(ydiv, np.int32), np.array(zdiv, np.int32)

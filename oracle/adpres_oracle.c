@@ -1413,3 +1413,89 @@ done:
   free(poff); free(xoff); free(roff); free(xs); free(rec); free(rr); free(xyz);
   return rc;
 }
+
+/* ---- output maps: io::AsmPow (src/mod_io.f90:3267-3403), AxiPow (:3407) -- */
+
+int orc_power_maps(const orc_problem *pb, int nx, int ny, int nz, const int *xdiv, const int *ydiv,
+                   const int *zdiv, const double *pow, double *fasm, int *pmax, double *faxi, int *amax) {
+  const int N = pb->nnod, NX = pb->nxx, NY = pb->nyy, NZ = pb->nzz;
+  int sx = 0, sy = 0, sz = 0;
+  for (int i = 0; i < nx; ++i) sx += xdiv[i];
+  for (int j = 0; j < ny; ++j) sy += ydiv[j];
+  for (int k = 0; k < nz; ++k) sz += zdiv[k];
+  if (sx != NX || sy != NY || sz != NZ) return ORC_E_ARG;
+  double *fx = (double *)calloc((size_t)NX * NY * NZ, sizeof(double));
+  double *vd = (double *)calloc((size_t)NX * NY * NZ, sizeof(double));
+  double *fnode = (double *)calloc((size_t)NX * NY, sizeof(double));
+  if (!fx || !vd || !fnode) { free(fx); free(vd); free(fnode); return ORC_E_ARG; }
+#define AT(i, j, k) ((i) + (long)NX * ((j) + (long)NY * (k)))
+  for (int n = 0; n < N; ++n) {
+    const int i = pb->ix[n] - 1, j = pb->iy[n] - 1, k = pb->iz[n] - 1;
+    fx[AT(i, j, k)] = pow[n];
+    vd[AT(i, j, k)] = pb->xdel[i] * pb->ydel[j] * pb->zdel[k];   /* vdel, src/mod_io.f90:1343 */
+  }
+  /* AsmPow: axially averaged node power, then area-weighted assembly power */
+  for (int j = 0; j < NY; ++j)
+    for (int i = pb->ystag_min[j] - 1; i <= pb->ystag_max[j] - 1; ++i) {
+      double summ = 0.0, vsumm = 0.0;
+      for (int k = 0; k < NZ; ++k) {
+        summ = summ + fx[AT(i, j, k)] * pb->zdel[k];
+        vsumm = vsumm + pb->zdel[k];
+      }
+      fnode[i + (long)NX * j] = summ / vsumm;
+    }
+  int nfuel = 0;
+  double totp = 0.0;
+  int ys = 0, yf = 0;
+  for (int j = 0; j < ny; ++j) {
+    yf += ydiv[j];
+    int xs = 0, xf = 0;
+    for (int i = 0; i < nx; ++i) {
+      xf += xdiv[i];
+      double summ = 0.0, vsumm = 0.0;
+      for (int ly = ys; ly < yf; ++ly)
+        for (int lx = xs; lx < xf; ++lx) {
+          summ = summ + fnode[lx + (long)NX * ly] * pb->xdel[lx] * pb->ydel[ly];
+          vsumm = vsumm + pb->xdel[lx] * pb->ydel[ly];
+        }
+      fasm[i + nx * j] = summ / vsumm;
+      xs += xdiv[i];
+      if (fasm[i + nx * j] > 0.0) { nfuel++; totp = totp + fasm[i + nx * j]; }
+    }
+    ys += ydiv[j];
+  }
+  pmax[0] = 1; pmax[1] = 1;
+  double fmax = 0.0;
+  for (int j = 0; j < ny; ++j)
+    for (int i = 0; i < nx; ++i) {
+      if (totp > 0.0) fasm[i + nx * j] = (double)nfuel / totp * fasm[i + nx * j];
+      if (fasm[i + nx * j] > fmax) { pmax[0] = i + 1; pmax[1] = j + 1; fmax = fasm[i + nx * j]; }
+    }
+  /* AxiPow: power over volume per axial plane (the node volumes of every
+     footprint position, zero where a plane has no node there) */
+  nfuel = 0;
+  totp = 0.0;
+  int zt = 0;
+  for (int k = 0; k < nz; ++k) {
+    double summ = 0.0, vsumm = 0.0;
+    for (int lz = 0; lz < zdiv[k]; ++lz) {
+      for (int j = 0; j < NY; ++j)
+        for (int i = pb->ystag_min[j] - 1; i <= pb->ystag_max[j] - 1; ++i) {
+          summ = summ + fx[AT(i, j, zt)];
+          vsumm = vsumm + vd[AT(i, j, zt)];
+        }
+      zt++;
+    }
+    faxi[k] = summ / vsumm;
+    if (faxi[k] > 0.0) { nfuel++; totp = totp + faxi[k]; }
+  }
+  *amax = 1;
+  fmax = 0.0;
+  for (int k = 0; k < nz; ++k) {
+    faxi[k] = (double)nfuel / totp * faxi[k];
+    if (faxi[k] > fmax) { *amax = k + 1; fmax = faxi[k]; }
+  }
+#undef AT
+  free(fx); free(vd); free(fnode);
+  return ORC_OK;
+}

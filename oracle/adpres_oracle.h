@@ -127,6 +127,16 @@ int orc_xtab_update(const orc_problem *pb, const orc_xtab_tables *t, double bcon
                     double *siga, double *nuf, double *sigf, double *sigs, double *D, double *sigr,
                     double *dc);
 
+/* The numbers of the radial and axial power maps, io::AsmPow
+   (src/mod_io.f90:3267-3403) and io::AxiPow (:3407-3498), from node powers
+   pow(N): assembly powers fasm (nx,ny) normalised to average 1 over the
+   assemblies with power, the position of the maximum pmax (i, j, 1-based),
+   plane powers faxi (nz) normalised the same way and amax.  nx, ny, nz and
+   xdiv, ydiv, zdiv are the %GEOM card's assemblies, planes and their node
+   divisions. */
+int orc_power_maps(const orc_problem *pb, int nx, int ny, int nz, const int *xdiv, const int *ydiv,
+                   const int *zdiv, const double *pow, double *fasm, int *pmax, double *faxi, int *amax);
+
 /* Lxyz (src/mod_nodal.f90:901-1005) of the state: L(N,G,3). */
 int orc_leakage(const orc_problem *pb, orc_state *st, double *L);
 

@@ -83,6 +83,8 @@ def lib():
         L.orc_coup_coef.argtypes = [C.POINTER(OrcProblem), C.POINTER(OrcState)]
         L.orc_sanm_coef.argtypes = [C.c_double, C.c_double, C.c_double, _dp]
         L.orc_sanm_coef.restype = None
+        L.orc_power_maps.argtypes = [C.POINTER(OrcProblem), C.c_int, C.c_int, C.c_int, _ip, _ip, _ip, _dp,
+                                     _dp, _ip, _dp, _ip]
         L.orc_xtab_update.argtypes = [C.POINTER(OrcProblem), C.POINTER(OrcXtabTables), C.c_double,
                                       _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
         L.orc_xs_update.argtypes = [C.POINTER(OrcProblem), C.POINTER(OrcXsTables), C.c_double,
@@ -285,3 +287,17 @@ def xtab_update(p, tables, bcon, ftem, mtem, cden, bpos=None):
     rc = lib().orc_xtab_update(C.byref(pb), C.byref(t), float(bcon), dp(ftem), dp(mtem), dp(cden), dp(bpos),
                                *[_d(out[k]) for k in ("sigtr", "siga", "nuf", "sigf", "sigs", "D", "sigr", "dc")])
     return rc, out
+
+
+def power_maps(p, nx, ny, nz, xdiv, ydiv, zdiv, pw):
+    """io::AsmPow / AxiPow numbers (src/mod_io.f90:3267-3498) of node powers
+    pw: (status, fasm (ny, nx), (i, j) of its maximum, faxi (nz,), plane of
+    its maximum), positions 1-based."""
+    xd, yd, zd = (np.ascontiguousarray(a, dtype=np.int32) for a in (xdiv, ydiv, zdiv))
+    pw = np.ascontiguousarray(pw, dtype=np.float64)
+    fasm = np.zeros((ny, nx)); faxi = np.zeros(nz)
+    pm = np.zeros(2, np.int32); am = np.zeros(1, np.int32)
+    o = Oracle(p)
+    rc = lib().orc_power_maps(C.byref(o.pb), nx, ny, nz, _i(xd), _i(yd), _i(zd), _d(pw), _d(fasm), _i(pm),
+                              _d(faxi), _i(am))
+    return rc, fasm, (int(pm[0]), int(pm[1])), faxi, int(am[0])

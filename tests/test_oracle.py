@@ -499,3 +499,57 @@ def test_oracle_xtab_interpolation_is_multilinear(oracle_lib):
     assert rc == 6
     rc, _ = oracle_lib.xtab_update(p, tab, bc, ft, mt, cd * 0.7)
     assert rc == 6
+
+
+MAP_DECKS = [("iaea3d", "IAEA3Ds"), ("iaea2d", "IAEA2D"), ("pnm", "PNM"), ("biblis", "BIBLIS"),
+             ("koeberg", "KOEBERG"), ("dvp", "DVP"), ("fdm", "FDM")]
+
+
+def printed_power_maps(fasm, pmax, faxi):
+    """The numbers io::AsmPow and io::AxiPow print (src/mod_io.f90:3356-3400,
+    3470-3495), in print order: the F6.3 assembly powers (blank below
+    1e-5) in chunks of 12 columns, rows from the top, the maximum; then each
+    plane's power from the top (its height column is left out) and the
+    maximum."""
+    ny, nx = fasm.shape
+    xm = 12
+    out = []
+
+    def row_chunk(xs, xf):
+        for j in range(ny, 0, -1):
+            for i in range(xs, xf + 1):
+                v = fasm[j - 1, i - 1]
+                if not (v - 0.0 < 1e-5):
+                    out.append(f"{v:.3f}")
+    xs = 1
+    for _ in range(nx // xm):
+        row_chunk(xs, xs + xm - 1)
+        xs += xm
+    ipr = nx % xm - 1
+    if xs + ipr > xs:
+        row_chunk(xs, xs + ipr)
+    out.append(f"{fasm[pmax[1] - 1, pmax[0] - 1]:.3f}")
+    axi = [f"{faxi[k - 1]:.3f}" for k in range(len(faxi), 0, -1)]
+    return out, axi, f"{np.max(faxi):.3f}"
+
+
+@pytest.mark.parametrize("fixture,deck", MAP_DECKS)
+def test_oracle_power_maps_match_printed(oracle_lib, fixture, deck):
+    """io::AsmPow and io::AxiPow restated (src/mod_io.f90:3267-3498): from
+    the reference's own node powers, the radial and axial power maps equal
+    what the reference program prints for the deck, digit for digit."""
+    import os
+    from conftest import GOLDEN
+    from adpres_amd.problem import deck_layout
+    d = golden(fixture + "_solve")
+    p = problem_from_dict(d)
+    nx, ny, nz, xdiv, ydiv, zdiv = deck_layout(os.path.join(GOLDEN, "decks", deck))
+    rc, fasm, pmax, faxi, amax = oracle_lib.power_maps(p, nx, ny, nz, xdiv, ydiv, zdiv, d["pow"])
+    assert rc == 0
+    rad, axi, amaxv = printed_power_maps(fasm, pmax, faxi)
+    toks = [str(t) for t in golden("outputs_sweep")[deck.replace("-", "_")]]
+    assert toks[:len(rad)] == rad
+    # the axial block: plane power and height alternate, then the maximum
+    ax = toks[len(rad):len(rad) + 2 * nz + 1]
+    assert ax[0:2 * nz:2] == axi and ax[2 * nz] == amaxv
+    assert faxi[amax - 1] == np.max(faxi)
