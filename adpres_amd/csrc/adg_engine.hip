@@ -27,6 +27,7 @@
 #include "adg_cmfd.h"
 #include "adg_common.h"
 #include "adg_nodal.h"
+#include "adg_output.h"
 #include "adg_xs.h"
 
 using namespace adg;
@@ -1578,6 +1579,96 @@ int adg_powdis_core(adg_ctx *c, int core, double *p) {
 }
 
 int adg_powdis(adg_ctx *c, double *p) { return adg_powdis_core(c, 0, p); }
+
+int adg_power_maps_core(adg_ctx *c, int core, int nx, int ny, int nz, const int *xdiv, const int *ydiv,
+                        const int *zdiv, double *fasm, int *pmax, double *faxi, int *amax) {
+  if (!core_ok(c, core) || !xdiv || !ydiv || !zdiv || !fasm || !faxi) return ADG_E_ARG;
+  if (!c->core[core].xs_set) return fail(c, ADG_E_NOT_SET, "cross sections not set");
+  const CoreHost &h = c->core[core];
+  if (nx < 1 || ny < 1 || nz < 1) return fail(c, ADG_E_ARG, "empty assembly layout");
+  // first node of each assembly column, row and plane
+  std::vector<int> ib;
+  ib.reserve(2L * h.nyy + nx + ny + nz + 6);
+  std::vector<int> ys0(h.nyy, h.nxx), ys1(h.nyy, -1);
+  for (int n = 0; n < h.N; ++n) {
+    const long cc = h.cell[n] - h.base;
+    const int i = (int)(cc % h.NXP) - 1, j = (int)((cc / h.NXP) % h.NYP) - 1;
+    ys0[j] = std::min(ys0[j], i);
+    ys1[j] = std::max(ys1[j], i);
+  }
+  ib.insert(ib.end(), ys0.begin(), ys0.end());
+  ib.insert(ib.end(), ys1.begin(), ys1.end());
+  const int *divs[3] = {xdiv, ydiv, zdiv};
+  const int cnt[3] = {nx, ny, nz}, tot[3] = {h.nxx, h.nyy, h.nzz};
+  for (int d = 0; d < 3; ++d) {
+    int s = 0;
+    ib.push_back(0);
+    for (int q = 0; q < cnt[d]; ++q) {
+      if (divs[d][q] < 1) return fail(c, ADG_E_ARG, "assembly division below 1");
+      s += divs[d][q];
+      ib.push_back(s);
+    }
+    if (s != tot[d]) return fail(c, ADG_E_ARG, "assembly divisions do not add up to the node counts");
+  }
+  const long nint = (long)ib.size() + 3, ndbl = (long)h.nxx * h.nyy + (long)nx * ny + nz;
+  int *di = nullptr;
+  double *dd = nullptr;
+  try {
+    const LV lv = c->lv(c->tflat);
+    hipLaunchKernelGGL(k_powdis, dim3(c->nflat), dim3(NTF), 0, c->s, lv, c->xs(), c->f0, c->pw, c->P(0));
+    hipLaunchKernelGGL(k_scale, dim3(c->nflat), dim3(NTF), 0, c->s, lv, c->pw, c->P(0));
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(c->hsc, c->sc, c->B * sizeof(Scal), hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    if (!(c->hsc[core].tpow > 0.0)) return fail(c, ADG_E_POWER_NONPOS, "total node power is zero or less");
+    di = dalloc<int>(c->s, nint);
+    dd = dalloc<double>(c->s, ndbl);
+    scopy(c->s, di, ib.data(), ib.size() * sizeof(int), hipMemcpyHostToDevice);
+    MapArgs a;
+    a.pw = c->pw;
+    a.hx = c->d_hx + h.hoff[0];
+    a.hy = c->d_hy + h.hoff[1];
+    a.hz = c->d_hz + h.hoff[2];
+    a.vdel = c->vdel;
+    a.ys0 = di;
+    a.ys1 = di + h.nyy;
+    a.xa = di + 2 * h.nyy;
+    a.ya = a.xa + nx + 1;
+    a.za = a.ya + ny + 1;
+    a.imax = di + ib.size();
+    a.fnode = dd;
+    a.fasm = dd + (long)h.nxx * h.nyy;
+    a.faxi = a.fasm + (long)nx * ny;
+    a.base = h.base;
+    a.sy = h.sy;
+    a.sz = h.sz;
+    a.nxx = h.nxx; a.nyy = h.nyy; a.nzz = h.nzz;
+    a.nx = nx; a.ny = ny; a.nz = nz;
+    hipLaunchKernelGGL(k_map_fnode, dim3((h.nxx * h.nyy + 255) / 256), dim3(256), 0, c->s, a);
+    hipLaunchKernelGGL(k_map_sums, dim3((nx * ny + nz + 255) / 256), dim3(256), 0, c->s, a);
+    hipLaunchKernelGGL(k_map_final, dim3(1), dim3(64), 0, c->s, a);
+    HIPCHK(hipGetLastError());
+    int im[3] = {0, 0, 0};
+    HIPCHK(hipMemcpyAsync(fasm, a.fasm, sizeof(double) * nx * ny, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipMemcpyAsync(faxi, a.faxi, sizeof(double) * nz, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipMemcpyAsync(im, a.imax, sizeof im, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    if (pmax) { pmax[0] = im[0]; pmax[1] = im[1]; }
+    if (amax) *amax = im[2];
+    HIPCHK(hipFree(di));
+    HIPCHK(hipFree(dd));
+  } catch (const HipError &e) {
+    if (di) (void)hipFree(di);
+    if (dd) (void)hipFree(dd);
+    return hip_fail(c, e);
+  }
+  return ADG_OK;
+}
+
+int adg_power_maps(adg_ctx *c, int nx, int ny, int nz, const int *xdiv, const int *ydiv, const int *zdiv,
+                   double *fasm, int *pmax, double *faxi, int *amax) {
+  return adg_power_maps_core(c, 0, nx, ny, nz, xdiv, ydiv, zdiv, fasm, pmax, faxi, amax);
+}
 
 int adg_set_transient_core(adg_ctx *c, int core, const double *wtr) {
   if (!core_ok(c, core) || !wtr) return ADG_E_ARG;

@@ -42,6 +42,7 @@ EXPORTS = ("adg_create", "adg_destroy", "adg_set_xs", "adg_set_state", "adg_get_
            "adg_set_transient", "adg_set_transient_core", "adg_leakage",
            "adg_set_xs_tables", "adg_set_xs_tables_core", "adg_update_xs", "adg_update_xs_core",
            "adg_get_xs", "adg_get_xs_core", "adg_set_xtab_tables", "adg_set_xtab_tables_core",
+           "adg_power_maps", "adg_power_maps_core",
            "adg_status_string", "adg_last_error")
 
 _dp = C.POINTER(C.c_double)
@@ -124,6 +125,9 @@ def load_library(path: str = LIB_PATH):
     L.adg_leakage.argtypes = [C.c_void_p, _dp]
     L.adg_set_xs_tables.argtypes = [C.c_void_p, C.POINTER(AdgXsTables), _ip, _dp, _dp]
     L.adg_set_xs_tables_core.argtypes = [C.c_void_p, C.c_int, C.POINTER(AdgXsTables), _ip, _dp, _dp]
+    L.adg_power_maps.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, _ip, _ip, _ip, _dp, _ip, _dp, _ip]
+    L.adg_power_maps_core.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, _ip, _ip, _ip, _dp, _ip,
+                                      _dp, _ip]
     L.adg_set_xtab_tables.argtypes = [C.c_void_p, C.POINTER(AdgXtabTables), _ip, _dp]
     L.adg_set_xtab_tables_core.argtypes = [C.c_void_p, C.c_int, C.POINTER(AdgXtabTables), _ip, _dp]
     L.adg_update_xs.argtypes = [C.c_void_p, C.c_double, _dp, _dp, _dp, _dp]
@@ -365,6 +369,17 @@ class Engine:
         out = np.zeros(self.p.nnod)
         self._check(self.lib.adg_powdis(self.h, _d(out)), "adg_powdis")
         return out
+
+    def power_maps(self, nx: int, ny: int, nz: int, xdiv, ydiv, zdiv):
+        """The radial and axial power maps the reference prints (io::AsmPow,
+        io::AxiPow) from the device's PowDis: (fasm (ny, nx), (i, j) of its
+        maximum, faxi (nz,), plane of its maximum), positions 1-based."""
+        xd, yd, zd = (np.ascontiguousarray(a, np.int32) for a in (xdiv, ydiv, zdiv))
+        fasm = np.zeros((ny, nx)); faxi = np.zeros(nz)
+        pm = np.zeros(2, np.int32); am = np.zeros(1, np.int32)
+        self._check(self.lib.adg_power_maps(self.h, nx, ny, nz, _i(xd), _i(yd), _i(zd), _d(fasm), _i(pm), _d(faxi),
+                                            _i(am)), "adg_power_maps")
+        return fasm, (int(pm[0]), int(pm[1])), faxi, int(am[0])
 
     def nodal_update(self, kern: Optional[str] = None, cmode: int = 1):
         nd = C.c_double()

@@ -177,6 +177,17 @@ int adg_get_xs(adg_ctx *ctx, double *D, double *sigr, double *nuf, double *sigf,
 
 /* cmfd::PowDis (src/mod_cmfd.f90:1290-1333): node powers normalised to 1. */
 int adg_powdis(adg_ctx *ctx, double *p);
+/* The numbers of the radial and axial power maps the reference prints,
+ * io::AsmPow (src/mod_io.f90:3267-3403) and io::AxiPow (:3407-3498), made on
+ * the device from the context's PowDis node powers: fasm (nx,ny) the
+ * axially averaged assembly powers normalised to average 1 over the
+ * assemblies with power, pmax the (i, j) of their maximum; faxi (nz) the
+ * plane powers normalised the same way, amax the plane of their maximum
+ * (1-based; pmax, amax may be NULL).  nx, ny, nz and the node divisions
+ * xdiv, ydiv, zdiv are the %GEOM card's.  Same sums in the same order as
+ * the reference: bit for bit its numbers for the same node powers. */
+int adg_power_maps(adg_ctx *ctx, int nx, int ny, int nz, const int *xdiv, const int *ydiv, const int *zdiv,
+                   double *fasm, int *pmax, double *faxi, int *amax);
 /* The per-node factor (1 - tbeta(mat(n)) + dfis(n)) of TSrcTr
  * (src/mod_cmfd.f90:1068-1096) and of the transient get_B
  * (src/mod_nodal.f90:1379-1391), for ADG_TRANSIENT outers: wtr(N). */
@@ -259,6 +270,8 @@ int adg_update_xs_core(adg_ctx *ctx, int core, double bcon, const double *ftem, 
 int adg_get_xs_core(adg_ctx *ctx, int core, double *D, double *sigr, double *nuf, double *sigf,
                     double *sigs, double *dc);
 int adg_powdis_core(adg_ctx *ctx, int core, double *p);
+int adg_power_maps_core(adg_ctx *ctx, int core, int nx, int ny, int nz, const int *xdiv, const int *ydiv,
+                        const int *zdiv, double *fasm, int *pmax, double *faxi, int *amax);
 
 /* Differentiation form.  The reference's only adjoint is cmfd::outer_ad's
  * physical adjoint (src/mod_cmfd.f90:602-699: the same matrices, transposed
