@@ -1670,6 +1670,85 @@ int adg_power_maps(adg_ctx *c, int nx, int ny, int nz, const int *xdiv, const in
   return adg_power_maps_core(c, 0, nx, ny, nz, xdiv, ydiv, zdiv, fasm, pmax, faxi, amax);
 }
 
+int adg_flux_map_core(adg_ctx *c, int core, int nx, int ny, const int *xdiv, const int *ydiv, double norm,
+                      double *fasm, int *negf) {
+  if (!core_ok(c, core) || !xdiv || !ydiv || !fasm) return ADG_E_ARG;
+  const CoreHost &h = c->core[core];
+  const int G = c->G;
+  if (nx < 1 || ny < 1) return fail(c, ADG_E_ARG, "empty assembly layout");
+  std::vector<int> ib;
+  std::vector<int> ys0(h.nyy, h.nxx), ys1(h.nyy, -1);
+  for (int n = 0; n < h.N; ++n) {
+    const long cc = h.cell[n] - h.base;
+    const int i = (int)(cc % h.NXP) - 1, j = (int)((cc / h.NXP) % h.NYP) - 1;
+    ys0[j] = std::min(ys0[j], i);
+    ys1[j] = std::max(ys1[j], i);
+  }
+  ib.insert(ib.end(), ys0.begin(), ys0.end());
+  ib.insert(ib.end(), ys1.begin(), ys1.end());
+  const int *divs[2] = {xdiv, ydiv};
+  const int cnt[2] = {nx, ny}, tot[2] = {h.nxx, h.nyy};
+  for (int d = 0; d < 2; ++d) {
+    int s = 0;
+    ib.push_back(0);
+    for (int q = 0; q < cnt[d]; ++q) {
+      if (divs[d][q] < 1) return fail(c, ADG_E_ARG, "assembly division below 1");
+      s += divs[d][q];
+      ib.push_back(s);
+    }
+    if (s != tot[d]) return fail(c, ADG_E_ARG, "assembly divisions do not add up to the node counts");
+  }
+  const long nint = (long)ib.size() + G, ndbl = (long)G * h.nxx * h.nyy + (long)G * nx * ny;
+  int *di = nullptr;
+  double *dd = nullptr;
+  try {
+    di = dalloc<int>(c->s, nint);
+    dd = dalloc<double>(c->s, ndbl);
+    scopy(c->s, di, ib.data(), ib.size() * sizeof(int), hipMemcpyHostToDevice);
+    MapArgs a{};
+    a.hx = c->d_hx + h.hoff[0];
+    a.hy = c->d_hy + h.hoff[1];
+    a.hz = c->d_hz + h.hoff[2];
+    a.ys0 = di;
+    a.ys1 = di + h.nyy;
+    a.xa = di + 2 * h.nyy;
+    a.ya = a.xa + nx + 1;
+    a.negf = di + ib.size();
+    a.fnode = dd;
+    a.fasm = dd + (long)G * h.nxx * h.nyy;
+    a.base = h.base;
+    a.sy = h.sy;
+    a.sz = h.sz;
+    a.nxx = h.nxx; a.nyy = h.nyy; a.nzz = h.nzz;
+    a.nx = nx; a.ny = ny; a.nz = 0;
+    a.f0 = c->f0;
+    a.NC = c->NC;
+    a.G = G;
+    a.norm = norm;
+    hipLaunchKernelGGL(k_flux_fnode, dim3((G * h.nxx * h.nyy + 255) / 256), dim3(256), 0, c->s, a);
+    hipLaunchKernelGGL(k_flux_asm, dim3((G * nx * ny + 255) / 256), dim3(256), 0, c->s, a);
+    hipLaunchKernelGGL(k_flux_final, dim3(1), dim3(64), 0, c->s, a);
+    HIPCHK(hipGetLastError());
+    std::vector<int> ng(G, 0);
+    HIPCHK(hipMemcpyAsync(fasm, a.fasm, sizeof(double) * G * nx * ny, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipMemcpyAsync(ng.data(), a.negf, sizeof(int) * G, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    if (negf) *negf = *std::max_element(ng.begin(), ng.end());
+    HIPCHK(hipFree(di));
+    HIPCHK(hipFree(dd));
+  } catch (const HipError &e) {
+    if (di) (void)hipFree(di);
+    if (dd) (void)hipFree(dd);
+    return hip_fail(c, e);
+  }
+  return ADG_OK;
+}
+
+int adg_flux_map(adg_ctx *c, int nx, int ny, const int *xdiv, const int *ydiv, double norm, double *fasm,
+                 int *negf) {
+  return adg_flux_map_core(c, 0, nx, ny, xdiv, ydiv, norm, fasm, negf);
+}
+
 int adg_set_transient_core(adg_ctx *c, int core, const double *wtr) {
   if (!core_ok(c, core) || !wtr) return ADG_E_ARG;
   try {

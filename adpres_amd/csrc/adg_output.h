@@ -23,6 +23,12 @@ struct MapArgs {
   int *imax;                   // i, j of the assembly maximum (1-based), plane of the axial maximum
   long base, sy, sz;           // cell(i, j, k) = base + (k + 1) sz + (j + 1) sy + i + 1
   int nxx, nyy, nzz, nx, ny, nz;
+  // the flux map (io::AsmFlux): flux per cell with group stride NC
+  const double *f0;
+  long NC;
+  int G;
+  double norm;                 // <= 0: absent (no normalisation)
+  int *negf;                   // (G) 1 where an assembly flux of the group is negative
 };
 
 __device__ __forceinline__ long map_cell(const MapArgs &a, int i, int j, int k) {
@@ -115,6 +121,58 @@ __global__ void k_map_final(MapArgs a) {
   a.imax[0] = xm;
   a.imax[1] = ym;
   a.imax[2] = km;
+}
+
+// io::AsmFlux (src/mod_io.f90:3502-3644): the same reductions per group on
+// the flux, with the node volume weight of its axial average
+__global__ void k_flux_fnode(MapArgs a) {
+#pragma clang fp contract(off)
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= a.nxx * a.nyy * a.G) return;
+  const int g = t / (a.nxx * a.nyy), r = t - g * a.nxx * a.nyy, i = r % a.nxx, j = r / a.nxx;
+  double f = 0.0;
+  if (i >= a.ys0[j] && i <= a.ys1[j]) {
+    double summ = 0.0, vsumm = 0.0;
+    for (int k = 0; k < a.nzz; ++k) {
+      summ = summ + a.f0[g * a.NC + map_cell(a, i, j, k)] * a.hx[i] * a.hy[j] * a.hz[k];
+      vsumm = vsumm + a.hx[i] * a.hy[j] * a.hz[k];
+    }
+    f = summ / vsumm;
+  }
+  a.fnode[t] = f;
+}
+
+__global__ void k_flux_asm(MapArgs a) {
+#pragma clang fp contract(off)
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= a.nx * a.ny * a.G) return;
+  const int g = t / (a.nx * a.ny), r = t - g * a.nx * a.ny, i = r % a.nx, j = r / a.nx;
+  const double *fn = a.fnode + (long)g * a.nxx * a.nyy;
+  double summ = 0.0, vsumm = 0.0;
+  for (int ly = a.ya[j]; ly < a.ya[j + 1]; ++ly)
+    for (int lx = a.xa[i]; lx < a.xa[i + 1]; ++lx) {
+      summ = summ + fn[lx + (long)a.nxx * ly] * a.hx[lx] * a.hy[ly];
+      vsumm = vsumm + a.hx[lx] * a.hy[ly];
+    }
+  a.fasm[t] = summ / vsumm;
+}
+
+// one thread per group: the positive total in the reference's order, the
+// negative-flux flag and the normalisation norm / totp * f * norm
+__global__ void k_flux_final(MapArgs a) {
+#pragma clang fp contract(off)
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= a.G) return;
+  double *f = a.fasm + (long)g * a.nx * a.ny;
+  double totp = 0.0;
+  int neg = 0;
+  for (int t = 0; t < a.nx * a.ny; ++t) {
+    if (f[t] > 0.0) totp = totp + f[t];
+    if (f[t] < 0.0) neg = 1;
+  }
+  a.negf[g] = neg;
+  if (a.norm > 0.0)
+    for (int t = 0; t < a.nx * a.ny; ++t) f[t] = a.norm / totp * f[t] * a.norm;
 }
 
 }  // namespace adg

@@ -42,7 +42,7 @@ EXPORTS = ("adg_create", "adg_destroy", "adg_set_xs", "adg_set_state", "adg_get_
            "adg_set_transient", "adg_set_transient_core", "adg_leakage",
            "adg_set_xs_tables", "adg_set_xs_tables_core", "adg_update_xs", "adg_update_xs_core",
            "adg_get_xs", "adg_get_xs_core", "adg_set_xtab_tables", "adg_set_xtab_tables_core",
-           "adg_power_maps", "adg_power_maps_core",
+           "adg_power_maps", "adg_power_maps_core", "adg_flux_map", "adg_flux_map_core",
            "adg_status_string", "adg_last_error")
 
 _dp = C.POINTER(C.c_double)
@@ -128,6 +128,8 @@ def load_library(path: str = LIB_PATH):
     L.adg_power_maps.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, _ip, _ip, _ip, _dp, _ip, _dp, _ip]
     L.adg_power_maps_core.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, _ip, _ip, _ip, _dp, _ip,
                                       _dp, _ip]
+    L.adg_flux_map.argtypes = [C.c_void_p, C.c_int, C.c_int, _ip, _ip, C.c_double, _dp, _ip]
+    L.adg_flux_map_core.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, _ip, _ip, C.c_double, _dp, _ip]
     L.adg_set_xtab_tables.argtypes = [C.c_void_p, C.POINTER(AdgXtabTables), _ip, _dp]
     L.adg_set_xtab_tables_core.argtypes = [C.c_void_p, C.c_int, C.POINTER(AdgXtabTables), _ip, _dp]
     L.adg_update_xs.argtypes = [C.c_void_p, C.c_double, _dp, _dp, _dp, _dp]
@@ -380,6 +382,17 @@ class Engine:
         self._check(self.lib.adg_power_maps(self.h, nx, ny, nz, _i(xd), _i(yd), _i(zd), _d(fasm), _i(pm), _d(faxi),
                                             _i(am)), "adg_power_maps")
         return fasm, (int(pm[0]), int(pm[1])), faxi, int(am[0])
+
+    def flux_map(self, nx: int, ny: int, xdiv, ydiv, norm: float = 1.0):
+        """The radial flux map the reference prints (io::AsmFlux) from the
+        device's flux: (fasm (G, ny, nx), negative-flux flag); norm <= 0
+        leaves it unnormalised."""
+        xd, yd = (np.ascontiguousarray(a, np.int32) for a in (xdiv, ydiv))
+        fasm = np.zeros((self.p.ng, ny, nx))
+        negf = np.zeros(1, np.int32)
+        self._check(self.lib.adg_flux_map(self.h, nx, ny, _i(xd), _i(yd), float(norm), _d(fasm), _i(negf)),
+                    "adg_flux_map")
+        return fasm, int(negf[0])
 
     def nodal_update(self, kern: Optional[str] = None, cmode: int = 1):
         nd = C.c_double()

@@ -188,6 +188,14 @@ int adg_powdis(adg_ctx *ctx, double *p);
  * the reference: bit for bit its numbers for the same node powers. */
 int adg_power_maps(adg_ctx *ctx, int nx, int ny, int nz, const int *xdiv, const int *ydiv, const int *zdiv,
                    double *fasm, int *pmax, double *faxi, int *amax);
+/* The numbers of the radial flux map the reference prints, io::AsmFlux
+ * (src/mod_io.f90:3502-3644), made on the device from the context's flux:
+ * fasm (nx,ny,G) the axially averaged assembly fluxes, normalised per group
+ * to norm (norm <= 0: no normalisation, the reference's call without it);
+ * negf (may be NULL) 1 where an assembly flux is negative.  Bit for bit the
+ * reference's numbers for the same flux. */
+int adg_flux_map(adg_ctx *ctx, int nx, int ny, const int *xdiv, const int *ydiv, double norm, double *fasm,
+                 int *negf);
 /* The per-node factor (1 - tbeta(mat(n)) + dfis(n)) of TSrcTr
  * (src/mod_cmfd.f90:1068-1096) and of the transient get_B
  * (src/mod_nodal.f90:1379-1391), for ADG_TRANSIENT outers: wtr(N). */
@@ -272,6 +280,8 @@ int adg_get_xs_core(adg_ctx *ctx, int core, double *D, double *sigr, double *nuf
 int adg_powdis_core(adg_ctx *ctx, int core, double *p);
 int adg_power_maps_core(adg_ctx *ctx, int core, int nx, int ny, int nz, const int *xdiv, const int *ydiv,
                         const int *zdiv, double *fasm, int *pmax, double *faxi, int *amax);
+int adg_flux_map_core(adg_ctx *ctx, int core, int nx, int ny, const int *xdiv, const int *ydiv, double norm,
+                      double *fasm, int *negf);
 
 /* Differentiation form.  The reference's only adjoint is cmfd::outer_ad's
  * physical adjoint (src/mod_cmfd.f90:602-699: the same matrices, transposed

@@ -1499,3 +1499,61 @@ int orc_power_maps(const orc_problem *pb, int nx, int ny, int nz, const int *xdi
   free(fx); free(vd); free(fnode);
   return ORC_OK;
 }
+
+/* io::AsmFlux (src/mod_io.f90:3502-3644) */
+int orc_flux_map(const orc_problem *pb, int nx, int ny, const int *xdiv, const int *ydiv, const double *f0,
+                 double norm, double *fasm, int *negf) {
+  const int N = pb->nnod, G = pb->ng, NX = pb->nxx, NY = pb->nyy, NZ = pb->nzz;
+  int sx = 0, sy = 0;
+  for (int i = 0; i < nx; ++i) sx += xdiv[i];
+  for (int j = 0; j < ny; ++j) sy += ydiv[j];
+  if (sx != NX || sy != NY) return ORC_E_ARG;
+  double *fx = (double *)calloc((size_t)NX * NY * NZ * G, sizeof(double));
+  double *fnode = (double *)calloc((size_t)NX * NY * G, sizeof(double));
+  double *totp = (double *)calloc((size_t)G, sizeof(double));
+  if (!fx || !fnode || !totp) { free(fx); free(fnode); free(totp); return ORC_E_ARG; }
+  for (int g = 0; g < G; ++g)
+    for (int n = 0; n < N; ++n)
+      fx[(pb->ix[n] - 1) + (long)NX * ((pb->iy[n] - 1) + (long)NY * ((pb->iz[n] - 1) + (long)NZ * g))] =
+          f0[n + (long)N * g];
+  for (int g = 0; g < G; ++g)
+    for (int j = 0; j < NY; ++j)
+      for (int i = pb->ystag_min[j] - 1; i <= pb->ystag_max[j] - 1; ++i) {
+        double summ = 0.0, vsumm = 0.0;
+        for (int k = 0; k < NZ; ++k) {
+          summ = summ + fx[i + (long)NX * (j + (long)NY * (k + (long)NZ * g))] * pb->xdel[i] * pb->ydel[j] *
+                            pb->zdel[k];
+          vsumm = vsumm + pb->xdel[i] * pb->ydel[j] * pb->zdel[k];
+        }
+        fnode[i + (long)NX * (j + (long)NY * g)] = summ / vsumm;
+      }
+  *negf = 0;
+  for (int g = 0; g < G; ++g) {
+    totp[g] = 0.0;
+    int ys = 0, yf = 0;
+    for (int j = 0; j < ny; ++j) {
+      yf += ydiv[j];
+      int xs = 0, xf = 0;
+      for (int i = 0; i < nx; ++i) {
+        xf += xdiv[i];
+        double summ = 0.0, vsumm = 0.0;
+        for (int ly = ys; ly < yf; ++ly)
+          for (int lx = xs; lx < xf; ++lx) {
+            summ = summ + fnode[lx + (long)NX * (ly + (long)NY * g)] * pb->xdel[lx] * pb->ydel[ly];
+            vsumm = vsumm + pb->xdel[lx] * pb->ydel[ly];
+          }
+        double *f = &fasm[i + (long)nx * (j + (long)ny * g)];
+        *f = summ / vsumm;
+        xs += xdiv[i];
+        if (*f > 0.0) totp[g] = totp[g] + *f;
+        if (*f < 0.0) *negf = 1;
+      }
+      ys += ydiv[j];
+    }
+  }
+  if (norm > 0.0)
+    for (int g = 0; g < G; ++g)
+      for (int t = 0; t < nx * ny; ++t) fasm[t + (long)nx * ny * g] = norm / totp[g] * fasm[t + (long)nx * ny * g] * norm;
+  free(fx); free(fnode); free(totp);
+  return ORC_OK;
+}

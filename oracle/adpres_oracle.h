@@ -137,6 +137,13 @@ int orc_xtab_update(const orc_problem *pb, const orc_xtab_tables *t, double bcon
 int orc_power_maps(const orc_problem *pb, int nx, int ny, int nz, const int *xdiv, const int *ydiv,
                    const int *zdiv, const double *pow, double *fasm, int *pmax, double *faxi, int *amax);
 
+/* The numbers of the radial flux map, io::AsmFlux (src/mod_io.f90:
+   3502-3644), of the flux f0 (N,G): fasm (nx,ny,G) axially averaged
+   assembly fluxes, normalised per group to norm (a norm <= 0: absent, as
+   in the adjoint call); negf = 1 where an assembly flux is negative. */
+int orc_flux_map(const orc_problem *pb, int nx, int ny, const int *xdiv, const int *ydiv, const double *f0,
+                 double norm, double *fasm, int *negf);
+
 /* Lxyz (src/mod_nodal.f90:901-1005) of the state: L(N,G,3). */
 int orc_leakage(const orc_problem *pb, orc_state *st, double *L);
 

@@ -83,6 +83,7 @@ def lib():
         L.orc_coup_coef.argtypes = [C.POINTER(OrcProblem), C.POINTER(OrcState)]
         L.orc_sanm_coef.argtypes = [C.c_double, C.c_double, C.c_double, _dp]
         L.orc_sanm_coef.restype = None
+        L.orc_flux_map.argtypes = [C.POINTER(OrcProblem), C.c_int, C.c_int, _ip, _ip, _dp, C.c_double, _dp, _ip]
         L.orc_power_maps.argtypes = [C.POINTER(OrcProblem), C.c_int, C.c_int, C.c_int, _ip, _ip, _ip, _dp,
                                      _dp, _ip, _dp, _ip]
         L.orc_xtab_update.argtypes = [C.POINTER(OrcProblem), C.POINTER(OrcXtabTables), C.c_double,
@@ -301,3 +302,15 @@ def power_maps(p, nx, ny, nz, xdiv, ydiv, zdiv, pw):
     rc = lib().orc_power_maps(C.byref(o.pb), nx, ny, nz, _i(xd), _i(yd), _i(zd), _d(pw), _d(fasm), _i(pm),
                               _d(faxi), _i(am))
     return rc, fasm, (int(pm[0]), int(pm[1])), faxi, int(am[0])
+
+
+def flux_map(p, nx, ny, xdiv, ydiv, f0, norm=1.0):
+    """io::AsmFlux numbers (src/mod_io.f90:3502-3644) of the flux f0 (G, N):
+    (status, fasm (G, ny, nx), negative-flux flag)."""
+    xd, yd = (np.ascontiguousarray(a, dtype=np.int32) for a in (xdiv, ydiv))
+    f0 = np.ascontiguousarray(f0, dtype=np.float64)
+    fasm = np.zeros((p.ng, ny, nx))
+    negf = np.zeros(1, np.int32)
+    rc = lib().orc_flux_map(C.byref(Oracle(p).pb), nx, ny, _i(xd), _i(yd), _d(f0), float(norm), _d(fasm),
+                            _i(negf))
+    return rc, fasm, int(negf[0])

@@ -1,7 +1,8 @@
-"""The radial and axial power maps on the device (adg_power_maps: the
-numbers of io::AsmPow and io::AxiPow, src/mod_io.f90:3267-3498) through the
-C-ABI: bit for bit the oracle's maps of the same node powers, and the
-reference's printed maps digit for digit."""
+"""The radial and axial power maps and the radial flux map on the device
+(adg_power_maps, adg_flux_map: the numbers of io::AsmPow, io::AxiPow and
+io::AsmFlux, src/mod_io.f90:3267-3644) through the C-ABI: bit for bit the
+oracle's maps of the same node powers and flux, and the reference's printed
+maps digit for digit."""
 
 import os
 
@@ -41,4 +42,15 @@ def test_gpu_power_maps(fixture, deck):
     nz = lay[2]
     ax = toks[len(rad):len(rad) + 2 * nz + 1]
     assert ax[0:2 * nz:2] == axi and ax[2 * nz] == amaxv
+    # the flux map of the same state
+    from oracle.oracle import flux_map
+    from test_oracle import deck_prints_flux, printed_flux_map
+    nx, ny, _, xdiv, ydiv, _ = lay
+    flx, negf = e.flux_map(nx, ny, xdiv, ydiv, 1.0)
+    rc, oflx, onegf = flux_map(p, nx, ny, xdiv, ydiv, e.state()["f0"], 1.0)
+    assert rc == 0 and np.array_equal(flx, oflx) and negf == onegf == 0
+    if deck_prints_flux(os.path.join(GOLDEN, "decks", deck)):
+        fl = printed_flux_map(flx)
+        i0 = len(rad) + 2 * nz + 1
+        assert toks[i0:i0 + len(fl)] == fl
     e.close()

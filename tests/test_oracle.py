@@ -533,11 +533,46 @@ def printed_power_maps(fasm, pmax, faxi):
     return out, axi, f"{np.max(faxi):.3f}"
 
 
+def deck_prints_flux(path):
+    """The third switch of the deck's %PRNT card (afrad: radial flux map),
+    on without the card (src/mod_io.f90's inp_prnt)."""
+    lines = [ln.split("!")[0].strip() for ln in open(path)]
+    for i, ln in enumerate(lines):
+        if ln.upper().startswith("%PRNT"):
+            vals = next(v for v in lines[i + 1:] if v).split()
+            return int(vals[2]) == 1
+    return True
+
+
+def printed_flux_map(fasm):
+    """The numbers io::AsmFlux prints (src/mod_io.f90:3608-3640): per group
+    the ES10.3 assembly fluxes (blank below 1e-5) in chunks of 12 columns,
+    rows from the top."""
+    G, ny, nx = fasm.shape
+    xm, out = 12, []
+    for g in range(G):
+        def chunk(xs, xf):
+            for j in range(ny, 0, -1):
+                for i in range(xs, xf + 1):
+                    v = fasm[g, j - 1, i - 1]
+                    if not (v - 0.0 < 1e-5):
+                        out.append(f"{v:.3E}")
+        xs = 1
+        for _ in range(nx // xm):
+            chunk(xs, xs + xm - 1)
+            xs += xm
+        ipr = nx % xm - 1
+        if xs + ipr > xs:
+            chunk(xs, xs + ipr)
+    return out
+
+
 @pytest.mark.parametrize("fixture,deck", MAP_DECKS)
 def test_oracle_power_maps_match_printed(oracle_lib, fixture, deck):
-    """io::AsmPow and io::AxiPow restated (src/mod_io.f90:3267-3498): from
-    the reference's own node powers, the radial and axial power maps equal
-    what the reference program prints for the deck, digit for digit."""
+    """io::AsmPow, io::AxiPow and io::AsmFlux restated (src/mod_io.f90:
+    3267-3644): from the reference's own node powers and flux, the radial
+    and axial power maps and the radial flux map equal what the reference
+    program prints for the deck, digit for digit."""
     import os
     from conftest import GOLDEN
     from adpres_amd.problem import deck_layout
@@ -553,3 +588,13 @@ def test_oracle_power_maps_match_printed(oracle_lib, fixture, deck):
     ax = toks[len(rad):len(rad) + 2 * nz + 1]
     assert ax[0:2 * nz:2] == axi and ax[2 * nz] == amaxv
     assert faxi[amax - 1] == np.max(faxi)
+    # io::AsmFlux (src/mod_io.f90:3502-3644) of the reference's flux, printed
+    # ES10.3 per group after the axial block (forward decks: norm 1)
+    rc, flx, negf = oracle_lib.flux_map(p, nx, ny, xdiv, ydiv, d["f0"], 1.0)
+    assert rc == 0 and negf == 0
+    fl = printed_flux_map(flx)
+    i0 = len(rad) + 2 * nz + 1
+    if deck_prints_flux(os.path.join(GOLDEN, "decks", deck)):
+        assert toks[i0:i0 + len(fl)] == fl
+    else:
+        assert toks[i0:i0 + len(fl)] != fl
