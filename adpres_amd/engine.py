@@ -547,6 +547,22 @@ class Batch:
         self._check(self.lib.adg_powdis_core(self.h, b, _d(out)), "adg_powdis_core")
         return out
 
+    def set_xs_tables(self, b: int, tables: dict):
+        """Feedback tables of core b (see Engine.set_xs_tables)."""
+        q, keep = self.problems[b], []
+        t = _xs_tables(tables, keep)
+        mat = np.ascontiguousarray(q.mat, np.int32)
+        chi = np.ascontiguousarray(q.chi, np.float64)
+        dc = np.ascontiguousarray(q.dc, np.float64)
+        self._check(self.lib.adg_set_xs_tables_core(self.h, b, C.byref(t), _i(mat), _d(chi), _d(dc)),
+                    "adg_set_xs_tables_core")
+
+    def update_xs(self, b: int, bcon: float, ftem=None, mtem=None, cden=None, bpos=None):
+        """xsec::XS_updt of core b on the device (see Engine.update_xs)."""
+        keep = []
+        self._check(self.lib.adg_update_xs_core(self.h, b, float(bcon), _opt(ftem, keep), _opt(mtem, keep),
+                                                _opt(cden, keep), _opt(bpos, keep)), "adg_update_xs_core")
+
     def state(self, b: int):
         q = self.problems[b]
         N, G = q.nnod, q.ng

@@ -303,3 +303,35 @@ def test_gpu_xtab_update_synthetic_8_groups(lds, monkeypatch):
     for k in XTAB_DEV:
         assert np.array_equal(out[k], ref[k]), k
     e.close()
+
+
+
+def test_gpu_xs_update_in_a_batch():
+    """Per-core cross-section updates in a batch context (adg_set_xs_tables_core /
+    adg_update_xs_core): each core of a batch of the 8-group core's cousins
+    gets its own tables and parameters; the batch solve then matches the
+    single-core solves on the oracle's cross sections of the same states to
+    1 pcm."""
+    import dataclasses
+    from adpres_amd.configs import batch_core, feedback_tables
+    from adpres_amd.engine import Batch
+    from oracle.oracle import Oracle, xs_update
+    cores = [batch_core(3, i) for i in (3, 5, 8)]            # 2-group cores
+    probs = [gc.problem for gc in cores]
+    b = Batch(probs)
+    refs = []
+    for j, gc in enumerate(cores):
+        t, par = feedback_tables(gc, seed=11 + j)
+        b.set_xs_tables(j, t)
+        b.update_xs(j, *par)
+        rc, x = xs_update(gc.problem, t, *par)
+        assert rc == 0
+        o = Oracle(dataclasses.replace(gc.problem, **x))
+        refs.append((o.outer(), o.Ke))
+    st, ni, ke = b.outer()
+    for j, (rc, k) in enumerate(refs):
+        assert rc == st[j]
+        if rc == 0:
+            assert abs(ke[j] - k) < 1e-5, j
+    assert any(rc == 0 for rc, _ in refs)
+    b.close()
