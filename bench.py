@@ -124,8 +124,19 @@ def batch_bench(args):
         roof = None
         if tot_t > 0:
             ach = tot_b / tot_t / 1e9
+            # measured HBM traffic per launch of the context the counter
+            # profile covers (the one with the most nodes)
+            kname = f"void adg::k_bicgA<{'float' if wsz == 4 else 'double'}, false, 1>"
+            traffic = measured_traffic("batch256", kname)
+            Gp = max(classes, key=lambda G: sum(cores[i].nnod for i in classes[G]))
+            if traffic and str(Gp) in roof_ctx:
+                c = roof_ctx[str(Gp)]
+                c["traffic"] = traffic
+                c["traffic_GBps"] = round(traffic / (c["avg_launch_us"] * 1e-6) / 1e9, 1)
             roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                    "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic,
+                    "traffic_source": f"profiles/pmc_bicg_batch256.txt: per launch over the {Gp}-group context "
+                                      "(rocprofv3 FETCH_SIZE x2 + WRITE_SIZE x1 KiB, calibrated on a 256 MiB copy)",
                     "kernel": f"k_bicgA<{'float' if wsz == 4 else 'double'}> launched over every core of a context "
                               "(17 words per node)",
                     "contexts": roof_ctx}
@@ -312,6 +323,48 @@ def measured_traffic(workload, kernel):
     return None
 
 
+def copy_calibration():
+    """Two device copies of 256 MiB: the kernel of known size the traffic
+    counters are calibrated on."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    nb = 256 << 20
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    for ptr in (a, b):
+        if hip.hipMalloc(ctypes.byref(ptr), ctypes.c_size_t(nb)) != 0:
+            raise SystemExit("hipMalloc failed")
+    hip.hipMemset(a, 1, ctypes.c_size_t(nb))
+    for _ in range(2):
+        hip.hipMemcpy(b, a, ctypes.c_size_t(nb), 3)
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    return nb
+
+
+def kernel_probe_batch():
+    """Hardware-counter program for the batching workload: the copy
+    calibration, then one outer iteration of the context holding the most
+    nodes of the 256-core batch (every core iterating, so each BiCGSTAB
+    launch covers every core of the context), in FP64 and in FP32."""
+    from adpres_amd.configs import batch_core
+    from adpres_amd.engine import Batch
+    nb = copy_calibration()
+    cores = [batch_core(BATCH_SEED, i).problem for i in range(256)]
+    classes = {}
+    for q in cores:
+        classes.setdefault(q.ng, []).append(q)
+    G, qs = max(classes.items(), key=lambda kv: sum(q.nnod for q in kv[1]))
+    nodes = int(sum(q.nnod for q in qs))
+    out = {"probe": "kernel-counters-batch", "copy_bytes": nb, "groups": G, "cores": len(qs), "nodes": nodes,
+           "ka_algorithmic_bytes": {"fp64": 17 * 8 * nodes, "fp32_refine": 17 * 4 * nodes}}
+    for prec in ("fp64", "fp32_refine"):
+        b = Batch(qs, prec)
+        b.outer(nout=1)
+        b.close()
+    print(json.dumps(out))
+
+
 def kernel_probe(p):
     """Short program for hardware-counter runs (rocprofv3 --pmc): two device
     copies of 256 MiB (the calibration kernel of known size), then one
@@ -363,7 +416,10 @@ def main():
         sys.exit(2)
 
     if args.workload == "batch256":
-        batch_bench(args)
+        if args.kernel_probe:
+            kernel_probe_batch()
+        else:
+            batch_bench(args)
         return
 
     from adpres_amd.configs import generate
