@@ -38,6 +38,26 @@ struct XsArgs {
   double bcon, rbcon, rftem, rmtem, rcden, rod0, ssize;   // rod0 = coreh - pos0
 };
 
+// The rod state of a node of a rodded column (bank | level << 16 in rodk):
+// 0 unrodded, 1 whole increment, 2 the tip node with its rodded fraction vf
+__device__ __forceinline__ void rod_state(int rodk, const double *bpos, double rod0, double ssize,
+                                          const double *zdum, const double *zdel, int nzz, int &rmode, double &vf) {
+#pragma clang fp contract(off)
+  const int b = rodk & 0xffff, k = rodk >> 16;
+  const double rodh = rod0 - bpos[b - 1] * ssize;
+  const bool above = k < nzz - 1 && rodh >= 0.0 && rodh <= zdum[k];
+  rmode = 0;
+  vf = 0.0;
+  if (!above) {
+    if (rodh >= zdum[k] && rodh <= zdum[k] + zdel[k]) {
+      rmode = 2;
+      vf = (rodh - zdum[k]) / zdel[k];
+    } else {
+      rmode = 1;
+    }
+  }
+}
+
 // One node: the element chains of XS_updt for all of its entries.  T holds
 // the tables at offsets ks[kind] (ks < 0: kind absent or folded into the
 // base slot); dv the per-node feedback multipliers.
@@ -64,21 +84,8 @@ __device__ __forceinline__ void xs_node(const XsArgs &a, int n, const double *T,
   // tile [0, zdum[k]] without gaps.
   int rmode = 0;           // 0 none, 1 whole, 2 partial
   double vf = 0.0;
-  bool incol = false;
-  if (ks[XK_ROD] >= 0 && a.rodk[n]) {
-    incol = true;
-    const int b = a.rodk[n] & 0xffff, k = a.rodk[n] >> 16;
-    const double rodh = a.rod0 - a.bpos[b - 1] * a.ssize;
-    const bool above = k < a.nzz - 1 && rodh >= 0.0 && rodh <= a.zdum[k];
-    if (!above) {
-      if (rodh >= a.zdum[k] && rodh <= a.zdum[k] + a.zdel[k]) {
-        rmode = 2;
-        vf = (rodh - a.zdum[k]) / a.zdel[k];
-      } else {
-        rmode = 1;
-      }
-    }
-  }
+  const bool incol = ks[XK_ROD] >= 0 && a.rodk[n];
+  if (incol) rod_state(a.rodk[n], a.bpos, a.rod0, a.ssize, a.zdum, a.zdel, a.nzz, rmode, vf);
   const double *TR = T + (ks[XK_ROD] >= 0 ? ks[XK_ROD] : 0);
   bool bad = false;
   for (int g = 0; g < G; ++g) {
@@ -307,21 +314,8 @@ __global__ void __launch_bounds__(256) k_xtab_update(XtabArgs a) {
   // rod state as in k_xs_update (crod_tab_updt walks the column the same way)
   int rmode = 0;
   double vf = 0.0;
-  bool incol = false;
-  if (a.rods && a.rodk[n]) {
-    incol = true;
-    const int b = a.rodk[n] & 0xffff, k = a.rodk[n] >> 16;
-    const double rodh = a.rod0 - a.bpos[b - 1] * a.ssize;
-    const bool above = k < a.nzz - 1 && rodh >= 0.0 && rodh <= a.zdum[k];
-    if (!above) {
-      if (rodh >= a.zdum[k] && rodh <= a.zdum[k] + a.zdel[k]) {
-        rmode = 2;
-        vf = (rodh - a.zdum[k]) / a.zdel[k];
-      } else {
-        rmode = 1;
-      }
-    }
-  }
+  const bool incol = a.rods && a.rodk[n];
+  if (incol) rod_state(a.rodk[n], a.bpos, a.rod0, a.ssize, a.zdum, a.zdel, a.nzz, rmode, vf);
   if (rmode && mm.trod != 1) {     // a rodded node of a material without rodded data
     *a.err = 1;
     return;
@@ -398,21 +392,8 @@ __global__ void __launch_bounds__(256) k_xtab_tile(XtabArgs a) {
     if (br_setup(a, mm, E, a.cden[n], a.bcon, a.ftem[n], a.mtem[n], P)) {
       int rmode = 0;
       double vf = 0.0;
-      bool incol = false;
-      if (a.rods && a.rodk[n]) {
-        incol = true;
-        const int b = a.rodk[n] & 0xffff, k = a.rodk[n] >> 16;
-        const double rodh = a.rod0 - a.bpos[b - 1] * a.ssize;
-        const bool above = k < a.nzz - 1 && rodh >= 0.0 && rodh <= a.zdum[k];
-        if (!above) {
-          if (rodh >= a.zdum[k] && rodh <= a.zdum[k] + a.zdel[k]) {
-            rmode = 2;
-            vf = (rodh - a.zdum[k]) / a.zdel[k];
-          } else {
-            rmode = 1;
-          }
-        }
-      }
+      const bool incol = a.rods && a.rodk[n];
+      if (incol) rod_state(a.rodk[n], a.bpos, a.rod0, a.ssize, a.zdum, a.zdel, a.nzz, rmode, vf);
       if (rmode && mm.trod != 1) {
         *a.err = 1;
       } else {
