@@ -342,6 +342,65 @@ def transients():
              banks=np.array([[float(x) for x in q[4].split()] for q in rows]))
 
 
+TIGHT_ITER = "3000 3 1.e-8 1.e-8 15 20 20 80"
+
+
+def tight_deck(text):
+    """A deck with its %ITER tolerances at 1e-8 (and at most 3000 outers);
+    a deck without the card gets the NEACRP decks' card with 1e-8."""
+    if "%ITER" not in text:
+        return text.rstrip("\n") + "\n\n! Iteration control, tolerances tightened to 1e-8\n%ITER\n" + TIGHT_ITER + "\n"
+    lines = text.split("\n")
+    i = next(k for k, ln in enumerate(lines) if ln.strip().upper().startswith("%ITER"))
+    j = next(k for k in range(i + 1, len(lines)) if lines[k].split("!")[0].strip())
+    f = lines[j].split("!")[0].split()
+    assert f[2] == "1.e-5" and f[3] == "1.e-5", f
+    f[0], f[2], f[3] = "3000", "1.e-8", "1.e-8"
+    lines[j] = " ".join(f)
+    return "\n".join(lines)
+
+
+def transients2(stdout_dir=None):
+    """The remaining NEACRP rod ejections (A2t, B2t, C2t: hot full power)
+    and the MOX part-4 rod ejection with branch tables (HELIOS set), each
+    with its %ITER tolerances at 1e-8 (tight_deck): the printed tables of
+    time, reactivity, relative power and bank positions.  The MOX deck
+    takes about 20 minutes of reference CPU time; with stdout_dir, a
+    reference run's saved standard output <deck>.stdout there is parsed
+    instead of running it again."""
+    import re
+    exe = os.path.join(ROOT, "oracle", "_ref", "adpres")
+    tr = os.path.join(os.path.dirname(REF), "transient")
+    mox = os.path.join(HERE, "decks", "mox")
+    decks = {d + "_tight": tight_deck(open(os.path.join(tr, "NEACRP", d)).read()) for d in ("A2t", "B2t", "C2t")}
+    text = re.sub(r"\S*/smpl/xsec/(SERPENT_CMM|HELIOS)/", "./", open(os.path.join(tr, "MOX", "part4_helios")).read())
+    decks["part4_helios_tight"] = tight_deck(text)
+    for name, body in decks.items():
+        where = mox if name.startswith("part4") else os.path.join(HERE, "decks")
+        with open(os.path.join(where, name), "w") as fh:
+            fh.write(body)
+        cached = os.path.join(stdout_dir, name + ".stdout") if stdout_dir else None
+        if cached and os.path.exists(cached):
+            out = open(cached).read()
+        else:
+            with tempfile.TemporaryDirectory() as d:
+                for fn in ([f for f in os.listdir(mox) if not f.startswith("part")] if where == mox else []):
+                    shutil.copy(os.path.join(mox, fn), os.path.join(d, fn))
+                shutil.copy(os.path.join(where, name), os.path.join(d, name))
+                r = subprocess.run(["bash", "-c", f"ulimit -s unlimited; exec {exe} {name}"], cwd=d,
+                                   capture_output=True, text=True)
+            if r.returncode != 0:
+                raise RuntimeError(r.stdout[-2000:])
+            out = r.stdout
+        if "ADPRES EXIT NORMALLY" not in out:
+            raise RuntimeError(name + out[-2000:])
+        rows = [m.groups() for m in map(re.compile(TRANS_LINE).match, out.splitlines()) if m]
+        save(name.lower() + "_transient", step=np.array([int(q[0]) for q in rows], np.int32),
+             time=np.array([float(q[1]) for q in rows]), rho=np.array([float(q[2]) for q in rows]),
+             power=np.array([float(q[3]) for q in rows]),
+             banks=np.array([[float(x) for x in q[4].split()] for q in rows]))
+
+
 def trstep():
     """One transient step of the reference on the LMW deck (ref_dump trstep:
     the steady state, then rod_eject / trans_calc's set-up with a rod-like
@@ -480,6 +539,8 @@ def main(names):
         xsupd()
     if not names or "xtab" in names:
         xtab()
+    if not names or "transients2" in names:
+        transients2(os.environ.get("ADPRES_REF_STDOUT"))
     if "batch" in names:          # 256 reference solves on 7 host cores
         batch()
     if "core3d_8g" in names:      # about an hour of reference CPU time

@@ -174,7 +174,13 @@ def test_dropin_output_files_match_reference(deck, tmp_path):
     assert not bad, bad[:10]
 
 
-@pytest.mark.parametrize("deck", ["LMW_tight", "LMW", "A1t_tight", "A1t", "B1t_tight", "C1t_tight"])
+LONG = pytest.mark.skipif(not os.environ.get("ADG_LONG_TESTS"),
+                          reason="about 130 s, bound by the host's branch-table update and T-H per step; "
+                                 "run with ADG_LONG_TESTS=1")
+
+
+@pytest.mark.parametrize("deck", ["LMW_tight", "LMW", "A1t_tight", "A1t", "B1t_tight", "C1t_tight", "A2t_tight",
+                                  "B2t_tight", "C2t_tight", pytest.param("part4_helios_tight", marks=LONG)])
 def test_dropin_transient(deck, tmp_path):
     """%MODE RODEJECT (the reference's LMW rod-movement transient, 240
     Crank-Nicolson time steps): the host's rod_eject / trans_calc
@@ -185,7 +191,10 @@ def test_dropin_transient(deck, tmp_path):
 
     A1t, B1t and C1t are NEACRP rod ejections at hot zero power with T-H
     feedback (rod_eject_th, src/mod_trans.f90:172-330; A1t: 280 steps, the
-    power rising from 1e-6 to 0.2 through a Doppler-limited excursion).
+    power rising from 1e-6 to 0.2 through a Doppler-limited excursion); A2t,
+    B2t and C2t the same at hot full power; part4_helios the MOX benchmark's
+    rod ejection with branch tables (%XTAB: the host's XStab_updt every
+    step, the transient source with per-material kinetics data).
 
     The *_tight decks have their %ITER tolerances at 1e-8: the printed
     table follows the reference's to one unit of the last printed digit of
@@ -198,7 +207,12 @@ def test_dropin_transient(deck, tmp_path):
     excursion as a shift of the power rise (the reference's own 1e-5 and
     1e-8 runs differ by 1.2e-2 in power and 6e-4 $), held to 2e-2 and
     1e-3 $."""
-    shutil.copy(os.path.join(GOLDEN, "decks", deck), tmp_path / deck)
+    if deck.startswith("part4_"):
+        mox = os.path.join(GOLDEN, "decks", "mox")
+        for f in os.listdir(mox):
+            shutil.copy(os.path.join(mox, f), tmp_path / f)
+    else:
+        shutil.copy(os.path.join(GOLDEN, "decks", deck), tmp_path / deck)
     r = subprocess.run(["bash", "-c", f"ulimit -s unlimited; exec {EXE} {deck}"], cwd=tmp_path,
                        capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-3000:]
