@@ -94,7 +94,7 @@ def load_library(path: str = LIB_PATH):
     if _lib is not None:
         return _lib
     if not os.path.exists(path):
-        raise FileNotFoundError(f"{path} not built: run __graft_entry__.build() or python -m adpres_amd._build")
+        raise FileNotFoundError(f"{path} not built: run python -m adpres_amd._build")
     L = C.CDLL(path)
     L.adg_create.argtypes = [C.POINTER(AdgGeom), C.POINTER(C.c_void_p)]
     L.adg_create.restype = C.c_int

@@ -1,7 +1,7 @@
 """Parity of the gfx950 engine (through the C-ABI) with the FP64 oracle and
 the reference's golden outputs.
 
-Bars (north-star): k within 1 pcm (1e-5) and node powers within 1e-4
+Bars: k within 1 pcm (1e-5) and node powers within 1e-4
 relative of the reference's FP64 solve on the same input, for both the FP64
 and the FP32+refinement forms.  Kernel-level probes are held tighter: the
 7-point matvec bit-exact (same operation order, no contraction), one
