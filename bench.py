@@ -237,6 +237,7 @@ def xs_update_probe(eng, p, workload, seed, reps=10):
         nbytes = per_node * p.nnod
         ach = nbytes / (us * 1e-6) / 1e9
         return {"bound": "hbm", "kernel": kernel, "algorithmic_bytes_per_launch": nbytes,
+                "traffic": measured_traffic(workload, f"void adg::{kernel}<{G}>"),
                 "bytes_per_node": per_node, "avg_launch_us": round(us, 2), "achieved": round(ach, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
                 "call_ms_with_upload": round(wall_ms, 3), "samples": reps,
@@ -368,7 +369,9 @@ def kernel_probe_batch():
 def kernel_probe(p):
     """Short program for hardware-counter runs (rocprofv3 --pmc): two device
     copies of 256 MiB (the calibration kernel of known size), then one
-    BiCGSTAB call (group 1, nin=20, from a flat guess) in each precision."""
+    BiCGSTAB call (group 1, nin=20, from a flat guess) in each precision; on
+    the 8-group core also one device cross-section update with feedback
+    tables and one with branch tables."""
     import ctypes
     from adpres_amd.engine import Engine
     hip = ctypes.CDLL("libamdhip64.so")
@@ -391,6 +394,17 @@ def kernel_probe(p):
         e = Engine(p, prec)
         x = e.bicg(0, 20, rhs, x0)
         out[prec + "_xmax"] = float(np.abs(x).max())
+        if prec == "fp64" and p.name == "core3d_8g":
+            # one cross-section update with feedback tables and one with
+            # branch tables on the same core
+            from adpres_amd.configs import branch_tables, feedback_tables, generate
+            gc = generate("core3d_8g", WORKLOADS["core3d_8g"]["seed"])
+            t, par = feedback_tables(gc)
+            e.set_xs_tables(t)
+            e.update_xs(*par)
+            bt, bpar = branch_tables(gc)
+            e.set_xtab_tables(bt)
+            e.update_xs(*bpar)
         e.close()
     print(json.dumps(out))
 
