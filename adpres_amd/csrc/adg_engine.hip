@@ -1353,7 +1353,7 @@ int adg_update_xs_core(adg_ctx *c, int core, double bcon, const double *ftem, co
       const int Er = 4 * c->G + c->G * c->G + 6 * c->G;
       const size_t lds = sizeof(double) * XT_NODES * Er;
       const bool tiled = c->xs_lds && lds <= 48 * 1024;
-      const dim3 grid = tiled ? dim3((N + XT_NODES - 1) / XT_NODES) : dim3((N + 255) / 256), blk(256);
+      const dim3 grid = tiled ? dim3((N + XT_NODES - 1) / XT_NODES) : dim3((N + 255) / 256), blk(tiled ? 512 : 256);
       HIPCHK(hipEventRecord(c->xev[0], c->s));
 #define XT_LAUNCH(GG)                                                                  \
   if (tiled)                                                                           \

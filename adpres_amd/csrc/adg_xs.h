@@ -374,7 +374,7 @@ __global__ void __launch_bounds__(256) k_xtab_update(XtabArgs a) {
 constexpr int XT_NODES = 32;
 
 template <int GG>
-__global__ void __launch_bounds__(256) k_xtab_tile(XtabArgs a) {
+__global__ void __launch_bounds__(512) k_xtab_tile(XtabArgs a) {
 #pragma clang fp contract(off)
   extern __shared__ double tile[];              // [XT_NODES][E]
   __shared__ BrPos pos[XT_NODES];
