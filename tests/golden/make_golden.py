@@ -362,10 +362,10 @@ def tight_deck(text):
 
 def transients2(stdout_dir=None):
     """The remaining NEACRP rod ejections (A2t, B2t, C2t: hot full power)
-    and the MOX part-4 rod ejection with branch tables (HELIOS set), each
-    with its %ITER tolerances at 1e-8 (tight_deck): the printed tables of
-    time, reactivity, relative power and bank positions.  The MOX deck
-    takes about 20 minutes of reference CPU time; with stdout_dir, a
+    and the MOX part-4 rod ejection with branch tables (HELIOS and SERPENT
+    sets), each with its %ITER tolerances at 1e-8 (tight_deck): the printed
+    tables of time, reactivity, relative power and bank positions.  Each MOX
+    deck takes about 20 minutes of reference CPU time; with stdout_dir, a
     reference run's saved standard output <deck>.stdout there is parsed
     instead of running it again."""
     import re
@@ -373,8 +373,9 @@ def transients2(stdout_dir=None):
     tr = os.path.join(os.path.dirname(REF), "transient")
     mox = os.path.join(HERE, "decks", "mox")
     decks = {d + "_tight": tight_deck(open(os.path.join(tr, "NEACRP", d)).read()) for d in ("A2t", "B2t", "C2t")}
-    text = re.sub(r"\S*/smpl/xsec/(SERPENT_CMM|HELIOS)/", "./", open(os.path.join(tr, "MOX", "part4_helios")).read())
-    decks["part4_helios_tight"] = tight_deck(text)
+    for part in ("part4_helios", "part4_serpent"):
+        text = re.sub(r"\S*/smpl/xsec/(SERPENT_CMM|HELIOS)/", "./", open(os.path.join(tr, "MOX", part)).read())
+        decks[part + "_tight"] = tight_deck(text)
     for name, body in decks.items():
         where = mox if name.startswith("part4") else os.path.join(HERE, "decks")
         with open(os.path.join(where, name), "w") as fh:

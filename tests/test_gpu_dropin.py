@@ -180,7 +180,8 @@ LONG = pytest.mark.skipif(not os.environ.get("ADG_LONG_TESTS"),
 
 
 @pytest.mark.parametrize("deck", ["LMW_tight", "LMW", "A1t_tight", "A1t", "B1t_tight", "C1t_tight", "A2t_tight",
-                                  "B2t_tight", "C2t_tight", pytest.param("part4_helios_tight", marks=LONG)])
+                                  "B2t_tight", "C2t_tight", pytest.param("part4_helios_tight", marks=LONG),
+                                  pytest.param("part4_serpent_tight", marks=LONG)])
 def test_dropin_transient(deck, tmp_path):
     """%MODE RODEJECT (the reference's LMW rod-movement transient, 240
     Crank-Nicolson time steps): the host's rod_eject / trans_calc
@@ -192,8 +193,8 @@ def test_dropin_transient(deck, tmp_path):
     A1t, B1t and C1t are NEACRP rod ejections at hot zero power with T-H
     feedback (rod_eject_th, src/mod_trans.f90:172-330; A1t: 280 steps, the
     power rising from 1e-6 to 0.2 through a Doppler-limited excursion); A2t,
-    B2t and C2t the same at hot full power; part4_helios the MOX benchmark's
-    rod ejection with branch tables (%XTAB: the host's XStab_updt every
+    B2t and C2t the same at hot full power; part4_helios and part4_serpent the
+    MOX benchmark's rod ejection with branch tables (%XTAB: the host's XStab_updt every
     step, the transient source with per-material kinetics data).
 
     The *_tight decks have their %ITER tolerances at 1e-8: the printed
