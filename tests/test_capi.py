@@ -71,3 +71,29 @@ def test_batch_needs_one_group_count_without_touching_the_gpu():
         Batch(cores)
     with pytest.raises(ValueError):
         Batch([])
+
+
+def test_struct_layouts_match_the_header(tmp_path):
+    """The ctypes structures of the Python binding have the sizes and field
+    offsets of the C structs in include/adpres_gpu.h (compiled here with the
+    host compiler), so a binding that lays them out differently fails on the
+    CPU rather than on the GPU."""
+    import subprocess
+    structs = {"adg_geom": engine.AdgGeom, "adg_iter": engine.AdgIter, "adg_trace": engine.AdgTrace,
+               "adg_xs_tables": engine.AdgXsTables, "adg_xtab_tables": engine.AdgXtabTables}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "adpres_gpu.h"', "int main(void) {"]
+    for cname, py in structs.items():
+        lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')
+        for f, _ in py._fields_:
+            lines.append(f'  printf("{cname} {f} %zu\\n", offsetof({cname}, {f}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {tuple(ln.split()[:2]): int(ln.split()[2]) for ln in out if ln.strip()}
+    for cname, py in structs.items():
+        assert got[(cname, "size")] == ctypes.sizeof(py), cname
+        for f, _ in py._fields_:
+            assert got[(cname, f)] == getattr(py, f).offset, (cname, f)
