@@ -361,7 +361,8 @@ def tight_deck(text):
 
 
 def transients2(stdout_dir=None):
-    """The remaining NEACRP rod ejections (A2t, B2t, C2t: hot full power)
+    """The remaining NEACRP rod ejections (A2t, B2t, C2t: hot full power;
+    also B1t, C1t, A2t, B2t and C2t as shipped, at their own tolerances)
     and the MOX part-4 rod ejection with branch tables (HELIOS and SERPENT
     sets), each with its %ITER tolerances at 1e-8 (tight_deck): the printed
     tables of time, reactivity, relative power and bank positions.  Each MOX
@@ -373,6 +374,8 @@ def transients2(stdout_dir=None):
     tr = os.path.join(os.path.dirname(REF), "transient")
     mox = os.path.join(HERE, "decks", "mox")
     decks = {d + "_tight": tight_deck(open(os.path.join(tr, "NEACRP", d)).read()) for d in ("A2t", "B2t", "C2t")}
+    # and the NEACRP decks as shipped (their own %ITER tolerances)
+    decks.update({d: open(os.path.join(tr, "NEACRP", d)).read() for d in ("B1t", "C1t", "A2t", "B2t", "C2t")})
     for part in ("part4_helios", "part4_serpent"):
         text = re.sub(r"\S*/smpl/xsec/(SERPENT_CMM|HELIOS)/", "./", open(os.path.join(tr, "MOX", part)).read())
         decks[part + "_tight"] = tight_deck(text)
@@ -380,8 +383,11 @@ def transients2(stdout_dir=None):
         where = mox if name.startswith("part4") else os.path.join(HERE, "decks")
         with open(os.path.join(where, name), "w") as fh:
             fh.write(body)
-        cached = os.path.join(stdout_dir, name + ".stdout") if stdout_dir else None
-        if cached and os.path.exists(cached):
+        cached = None
+        for sd in (stdout_dir or "").split(os.pathsep):
+            if sd and os.path.exists(os.path.join(sd, name + ".stdout")):
+                cached = os.path.join(sd, name + ".stdout")
+        if cached:
             out = open(cached).read()
         else:
             with tempfile.TemporaryDirectory() as d:

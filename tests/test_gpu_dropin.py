@@ -179,8 +179,12 @@ LONG = pytest.mark.skipif(not os.environ.get("ADG_LONG_TESTS"),
                                  "run with ADG_LONG_TESTS=1")
 
 
+NEACRP_SHIPPED = ("A1t", "B1t", "C1t", "A2t", "B2t", "C2t")
+
+
 @pytest.mark.parametrize("deck", ["LMW_tight", "LMW", "A1t_tight", "A1t", "B1t_tight", "C1t_tight", "A2t_tight",
-                                  "B2t_tight", "C2t_tight", pytest.param("part4_helios_tight", marks=LONG),
+                                  "B2t_tight", "C2t_tight", "B1t", "C1t", "A2t", "B2t", "C2t",
+                                  pytest.param("part4_helios_tight", marks=LONG),
                                   pytest.param("part4_serpent_tight", marks=LONG)])
 def test_dropin_transient(deck, tmp_path):
     """%MODE RODEJECT (the reference's LMW rod-movement transient, 240
@@ -204,10 +208,12 @@ def test_dropin_transient(deck, tmp_path):
     scheme carries forward: LMW's Crank-Nicolson steps as an undamped
     step-to-step oscillation (the reference's own 1e-5 and 1e-8 runs differ
     by 2e-3 in power), so there the pair means of consecutive steps are
-    compared, power within 2e-3 and reactivity within 5e-4 $; A1t's steep
-    excursion as a shift of the power rise (the reference's own 1e-5 and
-    1e-8 runs differ by 1.2e-2 in power and 6e-4 $), held to 2e-2 and
-    1e-3 $."""
+    compared, power within 2e-3 and reactivity within 5e-4 $; the NEACRP
+    decks as shipped (A1t's steep excursion as a shift of the power rise)
+    to 2e-2 in power and 1e-3 $: the reference's own runs at the decks'
+    tolerances and at 1e-8 differ by up to 1.2e-2 in power and 8e-4 $
+    (A1t, B1t), and the drop-in's runs at the decks' tolerances are as
+    close to the reference's 1e-8 tables as the reference's own."""
     if deck.startswith("part4_"):
         mox = os.path.join(GOLDEN, "decks", "mox")
         for f in os.listdir(mox):
@@ -230,7 +236,7 @@ def test_dropin_transient(deck, tmp_path):
     if deck.endswith("_tight"):
         assert np.max(np.abs(rho - d["rho"])) <= 1.0001e-4
         assert np.max(np.abs(pw / d["power"] - 1)) <= 2e-4
-    elif deck == "A1t":
+    elif deck in NEACRP_SHIPPED:
         assert np.max(np.abs(rho - d["rho"])) <= 1e-3
         assert np.max(np.abs(pw / d["power"] - 1)) <= 2e-2
     else:
