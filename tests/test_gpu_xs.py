@@ -335,3 +335,35 @@ def test_gpu_xs_update_in_a_batch():
             assert abs(ke[j] - k) < 1e-5, j
     assert any(rc == 0 for rc, _ in refs)
     b.close()
+
+
+def test_gpu_xs_and_maps_reject_bad_input():
+    """Host-side checks of the cross-section and output-map entry points:
+    an update before any tables, tables for another material count, a bank
+    map naming a bank beyond nb, and assembly divisions that do not add up
+    to the node counts are refused with ADG_E_NOT_SET / ADG_E_ARG before
+    anything is launched; the context stays usable."""
+    from adpres_amd.engine import ADG_E_ARG, ADG_E_NOT_SET
+    d = golden("neacrp_a1_xsupd")
+    p = problem_from_dict(d)
+    t = xs_tables_from_dict(d)
+    e = Engine(p)
+    with pytest.raises(ADGError) as ei:
+        e.update_xs(*state(d, 0))
+    assert ei.value.status == ADG_E_NOT_SET
+    bad = dict(t, nmat=t["nmat"] + 1)
+    with pytest.raises(ADGError) as ei:
+        e.set_xs_tables(bad)
+    assert ei.value.status == ADG_E_ARG
+    fb = t["fbmap"].copy()
+    fb[fb > 0] = t["nb"] + 1
+    with pytest.raises(ADGError) as ei:
+        e.set_xs_tables(dict(t, fbmap=fb))
+    assert ei.value.status == ADG_E_ARG
+    e.set_xs_tables(t)
+    e.update_xs(*state(d, 1))
+    assert np.array_equal(e.get_xs()["D"], d["s1_D"])
+    with pytest.raises(ADGError) as ei:
+        e.power_maps(2, 2, 1, [1, 1], [1, 1], [1])
+    assert ei.value.status == ADG_E_ARG
+    e.close()
