@@ -15,6 +15,7 @@ module adpres_gpu
   integer(c_int), parameter :: ADG_FORWARD = 0, ADG_ADJOINT = 1, ADG_FIXED_SOURCE = 2, ADG_TRANSIENT = 3
   integer(c_int), parameter :: ADG_ADJOINT_FROZEN = 4
   integer(c_int), parameter :: ADG_FP64 = 0, ADG_FP32_REFINE = 1
+  integer(c_int), parameter :: ADG_E_ARG = 5, ADG_E_HIP = 6, ADG_E_NOT_SET = 7, ADG_E_XS = 8
 
   type, bind(C) :: adg_geom_t
     integer(c_int) :: nnod, ng, nmat, nxx, nyy, nzz
@@ -32,6 +33,23 @@ module adpres_gpu
   type, bind(C) :: adg_trace_t
     integer(c_int) :: cap, n
     type(c_ptr)    :: k, ser, fer, ndmax, flags, loc
+  end type
+
+  ! feedback tables of XS_updt: tab(xs, kind) is C's tab[kind][xs]
+  ! (kind 1 base .. 6 rod, xs 1 sigtr .. 5 sigs; c_null_ptr: feedback off)
+  type, bind(C) :: adg_xs_tables_t
+    integer(c_int) :: nmat, nb
+    real(c_double) :: rbcon, rftem, rmtem, rcden, coreh, pos0, ssize
+    type(c_ptr)    :: tab(5, 6)
+    type(c_ptr)    :: fbmap
+  end type
+
+  ! branch tables of XStab_updt, flattened as include/adpres_gpu.h describes
+  type, bind(C) :: adg_xtab_tables_t
+    integer(c_int) :: nmat, nb
+    type(c_ptr)    :: dims, points, xsec, rxsec
+    real(c_double) :: coreh, pos0, ssize
+    type(c_ptr)    :: fbmap
   end type
 
   interface
@@ -108,6 +126,54 @@ module adpres_gpu
       import :: c_int, c_ptr, c_double
       type(c_ptr), value          :: ctx
       real(c_double), intent(out) :: L(*)
+    end function
+
+    integer(c_int) function adg_set_xs_tables(ctx, t, mat, chi, dc) bind(C, name='adg_set_xs_tables')
+      import :: c_int, c_ptr, c_double, adg_xs_tables_t
+      type(c_ptr), value                :: ctx
+      type(adg_xs_tables_t), intent(in) :: t
+      integer(c_int), intent(in)        :: mat(*)
+      real(c_double), intent(in)        :: chi(*), dc(*)
+    end function
+
+    integer(c_int) function adg_set_xtab_tables(ctx, t, mat, chi) bind(C, name='adg_set_xtab_tables')
+      import :: c_int, c_ptr, c_double, adg_xtab_tables_t
+      type(c_ptr), value                  :: ctx
+      type(adg_xtab_tables_t), intent(in) :: t
+      integer(c_int), intent(in)          :: mat(*)
+      real(c_double), intent(in)          :: chi(*)
+    end function
+
+    ! arrays of an inactive feedback may be c_null_ptr: passed by address
+    integer(c_int) function adg_update_xs(ctx, bcon, ftem, mtem, cden, bpos) bind(C, name='adg_update_xs')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value    :: ctx, ftem, mtem, cden, bpos
+      real(c_double), value :: bcon
+    end function
+
+    integer(c_int) function adg_get_xs(ctx, D, sigr, nuf, sigf, sigs, dc) bind(C, name='adg_get_xs')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, D, sigr, nuf, sigf, sigs, dc
+    end function
+
+    integer(c_int) function adg_power_maps(ctx, nx, ny, nz, xdiv, ydiv, zdiv, fasm, pmax, faxi, amax) &
+        bind(C, name='adg_power_maps')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value          :: ctx
+      integer(c_int), value       :: nx, ny, nz
+      integer(c_int), intent(in)  :: xdiv(*), ydiv(*), zdiv(*)
+      real(c_double), intent(out) :: fasm(*), faxi(*)
+      integer(c_int), intent(out) :: pmax(2), amax
+    end function
+
+    integer(c_int) function adg_flux_map(ctx, nx, ny, xdiv, ydiv, norm, fasm, negf) bind(C, name='adg_flux_map')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value          :: ctx
+      integer(c_int), value       :: nx, ny
+      integer(c_int), intent(in)  :: xdiv(*), ydiv(*)
+      real(c_double), value       :: norm
+      real(c_double), intent(out) :: fasm(*)
+      integer(c_int), intent(out) :: negf
     end function
 
   end interface

@@ -6,6 +6,7 @@ import os
 import re
 
 import numpy as np
+import pytest
 
 from conftest import ROOT, golden
 from adpres_amd import engine
@@ -97,3 +98,35 @@ def test_struct_layouts_match_the_header(tmp_path):
         assert got[(cname, "size")] == ctypes.sizeof(py), cname
         for f, _ in py._fields_:
             assert got[(cname, f)] == getattr(py, f).offset, (cname, f)
+
+
+def test_fortran_binding_types_match_the_header(tmp_path):
+    """The bind(C) derived types of the Fortran binding
+    (adpres_amd/fortran/adpres_gpu_mod.f90) have the sizes of the C structs
+    (compiled here with amdflang and gcc; skipped without amdflang)."""
+    import shutil
+    import subprocess
+    fc = shutil.which("amdflang") or ("/opt/rocm/bin/amdflang" if os.path.exists("/opt/rocm/bin/amdflang") else None)
+    if not fc:
+        pytest.skip("no Fortran compiler")
+    types = {"adg_geom": "adg_geom_t", "adg_iter": "adg_iter_t", "adg_trace": "adg_trace_t",
+             "adg_xs_tables": "adg_xs_tables_t", "adg_xtab_tables": "adg_xtab_tables_t"}
+    prog = ["program sizes", "  use iso_c_binding", "  use adpres_gpu"]
+    prog += [f"  type({ft}) :: v_{c}" for c, ft in types.items()]
+    prog += [f"  print '(A,1X,I0)', '{c}', c_sizeof(v_{c})" for c in types]
+    prog += ["end program sizes"]
+    (tmp_path / "sizes.f90").write_text("\n".join(prog) + "\n")
+    mod = os.path.join(ROOT, "adpres_amd", "fortran", "adpres_gpu_mod.f90")
+    subprocess.run([fc, "-J", str(tmp_path), mod, str(tmp_path / "sizes.f90"), "-o", str(tmp_path / "fsizes")],
+                   check=True, cwd=tmp_path, capture_output=True)
+    fout = subprocess.run([str(tmp_path / "fsizes")], check=True, capture_output=True, text=True).stdout.split()
+    fs = dict(zip(fout[0::2], (int(v) for v in fout[1::2])))
+    csrc = ['#include <stdio.h>', '#include "adpres_gpu.h"', "int main(void) {"]
+    csrc += [f'  printf("{c} %zu\\n", sizeof({c}));' for c in types]
+    csrc += ["  return 0;", "}"]
+    (tmp_path / "csizes.c").write_text("\n".join(csrc))
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(tmp_path / "csizes.c"), "-o",
+                    str(tmp_path / "csizes")], check=True)
+    cout = subprocess.run([str(tmp_path / "csizes")], check=True, capture_output=True, text=True).stdout.split()
+    cs = dict(zip(cout[0::2], (int(v) for v in cout[1::2])))
+    assert fs == cs
