@@ -598,3 +598,18 @@ def test_oracle_power_maps_match_printed(oracle_lib, fixture, deck):
         assert toks[i0:i0 + len(fl)] == fl
     else:
         assert toks[i0:i0 + len(fl)] != fl
+
+
+def test_deck_layout_reads_the_geometry_card():
+    """The %GEOM card's assembly layout with k*v repeats and comments, and
+    its divisions add up to the problem's node counts on every map deck."""
+    import os
+    from conftest import GOLDEN
+    from adpres_amd.problem import deck_layout
+    nx, ny, nz, xdiv, ydiv, zdiv = deck_layout(os.path.join(GOLDEN, "decks", "IAEA3Ds"))
+    assert (nx, ny, nz) == (9, 9, 19)
+    assert list(xdiv) == [1] + [2] * 8 and list(ydiv) == [2] * 8 + [1] and list(zdiv) == [1] * 19
+    for fixture, deck in MAP_DECKS:
+        p = problem_from_dict(golden(fixture + "_solve"))
+        _, _, _, xd, yd, zd = deck_layout(os.path.join(GOLDEN, "decks", deck))
+        assert (xd.sum(), yd.sum(), zd.sum()) == (p.nxx, p.nyy, p.nzz), deck
