@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -24,6 +25,7 @@
 
 #include "../../include/adpres_gpu.h"
 #include "adg_adjoint.h"
+#include "adg_bicgp.h"
 #include "adg_cmfd.h"
 #include "adg_common.h"
 #include "adg_nodal.h"
@@ -33,6 +35,7 @@
 using namespace adg;
 
 namespace {
+
 
 struct HipError {
   hipError_t e;
@@ -96,6 +99,7 @@ struct CoreHost {
 
 struct adg_ctx {
   int B = 0, G = 0, precision = 0;
+  int dev = 0;                     // the device current at adg_create (all of the context's memory is there)
   long NC = 0;                     // cells of all boxes = group stride of the device arrays
   std::vector<CoreHost> core;
   hipStream_t s = nullptr;
@@ -136,6 +140,14 @@ struct adg_ctx {
   bool xs_lds = true;              // cross-section tables staged in LDS when they fit (ADG_XS_LDS=0: caches)
   double timing[4] = {0, 0, 0, 0};
   std::string err;
+  // persistent BiCGSTAB (single core): the brick plan, its kernel instance
+  // (cells per thread, 0: no plan), whether group solves use it, and the
+  // zeroed synchronisation area
+  Brick pb{};
+  int pkc = 0;
+  bool persist = false;
+  double *psync = nullptr;
+  long pspin = PSPIN;              // polls before a wait of the persistent kernel gives up (ADG_PSPIN)
   // profiling: event pairs recorded on the context stream during adg_outer
   struct ProfRec {
     int type, e0, e1;   // type 0 KA kernel, 1 group solve, 2 nodal update
@@ -143,6 +155,9 @@ struct adg_ctx {
   bool prof = false;
   std::vector<hipEvent_t> evs;
   int ev_next = 0;
+  // without profiling: event pairs around the replays of nodal-update outers
+  // only, from which the nodal share of adg_last_timing is estimated
+  std::vector<hipEvent_t> nevs;
   std::vector<ProfRec> recs;
   double prof_out[6] = {0, 0, 0, 0, 0, 0};
   // trace buffers (device, cap outers per core) and the captured outer iteration
@@ -407,6 +422,8 @@ void install_flat(adg_ctx *c, long param) {
   }
 }
 
+void plan_persist(adg_ctx *c);
+
 void build_tables(adg_ctx *c) {
   std::vector<Blk> tc, tfc;
   for (int b = 0; b < c->B; ++b) {
@@ -426,11 +443,108 @@ void build_tables(adg_ctx *c) {
   scopy(c->s, c->tcell, tc.data(), tc.size() * sizeof(Blk), hipMemcpyHostToDevice);
   scopy(c->s, c->tface, tfc.data(), tfc.size() * sizeof(Blk), hipMemcpyHostToDevice);
   install_flat(c, default_flat_param(c));
+  plan_persist(c);
 }
 
 // ---------------------------------------------------------------------------
 // launches
 // ---------------------------------------------------------------------------
+
+// ---------------------------------------------------------------------------
+// persistent BiCGSTAB: plan, check, launch
+// ---------------------------------------------------------------------------
+
+// one persistent solve at a time in the process: the blocks of two such
+// launches on one GPU could hold each other's CUs while waiting
+std::mutex persist_mu;
+
+template <typename TV>
+const void *persist_fn(int kc) {
+  switch (kc) {
+    case 1: return (const void *)k_bicg_persist<TV, 1>;
+    case 2: return (const void *)k_bicg_persist<TV, 2>;
+    case 4: return (const void *)k_bicg_persist<TV, 4>;
+    case 8: return (const void *)k_bicg_persist<TV, 8>;
+  }
+  return nullptr;
+}
+
+// The brick plan of a single core (plan_bricks), checked on the host
+// (verify_plan), kept only when its blocks can all be resident on the device.
+void plan_persist(adg_ctx *c) {
+  c->pkc = 0;
+  c->persist = false;
+  if (c->B != 1) return;
+  if (const char *ev = getenv("ADG_PERSIST"))
+    if (atoi(ev) == 0) return;
+  const CoreHost &h = c->core[0];
+  const bool f32 = c->precision == ADG_FP32_REFINE;
+  int ncu = 0;
+  HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->dev));
+  Brick best{};
+  int kc = 0;
+  if (!plan_bricks(h.nxx, h.nyy, h.nzz, h.base, h.sy, h.sz, c->NC, f32, ncu, best, kc)) return;
+  const void *fn = f32 ? persist_fn<float>(kc) : persist_fn<double>(kc);
+  if (!fn) return;
+  const size_t lds = persist_lds(best, f32);
+  int occ = 0;
+  HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, PNT, lds));
+  if ((long)occ * ncu < best.nb) return;
+  const std::string e = verify_plan(best, kc, h.box);
+  if (!e.empty()) {
+    fprintf(stderr, "adg: persistent BiCGSTAB plan rejected (%s); the flat kernels are used\n", e.c_str());
+    return;
+  }
+  if (!c->psync) c->psync = dalloc<double>(c->s, PSYNC_DOUBLES);
+  c->pb = best;
+  c->pkc = kc;
+  c->persist = true;
+}
+
+// clear the synchronisation area after a timed-out persistent solve (flags
+// and epoch base back to zero)
+void reset_psync(adg_ctx *c) {
+  if (!c->psync) return;
+  HIPCHK(hipMemsetAsync(c->psync, 0, PSYNC_DOUBLES * sizeof(double), c->s));
+  HIPCHK(hipStreamSynchronize(c->s));
+}
+
+template <typename TV, int KC>
+void launch_persist_kc(adg_ctx *c, const PArgs<TV> &a, size_t lds, int ek0, int ek1) {
+  if (ek0 >= 0 && ek1 >= 0)
+    hipExtLaunchKernelGGL((k_bicg_persist<TV, KC>), dim3(a.P.nb), dim3(PNT), lds, c->s, c->evs[ek0], c->evs[ek1], 0,
+                          a);
+  else
+    hipLaunchKernelGGL((k_bicg_persist<TV, KC>), dim3(a.P.nb), dim3(PNT), lds, c->s, a);
+}
+
+// bicg of group g as one persistent launch (the flat form's buffers: S in
+// R[0], T in t, p and v by parity in P and V)
+template <typename TV>
+void launch_bicg_persist(adg_ctx *c, int g, int nin, const TV *A, TV *const *R, TV *const *P, TV *const *V, TV *t) {
+  PArgs<TV> a;
+  a.P = c->pb;
+  a.A64g = c->A64 + (size_t)g * 7 * c->NC;
+  a.ATg = A;
+  a.b = c->b;
+  a.x64 = c->f0 + (size_t)g * c->NC;
+  a.S = R[0]; a.T = t; a.P0 = P[0]; a.P1 = P[1]; a.V0 = V[0]; a.V1 = V[1];
+  a.sync = c->psync;
+  a.sc = c->sc;
+  a.nin = nin;
+  a.status_timeout = ADG_E_DEVICE_SYNC;
+  a.spin = c->pspin;
+  const size_t lds = persist_lds(c->pb, sizeof(TV) == 4);
+  const int ek0 = prof_slot(c), ek1 = prof_slot(c);
+  switch (c->pkc) {
+    case 1: launch_persist_kc<TV, 1>(c, a, lds, ek0, ek1); break;
+    case 2: launch_persist_kc<TV, 2>(c, a, lds, ek0, ek1); break;
+    case 4: launch_persist_kc<TV, 4>(c, a, lds, ek0, ek1); break;
+    case 8: launch_persist_kc<TV, 8>(c, a, lds, ek0, ek1); break;
+  }
+  HIPCHK(hipGetLastError());
+  prof_pair(c, 0, ek0, ek1);
+}
 
 void launch_coupcoef(adg_ctx *c) {
   hipLaunchKernelGGL(k_coupcoef, dim3(c->ncell, c->G), dim3(NT), 0, c->s, c->lv(c->tcell), c->xs(), c->df);
@@ -494,6 +608,11 @@ void launch_bicg(adg_ctx *c, int g, int nin) {
   const long NC = c->NC;
   float *A32 = c->A32 + (size_t)g * 7 * NC;
   double *A64 = c->A64 + (size_t)g * 7 * NC;
+  if (c->persist) {
+    if (c->precision == ADG_FP32_REFINE) launch_bicg_persist<float>(c, g, nin, A32, c->rr32, c->pp32, c->vv32, c->t32);
+    else launch_bicg_persist<double>(c, g, nin, A64, c->rr64, c->pp64, c->vv64, c->t64);
+    return;
+  }
   if (c->precision == ADG_FP32_REFINE) {
     if (c->B > 1) launch_bicg_t<float, 1>(c, g, nin, A32, c->rr32, c->pp32, c->vv32, c->t32, c->rs32, c->d32);
     else launch_bicg_t<float, 0>(c, g, nin, A32, c->rr32, c->pp32, c->vv32, c->t32, c->rs32, c->d32);
@@ -671,6 +790,9 @@ int core_status(adg_ctx *c, int b, const Scal &h) {
       char buf[160];
       snprintf(buf, sizeof buf, "Max. change in nodal coupling coefficient = %.1f", h.ndmax);
       c->err = buf;
+    } else if (rc == ADG_E_DEVICE_SYNC) {
+      c->err = "a device-wide wait of the persistent BiCGSTAB kernel timed out (are its blocks sharing the GPU "
+               "with another persistent kernel?); ADG_PERSIST=0 selects the flat kernels";
     }
     if (c->B > 1) c->err += " (core " + std::to_string(b) + ")";
   } else if (!h.done) {
@@ -700,6 +822,8 @@ int run_outer(adg_ctx *c, int mode, const adg_iter *it, int nit = 1) {
   const int adjoint = mode == ADG_ADJOINT || mode == ADG_ADJOINT_FROZEN;
   const int smode = adjoint ? 1 : (mode == ADG_TRANSIENT ? 2 : 0);
   const int nout = it->nout;
+  std::unique_lock<std::mutex> plk(persist_mu, std::defer_lock);
+  if (c->persist) plk.lock();
   const LV lv = c->lv(c->tflat);
   const XS x = c->xs();
   ensure_trace(c, nout);
@@ -758,12 +882,29 @@ int run_outer(adg_ctx *c, int mode, const adg_iter *it, int nit = 1) {
   }
   const int check = c->check_every;
   int launched = 0;
+  // replays of the outers the schedule makes nodal-update outers are
+  // bracketed by events (one schedule for every core; not while profiling)
+  const bool tnod = use_graph && it->kern != ADG_KERN_FDM && nit == 1 && mode != ADG_ADJOINT_FROZEN;
+  int nev = 0;
   while (launched < nout) {
     const int n = std::min(check, nout - launched);
     for (int k = 0; k < n; ++k) {
       ++launched;
+      const bool bracket = tnod && launched % it->nupd == 0 && nev + 2 <= 4096;
+      if (bracket) {
+        while ((int)c->nevs.size() < nev + 2) {
+          hipEvent_t ev;
+          HIPCHK(hipEventCreate(&ev));
+          c->nevs.push_back(ev);
+        }
+        HIPCHK(hipEventRecord(c->nevs[nev], c->s));
+      }
       if (use_graph) HIPCHK(hipGraphLaunch(c->gexec, c->s));
       else enqueue_iteration(c, smode, it->nin, it->kern, launched, it->nupd);
+      if (bracket) {
+        HIPCHK(hipEventRecord(c->nevs[nev + 1], c->s));
+        nev += 2;
+      }
     }
     HIPCHK(hipMemcpyAsync(c->hsc, c->sc, c->B * sizeof(Scal), hipMemcpyDeviceToHost, c->s));
     HIPCHK(hipStreamSynchronize(c->s));
@@ -793,6 +934,20 @@ int run_outer(adg_ctx *c, int mode, const adg_iter *it, int nit = 1) {
     c->timing[1] = sum[1];
     c->timing[2] = sum[2];
     c->timing[3] = c->prof ? ms - sum[1] - sum[2] : 0.0;
+    if (!c->prof && nev > 0) {
+      // nodal share = time of the nodal-update outers beyond the mean of the others
+      double tn = 0.0;
+      for (int q = 0; q < nev; q += 2) {
+        float t = 0.f;
+        HIPCHK(hipEventElapsedTime(&t, c->nevs[q], c->nevs[q + 1]));
+        tn += t;
+      }
+      const int nn = nev / 2, no = launched - nn;
+      const double other = no > 0 ? (ms - tn) / no : 0.0;
+      c->timing[2] = no > 0 ? std::max(0.0, tn - nn * other) : 0.0;
+      c->timing[1] = ms - c->timing[2];
+      c->timing[3] = 0.0;
+    }
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
@@ -808,6 +963,7 @@ int run_outer(adg_ctx *c, int mode, const adg_iter *it, int nit = 1) {
       first_err = c->err;
     }
   }
+  if (rc == ADG_E_DEVICE_SYNC) reset_psync(c);
   if (rc != ADG_OK) c->err = first_err;
   return rc;
 }
@@ -940,6 +1096,7 @@ int create_batch(const adg_geom *geoms, int ncore, adg_ctx **out) {
   adg_ctx *c = new adg_ctx();
   try {
     c->B = ncore;
+    HIPCHK(hipGetDevice(&c->dev));
     c->G = geoms[0].ng;
     c->precision = geoms[0].precision;
     if (const char *ev = getenv("ADG_GRAPH")) c->graphs = atoi(ev) != 0;
@@ -947,6 +1104,7 @@ int create_batch(const adg_geom *geoms, int ncore, adg_ctx **out) {
     if (const char *ev = getenv("ADG_NOD_COOP")) c->nod_coop = atoi(ev) != 0;
     if (const char *ev = getenv("ADG_SRC_FULL")) c->src_full = atoi(ev) != 0;
     if (const char *ev = getenv("ADG_XS_LDS")) c->xs_lds = atoi(ev) != 0;
+    if (const char *ev = getenv("ADG_PSPIN")) c->pspin = std::max(1L, atol(ev));
     c->core.resize(ncore);
     std::vector<std::vector<unsigned char>> fls(ncore);
     std::vector<std::vector<double>> vds(ncore);
@@ -1026,6 +1184,104 @@ int create_batch(const adg_geom *geoms, int ncore, adg_ctx **out) {
   return ADG_OK;
 }
 
+// The host table of the map kernels of core h: per node row j its
+// footprint [ys0, ys1] (2 nyy ints: all ys0, then all ys1), then for each of
+// the nd directions the first node of each of its cnt[d] assemblies and
+// the end (cnt[d] + 1 ints).  Returns the table's length, or -1 with the
+// message in c->err (at most cap ints are written).
+long map_table(adg_ctx *c, const CoreHost &h, int nd, const int *cnt, const int *const *divs, int *ib, long cap) {
+  for (int d = 0; d < nd; ++d)
+    if (cnt[d] < 1) {
+      fail(c, ADG_E_ARG, "empty assembly layout");
+      return -1;
+    }
+  long need = 2L * h.nyy;
+  for (int d = 0; d < nd; ++d) need += cnt[d] + 1;
+  if (need > cap) {
+    fail(c, ADG_E_ARG, "assembly layout table of " + std::to_string(need) + " ints exceeds " + std::to_string(cap));
+    return -1;
+  }
+  int *ys0 = ib, *ys1 = ib + h.nyy;
+  for (int j = 0; j < h.nyy; ++j) {
+    ys0[j] = h.nxx;
+    ys1[j] = -1;
+  }
+  for (int n = 0; n < h.N; ++n) {
+    const long cc = h.cell[n] - h.base;
+    const int i = (int)(cc % h.NXP) - 1, j = (int)((cc / h.NXP) % h.NYP) - 1;
+    ys0[j] = std::min(ys0[j], i);
+    ys1[j] = std::max(ys1[j], i);
+  }
+  const int tot[3] = {h.nxx, h.nyy, h.nzz};
+  long o = 2L * h.nyy;
+  for (int d = 0; d < nd; ++d) {
+    int s = 0;
+    ib[o++] = 0;
+    for (int q = 0; q < cnt[d]; ++q) {
+      if (divs[d][q] < 1) {
+        fail(c, ADG_E_ARG, "assembly division below 1");
+        return -1;
+      }
+      s += divs[d][q];
+      ib[o++] = s;
+    }
+    if (s != tot[d]) {
+      fail(c, ADG_E_ARG, "assembly divisions do not add up to the node counts");
+      return -1;
+    }
+  }
+  return o;
+}
+
+// AsmPow / AxiPow numbers of core `core` (PowDis first) with the device
+// buffers di (nib + 3 ints) and dd (nxx nyy + nx ny + nz doubles); every
+// element of both that the map kernels read is written by them or by the
+// table copy first, so the buffers need no clearing.
+int power_maps_run(adg_ctx *c, int core, int nx, int ny, int nz, const int *ib, long nib, int *di, double *dd,
+                   double *fasm, int *pmax, double *faxi, int *amax) {
+  const CoreHost &h = c->core[core];
+  const LV lv = c->lv(c->tflat);
+  hipLaunchKernelGGL(k_powdis, dim3(c->nflat), dim3(NTF), 0, c->s, lv, c->xs(), c->f0, c->pw, c->P(0));
+  hipLaunchKernelGGL(k_scale, dim3(c->nflat), dim3(NTF), 0, c->s, lv, c->pw, c->P(0));
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(c->hsc, c->sc, c->B * sizeof(Scal), hipMemcpyDeviceToHost, c->s));
+  HIPCHK(hipStreamSynchronize(c->s));
+  if (!(c->hsc[core].tpow > 0.0)) return fail(c, ADG_E_POWER_NONPOS, "total node power is zero or less");
+  scopy(c->s, di, ib, nib * sizeof(int), hipMemcpyHostToDevice);
+  MapArgs a;
+  a.pw = c->pw;
+  a.hx = c->d_hx + h.hoff[0];
+  a.hy = c->d_hy + h.hoff[1];
+  a.hz = c->d_hz + h.hoff[2];
+  a.vdel = c->vdel;
+  a.ys0 = di;
+  a.ys1 = di + h.nyy;
+  a.xa = di + 2 * h.nyy;
+  a.ya = a.xa + nx + 1;
+  a.za = a.ya + ny + 1;
+  a.imax = di + nib;
+  a.fnode = dd;
+  a.fasm = dd + (long)h.nxx * h.nyy;
+  a.faxi = a.fasm + (long)nx * ny;
+  a.base = h.base;
+  a.sy = h.sy;
+  a.sz = h.sz;
+  a.nxx = h.nxx; a.nyy = h.nyy; a.nzz = h.nzz;
+  a.nx = nx; a.ny = ny; a.nz = nz;
+  hipLaunchKernelGGL(k_map_fnode, dim3((h.nxx * h.nyy + 255) / 256), dim3(256), 0, c->s, a);
+  hipLaunchKernelGGL(k_map_sums, dim3((nx * ny + nz + 255) / 256), dim3(256), 0, c->s, a);
+  hipLaunchKernelGGL(k_map_final, dim3(1), dim3(64), 0, c->s, a);
+  HIPCHK(hipGetLastError());
+  int im[3] = {0, 0, 0};
+  HIPCHK(hipMemcpyAsync(fasm, a.fasm, sizeof(double) * nx * ny, hipMemcpyDeviceToHost, c->s));
+  HIPCHK(hipMemcpyAsync(faxi, a.faxi, sizeof(double) * nz, hipMemcpyDeviceToHost, c->s));
+  HIPCHK(hipMemcpyAsync(im, a.imax, sizeof im, hipMemcpyDeviceToHost, c->s));
+  HIPCHK(hipStreamSynchronize(c->s));
+  if (pmax) { pmax[0] = im[0]; pmax[1] = im[1]; }
+  if (amax) *amax = im[2];
+  return ADG_OK;
+}
+
 bool core_ok(const adg_ctx *c, int b) { return c && b >= 0 && b < c->B; }
 
 }  // namespace
@@ -1043,6 +1299,7 @@ const char *adg_status_string(int status) {
     case ADG_E_HIP: return "HIP runtime error";
     case ADG_E_NOT_SET: return "cross sections not set";
     case ADG_E_XS: return "cross section out of range (the reference stops in XS_updt)";
+    case ADG_E_DEVICE_SYNC: return "device-wide wait of the persistent BiCGSTAB kernel timed out";
   }
   return "unknown status";
 }
@@ -1064,11 +1321,12 @@ void adg_destroy(adg_ctx *c) {
                   c->pp64[1], c->vv64[0], c->vv64[1], c->A32, c->rr32[0], c->rr32[1], c->rs32, c->t32, c->d32,
                   c->pp32[0], c->pp32[1], c->vv32[0], c->vv32[1], c->S, c->a2, c->a4, c->L1, c->npart, c->part,
                   c->sc, c->psi, c->phis, c->fs0s, c->exs, c->gam, c->dJdP, c->grad, c->wtr, c->d_cell,
-                  c->dstage};
+                  c->dstage, c->psync};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (c->hsc) (void)hipHostFree(c->hsc);
   for (hipEvent_t e : c->evs) (void)hipEventDestroy(e);
+  for (hipEvent_t e : c->nevs) (void)hipEventDestroy(e);
   if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
   void *tptr[] = {c->tk, c->tser, c->tfer, c->tnd, c->tflags, c->tloc};
   for (void *p : tptr)
@@ -1203,6 +1461,8 @@ int adg_set_xs_tables_core(adg_ctx *c, int core, const adg_xs_tables *t, const i
     h.xtab = nullptr;
     if (h.xnode) HIPCHK(hipFree(h.xnode));
     h.xnode = nullptr;
+    if (h.xmeta) HIPCHK(hipFree(h.xmeta));   // left by an earlier branch-table install
+    h.xmeta = nullptr;
     h.xtab = dalloc<double>(c->s, tab.size());
     scopy(c->s, h.xtab, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice);
     h.xnode = dalloc<int>(c->s, node.size());
@@ -1249,13 +1509,14 @@ int adg_set_xtab_tables_core(adg_ctx *c, int core, const adg_xtab_tables *t, con
     const int *dm = t->dims + 5 * m;
     for (int q = 0; q < 4; ++q)
       if (dm[q] < 1 || dm[q] > 64) return fail(c, ADG_E_ARG, "branch dimension outside 1..64");
+    if (dm[4] != 0 && dm[4] != 1) return fail(c, ADG_E_ARG, "rodded-set flag (dims column 5) outside 0..1");
     const long nrec = (long)dm[0] * dm[1] * dm[2] * dm[3];
     meta[m] = BrMat{dm[0], dm[1], dm[2], dm[3], dm[4], 0, np, nx, dm[4] == 1 ? nr : 0};
     np += dm[0] + dm[1] + dm[2] + dm[3];
     nx += nrec * E;
     if (dm[4] == 1) nr += nrec * E;
   }
-  if (rod && nr > 0 && !t->rxsec) return fail(c, ADG_E_ARG, "rodded records missing");
+  if (nr > 0 && !t->rxsec) return fail(c, ADG_E_ARG, "rodded records missing (a material flags a rodded set)");
   std::vector<int> node;
   std::vector<double> zdz;
   const std::string e = xs_node_setup(h, mat, M, t->fbmap, t->nb, node, zdz);
@@ -1585,84 +1846,27 @@ int adg_power_maps_core(adg_ctx *c, int core, int nx, int ny, int nz, const int 
   if (!core_ok(c, core) || !xdiv || !ydiv || !zdiv || !fasm || !faxi) return ADG_E_ARG;
   if (!c->core[core].xs_set) return fail(c, ADG_E_NOT_SET, "cross sections not set");
   const CoreHost &h = c->core[core];
-  if (nx < 1 || ny < 1 || nz < 1) return fail(c, ADG_E_ARG, "empty assembly layout");
-  // first node of each assembly column, row and plane
-  std::vector<int> ib;
-  ib.reserve(2L * h.nyy + nx + ny + nz + 6);
-  std::vector<int> ys0(h.nyy, h.nxx), ys1(h.nyy, -1);
-  for (int n = 0; n < h.N; ++n) {
-    const long cc = h.cell[n] - h.base;
-    const int i = (int)(cc % h.NXP) - 1, j = (int)((cc / h.NXP) % h.NYP) - 1;
-    ys0[j] = std::min(ys0[j], i);
-    ys1[j] = std::max(ys1[j], i);
-  }
-  ib.insert(ib.end(), ys0.begin(), ys0.end());
-  ib.insert(ib.end(), ys1.begin(), ys1.end());
+  const int cnt[3] = {nx, ny, nz};
   const int *divs[3] = {xdiv, ydiv, zdiv};
-  const int cnt[3] = {nx, ny, nz}, tot[3] = {h.nxx, h.nyy, h.nzz};
-  for (int d = 0; d < 3; ++d) {
-    int s = 0;
-    ib.push_back(0);
-    for (int q = 0; q < cnt[d]; ++q) {
-      if (divs[d][q] < 1) return fail(c, ADG_E_ARG, "assembly division below 1");
-      s += divs[d][q];
-      ib.push_back(s);
-    }
-    if (s != tot[d]) return fail(c, ADG_E_ARG, "assembly divisions do not add up to the node counts");
-  }
-  const long nint = (long)ib.size() + 3, ndbl = (long)h.nxx * h.nyy + (long)nx * ny + nz;
+  std::vector<int> ib(2L * std::max(h.nyy, 0) + std::max(nx, 0) + std::max(ny, 0) + std::max(nz, 0) + 3);
+  const long nib = map_table(c, h, 3, cnt, divs, ib.data(), (long)ib.size());
+  if (nib < 0) return ADG_E_ARG;
+  const long ndbl = (long)h.nxx * h.nyy + (long)nx * ny + nz;
   int *di = nullptr;
   double *dd = nullptr;
   try {
-    const LV lv = c->lv(c->tflat);
-    hipLaunchKernelGGL(k_powdis, dim3(c->nflat), dim3(NTF), 0, c->s, lv, c->xs(), c->f0, c->pw, c->P(0));
-    hipLaunchKernelGGL(k_scale, dim3(c->nflat), dim3(NTF), 0, c->s, lv, c->pw, c->P(0));
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(c->hsc, c->sc, c->B * sizeof(Scal), hipMemcpyDeviceToHost, c->s));
-    HIPCHK(hipStreamSynchronize(c->s));
-    if (!(c->hsc[core].tpow > 0.0)) return fail(c, ADG_E_POWER_NONPOS, "total node power is zero or less");
-    di = dalloc<int>(c->s, nint);
+    di = dalloc<int>(c->s, nib + 3);
     dd = dalloc<double>(c->s, ndbl);
-    scopy(c->s, di, ib.data(), ib.size() * sizeof(int), hipMemcpyHostToDevice);
-    MapArgs a;
-    a.pw = c->pw;
-    a.hx = c->d_hx + h.hoff[0];
-    a.hy = c->d_hy + h.hoff[1];
-    a.hz = c->d_hz + h.hoff[2];
-    a.vdel = c->vdel;
-    a.ys0 = di;
-    a.ys1 = di + h.nyy;
-    a.xa = di + 2 * h.nyy;
-    a.ya = a.xa + nx + 1;
-    a.za = a.ya + ny + 1;
-    a.imax = di + ib.size();
-    a.fnode = dd;
-    a.fasm = dd + (long)h.nxx * h.nyy;
-    a.faxi = a.fasm + (long)nx * ny;
-    a.base = h.base;
-    a.sy = h.sy;
-    a.sz = h.sz;
-    a.nxx = h.nxx; a.nyy = h.nyy; a.nzz = h.nzz;
-    a.nx = nx; a.ny = ny; a.nz = nz;
-    hipLaunchKernelGGL(k_map_fnode, dim3((h.nxx * h.nyy + 255) / 256), dim3(256), 0, c->s, a);
-    hipLaunchKernelGGL(k_map_sums, dim3((nx * ny + nz + 255) / 256), dim3(256), 0, c->s, a);
-    hipLaunchKernelGGL(k_map_final, dim3(1), dim3(64), 0, c->s, a);
-    HIPCHK(hipGetLastError());
-    int im[3] = {0, 0, 0};
-    HIPCHK(hipMemcpyAsync(fasm, a.fasm, sizeof(double) * nx * ny, hipMemcpyDeviceToHost, c->s));
-    HIPCHK(hipMemcpyAsync(faxi, a.faxi, sizeof(double) * nz, hipMemcpyDeviceToHost, c->s));
-    HIPCHK(hipMemcpyAsync(im, a.imax, sizeof im, hipMemcpyDeviceToHost, c->s));
-    HIPCHK(hipStreamSynchronize(c->s));
-    if (pmax) { pmax[0] = im[0]; pmax[1] = im[1]; }
-    if (amax) *amax = im[2];
+    const int rc = power_maps_run(c, core, nx, ny, nz, ib.data(), nib, di, dd, fasm, pmax, faxi, amax);
     HIPCHK(hipFree(di));
+    di = nullptr;
     HIPCHK(hipFree(dd));
+    return rc;
   } catch (const HipError &e) {
     if (di) (void)hipFree(di);
     if (dd) (void)hipFree(dd);
     return hip_fail(c, e);
   }
-  return ADG_OK;
 }
 
 int adg_power_maps(adg_ctx *c, int nx, int ny, int nz, const int *xdiv, const int *ydiv, const int *zdiv,
@@ -1796,7 +2000,19 @@ int adg_nodal_update(adg_ctx *c, int kern, int cmode, double *ndmax, int *loc3) 
     HIPCHK(hipStreamSynchronize(c->s));
     if (ndmax) *ndmax = c->hsc->ndmax;
     if (loc3) { loc3[0] = c->hsc->im; loc3[1] = c->hsc->jm; loc3[2] = c->hsc->km; }
-    if (c->hsc->lu_key != ~0ull) return fail(c, ADG_E_LU_PIVOT, "LU pivot below threshold");
+    if (c->hsc->lu_key != ~0ull) {
+      // the first failing system in the reference's sweep order: its node in loc3
+      const unsigned long long key = c->hsc->lu_key;
+      if (loc3) {
+        loc3[0] = (int)((key >> 20) & 1023);
+        loc3[1] = (int)((key >> 10) & 1023);
+        loc3[2] = (int)(key & 1023);
+      }
+      char buf[160];
+      snprintf(buf, sizeof buf, "LU pivot below 1e-4 at node I = %d, J = %d, K = %d", (int)((key >> 20) & 1023),
+               (int)((key >> 10) & 1023), (int)(key & 1023));
+      return fail(c, ADG_E_LU_PIVOT, buf);
+    }
     if (c->hsc->ndmax > 1.e3) return fail(c, ADG_E_NODAL_UNSTABLE, "nodal update unstable");
   } catch (const HipError &e) {
     return hip_fail(c, e);
@@ -1820,9 +2036,18 @@ int adg_bicg(adg_ctx *c, int g, int nin, const double *b, double *x) {
     gather(c, 0, c->f0, c->G, full.data(), st);
     memcpy(full.data() + (size_t)g * N, x, N * sizeof(double));
     scatter(c, 0, full.data(), c->G, c->f0, st);
-    launch_bicg(c, g, nin);
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipStreamSynchronize(c->s));
+    {
+      std::unique_lock<std::mutex> plk(persist_mu, std::defer_lock);
+      if (c->persist) plk.lock();
+      launch_bicg(c, g, nin);
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipMemcpyAsync(c->hsc, c->sc, sizeof(Scal), hipMemcpyDeviceToHost, c->s));
+      HIPCHK(hipStreamSynchronize(c->s));
+      if (c->hsc->status == ADG_E_DEVICE_SYNC) {
+        reset_psync(c);
+        return fail(c, ADG_E_DEVICE_SYNC, "a device-wide wait of the persistent BiCGSTAB kernel timed out");
+      }
+    }
     gather(c, 0, c->f0, c->G, full.data(), st);
     memcpy(x, full.data() + (size_t)g * N, N * sizeof(double));
   } catch (const HipError &e) {
@@ -2002,8 +2227,7 @@ int adg_autotune(adg_ctx *c, int nin, double *out4) {
     c->prof = false;
     long best = c->flat_param;
     double tbest = 1e300;
-    for (long cp : cand) {
-      install_flat(c, cp);
+    auto time_solve = [&]() {
       launch_bicg(c, 0, nin);   // warm-up
       float ms = 1e30f;
       for (int rep = 0; rep < 3; ++rep) {
@@ -2015,23 +2239,48 @@ int adg_autotune(adg_ctx *c, int nin, double *out4) {
         HIPCHK(hipEventElapsedTime(&t, e0, e1));
         ms = std::min(ms, t);
       }
+      return ms;
+    };
+    // flat candidates, then (single core with a brick plan) the persistent kernel
+    const bool can_persist = c->pkc > 0;
+    std::unique_lock<std::mutex> plk(persist_mu, std::defer_lock);
+    if (can_persist) plk.lock();
+    bool best_persist = false;
+    c->persist = false;
+    for (long cp : cand) {
+      install_flat(c, cp);
+      const float ms = time_solve();
       if (ms < tbest) {
         tbest = ms;
         best = cp;
       }
     }
+    if (can_persist) {
+      c->persist = true;
+      const float ms = time_solve();
+      HIPCHK(hipMemcpyAsync(c->hsc, c->sc, sizeof(Scal), hipMemcpyDeviceToHost, c->s));
+      HIPCHK(hipStreamSynchronize(c->s));
+      if (c->hsc->status == ADG_E_DEVICE_SYNC) {
+        reset_psync(c);
+        set_scal_start(c, nullptr);
+      } else if (ms < tbest) {
+        tbest = ms;
+        best_persist = true;
+      }
+    }
     c->prof = prof;
     install_flat(c, best);
+    c->persist = best_persist;
     HIPCHK(hipMemcpyAsync(c->f0, keep, nf * sizeof(double), hipMemcpyDeviceToDevice, c->s));
     HIPCHK(hipStreamSynchronize(c->s));
     HIPCHK(hipFree(keep));
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (out4) {
-      out4[0] = (double)best;
+      out4[0] = best_persist ? 0.0 : (double)best;
       out4[1] = tbest;
-      out4[2] = (double)c->nflat;
-      out4[3] = (double)cand.size();
+      out4[2] = best_persist ? (double)c->pb.nb : (double)c->nflat;
+      out4[3] = (double)cand.size() + (can_persist ? 1.0 : 0.0);
     }
   } catch (const HipError &e) {
     return hip_fail(c, e);
@@ -2054,6 +2303,41 @@ int adg_read_profile(adg_ctx *c, double *out6) {
 int adg_last_timing(adg_ctx *c, double *ms4) {
   if (!c || !ms4) return ADG_E_ARG;
   for (int k = 0; k < 4; ++k) ms4[k] = c->timing[k];
+  return ADG_OK;
+}
+
+int adg_persistent_plan(int nxx, int nyy, int nzz, int precision, int ncu, int *plan8) {
+  if (!plan8 || nxx < 1 || nyy < 1 || nzz < 1 || nxx > 1022 || nyy > 1022 || nzz > 1022 || ncu < 8 ||
+      (precision != ADG_FP64 && precision != ADG_FP32_REFINE))
+    return ADG_E_ARG;
+  for (int k = 0; k < 8; ++k) plan8[k] = 0;
+  const long sy = nxx + 2, sz = sy * (nyy + 2), box = sz * (nzz + 2);
+  Brick P{};
+  int kc = 0;
+  if (!plan_bricks(nxx, nyy, nzz, 0, sy, sz, box, precision == ADG_FP32_REFINE, ncu, P, kc)) return ADG_OK;
+  plan8[0] = 1;
+  plan8[1] = P.nb;
+  plan8[2] = P.BX;
+  plan8[3] = P.BY;
+  plan8[4] = P.BZ;
+  plan8[5] = kc;
+  plan8[6] = P.H;
+  plan8[7] = verify_plan(P, kc, box).empty() ? 0 : 1;
+  return ADG_OK;
+}
+
+int adg_solver_info(adg_ctx *c, int *info8) {
+  if (!c || !info8) return ADG_E_ARG;
+  const Brick &P = c->pb;
+  const bool on = c->persist;
+  info8[0] = on ? 1 : 0;
+  info8[1] = on ? P.nb : c->nflat;
+  info8[2] = on ? PNT : NTF;
+  info8[3] = on ? P.BX : 0;
+  info8[4] = on ? P.BY : 0;
+  info8[5] = on ? P.BZ : 0;
+  info8[6] = on ? c->pkc : 0;
+  info8[7] = on ? P.H : 0;
   return ADG_OK;
 }
 

@@ -27,7 +27,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("ADG_LIBRARY") or os.path.join(HERE, "libadpres_gpu.so")
 
 ADG_OK, ADG_E_MAX_OUTER, ADG_E_NODAL_UNSTABLE, ADG_E_LU_PIVOT, ADG_E_POWER_NONPOS = 0, 1, 2, 3, 4
-ADG_E_ARG, ADG_E_HIP, ADG_E_NOT_SET, ADG_E_XS = 5, 6, 7, 8
+ADG_E_ARG, ADG_E_HIP, ADG_E_NOT_SET, ADG_E_XS, ADG_E_DEVICE_SYNC = 5, 6, 7, 8, 9
 ADG_FORWARD, ADG_ADJOINT, ADG_FIXED_SOURCE, ADG_TRANSIENT, ADG_ADJOINT_FROZEN = 0, 1, 2, 3, 4   # enum adg_mode
 FP64, FP32_REFINE = 0, 1
 PRECISIONS = {"fp64": FP64, "fp32_refine": FP32_REFINE}
@@ -43,7 +43,8 @@ EXPORTS = ("adg_create", "adg_destroy", "adg_set_xs", "adg_set_state", "adg_get_
            "adg_set_xs_tables", "adg_set_xs_tables_core", "adg_update_xs", "adg_update_xs_core",
            "adg_get_xs", "adg_get_xs_core", "adg_set_xtab_tables", "adg_set_xtab_tables_core",
            "adg_power_maps", "adg_power_maps_core", "adg_flux_map", "adg_flux_map_core",
-           "adg_status_string", "adg_last_error")
+           "adg_solver_info", "adg_persistent_plan", "adg_status_string",
+           "adg_last_error")
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
@@ -112,6 +113,8 @@ def load_library(path: str = LIB_PATH):
     L.adg_set_profiling.argtypes = [C.c_void_p, C.c_int]
     L.adg_read_profile.argtypes = [C.c_void_p, _dp]
     L.adg_autotune.argtypes = [C.c_void_p, C.c_int, _dp]
+    L.adg_solver_info.argtypes = [C.c_void_p, _ip]
+    L.adg_persistent_plan.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _ip]
     L.adg_create_batch.argtypes = [C.POINTER(AdgGeom), C.c_int, C.POINTER(C.c_void_p)]
     L.adg_core_count.argtypes = [C.c_void_p]
     L.adg_set_xs_core.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _ip, _dp, _dp]
@@ -414,10 +417,18 @@ class Engine:
         return y
 
     def autotune(self, nin: Optional[int] = None) -> dict:
-        """Pick the fastest launch shape of the flat kernels (see adg_autotune)."""
+        """Pick the fastest BiCGSTAB form and launch shape (see adg_autotune);
+        shape 0 is the persistent kernel."""
         out = np.zeros(4)
         self._check(self.lib.adg_autotune(self.h, int(nin or self.p.nin), _d(out)), "adg_autotune")
         return dict(shape=int(out[0]), group_solve_ms=float(out[1]), grid=int(out[2]), candidates=int(out[3]))
+
+    def solver_info(self) -> dict:
+        """How the group solves run (adg_solver_info)."""
+        i = np.zeros(8, np.int32)
+        self._check(self.lib.adg_solver_info(self.h, _i(i)), "adg_solver_info")
+        return dict(persistent=bool(i[0]), blocks=int(i[1]), threads=int(i[2]), brick=(int(i[3]), int(i[4]), int(i[5])),
+                    nodes_per_thread=int(i[6]), halo=int(i[7]))
 
     # -- differentiation ----------------------------------------------------
     def adjoint(self, **ov) -> float:
@@ -580,3 +591,15 @@ class Batch:
         self._check(self.lib.adg_trace_core(self.h, b, C.byref(tr)), "adg_trace_core")
         n = tr.n
         return {k: (v[:n] if k != "loc" else v[:3 * n].reshape(-1, 3)) for k, v in t.items()}
+
+
+def persistent_plan(nxx: int, nyy: int, nzz: int, precision: str = "fp64", ncu: int = 256) -> dict:
+    """The persistent BiCGSTAB kernel's brick plan for a node box, made and
+    checked on the host (adg_persistent_plan; no GPU needed)."""
+    L = load_library()
+    out = np.zeros(8, np.int32)
+    rc = L.adg_persistent_plan(int(nxx), int(nyy), int(nzz), PRECISIONS[precision], int(ncu), _i(out))
+    if rc != ADG_OK:
+        raise ADGError(rc, "adg_persistent_plan: bad argument")
+    return dict(ok=bool(out[0]), blocks=int(out[1]), brick=(int(out[2]), int(out[3]), int(out[4])),
+                nodes_per_thread=int(out[5]), halo=int(out[6]), checked=out[7] == 0)

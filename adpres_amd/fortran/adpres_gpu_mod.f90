@@ -15,7 +15,8 @@ module adpres_gpu
   integer(c_int), parameter :: ADG_FORWARD = 0, ADG_ADJOINT = 1, ADG_FIXED_SOURCE = 2, ADG_TRANSIENT = 3
   integer(c_int), parameter :: ADG_ADJOINT_FROZEN = 4
   integer(c_int), parameter :: ADG_FP64 = 0, ADG_FP32_REFINE = 1
-  integer(c_int), parameter :: ADG_E_ARG = 5, ADG_E_HIP = 6, ADG_E_NOT_SET = 7, ADG_E_XS = 8
+  integer(c_int), parameter :: ADG_E_ARG = 5, ADG_E_HIP = 6, ADG_E_NOT_SET = 7, ADG_E_XS = 8, &
+                               ADG_E_DEVICE_SYNC = 9
 
   type, bind(C) :: adg_geom_t
     integer(c_int) :: nnod, ng, nmat, nxx, nyy, nzz

@@ -26,7 +26,7 @@ module cmfd
   implicit none
   save
   private
-  public :: outer, outer_ad, outer_fs, outer_th, outer_tr, PowDis, Integrate, leakage_of
+  public :: outer, outer_ad, outer_fs, outer_th, outer_tr, PowDis, Integrate, leakage_of, dev_xs_update
 
   type(c_ptr) :: ctx = c_null_ptr
   logical     :: started = .false.
@@ -37,6 +37,15 @@ module cmfd
   real(c_double), allocatable, target :: gxd(:), gyd(:), gzd(:)
   real(c_double), allocatable, target :: tk(:), tser(:), tfer(:), tnd(:)
   integer(c_int), allocatable, target :: tfl(:), tloc(:)
+  ! node cross sections made on the device (dev_xs_update, behind the
+  ! xsec module's XS_updt / XStab_updt): the tables are installed once, and
+  ! xs_dev says the device holds the node cross sections of the last
+  ! update, so the next outer uploads nothing
+  logical :: xs_dev = .false.
+  logical :: tabs_in = .false.
+  logical :: host_xs_filled = .false.
+  real(c_double), allocatable, target :: tbuf(:)        ! feedback tables, or branch points and records
+  integer(c_int), allocatable, target :: tdims(:), tfb(:), tmat(:)
 
 contains
 
@@ -63,7 +72,11 @@ contains
       write(*,*) '  GPU ENGINE INITIALISATION FAILED, STATUS ', rc
       stop
     end if
-    rc = adg_set_profiling(ctx, 1_c_int)
+    ! per-phase events only on request: they turn the replayed outer graph
+    ! into direct launches; fdm_time / nod_time come from adg_last_timing
+    ! either way
+    call get_environment_variable('ADPRES_GPU_PROFILE', pstr)
+    if (trim(pstr) == '1') rc = adg_set_profiling(ctx, 1_c_int)
   end subroutine ensure_context
 
   subroutine check(rc, what)
@@ -100,7 +113,8 @@ contains
     fixed_mode = mode == ADG_FIXED_SOURCE
     allocate(imat(nnod))
     imat = mat
-    call check(adg_set_xs(ctx, D, sigr, sigs, nuf, sigf, chi, imat, dc, exsrc), 'ADG_SET_XS')
+    ! the device's own cross sections of the last XS_updt need no upload
+    if (.not. xs_dev) call check(adg_set_xs(ctx, D, sigr, sigs, nuf, sigf, chi, imat, dc, exsrc), 'ADG_SET_XS')
     first = 0
     if (.not. started) then
       if (.not. allocated(f0)) allocate(f0(nnod, ng))
@@ -184,6 +198,220 @@ contains
       end if
     end if
   end subroutine run_outer
+
+  ! xsec::XS_updt (kind 0, src/mod_xsec.f90:11-46) and xsec::XStab_updt
+  ! (kind 1, :48-84) on the device: the tables (feedback tables, or the
+  ! branch tables of a %XTAB deck) are installed on the first call -- and
+  ! on every call of a transient, whose host changes them (KNE1 scaling of
+  ! xnuf, src/mod_trans.f90:509-510) -- and adg_update_xs makes the node
+  ! cross sections from the parameters.  The host's node arrays D, sigr,
+  ! nuf, sigf, sigs and dc are filled from the device on the first call and
+  ! in the modes whose host reads or changes them (transient: sigr gets the
+  ! time terms; fixed source: the external source goes with the upload);
+  ! otherwise they are left as they were and the next outer uploads nothing.
+  subroutine dev_xs_update(kind, xbcon, xftem, xmtem, xcden, xbpos)
+    use sdata, only: nnod, ng, dc, D, sigr, nuf, sigf, sigs, mode, xs_time
+    use io, only: ounit, bftem, bmtem, bcden, bcrod
+    integer, intent(in) :: kind
+    real(dp), intent(in) :: xbcon
+    real(dp), dimension(:), intent(in) :: xftem, xmtem, xcden, xbpos
+    real(c_double), allocatable, target :: pf(:), pm(:), pc(:), pb(:)
+    real(c_double), allocatable, target :: gD(:,:), gr(:,:), gn(:,:), gf(:,:), gs(:,:,:), gdc(:,:,:)
+    real(c_double) :: ms(4)
+    type(c_ptr) :: af, am, ac, ab
+    integer(c_int) :: rc
+    logical :: host
+    call ensure_context()
+    host = (.not. host_xs_filled) .or. mode == 'RODEJECT' .or. mode == 'FIXEDSRC'
+    if (.not. tabs_in .or. mode == 'RODEJECT') then
+      if (kind == 0) then
+        call install_feedback_tables()
+      else
+        call install_branch_tables()
+      end if
+      tabs_in = .true.
+    end if
+    ! the parameters of the active feedbacks (a branch-table deck reads the
+    ! three node-wise ones always, src/mod_xsec.f90:70-73)
+    af = c_null_ptr; am = c_null_ptr; ac = c_null_ptr; ab = c_null_ptr
+    if (kind == 1 .or. bftem == 1) then
+      allocate(pf(nnod)); pf = xftem(1:nnod); af = c_loc(pf)
+    end if
+    if (kind == 1 .or. bmtem == 1) then
+      allocate(pm(nnod)); pm = xmtem(1:nnod); am = c_loc(pm)
+    end if
+    if (kind == 1 .or. bcden == 1) then
+      allocate(pc(nnod)); pc = xcden(1:nnod); ac = c_loc(pc)
+    end if
+    if (bcrod == 1) then
+      allocate(pb(size(xbpos))); pb = xbpos; ab = c_loc(pb)
+    end if
+    rc = adg_update_xs(ctx, real(xbcon, c_double), af, am, ac, ab)
+    if (rc == ADG_E_XS) then
+      write(*,*)
+      write(*,*) '  ERROR IN THE CROSS SECTIONS: ', trim(c_string(adg_last_error(ctx)))
+      write(ounit,*) '  ERROR IN THE CROSS SECTIONS: ', trim(c_string(adg_last_error(ctx)))
+      stop
+    end if
+    call check(rc, 'ADG_UPDATE_XS')
+    rc = adg_last_timing(ctx, ms)
+    xs_time = xs_time + ms(1) * 1.e-3_dp
+    if (host) then
+      allocate(gD(nnod, ng), gr(nnod, ng), gn(nnod, ng), gf(nnod, ng), gs(nnod, ng, ng), gdc(nnod, ng, 6))
+      call check(adg_get_xs(ctx, c_loc(gD), c_loc(gr), c_loc(gn), c_loc(gf), c_loc(gs), c_loc(gdc)), 'ADG_GET_XS')
+      D = gD; sigr = gr; nuf = gn; sigf = gf; sigs = gs
+      if (kind == 1) dc = gdc
+      host_xs_filled = .true.
+    end if
+    xs_dev = .not. (mode == 'RODEJECT' .or. mode == 'FIXEDSRC')
+  end subroutine dev_xs_update
+
+  ! sdata's feedback tables (base, boron, fuel and moderator temperature,
+  ! coolant density, rods) into one buffer, with pointers per kind and
+  ! cross section as adg_xs_tables takes them; a kind whose card is absent
+  ! gets no pointers
+  subroutine install_feedback_tables()
+    use sdata, only: nnod, ng, nmat, nxx, nyy, mat, chi, dc, nb, rbcon, rftem, rmtem, rcden, coreh, pos0, &
+                     ssize, fbmap, xsigtr, xsiga, xnuf, xsigf, xsigs, csigtr, csiga, cnuf, csigf, csigs, &
+                     fsigtr, fsiga, fnuf, fsigf, fsigs, msigtr, msiga, mnuf, msigf, msigs, &
+                     lsigtr, lsiga, lnuf, lsigf, lsigs, dsigtr, dsiga, dnuf, dsigf, dsigs
+    use io, only: bbcon, bcbcs, bftem, bmtem, bcden, bcrod
+    type(adg_xs_tables_t) :: t
+    integer :: kd, q, off, n2, n3
+    logical :: on(6)
+    n2 = nmat * ng
+    n3 = nmat * ng * ng
+    on = [.true., bbcon == 1 .or. bcbcs == 1, bftem == 1, bmtem == 1, bcden == 1, bcrod == 1]
+    if (allocated(tbuf)) deallocate(tbuf)
+    allocate(tbuf(6 * (4 * n2 + n3)))
+    tbuf = 0._c_double
+    t%nmat = nmat
+    t%nb = 0
+    if (bcrod == 1) t%nb = nb
+    t%rbcon = rbcon; t%rftem = rftem; t%rmtem = rmtem; t%rcden = rcden
+    t%coreh = coreh; t%pos0 = pos0; t%ssize = ssize
+    t%tab = c_null_ptr
+    t%fbmap = c_null_ptr
+    off = 0
+    do kd = 1, 6
+      if (on(kd)) then
+        select case (kd)
+        case (1)
+          call put5(xsigtr, xsiga, xnuf, xsigf, xsigs)
+        case (2)
+          call put5(csigtr, csiga, cnuf, csigf, csigs)
+        case (3)
+          call put5(fsigtr, fsiga, fnuf, fsigf, fsigs)
+        case (4)
+          call put5(msigtr, msiga, mnuf, msigf, msigs)
+        case (5)
+          call put5(lsigtr, lsiga, lnuf, lsigf, lsigs)
+        case (6)
+          call put5(dsigtr, dsiga, dnuf, dsigf, dsigs)
+        end select
+        do q = 1, 5
+          t%tab(q, kd) = c_loc(tbuf(off + 1 + (q - 1) * n2))
+        end do
+      end if
+      off = off + 4 * n2 + n3
+    end do
+    if (bcrod == 1) then
+      if (allocated(tfb)) deallocate(tfb)
+      allocate(tfb(nxx * nyy))
+      tfb = reshape(fbmap(1:nxx, 1:nyy), [nxx * nyy])
+      t%fbmap = c_loc(tfb)
+    end if
+    if (allocated(tmat)) deallocate(tmat)
+    allocate(tmat(nnod))
+    tmat = mat
+    call check(adg_set_xs_tables(ctx, t, tmat, chi, dc), 'ADG_SET_XS_TABLES')
+  contains
+    subroutine put5(a, b, c, d, e)
+      real(dp), intent(in) :: a(:,:), b(:,:), c(:,:), d(:,:), e(:,:,:)
+      tbuf(off + 1:off + n2) = reshape(a(1:nmat, 1:ng), [n2])
+      tbuf(off + n2 + 1:off + 2 * n2) = reshape(b(1:nmat, 1:ng), [n2])
+      tbuf(off + 2 * n2 + 1:off + 3 * n2) = reshape(c(1:nmat, 1:ng), [n2])
+      tbuf(off + 3 * n2 + 1:off + 4 * n2) = reshape(d(1:nmat, 1:ng), [n2])
+      tbuf(off + 4 * n2 + 1:off + 4 * n2 + n3) = reshape(e(1:nmat, 1:ng, 1:ng), [n3])
+    end subroutine
+  end subroutine install_feedback_tables
+
+  ! sdata's branch tables m(:) of a %XTAB deck, flattened as
+  ! adg_set_xtab_tables takes them: per material (nd, nb, nf, nm, trod),
+  ! its branch points, and its records (density fastest, then boron, fuel,
+  ! moderator temperature) of sigtr, siga, nuf, sigf, sigs, dc in storage
+  ! order; the rodded records of the materials with a rodded set after them
+  subroutine install_branch_tables()
+    use sdata, only: nnod, ng, nmat, nxx, nyy, mat, chi, nb, coreh, pos0, ssize, fbmap, m
+    use io, only: bcrod
+    type(adg_xtab_tables_t) :: t
+    real(c_double), allocatable, target :: rbuf(:)
+    integer :: mm, s, tt, q, v, E, npt, nrec, nrrec, o, orr
+    E = 4 * ng + ng * ng + 6 * ng
+    npt = 0; nrec = 0; nrrec = 0
+    do mm = 1, nmat
+      npt = npt + m(mm)%nd + m(mm)%nb + m(mm)%nf + m(mm)%nm
+      nrec = nrec + m(mm)%nd * m(mm)%nb * m(mm)%nf * m(mm)%nm
+      if (m(mm)%trod == 1) nrrec = nrrec + m(mm)%nd * m(mm)%nb * m(mm)%nf * m(mm)%nm
+    end do
+    if (allocated(tbuf)) deallocate(tbuf)
+    if (allocated(tdims)) deallocate(tdims)
+    allocate(tbuf(npt + nrec * E), tdims(5 * nmat), rbuf(max(1, nrrec * E)))
+    o = 0
+    do mm = 1, nmat
+      tdims(5 * (mm - 1) + 1:5 * mm) = [m(mm)%nd, m(mm)%nb, m(mm)%nf, m(mm)%nm, m(mm)%trod]
+      tbuf(o + 1:o + m(mm)%nd) = m(mm)%pd(1:m(mm)%nd); o = o + m(mm)%nd
+      tbuf(o + 1:o + m(mm)%nb) = m(mm)%pb(1:m(mm)%nb); o = o + m(mm)%nb
+      tbuf(o + 1:o + m(mm)%nf) = m(mm)%pf(1:m(mm)%nf); o = o + m(mm)%nf
+      tbuf(o + 1:o + m(mm)%nm) = m(mm)%pm(1:m(mm)%nm); o = o + m(mm)%nm
+    end do
+    orr = 0
+    do mm = 1, nmat
+      do v = 1, m(mm)%nm
+        do q = 1, m(mm)%nf
+          do tt = 1, m(mm)%nb
+            do s = 1, m(mm)%nd
+              call put_rec(tbuf, o, m(mm)%xsec(s, tt, q, v))
+              if (m(mm)%trod == 1) call put_rec(rbuf, orr, m(mm)%rxsec(s, tt, q, v))
+            end do
+          end do
+        end do
+      end do
+    end do
+    t%nmat = nmat
+    t%nb = 0
+    if (bcrod == 1) t%nb = nb
+    t%dims = c_loc(tdims)
+    t%points = c_loc(tbuf(1))
+    t%xsec = c_loc(tbuf(npt + 1))
+    t%rxsec = c_null_ptr
+    if (nrrec > 0) t%rxsec = c_loc(rbuf)
+    t%coreh = coreh; t%pos0 = pos0; t%ssize = ssize
+    t%fbmap = c_null_ptr
+    if (bcrod == 1) then
+      if (allocated(tfb)) deallocate(tfb)
+      allocate(tfb(nxx * nyy))
+      tfb = reshape(fbmap(1:nxx, 1:nyy), [nxx * nyy])
+      t%fbmap = c_loc(tfb)
+    end if
+    if (allocated(tmat)) deallocate(tmat)
+    allocate(tmat(nnod))
+    tmat = mat
+    call check(adg_set_xtab_tables(ctx, t, tmat, chi), 'ADG_SET_XTAB_TABLES')
+  contains
+    subroutine put_rec(buf, off, x)
+      use sdata, only: XBRANCH
+      real(c_double), intent(inout) :: buf(:)
+      integer, intent(inout) :: off
+      type(XBRANCH), intent(in) :: x
+      buf(off + 1:off + ng) = x%sigtr(1:ng); off = off + ng
+      buf(off + 1:off + ng) = x%siga(1:ng); off = off + ng
+      buf(off + 1:off + ng) = x%nuf(1:ng); off = off + ng
+      buf(off + 1:off + ng) = x%sigf(1:ng); off = off + ng
+      buf(off + 1:off + ng * ng) = reshape(x%sigs, [ng * ng]); off = off + ng * ng
+      buf(off + 1:off + 6 * ng) = reshape(x%dc, [6 * ng]); off = off + 6 * ng
+    end subroutine
+  end subroutine install_branch_tables
 
   subroutine say(dummy, line)
     use io, only: ounit, scr

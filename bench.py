@@ -15,7 +15,9 @@ Also reported:
                 solve in a profiled solve; traffic from the committed
                 rocprofv3 counter summary of the same kernel;
   cpu_baseline  the reference's own Fortran build (oracle/_ref) on one host
-                core, on a bounded sample of the same core.
+                core, on a bounded sample of the same core (outer and nodal
+                update times measured free of the deck parse), with the
+                host's CPU model and core count.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
 """
@@ -154,6 +156,10 @@ def batch_bench(args):
 
     r32 = run("fp32_refine")
     r64 = run("fp64") if not args.no_fp32 else None
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline_batch([batch_core(BATCH_SEED, i) for i in range(256)])
+        log(f"bench: batch256 cpu baseline {cpu}")
     out = {
         "metric": "k-eff solves/s (256-core batch, FP32+refine; FP64 beside it) at 1 pcm",
         "value": round(r32["solves_per_s"], 3), "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
@@ -163,7 +169,7 @@ def batch_bench(args):
         "config": {"workload": "batch256", "cores": len(cores), "nodes": int(sum(q.nnod for q in cores)),
                    "unknowns": int(sum(q.unknowns for q in cores)),
                    "contexts": {str(G): len(idx) for G, idx in classes.items()}},
-        "roofline": r32["roofline"], "fp32_refine": r32, "fp64": r64}
+        "roofline": r32["roofline"], "cpu_baseline": cpu, "fp32_refine": r32, "fp64": r64}
     print(json.dumps(out))
 
 
@@ -266,47 +272,184 @@ def xs_update_probe(eng, p, workload, seed, reps=10):
     return out
 
 
-def cpu_baseline(problem, outers_per_solve, nupd):
-    """The reference build (oracle/_ref/ref_dump: ADPRES's Fortran, amdflang -O3,
-    one host core) on the same core.  Small cores run the whole solve with the
-    same %ITER; large ones a bounded sample of NS outers (one of them with a
-    nodal update) scaled to the GPU solve's outer count."""
-    ref = os.path.join(ROOT, "oracle", "_ref", "ref_dump")
+def host_cpu():
+    """CPU model and the number of host cores this process may use."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    ncpu = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    return model, ncpu
+
+
+def ref_solve(gc, it, workdir, kern=None):
+    """One run of the reference build (oracle/_ref/ref_dump: ADPRES's Fortran,
+    amdflang -O3) on the deck of gc with %ITER it = (nout, nin, serc, ferc,
+    nac, nupd); returns (wall seconds incl. deck parse, the reference's own
+    (CMFD, nodal) CPU-time buckets or None when it stopped before printing
+    them, whether it converged)."""
     from adpres_amd.problem import write_deck, write_xs_override
+    ref = os.path.join(ROOT, "oracle", "_ref", "ref_dump")
+    os.makedirs(workdir, exist_ok=True)
+    write_deck(os.path.join(workdir, "deck"), kern=kern or gc.problem.kern, iter_card=it, **gc.deck)
+    args = [ref, "deck", "solve", "out.bin"]
+    if gc.perturbed:
+        write_xs_override(os.path.join(workdir, "xs.bin"), gc.problem)
+        args.append("xs.bin")
+    t0 = time.perf_counter()
+    r = subprocess.run(["bash", "-c", "ulimit -s unlimited; exec " + " ".join(args)], cwd=workdir,
+                       capture_output=True, text=True)
+    wall = time.perf_counter() - t0
+    buckets = None
+    for line in r.stdout.splitlines():
+        if line.strip().startswith("REF_TIMES"):
+            f = line.split()
+            buckets = (float(f[1]), float(f[2]))
+    return wall, buckets, buckets is not None
+
+
+def cpu_baseline(problem, outers_per_solve, nupd):
+    """The reference build on one host core on the same core.  Small cores:
+    the whole solve with the same %ITER, timed by the reference's own CPU-time
+    buckets (CMFD + nodal update, no deck parse).  Large cores: three bounded
+    runs at once on three host cores, each stopped at NOUT by the reference's
+    max-outer stop -- A: 2 outers, B: 4 outers (no nodal update in either),
+    C: 2 outers with a nodal update at the second -- give the time of one
+    outer, (B - A) / 2, and of one nodal update, C - A, free of the deck
+    parse; the solve is then the GPU solve's outers and nodal updates at
+    those costs.  (The reference refuses NUPD >= NOUT, so A and B run the
+    FDM kernel -- the same CMFD outer without nodal updates -- and C, the
+    deck's kernel, 3 outers with NUPD = 2: its nodal update is
+    C - A - t_outer.)"""
     from adpres_amd.configs import generate
-    if not os.path.exists(ref):
+    if not os.path.exists(os.path.join(ROOT, "oracle", "_ref", "ref_dump")):
         return None
+    import threading
+    model, ncpu = host_cpu()
     gc = generate(problem.name, WORKLOADS[problem.name]["seed"])
     p = gc.problem
-    full = p.nnod * p.ng <= 200_000
     d = tempfile.mkdtemp(prefix="adg_cpu_")
     try:
-        if full:
-            it = (p.nout, p.nin, p.serc, p.ferc, p.nac, p.nupd)
-        else:
-            ns = 3
-            it = (ns, p.nin, 1e-30, 1e-30, p.nac, ns)
-        write_deck(os.path.join(d, "deck"), kern=p.kern, iter_card=it, **gc.deck)
-        args = [ref, "deck", "solve", "out.bin"]
-        if gc.perturbed:
-            write_xs_override(os.path.join(d, "xs.bin"), p)
-            args.append("xs.bin")
-        t0 = time.perf_counter()
-        r = subprocess.run(["bash", "-c", "ulimit -s unlimited; exec " + " ".join(args)], cwd=d,
-                           capture_output=True, text=True)
-        wall = time.perf_counter() - t0
-        if full:
-            if r.returncode != 0:
+        if p.nnod * p.ng <= 200_000:
+            wall, bk, conv = ref_solve(gc, (p.nout, p.nin, p.serc, p.ferc, p.nac, p.nupd), os.path.join(d, "full"))
+            if not conv:
                 return None
-            return {"value": 1.0 / wall, "unit": "solves/s", "cores": 1, "kind": "reference",
-                    "sample": f"one whole solve of the same core by ADPRES's Fortran (amdflang -O3, "
-                              f"same %ITER), {wall:.2f} s wall incl. deck parse"}
-        # the sample stops at NOUT (the reference's max-outer STOP): ns outers + 1 nodal update
-        t_outer = wall / (ns + ns / nupd)
-        solve_s = t_outer * (outers_per_solve + outers_per_solve / nupd)
+            solve_s = bk[0] + bk[1]
+            return {"value": 1.0 / solve_s, "unit": "solves/s", "cores": 1, "kind": "reference",
+                    "sample": "one whole solve of the same core by ADPRES's Fortran (amdflang -O3, same %ITER), "
+                              "timed by its own CPU-time buckets",
+                    "cpu_model": model, "host_cores": ncpu, "cmfd_s": bk[0], "nodal_s": bk[1],
+                    "wall_incl_parse_s": round(wall, 3), "method": "reference buckets fdm_time + nod_time"}
+        runs = {"A": ((2, p.nin, 1e-30, 1e-30, p.nac, 1), "FDM"), "B": ((4, p.nin, 1e-30, 1e-30, p.nac, 1), "FDM"),
+                "C": ((3, p.nin, 1e-30, 1e-30, p.nac, 2), None)}
+        walls = {}
+
+        def go(tag):
+            walls[tag] = ref_solve(gc, runs[tag][0], os.path.join(d, tag), kern=runs[tag][1])[0]
+        th = [threading.Thread(target=go, args=(t,)) for t in runs]
+        for t in th:
+            t.start()
+        for t in th:
+            t.join()
+        t_outer = (walls["B"] - walls["A"]) / 2.0
+        t_nodal = walls["C"] - walls["A"] - t_outer
+        if not t_outer > 0.0:
+            return None
+        nnod_upd = outers_per_solve // nupd
+        solve_s = outers_per_solve * t_outer + nnod_upd * max(t_nodal, 0.0)
         return {"value": 1.0 / solve_s, "unit": "solves/s", "cores": 1, "kind": "reference",
-                "sample": f"ADPRES Fortran (amdflang -O3) on the same core: {ns} outers + 1 nodal update "
-                          f"({wall:.1f} s wall incl. deck parse), scaled to {outers_per_solve} outers"}
+                "sample": f"ADPRES Fortran (amdflang -O3) on the same core: runs of 2 and 4 FDM outers and of 3 "
+                          f"outers with a nodal update (walls {walls['A']:.1f}, {walls['B']:.1f}, {walls['C']:.1f} s, "
+                          f"concurrently on three host cores), composed into the GPU solve's "
+                          f"{outers_per_solve} outers and {nnod_upd} nodal updates",
+                "cpu_model": model, "host_cores": ncpu, "t_outer_s": round(t_outer, 4),
+                "t_nodal_s": round(t_nodal, 4), "outers": outers_per_solve, "nodal_updates": nnod_upd,
+                "method": "t_outer = (wall(4 FDM outers) - wall(2 FDM outers)) / 2, t_nodal = wall(3 outers with "
+                          "a nodal update at the 2nd) - wall(2 FDM outers) - t_outer; solve = outers * t_outer + "
+                          "nodal_updates * t_nodal"}
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+
+
+def cpu_baseline_batch(cores_gc, sample_every=8, workers=None):
+    """The batching workload's CPU baseline: the reference build solving a
+    bounded sample of the 256 cores (every sample_every-th) whole, one
+    process per host core (at most 16), aggregate converged solves per
+    wall-second of that pool."""
+    if not os.path.exists(os.path.join(ROOT, "oracle", "_ref", "ref_dump")):
+        return None
+    from concurrent.futures import ThreadPoolExecutor
+    model, ncpu = host_cpu()
+    nw = workers or max(1, min(16, ncpu))
+    idx = list(range(0, len(cores_gc), sample_every))
+    d = tempfile.mkdtemp(prefix="adg_cpu_batch_")
+    try:
+        def one(i):
+            q = cores_gc[i].problem
+            return ref_solve(cores_gc[i], (q.nout, q.nin, q.serc, q.ferc, q.nac, q.nupd), os.path.join(d, str(i)))
+        t0 = time.perf_counter()
+        with ThreadPoolExecutor(max_workers=nw) as ex:
+            res = list(ex.map(one, idx))
+        wall = time.perf_counter() - t0
+        conv = sum(1 for r in res if r[2])
+        return {"value": conv / wall, "unit": "solves/s", "cores": nw, "kind": "reference",
+                "sample": f"{len(idx)} of the {len(cores_gc)} cores (every {sample_every}th) solved whole by "
+                          f"ADPRES's Fortran (amdflang -O3), {nw} processes at once on {nw} host cores; "
+                          f"{conv} converged (the rest stop as the reference does) in {wall:.1f} s wall",
+                "cpu_model": model, "host_cores": ncpu,
+                "cmfd_plus_nodal_s_per_solve": round(float(np.mean([r[1][0] + r[1][1] for r in res if r[2]])), 3)
+                if conv else None}
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+
+
+def dropin_leg(workload, seed):
+    """The Fortran drop-in (adpres_amd/fortran/_build/adpres_gpu: ADPRES's
+    unchanged host linked with the GPU hot path) on the workload's deck with
+    its %ITER: one run from the deck, as a user runs it.  The solve phase is
+    the host's own CMFD + nodal-update buckets (filled from the engine's
+    device timers of the outer call); the wall time adds the deck parse, the
+    host's cross-section processing, context creation, the upload and the
+    printed outputs."""
+    import re
+    from adpres_amd.configs import generate
+    from adpres_amd.problem import write_deck
+    exe = os.path.join(ROOT, "adpres_amd", "fortran", "_build", "adpres_gpu")
+    if not os.path.exists(exe):
+        return None
+    gc = generate(workload, seed)
+    if gc.perturbed:
+        return None    # per-node perturbations are not expressible in a deck
+    p = gc.problem
+    d = tempfile.mkdtemp(prefix="adg_dropin_")
+    try:
+        write_deck(os.path.join(d, "deck"), kern=p.kern, iter_card=(p.nout, p.nin, p.serc, p.ferc, p.nac, p.nupd),
+                   **gc.deck)
+        t0 = time.perf_counter()
+        r = subprocess.run(["bash", "-c", f"ulimit -s unlimited; exec {exe} deck"], cwd=d, capture_output=True,
+                           text=True)
+        wall = time.perf_counter() - t0
+        if r.returncode != 0 or "ADPRES EXIT NORMALLY" not in r.stdout:
+            log(f"bench: drop-in run failed: {r.stdout[-800:]} {r.stderr[-800:]}")
+            return None
+        text = open(os.path.join(d, "deck.out")).read()
+
+        def bucket(name):
+            m = re.search(name + r"\s*:\s*([0-9.]+)", text)
+            return float(m.group(1)) if m else None
+        cmfd, nod = bucket("CMFD time"), bucket("Nodal update time")
+        k = float(re.search(r"MULTIPLICATION EFFECTIVE \(K-EFF\) =\s+([0-9.]+)", text).group(1))
+        outers = len(re.findall(r"^\s*\d+\s+\d+\.\d{6}\s+\S+E[+-]\d\d\s+\S+E[+-]\d\d\s*$", text, re.M))
+        return {"solves_per_s": round(1.0 / (cmfd + nod), 4) if cmfd and nod is not None else None,
+                "cmfd_s": cmfd, "nodal_s": nod, "wall_s": round(wall, 2), "k": k, "outers": outers,
+                "input_s": bucket("Input reading time"), "xsec_s": bucket("XSEC processing time"),
+                "how": "ADPRES's host with the GPU hot path on the workload's deck (no profiling: the outer is "
+                       "replayed as a captured graph); solves/s from its CMFD + nodal-update buckets"}
     finally:
         shutil.rmtree(d, ignore_errors=True)
 
@@ -417,6 +560,7 @@ def main():
     ap.add_argument("--workload", default="core3d_8g", choices=sorted(WORKLOADS) + ["batch256"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--no-dropin", action="store_true", help="skip the Fortran drop-in leg")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch kernels directly instead of replaying the captured outer (for profilers)")
     ap.add_argument("--kernel-probe", action="store_true",
@@ -452,7 +596,8 @@ def main():
 
     eng = Engine(p, "fp64")
     tune64 = eng.autotune()
-    log(f"bench: autotune fp64 {tune64}")
+    info64 = eng.solver_info()
+    log(f"bench: autotune fp64 {tune64} {info64}")
     r64 = time_solves(eng, args.steps, args.warmup)
     # the boundary takes host arrays: one cross-section upload (what a caller
     # that changes the cross sections between solves adds per solve)
@@ -466,7 +611,8 @@ def main():
     if not args.no_fp32:
         eng = Engine(p, "fp32_refine")
         tune32 = eng.autotune()
-        log(f"bench: autotune fp32 {tune32}")
+        info32 = eng.solver_info()
+        log(f"bench: autotune fp32 {tune32} {info32}")
         r32 = time_solves(eng, args.steps, args.warmup)
         eng.close()
         log(f"bench: fp32_refine {r32['solves_per_s']:.3f} solves/s k={r32['k']:.8f} outers={r32['niter']}")
@@ -478,11 +624,36 @@ def main():
         return dict(k_err_pcm=abs(r["k"] - float(ref["Ke"])) * 1e5,
                     pow_max_rel_err=float(np.max(np.abs(r["pow"][m] - ref["pow"][m]) / ref["pow"][m])))
 
-    # roofline of the BiCGSTAB stencil kernel KA (k_bicgA): r, p, v, t, x, rs and
-    # 7 matrix coefficients in; r, p, v, x out = 17 words per node per launch
-    k1_bytes = 17 * 8 * p.nnod
     roof = None
-    if r64["k1_avg_ms"]:
+    if r64["k1_avg_ms"] and info64["persistent"]:
+        # the persistent BiCGSTAB kernel: one launch per group solve; canonical
+        # count 11 words of setup + 31 words per iteration per node
+        kc = info64["nodes_per_thread"]
+        k1_bytes = (11 + 31 * p.nin) * 8 * p.nnod
+        ach = k1_bytes / (r64["k1_avg_ms"] * 1e-3) / 1e9
+        traffic = measured_traffic(args.workload, f"void adg::k_bicg_persist<double, {kc}>")
+        roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "traffic_source": f"profiles/pmc_bicg_{args.workload}.txt (rocprofv3 FETCH_SIZE x2 + WRITE_SIZE x1 "
+                                  f"KiB per dispatch, calibrated on a 256 MiB copy)",
+                "kernel": f"k_bicg_persist<double, {kc}> (one launch per BiCGSTAB group solve: setup and all "
+                          f"{p.nin} iterations, matrix and vectors kept in registers, brick halos exchanged)",
+                "algorithmic_bytes_per_launch": k1_bytes,
+                "algorithmic_count": f"(11 + 31 x nin) words x 8 B x {p.nnod} nodes (canonical BiCGSTAB count: "
+                                     f"11 words of setup and 31 per iteration per node)",
+                "avg_launch_us": round(r64["k1_avg_ms"] * 1e3, 3), "samples": r64["k1_samples"],
+                "timing": "HIP events attached to every persistent launch (hipExtLaunchKernelGGL start/stop events) "
+                          "on the engine stream, in a profiled solve after the timed ones",
+                "per_iteration_us": round(r64["k1_avg_ms"] * 1e3 / p.nin, 3),
+                "solver": info64,
+                "note": "the matrix and vectors stay on chip across the iterations, so the canonical bytes over the "
+                        "kernel time can exceed the HBM peak; traffic is what crosses HBM per launch",
+                "group_solve_avg_us": round(r64["gsolve_avg_ms"] * 1e3, 2),
+                "nodal_update_avg_us": round(r64["nodal_avg_ms"] * 1e3, 2)}
+    elif r64["k1_avg_ms"]:
+        # roofline of the BiCGSTAB stencil kernel KA (k_bicgA): r, p, v, t, x, rs and
+        # 7 matrix coefficients in; r, p, v, x out = 17 words per node per launch
+        k1_bytes = 17 * 8 * p.nnod
         ach = k1_bytes / (r64["k1_avg_ms"] * 1e-3) / 1e9
         roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(ach / HBM_PEAK_GBS, 4),
@@ -505,9 +676,14 @@ def main():
                              "frac": round(gf / 78600.0, 5),
                              "note": "LU_solve flops only (no-pivot LU of order 2G per interface, G per node and "
                                      "boundary); assembly, transcendentals and the D-hat epilogue not counted"}
+    dropin = None
+    if not args.no_dropin:
+        dropin = dropin_leg(args.workload, w["seed"])
+        log(f"bench: drop-in {dropin}")
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(p, r64["niter"], p.nupd)
+        log(f"bench: cpu baseline {cpu}")
 
     out = {
         "metric": "k-eff solves/s (FP64 GPU drop-in; FP32+refine beside it) at 1 pcm",
@@ -532,11 +708,13 @@ def main():
                            "solves_per_s": round(1.0 / (r64["seconds"] / args.steps + up_s), 4)},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "dropin": dropin,
     }
     if r32 is not None:
         out["fp32_refine"] = {"solves_per_s": round(r32["solves_per_s"], 4),
                               "speedup_vs_fp64": round(r32["solves_per_s"] / r64["solves_per_s"], 3),
                               "outers_per_solve": r32["niter"], "parity": errs(r32), "launch_shape": tune32,
+                              "solver": info32,
                               "k1_avg_us": round(r32["k1_avg_ms"] * 1e3, 3) if r32["k1_avg_ms"] else None}
     print(json.dumps(out))
 

@@ -36,8 +36,12 @@ enum adg_status {
   ADG_E_ARG = 5,            /* invalid argument / unsupported geometry              */
   ADG_E_HIP = 6,            /* HIP runtime error (adg_last_error has the text)       */
   ADG_E_NOT_SET = 7,        /* cross sections not set yet                            */
-  ADG_E_XS = 8              /* src/mod_xsec.f90:215,86-150 the reference STOPs:      */
+  ADG_E_XS = 8,             /* src/mod_xsec.f90:215,86-150 the reference STOPs:      */
                             /* sigtr < 1e-5, D ~ 0 or negative sigr, nuf or sigs     */
+  ADG_E_DEVICE_SYNC = 9     /* no reference counterpart: a device-wide wait of the   */
+                            /* persistent BiCGSTAB kernel timed out (its blocks were */
+                            /* not resident together, e.g. another process's         */
+                            /* persistent kernel on the same GPU)                    */
 };
 
 enum adg_kern { ADG_KERN_FDM = 0, ADG_KERN_PNM = 1, ADG_KERN_SANM = 2 };   /* %KERN */
@@ -222,6 +226,9 @@ int adg_matvec(adg_ctx *ctx, int g, const double *x, double *y);
  * and :356-378), measured with HIP events on the context's stream.
  * adg_last_timing: device time (ms) of the last adg_outer call,
  *   [0] total, [1] group solves (TSrc + BiCGSTAB), [2] nodal updates, [3] rest;
+ *   without profiling, [2] is estimated from events around the replays of
+ *   the nodal-update outers only (their time beyond the mean of the other
+ *   outers), [1] is the rest of the call and [3] zero;
  *   after adg_update_xs, [0] its kernel's time and [1..3] zero.
  * With profiling on (adg_set_profiling(ctx, 1)), adg_outer also brackets the
  * phases and one steady-state BiCGSTAB K1 launch per group solve with events;
@@ -232,13 +239,30 @@ int adg_last_timing(adg_ctx *ctx, double *ms4);
 /* Launch-shape autotuner (no counterpart in the reference): times one
  * BiCGSTAB group solve of nin iterations on the context's matrices for each
  * candidate shape of the flat kernels (one core: the largest grid, 256 to
- * 2048 blocks; a batch: 256, 512 or 1024 cells per block) and keeps the
- * fastest.  The solution state is left as it was; results stay within the
- * parity bars for any shape (only the summation order of the reductions
- * depends on it).  out4: chosen shape, its ms per group solve, grid size,
- * candidates tried (may be NULL). */
+ * 2048 blocks; a batch: 256, 512 or 1024 cells per block) and, for a single
+ * core with a brick plan, the persistent kernel, and keeps the fastest.
+ * The solution state is left as it was; results stay within the parity bars
+ * for any shape (only the summation order of the reductions depends on
+ * it).  out4: chosen shape (0: the persistent kernel), its ms per group
+ * solve, grid size, candidates tried (may be NULL). */
 int adg_autotune(adg_ctx *ctx, int nin, double *out4);
 int adg_set_profiling(adg_ctx *ctx, int on);
+/* How the context's group solves run (no counterpart in the reference):
+ * info8[0] 1 when one persistent launch per group solve keeps the
+ * BiCGSTAB state on chip for all nin iterations (single-core contexts whose
+ * brick plan fits the device; ADG_PERSIST=0 disables it), 0 for the flat
+ * two-kernels-per-iteration form; [1] blocks and [2] threads per block of
+ * the BiCGSTAB launches; with the persistent form [3..5] the brick extents
+ * BX, BY, BZ in nodes, [6] nodes per thread and [7] the largest halo of a
+ * brick. */
+int adg_solver_info(adg_ctx *ctx, int *info8);
+/* The persistent kernel's brick plan for an nxx x nyy x nzz node box on a
+ * device with ncu CUs, made and checked on the host (no device is touched):
+ * plan8[0] 1 when a plan exists, [1] blocks, [2..4] BX, BY, BZ, [5] nodes
+ * per thread, [6] largest halo, [7] 0 when every index the kernel forms is
+ * inside its buffers, every node is owned once and every stencil neighbour
+ * is on the tile (1 otherwise).  No counterpart in the reference. */
+int adg_persistent_plan(int nxx, int nyy, int nzz, int precision, int ncu, int *plan8);
 int adg_read_profile(adg_ctx *ctx, double *out6);
 
 /* Batching form.  The reference solves one core per process; these entry

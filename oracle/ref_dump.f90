@@ -111,6 +111,9 @@ program ref_dump
       end do
     end do
     write(u) Ke, f0, fs0, npw, dfout, dnout
+    ! the reference's own CPU-time buckets of the solve (CMFD, nodal update;
+    ! src/mod_cmfd.f90:442-489, :356-378), as ADPRES.f90 reports them
+    write(*,'(A,2ES16.8)') ' REF_TIMES ', fdm_time, nod_time
     ! a fixed-source deck's external source (%ESRC, mod_io.f90:1375) follows
     if (trim(mode) == 'FIXEDSRC') write(u) exsrc
 

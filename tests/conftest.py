@@ -57,3 +57,22 @@ def oracle_lib():
     from oracle import oracle
     oracle.lib()
     return oracle
+
+
+def lu_pivot_problem():
+    """The 18x18x2 two-group core with one fuel node's group-1 data set so
+    that the diagonal of its PNM node system (get_a2matvec's matrix
+    B E + 3 I, src/mod_nodal.f90:769-805, E = 2/7) vanishes at k = 1:
+    0.25 h^2 / D (sigr - nuf / k) = -10.5 with h = 10 cm, D = 1.4,
+    sigr = 0.05, nuf = 0.638.  The reference's LU_solve then stops with
+    "DIAGONAL ELEMENTS CLOSE TO ZERO" (src/mod_nodal.f90:856-862).
+    Returns (problem, node index, (i, j, k) of the node, 1-based)."""
+    import dataclasses
+    from adpres_amd.configs import generate
+    p = generate("core2d_small", 1).problem
+    n0 = int(np.nonzero((p.ix == 9) & (p.iy == 9) & (p.iz == 1))[0][0])
+    D, sigr, nuf, sigf = (np.array(a, copy=True) for a in (p.D, p.sigr, p.nuf, p.sigf))
+    D[0, n0], sigr[0, n0], nuf[0, n0] = 1.4, 0.05, 0.638
+    sigf[0, n0] = 0.638 / 2.43
+    q = dataclasses.replace(p, D=D, sigr=sigr, nuf=nuf, sigf=sigf, sigtr=1.0 / (3.0 * D))
+    return q, n0, (9, 9, 1)

@@ -199,9 +199,24 @@ def bcsearch():
 
 
 NEACRP_DECKS = ("A1", "B1", "C1")
+NEACRP_HFP = ("A2", "B2", "C2")     # the same searches at hot full power (output sweep only)
 NEACRP_FILES = ("neacrp_xsec", "neacrp_geom", "neacrp_cbcs", "neacrp_ftem", "neacrp_mtem", "neacrp_cden")
 SEARCH_TH_LINE = (r"^\s*(\d+)\s+(\d+\.\d\d)\s+(\d+\.\d{5})\s+(\S+E[+-]\d\d)\s+(\S+E[+-]\d\d)"
                   r"\s+(\S+E[+-]\d\d)\s*$")
+
+
+def neacrp_decks(decks):
+    """Copies of the NEACRP decks under decks/neacrp/ whose FILE cards name
+    the data files beside them (the shipped decks name the author's paths)."""
+    out = os.path.join(HERE, "decks", "neacrp")
+    os.makedirs(out, exist_ok=True)
+    for fn in NEACRP_FILES:
+        shutil.copy(os.path.join(REF, "NEACRP", fn), os.path.join(out, fn))
+    for deck in decks:
+        text = open(os.path.join(REF, "NEACRP", deck)).read().replace("/home/imronuke/ADPRES/smpl/static/NEACRP/",
+                                                                         "./")
+        with open(os.path.join(out, deck), "w") as fh:
+            fh.write(text)
 
 
 def neacrp():
@@ -217,6 +232,7 @@ def neacrp():
     os.makedirs(out, exist_ok=True)
     for fn in NEACRP_FILES:
         shutil.copy(os.path.join(REF, "NEACRP", fn), os.path.join(out, fn))
+    neacrp_decks(NEACRP_HFP)
     arrs = {}
     for deck in NEACRP_DECKS:
         text = open(os.path.join(REF, "NEACRP", deck)).read().replace("/home/imronuke/ADPRES/smpl/static/NEACRP/",
@@ -238,9 +254,11 @@ def neacrp():
     save("neacrp_search", **arrs)
 
 
+MOX_PART1BD = tuple(f"MOX/Part1b/aro{a}" for a in ("A1", "A3", "A5", "A7", "B6", "C3", "C7", "D6", "E5", "E7")) + \
+    tuple(f"MOX/Part1d/ari{a}" for a in ("A1", "A3", "A5", "A7", "B6", "C3", "C7", "D6", "E5", "E7"))
 SWEEP_DECKS = ("IAEA3Ds", "IAEA2D", "PNM", "BIBLIS", "KOEBERG", "DVP", "FDM", "FDM-1D", "adjoint", "fixed_source",
                "CBCsearch", "MOX/part1_aro_serpent", "MOX/part1_ari_serpent", "MOX/part1_aro_helios",
-               "MOX/part1_ari_helios")
+               "MOX/part1_ari_helios") + MOX_PART1BD
 
 
 MOX_XTAB_DECKS = ("part2_serpent", "part2_helios", "part3_serpent", "part3_helios")
@@ -277,12 +295,15 @@ def outputs():
     toks = {}
     jobs = [(os.path.basename(d), [os.path.join(REF, d)]) for d in SWEEP_DECKS]
     nea = os.path.join(HERE, "decks", "neacrp")
-    jobs += [(d, [os.path.join(nea, f) for f in os.listdir(nea)]) for d in NEACRP_DECKS]
+    neacrp_decks(NEACRP_HFP)
+    nea_files = [os.path.join(nea, f) for f in os.listdir(nea)]
+    jobs += [(d, nea_files) for d in NEACRP_DECKS + NEACRP_HFP]
     mox_xtab()
     mox = os.path.join(HERE, "decks", "mox")
     jobs += [(d, [os.path.join(mox, f) for f in os.listdir(mox)]) for d in MOX_XTAB_DECKS]
     for deck, files in jobs:
-        if deck not in NEACRP_DECKS + MOX_XTAB_DECKS and not os.path.exists(os.path.join(HERE, "decks", deck)):
+        if deck not in NEACRP_DECKS + NEACRP_HFP + MOX_XTAB_DECKS and not os.path.exists(os.path.join(HERE, "decks",
+                                                                                                     deck)):
             shutil.copy(files[0], os.path.join(HERE, "decks", deck))
         with tempfile.TemporaryDirectory() as d:
             for f in files:

@@ -130,3 +130,25 @@ def test_fortran_binding_types_match_the_header(tmp_path):
     cout = subprocess.run([str(tmp_path / "csizes")], check=True, capture_output=True, text=True).stdout.split()
     cs = dict(zip(cout[0::2], (int(v) for v in cout[1::2])))
     assert fs == cs
+
+
+def test_persistent_plans_are_checked_on_the_host():
+    """adg_persistent_plan (no GPU): every fixture geometry, the measured
+    cores and seeded random boxes get a brick plan whose every kernel index
+    is in bounds, every node owned once and every stencil neighbour on the
+    tile."""
+    from adpres_amd.engine import persistent_plan
+    shapes = [(68, 68, 96), (34, 34, 26), (18, 18, 2), (17, 17, 19), (136, 136, 2), (9, 9, 2), (2, 2, 2),
+              (24, 24, 2), (200, 3, 2), (3, 200, 2), (2, 2, 300)]
+    rng = np.random.default_rng(17)
+    shapes += [tuple(int(v) for v in rng.integers(2, 90, 3)) for _ in range(12)]
+    for sh in shapes:
+        for prec in ("fp64", "fp32_refine"):
+            for ncu in (256, 64):
+                pl = persistent_plan(*sh, precision=prec, ncu=ncu)
+                if pl["ok"]:
+                    assert pl["checked"], (sh, prec, ncu, pl)
+                    assert pl["blocks"] % 8 == 0 and pl["blocks"] <= ncu
+    # the 8-group core gets one brick per CU, 8 nodes per thread
+    pl = persistent_plan(68, 68, 96)
+    assert pl["ok"] and pl["blocks"] == 256 and pl["nodes_per_thread"] == 8

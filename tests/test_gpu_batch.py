@@ -70,6 +70,11 @@ def test_gpu_batch_matches_single_core_solves(precision):
     b.close()
 
 
+# cores of the batching workload at the edge of the reference's nodal
+# instability stop, whose outcome the FP32+refinement form may flip
+EDGE_FP32 = {93, 121, 234}
+
+
 @pytest.mark.parametrize("precision", ["fp64", "fp32_refine"])
 def test_gpu_batch256_matches_reference(precision):
     if not os.path.exists(os.path.join(GOLDEN, f"batch_s{SEED}_solve.npz")):
@@ -81,6 +86,7 @@ def test_gpu_batch256_matches_reference(precision):
     status = np.zeros(n, np.int32)
     powers = {}
     sample = [int(i) for i in d["pow_cores"]]
+    peak = {}
     for G in sorted({q.ng for q in cores}):
         idx = [i for i in range(n) if cores[i].ng == G]
         b = Batch([cores[i] for i in idx], precision)
@@ -90,13 +96,16 @@ def test_gpu_batch256_matches_reference(precision):
         for j, i in enumerate(idx):
             if i in sample:
                 powers[i] = b.powdis(j)
+            if i in EDGE_FP32:
+                peak[i] = float(np.max(b.trace(j, int(ni[j]))["ndmax"]))
         b.close()
     # the reference stops some cores with "two-node nonlinear iteration seems
     # not stable" (no k).  In FP64 the engine stops exactly those, with that
     # status, and solves every other core.  In FP32+refinement the FP32
-    # rounding of the iterates can move a core at the edge of that
-    # instability either way: at most 2% of the cores may differ from the
-    # reference's outcome, and every core both solve meets the k bar.
+    # rounding of the iterates can move only the cores of EDGE_FP32, which
+    # sit at the edge of that instability, to the other side: every other
+    # core has the reference's outcome, and every core both solve meets the
+    # k bar.
     ok = np.isfinite(d["Ke"])
     solved = ok & (status == ADG_OK)
     differ = np.nonzero(ok != (status == ADG_OK))[0]
@@ -104,7 +113,15 @@ def test_gpu_batch256_matches_reference(precision):
     if precision == "fp64":
         assert len(differ) == 0, (differ, status[differ])
     else:
-        assert len(differ) <= n // 50, (differ, status[differ])
+        assert set(differ.tolist()) <= EDGE_FP32, (differ, status[differ])
+        for i in differ:
+            # each side of the 1e3 stop threshold on the largest D-hat change
+            print(f"core {i}: reference {'solves' if ok[i] else 'stops'} it, FP32+refinement status {status[i]}, "
+                  f"peak ndmax {peak[i]:.4g}", flush=True)
+            if ok[i]:
+                assert status[i] == ADG_E_NODAL_UNSTABLE and peak[i] > 1.e3
+            else:
+                assert status[i] == ADG_OK and peak[i] <= 1.e3
     err = np.abs(keff[solved] - d["Ke"][solved])
     assert err.max() < 1e-5, (int(np.nonzero(solved)[0][err.argmax()]), float(err.max()))
     for i in sample:

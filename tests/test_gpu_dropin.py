@@ -143,21 +143,27 @@ def test_dropin_neacrp_boron_search_with_th(deck, tmp_path):
     assert np.allclose([q[3] for q in rows], d[deck + "_therr"], rtol=1e-2, atol=1e-7)
 
 
+MOX_PART1BD = [f"{c}{a}" for c in ("aro", "ari") for a in ("A1", "A3", "A5", "A7", "B6", "C3", "C7", "D6", "E5", "E7")]
+NEACRP_SEARCHES = ("A1", "B1", "C1", "A2", "B2", "C2")
 SWEEP = ["IAEA3Ds", "IAEA2D", "PNM", "BIBLIS", "KOEBERG", "DVP", "FDM", "FDM-1D", "adjoint", "fixed_source",
          "CBCsearch", "part1_aro_serpent", "part1_ari_serpent", "part1_aro_helios", "part1_ari_helios",
-         "A1", "B1", "C1", "part2_serpent", "part2_helios", "part3_serpent", "part3_helios"]
+         *NEACRP_SEARCHES, "part2_serpent", "part2_helios", "part3_serpent", "part3_helios", *MOX_PART1BD]
 
 
 @pytest.mark.parametrize("deck", SWEEP)
 def test_dropin_output_files_match_reference(deck, tmp_path):
-    """Regression sweep over the reference's self-contained static sample
-    decks, the NEACRP searches and the MOX part-2/3 searches with branch
-    tables (%XTAB: the host's XStab_updt, rodded tables in part 3): the result blocks that ADPRES's own
-    output routines write into <deck>.out from the GPU solution (radial and
-    axial power, radial flux, T-H summary) equal the reference's to one unit
-    of the last printed digit."""
+    """Regression sweep over all 45 static sample decks of the reference:
+    the self-contained decks (IAEA, BIBLIS, KOEBERG, DVP, FDM, adjoint,
+    fixed-source, CBCsearch, the MOX part-1 decks and the 20 MOX part-1b/1d
+    rod configurations), the NEACRP critical-boron searches with T-H at hot
+    zero and hot full power (A1-C1, A2-C2) and the MOX part-2/3 searches
+    with branch tables (%XTAB: the host's XStab_updt, rodded tables in part
+    3): the result blocks that ADPRES's own output routines write into
+    <deck>.out from the GPU solution (radial and axial power, radial flux,
+    T-H summary) equal the reference's to one unit of the last printed
+    digit."""
     from conftest import output_tokens, printed_unit
-    src = (os.path.join(GOLDEN, "decks", "neacrp") if deck in ("A1", "B1", "C1") else
+    src = (os.path.join(GOLDEN, "decks", "neacrp") if deck in NEACRP_SEARCHES else
            os.path.join(GOLDEN, "decks", "mox") if deck.startswith(("part2_", "part3_")) else None)
     files = [os.path.join(src, f) for f in os.listdir(src)] if src else [os.path.join(GOLDEN, "decks", deck)]
     for f in files:

@@ -265,3 +265,45 @@ def test_gpu_cooperative_nodal_pass_is_bit_identical(monkeypatch):
     assert np.array_equal(res["1"][0], res["0"][0])
     assert res["1"][1] == res["0"][1] and res["1"][2] == res["0"][2]
     assert res["1"][1] > 0.0
+
+
+def test_gpu_status_lu_pivot_and_power_nonpos():
+    """The engine's ADG_E_LU_PIVOT and ADG_E_POWER_NONPOS on the crafted
+    inputs the oracle stops on (tests/test_oracle.py): the nodal update at
+    k = 1 reports the pivot failure at the oracle's node (message and
+    location), a zero flux has no positive total power, and a fresh context
+    then solves normally."""
+    from conftest import lu_pivot_problem
+    from adpres_amd.engine import ADG_E_LU_PIVOT, ADG_E_POWER_NONPOS
+    from oracle.oracle import Oracle
+    q, n0, loc = lu_pivot_problem()
+    print(f"status paths: pivot node {loc}", flush=True)
+    f0 = np.ones((q.ng, q.nnod))
+    dn = np.zeros((6, q.ng, q.nnod))
+    rc_o, _, _, where = Oracle(q).nodal_update(1.0, f0, dn, kern="PNM")
+    e = Engine(q)
+    e.set_state(f0, np.zeros(q.nnod), 1.0, dn)
+    with pytest.raises(ADGError) as ei:
+        e.nodal_update(kern="PNM")
+    assert ei.value.status == ADG_E_LU_PIVOT == rc_o
+    assert "I = %d, J = %d, K = %d" % tuple(where) in str(ei.value)
+    # the same state through the returned location
+    import ctypes as C
+    from adpres_amd.engine import _i
+    nd = C.c_double()
+    l3 = np.zeros(3, np.int32)
+    assert e.lib.adg_nodal_update(e.h, 1, 1, C.byref(nd), _i(l3)) == ADG_E_LU_PIVOT
+    assert tuple(int(v) for v in l3) == tuple(where) == loc
+    # zero flux: no positive total power
+    e.set_state(np.zeros((q.ng, q.nnod)), np.zeros(q.nnod), 1.0, dn)
+    with pytest.raises(ADGError) as ei:
+        e.powdis()
+    assert ei.value.status == ADG_E_POWER_NONPOS
+    assert Oracle(q).powdis(np.zeros((q.ng, q.nnod)))[0] == ADG_E_POWER_NONPOS
+    # the context recovers: the unmodified core solves
+    e.close()
+    p = generate("core2d_small", 1).problem
+    e = Engine(p)
+    e.outer()
+    assert e.powdis().sum() > 0
+    e.close()

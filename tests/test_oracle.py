@@ -613,3 +613,23 @@ def test_deck_layout_reads_the_geometry_card():
         p = problem_from_dict(golden(fixture + "_solve"))
         _, _, _, xd, yd, zd = deck_layout(os.path.join(GOLDEN, "decks", deck))
         assert (xd.sum(), yd.sum(), zd.sum()) == (p.nxx, p.nyy, p.nzz), deck
+
+
+def test_oracle_status_paths_on_crafted_inputs():
+    """The two statuses no fixture reaches, on crafted inputs: a vanishing
+    diagonal of one node's PNM system stops the nodal update at that node
+    (LU_solve, src/mod_nodal.f90:856-862), while the unmodified core
+    updates; and a zero flux has no positive total power (PowDis,
+    src/mod_cmfd.f90:1322-1326)."""
+    from conftest import lu_pivot_problem
+    from oracle.oracle import ORC_E_LU_PIVOT, ORC_E_POWER_NONPOS, Oracle
+    q, n0, loc = lu_pivot_problem()
+    f0 = np.ones((q.ng, q.nnod))
+    dn = np.zeros((6, q.ng, q.nnod))
+    rc, _, _, where = Oracle(q).nodal_update(1.0, f0, dn, kern="PNM")
+    assert rc == ORC_E_LU_PIVOT and tuple(where) == loc
+    p = generate("core2d_small", 1).problem
+    rc, _, _, _ = Oracle(p).nodal_update(1.0, f0, dn, kern="PNM")
+    assert rc == 0
+    rc, _ = Oracle(q).powdis(np.zeros((q.ng, q.nnod)))
+    assert rc == ORC_E_POWER_NONPOS
