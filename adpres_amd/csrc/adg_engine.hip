@@ -34,8 +34,37 @@
 
 using namespace adg;
 
+// Scratch of the allocation-free output-map entry point
+// (adg_power_maps_noalloc): the map kernels' integer tables (footprint rows,
+// assembly bounds, the argmax) in the first SCR_INTS ints, their
+// intermediate sums (axially averaged node powers, assembly and plane powers)
+// in the next SCR_DBLS doubles.  One copy per GPU, held until the process
+// ends.  In host code `scratch` only names the symbol; its device address
+// on each GPU comes from hipGetSymbolAddress (scratch_init) and only that
+// address is passed to kernels or copies.
+constexpr long SCR_INTS = 8192;     // 32 KiB
+constexpr long SCR_DBLS = 131072;   // 1 MiB
+static __device__ __align__(256) unsigned char scratch[SCR_INTS * sizeof(int) + SCR_DBLS * sizeof(double)];
+
 namespace {
 
+constexpr int SCR_MAXDEV = 64;
+std::mutex scr_lock;
+void *scr_ptr[SCR_MAXDEV] = {};          // device address of `scratch` per device
+hipError_t scr_err[SCR_MAXDEV] = {};     // result of the lookup (hipSuccess once resolved)
+bool scr_done[SCR_MAXDEV] = {};
+
+// resolve the device address of `scratch` on device dev (current), once
+void scratch_init(int dev) {
+  if (dev < 0 || dev >= SCR_MAXDEV) return;
+  std::lock_guard<std::mutex> lk(scr_lock);
+  if (scr_done[dev]) return;
+  void *p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  scr_err[dev] = e;
+  scr_ptr[dev] = e == hipSuccess ? p : nullptr;
+  scr_done[dev] = true;
+}
 
 struct HipError {
   hipError_t e;
@@ -1097,6 +1126,7 @@ int create_batch(const adg_geom *geoms, int ncore, adg_ctx **out) {
   try {
     c->B = ncore;
     HIPCHK(hipGetDevice(&c->dev));
+    scratch_init(c->dev);
     c->G = geoms[0].ng;
     c->precision = geoms[0].precision;
     if (const char *ev = getenv("ADG_GRAPH")) c->graphs = atoi(ev) != 0;
@@ -1865,6 +1895,38 @@ int adg_power_maps_core(adg_ctx *c, int core, int nx, int ny, int nz, const int 
   } catch (const HipError &e) {
     if (di) (void)hipFree(di);
     if (dd) (void)hipFree(dd);
+    return hip_fail(c, e);
+  }
+}
+
+int adg_power_maps_noalloc(adg_ctx *c, int nx, int ny, int nz, const int *xdiv, const int *ydiv, const int *zdiv,
+                           double *fasm, int *pmax, double *faxi, int *amax) {
+  if (!core_ok(c, 0) || !xdiv || !ydiv || !zdiv || !fasm || !faxi) return ADG_E_ARG;
+  if (!c->core[0].xs_set) return fail(c, ADG_E_NOT_SET, "cross sections not set");
+  void *base = nullptr;
+  hipError_t lookup = hipErrorInvalidDevice;
+  if (c->dev >= 0 && c->dev < SCR_MAXDEV) {
+    std::lock_guard<std::mutex> lk(scr_lock);
+    base = scr_ptr[c->dev];
+    lookup = scr_err[c->dev];
+  }
+  if (!base)
+    return fail(c, ADG_E_HIP, std::string("device address of the map scratch unavailable: ") + hipGetErrorString(lookup));
+  const CoreHost &h = c->core[0];
+  const int cnt[3] = {nx, ny, nz};
+  const int *divs[3] = {xdiv, ydiv, zdiv};
+  int ib[SCR_INTS];   // host copy of the integer table, on the stack
+  const long nib = map_table(c, h, 3, cnt, divs, ib, SCR_INTS - 3);
+  if (nib < 0) return ADG_E_ARG;
+  const long ndbl = (long)h.nxx * h.nyy + (long)nx * ny + nz;
+  if (ndbl > SCR_DBLS)
+    return fail(c, ADG_E_ARG, "core too large for the allocation-free maps: nxx*nyy + nx*ny + nz = " +
+                                  std::to_string(ndbl) + " exceeds " + std::to_string(SCR_DBLS));
+  int *di = (int *)base;
+  double *dd = (double *)((char *)base + SCR_INTS * sizeof(int));
+  try {
+    return power_maps_run(c, 0, nx, ny, nz, ib, nib, di, dd, fasm, pmax, faxi, amax);
+  } catch (const HipError &e) {
     return hip_fail(c, e);
   }
 }

@@ -43,7 +43,7 @@ EXPORTS = ("adg_create", "adg_destroy", "adg_set_xs", "adg_set_state", "adg_get_
            "adg_set_xs_tables", "adg_set_xs_tables_core", "adg_update_xs", "adg_update_xs_core",
            "adg_get_xs", "adg_get_xs_core", "adg_set_xtab_tables", "adg_set_xtab_tables_core",
            "adg_power_maps", "adg_power_maps_core", "adg_flux_map", "adg_flux_map_core",
-           "adg_solver_info", "adg_persistent_plan", "adg_status_string",
+           "adg_power_maps_noalloc", "adg_solver_info", "adg_persistent_plan", "adg_status_string",
            "adg_last_error")
 
 _dp = C.POINTER(C.c_double)
@@ -131,6 +131,7 @@ def load_library(path: str = LIB_PATH):
     L.adg_power_maps.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, _ip, _ip, _ip, _dp, _ip, _dp, _ip]
     L.adg_power_maps_core.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, _ip, _ip, _ip, _dp, _ip,
                                       _dp, _ip]
+    L.adg_power_maps_noalloc.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, _ip, _ip, _ip, _dp, _ip, _dp, _ip]
     L.adg_flux_map.argtypes = [C.c_void_p, C.c_int, C.c_int, _ip, _ip, C.c_double, _dp, _ip]
     L.adg_flux_map_core.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, _ip, _ip, C.c_double, _dp, _ip]
     L.adg_set_xtab_tables.argtypes = [C.c_void_p, C.POINTER(AdgXtabTables), _ip, _dp]
@@ -375,15 +376,17 @@ class Engine:
         self._check(self.lib.adg_powdis(self.h, _d(out)), "adg_powdis")
         return out
 
-    def power_maps(self, nx: int, ny: int, nz: int, xdiv, ydiv, zdiv):
+    def power_maps(self, nx: int, ny: int, nz: int, xdiv, ydiv, zdiv, noalloc: bool = False):
         """The radial and axial power maps the reference prints (io::AsmPow,
         io::AxiPow) from the device's PowDis: (fasm (ny, nx), (i, j) of its
-        maximum, faxi (nz,), plane of its maximum), positions 1-based."""
+        maximum, faxi (nz,), plane of its maximum), positions 1-based.
+        noalloc: through adg_power_maps_noalloc (no device allocation)."""
         xd, yd, zd = (np.ascontiguousarray(a, np.int32) for a in (xdiv, ydiv, zdiv))
         fasm = np.zeros((ny, nx)); faxi = np.zeros(nz)
         pm = np.zeros(2, np.int32); am = np.zeros(1, np.int32)
-        self._check(self.lib.adg_power_maps(self.h, nx, ny, nz, _i(xd), _i(yd), _i(zd), _d(fasm), _i(pm), _d(faxi),
-                                            _i(am)), "adg_power_maps")
+        fn = self.lib.adg_power_maps_noalloc if noalloc else self.lib.adg_power_maps
+        self._check(fn(self.h, nx, ny, nz, _i(xd), _i(yd), _i(zd), _d(fasm), _i(pm), _d(faxi), _i(am)),
+                    "adg_power_maps_noalloc" if noalloc else "adg_power_maps")
         return fasm, (int(pm[0]), int(pm[1])), faxi, int(am[0])
 
     def flux_map(self, nx: int, ny: int, xdiv, ydiv, norm: float = 1.0):

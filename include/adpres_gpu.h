@@ -192,6 +192,17 @@ int adg_powdis(adg_ctx *ctx, double *p);
  * the reference: bit for bit its numbers for the same node powers. */
 int adg_power_maps(adg_ctx *ctx, int nx, int ny, int nz, const int *xdiv, const int *ydiv, const int *zdiv,
                    double *fasm, int *pmax, double *faxi, int *amax);
+/* adg_power_maps without any device allocation, for callers that cannot
+ * allocate (code under stream capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  No counterpart in the reference.  Same arguments,
+ * results and statuses as adg_power_maps (bit for bit); the map kernels'
+ * tables and intermediate sums go to a fixed per-GPU scratch array held
+ * until the process ends instead of buffers allocated per call.  Limits:
+ * nxx*nyy + nx*ny + nz <= 131072 and 2*nyy + nx + ny + nz + 6 <= 8192;
+ * larger layouts return ADG_E_ARG.  Calls on one GPU share that scratch, so
+ * calls that overlap in time on one GPU (from several host threads) race. */
+int adg_power_maps_noalloc(adg_ctx *ctx, int nx, int ny, int nz, const int *xdiv, const int *ydiv,
+                           const int *zdiv, double *fasm, int *pmax, double *faxi, int *amax);
 /* The numbers of the radial flux map the reference prints, io::AsmFlux
  * (src/mod_io.f90:3502-3644), made on the device from the context's flux:
  * fasm (nx,ny,G) the axially averaged assembly fluxes, normalised per group
