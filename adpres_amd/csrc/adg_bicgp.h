@@ -155,6 +155,22 @@ __host__ __device__ inline PCell halo_cell(const Brick &P, const BrickView &v, i
   return o;
 }
 
+// the block of the neighbour brick across face d (0..5: -x, +x, -y, +y, -z,
+// +z) of block b's brick, or -1
+__host__ __device__ inline int neighbour_block(const Brick &P, const BrickView &v, int d) {
+  if (!v.live || d > 5) return -1;
+  int bx = v.i0 / P.BX, by = v.j0 / P.BY, bz = v.k0 / P.BZ;
+  if (d == 0) --bx;
+  if (d == 1) ++bx;
+  if (d == 2) --by;
+  if (d == 3) ++by;
+  if (d == 4) --bz;
+  if (d == 5) ++bz;
+  if (bx < 0 || bx >= P.nbx || by < 0 || by >= P.nby || bz < 0 || bz >= P.nbz) return -1;
+  const int L = bx + P.nbx * (by + P.nby * bz), per = P.nb / 8;
+  return (L % per) * 8 + L / per;
+}
+
 // ---------------------------------------------------------------------------
 // host: the plan and its check
 // ---------------------------------------------------------------------------
@@ -242,6 +258,20 @@ inline std::string verify_plan(const Brick &P, int kc, long box) {
       if (mark[o.e] == b) return "tile slot written twice";
       mark[o.e] = b;
     }
+    for (int d = 0; d < 6 && v.live; ++d) {
+      // the neighbour brick across face d: its block holds the brick next to this one
+      const int nbb = neighbour_block(P, v, d);
+      const int di0 = d == 0 ? -P.BX : d == 1 ? P.BX : 0, dj0 = d == 2 ? -P.BY : d == 3 ? P.BY : 0,
+                dk0 = d == 4 ? -P.BZ : d == 5 ? P.BZ : 0;
+      const int ni0 = v.i0 + di0, nj0 = v.j0 + dj0, nk0 = v.k0 + dk0;
+      const bool exists = ni0 >= 0 && ni0 < P.nxx && nj0 >= 0 && nj0 < P.nyy && nk0 >= 0 && nk0 < P.nzz;
+      if (exists != (nbb >= 0)) return "neighbour block missing or spurious";
+      if (nbb >= 0) {
+        if (nbb >= P.nb) return "neighbour block out of range";
+        const BrickView w = brick_of(P, nbb);
+        if (!w.live || w.i0 != ni0 || w.j0 != nj0 || w.k0 != nk0) return "neighbour block holds another brick";
+      }
+    }
     const int H = halo_count(v);
     if (H > P.H || H > PKH * PNT) return "halo larger than planned";
     for (int h = 0; h < H; ++h) {
@@ -300,33 +330,78 @@ __device__ __forceinline__ void co_drain() {
 __device__ __forceinline__ bool epoch_reached(unsigned v, unsigned e) { return (int)(v - e) >= 0; }
 __device__ __forceinline__ unsigned *line_flag(double *line) { return (unsigned *)(line + 8); }
 
-// Device-wide synchronisation with a reduction of four values: `mine` holds
-// this block's partials (read in thread 0); every thread gets the sum over
-// all blocks in `red`.  Stores issued by any block before the call are seen
-// by coherent loads of every block after it.  Returns false when a wait
-// timed out or another block reported a timeout (the error flag is set).
-__device__ bool p_sync(double *sync, unsigned e, const double (&mine)[4], double (&red)[4], long spin) {
-  __shared__ double sred[4];
-  __shared__ int sbad;
-  const int t = threadIdx.x, b = blockIdx.x, nb = gridDim.x;
-  int *err = (int *)(sync + PSYNC_CTL) + 1;
+// Device-wide synchronisation with a reduction of four values, in steps so
+// that a block can load its halo while the reduction completes:
+//   p_publish   every block: its partials, then its epoch flag, in its line
+//               (after its earlier stores and loads have completed);
+//   p_lead      the first block of each XCD (block x leads the blocks
+//               b = x mod 8): waits for its XCD's flags, sums their
+//               partials in a fixed order, publishes the XCD sum and flag;
+//   p_wait_nb   every block: waits for the flags of its (at most six)
+//               neighbour bricks, whose surface values it may then load;
+//   p_finish    every block: waits for the eight XCD flags and sums the
+//               XCD sums in a fixed order (bit-identical in every block).
+// Stores issued by a block before p_publish are seen by coherent loads of
+// a neighbour after its p_wait_nb, and of every block after p_finish.  A
+// wait that gives up sets the error flag; p_finish then reports failure.
+
+__device__ __forceinline__ void p_publish(double *sync, unsigned e, const double (&mine)[4]) {
   co_drain();
   __syncthreads();
-  if (t == 0) {
-    sbad = 0;
-    double *ln = sync + (long)PLINE * b;
+  if (threadIdx.x == 0) {
+    double *ln = sync + (long)PLINE * blockIdx.x;
 #pragma unroll
     for (int k = 0; k < 4; ++k) co_st(ln + k, mine[k]);
     co_drain();
     co_st(line_flag(ln), e);
   }
-  if (b < 8 && t < 64) {
-    // XCD leader: the blocks l * 8 + b, l < nb / 8
-    const int per = nb / 8;
+}
+
+__device__ __forceinline__ void p_lead(double *sync, unsigned e, long spin) {
+  const int t = threadIdx.x, b = blockIdx.x;
+  if (b >= 8 || t >= 64) return;
+  int *err = (int *)(sync + PSYNC_CTL) + 1;
+  const int per = gridDim.x / 8;
+  bool ok = true;
+  long n = 0;
+  for (;;) {
+    const bool got = t >= per || epoch_reached(co_ld(line_flag(sync + (long)PLINE * (t * 8 + b))), e);
+    if (__all(got)) break;
+    __builtin_amdgcn_s_sleep(1);
+    if (++n > spin) {
+      ok = false;
+      break;
+    }
+  }
+  asm volatile("" ::: "memory");
+  double v[4] = {0.0, 0.0, 0.0, 0.0};
+  if (t < per) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) v[k] = co_ld(sync + (long)PLINE * (t * 8 + b) + k);
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_xor(v[k], off, 64);
+  if (t == 0) {
+    if (!ok) co_st(err, 1);
+    double *xl = sync + PSYNC_XCD + (long)PLINE * b;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) co_st(xl + k, v[k]);
+    co_drain();
+    co_st(line_flag(xl), e);
+  }
+}
+
+// nbr: this thread's neighbour block (lanes 0..5 of wave 0; -1 none)
+__device__ __forceinline__ void p_wait_nb(double *sync, unsigned e, int nbr, long spin) {
+  __shared__ int snb;
+  const int t = threadIdx.x;
+  if (t < 64) {
     bool ok = true;
     long n = 0;
     for (;;) {
-      const bool got = t >= per || epoch_reached(co_ld(line_flag(sync + (long)PLINE * (t * 8 + b))), e);
+      const bool got = nbr < 0 || epoch_reached(co_ld(line_flag(sync + (long)PLINE * nbr)), e);
       if (__all(got)) break;
       __builtin_amdgcn_s_sleep(1);
       if (++n > spin) {
@@ -334,25 +409,20 @@ __device__ bool p_sync(double *sync, unsigned e, const double (&mine)[4], double
         break;
       }
     }
-    asm volatile("" ::: "memory");
-    double v[4] = {0.0, 0.0, 0.0, 0.0};
-    if (t < per) {
-#pragma unroll
-      for (int k = 0; k < 4; ++k) v[k] = co_ld(sync + (long)PLINE * (t * 8 + b) + k);
-    }
-#pragma unroll
-    for (int k = 0; k < 4; ++k)
-#pragma unroll
-      for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_xor(v[k], off, 64);
     if (t == 0) {
-      if (!ok) co_st(err, 1);
-      double *xl = sync + PSYNC_XCD + (long)PLINE * b;
-#pragma unroll
-      for (int k = 0; k < 4; ++k) co_st(xl + k, v[k]);
-      co_drain();
-      co_st(line_flag(xl), e);
+      snb = ok ? 0 : 1;
+      if (!ok) co_st((int *)(sync + PSYNC_CTL) + 1, 1);
     }
   }
+  asm volatile("" ::: "memory");
+  __syncthreads();
+}
+
+__device__ __forceinline__ bool p_finish(double *sync, unsigned e, double (&red)[4], long spin) {
+  __shared__ double sred[4];
+  __shared__ int sbad;
+  const int t = threadIdx.x;
+  int *err = (int *)(sync + PSYNC_CTL) + 1;
   if (t < 64) {
     bool ok = true;
     long n = 0;
@@ -388,6 +458,7 @@ __device__ bool p_sync(double *sync, unsigned e, const double (&mine)[4], double
   for (int k = 0; k < 4; ++k) red[k] = sred[k];
   return sbad == 0;
 }
+
 
 // block-wide sums of four values in a fixed order; the result in thread 0
 template <int NT>
@@ -456,7 +527,9 @@ __global__ void __launch_bounds__(PNT) __attribute__((amdgpu_waves_per_eu(1, 1))
   const int sy = (int)P.sy, sz = (int)P.sz;
   const int EX = P.EX, EXY = P.EX * P.EY;
   const int tile = P.EX * P.EY * P.EZ;
+  const long spin = a.spin;
   for (int q = t; q < tile; q += PNT) W[q] = (TV)0;
+  const int nbr = t < 6 ? neighbour_block(P, bv, t) : -1;
   // this thread's halo nodes, decoded once: cell (-1: none or outside the
   // node box, where the tile keeps zero) and tile slot
   const int H = halo_count(bv);
@@ -477,8 +550,6 @@ __global__ void __launch_bounds__(PNT) __attribute__((amdgpu_waves_per_eu(1, 1))
   bool surf[KC];
   TV A[KC][7], x[KC], r[KC], p[KC], v[KC], rs[KC], s[KC], tt[KC];
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  // setup (bicg lines 1-4): r = b - A x in FP64, rs = r; r goes to S for the
-  // neighbours' first p
 #pragma unroll
   for (int k = 0; k < KC; ++k) {
     const PCell o = own_cell(P, bv, t + k * PNT);
@@ -489,6 +560,8 @@ __global__ void __launch_bounds__(PNT) __attribute__((amdgpu_waves_per_eu(1, 1))
 #pragma unroll
     for (int q = 0; q < 7; ++q) A[k][q] = (TV)0;
   }
+  // setup (bicg lines 1-4): r = b - A x in FP64, rs = r; r goes to S for the
+  // neighbours' first p
 #pragma unroll
   for (int k = 0; k < KC; ++k) {
     if (oc[k] < 0) continue;
@@ -511,18 +584,27 @@ __global__ void __launch_bounds__(PNT) __attribute__((amdgpu_waves_per_eu(1, 1))
     acc[0] += (double)r[k] * (double)r[k];
     if (surf[k]) co_st(a.S + c, r[k]);
   }
+  // halo values loaded while a reduction completes: phase A's s, t, p, v
+  // (the first iteration: r in S), phase B's v
+  TV hs[PKH], ht[PKH], hp[PKH], hv[PKH], rh[PKH];
   p_block_sum<PNT>(acc);
+  p_publish(a.sync, ++e, acc);
+  p_lead(a.sync, e, spin);
+  p_wait_nb(a.sync, e, nbr, spin);
+#pragma unroll
+  for (int k = 0; k < PKH; ++k) {
+    hs[k] = ht[k] = hp[k] = hv[k] = (TV)0;
+    if (hc[k] >= 0) hs[k] = co_ld(a.S + hc[k]);
+  }
   double red[4];
-  if (!p_sync(a.sync, ++e, acc, red, a.spin)) {
+  if (!p_finish(a.sync, e, red, spin)) {
     p_fail(a.sc, a.status_timeout);
     return;
   }
   double rho = red[0], alpha = 0.0, omega = 0.0;
-  TV rh[PKH];
   for (int it = 1; it <= a.nin; ++it) {
     const int cur = it & 1;
     TV *Pw = cur ? a.P1 : a.P0, *Vw = cur ? a.V1 : a.V0;
-    const TV *Pr = cur ? a.P0 : a.P1, *Vr = cur ? a.V0 : a.V1;
     double beta = 0.0;
     if (it > 1) {
       omega = red[1] / red[0];
@@ -537,15 +619,13 @@ __global__ void __launch_bounds__(PNT) __attribute__((amdgpu_waves_per_eu(1, 1))
       rh[k] = (TV)0;
       if (he[k] < 0) continue;
       TV pv = (TV)0;
-      const int c = hc[k];
-      if (c >= 0) {
+      if (hc[k] >= 0) {
         if (it == 1) {
-          rh[k] = co_ld(a.S + c);
+          rh[k] = hs[k];
           pv = rh[k];
         } else {
-          const TV sv = co_ld(a.S + c), tv = co_ld(a.T + c), pp = co_ld(Pr + c), vv = co_ld(Vr + c);
-          rh[k] = (TV)(sv - tw * tv);
-          pv = (TV)(rh[k] + tb * (pp - tw * vv));
+          rh[k] = (TV)(hs[k] - tw * ht[k]);
+          pv = (TV)(rh[k] + tb * (hp[k] - tw * hv[k]));
         }
       }
       W[he[k]] = pv;
@@ -561,6 +641,7 @@ __global__ void __launch_bounds__(PNT) __attribute__((amdgpu_waves_per_eu(1, 1))
         p[k] = r[k];
       }
       W[oe[k]] = p[k];
+      if (surf[k]) co_st(Pw + oc[k], p[k]);
     }
     __syncthreads();
     double acca[4] = {0.0, 0.0, 0.0, 0.0};
@@ -574,13 +655,16 @@ __global__ void __launch_bounds__(PNT) __attribute__((amdgpu_waves_per_eu(1, 1))
       for (int m = 0; m < 7; ++m) sv = sv + (double)(A[k][m] * pn[m]);
       v[k] = (TV)sv;
       acca[0] += (double)rs[k] * (double)v[k];
-      if (surf[k]) {
-        co_st(Pw + oc[k], p[k]);
-        co_st(Vw + oc[k], v[k]);
-      }
+      if (surf[k]) co_st(Vw + oc[k], v[k]);
     }
     p_block_sum<PNT>(acca);
-    if (!p_sync(a.sync, ++e, acca, red, a.spin)) {
+    p_publish(a.sync, ++e, acca);
+    p_lead(a.sync, e, spin);
+    p_wait_nb(a.sync, e, nbr, spin);
+#pragma unroll
+    for (int k = 0; k < PKH; ++k)
+      if (hc[k] >= 0) hv[k] = co_ld(Vw + hc[k]);
+    if (!p_finish(a.sync, e, red, spin)) {
       p_fail(a.sc, a.status_timeout);
       return;
     }
@@ -590,15 +674,14 @@ __global__ void __launch_bounds__(PNT) __attribute__((amdgpu_waves_per_eu(1, 1))
 #pragma unroll
     for (int k = 0; k < PKH; ++k) {
       if (he[k] < 0) continue;
-      TV sv = (TV)0;
-      if (hc[k] >= 0) sv = (TV)(rh[k] - ta2 * co_ld(Vw + hc[k]));
-      W[he[k]] = sv;
+      W[he[k]] = hc[k] >= 0 ? (TV)(rh[k] - ta2 * hv[k]) : (TV)0;
     }
 #pragma unroll
     for (int k = 0; k < KC; ++k) {
       if (oc[k] < 0) continue;
       s[k] = (TV)(r[k] - ta2 * v[k]);
       W[oe[k]] = s[k];
+      if (surf[k]) co_st(a.S + oc[k], s[k]);
     }
     __syncthreads();
     double accb[4] = {0.0, 0.0, 0.0, 0.0};
@@ -616,13 +699,24 @@ __global__ void __launch_bounds__(PNT) __attribute__((amdgpu_waves_per_eu(1, 1))
       accb[1] += (double)tt[k] * (double)s[k];
       accb[2] += rsd * (double)tt[k];
       accb[3] += rsd * (double)s[k];
-      if (surf[k]) {
-        co_st(a.S + oc[k], s[k]);
-        co_st(a.T + oc[k], tt[k]);
-      }
+      if (surf[k]) co_st(a.T + oc[k], tt[k]);
     }
     p_block_sum<PNT>(accb);
-    if (!p_sync(a.sync, ++e, accb, red, a.spin)) {
+    p_publish(a.sync, ++e, accb);
+    p_lead(a.sync, e, spin);
+    if (it < a.nin) {
+      // the next phase A's halo: s, t of this phase and p, v of phase A
+      p_wait_nb(a.sync, e, nbr, spin);
+#pragma unroll
+      for (int k = 0; k < PKH; ++k)
+        if (hc[k] >= 0) {
+          hs[k] = co_ld(a.S + hc[k]);
+          ht[k] = co_ld(a.T + hc[k]);
+          hp[k] = co_ld(Pw + hc[k]);
+          hv[k] = co_ld(Vw + hc[k]);
+        }
+    }
+    if (!p_finish(a.sync, e, red, spin)) {
       p_fail(a.sc, a.status_timeout);
       return;
     }
