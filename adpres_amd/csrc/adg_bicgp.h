@@ -36,6 +36,7 @@
 // runs one such solve at a time per process.
 #pragma once
 
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -179,17 +180,24 @@ constexpr int PKCMAX = 8;   // most nodes per thread (the kernel instances 1, 2,
 
 inline size_t persist_lds(const Brick &P, bool f32) { return (size_t)P.EX * P.EY * P.EZ * (f32 ? 4 : 8); }
 
-// The brick plan of a node box nxx x nyy x nzz at cell base with strides sy,
-// sz: bricks of at most PKCMAX x PNT nodes and PKH x PNT halo nodes whose
-// tile fits 64 KB of LDS, at most one block per CU (ncu), minimising
+// The brick plans of a node box nxx x nyy x nzz at cell base with strides
+// sy, sz: bricks of at most PKCMAX x PNT nodes and PKH x PNT halo nodes whose
+// tile fits 64 KB of LDS, at most one block per CU (ncu), ranked by
 // (nodes + 2 halo nodes) per brick plus 4 per block (the synchronisation
-// grows with the grid).  kc: nodes per thread of the kernel instance.
-// Returns false when no brick shape fits.
-inline bool plan_bricks(int nxx, int nyy, int nzz, long base, long sy, long sz, long NC, bool f32, int ncu,
-                        Brick &best, int &kc) {
+// grows with the grid), cheapest first (ties: the first in z, y, x extent
+// order); at most kmax of them.  kcs: nodes per thread of the kernel
+// instance of each plan.  Returns the number of plans.
+inline int plan_brick_list(int nxx, int nyy, int nzz, long base, long sy, long sz, long NC, bool f32, int ncu,
+                           std::vector<Brick> &plans, std::vector<int> &kcs, int kmax) {
+  plans.clear();
+  kcs.clear();
   const int maxb = (ncu / 8 * 8 < PMAXB ? ncu / 8 * 8 : PMAXB);
-  if (maxb < 8 || NC >= 2147483647L) return false;
-  long bestcost = -1;
+  if (maxb < 8 || NC >= 2147483647L || kmax < 1) return 0;
+  struct Cand {
+    long cost;
+    int BX, BY, BZ;
+  };
+  std::vector<Cand> cand;
   for (int BZ = 1; BZ <= nzz; ++BZ) {
     const int nbz = (nzz + BZ - 1) / BZ;
     if (BZ > 1 && (nzz + BZ - 2) / (BZ - 1) == nbz) continue;   // the same count with a smaller extent
@@ -206,25 +214,41 @@ inline bool plan_bricks(int nxx, int nyy, int nzz, long base, long sy, long sz, 
         const long H = 2L * ((long)BY * BZ + (long)BX * BZ + (long)BX * BY);
         if (cells > (long)PKCMAX * PNT || H > (long)PKH * PNT) continue;
         if ((long)(BX + 2) * (BY + 2) * (BZ + 2) * (f32 ? 4 : 8) > 65536) continue;
-        const long cost = cells + 2 * H + 4 * nb;
-        if (bestcost < 0 || cost < bestcost) {
-          bestcost = cost;
-          best.BX = BX; best.BY = BY; best.BZ = BZ;
-          best.nbx = nbx; best.nby = nby; best.nbz = nbz;
-          best.nbr = (int)nbr; best.nb = (int)nb;
-          best.H = (int)H;
-        }
+        cand.push_back(Cand{cells + 2 * H + 4 * nb, BX, BY, BZ});
       }
     }
   }
-  if (bestcost < 0) return false;
-  best.nxx = nxx; best.nyy = nyy; best.nzz = nzz;
-  best.EX = best.BX + 2; best.EY = best.BY + 2; best.EZ = best.BZ + 2;
-  best.base = base; best.sy = sy; best.sz = sz; best.NC = NC;
-  const long cells = (long)best.BX * best.BY * best.BZ;
-  kc = 1;
-  while ((long)kc * PNT < cells) kc *= 2;
-  return kc <= PKCMAX;
+  std::stable_sort(cand.begin(), cand.end(), [](const Cand &a, const Cand &b) { return a.cost < b.cost; });
+  for (const Cand &c : cand) {
+    if ((int)plans.size() >= kmax) break;
+    Brick P{};
+    P.BX = c.BX; P.BY = c.BY; P.BZ = c.BZ;
+    P.nbx = (nxx + c.BX - 1) / c.BX; P.nby = (nyy + c.BY - 1) / c.BY; P.nbz = (nzz + c.BZ - 1) / c.BZ;
+    P.nbr = P.nbx * P.nby * P.nbz;
+    P.nb = (P.nbr + 7) / 8 * 8;
+    P.H = 2 * (c.BY * c.BZ + c.BX * c.BZ + c.BX * c.BY);
+    P.nxx = nxx; P.nyy = nyy; P.nzz = nzz;
+    P.EX = P.BX + 2; P.EY = P.BY + 2; P.EZ = P.BZ + 2;
+    P.base = base; P.sy = sy; P.sz = sz; P.NC = NC;
+    const long cells = (long)P.BX * P.BY * P.BZ;
+    int kc = 1;
+    while ((long)kc * PNT < cells) kc *= 2;
+    if (kc > PKCMAX) continue;
+    plans.push_back(P);
+    kcs.push_back(kc);
+  }
+  return (int)plans.size();
+}
+
+// the cheapest plan of plan_brick_list; false when no brick shape fits
+inline bool plan_bricks(int nxx, int nyy, int nzz, long base, long sy, long sz, long NC, bool f32, int ncu,
+                        Brick &best, int &kc) {
+  std::vector<Brick> plans;
+  std::vector<int> kcs;
+  if (plan_brick_list(nxx, nyy, nzz, base, sy, sz, NC, f32, ncu, plans, kcs, 1) < 1) return false;
+  best = plans[0];
+  kc = kcs[0];
+  return true;
 }
 
 // Every index the kernel forms for plan P with kc nodes per thread, checked

@@ -55,7 +55,7 @@ struct Geo {
   int bc[6];           // per face s: boundary code (east, west, north, south, top, bottom)
   int pfirst, pn;      // partial slots [pfirst, pfirst + pn) of its blocks in the flat launches
   int ffirst, fnb;     // its blocks [ffirst, ffirst + fnb) of the face (nodal pass 2) launches
-  int pad0, pad1;
+  int bfirst, bnb;      // its blocks [bfirst, bfirst + bnb) of the boundary-face list launch (0: none)
   const double *hx, *hy, *hz;
   const unsigned char *flag;   // batch-wide, indexed by cell
 };

@@ -112,6 +112,7 @@ struct CoreHost {
   bool xs_set = false, first = true;
   double Ke_host = 1.0;
   int pfirst = 0, pn = 0, ffirst = 0, fnb = 0;
+  int bfirst = 0, bnb = 0;         // its blocks of the boundary-face list launch
   int status = 0, niter = 0;
   // feedback tables of adg_set_xs_tables (device)
   double *xtab = nullptr;          // [6 kinds][4 G M + G G M], then zdum(nzz), zdel(nzz)
@@ -136,6 +137,11 @@ struct adg_ctx {
   Geo *geo = nullptr;
   Blk *tflat = nullptr, *tcell = nullptr, *tface = nullptr;
   int nflat = 0, ncell = 0, nface = 0;
+  // boundary faces (one node active) of the cooperative nodal form: list of
+  // (direction << 40 | face index) per core, NT entries per block
+  Blk *tbface = nullptr;
+  long *bfl = nullptr;
+  int nbface = 0;
   int flat_cap = 0;                // allocated entries of tflat (and 12 x of part)
   long flat_param = 0;             // launch shape of the flat kernels (see build_flat)
   double *d_hx = nullptr, *d_hy = nullptr, *d_hz = nullptr;
@@ -175,6 +181,8 @@ struct adg_ctx {
   Brick pb{};
   int pkc = 0;
   bool persist = false;
+  std::vector<Brick> pcand;        // checked brick plans, cheapest by the cost model first (the autotuner times them)
+  std::vector<int> pcand_kc;
   double *psync = nullptr;
   long pspin = PSPIN;              // polls before a wait of the persistent kernel gives up (ADG_PSPIN)
   // profiling: event pairs recorded on the context stream during adg_outer
@@ -193,7 +201,8 @@ struct adg_ctx {
   double *tk = nullptr, *tser = nullptr, *tfer = nullptr, *tnd = nullptr;
   int *tflags = nullptr, *tloc = nullptr;
   int tcap = 0;
-  hipGraphExec_t gexec = nullptr;
+  hipGraphExec_t gexec = nullptr;         // one outer, nodal-update kernels included
+  hipGraphExec_t gexec_plain = nullptr;   // one outer without them (outers the schedule leaves out)
   int gkey[6] = {-1, -1, -1, -1, -1, -1};   // source mode, nin, kern, nupd, variant, transpose of the captured outer
   // differentiation (single core): saved forward state, adjoint, generalized
   // adjoint and gradients
@@ -399,6 +408,15 @@ void build_flat(adg_ctx *c, std::vector<Blk> &t, long param) {
       t[8 * j + x] = j < xl[x].size() ? xl[x][j] : Blk{0, pad++, c->core[0].pfirst, c->core[0].pn, 0, 0, 1, 1};
 }
 
+// the captured outers (re-captured at the next outer call)
+void drop_graphs(adg_ctx *c) {
+  for (hipGraphExec_t *g : {&c->gexec, &c->gexec_plain})
+    if (*g) {
+      (void)hipGraphExecDestroy(*g);
+      *g = nullptr;
+    }
+}
+
 void upload_geo(adg_ctx *c) {
   std::vector<Geo> gq(c->B);
   for (int b = 0; b < c->B; ++b) {
@@ -412,6 +430,7 @@ void upload_geo(adg_ctx *c) {
     q.c0 = h.base + h.sz; q.M = (long)h.nzz * h.sz;
     for (int s6 = 0; s6 < 6; ++s6) q.bc[s6] = h.bcface[s6];
     q.pfirst = h.pfirst; q.pn = h.pn; q.ffirst = h.ffirst; q.fnb = h.fnb;
+    q.bfirst = h.bfirst; q.bnb = h.bnb;
     q.hx = c->d_hx + h.hoff[0]; q.hy = c->d_hy + h.hoff[1]; q.hz = c->d_hz + h.hoff[2];
     q.flag = c->flag;
   }
@@ -445,16 +464,19 @@ void install_flat(adg_ctx *c, long param) {
   scopy(c->s, c->tflat, tf.data(), tf.size() * sizeof(Blk), hipMemcpyHostToDevice);
   c->flat_param = param;
   upload_geo(c);
-  if (c->gexec) {
-    (void)hipGraphExecDestroy(c->gexec);
-    c->gexec = nullptr;
-  }
+  drop_graphs(c);
 }
 
 void plan_persist(adg_ctx *c);
 
-void build_tables(adg_ctx *c) {
-  std::vector<Blk> tc, tfc;
+// the cooperative nodal kernels (one lane per row of the node and face
+// systems) serve the larger group counts
+bool nod_coop(const adg_ctx *c) { return c->G >= 5 && c->nod_coop; }
+
+// fls: each core's cell flags (padded box, F_ACTIVE and neighbour bits)
+void build_tables(adg_ctx *c, const std::vector<std::vector<unsigned char>> &fls) {
+  std::vector<Blk> tc, tfc, tbf;
+  std::vector<long> bl;
   for (int b = 0; b < c->B; ++b) {
     CoreHost &h = c->core[b];
     const long nn = (long)h.nxx * h.nyy * h.nzz;
@@ -464,13 +486,44 @@ void build_tables(adg_ctx *c) {
     h.ffirst = (int)tfc.size();
     for (long t0 = 0; t0 < maxf; t0 += NT) tfc.push_back(Blk{b, 0, 0, 0, t0, maxf, h.sy, h.sz});
     h.fnb = (int)tfc.size() - h.ffirst;
+    h.bfirst = (int)tbf.size();
+    h.bnb = 0;
+    if (!nod_coop(c)) continue;
+    // boundary faces in k_nod_pass2's enumeration: direction u, face index
+    // t = fi + (nu + 1) (o1 + e1 o2), one of its two nodes active
+    const std::vector<unsigned char> &fl = fls[b];
+    const long l0 = (long)bl.size();
+    for (int u = 0; u < 3; ++u) {
+      const int nu = u == 0 ? h.nxx : (u == 1 ? h.nyy : h.nzz);
+      const int e1 = u == 0 ? h.nyy : h.nxx, e2 = u == 2 ? h.nyy : h.nzz;
+      const long st = u == 0 ? 1 : (u == 1 ? h.sy : h.sz);
+      for (long t = 0; t < (long)(nu + 1) * e1 * e2; ++t) {
+        const int fi = (int)(t % (nu + 1));
+        const long r = t / (nu + 1);
+        const int o1 = (int)(r % e1), o2 = (int)(r / e1);
+        const int i = u == 0 ? fi : o1, j = u == 0 ? o1 : (u == 1 ? fi : o2), k = u == 2 ? fi : o2;
+        const long cp = ((long)(k + 1) * h.NYP + (j + 1)) * h.NXP + (i + 1);
+        const bool hp = (fl[cp] & F_ACTIVE) != 0, hn = (fl[cp - st] & F_ACTIVE) != 0;
+        if (hp != hn) bl.push_back(((long)u << 40) | t);
+      }
+    }
+    const long l1 = (long)bl.size();
+    for (long t0 = l0; t0 < l1; t0 += NT) tbf.push_back(Blk{b, 0, 0, 0, t0, l1, h.sy, h.sz});
+    h.bnb = (int)tbf.size() - h.bfirst;
   }
   c->ncell = (int)tc.size();
   c->nface = (int)tfc.size();
+  c->nbface = (int)tbf.size();
   c->tcell = dalloc<Blk>(c->s, tc.size());
   c->tface = dalloc<Blk>(c->s, tfc.size());
   scopy(c->s, c->tcell, tc.data(), tc.size() * sizeof(Blk), hipMemcpyHostToDevice);
   scopy(c->s, c->tface, tfc.data(), tfc.size() * sizeof(Blk), hipMemcpyHostToDevice);
+  if (!tbf.empty()) {
+    c->tbface = dalloc<Blk>(c->s, tbf.size());
+    c->bfl = dalloc<long>(c->s, bl.size());
+    scopy(c->s, c->tbface, tbf.data(), tbf.size() * sizeof(Blk), hipMemcpyHostToDevice);
+    scopy(c->s, c->bfl, bl.data(), bl.size() * sizeof(long), hipMemcpyHostToDevice);
+  }
   install_flat(c, default_flat_param(c));
   plan_persist(c);
 }
@@ -503,6 +556,8 @@ const void *persist_fn(int kc) {
 void plan_persist(adg_ctx *c) {
   c->pkc = 0;
   c->persist = false;
+  c->pcand.clear();
+  c->pcand_kc.clear();
   if (c->B != 1) return;
   if (const char *ev = getenv("ADG_PERSIST"))
     if (atoi(ev) == 0) return;
@@ -510,23 +565,31 @@ void plan_persist(adg_ctx *c) {
   const bool f32 = c->precision == ADG_FP32_REFINE;
   int ncu = 0;
   HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->dev));
-  Brick best{};
-  int kc = 0;
-  if (!plan_bricks(h.nxx, h.nyy, h.nzz, h.base, h.sy, h.sz, c->NC, f32, ncu, best, kc)) return;
-  const void *fn = f32 ? persist_fn<float>(kc) : persist_fn<double>(kc);
-  if (!fn) return;
-  const size_t lds = persist_lds(best, f32);
-  int occ = 0;
-  HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, PNT, lds));
-  if ((long)occ * ncu < best.nb) return;
-  const std::string e = verify_plan(best, kc, h.box);
-  if (!e.empty()) {
-    fprintf(stderr, "adg: persistent BiCGSTAB plan rejected (%s); the flat kernels are used\n", e.c_str());
-    return;
+  std::vector<Brick> plans;
+  std::vector<int> kcs;
+  plan_brick_list(h.nxx, h.nyy, h.nzz, h.base, h.sy, h.sz, c->NC, f32, ncu, plans, kcs, 4);
+  // keep the plans whose kernel instance exists, whose blocks are resident
+  // together (one per CU at most) and whose every index checks out on the host
+  for (size_t q = 0; q < plans.size(); ++q) {
+    const Brick &P = plans[q];
+    const int kc = kcs[q];
+    const void *fn = f32 ? persist_fn<float>(kc) : persist_fn<double>(kc);
+    if (!fn) continue;
+    int occ = 0;
+    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, PNT, persist_lds(P, f32)));
+    if ((long)occ * ncu < P.nb) continue;
+    const std::string e = verify_plan(P, kc, h.box);
+    if (!e.empty()) {
+      fprintf(stderr, "adg: persistent BiCGSTAB plan %dx%dx%d rejected (%s)\n", P.BX, P.BY, P.BZ, e.c_str());
+      continue;
+    }
+    c->pcand.push_back(P);
+    c->pcand_kc.push_back(kc);
   }
+  if (c->pcand.empty()) return;
   if (!c->psync) c->psync = dalloc<double>(c->s, PSYNC_DOUBLES);
-  c->pb = best;
-  c->pkc = kc;
+  c->pb = c->pcand[0];
+  c->pkc = c->pcand_kc[0];
   c->persist = true;
 }
 
@@ -694,21 +757,29 @@ void nodal_update(adg_ctx *c, int kern, int cmode, int sched) {
   a.kern = kern; a.cmode = cmode; a.sched = sched;
   a.part = c->npart;
   a.coop = 0;
-  NodArgs an = a, af = a;
+  NodArgs an = a, af = a, ab = a;
   an.lv = c->lv(c->tcell);
   af.lv = c->lv(c->tface);
-  // interior faces of larger group counts in the cooperative pass 2 (one
-  // lane per row of the 2G x 2G system); ADG_NOD_COOP=0 keeps them in the
-  // per-thread kernel
-  af.coop = c->G >= 5 && c->nod_coop ? 1 : 0;
-  const dim3 g2(c->nface, 3, 1);
+  ab.lv = c->lv(c->tbface);
+  // larger group counts: the cooperative passes (one lane per row of the
+  // G x G node systems and the 2G x 2G interface systems) and the boundary
+  // faces from their list; ADG_NOD_COOP=0 keeps every system per thread
+  const bool coop = nod_coop(c);
+  af.coop = coop ? 1 : 0;
+  const dim3 g1(c->ncell, 3, 1), g2(c->nface, 3, 1);
   switch (c->G) {
-#define NOD_CASE(GG)                                                                    \
-  case GG:                                                                              \
-    hipLaunchKernelGGL(k_nod_src, dim3(c->ncell, GG), dim3(NT), 0, c->s, an);           \
-    hipLaunchKernelGGL(k_nod_pass1<GG>, dim3(c->ncell, 3), dim3(NT), 0, c->s, an);      \
-    hipLaunchKernelGGL(k_nod_pass2<GG>, g2, dim3(NT), 0, c->s, af);                     \
-    if (af.coop) hipLaunchKernelGGL(k_nod_pass2c<GG>, g2, dim3(NT), 0, c->s, af);       \
+#define NOD_CASE(GG)                                                                                     \
+  case GG:                                                                                               \
+    hipLaunchKernelGGL(k_nod_src, dim3(c->ncell, GG), dim3(NT), 0, c->s, an);                            \
+    if (coop) {                                                                                          \
+      hipLaunchKernelGGL(k_nod_pass1c<GG>, g1, dim3(NT), 0, c->s, an);                                   \
+      hipLaunchKernelGGL(k_nod_pass2c<GG>, g2, dim3(NT), 0, c->s, af);                                   \
+      if (c->nbface) hipLaunchKernelGGL(k_nod_pass2b<GG>, dim3(c->nbface), dim3(NT), 0, c->s, ab, c->bfl, \
+                                        c->nface);                                                       \
+    } else {                                                                                             \
+      hipLaunchKernelGGL(k_nod_pass1<GG>, g1, dim3(NT), 0, c->s, an);                                    \
+      hipLaunchKernelGGL(k_nod_pass2<GG>, g2, dim3(NT), 0, c->s, af);                                    \
+    }                                                                                                    \
     break;
     NOD_CASE(1) NOD_CASE(2) NOD_CASE(3) NOD_CASE(4) NOD_CASE(5) NOD_CASE(6) NOD_CASE(7) NOD_CASE(8)
 #undef NOD_CASE
@@ -757,10 +828,7 @@ void ensure_trace(adg_ctx *c, int nout) {
   c->tk = dalloc<double>(c->s, n); c->tser = dalloc<double>(c->s, n); c->tfer = dalloc<double>(c->s, n);
   c->tnd = dalloc<double>(c->s, n); c->tflags = dalloc<int>(c->s, n); c->tloc = dalloc<int>(c->s, 3 * n);
   c->tcap = cap;
-  if (c->gexec) {   // the captured k_end holds the old trace pointers
-    (void)hipGraphExecDestroy(c->gexec);
-    c->gexec = nullptr;
-  }
+  drop_graphs(c);   // the captured k_end holds the old trace pointers
 }
 
 // One outer iteration (src/mod_cmfd.f90:465-496) as a fixed kernel sequence:
@@ -770,7 +838,7 @@ void ensure_trace(adg_ctx *c, int nout) {
 // end-of-outer kernel.  p_host is the outer index seen by the host
 // (profiling only).  smode: 0 forward (outer, outer_fs, outer_th), 1
 // adjoint (outer_ad), 2 transient (outer_tr).
-void enqueue_iteration(adg_ctx *c, int smode, int nin, int kern, int p_host, int nupd) {
+void enqueue_iteration(adg_ctx *c, int smode, int nin, int kern, int p_host, int nupd, bool with_nodal = true) {
   const LV lv = c->lv(c->tflat);
   const XS x = c->xs();
   const int adjoint = smode == 1;
@@ -792,7 +860,7 @@ void enqueue_iteration(adg_ctx *c, int smode, int nin, int kern, int p_host, int
                      c->errn, P1);
   hipLaunchKernelGGL(k_post, dim3(c->nflat), dim3(NTF), 0, c->s, lv, x, c->fs0, c->fs0c, c->errn, P1, P2);
   hipLaunchKernelGGL(k_fin_outer, dim3(c->B), dim3(NTF), 0, c->s, lv, P2);
-  if (kern != ADG_KERN_FDM) {
+  if (kern != ADG_KERN_FDM && with_nodal) {
     const bool timed = p_host > 0 && (p_host % nupd) == 0;
     const int en0 = timed ? prof_event(c) : -1;
     nodal_update(c, kern, adjoint ? 0 : (smode == 2 ? 2 : 1), 1);
@@ -890,22 +958,29 @@ int run_outer(adg_ctx *c, int mode, const adg_iter *it, int nit = 1) {
   // each core written by k_end turns its blocks into no-ops, and the host
   // polls the flags every `check` outers
   const bool use_graph = !c->prof && c->graphs;
+  // with one schedule for every core (nit = 1) the host knows which outers
+  // update the D-hats (outer q when q % nupd == 0; none when frozen): the
+  // others replay a graph without the nodal kernels, which would return at
+  // once there.  With one schedule per core every outer replays the full graph.
+  const bool split = use_graph && nit == 1 && it->kern != ADG_KERN_FDM;
+  const bool frozen = mode == ADG_ADJOINT_FROZEN;
   if (use_graph) {
     const int key[6] = {smode, it->nin, it->kern, it->nupd, c->variant, c->transpose ? 1 : 0};
-    if (!c->gexec || memcmp(key, c->gkey, sizeof key) != 0) {
-      if (c->gexec) {
-        (void)hipGraphExecDestroy(c->gexec);
-        c->gexec = nullptr;
-      }
+    if (memcmp(key, c->gkey, sizeof key) != 0) drop_graphs(c);
+    if (!c->gexec || (split && !c->gexec_plain)) {
       // one capture at a time in the process (contexts may run on several host threads)
       static std::mutex capture_lock;
       std::lock_guard<std::mutex> lk(capture_lock);
-      hipGraph_t graph;
-      HIPCHK(hipStreamBeginCapture(c->s, hipStreamCaptureModeThreadLocal));
-      enqueue_iteration(c, smode, it->nin, it->kern, 0, it->nupd);
-      HIPCHK(hipStreamEndCapture(c->s, &graph));
-      HIPCHK(hipGraphInstantiate(&c->gexec, graph, nullptr, nullptr, 0));
-      (void)hipGraphDestroy(graph);
+      for (int q = 0; q < 2; ++q) {
+        hipGraphExec_t &ge = q == 0 ? c->gexec : c->gexec_plain;
+        if (ge || (q == 1 && !split)) continue;
+        hipGraph_t graph;
+        HIPCHK(hipStreamBeginCapture(c->s, hipStreamCaptureModeThreadLocal));
+        enqueue_iteration(c, smode, it->nin, it->kern, 0, it->nupd, q == 0);
+        HIPCHK(hipStreamEndCapture(c->s, &graph));
+        HIPCHK(hipGraphInstantiate(&ge, graph, nullptr, nullptr, 0));
+        (void)hipGraphDestroy(graph);
+      }
       memcpy(c->gkey, key, sizeof key);
     }
   }
@@ -928,7 +1003,10 @@ int run_outer(adg_ctx *c, int mode, const adg_iter *it, int nit = 1) {
         }
         HIPCHK(hipEventRecord(c->nevs[nev], c->s));
       }
-      if (use_graph) HIPCHK(hipGraphLaunch(c->gexec, c->s));
+      if (use_graph) {
+        const bool plain = split && (frozen || launched % it->nupd != 0);
+        HIPCHK(hipGraphLaunch(plain ? c->gexec_plain : c->gexec, c->s));
+      }
       else enqueue_iteration(c, smode, it->nin, it->kern, launched, it->nupd);
       if (bracket) {
         HIPCHK(hipEventRecord(c->nevs[nev + 1], c->s));
@@ -1196,8 +1274,8 @@ int create_batch(const adg_geom *geoms, int ncore, adg_ctx **out) {
     }
     c->S = dalloc<double>(c->s, 3 * G * NC); c->a2 = dalloc<double>(c->s, 3 * G * NC); c->a4 = dalloc<double>(c->s, 3 * G * NC);
     c->L1 = dalloc<double>(c->s, 3 * G * NC);
-    build_tables(c);
-    c->npart = dalloc<double>(c->s, (size_t)3 * 3 * c->nface);
+    build_tables(c, fls);
+    c->npart = dalloc<double>(c->s, (size_t)3 * (3 * c->nface + c->nbface));
     c->sc = dalloc<Scal>(c->s, ncore);
     HIPCHK(hipHostMalloc((void **)&c->hsc, ncore * sizeof(Scal), hipHostMallocDefault));
   } catch (const HipError &e) {
@@ -1345,7 +1423,7 @@ int adg_core_count(adg_ctx *c) { return c ? c->B : 0; }
 void adg_destroy(adg_ctx *c) {
   if (!c) return;
   if (c->s) (void)hipStreamSynchronize(c->s);
-  void *ptrs[] = {c->geo, c->tflat, c->tcell, c->tface, c->d_hx, c->d_hy, c->d_hz, c->flag, c->D, c->sigr, c->nuf,
+  void *ptrs[] = {c->geo, c->tflat, c->tcell, c->tface, c->tbface, c->bfl, c->d_hx, c->d_hy, c->d_hz, c->flag, c->D, c->sigr, c->nuf,
                   c->sigf, c->chi, c->sigs, c->dc, c->exsrc, c->vdel, c->f0, c->f0c, c->fs0, c->fs0c, c->errn,
                   c->dn, c->df, c->A64, c->b, c->pw, c->tmp, c->rr64[0], c->rr64[1], c->rs64, c->t64, c->pp64[0],
                   c->pp64[1], c->vv64[0], c->vv64[1], c->A32, c->rr32[0], c->rr32[1], c->rs32, c->t32, c->d32,
@@ -1357,7 +1435,7 @@ void adg_destroy(adg_ctx *c) {
   if (c->hsc) (void)hipHostFree(c->hsc);
   for (hipEvent_t e : c->evs) (void)hipEventDestroy(e);
   for (hipEvent_t e : c->nevs) (void)hipEventDestroy(e);
-  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+  drop_graphs(c);
   void *tptr[] = {c->tk, c->tser, c->tfer, c->tnd, c->tflags, c->tloc};
   for (void *p : tptr)
     if (p) (void)hipFree(p);
@@ -2317,18 +2395,27 @@ int adg_autotune(adg_ctx *c, int nin, double *out4) {
         best = cp;
       }
     }
+    // the persistent kernel with each checked brick plan
+    size_t best_plan = 0;
     if (can_persist) {
       c->persist = true;
-      const float ms = time_solve();
-      HIPCHK(hipMemcpyAsync(c->hsc, c->sc, sizeof(Scal), hipMemcpyDeviceToHost, c->s));
-      HIPCHK(hipStreamSynchronize(c->s));
-      if (c->hsc->status == ADG_E_DEVICE_SYNC) {
-        reset_psync(c);
-        set_scal_start(c, nullptr);
-      } else if (ms < tbest) {
-        tbest = ms;
-        best_persist = true;
+      for (size_t q = 0; q < c->pcand.size(); ++q) {
+        c->pb = c->pcand[q];
+        c->pkc = c->pcand_kc[q];
+        const float ms = time_solve();
+        HIPCHK(hipMemcpyAsync(c->hsc, c->sc, sizeof(Scal), hipMemcpyDeviceToHost, c->s));
+        HIPCHK(hipStreamSynchronize(c->s));
+        if (c->hsc->status == ADG_E_DEVICE_SYNC) {
+          reset_psync(c);
+          set_scal_start(c, nullptr);
+        } else if (ms < tbest) {
+          tbest = ms;
+          best_persist = true;
+          best_plan = q;
+        }
       }
+      c->pb = c->pcand[best_plan];
+      c->pkc = c->pcand_kc[best_plan];
     }
     c->prof = prof;
     install_flat(c, best);
@@ -2342,7 +2429,7 @@ int adg_autotune(adg_ctx *c, int nin, double *out4) {
       out4[0] = best_persist ? 0.0 : (double)best;
       out4[1] = tbest;
       out4[2] = best_persist ? (double)c->pb.nb : (double)c->nflat;
-      out4[3] = (double)cand.size() + (can_persist ? 1.0 : 0.0);
+      out4[3] = (double)(cand.size() + (can_persist ? c->pcand.size() : 0));
     }
   } catch (const HipError &e) {
     return hip_fail(c, e);

@@ -394,23 +394,18 @@ __global__ void __launch_bounds__(NT) k_nod_pass1(NodArgs a) {
 // pass 2: per face -- get_a1matvec(_first/_last) (:483-698), LU_solve,
 // get_a3 (:702-736) and nodal_coup_upd (:282-371).  blockIdx.y = direction.
 // ---------------------------------------------------------------------------
-template <int G>
-__global__ void __launch_bounds__(NT) k_nod_pass2(NodArgs a) {
+// One face of direction u, face index t of that direction (faces of
+// direction u: (n_u + 1) x (other two extents), the face index fastest):
+// the new D-hat of the face and the running argmax (best, bkey, bloc).
+// BONLY: boundary faces only (the interface system is not compiled in).
+template <int G, bool BONLY>
+__device__ __forceinline__ void face_update(const NodArgs &a, const Geo &q, Scal *sc, int u, long t, double &best,
+                                            double &bkey, double &bloc) {
 #pragma clang fp contract(off)
-  const BlkView v = blk_view(a.lv);
-  const Geo &q = *v.q;
-  Scal *sc = v.sc;
-  if (a.sched && !nodal_outer(sc)) return;   // uniform over the core's blocks
-  const int u = blockIdx.y;
   const long NC = q.NC;
-  // faces of direction u: (n_u + 1) x (other two extents)
   const int nu = u == 0 ? q.nxx : (u == 1 ? q.nyy : q.nzz);
   const int e1 = u == 0 ? q.nyy : q.nxx;          // second index extent (j for x; i for y, z)
-  const int e2 = u == 2 ? q.nyy : q.nzz;          // slowest (k for x, y; j for z)
-  const long nf = (long)(nu + 1) * e1 * e2;
-  const long t = v.b.lo + threadIdx.x;
-  double best = -1.0, bkey = 0.0, bloc = 0.0;
-  if (t < nf && !sc->done) {
+  {
     const int fi = (int)(t % (nu + 1));
     const long r = t / (nu + 1);
     const int o1 = (int)(r % e1), o2 = (int)(r / e1);
@@ -423,7 +418,7 @@ __global__ void __launch_bounds__(NT) k_nod_pass2(NodArgs a) {
     const long cn = cp - st;                // node below the face ("n")
     const bool hasp = (q.flag[cp] & F_ACTIVE) != 0;
     const bool hasn = (q.flag[cn] & F_ACTIVE) != 0;
-    if ((hasp || hasn) && !(a.coop && hasp && hasn)) {
+    if ((hasp || hasn) && !((BONLY || a.coop) && hasp && hasn)) {
       const double Ke = sc->Ke;
       const int sf = 2 * u;                 // upper face of n (x+, y+, z+), sf+1 lower face of p
       const unsigned long long ord = face_order(q, u, fi, o1, o2);
@@ -434,6 +429,7 @@ __global__ void __launch_bounds__(NT) k_nod_pass2(NodArgs a) {
       Coef6 cfn[G], cfp[G];
       double Bn[G * G], Bp[G * G];
       if (hasn && hasp) {
+        if constexpr (!BONLY) {
         const double hn = hdir(q, u, u == 0 ? i - 1 : i, u == 1 ? j - 1 : j, u == 2 ? k - 1 : k);
         const double hp = hdir(q, u, i, j, k);
         nod_B<G>(a, NC, cn, hn, Ke, Bn);
@@ -499,6 +495,7 @@ __global__ void __launch_bounds__(NT) k_nod_pass2(NodArgs a) {
           a.dn[op] = v;
           const double nder = fabs(v - ndpr);
           if (nder > best) { best = nder; bkey = (double)(ord * G + g); }
+        }
         }
       } else if (hasp) {
         // first node of a line (get_coefs_first, :375-407)
@@ -595,7 +592,11 @@ __global__ void __launch_bounds__(NT) k_nod_pass2(NodArgs a) {
       bloc = (double)(((li + 1) << 20) | ((lj + 1) << 10) | (lk + 1));
     }
   }
-  // block argmax (largest value, then smallest sweep key) -> partials
+}
+
+// block argmax (largest value, then smallest sweep key) into the block's
+// partial slot
+__device__ __forceinline__ void face_argmax(double best, double bkey, double bloc, double *slot3) {
   __shared__ double sv[NT], sk[NT], sl[NT];
   sv[threadIdx.x] = best;
   sk[threadIdx.x] = bkey;
@@ -611,23 +612,186 @@ __global__ void __launch_bounds__(NT) k_nod_pass2(NodArgs a) {
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    const size_t slot = ((size_t)blockIdx.y * gridDim.x + blockIdx.x) * 3;
-    a.part[slot] = sv[0];
-    a.part[slot + 1] = sk[0];
-    a.part[slot + 2] = sl[0];
+    slot3[0] = sv[0];
+    slot3[1] = sk[0];
+    slot3[2] = sl[0];
+  }
+}
+
+// every face of the launch's face range (blockIdx.y = direction); with
+// coop set the interior faces are left to k_nod_pass2c and the boundary
+// faces to k_nod_pass2b, so this kernel runs only without coop
+template <int G>
+__global__ void __launch_bounds__(NT) k_nod_pass2(NodArgs a) {
+  const BlkView v = blk_view(a.lv);
+  const Geo &q = *v.q;
+  Scal *sc = v.sc;
+  if (a.sched && !nodal_outer(sc)) return;   // uniform over the core's blocks
+  const int u = blockIdx.y;
+  const int nu = u == 0 ? q.nxx : (u == 1 ? q.nyy : q.nzz);
+  const int e1 = u == 0 ? q.nyy : q.nxx;
+  const int e2 = u == 2 ? q.nyy : q.nzz;
+  const long nf = (long)(nu + 1) * e1 * e2;
+  const long t = v.b.lo + threadIdx.x;
+  double best = -1.0, bkey = 0.0, bloc = 0.0;
+  if (t < nf && !sc->done) face_update<G, false>(a, q, sc, u, t, best, bkey, bloc);
+  face_argmax(best, bkey, bloc, a.part + ((size_t)blockIdx.y * gridDim.x + blockIdx.x) * 3);
+}
+
+// the boundary faces (one node active) from the list bfl of the core's
+// (direction << 40 | face index) entries, one per thread; the partials go
+// after the 3 x nfb slots of the face launches
+template <int G>
+__global__ void __launch_bounds__(NT) k_nod_pass2b(NodArgs a, const long *bfl, int nfb) {
+  const BlkView v = blk_view(a.lv);
+  const Geo &q = *v.q;
+  Scal *sc = v.sc;
+  if (a.sched && !nodal_outer(sc)) return;
+  const long t = v.b.lo + threadIdx.x;
+  double best = -1.0, bkey = 0.0, bloc = 0.0;
+  if (t < v.b.hi && !sc->done) {
+    const long e = bfl[t];
+    face_update<G, true>(a, q, sc, (int)(e >> 40), e & ((1L << 40) - 1), best, bkey, bloc);
+  }
+  face_argmax(best, bkey, bloc, a.part + ((size_t)3 * nfb + blockIdx.x) * 3);
+}
+
+// ---------------------------------------------------------------------------
+// LU_solve (:829-897) of an M x M system held one row per lane: lane S r + f
+// of a wavefront owns row r of item f (S items side by side).  The
+// elimination broadcasts each pivot row with lane shuffles and the
+// substitutions each solved unknown, so every entry sees the operations of
+// the per-thread lu_solve in the same order and no lane holds more than its
+// own row.  On return every lane of the item holds the solution in xs.
+// Lanes with r >= M take part in the shuffles only.
+// ---------------------------------------------------------------------------
+
+// the reference's input-diagonal test for this lane's row
+template <int M>
+__device__ __forceinline__ bool coop_lu_diag_small(const double (&R)[M], int r) {
+  double dg = 0.0;
+#pragma unroll
+  for (int c2 = 0; c2 < M; ++c2)
+    if (c2 == r) dg = R[c2];
+  return r < M && fabs(dg) < (double)10e-5f;
+}
+
+template <int M, int S>
+__device__ __forceinline__ void coop_lu(double (&R)[M], double s, int r, int f, double (&xs)[M]) {
+#pragma clang fp contract(off)
+  const bool rowlane = r < M;
+#pragma unroll
+  for (int p = 0; p < M; ++p) {
+    // pivot row p (its entries p..M-1, final after the steps before p)
+    double pr[M];
+#pragma unroll
+    for (int c2 = p; c2 < M; ++c2) pr[c2] = __shfl(R[c2], S * p + f, 64);
+    if (r > p && rowlane) {
+      const double piv = R[p] / pr[p];
+#pragma unroll
+      for (int c2 = p + 1; c2 < M; ++c2) R[c2] = R[c2] - piv * pr[c2];
+      R[p] = piv;
+    }
+  }
+  // forward substitution y(i) = b(i) - sum_k<i L(i,k) y(k), k ascending
+  double isum = 0.0, y = 0.0;
+#pragma unroll
+  for (int p = 0; p < M; ++p) {
+    if (r == p) y = s - isum;
+    const double yp = __shfl(y, S * p + f, 64);
+    if (r > p) isum = isum + R[p] * yp;
+  }
+  // back substitution x(i) = (y(i) - sum_k>i U(i,k) x(k)) / U(i,i), k ascending
+#pragma unroll
+  for (int p = M - 1; p >= 0; --p) {
+    double xv;
+    if (p == M - 1) {
+      xv = y / R[p];
+    } else {
+      double bs = 0.0;
+#pragma unroll
+      for (int c2 = p + 1; c2 < M; ++c2) bs = bs + R[c2] * xs[c2];
+      xv = (y - bs) / R[p];
+    }
+    xs[p] = __shfl(xv, S * p + f, 64);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// pass 1, cooperative form for larger group counts: eight lanes per (node,
+// direction), lane 8 r + f owning row r of node f's G x G system of
+// get_a2matvec (eight neighbouring nodes per wavefront).  Same arithmetic,
+// in the same order, as k_nod_pass1.
+// ---------------------------------------------------------------------------
+template <int G>
+__global__ void __launch_bounds__(NT) k_nod_pass1c(NodArgs a) {
+#pragma clang fp contract(off)
+  static_assert(G <= 8, "eight lanes per node");
+  const BlkView v = blk_view(a.lv);
+  const Geo &q = *v.q;
+  Scal *sc = v.sc;
+  if (sc->done || (a.sched && !nodal_outer(sc))) return;   // uniform over the core's blocks
+  const int u = blockIdx.y;
+  const long NC = q.NC;
+  const long nxy = (long)q.nxx * q.nyy;
+  const int lane = threadIdx.x & 63, r = lane >> 3, f = lane & 7;
+  const int slot32 = (int)(threadIdx.x >> 6) * 8 + f;
+  const int g = r < G ? r : 0;   // lanes past the last row repeat row 0's loads
+  for (int it = 0; it < NT / 32; ++it) {
+    const long t = v.b.lo + (long)it * 32 + slot32;
+    if (!(t < v.b.hi && t < nxy * q.nzz)) continue;   // uniform over the node's eight lanes
+    const int k = (int)(t / nxy);
+    const long rr = t - (long)k * nxy;
+    const int j = (int)(rr / q.nxx), i = (int)(rr - (long)j * q.nxx);
+    const long c = cell_of(q, i, j, k);
+    if (!(q.flag[c] & F_ACTIVE)) continue;
+    const Line L = line_at(q, u, i, j, k, c);
+    const int bclo = q.bc[2 * u + 1], bchi = q.bc[2 * u];
+    if (L.first && L.last) {   // single-node line: the reference reads an undefined neighbour
+      if (r == 0) atomicMin(&sc->lu_key, lu_key(0, i, j, k));
+      continue;
+    }
+    const double h = L.hn, Ke = sc->Ke;
+    double B[G], R[G];
+    nod_B_row<G>(a, NC, c, h, Ke, g, B);
+    const Coef6 cf = nod_coef(a.kern, a.xs.sigr[(size_t)g * NC + c], a.xs.D[(size_t)g * NC + c], h);
+    const double *Su = a.S + (size_t)u * G * NC;
+    const double D = a.xs.D[(size_t)g * NC + c];
+    const double S = a.cmode == 2 ? 0.25 * (h * h) / D * Su[(size_t)g * NC + c]
+                                  : 0.25 * (h * h) / D * (Su[(size_t)g * NC + c] - a.xs.exsrc[(size_t)g * NC + c]);
+    double Bf = 0.0;
+#pragma unroll
+    for (int hh = 0; hh < G; ++hh) {
+      R[hh] = (hh == g) ? B[hh] * cf.E + 3.0 : B[hh] * cf.E;
+      Bf = Bf + B[hh] * a.f0[(size_t)hh * NC + c];
+    }
+    const double Lm2 = 0.25 * (h * h) / D * tl2(L, Su + (size_t)g * NC, c, bclo, bchi);
+    const double rhs = Bf - cf.E * Lm2 + S;
+    if (coop_lu_diag_small<G>(R, r)) {
+      const int fi = u == 0 ? i : (u == 1 ? j : k);
+      const int o1 = u == 0 ? j : i, o2 = u == 2 ? j : k;
+      atomicMin(&sc->lu_key, lu_key(face_order(q, u, fi, o1, o2), i, j, k));
+    }
+    double xs[G];
+    coop_lu<G, 8>(R, rhs, r, f, xs);
+    if (r < G) {
+      double Bx = 0.0;
+#pragma unroll
+      for (int hh = 0; hh < G; ++hh) Bx = Bx + B[hh] * xs[hh];
+      const size_t o = ((size_t)u * G + g) * NC + c;
+      a.a2[o] = xs[g];
+      a.a4[o] = cf.B * (Bx + Lm2);
+      a.L1[o] = 0.25 * (h * h) / D * tl1(L, Su + (size_t)g * NC, c, bclo, bchi);
+    }
   }
 }
 
 // ---------------------------------------------------------------------------
 // pass 2, interior faces, cooperative form for larger group counts: 16 lanes
 // per face (four faces per wavefront), lane 4 r + f owning row r of face f's
-// 2G x 2G system of get_a1matvec
-// (:630-698).  The elimination broadcasts the pivot row with lane shuffles,
-// so no lane holds more than its own row; every entry sees the operations
-// of the per-thread lu_solve in the same order.  A block covers the same
-// 256-face range as k_nod_pass2 (16 faces at a time) and merges its argmax
-// partial with the one that k_nod_pass2 (boundary faces only, coop set)
-// wrote for the block.
+// 2G x 2G system of get_a1matvec (:630-698), solved by coop_lu.  A block
+// covers the same 256-face range as k_nod_pass2 (16 faces at a time); the
+// boundary faces are k_nod_pass2b's.
 // ---------------------------------------------------------------------------
 template <int G>
 __global__ void __launch_bounds__(NT) k_nod_pass2c(NodArgs a) {
@@ -649,7 +813,6 @@ __global__ void __launch_bounds__(NT) k_nod_pass2c(NodArgs a) {
   const int lane = threadIdx.x & 63, r = lane >> 2, fw = lane & 3;
   const int slot16 = (int)(threadIdx.x >> 6) * 4 + fw;
   const int g = r % G;                         // group of this lane's row (rows r and r + G)
-  const bool rowlane = r < M2;
   double best = -1.0, bkey = 0.0, bloc = 0.0;
   for (int it = 0; it < NT / 16; ++it) {
     const long t = v.b.lo + (long)it * 16 + slot16;
@@ -682,20 +845,49 @@ __global__ void __launch_bounds__(NT) k_nod_pass2c(NodArgs a) {
     if (u == 0) li = i - 1; else if (u == 1) lj = j - 1; else lk = k - 1;
     const double hn = hdir(q, u, u == 0 ? i - 1 : i, u == 1 ? j - 1 : j, u == 2 ? k - 1 : k);
     const double hp = hdir(q, u, i, j, k);
-    // this lane's group: rows g of B at n and p, SANM/PNM coefficients
+    // this lane's group g: rows 0..G-1 make node n's row of B, coefficients
+    // and moments, rows G..2G-1 node p's; the partner row (r + G or r - G)
+    // swaps its side in, so every lane holds both sides, computed as the
+    // per-thread kernel computes them
+    const bool nside = r < G;
+    const long cs = nside ? cn : cp;
+    const double hsd = nside ? hn : hp;
+    double Bs[G];
+    nod_B_row<G>(a, NC, cs, hsd, Ke, g, Bs);
+    const Coef6 cfs = nod_coef(a.kern, a.xs.sigr[(size_t)g * NC + cs], a.xs.D[(size_t)g * NC + cs], hsd);
+    const double Ls = a.L1[ub + (size_t)g * NC + cs];
+    const double a2s = a.a2[ub + (size_t)g * NC + cs], a4s = a.a4[ub + (size_t)g * NC + cs];
+    const double Ps = 2.0 * a.xs.D[(size_t)g * NC + cs] / hsd;
+    const double dcs = a.xs.dc[((size_t)(nside ? sf : sf + 1) * G + g) * NC + cs];
+    const double fs = a.f0[(size_t)g * NC + cs];
+    const int pl = 4 * (nside ? r + G : r - G) + fw;
+    double Bo[G];
+#pragma unroll
+    for (int hh = 0; hh < G; ++hh) Bo[hh] = __shfl(Bs[hh], pl, 64);
+    Coef6 cfo;
+    cfo.A = __shfl(cfs.A, pl, 64);
+    cfo.B = 0.0;
+    cfo.E = 0.0;
+    cfo.F = __shfl(cfs.F, pl, 64);
+    cfo.G = __shfl(cfs.G, pl, 64);
+    cfo.H = __shfl(cfs.H, pl, 64);
+    const double Lo = __shfl(Ls, pl, 64), a2o = __shfl(a2s, pl, 64), a4o = __shfl(a4s, pl, 64);
+    const double Po = __shfl(Ps, pl, 64), dco = __shfl(dcs, pl, 64), fo = __shfl(fs, pl, 64);
     double Bn[G], Bp[G];
-    nod_B_row<G>(a, NC, cn, hn, Ke, g, Bn);
-    nod_B_row<G>(a, NC, cp, hp, Ke, g, Bp);
-    const Coef6 cfn = nod_coef(a.kern, a.xs.sigr[(size_t)g * NC + cn], a.xs.D[(size_t)g * NC + cn], hn);
-    const Coef6 cfp = nod_coef(a.kern, a.xs.sigr[(size_t)g * NC + cp], a.xs.D[(size_t)g * NC + cp], hp);
-    const double Ln1 = a.L1[ub + (size_t)g * NC + cn], Lp1 = a.L1[ub + (size_t)g * NC + cp];
-    const double a2n = a.a2[ub + (size_t)g * NC + cn], a4n = a.a4[ub + (size_t)g * NC + cn];
-    const double a2p = a.a2[ub + (size_t)g * NC + cp], a4p = a.a4[ub + (size_t)g * NC + cp];
+#pragma unroll
+    for (int hh = 0; hh < G; ++hh) {
+      Bn[hh] = nside ? Bs[hh] : Bo[hh];
+      Bp[hh] = nside ? Bo[hh] : Bs[hh];
+    }
+    const Coef6 cfn = nside ? cfs : cfo, cfp = nside ? cfo : cfs;
+    const double Ln1 = nside ? Ls : Lo, Lp1 = nside ? Lo : Ls;
+    const double a2n = nside ? a2s : a2o, a4n = nside ? a4s : a4o;
+    const double a2p = nside ? a2o : a2s, a4p = nside ? a4o : a4s;
+    const double fn0 = nside ? fs : fo, fp0 = nside ? fo : fs;
     // row r of the system
     double R[M2], s;
     if (r < G) {
-      const double Pn = 2.0 * a.xs.D[(size_t)g * NC + cn] / hn;
-      const double Pp = 2.0 * a.xs.D[(size_t)g * NC + cp] / hp;
+      const double Pn = Ps, Pp = Po;   // 2 D / h of n and p
 #pragma unroll
       for (int hh = 0; hh < G; ++hh) {
         if (hh == g) {
@@ -708,8 +900,7 @@ __global__ void __launch_bounds__(NT) k_nod_pass2c(NodArgs a) {
       }
       s = Pn * (3.0 * a2n + cfn.G * a4n + cfn.F * Ln1) + Pp * (3.0 * a2p + cfp.G * a4p - cfp.F * Lp1);
     } else {
-      const double dcn = a.xs.dc[((size_t)sf * G + g) * NC + cn];
-      const double dcp = a.xs.dc[((size_t)(sf + 1) * G + g) * NC + cp];
+      const double dcn = dco, dcp = dcs;   // dc of n's upper face and of p's lower face
 #pragma unroll
       for (int hh = 0; hh < G; ++hh) {
         if (hh == g) {
@@ -721,50 +912,12 @@ __global__ void __launch_bounds__(NT) k_nod_pass2c(NodArgs a) {
         }
       }
       // the reference pairs An with Ln1 and Ap with Lp1 here (:693-694)
-      s = dcp * (a2p + a4p + a.f0[(size_t)g * NC + cp] - cfn.A * Ln1) -
-          dcn * (a2n + a4n + a.f0[(size_t)g * NC + cn] + cfp.A * Lp1);
+      s = dcp * (a2p + a4p + fp0 - cfn.A * Ln1) - dcn * (a2n + a4n + fn0 + cfp.A * Lp1);
     }
     // LU_solve (:829-897): the input-diagonal test, then the elimination
-    double dg = 0.0;
-#pragma unroll
-    for (int c2 = 0; c2 < M2; ++c2) if (c2 == r) dg = R[c2];
-    if (rowlane && fabs(dg) < (double)10e-5f) atomicMin(&sc->lu_key, lu_key(ord, li, lj, lk));
-#pragma unroll
-    for (int p = 0; p < M2; ++p) {
-      // pivot row p (its entries p..M2-1, final after the steps before p)
-      double pr[M2];
-#pragma unroll
-      for (int c2 = p; c2 < M2; ++c2) pr[c2] = __shfl(R[c2], 4 * p + fw, 64);
-      if (r > p && rowlane) {
-        const double piv = R[p] / pr[p];
-#pragma unroll
-        for (int c2 = p + 1; c2 < M2; ++c2) R[c2] = R[c2] - piv * pr[c2];
-        R[p] = piv;
-      }
-    }
-    // forward substitution y(i) = b(i) - sum_k<i L(i,k) y(k), k ascending
-    double isum = 0.0, y = 0.0;
-#pragma unroll
-    for (int p = 0; p < M2; ++p) {
-      if (r == p) y = s - isum;
-      const double yp = __shfl(y, 4 * p + fw, 64);
-      if (r > p) isum = isum + R[p] * yp;
-    }
-    // back substitution x(i) = (y(i) - sum_k>i U(i,k) x(k)) / U(i,i), k ascending
+    if (coop_lu_diag_small<M2>(R, r)) atomicMin(&sc->lu_key, lu_key(ord, li, lj, lk));
     double xs[M2];
-#pragma unroll
-    for (int p = M2 - 1; p >= 0; --p) {
-      double xv;
-      if (p == M2 - 1) {
-        xv = y / R[p];
-      } else {
-        double bs = 0.0;
-#pragma unroll
-        for (int c2 = p + 1; c2 < M2; ++c2) bs = bs + R[c2] * xs[c2];
-        xv = (y - bs) / R[p];
-      }
-      xs[p] = __shfl(xv, 4 * p + fw, 64);
-    }
+    coop_lu<M2, 4>(R, s, r, fw, xs);
     // get_a3 and nodal_coup_upd for group g (lanes 0..G-1)
     if (r < G) {
       double Bf = 0.0;
@@ -774,7 +927,7 @@ __global__ void __launch_bounds__(NT) k_nod_pass2c(NodArgs a) {
       const double jp = -2.0 * a.xs.D[(size_t)g * NC + cn] / hn * (xs[g] + 3.0 * a2n + cfn.H * a3 + cfn.G * a4n);
       const size_t on = ((size_t)sf * G + g) * NC + cn, op = ((size_t)(sf + 1) * G + g) * NC + cp;
       const double ndpr = a.dn[on];
-      const double fn = a.f0[(size_t)g * NC + cn], fp = a.f0[(size_t)g * NC + cp];
+      const double fn = fn0, fp = fp0;
       const double dv = (a.df[on] * (fn - fp) - jp) / (fn + fp);
       a.dn[on] = dv;
       a.dn[op] = dv;
@@ -804,8 +957,6 @@ __global__ void __launch_bounds__(NT) k_nod_pass2c(NodArgs a) {
   }
   if (threadIdx.x == 0) {
     const size_t slot = ((size_t)blockIdx.y * gridDim.x + blockIdx.x) * 3;
-    const double v0 = a.part[slot], k0 = a.part[slot + 1];
-    if (v0 > sv[0] || (v0 == sv[0] && k0 < sk[0])) return;
     a.part[slot] = sv[0];
     a.part[slot + 1] = sk[0];
     a.part[slot + 2] = sl[0];
@@ -814,7 +965,8 @@ __global__ void __launch_bounds__(NT) k_nod_pass2c(NodArgs a) {
 
 // final argmax over the pass-2 partials, one block per core (blockIdx.x =
 // core): its face blocks [ffirst, ffirst + fnb) of each direction, nfb face
-// blocks per direction in the launch
+// blocks per direction in the launch, then its boundary-face blocks
+// [bfirst, bfirst + bnb) of k_nod_pass2b
 __global__ void __launch_bounds__(NT) k_nod_argmax(LV lv, const double *part, int nfb, int sched) {
   const Geo &q = lv.geo[blockIdx.x];
   Scal *sc = lv.sc + blockIdx.x;
@@ -822,9 +974,14 @@ __global__ void __launch_bounds__(NT) k_nod_argmax(LV lv, const double *part, in
   __shared__ double sv[NT], sk[NT], sl[NT];
   double best = -1.0, bkey = 0.0, bloc = 0.0;
   const int nslots = 3 * q.fnb;
-  for (int s = threadIdx.x; s < nslots; s += NT) {
-    const int u = s / q.fnb, fb = s - u * q.fnb;
-    const size_t o = ((size_t)u * nfb + q.ffirst + fb) * 3;
+  for (int s = threadIdx.x; s < nslots + q.bnb; s += NT) {
+    size_t o;
+    if (s < nslots) {
+      const int u = s / q.fnb, fb = s - u * q.fnb;
+      o = ((size_t)u * nfb + q.ffirst + fb) * 3;
+    } else {
+      o = ((size_t)3 * nfb + q.bfirst + (s - nslots)) * 3;   // the boundary-face blocks
+    }
     const double v = part[o], kk = part[o + 1];
     if (v > best || (v == best && kk < bkey)) { best = v; bkey = kk; bloc = part[o + 2]; }
   }

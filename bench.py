@@ -456,14 +456,16 @@ def dropin_leg(workload, seed):
 
 def measured_traffic(workload, kernel):
     """HBM-side bytes per launch of `kernel` from the committed counter
-    summary profiles/pmc_bicg_<workload>.txt (FETCH_SIZE and WRITE_SIZE runs
-    of --kernel-probe, scaled by the 256 MiB copy calibration), or None."""
-    path = os.path.join(ROOT, "profiles", f"pmc_bicg_{workload}.txt")
-    if not os.path.exists(path):
-        return None
-    for line in open(path):
-        if line.startswith(kernel + " "):
-            return int(round(float(line[len(kernel):].split()[3]) * 1e6))
+    summaries profiles/pmc_bicg_<workload>.txt and (the flat BiCGSTAB form)
+    profiles/pmc_bicg_flat_<workload>.txt (FETCH_SIZE and WRITE_SIZE runs of
+    --kernel-probe, scaled by the 256 MiB copy calibration), or None."""
+    for name in (f"pmc_bicg_{workload}.txt", f"pmc_bicg_flat_{workload}.txt"):
+        path = os.path.join(ROOT, "profiles", name)
+        if not os.path.exists(path):
+            continue
+        for line in open(path):
+            if line.startswith(kernel + " "):
+                return int(round(float(line[len(kernel):].split()[3]) * 1e6))
     return None
 
 
@@ -552,6 +554,22 @@ def kernel_probe(p):
     print(json.dumps(out))
 
 
+def nodal_probe(p):
+    """Short program for hardware-counter runs of the nodal update: 20 FDM
+    outers from the first-call state, then two SANM nodal updates (each a
+    leakage pass, the node pass, the interface pass, the boundary-face pass
+    and the argmax) on that state, in FP64."""
+    from adpres_amd.engine import Engine
+    e = Engine(p, "fp64")
+    e.outer(nout=20, kern="FDM", raise_on_error=False)
+    out = {"probe": "nodal-counters", "nodes": p.nnod, "groups": p.ng}
+    for k in range(2):
+        nd, loc = e.nodal_update()
+        out[f"ndmax_{k}"] = float(nd)
+    e.close()
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -565,6 +583,8 @@ def main():
                     help="launch kernels directly instead of replaying the captured outer (for profilers)")
     ap.add_argument("--kernel-probe", action="store_true",
                     help="run only a copy of known size and one BiCGSTAB call per precision (for counter runs)")
+    ap.add_argument("--nodal-probe", action="store_true",
+                    help="run only two nodal updates of the workload core (for counter runs)")
     args = ap.parse_args()
     if args.no_graph:
         os.environ["ADG_GRAPH"] = "0"
@@ -592,6 +612,9 @@ def main():
     log(f"bench: {args.workload} N={p.nnod} G={p.ng} unknowns={p.unknowns}")
     if args.kernel_probe:
         kernel_probe(p)
+        return
+    if args.nodal_probe:
+        nodal_probe(p)
         return
 
     eng = Engine(p, "fp64")

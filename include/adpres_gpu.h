@@ -251,7 +251,9 @@ int adg_last_timing(adg_ctx *ctx, double *ms4);
  * BiCGSTAB group solve of nin iterations on the context's matrices for each
  * candidate shape of the flat kernels (one core: the largest grid, 256 to
  * 2048 blocks; a batch: 256, 512 or 1024 cells per block) and, for a single
- * core with a brick plan, the persistent kernel, and keeps the fastest.
+ * core, the persistent kernel with each of its brick plans (up to four, the
+ * cheapest by the planner's cost model, each checked on the host), and
+ * keeps the fastest.
  * The solution state is left as it was; results stay within the parity bars
  * for any shape (only the summation order of the reductions depends on
  * it).  out4: chosen shape (0: the persistent kernel), its ms per group

@@ -129,6 +129,45 @@ def test_gpu_batch256_matches_reference(precision):
             assert rel_err(powers[i], d[f"pow_{i}"].astype(np.float64)) < 1e-4, i
 
 
+def test_gpu_batch_of_eight_group_cores_nodal_passes(monkeypatch):
+    """Two 8-group cores in one batch (the 68x68x96 core and the same core
+    with nu-sigma_f raised by 0.2%), 186 outers (two nodal updates; no
+    convergence test).  The cooperative nodal passes and the per-core
+    boundary-face lists (their blocks per core, their argmax slots after the
+    face blocks') give D-hats, k and node powers bit-identical to the
+    per-thread nodal kernels (ADG_NOD_COOP=0) in the same batch; and each
+    core's k is within 1e-9 of its own context's.  (The D-hats themselves
+    are not compared across the two BiCGSTAB forms: at a few faces of the
+    fast group near the outer reflector the interface systems amplify the
+    summation-order differences of the flux to 1e-2 relative.)"""
+    import dataclasses
+    from adpres_amd.configs import generate
+    p = generate("core3d_8g", 2).problem
+    probs = [p, dataclasses.replace(p, nuf=p.nuf * 1.002, sigf=p.sigf * 1.002)]
+    ov = dict(nout=186, serc=0.0, ferc=0.0)
+    res = {}
+    for coop in ("1", "0"):
+        monkeypatch.setenv("ADG_NOD_COOP", coop)
+        print(f"batch of two 8-group cores, 186 outers, ADG_NOD_COOP={coop}", flush=True)
+        bt = Batch(probs)
+        st, ni, ke = bt.outer(**ov)
+        assert list(ni) == [186, 186], (st, ni)
+        res[coop] = (ke.copy(), [bt.state(b)["dn"] for b in range(2)], [bt.powdis(b) for b in range(2)])
+        bt.close()
+    (k1, d1, p1), (k0, d0, p0) = res["1"], res["0"]
+    assert np.array_equal(k1, k0)
+    for b in range(2):
+        assert np.array_equal(d1[b], d0[b]), b
+        assert np.array_equal(p1[b], p0[b]), b
+    monkeypatch.delenv("ADG_NOD_COOP")
+    for b, q in enumerate(probs):
+        e = Engine(q)
+        e.outer(raise_on_error=False, **ov)
+        assert e.niter == 186
+        assert abs(k1[b] - e.Ke) < 1e-9 * e.Ke, (b, k1[b], e.Ke)
+        e.close()
+
+
 def test_gpu_contexts_on_concurrent_threads_match_sequential():
     """Independent contexts driven from several host threads at once (each on
     its own stream, graph captures serialized inside the library) give the

@@ -245,11 +245,12 @@ def test_gpu_frozen_adjoint_matches_reference(precision):
 
 
 def test_gpu_cooperative_nodal_pass_is_bit_identical(monkeypatch):
-    """For G >= 5 the interior faces of nodal pass 2 run in the cooperative
-    kernel (one lane per row of the 2G x 2G system, pivot rows by lane
-    shuffles).  Every entry sees the per-thread kernel's operations in the
-    same order, so the D-hats, ndmax and its location are bit-identical on
-    the 8-group core."""
+    """For G >= 5 nodal pass 1 and the interior faces of pass 2 run in the
+    cooperative kernels (one lane per row of the G x G and 2G x 2G systems,
+    pivot rows by lane shuffles) and the boundary faces from their list.
+    Every entry sees the per-thread kernels' operations in the same order,
+    so the D-hats, ndmax and its location are bit-identical on the 8-group
+    core."""
     p = generate("core3d_8g", 2).problem
     rng = np.random.default_rng(5)
     f0 = rng.uniform(0.9, 1.1, (p.ng, p.nnod))
