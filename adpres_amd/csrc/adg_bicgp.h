@@ -369,18 +369,21 @@ __device__ __forceinline__ unsigned *line_flag(double *line) { return (unsigned 
 // a neighbour after its p_wait_nb, and of every block after p_finish.  A
 // wait that gives up sets the error flag; p_finish then reports failure.
 
-__device__ __forceinline__ void p_publish(double *sync, unsigned e, const double (&mine)[4]) {
+template <int NV>
+__device__ __forceinline__ void p_publish(double *sync, unsigned e, const double (&mine)[NV]) {
+  static_assert(NV <= 8, "the partials and the flag share a 128-byte line");
   co_drain();
   __syncthreads();
   if (threadIdx.x == 0) {
     double *ln = sync + (long)PLINE * blockIdx.x;
 #pragma unroll
-    for (int k = 0; k < 4; ++k) co_st(ln + k, mine[k]);
+    for (int k = 0; k < NV; ++k) co_st(ln + k, mine[k]);
     co_drain();
     co_st(line_flag(ln), e);
   }
 }
 
+template <int NV>
 __device__ __forceinline__ void p_lead(double *sync, unsigned e, long spin) {
   const int t = threadIdx.x, b = blockIdx.x;
   if (b >= 8 || t >= 64) return;
@@ -398,20 +401,22 @@ __device__ __forceinline__ void p_lead(double *sync, unsigned e, long spin) {
     }
   }
   asm volatile("" ::: "memory");
-  double v[4] = {0.0, 0.0, 0.0, 0.0};
+  double v[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) v[k] = 0.0;
   if (t < per) {
 #pragma unroll
-    for (int k = 0; k < 4; ++k) v[k] = co_ld(sync + (long)PLINE * (t * 8 + b) + k);
+    for (int k = 0; k < NV; ++k) v[k] = co_ld(sync + (long)PLINE * (t * 8 + b) + k);
   }
 #pragma unroll
-  for (int k = 0; k < 4; ++k)
+  for (int k = 0; k < NV; ++k)
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_xor(v[k], off, 64);
   if (t == 0) {
     if (!ok) co_st(err, 1);
     double *xl = sync + PSYNC_XCD + (long)PLINE * b;
 #pragma unroll
-    for (int k = 0; k < 4; ++k) co_st(xl + k, v[k]);
+    for (int k = 0; k < NV; ++k) co_st(xl + k, v[k]);
     co_drain();
     co_st(line_flag(xl), e);
   }
@@ -442,8 +447,9 @@ __device__ __forceinline__ void p_wait_nb(double *sync, unsigned e, int nbr, lon
   __syncthreads();
 }
 
-__device__ __forceinline__ bool p_finish(double *sync, unsigned e, double (&red)[4], long spin) {
-  __shared__ double sred[4];
+template <int NV>
+__device__ __forceinline__ bool p_finish(double *sync, unsigned e, double (&red)[NV], long spin) {
+  __shared__ double sred[NV];
   __shared__ int sbad;
   const int t = threadIdx.x;
   int *err = (int *)(sync + PSYNC_CTL) + 1;
@@ -460,50 +466,52 @@ __device__ __forceinline__ bool p_finish(double *sync, unsigned e, double (&red)
       }
     }
     asm volatile("" ::: "memory");
-    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    double v[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) v[k] = 0.0;
     if (t < 8) {
 #pragma unroll
-      for (int k = 0; k < 4; ++k) v[k] = co_ld(sync + PSYNC_XCD + (long)PLINE * t + k);
+      for (int k = 0; k < NV; ++k) v[k] = co_ld(sync + PSYNC_XCD + (long)PLINE * t + k);
     }
     const int flagged = t == 0 ? co_ld(err) : 0;
 #pragma unroll
-    for (int k = 0; k < 4; ++k)
+    for (int k = 0; k < NV; ++k)
 #pragma unroll
       for (int off = 4; off > 0; off >>= 1) v[k] += __shfl_xor(v[k], off, 64);
     if (t == 0) {
       if (!ok) co_st(err, 1);
 #pragma unroll
-      for (int k = 0; k < 4; ++k) sred[k] = v[k];
+      for (int k = 0; k < NV; ++k) sred[k] = v[k];
       sbad = (!ok || flagged) ? 1 : 0;
     }
   }
   __syncthreads();
 #pragma unroll
-  for (int k = 0; k < 4; ++k) red[k] = sred[k];
+  for (int k = 0; k < NV; ++k) red[k] = sred[k];
   return sbad == 0;
 }
 
 
-// block-wide sums of four values in a fixed order; the result in thread 0
-template <int NT>
-__device__ __forceinline__ void p_block_sum(double (&v)[4]) {
-  __shared__ double sm[(NT / 64) * 4];
+// block-wide sums of NV values in a fixed order; the result in thread 0
+template <int NT, int NV>
+__device__ __forceinline__ void p_block_sum(double (&v)[NV]) {
+  __shared__ double sm[(NT / 64) * NV];
 #pragma unroll
-  for (int k = 0; k < 4; ++k)
+  for (int k = 0; k < NV; ++k)
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_xor(v[k], off, 64);
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   __syncthreads();
   if (lane == 0) {
 #pragma unroll
-    for (int k = 0; k < 4; ++k) sm[w * 4 + k] = v[k];
+    for (int k = 0; k < NV; ++k) sm[w * NV + k] = v[k];
   }
   __syncthreads();
   if (threadIdx.x == 0) {
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
+    for (int k = 0; k < NV; ++k) {
       double a = sm[k];
-      for (int ww = 1; ww < NT / 64; ++ww) a += sm[ww * 4 + k];
+      for (int ww = 1; ww < NT / 64; ++ww) a += sm[ww * NV + k];
       v[k] = a;
     }
   }
@@ -536,9 +544,12 @@ __device__ __forceinline__ void p_fail(Scal *sc, int status) {
   }
 }
 
+// The reduction after v = A p carries one value, (rs, v); the one after
+// t = A s four, (t,t), (t,s), (rs,t), (rs,s); the setup's one, (r, r).
 template <typename TV, int KC>
 __global__ void __launch_bounds__(PNT) __attribute__((amdgpu_waves_per_eu(1, 1))) k_bicg_persist(PArgs<TV> a) {
 #pragma clang fp contract(off)
+  constexpr int NV = 4;   // values of the reduction after t = A s
   extern __shared__ unsigned char p_lds[];
   TV *W = (TV *)p_lds;
   if (a.sc->done) return;
@@ -573,7 +584,7 @@ __global__ void __launch_bounds__(PNT) __attribute__((amdgpu_waves_per_eu(1, 1))
   int oc[KC], oe[KC];
   bool surf[KC];
   TV A[KC][7], x[KC], r[KC], p[KC], v[KC], rs[KC], s[KC], tt[KC];
-  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  double acc[1] = {0.0};
 #pragma unroll
   for (int k = 0; k < KC; ++k) {
     const PCell o = own_cell(P, bv, t + k * PNT);
@@ -608,35 +619,38 @@ __global__ void __launch_bounds__(PNT) __attribute__((amdgpu_waves_per_eu(1, 1))
     acc[0] += (double)r[k] * (double)r[k];
     if (surf[k]) co_st(a.S + c, r[k]);
   }
-  // halo values loaded while a reduction completes: phase A's s, t, p, v
-  // (the first iteration: r in S), phase B's v
+  // halo values loaded while a synchronisation completes: phase A's s, t, p,
+  // v (the first iteration: r in S), phase B's v
   TV hs[PKH], ht[PKH], hp[PKH], hv[PKH], rh[PKH];
-  p_block_sum<PNT>(acc);
+#pragma unroll
+  for (int k = 0; k < PKH; ++k) hs[k] = ht[k] = hp[k] = hv[k] = (TV)0;
+  p_block_sum<PNT, 1>(acc);
   p_publish(a.sync, ++e, acc);
-  p_lead(a.sync, e, spin);
+  p_lead<1>(a.sync, e, spin);
   p_wait_nb(a.sync, e, nbr, spin);
 #pragma unroll
-  for (int k = 0; k < PKH; ++k) {
-    hs[k] = ht[k] = hp[k] = hv[k] = (TV)0;
+  for (int k = 0; k < PKH; ++k)
     if (hc[k] >= 0) hs[k] = co_ld(a.S + hc[k]);
-  }
-  double red[4];
-  if (!p_finish(a.sync, e, red, spin)) {
+  double red0[1];
+  if (!p_finish(a.sync, e, red0, spin)) {
     p_fail(a.sc, a.status_timeout);
     return;
   }
-  double rho = red[0], alpha = 0.0, omega = 0.0;
+  // alpha of the iteration and of the one before; red: the last reduction
+  // after t = A s
+  double rho = red0[0], alpha = 0.0, alpha_prev = 0.0, omega = 0.0, beta = 0.0;
+  double red[NV];
   for (int it = 1; it <= a.nin; ++it) {
     const int cur = it & 1;
     TV *Pw = cur ? a.P1 : a.P0, *Vw = cur ? a.V1 : a.V0;
-    double beta = 0.0;
     if (it > 1) {
       omega = red[1] / red[0];
       const double rn = red[3] - omega * red[2];
       beta = (rn / rho) * (alpha / omega);
       rho = rn;
+      alpha_prev = alpha;
     }
-    const TV ta = (TV)alpha, tw = (TV)omega, tb = (TV)beta;
+    const TV ta = (TV)alpha_prev, tw = (TV)omega, tb = (TV)beta;
     // ---- phase A: p on the tile (halo nodes from their stored s, t, p, v)
 #pragma unroll
     for (int k = 0; k < PKH; ++k) {
@@ -668,7 +682,7 @@ __global__ void __launch_bounds__(PNT) __attribute__((amdgpu_waves_per_eu(1, 1))
       if (surf[k]) co_st(Pw + oc[k], p[k]);
     }
     __syncthreads();
-    double acca[4] = {0.0, 0.0, 0.0, 0.0};
+    double acca[1] = {0.0};
 #pragma unroll
     for (int k = 0; k < KC; ++k) {
       if (oc[k] < 0) continue;
@@ -681,18 +695,19 @@ __global__ void __launch_bounds__(PNT) __attribute__((amdgpu_waves_per_eu(1, 1))
       acca[0] += (double)rs[k] * (double)v[k];
       if (surf[k]) co_st(Vw + oc[k], v[k]);
     }
-    p_block_sum<PNT>(acca);
+    p_block_sum<PNT, 1>(acca);
     p_publish(a.sync, ++e, acca);
-    p_lead(a.sync, e, spin);
+    p_lead<1>(a.sync, e, spin);
     p_wait_nb(a.sync, e, nbr, spin);
 #pragma unroll
     for (int k = 0; k < PKH; ++k)
       if (hc[k] >= 0) hv[k] = co_ld(Vw + hc[k]);
-    if (!p_finish(a.sync, e, red, spin)) {
+    double ra[1];
+    if (!p_finish(a.sync, e, ra, spin)) {
       p_fail(a.sc, a.status_timeout);
       return;
     }
-    alpha = rho / red[0];
+    alpha = rho / ra[0];
     const TV ta2 = (TV)alpha;
     // ---- phase B: s on the tile (halo nodes from their stored v)
 #pragma unroll
@@ -708,7 +723,9 @@ __global__ void __launch_bounds__(PNT) __attribute__((amdgpu_waves_per_eu(1, 1))
       if (surf[k]) co_st(a.S + oc[k], s[k]);
     }
     __syncthreads();
-    double accb[4] = {0.0, 0.0, 0.0, 0.0};
+    double accb[NV];
+#pragma unroll
+    for (int q = 0; q < NV; ++q) accb[q] = 0.0;
 #pragma unroll
     for (int k = 0; k < KC; ++k) {
       if (oc[k] < 0) continue;
@@ -725,9 +742,9 @@ __global__ void __launch_bounds__(PNT) __attribute__((amdgpu_waves_per_eu(1, 1))
       accb[3] += rsd * (double)s[k];
       if (surf[k]) co_st(a.T + oc[k], tt[k]);
     }
-    p_block_sum<PNT>(accb);
+    p_block_sum<PNT, NV>(accb);
     p_publish(a.sync, ++e, accb);
-    p_lead(a.sync, e, spin);
+    p_lead<NV>(a.sync, e, spin);
     if (it < a.nin) {
       // the next phase A's halo: s, t of this phase and p, v of phase A
       p_wait_nb(a.sync, e, nbr, spin);

@@ -690,6 +690,7 @@ def main():
                                                         "stream, in a profiled solve after the timed ones",
                 "group_solve_avg_us": round(r64["gsolve_avg_ms"] * 1e3, 2),
                 "nodal_update_avg_us": round(r64["nodal_avg_ms"] * 1e3, 2)}
+    if roof is not None:
         roof["xs_update"] = xs_dev
         if r64["nodal_avg_ms"]:
             nf = nodal_flops(p)

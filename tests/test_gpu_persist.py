@@ -36,10 +36,16 @@ CORES = [("core2d_small", 1), ("core3d_2g", 1), ("core3d_8g", 2)]
 @pytest.mark.parametrize("precision", ["fp64", "fp32_refine"])
 @pytest.mark.parametrize("name,seed", CORES)
 def test_gpu_persistent_bicg_matches_flat(name, seed, precision):
+    """FP64: within 1e-11 of the flat form (the same operations; only the
+    summation order of the reductions differs).  FP32: 20 iterations carry
+    any rounding difference to 1e-5-1e-4 between two FP32 runs, so the FP32
+    persistent form is held to the flat FP64 result: no further from it
+    than twice the flat FP32 form's distance, plus 1e-6."""
     p = generate(name, seed).problem
     print(f"persist vs flat bicg: {name} {precision}", flush=True)
     ep = engine(p, precision)
     ef = engine(p, precision, ADG_PERSIST=0)
+    e64 = engine(p, "fp64", ADG_PERSIST=0) if precision != "fp64" else None
     info = ep.solver_info()
     assert info["persistent"] and not ef.solver_info()["persistent"], info
     rng = np.random.default_rng(3)
@@ -49,11 +55,18 @@ def test_gpu_persistent_bicg_matches_flat(name, seed, precision):
         for nin in (1, 2, 20):
             xp = ep.bicg(g, nin, b, x0)
             xf = ef.bicg(g, nin, b, x0)
-            tol = 1e-11 if precision == "fp64" else 2e-5
-            err = np.max(np.abs(xp - xf)) / np.max(np.abs(xf))
-            assert err <= tol, (g, nin, err)
-    ep.close()
-    ef.close()
+            if e64 is None:
+                err = np.max(np.abs(xp - xf)) / np.max(np.abs(xf))
+                assert err <= 1e-11, (g, nin, err)
+            else:
+                x64 = e64.bicg(g, nin, b, x0)
+                scale = np.max(np.abs(x64))
+                ep_err = np.max(np.abs(xp - x64)) / scale
+                ef_err = np.max(np.abs(xf - x64)) / scale
+                assert ep_err <= 2.0 * ef_err + 1e-6, (g, nin, ep_err, ef_err)
+    for e in (ep, ef, e64):
+        if e is not None:
+            e.close()
 
 
 @pytest.mark.parametrize("name", probe_fixtures())
