@@ -17,7 +17,11 @@ coupling coefficients directly (``coup_coef``, src/mod_cmfd.f90:11-137).
 static solve (``cmfd::outer`` with the nodal D-hat updates) and PowDis;
 backward runs the hand-written adjoint kernels of the engine
 (``adg_adjoint`` / ``adg_backward``), i.e. the gradient of the converged
-CMFD eigenproblem with D-hat held at its converged value.
+CMFD eigenproblem with D-hat held at its converged value.  That is not the
+gradient of the full SANM model, whose D-hats move with the cross sections
+through the nodal update: against central differences of the full solve on
+the 34x34x26 two-group core the two differ by up to 6% of the largest
+gradient entry (4-31% on individual significant entries).
 """
 
 from __future__ import annotations

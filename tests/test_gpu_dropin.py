@@ -180,18 +180,12 @@ def test_dropin_output_files_match_reference(deck, tmp_path):
     assert not bad, bad[:10]
 
 
-LONG = pytest.mark.skipif(not os.environ.get("ADG_LONG_TESTS"),
-                          reason="about 130 s, bound by the host's branch-table update and T-H per step; "
-                                 "run with ADG_LONG_TESTS=1")
-
-
 NEACRP_SHIPPED = ("A1t", "B1t", "C1t", "A2t", "B2t", "C2t")
 
 
 @pytest.mark.parametrize("deck", ["LMW_tight", "LMW", "A1t_tight", "A1t", "B1t_tight", "C1t_tight", "A2t_tight",
                                   "B2t_tight", "C2t_tight", "B1t", "C1t", "A2t", "B2t", "C2t",
-                                  pytest.param("part4_helios_tight", marks=LONG),
-                                  pytest.param("part4_serpent_tight", marks=LONG)])
+                                  "part4_helios_tight", "part4_serpent_tight"])
 def test_dropin_transient(deck, tmp_path):
     """%MODE RODEJECT (the reference's LMW rod-movement transient, 240
     Crank-Nicolson time steps): the host's rod_eject / trans_calc
@@ -204,8 +198,9 @@ def test_dropin_transient(deck, tmp_path):
     feedback (rod_eject_th, src/mod_trans.f90:172-330; A1t: 280 steps, the
     power rising from 1e-6 to 0.2 through a Doppler-limited excursion); A2t,
     B2t and C2t the same at hot full power; part4_helios and part4_serpent the
-    MOX benchmark's rod ejection with branch tables (%XTAB: the host's XStab_updt every
-    step, the transient source with per-material kinetics data).
+    MOX benchmark's rod ejection with branch tables (%XTAB: XStab_updt, made
+    on the device, every step; the transient source with per-material
+    kinetics data).
 
     The *_tight decks have their %ITER tolerances at 1e-8: the printed
     table follows the reference's to one unit of the last printed digit of
