@@ -164,6 +164,7 @@ struct adg_ctx {
   float *rr32[2] = {nullptr, nullptr}, *pp32[2] = {nullptr, nullptr}, *vv32[2] = {nullptr, nullptr};
   // nodal
   double *S = nullptr, *a2 = nullptr, *a4 = nullptr, *L1 = nullptr, *npart = nullptr;
+  double *Bd = nullptr, *cf4 = nullptr;   // cooperative nodal passes: get_B node terms, A F G H per direction
   // reductions / scalars
   double *part = nullptr;          // three rotating partial buffers of 4 x nflat
   Scal *sc = nullptr;
@@ -757,6 +758,8 @@ void nodal_update(adg_ctx *c, int kern, int cmode, int sched) {
   a.kern = kern; a.cmode = cmode; a.sched = sched;
   a.part = c->npart;
   a.coop = 0;
+  a.Bd = nod_coop(c) ? c->Bd : nullptr;
+  a.cf4 = c->cf4;
   NodArgs an = a, af = a, ab = a;
   an.lv = c->lv(c->tcell);
   af.lv = c->lv(c->tface);
@@ -774,8 +777,8 @@ void nodal_update(adg_ctx *c, int kern, int cmode, int sched) {
     if (coop) {                                                                                          \
       hipLaunchKernelGGL(k_nod_pass1c<GG>, g1, dim3(NT), 0, c->s, an);                                   \
       hipLaunchKernelGGL(k_nod_pass2c<GG>, g2, dim3(NT), 0, c->s, af);                                   \
-      if (c->nbface) hipLaunchKernelGGL(k_nod_pass2b<GG>, dim3(c->nbface), dim3(NT), 0, c->s, ab, c->bfl, \
-                                        c->nface);                                                       \
+      if (c->nbface)                                                                                     \
+        hipLaunchKernelGGL(k_nod_pass2b<GG>, dim3(c->nbface), dim3(NT), 0, c->s, ab, c->bfl, c->nface);  \
     } else {                                                                                             \
       hipLaunchKernelGGL(k_nod_pass1<GG>, g1, dim3(NT), 0, c->s, an);                                    \
       hipLaunchKernelGGL(k_nod_pass2<GG>, g2, dim3(NT), 0, c->s, af);                                    \
@@ -1275,6 +1278,10 @@ int create_batch(const adg_geom *geoms, int ncore, adg_ctx **out) {
     c->S = dalloc<double>(c->s, 3 * G * NC); c->a2 = dalloc<double>(c->s, 3 * G * NC); c->a4 = dalloc<double>(c->s, 3 * G * NC);
     c->L1 = dalloc<double>(c->s, 3 * G * NC);
     build_tables(c, fls);
+    if (nod_coop(c)) {
+      c->Bd = dalloc<double>(c->s, G * G * NC);
+      c->cf4 = dalloc<double>(c->s, 12 * G * NC);
+    }
     c->npart = dalloc<double>(c->s, (size_t)3 * (3 * c->nface + c->nbface));
     c->sc = dalloc<Scal>(c->s, ncore);
     HIPCHK(hipHostMalloc((void **)&c->hsc, ncore * sizeof(Scal), hipHostMallocDefault));
@@ -1427,7 +1434,7 @@ void adg_destroy(adg_ctx *c) {
                   c->sigf, c->chi, c->sigs, c->dc, c->exsrc, c->vdel, c->f0, c->f0c, c->fs0, c->fs0c, c->errn,
                   c->dn, c->df, c->A64, c->b, c->pw, c->tmp, c->rr64[0], c->rr64[1], c->rs64, c->t64, c->pp64[0],
                   c->pp64[1], c->vv64[0], c->vv64[1], c->A32, c->rr32[0], c->rr32[1], c->rs32, c->t32, c->d32,
-                  c->pp32[0], c->pp32[1], c->vv32[0], c->vv32[1], c->S, c->a2, c->a4, c->L1, c->npart, c->part,
+                  c->pp32[0], c->pp32[1], c->vv32[0], c->vv32[1], c->S, c->a2, c->a4, c->L1, c->Bd, c->cf4, c->npart, c->part,
                   c->sc, c->psi, c->phis, c->fs0s, c->exs, c->gam, c->dJdP, c->grad, c->wtr, c->d_cell,
                   c->dstage, c->psync};
   for (void *p : ptrs)

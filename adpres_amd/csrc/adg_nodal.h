@@ -32,7 +32,14 @@ struct NodArgs {
   int sched;            // 1: run only in a nodal-update outer (device-side schedule)
   int coop;             // pass 2 leaves the interior faces to k_nod_pass2c
   double *part;
+  // cooperative passes: the node terms of get_B's rows (S1 of the reference's
+  // dum, [g][h][NC], made once per update by k_nod_src) and each node's A, F,
+  // G, H per direction ([u][g][4][NC], made by k_nod_pass1c), read by the
+  // interface pass instead of being made again for every face
+  double *Bd, *cf4;
 };
+
+__device__ __forceinline__ double nod_dum(const NodArgs &a, int G, long NC, long c, double Ke, int g, int hh);
 
 __device__ __forceinline__ long cell_of(const Geo &q, int i, int j, int k) {
   return q.base + ((long)(k + 1) * q.NYP + (j + 1)) * q.NXP + (i + 1);
@@ -109,6 +116,10 @@ __global__ void __launch_bounds__(NT) k_nod_src(NodArgs a) {
     a.S[((size_t)1 * G + g) * NC + c] = L[0] + L[2];
     a.S[((size_t)2 * G + g) * NC + c] = L[0] + L[1];
   }
+  if (a.Bd) {   // row g of get_B's node terms, once per update for the cooperative passes
+    const double Ke = v.sc->Ke;
+    for (int hh = 0; hh < G; ++hh) a.Bd[((size_t)g * G + hh) * NC + c] = nod_dum(a, G, NC, c, Ke, g, hh);
+  }
 }
 
 // Lxyz of every node and group into out[u][g][cell] (the leakages the
@@ -139,29 +150,46 @@ __global__ void __launch_bounds__(NT) k_leak(NodArgs a, double *out) {
 // per-node helpers
 // ---------------------------------------------------------------------------
 
-// get_B (src/mod_nodal.f90:1345-1405), row g: B(g, 0..G-1)
+// get_B (src/mod_nodal.f90:1345-1405): the node term dum of entry (g, hh)
+__device__ __forceinline__ double nod_dum(const NodArgs &a, int G, long NC, long c, double Ke, int g, int hh) {
+#pragma clang fp contract(off)
+  const XS &x = a.xs;
+  double dum;
+  if (a.cmode == 1) {
+    if (g == hh) dum = x.sigr[(size_t)g * NC + c] - x.chi[(size_t)g * NC + c] * x.nuf[(size_t)hh * NC + c] / Ke;
+    else dum = -x.sigs[((size_t)g * G + hh) * NC + c] - x.chi[(size_t)g * NC + c] * x.nuf[(size_t)hh * NC + c] / Ke;
+  } else if (a.cmode == 2) {
+    // transient (:1379-1391): fission weighted by 1 - beta + dfis, no 1/k;
+    // the diagonal term carries chi twice, as the reference has it
+    const double wc = x.wtr[c] * x.chi[(size_t)g * NC + c];
+    if (g == hh) dum = x.sigr[(size_t)g * NC + c] - wc * x.chi[(size_t)g * NC + c] * x.nuf[(size_t)hh * NC + c];
+    else dum = -x.sigs[((size_t)g * G + hh) * NC + c] - wc * x.nuf[(size_t)hh * NC + c];
+  } else {
+    if (g == hh) dum = x.sigr[(size_t)g * NC + c] - x.chi[(size_t)g * NC + c] * x.nuf[(size_t)hh * NC + c] / Ke;
+    else dum = -x.sigs[((size_t)hh * G + g) * NC + c] - x.chi[(size_t)hh * NC + c] * x.nuf[(size_t)g * NC + c] / Ke;
+  }
+  return dum;
+}
+
+// get_B, row g: B(g, 0..G-1) = h^2 / (4 D_g) dum(g, .)
 template <int G>
 __device__ __forceinline__ void nod_B_row(const NodArgs &a, long NC, long c, double h, double Ke, int g, double *B) {
 #pragma clang fp contract(off)
-  const XS &x = a.xs;
-  const double D = x.D[(size_t)g * NC + c];
+  const double D = a.xs.D[(size_t)g * NC + c];
   for (int hh = 0; hh < G; ++hh) {
-    double dum;
-    if (a.cmode == 1) {
-      if (g == hh) dum = x.sigr[(size_t)g * NC + c] - x.chi[(size_t)g * NC + c] * x.nuf[(size_t)hh * NC + c] / Ke;
-      else dum = -x.sigs[((size_t)g * G + hh) * NC + c] - x.chi[(size_t)g * NC + c] * x.nuf[(size_t)hh * NC + c] / Ke;
-    } else if (a.cmode == 2) {
-      // transient (:1379-1391): fission weighted by 1 - beta + dfis, no 1/k;
-      // the diagonal term carries chi twice, as the reference has it
-      const double wc = x.wtr[c] * x.chi[(size_t)g * NC + c];
-      if (g == hh) dum = x.sigr[(size_t)g * NC + c] - wc * x.chi[(size_t)g * NC + c] * x.nuf[(size_t)hh * NC + c];
-      else dum = -x.sigs[((size_t)g * G + hh) * NC + c] - wc * x.nuf[(size_t)hh * NC + c];
-    } else {
-      if (g == hh) dum = x.sigr[(size_t)g * NC + c] - x.chi[(size_t)g * NC + c] * x.nuf[(size_t)hh * NC + c] / Ke;
-      else dum = -x.sigs[((size_t)hh * G + g) * NC + c] - x.chi[(size_t)hh * NC + c] * x.nuf[(size_t)g * NC + c] / Ke;
-    }
+    const double dum = nod_dum(a, G, NC, c, Ke, g, hh);
     B[hh] = 0.25 * (h * h) / D * dum;
   }
+}
+
+// the same row from the dum terms k_nod_src stored (a.Bd)
+template <int G>
+__device__ __forceinline__ void nod_B_row_pre(const NodArgs &a, long NC, long c, double h, int g, double *B) {
+#pragma clang fp contract(off)
+  const double D = a.xs.D[(size_t)g * NC + c];
+  const double fac = 0.25 * (h * h) / D;
+#pragma unroll
+  for (int hh = 0; hh < G; ++hh) B[hh] = fac * a.Bd[((size_t)g * G + hh) * NC + c];
 }
 
 // get_B, all rows; B row-major [g*G+h]
@@ -397,8 +425,7 @@ __global__ void __launch_bounds__(NT) k_nod_pass1(NodArgs a) {
 // One face of direction u, face index t of that direction (faces of
 // direction u: (n_u + 1) x (other two extents), the face index fastest):
 // the new D-hat of the face and the running argmax (best, bkey, bloc).
-// BONLY: boundary faces only (the interface system is not compiled in).
-template <int G, bool BONLY>
+template <int G>
 __device__ __forceinline__ void face_update(const NodArgs &a, const Geo &q, Scal *sc, int u, long t, double &best,
                                             double &bkey, double &bloc) {
 #pragma clang fp contract(off)
@@ -418,7 +445,7 @@ __device__ __forceinline__ void face_update(const NodArgs &a, const Geo &q, Scal
     const long cn = cp - st;                // node below the face ("n")
     const bool hasp = (q.flag[cp] & F_ACTIVE) != 0;
     const bool hasn = (q.flag[cn] & F_ACTIVE) != 0;
-    if ((hasp || hasn) && !((BONLY || a.coop) && hasp && hasn)) {
+    if ((hasp || hasn) && !(a.coop && hasp && hasn)) {
       const double Ke = sc->Ke;
       const int sf = 2 * u;                 // upper face of n (x+, y+, z+), sf+1 lower face of p
       const unsigned long long ord = face_order(q, u, fi, o1, o2);
@@ -429,7 +456,6 @@ __device__ __forceinline__ void face_update(const NodArgs &a, const Geo &q, Scal
       Coef6 cfn[G], cfp[G];
       double Bn[G * G], Bp[G * G];
       if (hasn && hasp) {
-        if constexpr (!BONLY) {
         const double hn = hdir(q, u, u == 0 ? i - 1 : i, u == 1 ? j - 1 : j, u == 2 ? k - 1 : k);
         const double hp = hdir(q, u, i, j, k);
         nod_B<G>(a, NC, cn, hn, Ke, Bn);
@@ -495,7 +521,6 @@ __device__ __forceinline__ void face_update(const NodArgs &a, const Geo &q, Scal
           a.dn[op] = v;
           const double nder = fabs(v - ndpr);
           if (nder > best) { best = nder; bkey = (double)(ord * G + g); }
-        }
         }
       } else if (hasp) {
         // first node of a line (get_coefs_first, :375-407)
@@ -618,9 +643,9 @@ __device__ __forceinline__ void face_argmax(double best, double bkey, double blo
   }
 }
 
-// every face of the launch's face range (blockIdx.y = direction); with
-// coop set the interior faces are left to k_nod_pass2c and the boundary
-// faces to k_nod_pass2b, so this kernel runs only without coop
+// every face of the launch's face range (blockIdx.y = direction), one per
+// thread: the form for smaller group counts (the cooperative kernels serve
+// G >= 5)
 template <int G>
 __global__ void __launch_bounds__(NT) k_nod_pass2(NodArgs a) {
   const BlkView v = blk_view(a.lv);
@@ -634,26 +659,8 @@ __global__ void __launch_bounds__(NT) k_nod_pass2(NodArgs a) {
   const long nf = (long)(nu + 1) * e1 * e2;
   const long t = v.b.lo + threadIdx.x;
   double best = -1.0, bkey = 0.0, bloc = 0.0;
-  if (t < nf && !sc->done) face_update<G, false>(a, q, sc, u, t, best, bkey, bloc);
+  if (t < nf && !sc->done) face_update<G>(a, q, sc, u, t, best, bkey, bloc);
   face_argmax(best, bkey, bloc, a.part + ((size_t)blockIdx.y * gridDim.x + blockIdx.x) * 3);
-}
-
-// the boundary faces (one node active) from the list bfl of the core's
-// (direction << 40 | face index) entries, one per thread; the partials go
-// after the 3 x nfb slots of the face launches
-template <int G>
-__global__ void __launch_bounds__(NT) k_nod_pass2b(NodArgs a, const long *bfl, int nfb) {
-  const BlkView v = blk_view(a.lv);
-  const Geo &q = *v.q;
-  Scal *sc = v.sc;
-  if (a.sched && !nodal_outer(sc)) return;
-  const long t = v.b.lo + threadIdx.x;
-  double best = -1.0, bkey = 0.0, bloc = 0.0;
-  if (t < v.b.hi && !sc->done) {
-    const long e = bfl[t];
-    face_update<G, true>(a, q, sc, (int)(e >> 40), e & ((1L << 40) - 1), best, bkey, bloc);
-  }
-  face_argmax(best, bkey, bloc, a.part + ((size_t)3 * nfb + blockIdx.x) * 3);
 }
 
 // ---------------------------------------------------------------------------
@@ -676,39 +683,39 @@ __device__ __forceinline__ bool coop_lu_diag_small(const double (&R)[M], int r) 
   return r < M && fabs(dg) < (double)10e-5f;
 }
 
-template <int M, int S>
-__device__ __forceinline__ void coop_lu(double (&R)[M], double s, int r, int f, double (&xs)[M]) {
+template <int M, int S, typename T>
+__device__ __forceinline__ void coop_lu(T (&R)[M], T s, int r, int f, T (&xs)[M]) {
 #pragma clang fp contract(off)
   const bool rowlane = r < M;
 #pragma unroll
   for (int p = 0; p < M; ++p) {
     // pivot row p (its entries p..M-1, final after the steps before p)
-    double pr[M];
+    T pr[M];
 #pragma unroll
     for (int c2 = p; c2 < M; ++c2) pr[c2] = __shfl(R[c2], S * p + f, 64);
     if (r > p && rowlane) {
-      const double piv = R[p] / pr[p];
+      const T piv = R[p] / pr[p];
 #pragma unroll
       for (int c2 = p + 1; c2 < M; ++c2) R[c2] = R[c2] - piv * pr[c2];
       R[p] = piv;
     }
   }
   // forward substitution y(i) = b(i) - sum_k<i L(i,k) y(k), k ascending
-  double isum = 0.0, y = 0.0;
+  T isum = (T)0, y = (T)0;
 #pragma unroll
   for (int p = 0; p < M; ++p) {
     if (r == p) y = s - isum;
-    const double yp = __shfl(y, S * p + f, 64);
+    const T yp = __shfl(y, S * p + f, 64);
     if (r > p) isum = isum + R[p] * yp;
   }
   // back substitution x(i) = (y(i) - sum_k>i U(i,k) x(k)) / U(i,i), k ascending
 #pragma unroll
   for (int p = M - 1; p >= 0; --p) {
-    double xv;
+    T xv;
     if (p == M - 1) {
       xv = y / R[p];
     } else {
-      double bs = 0.0;
+      T bs = (T)0;
 #pragma unroll
       for (int c2 = p + 1; c2 < M; ++c2) bs = bs + R[c2] * xs[c2];
       xv = (y - bs) / R[p];
@@ -751,10 +758,17 @@ __global__ void __launch_bounds__(NT) k_nod_pass1c(NodArgs a) {
       if (r == 0) atomicMin(&sc->lu_key, lu_key(0, i, j, k));
       continue;
     }
-    const double h = L.hn, Ke = sc->Ke;
+    const double h = L.hn;
     double B[G], R[G];
-    nod_B_row<G>(a, NC, c, h, Ke, g, B);
+    nod_B_row_pre<G>(a, NC, c, h, g, B);
     const Coef6 cf = nod_coef(a.kern, a.xs.sigr[(size_t)g * NC + c], a.xs.D[(size_t)g * NC + c], h);
+    if (r < G) {   // A, F, G, H of this node and direction for the interface pass
+      double *cq = a.cf4 + ((size_t)(u * G + g) * 4) * NC + c;
+      cq[0] = cf.A;
+      cq[NC] = cf.F;
+      cq[2 * NC] = cf.G;
+      cq[3 * NC] = cf.H;
+    }
     const double *Su = a.S + (size_t)u * G * NC;
     const double D = a.xs.D[(size_t)g * NC + c];
     const double S = a.cmode == 2 ? 0.25 * (h * h) / D * Su[(size_t)g * NC + c]
@@ -784,6 +798,124 @@ __global__ void __launch_bounds__(NT) k_nod_pass1c(NodArgs a) {
       a.L1[o] = 0.25 * (h * h) / D * tl1(L, Su + (size_t)g * NC, c, bclo, bchi);
     }
   }
+}
+
+// ---------------------------------------------------------------------------
+// pass 2, boundary faces (one node active), cooperative form: the faces
+// listed in bfl ((direction << 40) | face index, per core, NT per block),
+// eight lanes per face, lane 8 r + f owning row r of face f's G x G system of
+// get_a1matvec_first (the active node above the face, :375-407) or
+// get_a1matvec_last (below it, :411-441), solved by coop_lu; then get_a3
+// and nodal_coup_upd's boundary forms (:330-368).  Same arithmetic, in the
+// same order, as the per-thread face_update.  The argmax partials go after
+// the 3 x nfb slots of the face launches.
+// ---------------------------------------------------------------------------
+template <int G>
+__global__ void __launch_bounds__(NT) k_nod_pass2b(NodArgs a, const long *bfl, int nfb) {
+#pragma clang fp contract(off)
+  static_assert(G <= 8, "eight lanes per face");
+  const BlkView v = blk_view(a.lv);
+  const Geo &q = *v.q;
+  Scal *sc = v.sc;
+  if (a.sched && !nodal_outer(sc)) return;   // uniform over the core's blocks
+  const long NC = q.NC;
+  const int lane = threadIdx.x & 63, r = lane >> 3, f = lane & 7;
+  const int slot32 = (int)(threadIdx.x >> 6) * 8 + f;
+  const int g = r < G ? r : 0;   // lanes past the last row repeat row 0's loads
+  double best = -1.0, bkey = 0.0, bloc = 0.0;
+  for (int it = 0; it < NT / 32; ++it) {
+    const long t = v.b.lo + (long)it * 32 + slot32;
+    if (!(t < v.b.hi) || sc->done) continue;   // uniform over the face's eight lanes
+    const long e = bfl[t];
+    const int u = (int)(e >> 40);
+    const long tf = e & ((1L << 40) - 1);
+    const int nu = u == 0 ? q.nxx : (u == 1 ? q.nyy : q.nzz);
+    const int e1 = u == 0 ? q.nyy : q.nxx;
+    const int fi = (int)(tf % (nu + 1));
+    const long rr = tf / (nu + 1);
+    const int o1 = (int)(rr % e1), o2 = (int)(rr / e1);
+    int i, j, k;
+    if (u == 0) { i = fi; j = o1; k = o2; }
+    else if (u == 1) { i = o1; j = fi; k = o2; }
+    else { i = o1; j = o2; k = fi; }
+    const long st = u == 0 ? 1 : (u == 1 ? q.sy : q.sz);
+    const long cp = cell_of(q, i, j, k);
+    const long cn = cp - st;
+    const bool hasp = (q.flag[cp] & F_ACTIVE) != 0;   // else the node below (n) is the active one
+    const int sf = 2 * u;
+    const unsigned long long ord = face_order(q, u, fi, o1, o2);
+    const size_t ub = (size_t)u * G * NC;
+    int li = i, lj = j, lk = k;
+    if (!hasp) { if (u == 0) li = i - 1; else if (u == 1) lj = j - 1; else lk = k - 1; }
+    // the active node, its size along u, its face on this side and its boundary code
+    const long cs = hasp ? cp : cn;
+    const double hs = hasp ? hdir(q, u, i, j, k) : hdir(q, u, li, lj, lk);
+    const int face = hasp ? sf + 1 : sf;
+    const int bc = hasp ? q.bc[2 * u + 1] : q.bc[2 * u];
+    double B[G];
+    nod_B_row_pre<G>(a, NC, cs, hs, g, B);
+    const double *cq = a.cf4 + ((size_t)(u * G + g) * 4) * NC + cs;
+    const double A = cq[0], F = cq[NC], Gc = cq[2 * NC], H = cq[3 * NC];
+    const double L1v = a.L1[ub + (size_t)g * NC + cs];
+    const double a2v = a.a2[ub + (size_t)g * NC + cs], a4v = a.a4[ub + (size_t)g * NC + cs];
+    const double P = 2.0 * a.xs.D[(size_t)g * NC + cs] / hs;
+    const double dc = a.xs.dc[((size_t)face * G + g) * NC + cs];
+    const double f0v = a.f0[(size_t)g * NC + cs];
+    double R[G], bv;
+    if (hasp) {   // get_a1matvec_first
+#pragma unroll
+      for (int hh = 0; hh < G; ++hh) {
+        const double Bv = B[hh];
+        if (bc == 2) R[hh] = (hh == g) ? P * (Bv * F + 1.0) : P * Bv * F;
+        else if (bc == 1)
+          R[hh] = (hh == g) ? -dc * (1.0 + A * Bv) - 2.0 * P * (A * Bv * H + 1.0) : -dc * A * Bv - 2.0 * P * A * Bv * H;
+        else R[hh] = (hh == g) ? dc * (1.0 + A * Bv) : dc * A * Bv;
+      }
+      if (bc == 2) bv = P * (3.0 * a2v + Gc * a4v - F * L1v);
+      else if (bc == 1) bv = 2.0 * P * (A * H * L1v - 3.0 * a2v - Gc * a4v) - dc * (a2v + a4v + f0v - A * L1v);
+      else bv = dc * (a2v + a4v + f0v - A * L1v);
+    } else {      // get_a1matvec_last
+#pragma unroll
+      for (int hh = 0; hh < G; ++hh) {
+        const double Bv = B[hh];
+        if (bc == 2) R[hh] = (hh == g) ? -P * (Bv * F + 1.0) : -P * Bv * F;
+        else if (bc == 1)
+          R[hh] = (hh == g) ? dc * (1.0 + A * Bv) + 2.0 * P * (A * Bv * H + 1.0) : dc * A * Bv + 2.0 * P * A * Bv * H;
+        else R[hh] = (hh == g) ? dc * (1.0 + A * Bv) : dc * A * Bv;
+      }
+      if (bc == 2) bv = P * (3.0 * a2v + Gc * a4v + F * L1v);
+      else if (bc == 1) bv = -2.0 * P * (A * H * L1v + 3.0 * a2v + Gc * a4v) - dc * (a2v + a4v + f0v + A * L1v);
+      else bv = -dc * (a2v + a4v + f0v + A * L1v);
+    }
+    if (coop_lu_diag_small<G>(R, r)) atomicMin(&sc->lu_key, lu_key(ord, li, lj, lk));
+    double xs[G];
+    coop_lu<G, 8>(R, bv, r, f, xs);
+    if (r < G) {
+      // get_a3 and the boundary forms of nodal_coup_upd for group g
+      double Bf = 0.0;
+#pragma unroll
+      for (int hh = 0; hh < G; ++hh) Bf = Bf + B[hh] * xs[hh];
+      const double a3 = A * (Bf + L1v);
+      const size_t o = ((size_t)face * G + g) * NC + cs;
+      const double ndpr = a.dn[o];
+      double dv;
+      if (hasp) {
+        const double jp = -2.0 * a.xs.D[(size_t)g * NC + cs] / hs * (xs[g] - 3.0 * a2v + H * a3 - Gc * a4v);
+        dv = -(jp / f0v + a.df[o]);
+      } else {
+        const double jp = -2.0 * a.xs.D[(size_t)g * NC + cs] / hs * (xs[g] + 3.0 * a2v + H * a3 + Gc * a4v);
+        dv = -(jp / f0v - a.df[o]);
+      }
+      a.dn[o] = dv;
+      const double nder = fabs(dv - ndpr), key = (double)(ord * G + g);
+      if (nder > best || (nder == best && key < bkey)) {
+        best = nder;
+        bkey = key;
+        bloc = (double)(((li + 1) << 20) | ((lj + 1) << 10) | (lk + 1));
+      }
+    }
+  }
+  face_argmax(best, bkey, bloc, a.part + ((size_t)3 * nfb + blockIdx.x) * 3);
 }
 
 // ---------------------------------------------------------------------------
@@ -837,7 +969,6 @@ __global__ void __launch_bounds__(NT) k_nod_pass2c(NodArgs a) {
     const long cp = cell_of(q, i, j, k);
     const long cn = cp - st;
     if (!((q.flag[cp] & F_ACTIVE) && (q.flag[cn] & F_ACTIVE))) continue;   // interior faces only
-    const double Ke = sc->Ke;
     const int sf = 2 * u;
     const unsigned long long ord = face_order(q, u, fi, o1, o2);
     const size_t ub = (size_t)u * G * NC;
@@ -853,8 +984,17 @@ __global__ void __launch_bounds__(NT) k_nod_pass2c(NodArgs a) {
     const long cs = nside ? cn : cp;
     const double hsd = nside ? hn : hp;
     double Bs[G];
-    nod_B_row<G>(a, NC, cs, hsd, Ke, g, Bs);
-    const Coef6 cfs = nod_coef(a.kern, a.xs.sigr[(size_t)g * NC + cs], a.xs.D[(size_t)g * NC + cs], hsd);
+    nod_B_row_pre<G>(a, NC, cs, hsd, g, Bs);
+    Coef6 cfs;   // A, F, G, H of this side's node as pass 1 made them
+    {
+      const double *cq = a.cf4 + ((size_t)(u * G + g) * 4) * NC + cs;
+      cfs.A = cq[0];
+      cfs.B = 0.0;
+      cfs.E = 0.0;
+      cfs.F = cq[NC];
+      cfs.G = cq[2 * NC];
+      cfs.H = cq[3 * NC];
+    }
     const double Ls = a.L1[ub + (size_t)g * NC + cs];
     const double a2s = a.a2[ub + (size_t)g * NC + cs], a4s = a.a4[ub + (size_t)g * NC + cs];
     const double Ps = 2.0 * a.xs.D[(size_t)g * NC + cs] / hsd;
@@ -918,11 +1058,14 @@ __global__ void __launch_bounds__(NT) k_nod_pass2c(NodArgs a) {
     if (coop_lu_diag_small<M2>(R, r)) atomicMin(&sc->lu_key, lu_key(ord, li, lj, lk));
     double xs[M2];
     coop_lu<M2, 4>(R, s, r, fw, xs);
-    // get_a3 and nodal_coup_upd for group g (lanes 0..G-1)
+    // get_a3 and nodal_coup_upd for group g (lanes 0..G-1); node n's B row
+    // read again rather than kept live through the elimination
     if (r < G) {
+      double Bn2[G];
+      nod_B_row_pre<G>(a, NC, cn, hn, g, Bn2);
       double Bf = 0.0;
 #pragma unroll
-      for (int hh = 0; hh < G; ++hh) Bf = Bf + Bn[hh] * xs[hh];
+      for (int hh = 0; hh < G; ++hh) Bf = Bf + Bn2[hh] * xs[hh];
       const double a3 = cfn.A * (Bf + Ln1);
       const double jp = -2.0 * a.xs.D[(size_t)g * NC + cn] / hn * (xs[g] + 3.0 * a2n + cfn.H * a3 + cfn.G * a4n);
       const size_t on = ((size_t)sf * G + g) * NC + cn, op = ((size_t)(sf + 1) * G + g) * NC + cp;

@@ -558,15 +558,17 @@ def nodal_probe(p):
     """Short program for hardware-counter runs of the nodal update: 20 FDM
     outers from the first-call state, then two SANM nodal updates (each a
     leakage pass, the node pass, the interface pass, the boundary-face pass
-    and the argmax) on that state, in FP64."""
+    and the argmax) on that state; in the FP64 form, then in the FP32+refine
+    form (whose nodal update is the same FP64 update)."""
     from adpres_amd.engine import Engine
-    e = Engine(p, "fp64")
-    e.outer(nout=20, kern="FDM", raise_on_error=False)
     out = {"probe": "nodal-counters", "nodes": p.nnod, "groups": p.ng}
-    for k in range(2):
-        nd, loc = e.nodal_update()
-        out[f"ndmax_{k}"] = float(nd)
-    e.close()
+    for prec in ("fp64", "fp32_refine"):
+        e = Engine(p, prec)
+        e.outer(nout=20, kern="FDM", raise_on_error=False)
+        for k in range(2):
+            nd, loc = e.nodal_update()
+            out[f"{prec}_ndmax_{k}"] = float(nd)
+        e.close()
     print(json.dumps(out))
 
 
