@@ -29,6 +29,7 @@
 #include "adg_cmfd.h"
 #include "adg_common.h"
 #include "adg_nodal.h"
+#include "adg_nodal_vjp.h"
 #include "adg_output.h"
 #include "adg_xs.h"
 
@@ -212,6 +213,14 @@ struct adg_ctx {
   bool transpose = false, have_psi = false;
   int variant = 0;                 // 0 eigenvalue outer, 1 generalized-adjoint outer (k fixed, projection)
   double psifphi = 0.0, kfwd = 0.0;
+  // transpose of the nodal update and the full-model adjoint: one block,
+  // the scratch of one transpose first (zeroed before each), then the
+  // adjoint iterate, its successor, the CMFD D-hat gradient and the
+  // response's adjoint source
+  double *vjp = nullptr;
+  size_t vjp_scratch = 0;
+  VjpArgs va{};
+  double *vw = nullptr, *vwn = nullptr, *vgdn = nullptr, *vsst = nullptr;
 
   long s_c0 = 0, s_end = 0, s_chunk = 0;   // single core: flat blocks computed in the kernels
   LV lv(const Blk *t) const {
@@ -1140,6 +1149,158 @@ double host_sum(adg_ctx *c, const double *part) {
   return s;
 }
 
+// sums of the nflat two-value partials
+void host_sum2(adg_ctx *c, const double *part, double (&out)[2]) {
+  std::vector<double> h(2 * (size_t)c->nflat);
+  HIPCHK(hipMemcpyAsync(h.data(), part, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->s));
+  HIPCHK(hipStreamSynchronize(c->s));
+  out[0] = out[1] = 0.0;
+  for (int b = 0; b < c->nflat; ++b) {
+    out[0] += h[2 * (size_t)b];
+    out[1] += h[2 * (size_t)b + 1];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// transpose of the nodal update and the full-model adjoint (single core)
+// ---------------------------------------------------------------------------
+
+void vjp_alloc(adg_ctx *c) {
+  if (c->vjp) return;
+  const size_t NC = c->NC, G = c->G, PX = vjp_px(c->G), P = nparam(c->G);
+  const size_t scratch = (24 * G + 6 * PX + 6 * G + 12 * G + G + PX + 12 * G + G + G + P + 1) * NC;
+  const size_t tail = (18 * G + G) * NC;
+  c->vjp = dalloc<double>(c->s, scratch + tail);
+  c->vjp_scratch = scratch;
+  double *p = c->vjp;
+  auto take = [&](size_t k) {
+    double *r = p;
+    p += k * NC;
+    return r;
+  };
+  VjpArgs &v = c->va;
+  v.a2b = take(6 * G); v.a4b = take(6 * G); v.l1b = take(6 * G); v.f0b2 = take(6 * G);
+  v.xb2 = take(6 * PX); v.dfb2 = take(6 * G);
+  v.sdir = take(3 * G); v.y1 = take(3 * G); v.y2 = take(3 * G); v.sb = take(3 * G);
+  v.f0b1 = take(G); v.xb1 = take(PX);
+  v.dnb = take(6 * G); v.dfb0 = take(6 * G); v.f0b0 = take(G);
+  v.phib = take(G); v.grad = take(P); v.kb = take(1);
+  c->vw = take(6 * G); c->vwn = take(6 * G); c->vgdn = take(6 * G); c->vsst = take(G);
+}
+
+// the forward static nodal passes' arguments at the context's state (per-thread forms)
+NodArgs nod_args(adg_ctx *c, int kern) {
+  NodArgs a;
+  memset(&a, 0, sizeof a);
+  a.lv = c->lv(c->tcell);
+  a.xs = c->xs();
+  a.f0 = c->f0; a.df = c->df; a.dn = c->dn;
+  a.S = c->S; a.a2 = c->a2; a.a4 = c->a4; a.L1 = c->L1;
+  a.kern = kern; a.cmode = 1;
+  a.part = c->npart;
+  return a;
+}
+
+// df and passes 0 and 1 of the nodal update at the current state (S, a2,
+// a4, L1), which the transpose reads
+void nodal_fwd(adg_ctx *c, int kern) {
+  set_scal_start(c, nullptr);
+  launch_coupcoef(c);
+  const NodArgs a = nod_args(c, kern);
+  switch (c->G) {
+#define FWD_CASE(GG)                                                                   \
+  case GG:                                                                             \
+    hipLaunchKernelGGL(k_nod_src, dim3(c->ncell, GG), dim3(NT), 0, c->s, a);           \
+    hipLaunchKernelGGL(k_nod_pass1<GG>, dim3(c->ncell, 3), dim3(NT), 0, c->s, a);      \
+    break;
+    FWD_CASE(1) FWD_CASE(2) FWD_CASE(3) FWD_CASE(4) FWD_CASE(5) FWD_CASE(6) FWD_CASE(7) FWD_CASE(8)
+#undef FWD_CASE
+  }
+  HIPCHK(hipGetLastError());
+}
+
+// transpose of one nodal update at the state nodal_fwd prepared, for the
+// cotangent w ([6][G][NC]); exsrc is the external source that update read
+// (the generalized-adjoint outers use the context's own array for their
+// sources); the flux, old D-hat and parameter cotangents are left in
+// c->va, the k cotangent is returned
+double nodal_vjp(adg_ctx *c, int kern, const double *w, const double *exsrc) {
+  VjpArgs &v = c->va;
+  HIPCHK(hipMemsetAsync(c->vjp, 0, c->vjp_scratch * sizeof(double), c->s));
+  v.na = nod_args(c, kern);
+  v.na.xs.exsrc = exsrc;
+  v.w = w;
+  VjpArgs vf = v;
+  vf.na.lv = c->lv(c->tface);
+  const LV lf = c->lv(c->tflat);
+  switch (c->G) {
+#define VJP_CASE(GG)                                                                               \
+  case GG:                                                                                         \
+    hipLaunchKernelGGL(k_vjp_face<GG>, dim3(c->nface, 3), dim3(NT), 0, c->s, vf);                   \
+    hipLaunchKernelGGL(k_vjp_node<GG>, dim3(c->ncell), dim3(NT), 0, c->s, v);                       \
+    hipLaunchKernelGGL(k_vjp_tl, dim3(c->ncell, GG), dim3(NT), 0, c->s, v);                         \
+    hipLaunchKernelGGL(k_vjp_src, dim3(c->ncell, GG), dim3(NT), 0, c->s, v);                        \
+    hipLaunchKernelGGL(k_vjp_final<GG>, dim3(c->nflat), dim3(NTF), 0, c->s, v, lf);                 \
+    break;
+    VJP_CASE(1) VJP_CASE(2) VJP_CASE(3) VJP_CASE(4) VJP_CASE(5) VJP_CASE(6) VJP_CASE(7) VJP_CASE(8)
+#undef VJP_CASE
+  }
+  hipLaunchKernelGGL(k_cell_sum, dim3(c->nflat), dim3(NTF), 0, c->s, lf, v.kb, c->P(0));
+  HIPCHK(hipGetLastError());
+  return host_sum(c, c->P(0));
+}
+
+// generalized-adjoint outer (k fixed, the psi component projected out) with
+// the source already in exsrc, started from gam (warm) or from zero; the
+// solution is left in gam and the forward state restored
+int gen_adjoint(adg_ctx *c, const adg_iter *it, bool warm) {
+  const size_t n = (size_t)c->G * c->NC;
+  save_forward(c);
+  if (warm) {
+    HIPCHK(hipMemcpyAsync(c->f0, c->gam, n * sizeof(double), hipMemcpyDeviceToDevice, c->s));
+    hipLaunchKernelGGL(k_fsrc_init, dim3(c->nflat), dim3(NTF), 0, c->s, c->lv(c->tflat), c->xs(), 1, c->f0, c->fs0,
+                       1);
+  } else {
+    HIPCHK(hipMemsetAsync(c->f0, 0, n * sizeof(double), c->s));
+    HIPCHK(hipMemsetAsync(c->fs0, 0, c->NC * sizeof(double), c->s));
+  }
+  c->transpose = true;
+  c->variant = 1;
+  adg_iter ig = *it;
+  ig.kern = ADG_KERN_FDM;
+  const int rc = run_outer(c, ADG_ADJOINT, &ig);
+  HIPCHK(hipMemcpyAsync(c->gam, c->f0, n * sizeof(double), hipMemcpyDeviceToDevice, c->s));
+  restore_forward(c);
+  return rc;
+}
+
+// dJ/dP of the normalised node powers for a response with gradient dJdp
+// (PowDis) into c->dJdP; resp = false when it vanishes
+int load_response(adg_ctx *c, const double *dJdp, bool &resp) {
+  resp = false;
+  if (!dJdp) return ADG_OK;
+  const LV lv = c->lv(c->tflat);
+  const XS x = c->xs();
+  const int N = c->core[0].N;
+  std::vector<double> st;
+  hipLaunchKernelGGL(k_powdis, dim3(c->nflat), dim3(NTF), 0, c->s, lv, x, c->f0, c->pw, c->P(0));
+  hipLaunchKernelGGL(k_scale, dim3(c->nflat), dim3(NTF), 0, c->s, lv, c->pw, c->P(0));
+  HIPCHK(hipMemcpyAsync(c->hsc, c->sc, sizeof(Scal), hipMemcpyDeviceToHost, c->s));
+  HIPCHK(hipStreamSynchronize(c->s));
+  const double T = c->hsc->tpow;
+  if (!(T > 0.0)) return fail(c, ADG_E_POWER_NONPOS, "total node power is zero or less");
+  std::vector<double> p(N), q(N);
+  gather(c, 0, c->pw, 1, p.data(), st);
+  double sp = 0.0;
+  for (int n = 0; n < N; ++n) sp += dJdp[n] * p[n];
+  for (int n = 0; n < N; ++n) {
+    q[n] = (dJdp[n] - sp) / T;
+    if (q[n] != 0.0) resp = true;
+  }
+  scatter(c, 0, q.data(), 1, c->dJdP, st);
+  return ADG_OK;
+}
+
 bool valid_geom(const adg_geom *g) {
   return g && g->nnod > 0 && g->ng > 0 && g->ng <= MAXG && g->nxx > 0 && g->nyy > 0 && g->nzz > 0 && g->ix &&
          g->iy && g->iz && g->xdel && g->ydel && g->zdel &&
@@ -1436,7 +1597,7 @@ void adg_destroy(adg_ctx *c) {
                   c->pp64[1], c->vv64[0], c->vv64[1], c->A32, c->rr32[0], c->rr32[1], c->rs32, c->t32, c->d32,
                   c->pp32[0], c->pp32[1], c->vv32[0], c->vv32[1], c->S, c->a2, c->a4, c->L1, c->Bd, c->cf4, c->npart, c->part,
                   c->sc, c->psi, c->phis, c->fs0s, c->exs, c->gam, c->dJdP, c->grad, c->wtr, c->d_cell,
-                  c->dstage, c->psync};
+                  c->dstage, c->psync, c->vjp};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (c->hsc) (void)hipHostFree(c->hsc);
@@ -2291,28 +2452,13 @@ int adg_backward(adg_ctx *c, const adg_iter *it, double dJdk, const double *dJdp
   try {
     const LV lv = c->lv(c->tflat);
     const XS x = c->xs();
-    const int N = c->core[0].N, G = c->G;
+    const int G = c->G;
     std::vector<double> st;
     const double k = c->core[0].Ke_host;
     bool resp = false;
-    if (dJdp) {
-      // node powers at the forward state (PowDis) and dJ/dP of the unnormalised powers
-      hipLaunchKernelGGL(k_powdis, dim3(c->nflat), dim3(NTF), 0, c->s, lv, x, c->f0, c->pw, c->P(0));
-      hipLaunchKernelGGL(k_scale, dim3(c->nflat), dim3(NTF), 0, c->s, lv, c->pw, c->P(0));
-      HIPCHK(hipMemcpyAsync(c->hsc, c->sc, sizeof(Scal), hipMemcpyDeviceToHost, c->s));
-      HIPCHK(hipStreamSynchronize(c->s));
-      const double T = c->hsc->tpow;
-      if (!(T > 0.0)) return fail(c, ADG_E_POWER_NONPOS, "total node power is zero or less");
-      std::vector<double> p(N), q(N);
-      gather(c, 0, c->pw, 1, p.data(), st);
-      double sp = 0.0;
-      for (int n = 0; n < N; ++n) sp += dJdp[n] * p[n];
-      for (int n = 0; n < N; ++n) {
-        q[n] = (dJdp[n] - sp) / T;
-        if (q[n] != 0.0) resp = true;
-      }
-      scatter(c, 0, q.data(), 1, c->dJdP, st);
-    }
+    // node powers at the forward state (PowDis) and dJ/dP of the unnormalised powers
+    rc = load_response(c, dJdp, resp);
+    if (rc != ADG_OK) return rc;
     if (resp) {
       // generalized adjoint: S* as the external source of an adjoint outer with k fixed
       const size_t n = (size_t)G * c->NC;
@@ -2340,6 +2486,106 @@ int adg_backward(adg_ctx *c, const adg_iter *it, double dJdk, const double *dJdp
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(c->s));
     gather(c, 0, c->grad, P, grad, st);
+  } catch (const HipError &e) {
+    rc = hip_fail(c, e);
+  }
+  return rc;
+}
+
+int adg_nodal_vjp(adg_ctx *c, int kern, const double *w, double *phib, double *kb, double *dnb, double *grad) {
+  if (!c || !w) return ADG_E_ARG;
+  if (c->B != 1) return fail(c, ADG_E_ARG, "the differentiation entry points take a single-core context");
+  if (!c->all_xs()) return fail(c, ADG_E_NOT_SET, "adg_set_xs has not been called");
+  if (c->core[0].first) return fail(c, ADG_E_ARG, "no state: call adg_set_state or adg_outer first");
+  if (kern != ADG_KERN_PNM && kern != ADG_KERN_SANM) return fail(c, ADG_E_ARG, "nodal kernel must be PNM or SANM");
+  try {
+    vjp_alloc(c);
+    std::vector<double> st;
+    HIPCHK(hipMemsetAsync(c->vw, 0, (size_t)6 * c->G * c->NC * sizeof(double), c->s));
+    scatter(c, 0, w, 6 * c->G, c->vw, st);
+    nodal_fwd(c, kern);
+    const double k = nodal_vjp(c, kern, c->vw, c->exsrc);
+    if (kb) *kb = k;
+    if (phib) gather(c, 0, c->va.phib, c->G, phib, st);
+    if (dnb) gather(c, 0, c->va.dnb, 6 * c->G, dnb, st);
+    if (grad) gather(c, 0, c->va.grad, nparam(c->G), grad, st);
+  } catch (const HipError &e) {
+    return hip_fail(c, e);
+  }
+  return ADG_OK;
+}
+
+int adg_backward_nodal(adg_ctx *c, const adg_iter *it, int kern, double dJdk, const double *dJdp, double tol,
+                       int maxit, double *grad, int *nit, double *resid) {
+  if (!c || !it || !grad || !(tol > 0.0) || maxit < 1) return ADG_E_ARG;
+  if (c->B != 1) return fail(c, ADG_E_ARG, "the differentiation entry points take a single-core context");
+  if (!c->have_psi) return fail(c, ADG_E_NOT_SET, "adg_adjoint has not been called");
+  if (kern != ADG_KERN_PNM && kern != ADG_KERN_SANM) return fail(c, ADG_E_ARG, "nodal kernel must be PNM or SANM");
+  int rc = ADG_OK;
+  try {
+    vjp_alloc(c);
+    const LV lv = c->lv(c->tflat);
+    const XS x = c->xs();
+    const int G = c->G, P = nparam(c->G);
+    const size_t n = (size_t)G * c->NC, nd = 6 * n;
+    std::vector<double> st;
+    const double k = c->core[0].Ke_host;
+    bool resp = false;
+    rc = load_response(c, dJdp, resp);
+    if (rc != ADG_OK) return rc;
+    if (resp) hipLaunchKernelGGL(k_sstar, dim3(c->nflat), dim3(NTF), 0, c->s, lv, x, c->f0, c->dJdP, c->vsst);
+    else HIPCHK(hipMemsetAsync(c->vsst, 0, n * sizeof(double), c->s));
+    nodal_fwd(c, kern);
+    HIPCHK(hipMemcpyAsync(c->exs, c->exsrc, n * sizeof(double), hipMemcpyDeviceToDevice, c->s));
+    HIPCHK(hipMemsetAsync(c->vw, 0, nd * sizeof(double), c->s));
+    // w = dJ/d(D-hat) of the CMFD adjoint for the response J + w . N, where
+    // N is the nodal update and its transpose adds the flux and k
+    // cotangents to the adjoint's sources and the old-D-hat cotangent to w
+    double apsi = 0.0, res = 1.0;
+    bool have_gam = false, gam_on = false;
+    int iter;
+    for (iter = 1; iter <= maxit; ++iter) {
+      const bool nod = iter > 1;
+      const double kbN = nod ? nodal_vjp(c, kern, c->vw, c->exs) : 0.0;
+      gam_on = resp || nod;
+      if (gam_on) {
+        hipLaunchKernelGGL(k_add2, dim3(c->nflat), dim3(NTF), 0, c->s, lv, (const double *)c->vsst,
+                           nod ? (const double *)c->va.phib : (const double *)nullptr, G, c->exsrc);
+        rc = gen_adjoint(c, it, have_gam);
+        if (rc != ADG_OK) break;
+        have_gam = true;
+      }
+      apsi = (dJdk + kbN) * k * k / c->psifphi;
+      HIPCHK(hipMemsetAsync(c->vgdn, 0, nd * sizeof(double), c->s));
+      hipLaunchKernelGGL(k_grad_dn, dim3(c->nflat), dim3(NTF), 0, c->s, lv, (const double *)c->f0,
+                         (const double *)c->psi, apsi, gam_on ? (const double *)c->gam : (const double *)nullptr,
+                         c->vgdn);
+      hipLaunchKernelGGL(k_add2, dim3(c->nflat), dim3(NTF), 0, c->s, lv, (const double *)c->vgdn,
+                         nod ? (const double *)c->va.dnb : (const double *)nullptr, 6 * G, c->vwn);
+      hipLaunchKernelGGL(k_diff_norm, dim3(c->nflat), dim3(NTF), 0, c->s, lv, (const double *)c->vwn,
+                         (const double *)c->vw, 6 * G, c->P(0));
+      HIPCHK(hipGetLastError());
+      double s2[2];
+      host_sum2(c, c->P(0), s2);
+      res = s2[1] > 0.0 ? sqrt(s2[0] / s2[1]) : 0.0;
+      std::swap(c->vw, c->vwn);
+      if (nod && res <= tol) break;
+    }
+    HIPCHK(hipMemcpyAsync(c->exsrc, c->exs, n * sizeof(double), hipMemcpyDeviceToDevice, c->s));
+    if (nit) *nit = std::min(iter, maxit);
+    if (resid) *resid = res;
+    if (rc != ADG_OK) return rc;
+    HIPCHK(hipMemsetAsync(c->grad, 0, (size_t)P * c->NC * sizeof(double), c->s));
+    hipLaunchKernelGGL(k_grad, dim3(c->nflat), dim3(NTF), 0, c->s, lv, x, c->f0, c->psi, apsi,
+                       gam_on ? (const double *)c->gam : (const double *)nullptr,
+                       resp ? (const double *)c->dJdP : (const double *)nullptr, 1.0 / k, c->grad);
+    if (std::min(iter, maxit) > 1)
+      hipLaunchKernelGGL(k_add2, dim3(c->nflat), dim3(NTF), 0, c->s, lv, (const double *)c->grad,
+                         (const double *)c->va.grad, P, c->grad);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->s));
+    gather(c, 0, c->grad, P, grad, st);
+    if (res > tol) rc = fail(c, ADG_E_MAX_OUTER, "full-model adjoint iteration did not reach its tolerance");
   } catch (const HipError &e) {
     rc = hip_fail(c, e);
   }

@@ -13,15 +13,23 @@ nu-sigma_f (fixed kappa/nu), sigma_r = sigma_a + sum_h sigma_s(g->h) as
 ``Dsigr_updt`` forms it (src/mod_xsec.f90:199-226), and D enters the
 coupling coefficients directly (``coup_coef``, src/mod_cmfd.f90:11-137).
 
-:class:`KeffPower` is the torch.autograd.Function: forward runs the full
-static solve (``cmfd::outer`` with the nodal D-hat updates) and PowDis;
-backward runs the hand-written adjoint kernels of the engine
-(``adg_adjoint`` / ``adg_backward``), i.e. the gradient of the converged
-CMFD eigenproblem with D-hat held at its converged value.  That is not the
-gradient of the full SANM model, whose D-hats move with the cross sections
-through the nodal update: against central differences of the full solve on
-the 34x34x26 two-group core the two differ by up to 6% of the largest
-gradient entry (4-31% on individual significant entries).
+Two models are differentiated:
+
+* ``"full"`` (the default with a nodal kernel): the static solve itself,
+  ``cmfd::outer`` with the nodal D-hat updates (src/mod_nodal.f90:18-147),
+  converged to a fixed point of both the CMFD eigenproblem and the D-hat
+  update.  Its gradient (``adg_backward_nodal``) adds to the CMFD adjoint
+  the transpose of the nodal update: the D-hat cotangent w solves
+  w = dJ/d(D-hat) + A^T w, A the Jacobian of "D-hat -> CMFD eigenpair ->
+  nodal update", by fixed-point iteration, each step one transpose of the
+  nodal update (hand-written kernels) and one generalized-adjoint outer.
+* ``"frozen"``: the converged CMFD eigenproblem with D-hat held at its
+  converged value (``adg_backward``).  It differs from the full model's
+  gradient by the nodal closure's own sensitivity, up to 6% of the largest
+  entry on the 34x34x26 two-group core (4-31% on individual entries); with
+  the FDM kernel (no nodal update) the two models are the same.
+
+:class:`KeffPower` is the torch.autograd.Function around either.
 """
 
 from __future__ import annotations
@@ -43,15 +51,27 @@ def param_names(G: int):
 
 
 class CoreSensitivity:
-    """One core, its parameters and the adjoint-based gradients."""
+    """One core, its parameters and the adjoint-based gradients of the
+    ``model`` ("full" or "frozen"; default "full" unless the core's kernel
+    is FDM)."""
 
-    def __init__(self, p: Problem, precision: str = "fp64", adjoint_iter: Optional[dict] = None):
+    def __init__(self, p: Problem, precision: str = "fp64", adjoint_iter: Optional[dict] = None,
+                 model: Optional[str] = None, nodal_tol: float = 1e-9, nodal_maxit: int = 300):
         self.base = p
         self.G, self.N = p.ng, p.nnod
+        self.model = model or ("frozen" if p.kern == "FDM" else "full")
+        if self.model not in ("full", "frozen"):
+            raise ValueError(f"model must be 'full' or 'frozen', not {self.model!r}")
+        if self.model == "full" and p.kern == "FDM":
+            raise ValueError("the full model needs a nodal kernel (PNM or SANM)")
         self.engine = Engine(p, precision)
         self.ratio = np.where(p.nuf > 0, p.sigf / np.where(p.nuf > 0, p.nuf, 1.0), 0.0)
         self.adj_iter = dict(nout=20000, serc=1e-10, ferc=1e-10, nac=1000)
         self.adj_iter.update(adjoint_iter or {})
+        # the full model's forward solve: the outer iteration with nodal
+        # updates converged tightly, so that the state is a fixed point
+        self.fwd_iter = dict(nout=20000, serc=1e-10, ferc=1e-10)
+        self.nodal_tol, self.nodal_maxit = nodal_tol, nodal_maxit
         self._adjoint_ok = False
         self.k = None
         self.p = None
@@ -93,14 +113,18 @@ class CoreSensitivity:
     # -- forward ---------------------------------------------------------------
     def solve(self, x: np.ndarray, polish: bool = True):
         """Full static solve (nodal D-hat updates included) at x; returns (k,
-        powers).  With polish, the converged state is then iterated to
-        1e-11 with D-hat fixed, so that the adjoint and the gradients work on
-        an exact eigenpair of the model they differentiate."""
+        powers).  The full model converges it to 1e-10; the frozen model
+        (with polish) then iterates the converged state to 1e-11 with D-hat
+        fixed, so that the adjoint and the gradients work on an exact
+        eigenpair of the model they differentiate."""
         self.engine.set_xs(self.problem_at(x))
         self.engine.reset()
-        self.engine.outer()
-        if polish:
-            self.solve_frozen(x, self.engine.state())
+        if self.model == "full":
+            self.engine.outer(**self.fwd_iter)
+        else:
+            self.engine.outer()
+            if polish:
+                self.solve_frozen(x, self.engine.state())
         self.k, self.p = self.engine.Ke, self.engine.powdis()
         self._adjoint_ok = False
         return self.k, self.p
@@ -126,13 +150,31 @@ class CoreSensitivity:
         self._adjoint_ok = False
         return self.engine.Ke, self.engine.powdis()
 
+    def solve_full(self, x: np.ndarray, state: dict, outers: int):
+        """The full model's outer iteration (nodal updates on) at x from
+        state for exactly `outers` outers; returns (k, powers).  Two such
+        solves of nearby parameters carry the same iteration error, which
+        cancels in their difference."""
+        self.engine.set_xs(self.problem_at(x))
+        self.engine.set_state(state["f0"], state["fs0"], state["Ke"], state["dn"])
+        rc = self.engine.outer(serc=0.0, ferc=0.0, nout=outers, raise_on_error=False)
+        if rc not in (0, 1):   # ADG_OK or ADG_E_MAX_OUTER (the intended stop)
+            self.engine._check(rc, "adg_outer")
+        self._adjoint_ok = False
+        return self.engine.Ke, self.engine.powdis()
+
     # -- backward --------------------------------------------------------------
     def gradient(self, dJdk: float, dJdp: Optional[np.ndarray] = None) -> np.ndarray:
-        """(N, P) gradient of dJdk * k + <dJdp, powers> at the last solve."""
+        """(N, P) gradient of dJdk * k + <dJdp, powers> at the last solve,
+        of the sensitivity's model."""
         if not self._adjoint_ok:
             self.kadj = self.engine.adjoint(**self.adj_iter)
             self._adjoint_ok = True
-        return np.ascontiguousarray(self.engine.backward(dJdk, dJdp, **self.adj_iter).T)
+        if self.model == "full":
+            g = self.engine.backward_nodal(dJdk, dJdp, tol=self.nodal_tol, maxit=self.nodal_maxit, **self.adj_iter)
+        else:
+            g = self.engine.backward(dJdk, dJdp, **self.adj_iter)
+        return np.ascontiguousarray(g.T)
 
     def first_order_dk(self, kern: Optional[str] = None) -> np.ndarray:
         """(N, P) first-order perturbation estimate of dk with the physical
@@ -152,7 +194,7 @@ class CoreSensitivity:
 
 class KeffPower(torch.autograd.Function):
     """(N, P) float64 parameters -> (k-eff, node powers (N,)), with the
-    adjoint-kernel backward."""
+    adjoint-kernel backward of the sensitivity's model."""
 
     @staticmethod
     def forward(ctx, x, sens: CoreSensitivity):

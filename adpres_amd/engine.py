@@ -38,7 +38,7 @@ EXPORTS = ("adg_create", "adg_destroy", "adg_set_xs", "adg_set_state", "adg_get_
            "adg_set_profiling", "adg_read_profile", "adg_autotune",
            "adg_create_batch", "adg_core_count", "adg_set_xs_core", "adg_set_state_core", "adg_get_state_core",
            "adg_outer_batch", "adg_trace_core", "adg_powdis_core",
-           "adg_adjoint", "adg_get_adjoint", "adg_set_adjoint", "adg_backward",
+           "adg_adjoint", "adg_get_adjoint", "adg_set_adjoint", "adg_backward", "adg_nodal_vjp", "adg_backward_nodal",
            "adg_set_transient", "adg_set_transient_core", "adg_leakage",
            "adg_set_xs_tables", "adg_set_xs_tables_core", "adg_update_xs", "adg_update_xs_core",
            "adg_get_xs", "adg_get_xs_core", "adg_set_xtab_tables", "adg_set_xtab_tables_core",
@@ -144,6 +144,9 @@ def load_library(path: str = LIB_PATH):
     L.adg_get_adjoint.argtypes = [C.c_void_p, _dp, _dp]
     L.adg_set_adjoint.argtypes = [C.c_void_p, _dp]
     L.adg_backward.argtypes = [C.c_void_p, C.POINTER(AdgIter), C.c_double, _dp, _dp]
+    L.adg_nodal_vjp.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp, _dp, _dp]
+    L.adg_backward_nodal.argtypes = [C.c_void_p, C.POINTER(AdgIter), C.c_int, C.c_double, _dp, C.c_double, C.c_int,
+                                     _dp, _ip, _dp]
     L.adg_status_string.argtypes = [C.c_int]
     L.adg_status_string.restype = C.c_char_p
     L.adg_last_error.argtypes = [C.c_void_p]
@@ -462,6 +465,37 @@ class Engine:
         dp = None if dJdp is None else np.ascontiguousarray(dJdp, np.float64)
         self._check(self.lib.adg_backward(self.h, C.byref(it), float(dJdk), None if dp is None else _d(dp), _d(out)),
                     "adg_backward")
+        return out
+
+    def nodal_vjp(self, w, kern: Optional[str] = None):
+        """Transpose of one nodal D-hat update at the current state for the
+        cotangent w (6, G, N) of the new D-hat: (flux cotangent (G, N), k
+        cotangent, old-D-hat cotangent (6, G, N), parameter cotangents (P, N))."""
+        N, G = self.p.nnod, self.p.ng
+        P = 3 * G + G * (G - 1)
+        w = np.ascontiguousarray(w, np.float64)
+        phib = np.zeros((G, N)); dnb = np.zeros((6, G, N)); grad = np.zeros((P, N))
+        kb = C.c_double()
+        self._check(self.lib.adg_nodal_vjp(self.h, KERNS[kern or self.p.kern], _d(w), _d(phib), C.byref(kb), _d(dnb),
+                                           _d(grad)), "adg_nodal_vjp")
+        return phib, kb.value, dnb, grad
+
+    def backward_nodal(self, dJdk: float, dJdp=None, kern: Optional[str] = None, tol: float = 1e-9,
+                       maxit: int = 200, **ov):
+        """Per-node gradient (P, N) of dJdk * k + <dJdp, node powers> for the
+        full nodal model (D-hat moving with the cross sections through the
+        nodal update), with the fixed-point steps and the last relative change."""
+        N, G = self.p.nnod, self.p.ng
+        it = _iter(self.p, ov)
+        P = 3 * G + G * (G - 1)
+        out = np.zeros((P, N))
+        dp = None if dJdp is None else np.ascontiguousarray(dJdp, np.float64)
+        nit = C.c_int()
+        res = C.c_double()
+        self._check(self.lib.adg_backward_nodal(self.h, C.byref(it), KERNS[kern or self.p.kern], float(dJdk),
+                                                None if dp is None else _d(dp), float(tol), int(maxit), _d(out),
+                                                C.byref(nit), C.byref(res)), "adg_backward_nodal")
+        self.nodal_adjoint_steps, self.nodal_adjoint_resid = nit.value, res.value
         return out
 
     def last_timing_ms(self):

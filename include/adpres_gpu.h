@@ -347,6 +347,28 @@ int adg_get_adjoint(adg_ctx *ctx, double *psi, double *psifphi);
  * estimate of dk with it; the context must hold the forward flux. */
 int adg_set_adjoint(adg_ctx *ctx, const double *psi);
 int adg_backward(adg_ctx *ctx, const adg_iter *iter, double dJdk, const double *dJdp, double *grad);
+/* Transpose of one nodal D-hat update (nodal::nodal_update / _pnm,
+ * src/mod_nodal.f90:18-147 with 282-371 and 483-1451, forward static form)
+ * at the context's state (flux, k, D-hat as adg_set_state or adg_outer
+ * left them, cross sections): for a cotangent w(N, G, 6) of the new D-hat
+ * (laid out as adg_get_state's dn, one value per face side), the
+ * cotangents of the flux phib(N, G), of k (kb), of the old D-hat dnb(N, G,
+ * 6) and of the node parameters grad (N, 3G + G(G-1), adg_backward's
+ * columns; D through the coupling coefficients as well).  Any output may
+ * be NULL.  The state is left as it was. */
+int adg_nodal_vjp(adg_ctx *ctx, int kern, const double *w, double *phib, double *kb, double *dnb, double *grad);
+/* Gradient of the full nodal model (adg_backward's response and columns):
+ * for a context holding a converged solution of the outer iteration with
+ * nodal updates (a fixed point of both the CMFD eigenproblem and the D-hat
+ * update, kern as solved) and its adjoint (adg_adjoint), the D-hat
+ * cotangent w solves w = dJ/d(D-hat) + A^T w, A the Jacobian of "D-hat ->
+ * CMFD eigenpair -> nodal update", by fixed-point iteration: each step
+ * applies adg_nodal_vjp's transpose and one generalized-adjoint outer (from
+ * the previous one).  Stops when the relative 2-norm change of w is at most
+ * tol or after maxit steps (then ADG_E_MAX_OUTER, grad still filled);
+ * nit and resid (may be NULL) return the steps and the last change. */
+int adg_backward_nodal(adg_ctx *ctx, const adg_iter *iter, int kern, double dJdk, const double *dJdp, double tol,
+                       int maxit, double *grad, int *nit, double *resid);
 
 const char *adg_status_string(int status);
 const char *adg_last_error(adg_ctx *ctx);
