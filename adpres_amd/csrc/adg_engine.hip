@@ -187,6 +187,7 @@ struct adg_ctx {
   std::vector<int> pcand_kc;
   double *psync = nullptr;
   long pspin = PSPIN;              // polls before a wait of the persistent kernel gives up (ADG_PSPIN)
+  int adj_inner = 16;              // generalized-adjoint outers per full-model adjoint step (ADG_ADJ_INNER; 0: to convergence)
   // profiling: event pairs recorded on the context stream during adg_outer
   struct ProfRec {
     int type, e0, e1;   // type 0 KA kernel, 1 group solve, 2 nodal update
@@ -1250,9 +1251,10 @@ double nodal_vjp(adg_ctx *c, int kern, const double *w, const double *exsrc) {
   return host_sum(c, c->P(0));
 }
 
-// generalized-adjoint outer (k fixed, the psi component projected out) with
-// the source already in exsrc, started from gam (warm) or from zero; the
-// solution is left in gam and the forward state restored
+// generalized-adjoint outers (k fixed, the psi component projected out)
+// with the source already in exsrc, started from gam (warm) or from zero,
+// until its tolerance or its nout; the iterate is left in gam and the
+// forward state restored
 int gen_adjoint(adg_ctx *c, const adg_iter *it, bool warm) {
   const size_t n = (size_t)c->G * c->NC;
   save_forward(c);
@@ -1377,6 +1379,7 @@ int create_batch(const adg_geom *geoms, int ncore, adg_ctx **out) {
     if (const char *ev = getenv("ADG_SRC_FULL")) c->src_full = atoi(ev) != 0;
     if (const char *ev = getenv("ADG_XS_LDS")) c->xs_lds = atoi(ev) != 0;
     if (const char *ev = getenv("ADG_PSPIN")) c->pspin = std::max(1L, atol(ev));
+    if (const char *ev = getenv("ADG_ADJ_INNER")) c->adj_inner = std::max(0, atoi(ev));
     c->core.resize(ncore);
     std::vector<std::vector<unsigned char>> fls(ncore);
     std::vector<std::vector<double>> vds(ncore);
@@ -2540,9 +2543,15 @@ int adg_backward_nodal(adg_ctx *c, const adg_iter *it, int kern, double dJdk, co
     HIPCHK(hipMemsetAsync(c->vw, 0, nd * sizeof(double), c->s));
     // w = dJ/d(D-hat) of the CMFD adjoint for the response J + w . N, where
     // N is the nodal update and its transpose adds the flux and k
-    // cotangents to the adjoint's sources and the old-D-hat cotangent to w
+    // cotangents to the adjoint's sources and the old-D-hat cotangent to w.
+    // As the forward outer updates D-hat every few outers, each step runs
+    // adj_inner generalized-adjoint outers (warm) between updates of w, so
+    // that Gamma and w converge together; the iteration ends when w has
+    // settled and the last outers met the adjoint tolerance.
     double apsi = 0.0, res = 1.0;
-    bool have_gam = false, gam_on = false;
+    bool have_gam = false, gam_on = false, gam_conv = true;
+    adg_iter ia = *it;
+    if (c->adj_inner > 0) ia.nout = std::min(it->nout, c->adj_inner);
     int iter;
     for (iter = 1; iter <= maxit; ++iter) {
       const bool nod = iter > 1;
@@ -2551,7 +2560,9 @@ int adg_backward_nodal(adg_ctx *c, const adg_iter *it, int kern, double dJdk, co
       if (gam_on) {
         hipLaunchKernelGGL(k_add2, dim3(c->nflat), dim3(NTF), 0, c->s, lv, (const double *)c->vsst,
                            nod ? (const double *)c->va.phib : (const double *)nullptr, G, c->exsrc);
-        rc = gen_adjoint(c, it, have_gam);
+        rc = gen_adjoint(c, &ia, have_gam);
+        gam_conv = rc == ADG_OK;
+        if (rc == ADG_E_MAX_OUTER && ia.nout < it->nout) rc = ADG_OK;   // the step's share of outers
         if (rc != ADG_OK) break;
         have_gam = true;
       }
@@ -2569,7 +2580,7 @@ int adg_backward_nodal(adg_ctx *c, const adg_iter *it, int kern, double dJdk, co
       host_sum2(c, c->P(0), s2);
       res = s2[1] > 0.0 ? sqrt(s2[0] / s2[1]) : 0.0;
       std::swap(c->vw, c->vwn);
-      if (nod && res <= tol) break;
+      if (nod && res <= tol && gam_conv) break;
     }
     HIPCHK(hipMemcpyAsync(c->exsrc, c->exs, n * sizeof(double), hipMemcpyDeviceToDevice, c->s));
     if (nit) *nit = std::min(iter, maxit);
@@ -2585,7 +2596,7 @@ int adg_backward_nodal(adg_ctx *c, const adg_iter *it, int kern, double dJdk, co
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(c->s));
     gather(c, 0, c->grad, P, grad, st);
-    if (res > tol) rc = fail(c, ADG_E_MAX_OUTER, "full-model adjoint iteration did not reach its tolerance");
+    if (res > tol || !gam_conv) rc = fail(c, ADG_E_MAX_OUTER, "full-model adjoint iteration did not reach its tolerance");
   } catch (const HipError &e) {
     rc = hip_fail(c, e);
   }

@@ -22,7 +22,8 @@ Two models are differentiated:
   the transpose of the nodal update: the D-hat cotangent w solves
   w = dJ/d(D-hat) + A^T w, A the Jacobian of "D-hat -> CMFD eigenpair ->
   nodal update", by fixed-point iteration, each step one transpose of the
-  nodal update (hand-written kernels) and one generalized-adjoint outer.
+  nodal update (hand-written kernels) and a few outers of the generalized
+  adjoint, continued across the steps.
 * ``"frozen"``: the converged CMFD eigenproblem with D-hat held at its
   converged value (``adg_backward``).  It differs from the full model's
   gradient by the nodal closure's own sensitivity, up to 6% of the largest
@@ -56,7 +57,7 @@ class CoreSensitivity:
     is FDM)."""
 
     def __init__(self, p: Problem, precision: str = "fp64", adjoint_iter: Optional[dict] = None,
-                 model: Optional[str] = None, nodal_tol: float = 1e-9, nodal_maxit: int = 300):
+                 model: Optional[str] = None, nodal_tol: float = 1e-9, nodal_maxit: int = 5000):
         self.base = p
         self.G, self.N = p.ng, p.nnod
         self.model = model or ("frozen" if p.kern == "FDM" else "full")

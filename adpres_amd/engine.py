@@ -481,7 +481,7 @@ class Engine:
         return phib, kb.value, dnb, grad
 
     def backward_nodal(self, dJdk: float, dJdp=None, kern: Optional[str] = None, tol: float = 1e-9,
-                       maxit: int = 200, **ov):
+                       maxit: int = 5000, **ov):
         """Per-node gradient (P, N) of dJdk * k + <dJdp, node powers> for the
         full nodal model (D-hat moving with the cross sections through the
         nodal update), with the fixed-point steps and the last relative change."""

@@ -572,6 +572,57 @@ def nodal_probe(p):
     print(json.dumps(out))
 
 
+def grad_leg():
+    """Differentiation form on its workload, the 34x34x26-node 2-group core
+    (210k parameters): the full static solve converged to 1e-10, the
+    discrete adjoint, and the gradients of k and of the peak-power objective
+    (log-sum-exp, temperature 50) of the full nodal model, with the frozen
+    D-hat gradient and one transpose of the nodal update beside them."""
+    from adpres_amd.configs import generate
+    from adpres_amd.diff import CoreSensitivity
+    beta = 50.0
+    sens = CoreSensitivity(generate("core3d_2g", 1).problem, model="full")
+    e = sens.engine
+    x0 = sens.params()
+    t0 = time.perf_counter()
+    k0, p0 = sens.solve(x0)
+    t_solve = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    e.adjoint(**sens.adj_iter)
+    sens._adjoint_ok = True
+    t_adj = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    sens.gradient(1.0, None)
+    t_gk, steps_k = time.perf_counter() - t0, e.nodal_adjoint_steps
+    nf = int((p0 > 0).sum())
+    r = beta * p0 * nf
+    wgt = np.exp(r - r.max())
+    dJ = wgt / wgt.sum() * nf
+    t0 = time.perf_counter()
+    sens.gradient(0.0, dJ)
+    t_gJ, steps_J = time.perf_counter() - t0, e.nodal_adjoint_steps
+    t0 = time.perf_counter()
+    e.backward(1.0, None, **sens.adj_iter)
+    t_frz = time.perf_counter() - t0
+    w = np.ones((6, sens.G, sens.N))
+    e.nodal_vjp(w)
+    t0 = time.perf_counter()
+    for _ in range(10):
+        e.nodal_vjp(w)
+    t_vjp = (time.perf_counter() - t0) / 10
+    out = {"workload": "core3d_2g", "nodes": sens.N, "parameters": int(x0.size), "k": round(k0, 10),
+           "solve_s": round(t_solve, 3), "adjoint_s": round(t_adj, 3),
+           "grad_k_full_s": round(t_gk, 3), "fixed_point_steps_k": steps_k,
+           "grad_objective_full_s": round(t_gJ, 3), "fixed_point_steps_objective": steps_J,
+           "grad_k_frozen_ms": round(t_frz * 1e3, 3), "nodal_transpose_ms": round(t_vjp * 1e3, 3),
+           "note": "full: adg_backward_nodal (per fixed-point step one transpose of the nodal update and 16 "
+                   "generalized-adjoint outers continued across steps, tol 1e-9); frozen: adg_backward (D-hat "
+                   "held fixed, with the adjoint already solved); the transpose time includes the host copies of "
+                   "w and of the cotangents"}
+    sens.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -587,6 +638,9 @@ def main():
                     help="run only a copy of known size and one BiCGSTAB call per precision (for counter runs)")
     ap.add_argument("--nodal-probe", action="store_true",
                     help="run only two nodal updates of the workload core (for counter runs)")
+    ap.add_argument("--no-grad", action="store_true", help="skip the differentiation leg")
+    ap.add_argument("--grad-probe", action="store_true",
+                    help="run only the differentiation leg (for profilers)")
     args = ap.parse_args()
     if args.no_graph:
         os.environ["ADG_GRAPH"] = "0"
@@ -595,6 +649,9 @@ def main():
               f"--gpus {args.gpus} is not supported (multi-GPU sharding is a design note)", file=sys.stderr)
         sys.exit(2)
 
+    if args.grad_probe:
+        print(json.dumps(grad_leg()))
+        return
     if args.workload == "batch256":
         if args.kernel_probe:
             kernel_probe_batch()
@@ -706,6 +763,10 @@ def main():
     if not args.no_dropin:
         dropin = dropin_leg(args.workload, w["seed"])
         log(f"bench: drop-in {dropin}")
+    grad = None
+    if not args.no_grad:
+        grad = grad_leg()
+        log(f"bench: differentiation {grad}")
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(p, r64["niter"], p.nupd)
@@ -735,6 +796,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "dropin": dropin,
+        "differentiation": grad,
     }
     if r32 is not None:
         out["fp32_refine"] = {"solves_per_s": round(r32["solves_per_s"], 4),

@@ -363,10 +363,12 @@ int adg_nodal_vjp(adg_ctx *ctx, int kern, const double *w, double *phib, double 
  * update, kern as solved) and its adjoint (adg_adjoint), the D-hat
  * cotangent w solves w = dJ/d(D-hat) + A^T w, A the Jacobian of "D-hat ->
  * CMFD eigenpair -> nodal update", by fixed-point iteration: each step
- * applies adg_nodal_vjp's transpose and one generalized-adjoint outer (from
- * the previous one).  Stops when the relative 2-norm change of w is at most
- * tol or after maxit steps (then ADG_E_MAX_OUTER, grad still filled);
- * nit and resid (may be NULL) return the steps and the last change. */
+ * applies adg_nodal_vjp's transpose and a few outers of the generalized
+ * adjoint, continued from the previous step (16; environment ADG_ADJ_INNER,
+ * 0 = to iter's tolerance every step).  Stops when the relative 2-norm
+ * change of w is at most tol and the last outers met iter's tolerance, or
+ * after maxit steps (then ADG_E_MAX_OUTER, grad still filled); nit and
+ * resid (may be NULL) return the steps and the last change. */
 int adg_backward_nodal(adg_ctx *ctx, const adg_iter *iter, int kern, double dJdk, const double *dJdp, double tol,
                        int maxit, double *grad, int *nit, double *resid);
 

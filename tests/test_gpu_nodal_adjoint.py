@@ -144,7 +144,7 @@ def test_gpu_nodal_transpose_eight_group():
 
 
 def test_gpu_full_model_adjoint_converges(full):
-    assert 2 <= full["steps"] <= 100
+    assert 2 <= full["steps"] < full["sens"].nodal_maxit
     assert full["sens"].engine.nodal_adjoint_resid <= 1e-9
 
 
