@@ -169,10 +169,17 @@ __global__ void __launch_bounds__(NTF) k_matvec(LV lv, const double *A64, int g,
 // smode 0: TSrc (forward, :1006-1033), 1: TSrcAd (:1037-1064), 2: TSrcTr
 // (transient, :1068-1096: fission source weighted by 1 - beta + dfis, no 1/k)
 // (compiled per group count GG, so that the in-scatter loads of a cell are
-// issued together)
+// issued together).  smask[(core * 2 + adjoint) * G + g]: the in-scatter
+// source groups h whose sigma_s term is nonzero somewhere in the core (bit
+// h) and whether the external source is (SMASK_EX); the terms left out are
+// exactly +0 (sigma_s = 0 times a finite flux, or a zero source), so the
+// sums are the same as with every term.
+constexpr unsigned SMASK_EX = 1u << 30;
+
 template <int TAB, int GG>
 __global__ void __launch_bounds__(NTF) k_src(LV lv, XS xs, int g, int smode, int first_group, const double *f0,
-                                            double *f0c, const double *fs0, double *fs0c, double *b) {
+                                            double *f0c, const double *fs0, double *fs0c, double *b,
+                                            const unsigned *smask) {
 #pragma clang fp contract(off)
   const BlkView v = blk_view_t<TAB>(lv);
   if (v.sc->done) return;
@@ -180,22 +187,25 @@ __global__ void __launch_bounds__(NTF) k_src(LV lv, XS xs, int g, int smode, int
   const long NC = lv.NC;
   constexpr int G = GG;
   const int adjoint = smode == 1;
+  const unsigned msk = smask[((size_t)v.b.core * 2 + adjoint) * G + g];
+  const bool ex = (msk & SMASK_EX) != 0;
   for (long c = v.b.lo + threadIdx.x; c < v.b.hi; c += NTF) {
     if (first_group) fs0c[c] = fs0[c];
     f0c[(size_t)g * NC + c] = f0[(size_t)g * NC + c];
     double s0 = 0.0;
     #pragma unroll
     for (int h = 0; h < G; ++h) {
-      if (h == g) continue;
+      if (h == g || !((msk >> h) & 1u)) continue;
       const double sg = adjoint ? xs.sigs[((size_t)h * G + g) * NC + c] : xs.sigs[((size_t)g * G + h) * NC + c];
       s0 = s0 + sg * f0[(size_t)h * NC + c];
     }
+    const double e = ex ? xs.exsrc[(size_t)g * NC + c] : 0.0;
     if (smode == 2) {
-      b[c] = xs.wtr[c] * xs.chi[(size_t)g * NC + c] * fs0[c] + s0 + xs.exsrc[(size_t)g * NC + c];
+      b[c] = xs.wtr[c] * xs.chi[(size_t)g * NC + c] * fs0[c] + s0 + e;
       continue;
     }
     const double w = adjoint ? xs.nuf[(size_t)g * NC + c] : xs.chi[(size_t)g * NC + c];
-    b[c] = w * fs0[c] / Ke + s0 + xs.exsrc[(size_t)g * NC + c];
+    b[c] = w * fs0[c] / Ke + s0 + e;
   }
 }
 

@@ -166,6 +166,12 @@ struct adg_ctx {
   // nodal
   double *S = nullptr, *a2 = nullptr, *a4 = nullptr, *L1 = nullptr, *npart = nullptr;
   double *Bd = nullptr, *cf4 = nullptr;   // cooperative nodal passes: get_B node terms, A F G H per direction
+  // per core, direction (forward, adjoint) and group: the in-scatter source
+  // groups with a nonzero sigma_s anywhere in the core (bits 0..G-1) and a
+  // nonzero external source (SMASK_EX); the source kernel leaves out the
+  // other terms, which add exactly +0.  Device copy and host copy.
+  unsigned *smask = nullptr;
+  std::vector<unsigned> hmask;
   // reductions / scalars
   double *part = nullptr;          // three rotating partial buffers of 4 x nflat
   Scal *sc = nullptr;
@@ -747,10 +753,10 @@ void group_solve(adg_ctx *c, int g, int smode, int first_group, int nin) {
   case GG:                                                                                                    \
     if (c->B > 1)                                                                                             \
       hipLaunchKernelGGL((k_src<1, GG>), dim3(c->nflat), dim3(NTF), 0, c->s, c->lv(c->tflat), c->xs(), g, smode, \
-                         first_group, c->f0, c->f0c, c->fs0, c->fs0c, c->b);                                  \
+                         first_group, c->f0, c->f0c, c->fs0, c->fs0c, c->b, (const unsigned *)c->smask);        \
     else                                                                                                      \
       hipLaunchKernelGGL((k_src<0, GG>), dim3(nbs), dim3(NTF), 0, c->s, lvs, c->xs(), g, smode,                \
-                         first_group, c->f0, c->f0c, c->fs0, c->fs0c, c->b);                                  \
+                         first_group, c->f0, c->f0c, c->fs0, c->fs0c, c->b, (const unsigned *)c->smask);        \
     break;
     SRC_CASE(1) SRC_CASE(2) SRC_CASE(3) SRC_CASE(4) SRC_CASE(5) SRC_CASE(6) SRC_CASE(7) SRC_CASE(8)
 #undef SRC_CASE
@@ -1162,6 +1168,51 @@ void host_sum2(adg_ctx *c, const double *part, double (&out)[2]) {
   }
 }
 
+// the source kernel's masks of core `core` (host copy -> device)
+void push_mask(adg_ctx *c, int core) {
+  const size_t o = (size_t)core * 2 * c->G;
+  scopy(c->s, c->smask + o, c->hmask.data() + o, 2 * c->G * sizeof(unsigned), hipMemcpyHostToDevice);
+}
+
+// every in-scatter group of core `core`, external source as ex (cross
+// sections made on the device, or a source written by a kernel)
+void mask_all(adg_ctx *c, int core, bool ex) {
+  const size_t o = (size_t)core * 2 * c->G;
+  const unsigned m = ((1u << c->G) - 1u) | (ex ? SMASK_EX : 0u);
+  bool same = true;
+  for (int i = 0; i < 2 * c->G; ++i) same = same && c->hmask[o + i] == m;
+  if (same) return;
+  for (int i = 0; i < 2 * c->G; ++i) c->hmask[o + i] = m;
+  push_mask(c, core);
+}
+
+// every in-scatter group of core `core`, the external-source bits kept
+// (cross sections made on the device)
+void mask_scatter_all(adg_ctx *c, int core) {
+  const size_t o = (size_t)core * 2 * c->G;
+  const unsigned all = (1u << c->G) - 1u;
+  bool same = true;
+  for (int i = 0; i < 2 * c->G; ++i) same = same && (c->hmask[o + i] & all) == all;
+  if (same) return;
+  for (int i = 0; i < 2 * c->G; ++i) c->hmask[o + i] |= all;
+  push_mask(c, core);
+}
+
+// the external-source bit of every group of core `core` on (a kernel
+// writes the source); the previous masks are returned for restore_mask
+std::vector<unsigned> mask_source_on(adg_ctx *c, int core) {
+  std::vector<unsigned> saved = c->hmask;
+  const size_t o = (size_t)core * 2 * c->G;
+  for (int i = 0; i < 2 * c->G; ++i) c->hmask[o + i] |= SMASK_EX;
+  push_mask(c, core);
+  return saved;
+}
+
+void restore_mask(adg_ctx *c, int core, const std::vector<unsigned> &saved) {
+  c->hmask = saved;
+  push_mask(c, core);
+}
+
 // ---------------------------------------------------------------------------
 // transpose of the nodal update and the full-model adjoint (single core)
 // ---------------------------------------------------------------------------
@@ -1448,6 +1499,9 @@ int create_batch(const adg_geom *geoms, int ncore, adg_ctx **out) {
     }
     c->npart = dalloc<double>(c->s, (size_t)3 * (3 * c->nface + c->nbface));
     c->sc = dalloc<Scal>(c->s, ncore);
+    c->smask = dalloc<unsigned>(c->s, (size_t)ncore * 2 * G);
+    c->hmask.assign((size_t)ncore * 2 * G, ((1u << G) - 1u) | SMASK_EX);
+    scopy(c->s, c->smask, c->hmask.data(), c->hmask.size() * sizeof(unsigned), hipMemcpyHostToDevice);
     HIPCHK(hipHostMalloc((void **)&c->hsc, ncore * sizeof(Scal), hipHostMallocDefault));
   } catch (const HipError &e) {
     const int rc = hip_fail(c, e);
@@ -1600,7 +1654,7 @@ void adg_destroy(adg_ctx *c) {
                   c->pp64[1], c->vv64[0], c->vv64[1], c->A32, c->rr32[0], c->rr32[1], c->rs32, c->t32, c->d32,
                   c->pp32[0], c->pp32[1], c->vv32[0], c->vv32[1], c->S, c->a2, c->a4, c->L1, c->Bd, c->cf4, c->npart, c->part,
                   c->sc, c->psi, c->phis, c->fs0s, c->exs, c->gam, c->dJdP, c->grad, c->wtr, c->d_cell,
-                  c->dstage, c->psync, c->vjp};
+                  c->dstage, c->psync, c->vjp, c->smask};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (c->hsc) (void)hipHostFree(c->hsc);
@@ -1639,11 +1693,20 @@ int adg_set_xs_core(adg_ctx *c, int core, const double *D, const double *sigr, c
     scatter(c, core, sigr, G, c->sigr, st);
     scatter(c, core, nuf, G, c->nuf, st);
     scatter(c, core, sigf, G, c->sigf, st);
-    // sigs(n, from, to) -> device [to][from][cell]
+    // sigs(n, from, to) -> device [to][from][cell]; which (to, from) pairs
+    // are nonzero anywhere, for the source kernel's masks
     std::vector<double> tr((size_t)G * G * N);
+    std::vector<unsigned char> nz((size_t)G * G, 0);
     for (int f = 0; f < G; ++f)
-      for (int t = 0; t < G; ++t)
-        for (int n = 0; n < N; ++n) tr[((size_t)t * G + f) * N + n] = sigs[((size_t)t * G + f) * N + n];
+      for (int t = 0; t < G; ++t) {
+        unsigned char any = 0;
+        for (int n = 0; n < N; ++n) {
+          const double v = sigs[((size_t)t * G + f) * N + n];
+          tr[((size_t)t * G + f) * N + n] = v;
+          any |= v != 0.0;
+        }
+        nz[(size_t)t * G + f] = any;
+      }
     scatter(c, core, tr.data(), G * G, c->sigs, st);
     // chi(mat(n), g) expanded per node
     std::vector<double> ch((size_t)G * N);
@@ -1651,11 +1714,27 @@ int adg_set_xs_core(adg_ctx *c, int core, const double *D, const double *sigr, c
       for (int n = 0; n < N; ++n) ch[(size_t)g * N + n] = chi[(size_t)g * h.nmat + (mat[n] - 1)];
     scatter(c, core, ch.data(), G, c->chi, st);
     scatter(c, core, dc, 6 * G, c->dc, st);
+    std::vector<unsigned char> exnz(G, 0);
     if (exsrc) {
+      for (int g = 0; g < G; ++g)
+        for (int n = 0; n < N; ++n) exnz[g] |= exsrc[(size_t)g * N + n] != 0.0;
       scatter(c, core, exsrc, G, c->exsrc, st);
     } else {
       std::vector<double> z((size_t)G * N, 0.0);
       scatter(c, core, z.data(), G, c->exsrc, st);
+    }
+    {
+      unsigned *m = c->hmask.data() + (size_t)core * 2 * G;
+      for (int g = 0; g < G; ++g) {
+        unsigned fw = exnz[g] ? SMASK_EX : 0u, ad = fw;
+        for (int hh = 0; hh < G; ++hh) {
+          if (nz[(size_t)g * G + hh]) fw |= 1u << hh;    // into g from hh (forward)
+          if (nz[(size_t)hh * G + g]) ad |= 1u << hh;    // into hh from g (adjoint)
+        }
+        m[g] = fw;
+        m[G + g] = ad;
+      }
+      push_mask(c, core);
     }
     h.xs_set = true;
   } catch (const HipError &e) {
@@ -1764,6 +1843,7 @@ int adg_set_xs_tables_core(adg_ctx *c, int core, const adg_xs_tables *t, const i
     scatter(c, core, dc, 6 * G, c->dc, st);
     std::vector<double> z((size_t)G * N, 0.0);
     scatter(c, core, z.data(), G, c->exsrc, st);
+    mask_all(c, core, false);
     h.xs_set = false;
   } catch (const HipError &e) {
     return hip_fail(c, e);
@@ -1839,6 +1919,7 @@ int adg_set_xtab_tables_core(adg_ctx *c, int core, const adg_xtab_tables *t, con
     scatter(c, core, ch.data(), G, c->chi, st);
     std::vector<double> z((size_t)G * N, 0.0);
     scatter(c, core, z.data(), G, c->exsrc, st);
+    mask_all(c, core, false);
     h.xs_set = false;
   } catch (const HipError &ex) {
     return hip_fail(c, ex);
@@ -1862,6 +1943,7 @@ int adg_update_xs_core(adg_ctx *c, int core, double bcon, const double *ftem, co
   for (int q = 0; q < 4; ++q)
     if ((on & (1 << need[q])) && !in[q]) return fail(c, ADG_E_ARG, "parameter array of an active feedback is null");
   try {
+    mask_scatter_all(c, core);
     ensure_stage(c, 3L * N + std::max(h.xnb, 1));
     for (int q = 0; q < 4; ++q)
       if (in[q])
@@ -2467,6 +2549,7 @@ int adg_backward(adg_ctx *c, const adg_iter *it, double dJdk, const double *dJdp
       const size_t n = (size_t)G * c->NC;
       HIPCHK(hipMemcpyAsync(c->exs, c->exsrc, n * sizeof(double), hipMemcpyDeviceToDevice, c->s));
       hipLaunchKernelGGL(k_sstar, dim3(c->nflat), dim3(NTF), 0, c->s, lv, x, c->f0, c->dJdP, c->exsrc);
+      const std::vector<unsigned> msaved = mask_source_on(c, 0);
       save_forward(c);
       HIPCHK(hipMemsetAsync(c->f0, 0, n * sizeof(double), c->s));
       HIPCHK(hipMemsetAsync(c->fs0, 0, c->NC * sizeof(double), c->s));
@@ -2478,6 +2561,7 @@ int adg_backward(adg_ctx *c, const adg_iter *it, double dJdk, const double *dJdp
       HIPCHK(hipMemcpyAsync(c->gam, c->f0, n * sizeof(double), hipMemcpyDeviceToDevice, c->s));
       HIPCHK(hipMemcpyAsync(c->exsrc, c->exs, n * sizeof(double), hipMemcpyDeviceToDevice, c->s));
       restore_forward(c);
+      restore_mask(c, 0, msaved);
       if (rc != ADG_OK) return rc;
     }
     const double apsi = dJdk * k * k / c->psifphi;
@@ -2541,6 +2625,7 @@ int adg_backward_nodal(adg_ctx *c, const adg_iter *it, int kern, double dJdk, co
     nodal_fwd(c, kern);
     HIPCHK(hipMemcpyAsync(c->exs, c->exsrc, n * sizeof(double), hipMemcpyDeviceToDevice, c->s));
     HIPCHK(hipMemsetAsync(c->vw, 0, nd * sizeof(double), c->s));
+    const std::vector<unsigned> msaved = mask_source_on(c, 0);
     // w = dJ/d(D-hat) of the CMFD adjoint for the response J + w . N, where
     // N is the nodal update and its transpose adds the flux and k
     // cotangents to the adjoint's sources and the old-D-hat cotangent to w.
@@ -2583,6 +2668,7 @@ int adg_backward_nodal(adg_ctx *c, const adg_iter *it, int kern, double dJdk, co
       if (nod && res <= tol && gam_conv) break;
     }
     HIPCHK(hipMemcpyAsync(c->exsrc, c->exs, n * sizeof(double), hipMemcpyDeviceToDevice, c->s));
+    restore_mask(c, 0, msaved);
     if (nit) *nit = std::min(iter, maxit);
     if (resid) *resid = res;
     if (rc != ADG_OK) return rc;
