@@ -47,6 +47,21 @@ def test_create_rejects_bad_geometry_without_touching_the_gpu():
     assert lib.adg_create(ctypes.byref(g), ctypes.byref(h)) == engine.ADG_E_ARG
 
 
+def test_differentiation_entry_points_reject_bad_arguments_without_touching_the_gpu():
+    lib = engine.load_library()
+    w = np.zeros(12)
+    it = engine.AdgIter()
+    out = np.zeros(8)
+    dp = ctypes.POINTER(ctypes.c_double)
+    # no context, no cotangent, no output, tol <= 0, maxit < 1
+    assert lib.adg_nodal_vjp(None, 2, w.ctypes.data_as(dp), None, None, None, None) == engine.ADG_E_ARG
+    assert lib.adg_nodal_vjp(None, 2, None, None, None, None, None) == engine.ADG_E_ARG
+    for tol, maxit, grad in ((1e-9, 10, None), (0.0, 10, out), (1e-9, 0, out)):
+        g = grad.ctypes.data_as(dp) if grad is not None else None
+        assert lib.adg_backward_nodal(None, ctypes.byref(it), 2, 1.0, None, tol, maxit, g, None, None) \
+            == engine.ADG_E_ARG
+
+
 def test_generators_reproduce_the_fixture_cores():
     import hashlib
     d = golden("core3d_2g_s1_solve")
