@@ -791,6 +791,7 @@ void nodal_update(adg_ctx *c, int kern, int cmode, int sched) {
   case GG:                                                                                               \
     hipLaunchKernelGGL(k_nod_src, dim3(c->ncell, GG), dim3(NT), 0, c->s, an);                            \
     if (coop) {                                                                                          \
+      hipLaunchKernelGGL(k_nod_bd<GG>, dim3(c->ncell, (NT + NT / GG - 1) / (NT / GG)), dim3(NT), 0, c->s, an); \
       hipLaunchKernelGGL(k_nod_pass1c<GG>, g1, dim3(NT), 0, c->s, an);                                   \
       hipLaunchKernelGGL(k_nod_pass2c<GG>, g2, dim3(NT), 0, c->s, af);                                   \
       if (c->nbface)                                                                                     \

@@ -33,9 +33,10 @@ struct NodArgs {
   int coop;             // pass 2 leaves the interior faces to k_nod_pass2c
   double *part;
   // cooperative passes: the node terms of get_B's rows (S1 of the reference's
-  // dum, [g][h][NC], made once per update by k_nod_src) and each node's A, F,
-  // G, H per direction ([u][g][4][NC], made by k_nod_pass1c), read by the
-  // interface pass instead of being made again for every face
+  // dum, [cell][g][h], made once per update by k_nod_src) and each node's A,
+  // F, G, H per direction ([u][cell][g][4], made by k_nod_pass1c), read by the
+  // interface pass instead of being made again for every face; node-major,
+  // so that a lane's row or coefficients are one contiguous read
   double *Bd, *cf4;
 };
 
@@ -116,10 +117,41 @@ __global__ void __launch_bounds__(NT) k_nod_src(NodArgs a) {
     a.S[((size_t)1 * G + g) * NC + c] = L[0] + L[2];
     a.S[((size_t)2 * G + g) * NC + c] = L[0] + L[1];
   }
-  if (a.Bd) {   // row g of get_B's node terms, once per update for the cooperative passes
-    const double Ke = v.sc->Ke;
-    for (int hh = 0; hh < G; ++hh) a.Bd[((size_t)g * G + hh) * NC + c] = nod_dum(a, G, NC, c, Ke, g, hh);
-  }
+}
+
+// get_B's node terms of every row of a node (S1 of the reference's dum,
+// src/mod_nodal.f90:1345-1405), once per update for the cooperative passes,
+// node-major: one thread per (node, row g), g fastest, so that the G rows
+// of a node, and the nodes of a wavefront, are stored side by side and each
+// wavefront's stores fill whole lines.  The node-table entry of NT nodes is
+// covered by blockIdx.y = 0 .. ceil(NT / (NT / G)) - 1 blocks of NT / G nodes.
+template <int G>
+__global__ void __launch_bounds__(NT) k_nod_bd(NodArgs a) {
+#pragma clang fp contract(off)
+  constexpr int NPB = NT / G;
+  const BlkView v = blk_view(a.lv);
+  const Geo &q = *v.q;
+  if ((int)threadIdx.x >= NPB * G) return;
+  const int local = (int)blockIdx.y * NPB + (int)threadIdx.x / G, g = (int)threadIdx.x % G;
+  if (local >= NT) return;
+  const long nxy = (long)q.nxx * q.nyy;
+  const long t = v.b.lo + local;
+  if (t >= v.b.hi || t >= nxy * q.nzz) return;
+  if (v.sc->done || (a.sched && !nodal_outer(v.sc))) return;
+  int i, j, k;
+  k = (int)(t / nxy);
+  { long r = t - (long)k * nxy; j = (int)(r / q.nxx); i = (int)(r - (long)j * q.nxx); }
+  const long c = cell_of(q, i, j, k);
+  if (!(q.flag[c] & F_ACTIVE)) return;
+  const double Ke = v.sc->Ke;
+  // the whole row first (every load before any store: the output may not
+  // alias the inputs as far as the compiler knows), then stored at once
+  double row[G];
+#pragma unroll
+  for (int hh = 0; hh < G; ++hh) row[hh] = nod_dum(a, G, q.NC, c, Ke, g, hh);
+  double *out = a.Bd + ((size_t)c * G + g) * G;
+#pragma unroll
+  for (int hh = 0; hh < G; ++hh) out[hh] = row[hh];
 }
 
 // Lxyz of every node and group into out[u][g][cell] (the leakages the
@@ -189,7 +221,7 @@ __device__ __forceinline__ void nod_B_row_pre(const NodArgs &a, long NC, long c,
   const double D = a.xs.D[(size_t)g * NC + c];
   const double fac = 0.25 * (h * h) / D;
 #pragma unroll
-  for (int hh = 0; hh < G; ++hh) B[hh] = fac * a.Bd[((size_t)g * G + hh) * NC + c];
+  for (int hh = 0; hh < G; ++hh) B[hh] = fac * a.Bd[((size_t)c * G + g) * G + hh];
 }
 
 // get_B, all rows; B row-major [g*G+h]
@@ -763,11 +795,11 @@ __global__ void __launch_bounds__(NT) k_nod_pass1c(NodArgs a) {
     nod_B_row_pre<G>(a, NC, c, h, g, B);
     const Coef6 cf = nod_coef(a.kern, a.xs.sigr[(size_t)g * NC + c], a.xs.D[(size_t)g * NC + c], h);
     if (r < G) {   // A, F, G, H of this node and direction for the interface pass
-      double *cq = a.cf4 + ((size_t)(u * G + g) * 4) * NC + c;
+      double *cq = a.cf4 + (((size_t)u * NC + c) * G + g) * 4;
       cq[0] = cf.A;
-      cq[NC] = cf.F;
-      cq[2 * NC] = cf.G;
-      cq[3 * NC] = cf.H;
+      cq[1] = cf.F;
+      cq[2] = cf.G;
+      cq[3] = cf.H;
     }
     const double *Su = a.S + (size_t)u * G * NC;
     const double D = a.xs.D[(size_t)g * NC + c];
@@ -854,8 +886,8 @@ __global__ void __launch_bounds__(NT) k_nod_pass2b(NodArgs a, const long *bfl, i
     const int bc = hasp ? q.bc[2 * u + 1] : q.bc[2 * u];
     double B[G];
     nod_B_row_pre<G>(a, NC, cs, hs, g, B);
-    const double *cq = a.cf4 + ((size_t)(u * G + g) * 4) * NC + cs;
-    const double A = cq[0], F = cq[NC], Gc = cq[2 * NC], H = cq[3 * NC];
+    const double *cq = a.cf4 + (((size_t)u * NC + cs) * G + g) * 4;
+    const double A = cq[0], F = cq[1], Gc = cq[2], H = cq[3];
     const double L1v = a.L1[ub + (size_t)g * NC + cs];
     const double a2v = a.a2[ub + (size_t)g * NC + cs], a4v = a.a4[ub + (size_t)g * NC + cs];
     const double P = 2.0 * a.xs.D[(size_t)g * NC + cs] / hs;
@@ -987,13 +1019,13 @@ __global__ void __launch_bounds__(NT) k_nod_pass2c(NodArgs a) {
     nod_B_row_pre<G>(a, NC, cs, hsd, g, Bs);
     Coef6 cfs;   // A, F, G, H of this side's node as pass 1 made them
     {
-      const double *cq = a.cf4 + ((size_t)(u * G + g) * 4) * NC + cs;
+      const double *cq = a.cf4 + (((size_t)u * NC + cs) * G + g) * 4;
       cfs.A = cq[0];
       cfs.B = 0.0;
       cfs.E = 0.0;
-      cfs.F = cq[NC];
-      cfs.G = cq[2 * NC];
-      cfs.H = cq[3 * NC];
+      cfs.F = cq[1];
+      cfs.G = cq[2];
+      cfs.H = cq[3];
     }
     const double Ls = a.L1[ub + (size_t)g * NC + cs];
     const double a2s = a.a2[ub + (size_t)g * NC + cs], a4s = a.a4[ub + (size_t)g * NC + cs];
