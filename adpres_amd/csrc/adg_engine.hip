@@ -179,6 +179,7 @@ struct adg_ctx {
   bool graphs = true;              // replay the outer iteration as a captured hipGraph
   int check_every = 8;             // outers between host polls of the done flags
   bool nod_coop = true;            // cooperative pass 2 for G >= 5 (ADG_NOD_COOP=0: per-thread)
+  bool nod_lu32 = true;            // FP32+refine: interface-pass solves in FP32 + FP64 correction (ADG_NOD_LU32=0: FP64)
   bool src_full = true;            // source kernel on one chunk per block (ADG_SRC_FULL=0: flat shape)
   bool xs_lds = true;              // cross-section tables staged in LDS when they fit (ADG_XS_LDS=0: caches)
   double timing[4] = {0, 0, 0, 0};
@@ -782,8 +783,11 @@ void nodal_update(adg_ctx *c, int kern, int cmode, int sched) {
   ab.lv = c->lv(c->tbface);
   // larger group counts: the cooperative passes (one lane per row of the
   // G x G node systems and the 2G x 2G interface systems) and the boundary
-  // faces from their list; ADG_NOD_COOP=0 keeps every system per thread
+  // faces from their list; ADG_NOD_COOP=0 keeps every system per thread.
+  // The FP32+refine form solves the interface systems in FP32 with one FP64
+  // defect correction (ADG_NOD_LU32=0: in FP64)
   const bool coop = nod_coop(c);
+  const bool lu32 = coop && c->precision == ADG_FP32_REFINE && c->nod_lu32;
   af.coop = coop ? 1 : 0;
   const dim3 g1(c->ncell, 3, 1), g2(c->nface, 3, 1);
   switch (c->G) {
@@ -793,7 +797,10 @@ void nodal_update(adg_ctx *c, int kern, int cmode, int sched) {
     if (coop) {                                                                                          \
       hipLaunchKernelGGL(k_nod_bd<GG>, dim3(c->ncell, (NT + NT / GG - 1) / (NT / GG)), dim3(NT), 0, c->s, an); \
       hipLaunchKernelGGL(k_nod_pass1c<GG>, g1, dim3(NT), 0, c->s, an);                                   \
-      hipLaunchKernelGGL(k_nod_pass2c<GG>, g2, dim3(NT), 0, c->s, af);                                   \
+      if (lu32)                                                                                          \
+        hipLaunchKernelGGL((k_nod_pass2c<GG, true>), g2, dim3(NT), 0, c->s, af);                         \
+      else                                                                                               \
+        hipLaunchKernelGGL((k_nod_pass2c<GG, false>), g2, dim3(NT), 0, c->s, af);                        \
       if (c->nbface)                                                                                     \
         hipLaunchKernelGGL(k_nod_pass2b<GG>, dim3(c->nbface), dim3(NT), 0, c->s, ab, c->bfl, c->nface);  \
     } else {                                                                                             \
@@ -1428,6 +1435,7 @@ int create_batch(const adg_geom *geoms, int ncore, adg_ctx **out) {
     if (const char *ev = getenv("ADG_GRAPH")) c->graphs = atoi(ev) != 0;
     if (const char *ev = getenv("ADG_CHECK")) c->check_every = std::max(1, atoi(ev));
     if (const char *ev = getenv("ADG_NOD_COOP")) c->nod_coop = atoi(ev) != 0;
+    if (const char *ev = getenv("ADG_NOD_LU32")) c->nod_lu32 = atoi(ev) != 0;
     if (const char *ev = getenv("ADG_SRC_FULL")) c->src_full = atoi(ev) != 0;
     if (const char *ev = getenv("ADG_XS_LDS")) c->xs_lds = atoi(ev) != 0;
     if (const char *ev = getenv("ADG_PSPIN")) c->pspin = std::max(1L, atol(ev));

@@ -715,8 +715,9 @@ __device__ __forceinline__ bool coop_lu_diag_small(const double (&R)[M], int r) 
   return r < M && fabs(dg) < (double)10e-5f;
 }
 
+// the elimination: R becomes L (below the diagonal) and U (on and above it)
 template <int M, int S, typename T>
-__device__ __forceinline__ void coop_lu(T (&R)[M], T s, int r, int f, T (&xs)[M]) {
+__device__ __forceinline__ void coop_lu_factor(T (&R)[M], int r, int f) {
 #pragma clang fp contract(off)
   const bool rowlane = r < M;
 #pragma unroll
@@ -732,6 +733,12 @@ __device__ __forceinline__ void coop_lu(T (&R)[M], T s, int r, int f, T (&xs)[M]
       R[p] = piv;
     }
   }
+}
+
+// the substitutions with those factors, for right-hand side s (this lane's row)
+template <int M, int S, typename T>
+__device__ __forceinline__ void coop_lu_subst(const T (&R)[M], T s, int r, int f, T (&xs)[M]) {
+#pragma clang fp contract(off)
   // forward substitution y(i) = b(i) - sum_k<i L(i,k) y(k), k ascending
   T isum = (T)0, y = (T)0;
 #pragma unroll
@@ -754,6 +761,35 @@ __device__ __forceinline__ void coop_lu(T (&R)[M], T s, int r, int f, T (&xs)[M]
     }
     xs[p] = __shfl(xv, S * p + f, 64);
   }
+}
+
+template <int M, int S, typename T>
+__device__ __forceinline__ void coop_lu(T (&R)[M], T s, int r, int f, T (&xs)[M]) {
+  coop_lu_factor<M, S, T>(R, r, f);
+  coop_lu_subst<M, S, T>(R, s, r, f, xs);
+}
+
+// The FP32+refine form's nodal solves: the elimination and the solve in
+// FP32, then one FP64 defect correction -- the residual s - R x of the FP64
+// system formed in FP64, its correction solved with the FP32 factors and
+// added in FP64.  R (this lane's FP64 row) is left as it was.
+template <int M, int S>
+__device__ __forceinline__ void coop_lu_refine(const double (&R)[M], double s, int r, int f, double (&xs)[M]) {
+#pragma clang fp contract(off)
+  float Rf[M], xf[M];
+#pragma unroll
+  for (int c2 = 0; c2 < M; ++c2) Rf[c2] = (float)R[c2];
+  coop_lu_factor<M, S, float>(Rf, r, f);
+  coop_lu_subst<M, S, float>(Rf, (float)s, r, f, xf);
+  double res = s;
+#pragma unroll
+  for (int c2 = 0; c2 < M; ++c2) {
+    xs[c2] = (double)xf[c2];
+    res = res - R[c2] * xs[c2];
+  }
+  coop_lu_subst<M, S, float>(Rf, (float)res, r, f, xf);
+#pragma unroll
+  for (int c2 = 0; c2 < M; ++c2) xs[c2] = xs[c2] + (double)xf[c2];
 }
 
 // ---------------------------------------------------------------------------
@@ -953,11 +989,13 @@ __global__ void __launch_bounds__(NT) k_nod_pass2b(NodArgs a, const long *bfl, i
 // ---------------------------------------------------------------------------
 // pass 2, interior faces, cooperative form for larger group counts: 16 lanes
 // per face (four faces per wavefront), lane 4 r + f owning row r of face f's
-// 2G x 2G system of get_a1matvec (:630-698), solved by coop_lu.  A block
-// covers the same 256-face range as k_nod_pass2 (16 faces at a time); the
-// boundary faces are k_nod_pass2b's.
+// 2G x 2G system of get_a1matvec (:630-698), solved by coop_lu (L32: by
+// coop_lu_refine, FP32 factors and one FP64 defect correction; the system and
+// everything after the solve stay FP64).  A block covers the same 256-face
+// range as k_nod_pass2 (16 faces at a time); the boundary faces are
+// k_nod_pass2b's.
 // ---------------------------------------------------------------------------
-template <int G>
+template <int G, bool L32 = false>
 __global__ void __launch_bounds__(NT) k_nod_pass2c(NodArgs a) {
 #pragma clang fp contract(off)
   constexpr int M2 = 2 * G;
@@ -1089,7 +1127,8 @@ __global__ void __launch_bounds__(NT) k_nod_pass2c(NodArgs a) {
     // LU_solve (:829-897): the input-diagonal test, then the elimination
     if (coop_lu_diag_small<M2>(R, r)) atomicMin(&sc->lu_key, lu_key(ord, li, lj, lk));
     double xs[M2];
-    coop_lu<M2, 4>(R, s, r, fw, xs);
+    if constexpr (L32) coop_lu_refine<M2, 4>(R, s, r, fw, xs);
+    else coop_lu<M2, 4>(R, s, r, fw, xs);
     // get_a3 and nodal_coup_upd for group g (lanes 0..G-1); node n's B row
     // read again rather than kept live through the elimination
     if (r < G) {

@@ -559,7 +559,8 @@ def nodal_probe(p):
     outers from the first-call state, then two SANM nodal updates (each a
     leakage pass, the node pass, the interface pass, the boundary-face pass
     and the argmax) on that state; in the FP64 form, then in the FP32+refine
-    form (whose nodal update is the same FP64 update)."""
+    form (the same update, with the interface systems solved in FP32 and
+    corrected once in FP64)."""
     from adpres_amd.engine import Engine
     out = {"probe": "nodal-counters", "nodes": p.nnod, "groups": p.ng}
     for prec in ("fp64", "fp32_refine"):

@@ -268,6 +268,33 @@ def test_gpu_cooperative_nodal_pass_is_bit_identical(monkeypatch):
     assert res["1"][1] > 0.0
 
 
+def test_gpu_fp32_interface_solves_keep_the_fp64_update(monkeypatch):
+    """The FP32+refinement form solves the 2G x 2G interface systems of the
+    cooperative pass in FP32 and corrects once in FP64 (the system, the
+    residual and everything after the solve are FP64).  Two updates on the
+    8-group core: D-hats within 1e-10 of the scale of the FP64-solve update
+    (ADG_NOD_LU32=0), ndmax within 1e-9 relative at the same node."""
+    p = generate("core3d_8g", 2).problem
+    rng = np.random.default_rng(5)
+    f0 = rng.uniform(0.9, 1.1, (p.ng, p.nnod))
+    dn = np.zeros((6, p.ng, p.nnod))
+    res = {}
+    for lu in ("1", "0"):
+        monkeypatch.setenv("ADG_NOD_LU32", lu)
+        e = Engine(p, "fp32_refine")
+        e.set_state(f0, np.zeros(p.nnod), 1.0, dn)
+        out = []
+        for _ in range(2):
+            nd, loc = e.nodal_update()
+            out.append((e.state()["dn"], nd, loc))
+        res[lu] = out
+        e.close()
+    for (d32, nd32, loc32), (d64, nd64, loc64) in zip(res["1"], res["0"]):
+        assert not np.array_equal(d32, d64)      # the FP32 path ran
+        assert np.max(np.abs(d32 - d64)) <= 1e-10 * np.max(np.abs(d64))
+        assert abs(nd32 - nd64) <= 1e-9 * nd64 and loc32 == loc64
+
+
 def test_gpu_status_lu_pivot_and_power_nonpos():
     """The engine's ADG_E_LU_PIVOT and ADG_E_POWER_NONPOS on the crafted
     inputs the oracle stops on (tests/test_oracle.py): the nodal update at
