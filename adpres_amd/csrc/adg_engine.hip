@@ -179,7 +179,7 @@ struct adg_ctx {
   bool graphs = true;              // replay the outer iteration as a captured hipGraph
   int check_every = 8;             // outers between host polls of the done flags
   bool nod_coop = true;            // cooperative pass 2 for G >= 5 (ADG_NOD_COOP=0: per-thread)
-  bool nod_lu32 = true;            // FP32+refine: interface-pass solves in FP32 + FP64 correction (ADG_NOD_LU32=0: FP64)
+  bool nod_lu32 = false;           // FP32+refine: interface-pass solves in FP32 + FP64 correction (ADG_NOD_LU32=1)
   bool src_full = true;            // source kernel on one chunk per block (ADG_SRC_FULL=0: flat shape)
   bool xs_lds = true;              // cross-section tables staged in LDS when they fit (ADG_XS_LDS=0: caches)
   double timing[4] = {0, 0, 0, 0};
@@ -784,8 +784,9 @@ void nodal_update(adg_ctx *c, int kern, int cmode, int sched) {
   // larger group counts: the cooperative passes (one lane per row of the
   // G x G node systems and the 2G x 2G interface systems) and the boundary
   // faces from their list; ADG_NOD_COOP=0 keeps every system per thread.
-  // The FP32+refine form solves the interface systems in FP32 with one FP64
-  // defect correction (ADG_NOD_LU32=0: in FP64)
+  // ADG_NOD_LU32=1 has the FP32+refine form solve the interface systems in
+  // FP32 with one FP64 defect correction (slower than the FP64 solves since
+  // their broadcasts moved to DPP, so off by default)
   const bool coop = nod_coop(c);
   const bool lu32 = coop && c->precision == ADG_FP32_REFINE && c->nod_lu32;
   af.coop = coop ? 1 : 0;

@@ -696,14 +696,57 @@ __global__ void __launch_bounds__(NT) k_nod_pass2(NodArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// LU_solve (:829-897) of an M x M system held one row per lane: lane S r + f
-// of a wavefront owns row r of item f (S items side by side).  The
-// elimination broadcasts each pivot row with lane shuffles and the
+// LU_solve (:829-897) of an M x M system held one row per lane: each item
+// (node or face) of a wavefront has M (or more) lanes, one per row.  The
+// elimination broadcasts each pivot row to the item's lanes and the
 // substitutions each solved unknown, so every entry sees the operations of
 // the per-thread lu_solve in the same order and no lane holds more than its
 // own row.  On return every lane of the item holds the solution in xs.
-// Lanes with r >= M take part in the shuffles only.
+// Lanes with r >= M take part in the broadcasts only.
+//
+// The broadcasts are DPP moves within 16-lane rows of the wavefront (no LDS
+// round trip in the elimination's dependent chain).  Two lane layouts (L):
+//   LaneRow16: lane 16 f + r, item f is one row; row p of it is broadcast
+//              by row_newbcast:p;
+//   LaneRow8:  lane 8 f + r, items 2m and 2m + 1 share a row; row_newbcast:p
+//              and row_newbcast:8+p, each lane keeping its own item's.
+// An item's lanes are active or inactive together, so a lane reads only
+// lanes of its own item.
 // ---------------------------------------------------------------------------
+
+template <int B, int E, typename F>
+__device__ __forceinline__ void static_for(F &&fn) {
+  if constexpr (B < E) {
+    fn(std::integral_constant<int, B>{});
+    static_for<B + 1, E>(fn);
+  }
+}
+
+struct LaneRow16 {
+  template <int P>
+  static __device__ __forceinline__ float bcast(float v, int) {
+    static_assert(P >= 0 && P < 16, "row lane");
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x150 + P, 0xf, 0xf, false));
+  }
+  template <int P>
+  static __device__ __forceinline__ double bcast(double v, int) {
+    static_assert(P >= 0 && P < 16, "row lane");
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_mov_dpp((int)(unsigned)(b & 0xffffffffll), 0x150 + P, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), 0x150 + P, 0xf, 0xf, false);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+  }
+};
+
+struct LaneRow8 {
+  template <int P, typename T>
+  static __device__ __forceinline__ T bcast(T v, int f) {
+    static_assert(P >= 0 && P < 8, "half-row lane");
+    const T lo = LaneRow16::template bcast<P>(v, f);
+    const T hi = LaneRow16::template bcast<8 + P>(v, f);
+    return (f & 1) ? hi : lo;
+  }
+};
 
 // the reference's input-diagonal test for this lane's row
 template <int M>
@@ -716,42 +759,44 @@ __device__ __forceinline__ bool coop_lu_diag_small(const double (&R)[M], int r) 
 }
 
 // the elimination: R becomes L (below the diagonal) and U (on and above it)
-template <int M, int S, typename T>
+template <int M, typename L, typename T>
 __device__ __forceinline__ void coop_lu_factor(T (&R)[M], int r, int f) {
 #pragma clang fp contract(off)
   const bool rowlane = r < M;
+  static_for<0, M>([&](auto pc) {
+    constexpr int p = decltype(pc)::value;
+    // pivot row p (its entries p..M-1, final after the steps before p; lane
+    // p's row does not change in this step), each entry broadcast where it
+    // is used; the broadcasts run on every lane, the updates on rows r > p
+    const bool upd = r > p && rowlane;
+    const T piv = R[p] / L::template bcast<p>(R[p], f);
 #pragma unroll
-  for (int p = 0; p < M; ++p) {
-    // pivot row p (its entries p..M-1, final after the steps before p)
-    T pr[M];
-#pragma unroll
-    for (int c2 = p; c2 < M; ++c2) pr[c2] = __shfl(R[c2], S * p + f, 64);
-    if (r > p && rowlane) {
-      const T piv = R[p] / pr[p];
-#pragma unroll
-      for (int c2 = p + 1; c2 < M; ++c2) R[c2] = R[c2] - piv * pr[c2];
-      R[p] = piv;
+    for (int c2 = p + 1; c2 < M; ++c2) {
+      const T e = L::template bcast<p>(R[c2], f);
+      const T nv = R[c2] - piv * e;
+      R[c2] = upd ? nv : R[c2];
     }
-  }
+    R[p] = upd ? piv : R[p];
+  });
 }
 
 // the substitutions with those factors, for right-hand side s (this lane's row)
-template <int M, int S, typename T>
+template <int M, typename L, typename T>
 __device__ __forceinline__ void coop_lu_subst(const T (&R)[M], T s, int r, int f, T (&xs)[M]) {
 #pragma clang fp contract(off)
   // forward substitution y(i) = b(i) - sum_k<i L(i,k) y(k), k ascending
   T isum = (T)0, y = (T)0;
-#pragma unroll
-  for (int p = 0; p < M; ++p) {
+  static_for<0, M>([&](auto pc) {
+    constexpr int p = decltype(pc)::value;
     if (r == p) y = s - isum;
-    const T yp = __shfl(y, S * p + f, 64);
+    const T yp = L::template bcast<p>(y, f);
     if (r > p) isum = isum + R[p] * yp;
-  }
+  });
   // back substitution x(i) = (y(i) - sum_k>i U(i,k) x(k)) / U(i,i), k ascending
-#pragma unroll
-  for (int p = M - 1; p >= 0; --p) {
+  static_for<0, M>([&](auto qc) {
+    constexpr int p = M - 1 - decltype(qc)::value;
     T xv;
-    if (p == M - 1) {
+    if constexpr (p == M - 1) {
       xv = y / R[p];
     } else {
       T bs = (T)0;
@@ -759,43 +804,43 @@ __device__ __forceinline__ void coop_lu_subst(const T (&R)[M], T s, int r, int f
       for (int c2 = p + 1; c2 < M; ++c2) bs = bs + R[c2] * xs[c2];
       xv = (y - bs) / R[p];
     }
-    xs[p] = __shfl(xv, S * p + f, 64);
-  }
+    xs[p] = L::template bcast<p>(xv, f);
+  });
 }
 
-template <int M, int S, typename T>
+template <int M, typename L, typename T>
 __device__ __forceinline__ void coop_lu(T (&R)[M], T s, int r, int f, T (&xs)[M]) {
-  coop_lu_factor<M, S, T>(R, r, f);
-  coop_lu_subst<M, S, T>(R, s, r, f, xs);
+  coop_lu_factor<M, L, T>(R, r, f);
+  coop_lu_subst<M, L, T>(R, s, r, f, xs);
 }
 
 // The FP32+refine form's nodal solves: the elimination and the solve in
 // FP32, then one FP64 defect correction -- the residual s - R x of the FP64
 // system formed in FP64, its correction solved with the FP32 factors and
 // added in FP64.  R (this lane's FP64 row) is left as it was.
-template <int M, int S>
+template <int M, typename L>
 __device__ __forceinline__ void coop_lu_refine(const double (&R)[M], double s, int r, int f, double (&xs)[M]) {
 #pragma clang fp contract(off)
   float Rf[M], xf[M];
 #pragma unroll
   for (int c2 = 0; c2 < M; ++c2) Rf[c2] = (float)R[c2];
-  coop_lu_factor<M, S, float>(Rf, r, f);
-  coop_lu_subst<M, S, float>(Rf, (float)s, r, f, xf);
+  coop_lu_factor<M, L, float>(Rf, r, f);
+  coop_lu_subst<M, L, float>(Rf, (float)s, r, f, xf);
   double res = s;
 #pragma unroll
   for (int c2 = 0; c2 < M; ++c2) {
     xs[c2] = (double)xf[c2];
     res = res - R[c2] * xs[c2];
   }
-  coop_lu_subst<M, S, float>(Rf, (float)res, r, f, xf);
+  coop_lu_subst<M, L, float>(Rf, (float)res, r, f, xf);
 #pragma unroll
   for (int c2 = 0; c2 < M; ++c2) xs[c2] = xs[c2] + (double)xf[c2];
 }
 
 // ---------------------------------------------------------------------------
 // pass 1, cooperative form for larger group counts: eight lanes per (node,
-// direction), lane 8 r + f owning row r of node f's G x G system of
-// get_a2matvec (eight neighbouring nodes per wavefront).  Same arithmetic,
+// direction), lane 8 f + r owning row r of node f's G x G system of
+// get_a2matvec (eight neighbouring nodes per wavefront, LaneRow8).  Same arithmetic,
 // in the same order, as k_nod_pass1.
 // ---------------------------------------------------------------------------
 template <int G>
@@ -809,7 +854,7 @@ __global__ void __launch_bounds__(NT) k_nod_pass1c(NodArgs a) {
   const int u = blockIdx.y;
   const long NC = q.NC;
   const long nxy = (long)q.nxx * q.nyy;
-  const int lane = threadIdx.x & 63, r = lane >> 3, f = lane & 7;
+  const int lane = threadIdx.x & 63, r = lane & 7, f = lane >> 3;
   const int slot32 = (int)(threadIdx.x >> 6) * 8 + f;
   const int g = r < G ? r : 0;   // lanes past the last row repeat row 0's loads
   for (int it = 0; it < NT / 32; ++it) {
@@ -855,7 +900,7 @@ __global__ void __launch_bounds__(NT) k_nod_pass1c(NodArgs a) {
       atomicMin(&sc->lu_key, lu_key(face_order(q, u, fi, o1, o2), i, j, k));
     }
     double xs[G];
-    coop_lu<G, 8>(R, rhs, r, f, xs);
+    coop_lu<G, LaneRow8>(R, rhs, r, f, xs);
     if (r < G) {
       double Bx = 0.0;
 #pragma unroll
@@ -871,7 +916,7 @@ __global__ void __launch_bounds__(NT) k_nod_pass1c(NodArgs a) {
 // ---------------------------------------------------------------------------
 // pass 2, boundary faces (one node active), cooperative form: the faces
 // listed in bfl ((direction << 40) | face index, per core, NT per block),
-// eight lanes per face, lane 8 r + f owning row r of face f's G x G system of
+// eight lanes per face, lane 8 f + r owning row r of face f's G x G system of
 // get_a1matvec_first (the active node above the face, :375-407) or
 // get_a1matvec_last (below it, :411-441), solved by coop_lu; then get_a3
 // and nodal_coup_upd's boundary forms (:330-368).  Same arithmetic, in the
@@ -887,7 +932,7 @@ __global__ void __launch_bounds__(NT) k_nod_pass2b(NodArgs a, const long *bfl, i
   Scal *sc = v.sc;
   if (a.sched && !nodal_outer(sc)) return;   // uniform over the core's blocks
   const long NC = q.NC;
-  const int lane = threadIdx.x & 63, r = lane >> 3, f = lane & 7;
+  const int lane = threadIdx.x & 63, r = lane & 7, f = lane >> 3;
   const int slot32 = (int)(threadIdx.x >> 6) * 8 + f;
   const int g = r < G ? r : 0;   // lanes past the last row repeat row 0's loads
   double best = -1.0, bkey = 0.0, bloc = 0.0;
@@ -957,7 +1002,7 @@ __global__ void __launch_bounds__(NT) k_nod_pass2b(NodArgs a, const long *bfl, i
     }
     if (coop_lu_diag_small<G>(R, r)) atomicMin(&sc->lu_key, lu_key(ord, li, lj, lk));
     double xs[G];
-    coop_lu<G, 8>(R, bv, r, f, xs);
+    coop_lu<G, LaneRow8>(R, bv, r, f, xs);
     if (r < G) {
       // get_a3 and the boundary forms of nodal_coup_upd for group g
       double Bf = 0.0;
@@ -988,15 +1033,16 @@ __global__ void __launch_bounds__(NT) k_nod_pass2b(NodArgs a, const long *bfl, i
 
 // ---------------------------------------------------------------------------
 // pass 2, interior faces, cooperative form for larger group counts: 16 lanes
-// per face (four faces per wavefront), lane 4 r + f owning row r of face f's
-// 2G x 2G system of get_a1matvec (:630-698), solved by coop_lu (L32: by
+// per face (four faces per wavefront), lane 16 f + r owning row r of face f's
+// 2G x 2G system of get_a1matvec (:630-698), solved by coop_lu in the
+// LaneRow16 layout (L32: by
 // coop_lu_refine, FP32 factors and one FP64 defect correction; the system and
 // everything after the solve stay FP64).  A block covers the same 256-face
 // range as k_nod_pass2 (16 faces at a time); the boundary faces are
 // k_nod_pass2b's.
 // ---------------------------------------------------------------------------
 template <int G, bool L32 = false>
-__global__ void __launch_bounds__(NT) k_nod_pass2c(NodArgs a) {
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) k_nod_pass2c(NodArgs a) {
 #pragma clang fp contract(off)
   constexpr int M2 = 2 * G;
   static_assert(M2 <= 16, "16 lanes per face");
@@ -1010,9 +1056,9 @@ __global__ void __launch_bounds__(NT) k_nod_pass2c(NodArgs a) {
   const int e1 = u == 0 ? q.nyy : q.nxx;
   const int e2 = u == 2 ? q.nyy : q.nzz;
   const long nf = (long)(nu + 1) * e1 * e2;
-  // lane = 4 r + f: the four faces f of a wavefront side by side, so that a
-  // row's loads of four neighbouring cells share cache lines
-  const int lane = threadIdx.x & 63, r = lane >> 2, fw = lane & 3;
+  // lane = 16 f + r: face f of the wavefront's four neighbouring faces is
+  // one 16-lane row (its pivot rows broadcast by DPP), row r of its system
+  const int lane = threadIdx.x & 63, r = lane & 15, fw = lane >> 4;
   const int slot16 = (int)(threadIdx.x >> 6) * 4 + fw;
   const int g = r % G;                         // group of this lane's row (rows r and r + G)
   double best = -1.0, bkey = 0.0, bloc = 0.0;
@@ -1070,7 +1116,7 @@ __global__ void __launch_bounds__(NT) k_nod_pass2c(NodArgs a) {
     const double Ps = 2.0 * a.xs.D[(size_t)g * NC + cs] / hsd;
     const double dcs = a.xs.dc[((size_t)(nside ? sf : sf + 1) * G + g) * NC + cs];
     const double fs = a.f0[(size_t)g * NC + cs];
-    const int pl = 4 * (nside ? r + G : r - G) + fw;
+    const int pl = 16 * fw + (nside ? r + G : r - G);
     double Bo[G];
 #pragma unroll
     for (int hh = 0; hh < G; ++hh) Bo[hh] = __shfl(Bs[hh], pl, 64);
@@ -1127,8 +1173,8 @@ __global__ void __launch_bounds__(NT) k_nod_pass2c(NodArgs a) {
     // LU_solve (:829-897): the input-diagonal test, then the elimination
     if (coop_lu_diag_small<M2>(R, r)) atomicMin(&sc->lu_key, lu_key(ord, li, lj, lk));
     double xs[M2];
-    if constexpr (L32) coop_lu_refine<M2, 4>(R, s, r, fw, xs);
-    else coop_lu<M2, 4>(R, s, r, fw, xs);
+    if constexpr (L32) coop_lu_refine<M2, LaneRow16>(R, s, r, fw, xs);
+    else coop_lu<M2, LaneRow16>(R, s, r, fw, xs);
     // get_a3 and nodal_coup_upd for group g (lanes 0..G-1); node n's B row
     // read again rather than kept live through the elimination
     if (r < G) {

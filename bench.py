@@ -559,8 +559,7 @@ def nodal_probe(p):
     outers from the first-call state, then two SANM nodal updates (each a
     leakage pass, the node pass, the interface pass, the boundary-face pass
     and the argmax) on that state; in the FP64 form, then in the FP32+refine
-    form (the same update, with the interface systems solved in FP32 and
-    corrected once in FP64)."""
+    form (the same FP64 update unless ADG_NOD_LU32=1)."""
     from adpres_amd.engine import Engine
     out = {"probe": "nodal-counters", "nodes": p.nnod, "groups": p.ng}
     for prec in ("fp64", "fp32_refine"):

@@ -269,11 +269,12 @@ def test_gpu_cooperative_nodal_pass_is_bit_identical(monkeypatch):
 
 
 def test_gpu_fp32_interface_solves_keep_the_fp64_update(monkeypatch):
-    """The FP32+refinement form solves the 2G x 2G interface systems of the
-    cooperative pass in FP32 and corrects once in FP64 (the system, the
-    residual and everything after the solve are FP64).  Two updates on the
-    8-group core: D-hats within 1e-10 of the scale of the FP64-solve update
-    (ADG_NOD_LU32=0), ndmax within 1e-9 relative at the same node."""
+    """With ADG_NOD_LU32=1 the FP32+refinement form solves the 2G x 2G
+    interface systems of the cooperative pass in FP32 and corrects once in
+    FP64 (the system, the residual and everything after the solve are FP64).
+    Two updates on the 8-group core: D-hats within 1e-10 of the scale of the
+    FP64-solve update (ADG_NOD_LU32=0, the default), ndmax within 1e-9
+    relative at the same node."""
     p = generate("core3d_8g", 2).problem
     rng = np.random.default_rng(5)
     f0 = rng.uniform(0.9, 1.1, (p.ng, p.nnod))
