@@ -731,10 +731,8 @@ struct LaneRow16 {
   template <int P>
   static __device__ __forceinline__ double bcast(double v, int) {
     static_assert(P >= 0 && P < 16, "row lane");
-    const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_mov_dpp((int)(unsigned)(b & 0xffffffffll), 0x150 + P, 0xf, 0xf, false);
-    const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), 0x150 + P, 0xf, 0xf, false);
-    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+    // one 64-bit move (v_mov_b64_dpp) rather than two 32-bit halves
+    return __longlong_as_double(__builtin_amdgcn_mov_dpp(__double_as_longlong(v), 0x150 + P, 0xf, 0xf, false));
   }
 };
 
@@ -1116,19 +1114,27 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) k_
     const double Ps = 2.0 * a.xs.D[(size_t)g * NC + cs] / hsd;
     const double dcs = a.xs.dc[((size_t)(nside ? sf : sf + 1) * G + g) * NC + cs];
     const double fs = a.f0[(size_t)g * NC + cs];
+    // the partner row's value: for G = 8 the partner is lane r ^ 8 of the
+    // face's row, a DPP row_ror:8 move; otherwise a lane shuffle
     const int pl = 16 * fw + (nside ? r + G : r - G);
+    auto partner = [&](double x) -> double {
+      if constexpr (G == 8)
+        return __longlong_as_double(__builtin_amdgcn_mov_dpp(__double_as_longlong(x), 0x128, 0xf, 0xf, false));
+      else
+        return __shfl(x, pl, 64);
+    };
     double Bo[G];
 #pragma unroll
-    for (int hh = 0; hh < G; ++hh) Bo[hh] = __shfl(Bs[hh], pl, 64);
+    for (int hh = 0; hh < G; ++hh) Bo[hh] = partner(Bs[hh]);
     Coef6 cfo;
-    cfo.A = __shfl(cfs.A, pl, 64);
+    cfo.A = partner(cfs.A);
     cfo.B = 0.0;
     cfo.E = 0.0;
-    cfo.F = __shfl(cfs.F, pl, 64);
-    cfo.G = __shfl(cfs.G, pl, 64);
-    cfo.H = __shfl(cfs.H, pl, 64);
-    const double Lo = __shfl(Ls, pl, 64), a2o = __shfl(a2s, pl, 64), a4o = __shfl(a4s, pl, 64);
-    const double Po = __shfl(Ps, pl, 64), dco = __shfl(dcs, pl, 64), fo = __shfl(fs, pl, 64);
+    cfo.F = partner(cfs.F);
+    cfo.G = partner(cfs.G);
+    cfo.H = partner(cfs.H);
+    const double Lo = partner(Ls), a2o = partner(a2s), a4o = partner(a4s);
+    const double Po = partner(Ps), dco = partner(dcs), fo = partner(fs);
     double Bn[G], Bp[G];
 #pragma unroll
     for (int hh = 0; hh < G; ++hh) {
