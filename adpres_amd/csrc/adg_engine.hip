@@ -791,17 +791,19 @@ void nodal_update(adg_ctx *c, int kern, int cmode, int sched) {
   const bool lu32 = coop && c->precision == ADG_FP32_REFINE && c->nod_lu32;
   af.coop = coop ? 1 : 0;
   const dim3 g1(c->ncell, 3, 1), g2(c->nface, 3, 1);
+  // the cooperative passes' direction-interleaved 1D grids (dir_blk)
+  const dim3 g1c(3 * ((c->ncell + 7) / 8 * 8)), g2c(3 * ((c->nface + 7) / 8 * 8));
   switch (c->G) {
 #define NOD_CASE(GG)                                                                                     \
   case GG:                                                                                               \
     hipLaunchKernelGGL(k_nod_src, dim3(c->ncell, GG), dim3(NT), 0, c->s, an);                            \
     if (coop) {                                                                                          \
       hipLaunchKernelGGL(k_nod_bd<GG>, dim3(c->ncell, (NT + NT / GG - 1) / (NT / GG)), dim3(NT), 0, c->s, an); \
-      hipLaunchKernelGGL(k_nod_pass1c<GG>, g1, dim3(NT), 0, c->s, an);                                   \
+      hipLaunchKernelGGL(k_nod_pass1c<GG>, g1c, dim3(NT), 0, c->s, an, (int)c->ncell);                  \
       if (lu32)                                                                                          \
-        hipLaunchKernelGGL((k_nod_pass2c<GG, true>), g2, dim3(NT), 0, c->s, af);                         \
+        hipLaunchKernelGGL((k_nod_pass2c<GG, true>), g2c, dim3(NT), 0, c->s, af, (int)c->nface);        \
       else                                                                                               \
-        hipLaunchKernelGGL((k_nod_pass2c<GG, false>), g2, dim3(NT), 0, c->s, af);                        \
+        hipLaunchKernelGGL((k_nod_pass2c<GG, false>), g2c, dim3(NT), 0, c->s, af, (int)c->nface);       \
       if (c->nbface)                                                                                     \
         hipLaunchKernelGGL(k_nod_pass2b<GG>, dim3(c->nbface), dim3(NT), 0, c->s, ab, c->bfl, c->nface);  \
     } else {                                                                                             \

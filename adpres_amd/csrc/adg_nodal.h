@@ -835,6 +835,27 @@ __device__ __forceinline__ void coop_lu_refine(const double (&R)[M], double s, i
   for (int c2 = 0; c2 < M; ++c2) xs[c2] = xs[c2] + (double)xf[c2];
 }
 
+// Direction-interleaved grid of the cooperative passes: block B of a 1D grid
+// of 3 nbp blocks (nbp: the table's blocks rounded up to a multiple of 8)
+// serves table block (B / 24) 8 + B % 8 in direction (B / 8) % 3.  Blocks go
+// to the 8 XCDs round-robin, so B, B + 8 and B + 16 -- the three directions
+// of one node or face range -- run at nearly the same time on one XCD, and
+// the node terms all three read come from that XCD's L2 after the first.
+struct DirBlk {
+  int u, bx;
+};
+__device__ __forceinline__ DirBlk dir_blk() {
+  const int B = (int)blockIdx.x;
+  return DirBlk{(B >> 3) % 3, (B / 24) * 8 + (B & 7)};
+}
+__device__ __forceinline__ BlkView blk_view_at(const LV &lv, int bx) {
+  BlkView v;
+  v.b = lv.blk[bx];
+  v.q = lv.geo + v.b.core;
+  v.sc = lv.sc + v.b.core;
+  return v;
+}
+
 // ---------------------------------------------------------------------------
 // pass 1, cooperative form for larger group counts: eight lanes per (node,
 // direction), lane 8 f + r owning row r of node f's G x G system of
@@ -842,14 +863,16 @@ __device__ __forceinline__ void coop_lu_refine(const double (&R)[M], double s, i
 // in the same order, as k_nod_pass1.
 // ---------------------------------------------------------------------------
 template <int G>
-__global__ void __launch_bounds__(NT) k_nod_pass1c(NodArgs a) {
+__global__ void __launch_bounds__(NT) k_nod_pass1c(NodArgs a, int nbx) {
 #pragma clang fp contract(off)
   static_assert(G <= 8, "eight lanes per node");
-  const BlkView v = blk_view(a.lv);
+  const DirBlk db = dir_blk();
+  if (db.bx >= nbx) return;                                  // the grid's padding blocks
+  const BlkView v = blk_view_at(a.lv, db.bx);
   const Geo &q = *v.q;
   Scal *sc = v.sc;
   if (sc->done || (a.sched && !nodal_outer(sc))) return;   // uniform over the core's blocks
-  const int u = blockIdx.y;
+  const int u = db.u;
   const long NC = q.NC;
   const long nxy = (long)q.nxx * q.nyy;
   const int lane = threadIdx.x & 63, r = lane & 7, f = lane >> 3;
@@ -1040,15 +1063,17 @@ __global__ void __launch_bounds__(NT) k_nod_pass2b(NodArgs a, const long *bfl, i
 // k_nod_pass2b's.
 // ---------------------------------------------------------------------------
 template <int G, bool L32 = false>
-__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) k_nod_pass2c(NodArgs a) {
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) k_nod_pass2c(NodArgs a, int nbx) {
 #pragma clang fp contract(off)
   constexpr int M2 = 2 * G;
   static_assert(M2 <= 16, "16 lanes per face");
-  const BlkView v = blk_view(a.lv);
+  const DirBlk db = dir_blk();
+  if (db.bx >= nbx) return;                  // the grid's padding blocks
+  const BlkView v = blk_view_at(a.lv, db.bx);
   const Geo &q = *v.q;
   Scal *sc = v.sc;
   if (a.sched && !nodal_outer(sc)) return;   // uniform over the core's blocks
-  const int u = blockIdx.y;
+  const int u = db.u;
   const long NC = q.NC;
   const int nu = u == 0 ? q.nxx : (u == 1 ? q.nyy : q.nzz);
   const int e1 = u == 0 ? q.nyy : q.nxx;
@@ -1222,7 +1247,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) k_
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    const size_t slot = ((size_t)blockIdx.y * gridDim.x + blockIdx.x) * 3;
+    const size_t slot = ((size_t)u * nbx + db.bx) * 3;   // k_nod_pass2's slot of this (direction, block)
     a.part[slot] = sv[0];
     a.part[slot + 1] = sk[0];
     a.part[slot + 2] = sl[0];
