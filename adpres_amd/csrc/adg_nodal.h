@@ -121,9 +121,10 @@ __global__ void __launch_bounds__(NT) k_nod_src(NodArgs a) {
 
 // get_B's node terms of every row of a node (S1 of the reference's dum,
 // src/mod_nodal.f90:1345-1405), once per update for the cooperative passes,
-// node-major: one thread per (node, row g), g fastest, so that the G rows
-// of a node, and the nodes of a wavefront, are stored side by side and each
-// wavefront's stores fill whole lines.  The node-table entry of NT nodes is
+// node-major: one thread per (node, row g), the node fastest, so that each
+// cross-section load of a wavefront reads whole lines of consecutive cells
+// (for G = 8 a wavefront holds rows 2w and 2w + 1 of 32 nodes, and a node's
+// two rows fill one stored line).  The node-table entry of NT nodes is
 // covered by blockIdx.y = 0 .. ceil(NT / (NT / G)) - 1 blocks of NT / G nodes.
 template <int G>
 __global__ void __launch_bounds__(NT) k_nod_bd(NodArgs a) {
@@ -132,7 +133,7 @@ __global__ void __launch_bounds__(NT) k_nod_bd(NodArgs a) {
   const BlkView v = blk_view(a.lv);
   const Geo &q = *v.q;
   if ((int)threadIdx.x >= NPB * G) return;
-  const int local = (int)blockIdx.y * NPB + (int)threadIdx.x / G, g = (int)threadIdx.x % G;
+  const int local = (int)blockIdx.y * NPB + (int)threadIdx.x % NPB, g = (int)threadIdx.x / NPB;
   if (local >= NT) return;
   const long nxy = (long)q.nxx * q.nyy;
   const long t = v.b.lo + local;
