@@ -33,7 +33,7 @@ struct NodArgs {
   int coop;             // pass 2 leaves the interior faces to k_nod_pass2c
   double *part;
   // cooperative passes: the node terms of get_B's rows (S1 of the reference's
-  // dum, [cell][g][h], made once per update by k_nod_src) and each node's A,
+  // dum, [cell][g][h], made once per update by k_nod_bd) and each node's A,
   // F, G, H per direction ([u][cell][g][4], made by k_nod_pass1c), read by the
   // interface pass instead of being made again for every face; node-major,
   // so that a lane's row or coefficients are one contiguous read
@@ -215,7 +215,7 @@ __device__ __forceinline__ void nod_B_row(const NodArgs &a, long NC, long c, dou
   }
 }
 
-// the same row from the dum terms k_nod_src stored (a.Bd)
+// the same row from the dum terms k_nod_bd stored (a.Bd)
 template <int G>
 __device__ __forceinline__ void nod_B_row_pre(const NodArgs &a, long NC, long c, double h, int g, double *B) {
 #pragma clang fp contract(off)
