@@ -270,6 +270,27 @@ def batch_cores(seed: int = 3, n: int = 256):
     return [batch_core(seed, i) for i in range(n)]
 
 
+def small_multigroup_core(G: int, seed: int) -> GeneratedCore:
+    """A 10x10x12-node core of G-group materials (the 8-group core's material
+    generator at G groups: three fuel materials in a quarter layout of
+    radius 4.5 assemblies, a reflector around them and above and below),
+    for the cooperative nodal kernels' instances of 5 to 7 groups."""
+    rng = np.random.default_rng(seed)
+    nfuel, nx, nz = 3, 5, 6
+    mats, chis = [], []
+    for m in range(nfuel + 1):
+        mt, ch = _eight_group_material(rng, m < nfuel, G)
+        mats.append(mt)
+        chis.append(ch)
+    chis[-1] = chis[0]
+    fuel_pl = _quarter_layout(rng, nx, nfuel, 4.5)
+    refl_pl = np.full((nx, nx), nfuel + 1, dtype=np.int32)
+    geo = dict(xsize=[21.5] * nx, xdiv=[2] * nx, ysize=[21.5] * nx, ydiv=[2] * nx,
+               zsize=[20.0] * nz, zdiv=[2] * nz, planars=[refl_pl, fuel_pl], zpln=[1] + [2] * (nz - 2) + [1])
+    p, deck = _mk_problem(f"small_{G}g", G, _stack(mats), geo, (1, 2, 1, 2, 1, 1), chis)
+    return GeneratedCore(p, deck, False, seed)
+
+
 CORES = {"core2d_small": core2d_small, "core3d_2g": core3d_2g, "core3d_8g": core3d_8g}
 
 

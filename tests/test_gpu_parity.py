@@ -268,33 +268,14 @@ def test_gpu_cooperative_nodal_pass_is_bit_identical(monkeypatch):
     assert res["1"][1] > 0.0
 
 
-def _small_core(G, seed):
-    """A 10x10x12-node core of G-group materials (the 8-group core's material
-    generator at G groups), for the cooperative kernels' G = 5..7 instances."""
-    from adpres_amd.configs import _eight_group_material, _mk_problem, _quarter_layout, _stack
-    rng = np.random.default_rng(seed)
-    nfuel, nx, nz = 3, 5, 6
-    mats, chis = [], []
-    for m in range(nfuel + 1):
-        mt, ch = _eight_group_material(rng, m < nfuel, G)
-        mats.append(mt)
-        chis.append(ch)
-    chis[-1] = chis[0]
-    fuel_pl = _quarter_layout(rng, nx, nfuel, 4.5)
-    refl_pl = np.full((nx, nx), nfuel + 1, dtype=np.int32)
-    geo = dict(xsize=[21.5] * nx, xdiv=[2] * nx, ysize=[21.5] * nx, ydiv=[2] * nx,
-               zsize=[20.0] * nz, zdiv=[2] * nz, planars=[refl_pl, fuel_pl], zpln=[1] + [2] * (nz - 2) + [1])
-    p, _ = _mk_problem(f"small_{G}g", G, _stack(mats), geo, (1, 2, 1, 2, 1, 1), chis)
-    return p
-
-
 @pytest.mark.parametrize("G", [5, 6, 7])
 def test_gpu_cooperative_nodal_pass_is_bit_identical_for_5_to_7_groups(G, monkeypatch):
     """The cooperative kernels' instances for G = 5..7 (rows G..15 of a
     face's 16 lanes and of a node's 8 idle, partner rows by lane shuffle)
     give the per-thread kernels' D-hats, ndmax and location bit for bit, on
     two updates of a small G-group core."""
-    p = _small_core(G, 40 + G)
+    from adpres_amd.configs import small_multigroup_core
+    p = small_multigroup_core(G, 40 + G).problem
     assert p.ng == G and np.count_nonzero(p.nuf) > 0
     rng = np.random.default_rng(G)
     f0 = rng.uniform(0.9, 1.1, (p.ng, p.nnod))
