@@ -383,6 +383,40 @@ __device__ __forceinline__ void p_publish(double *sync, unsigned e, const double
   }
 }
 
+// Sums over lanes by DPP moves (no LDS round trip), in a fixed order, so
+// every block that sums the same values gets the same bits.  All 64 lanes
+// must be active.  dpp_f64 moves a double by a 32-bit DPP control.
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_mov_dpp((int)(unsigned)(b & 0xffffffffll), CTRL, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xf, 0xf, false);
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+// the sum of each 8-lane group, in every lane of the group: pairs (quad_perm
+// 1,0,3,2), quads (quad_perm 2,3,0,1), then the two quads of the group
+// (row_half_mirror); each step adds equal operands in both lanes
+__device__ __forceinline__ double dpp_sum8(double v) {
+  v = v + dpp_f64<0xB1>(v);
+  v = v + dpp_f64<0x4E>(v);
+  v = v + dpp_f64<0x141>(v);
+  return v;
+}
+// the sum of all 64 lanes, in every lane: the 8-lane sums, the two halves of
+// each 16-lane row (row_mirror), then the four row sums read lane by lane
+__device__ __forceinline__ double dpp_sum64(double v) {
+  v = dpp_sum8(v);
+  v = v + dpp_f64<0x140>(v);
+  const long long b = __double_as_longlong(v);
+  const int lo = (int)(unsigned)(b & 0xffffffffll), hi = (int)(b >> 32);
+  double r[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+    r[q] = __longlong_as_double((long long)(((unsigned long long)(unsigned)__builtin_amdgcn_readlane(hi, 16 * q) << 32) |
+                                            (unsigned)__builtin_amdgcn_readlane(lo, 16 * q)));
+  return (r[0] + r[1]) + (r[2] + r[3]);
+}
+
 template <int NV>
 __device__ __forceinline__ void p_lead(double *sync, unsigned e, long spin) {
   const int t = threadIdx.x, b = blockIdx.x;
@@ -409,9 +443,7 @@ __device__ __forceinline__ void p_lead(double *sync, unsigned e, long spin) {
     for (int k = 0; k < NV; ++k) v[k] = co_ld(sync + (long)PLINE * (t * 8 + b) + k);
   }
 #pragma unroll
-  for (int k = 0; k < NV; ++k)
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_xor(v[k], off, 64);
+  for (int k = 0; k < NV; ++k) v[k] = dpp_sum64(v[k]);
   if (t == 0) {
     if (!ok) co_st(err, 1);
     double *xl = sync + PSYNC_XCD + (long)PLINE * b;
@@ -475,9 +507,7 @@ __device__ __forceinline__ bool p_finish(double *sync, unsigned e, double (&red)
     }
     const int flagged = t == 0 ? co_ld(err) : 0;
 #pragma unroll
-    for (int k = 0; k < NV; ++k)
-#pragma unroll
-      for (int off = 4; off > 0; off >>= 1) v[k] += __shfl_xor(v[k], off, 64);
+    for (int k = 0; k < NV; ++k) v[k] = dpp_sum8(v[k]);   // lanes 0..7: the eight XCD sums
     if (t == 0) {
       if (!ok) co_st(err, 1);
 #pragma unroll
@@ -497,9 +527,7 @@ template <int NT, int NV>
 __device__ __forceinline__ void p_block_sum(double (&v)[NV]) {
   __shared__ double sm[(NT / 64) * NV];
 #pragma unroll
-  for (int k = 0; k < NV; ++k)
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_xor(v[k], off, 64);
+  for (int k = 0; k < NV; ++k) v[k] = dpp_sum64(v[k]);
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   __syncthreads();
   if (lane == 0) {
