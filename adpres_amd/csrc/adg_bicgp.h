@@ -383,39 +383,7 @@ __device__ __forceinline__ void p_publish(double *sync, unsigned e, const double
   }
 }
 
-// Sums over lanes by DPP moves (no LDS round trip), in a fixed order, so
-// every block that sums the same values gets the same bits.  All 64 lanes
-// must be active.  dpp_f64 moves a double by a 32-bit DPP control.
-template <int CTRL>
-__device__ __forceinline__ double dpp_f64(double v) {
-  const long long b = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_mov_dpp((int)(unsigned)(b & 0xffffffffll), CTRL, 0xf, 0xf, false);
-  const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xf, 0xf, false);
-  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
-}
-// the sum of each 8-lane group, in every lane of the group: pairs (quad_perm
-// 1,0,3,2), quads (quad_perm 2,3,0,1), then the two quads of the group
-// (row_half_mirror); each step adds equal operands in both lanes
-__device__ __forceinline__ double dpp_sum8(double v) {
-  v = v + dpp_f64<0xB1>(v);
-  v = v + dpp_f64<0x4E>(v);
-  v = v + dpp_f64<0x141>(v);
-  return v;
-}
-// the sum of all 64 lanes, in every lane: the 8-lane sums, the two halves of
-// each 16-lane row (row_mirror), then the four row sums read lane by lane
-__device__ __forceinline__ double dpp_sum64(double v) {
-  v = dpp_sum8(v);
-  v = v + dpp_f64<0x140>(v);
-  const long long b = __double_as_longlong(v);
-  const int lo = (int)(unsigned)(b & 0xffffffffll), hi = (int)(b >> 32);
-  double r[4];
-#pragma unroll
-  for (int q = 0; q < 4; ++q)
-    r[q] = __longlong_as_double((long long)(((unsigned long long)(unsigned)__builtin_amdgcn_readlane(hi, 16 * q) << 32) |
-                                            (unsigned)__builtin_amdgcn_readlane(lo, 16 * q)));
-  return (r[0] + r[1]) + (r[2] + r[3]);
-}
+// the lane sums below are dpp_sum8 / dpp_sum64 of adg_common.h
 
 template <int NV>
 __device__ __forceinline__ void p_lead(double *sync, unsigned e, long spin) {

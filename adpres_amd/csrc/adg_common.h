@@ -173,6 +173,45 @@ __device__ __forceinline__ void core_slots(const LV &lv, int core, int &first, i
 // the partial stores before their readers.
 // ---------------------------------------------------------------------------
 
+// Sums over the lanes of a wavefront by DPP moves (no LDS round trip), in a
+// fixed order, so every wavefront that sums the same values gets the same
+// bits, and the result is in every lane.  All 64 lanes must be active.
+// dpp_f64 moves a double by a 32-bit DPP control.
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_mov_dpp((int)(unsigned)(b & 0xffffffffll), CTRL, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xf, 0xf, false);
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+// each 8-lane group's sum in every lane of the group: pairs (quad_perm
+// 1,0,3,2), quads (quad_perm 2,3,0,1), then the two quads of the group
+// (row_half_mirror); each step adds equal operands in both lanes
+__device__ __forceinline__ double dpp_sum8(double v) {
+  v = v + dpp_f64<0xB1>(v);
+  v = v + dpp_f64<0x4E>(v);
+  v = v + dpp_f64<0x141>(v);
+  return v;
+}
+// all 64 lanes: the 8-lane sums, the two halves of each 16-lane row
+// (row_mirror), then the four row sums read lane by lane
+__device__ __forceinline__ double dpp_sum64(double v) {
+  v = dpp_sum8(v);
+  v = v + dpp_f64<0x140>(v);
+  const long long b = __double_as_longlong(v);
+  const int lo = (int)(unsigned)(b & 0xffffffffll), hi = (int)(b >> 32);
+  double r[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+    r[q] = __longlong_as_double((long long)(((unsigned long long)(unsigned)__builtin_amdgcn_readlane(hi, 16 * q) << 32) |
+                                            (unsigned)__builtin_amdgcn_readlane(lo, 16 * q)));
+  return (r[0] + r[1]) + (r[2] + r[3]);
+}
+
+// the flat kernels' lane reduction (xor butterfly by lane shuffles): its
+// summation order is the one the batching form's FP32+refine outcomes were
+// pinned with, so it is kept; the persistent kernel's synchronisation uses
+// the DPP sums above
 template <int NV, unsigned MAXMASK>
 __device__ __forceinline__ void wave_reduce(double (&v)[NV]) {
 #pragma unroll
