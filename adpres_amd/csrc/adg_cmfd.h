@@ -234,7 +234,7 @@ __global__ void __launch_bounds__(NTF) k_bicg_setup(LV lv, const double *A64g, c
     if (d) d[c] = (TV)0;
     acc[0] += (double)rv * (double)rv;
   }
-  block_partial<1, 0u>(acc, pout, v.b.slot);
+  block_partial<1, 0u, TAB == 0>(acc, pout, v.b.slot);
   if (core_leader(v)) {   // rho = alpha = omega = 1 before iteration 1
     v.sc->rho_s[0] = 1.0;
     v.sc->alpha_s[0] = 1.0;
@@ -401,11 +401,11 @@ k_bicgA(LV lv, const TV *__restrict__ AT, int it, const TV *__restrict__ r, cons
   double rho, alpha_d = 0.0, omega_d = 0.0, beta = 0.0;
   if (FIRST) {
     double red[1];
-    reduce_partials<1, 0u>(pin, bv.b.pfirst, bv.b.pn, red);
+    reduce_partials<1, 0u, TAB == 0>(pin, bv.b.pfirst, bv.b.pn, red);
     rho = red[0];
   } else {
     double red[4];   // (t,t), (t,s), (rs,t), (rs,s) of iteration it-1
-    reduce_partials<4, 0u>(pin, bv.b.pfirst, bv.b.pn, red);
+    reduce_partials<4, 0u, TAB == 0>(pin, bv.b.pfirst, bv.b.pn, red);
     omega_d = red[1] / red[0];
     rho = red[3] - omega_d * red[2];
     alpha_d = sc->alpha_s[prv];
@@ -428,7 +428,7 @@ k_bicgA(LV lv, const TV *__restrict__ AT, int it, const TV *__restrict__ r, cons
     if (h2) ka_cell<TV, FIRST>(L2, c2, ta, tw, tb, x, rout, pout, vout, acc[0]);
 #endif
   }
-  block_partial<1, 0u>(acc, partout, bv.b.slot);
+  block_partial<1, 0u, TAB == 0>(acc, partout, bv.b.slot);
   if (core_leader(bv)) {
     sc->rho_s[cur] = rho;
     if (!FIRST) sc->omega_s[prv] = omega_d;
@@ -449,7 +449,7 @@ k_bicgB(LV lv, const TV *__restrict__ AT, int it, const TV *__restrict__ r, cons
   if (TAB == 0 && c0 < bv.b.hi) kb_load<TV>(L, AT, r, v, rs, NC, off, c0);
   if (sc->done) return;
   double red[1];
-  reduce_partials<1, 0u>(pin, bv.b.pfirst, bv.b.pn, red);
+  reduce_partials<1, 0u, TAB == 0>(pin, bv.b.pfirst, bv.b.pn, red);
   const int cur = it & 1;
   const double alpha_d = sc->rho_s[cur] / red[0];
   const TV alpha = (TV)alpha_d;
@@ -467,7 +467,7 @@ k_bicgB(LV lv, const TV *__restrict__ AT, int it, const TV *__restrict__ r, cons
     if (h2) kb_cell<TV>(L2, c2, alpha, t, acc);
 #endif
   }
-  block_partial<4, 0u>(acc, partout, bv.b.slot);
+  block_partial<4, 0u, TAB == 0>(acc, partout, bv.b.slot);
   if (core_leader(bv)) sc->alpha_s[cur] = alpha_d;
 }
 
@@ -480,7 +480,7 @@ __global__ void __launch_bounds__(NTF) k_bicgT(LV lv, int it, const TV *r, const
   const BlkView bv = blk_view_t<TAB>(lv);
   if (bv.sc->done) return;
   double red[4];
-  reduce_partials<4, 0u>(pin, bv.b.pfirst, bv.b.pn, red);
+  reduce_partials<4, 0u, TAB == 0>(pin, bv.b.pfirst, bv.b.pn, red);
   const double omega_d = red[1] / red[0];
   const TV ta = (TV)bv.sc->alpha_s[it & 1], tw = (TV)omega_d;
   for (long c = bv.b.lo + threadIdx.x; c < bv.b.hi; c += NTF) {

@@ -208,12 +208,18 @@ __device__ __forceinline__ double dpp_sum64(double v) {
   return (r[0] + r[1]) + (r[2] + r[3]);
 }
 
-// the flat kernels' lane reduction (xor butterfly by lane shuffles): its
-// summation order is the one the batching form's FP32+refine outcomes were
-// pinned with, so it is kept; the persistent kernel's synchronisation uses
-// the DPP sums above
-template <int NV, unsigned MAXMASK>
+// the flat kernels' lane reduction: the xor butterfly by lane shuffles,
+// whose summation order the batching form's FP32+refine outcomes were pinned
+// with (the batch tables' launches keep it), or with DPR the DPP sums above
+// (single-core launches; sums only)
+template <int NV, unsigned MAXMASK, bool DPR = false>
 __device__ __forceinline__ void wave_reduce(double (&v)[NV]) {
+  static_assert(!DPR || MAXMASK == 0u, "DPP form: sums only");
+  if constexpr (DPR) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) v[k] = dpp_sum64(v[k]);
+    return;
+  }
 #pragma unroll
   for (int k = 0; k < NV; ++k) {
 #pragma unroll
@@ -225,10 +231,10 @@ __device__ __forceinline__ void wave_reduce(double (&v)[NV]) {
 }
 
 // block-wide reduction over NTF threads; the result is returned in every thread
-template <int NV, unsigned MAXMASK>
+template <int NV, unsigned MAXMASK, bool DPR = false>
 __device__ __forceinline__ void block_reduce(double (&v)[NV]) {
   __shared__ double sm[(NTF / 64) * NV];
-  wave_reduce<NV, MAXMASK>(v);
+  wave_reduce<NV, MAXMASK, DPR>(v);
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   __syncthreads();
   if (lane == 0) {
@@ -246,9 +252,9 @@ __device__ __forceinline__ void block_reduce(double (&v)[NV]) {
 }
 
 // this block's contribution -> part[slot * NV + k]
-template <int NV, unsigned MAXMASK>
+template <int NV, unsigned MAXMASK, bool DPR = false>
 __device__ __forceinline__ void block_partial(double (&v)[NV], double *part, int slot) {
-  block_reduce<NV, MAXMASK>(v);
+  block_reduce<NV, MAXMASK, DPR>(v);
   if (threadIdx.x == 0) {
 #pragma unroll
     for (int k = 0; k < NV; ++k) part[(size_t)slot * NV + k] = v[k];
@@ -259,7 +265,7 @@ __device__ __forceinline__ void block_partial(double (&v)[NV], double *part, int
 // kernel.  In the per-wavefront form every wavefront reduces all partials
 // on its own (same order in every wavefront, so bit-identical results) with
 // no LDS and no barrier.
-template <int NV, unsigned MAXMASK>
+template <int NV, unsigned MAXMASK, bool DPR = false>
 __device__ __forceinline__ void reduce_partials(const double *part, int first, int n, double (&out)[NV]) {
 #pragma unroll
   for (int k = 0; k < NV; ++k) out[k] = 0.0;
@@ -272,7 +278,7 @@ __device__ __forceinline__ void reduce_partials(const double *part, int first, i
         out[k] = ((MAXMASK >> k) & 1u) ? fmax(out[k], x) : out[k] + x;
       }
     }
-    wave_reduce<NV, MAXMASK>(out);
+    wave_reduce<NV, MAXMASK, DPR>(out);
     return;
   }
   // four partials per thread are requested before any is added (same
@@ -292,7 +298,7 @@ __device__ __forceinline__ void reduce_partials(const double *part, int first, i
       for (int k = 0; k < NV; ++k) out[k] = ((MAXMASK >> k) & 1u) ? fmax(out[k], x[u][k]) : out[k] + x[u][k];
     }
   }
-  block_reduce<NV, MAXMASK>(out);
+  block_reduce<NV, MAXMASK, DPR>(out);
 }
 
 }  // namespace adg
