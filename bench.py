@@ -129,7 +129,7 @@ def batch_bench(args):
             # measured HBM traffic per launch of the context the counter
             # profile covers (the one with the most nodes)
             kname = f"void adg::k_bicgA<{'float' if wsz == 4 else 'double'}, false, 1>"
-            traffic = measured_traffic("batch256", kname)
+            traffic, _ = measured_traffic("batch256", kname)
             Gp = max(classes, key=lambda G: sum(cores[i].nnod for i in classes[G]))
             if traffic and str(Gp) in roof_ctx:
                 c = roof_ctx[str(Gp)]
@@ -243,7 +243,7 @@ def xs_update_probe(eng, p, workload, seed, reps=10):
         nbytes = per_node * p.nnod
         ach = nbytes / (us * 1e-6) / 1e9
         return {"bound": "hbm", "kernel": kernel, "algorithmic_bytes_per_launch": nbytes,
-                "traffic": measured_traffic(workload, f"void adg::{kernel}<{G}>"),
+                "traffic": measured_traffic(workload, f"void adg::{kernel}<{G}>")[0],
                 "bytes_per_node": per_node, "avg_launch_us": round(us, 2), "achieved": round(ach, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
                 "call_ms_with_upload": round(wall_ms, 3), "samples": reps,
@@ -458,15 +458,18 @@ def measured_traffic(workload, kernel):
     """HBM-side bytes per launch of `kernel` from the committed counter
     summaries profiles/pmc_bicg_<workload>.txt and (the flat BiCGSTAB form)
     profiles/pmc_bicg_flat_<workload>.txt (FETCH_SIZE and WRITE_SIZE runs of
-    --kernel-probe, scaled by the 256 MiB copy calibration), or None."""
+    --kernel-probe, scaled by the 256 MiB copy calibration), with the source
+    to cite: (bytes, source) or (None, None)."""
     for name in (f"pmc_bicg_{workload}.txt", f"pmc_bicg_flat_{workload}.txt"):
         path = os.path.join(ROOT, "profiles", name)
         if not os.path.exists(path):
             continue
         for line in open(path):
             if line.startswith(kernel + " "):
-                return int(round(float(line[len(kernel):].split()[3]) * 1e6))
-    return None
+                return (int(round(float(line[len(kernel):].split()[3]) * 1e6)),
+                        f"profiles/{name} (rocprofv3 FETCH_SIZE x2 + WRITE_SIZE x1 KiB per dispatch, calibrated on a "
+                        f"256 MiB copy)")
+    return None, None
 
 
 def copy_calibration():
@@ -713,11 +716,9 @@ def main():
         kc = info64["nodes_per_thread"]
         k1_bytes = (11 + 31 * p.nin) * 8 * p.nnod
         ach = k1_bytes / (r64["k1_avg_ms"] * 1e-3) / 1e9
-        traffic = measured_traffic(args.workload, f"void adg::k_bicg_persist<double, {kc}>")
+        traffic, tsrc = measured_traffic(args.workload, f"void adg::k_bicg_persist<double, {kc}>")
         roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "traffic_source": f"profiles/pmc_bicg_{args.workload}.txt (rocprofv3 FETCH_SIZE x2 + WRITE_SIZE x1 "
-                                  f"KiB per dispatch, calibrated on a 256 MiB copy)",
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": tsrc,
                 "kernel": f"k_bicg_persist<double, {kc}> (one launch per BiCGSTAB group solve: setup and all "
                           f"{p.nin} iterations, matrix and vectors kept in registers, brick halos exchanged)",
                 "algorithmic_bytes_per_launch": k1_bytes,
@@ -737,11 +738,9 @@ def main():
         # 7 matrix coefficients in; r, p, v, x out = 17 words per node per launch
         k1_bytes = 17 * 8 * p.nnod
         ach = k1_bytes / (r64["k1_avg_ms"] * 1e-3) / 1e9
+        traffic, tsrc = measured_traffic(args.workload, "void adg::k_bicgA<double, false, 0>")
         roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ach / HBM_PEAK_GBS, 4),
-                "traffic": measured_traffic(args.workload, "void adg::k_bicgA<double, false, 0>"),
-                "traffic_source": f"profiles/pmc_bicg_{args.workload}.txt (rocprofv3 FETCH_SIZE x2 + WRITE_SIZE x1 "
-                                  f"KiB per dispatch, calibrated on a 256 MiB copy)",
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": tsrc,
                 "kernel": "k_bicgA<double> (BiCGSTAB: previous x/r update + p update + 7-point SpMV + (rs,v))",
                 "algorithmic_bytes_per_launch": k1_bytes, "avg_launch_us": round(r64["k1_avg_ms"] * 1e3, 3),
                 "samples": r64["k1_samples"], "timing": "HIP events attached to one steady-state KA dispatch per group "
