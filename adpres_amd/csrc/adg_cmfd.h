@@ -192,12 +192,19 @@ __global__ void __launch_bounds__(NTF) k_src(LV lv, XS xs, int g, int smode, int
   for (long c = v.b.lo + threadIdx.x; c < v.b.hi; c += NTF) {
     if (first_group) fs0c[c] = fs0[c];
     f0c[(size_t)g * NC + c] = f0[(size_t)g * NC + c];
+    // every in-scatter term's loads issued first, then summed in order
+    double sgv[G], fv[G];
+#pragma unroll
+    for (int h = 0; h < G; ++h) {
+      const bool on = h != g && ((msk >> h) & 1u);
+      sgv[h] = on ? (adjoint ? xs.sigs[((size_t)h * G + g) * NC + c] : xs.sigs[((size_t)g * G + h) * NC + c]) : 0.0;
+      fv[h] = on ? f0[(size_t)h * NC + c] : 0.0;
+    }
     double s0 = 0.0;
-    #pragma unroll
+#pragma unroll
     for (int h = 0; h < G; ++h) {
       if (h == g || !((msk >> h) & 1u)) continue;
-      const double sg = adjoint ? xs.sigs[((size_t)h * G + g) * NC + c] : xs.sigs[((size_t)g * G + h) * NC + c];
-      s0 = s0 + sg * f0[(size_t)h * NC + c];
+      s0 = s0 + sgv[h] * fv[h];
     }
     const double e = ex ? xs.exsrc[(size_t)g * NC + c] : 0.0;
     if (smode == 2) {
