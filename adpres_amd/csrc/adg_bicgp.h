@@ -497,7 +497,9 @@ __device__ __forceinline__ void p_block_sum(double (&v)[NV]) {
 #pragma unroll
   for (int k = 0; k < NV; ++k) v[k] = dpp_sum64(v[k]);
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  __syncthreads();
+  // no barrier before these writes: between two calls every thread passes
+  // p_publish's and p_finish's barriers, which thread 0 reaches only after
+  // its reads of sm below
   if (lane == 0) {
 #pragma unroll
     for (int k = 0; k < NV; ++k) sm[w * NV + k] = v[k];
